@@ -1,0 +1,212 @@
+/*
+ * smdlt_hip.h -- C ABI of the MI355X (gfx950) timestep hot path for bead-spring
+ * Brownian dynamics with Ewald-summed RPY hydrodynamics, flowing-box remaps and
+ * charge-hopping Monte Carlo.
+ *
+ * The reference has no plugin or FFI layer: its hot path is a set of
+ * `void f(void)` functions sharing globals (S/main.h:5-65, S/Parameters.h:40-201;
+ * S/ = shear/src/, X/ = extensional/src/).  Each entry point below names the
+ * reference function whose body it replaces, so a maintainer can swap a body for
+ * a call.  Plain C types only; host arrays belong to the caller and are copied.
+ *
+ * Conventions
+ *  - Return 0 on success, a negative SMDLT_E_* code otherwise (the reference
+ *    prints and exit(1)s instead).  smdlt_strerror() names the code.
+ *  - A context owns its device memory and one HIP stream; calls on one context
+ *    are serialised on that stream and are not re-entrant.
+ *  - Bead arrays are concatenated over replicas in replica order; replica r has
+ *    nb*nc beads, chain-major like the reference (bead = chain*nb + j).
+ *  - Positions are SoA doubles (the reference's rx, ry, rz); vectors (forces,
+ *    noise, velocities) are AoS 3N doubles (the reference's R[3N] layout).
+ */
+#ifndef SMDLT_HIP_H
+#define SMDLT_HIP_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SMDLT_OK 0
+#define SMDLT_E_ARG (-1)             /* bad argument or size */
+#define SMDLT_E_HIP (-2)             /* a HIP runtime call failed */
+#define SMDLT_E_FENE_OVERSTRETCH (-3)/* S/main26.c:553-558 */
+#define SMDLT_E_LANCZOS_NOCONV (-4)  /* tolerance not reached within max iterations */
+#define SMDLT_E_NLIST_OVERFLOW (-5)  /* neighbour list capacity exceeded (retried internally) */
+#define SMDLT_E_GEOMETRY (-6)        /* cutoff too large for the (deformed) box */
+#define SMDLT_E_STATE (-7)           /* call order violated (e.g. no positions uploaded) */
+#define SMDLT_E_NOMEM (-8)
+#define SMDLT_E_MC_STREAM (-9)       /* MC ran out of host-fed uniforms */
+
+/* Flow types: the reference's -o flag and its two programs. */
+#define SMDLT_FLOW_EQUILIBRIUM 0
+#define SMDLT_FLOW_SHEAR 1       /* Lees-Edwards deforming brick (S/) */
+#define SMDLT_FLOW_EXTENSION 2   /* planar extension, Kraynik-Reinelt lattice (X/) */
+
+/* Mobility precision. */
+#define SMDLT_PREC_MIXED 0  /* FP32 pair/grid/FFT sums, compensated per-bead sums, FP64 Lanczos */
+#define SMDLT_PREC_FP64 1   /* every kernel in FP64 */
+
+/* Brownian term. */
+#define SMDLT_NOISE_LANCZOS 0 /* M^(1/2) z by Krylov (Lanczos) */
+#define SMDLT_NOISE_TEA 1     /* the reference's CA-TEA product (S/main26.c:787-804) */
+
+/* MC modes. */
+#define SMDLT_MC_REFERENCE 0 /* chargeHop's state machine and acceptance (S/main26.c:1926-2390) */
+#define SMDLT_MC_MH_UNIFORM 1/* exact Metropolis-Hastings, uniform hop proposal within hop radius */
+#define SMDLT_MC_MH_LEARNED 2/* exact Metropolis-Hastings, learned (MLP) proposal */
+
+/* Per-replica parameters: the reference's CLI flags and Input.txt keys
+ * (S/Initialization.c:52-174) plus the mobility settings of this build.  A zero
+ * in an "auto" field lets smdlt_create pick the value (autotuner). */
+typedef struct smdlt_params {
+  /* chain set and box */
+  int nb, nc;              /* -b beads per chain, -a chains */
+  double L;                /* cubic box edge (initBox, S/Initialization.c:176-215) */
+  int flow_type;           /* -o */
+  double edot;             /* -f strain rate */
+  double dt;               /* -d */
+  /* Input.txt */
+  int spring_fene;         /* spring = FENE (1) or Hookean (0) */
+  double epsilon, kappab, kappas, qmax;
+  double lambda_b;         /* Bjerrum length */
+  /* charges */
+  double lambda_d;         /* -l Debye length */
+  double barrier, barrier2;/* -x, -y */
+  int lj_ext_form;         /* 0: shear LJ 12eps(r6^2-r6), rc=5; 1: extensional WCA 48eps form */
+  int pbc_compat;          /* 1: reproduce the shear +-delx wrap shift (S/main26.c:1412-1423) */
+  /* mobility (this build) */
+  int precision;           /* SMDLT_PREC_* */
+  int noise_mode;          /* SMDLT_NOISE_* */
+  double ewald_alpha;      /* splitting parameter xi [1/a]; 0 = auto */
+  double rc_real;          /* real-space cutoff [a]; 0 = auto */
+  double skin;             /* Verlet skin of the real-space list [a]; 0 = 1.0 */
+  int grid[3];             /* PME grid along a1,a2,a3 (powers of two); 0 = auto */
+  int spline_order;        /* B-spline order 4..8; 0 = auto */
+  double mobility_tol;     /* target relative M.F error for the auto choices; 0 = 1e-5 */
+  double lanczos_tol;      /* 0 = 1e-6 */
+  int lanczos_maxit;       /* 0 = 30 */
+  uint64_t seed;           /* device noise stream seed for this replica */
+} smdlt_params;
+
+typedef struct smdlt_ctx smdlt_ctx;
+
+/* Library version string. */
+const char* smdlt_version(void);
+/* Name of an error code. */
+const char* smdlt_strerror(int code);
+/* Fill p with the reference defaults (S/Initialization.c:30-48, S/Input.txt). */
+void smdlt_default_params(smdlt_params* p);
+
+/* Create a context for n_replicas independent boxes (allocate + initLattice +
+ * initVerlet: S/Initialization.c:217-457).  device = HIP device ordinal. */
+int smdlt_create(const smdlt_params* params, int n_replicas, int device, smdlt_ctx** out);
+void smdlt_destroy(smdlt_ctx* ctx);
+int smdlt_num_beads(const smdlt_ctx* ctx, int replica); /* replica = -1: total */
+
+/* Effective mobility settings chosen for a replica: alpha, rc, skin, K1, K2, K3, order. */
+int smdlt_get_mobility_config(const smdlt_ctx* ctx, int replica, double out[7]);
+
+/* Host <-> device state.  Positions SoA over all beads (reference rx, ry, rz). */
+int smdlt_upload_positions(smdlt_ctx* ctx, const double* rx, const double* ry, const double* rz);
+int smdlt_download_positions(smdlt_ctx* ctx, double* rx, double* ry, double* rz);
+/* Charges: per replica nq[r] charges; bead index of each (Charge_track, local to
+ * the replica) and its sign (+1 in the reference; -1 allowed). */
+int smdlt_upload_charges(smdlt_ctx* ctx, const int* nq, const int* track, const int* sign);
+int smdlt_download_charges(smdlt_ctx* ctx, int* track, int* old);
+
+/* updateLattice(t) for every replica (S/main26.c:1382-1408, X/main14.c:1426-1450).
+ * Host FP64; out (optional, per replica 8 doubles): L1x L1y L2x L2y L1xp L2xp L2yp delx. */
+int smdlt_update_lattice(smdlt_ctx* ctx, uint64_t t, double* out);
+
+/* The per-step remap on the device, FP64: KRBC transformCoords/inverseCoords
+ * round trip (X/main14.c:69-71), updateLattice(t), transformCoords (X) and
+ * applyPBC (S/main26.c:1409-1428, X/main14.c:1451-1461). */
+int smdlt_remap(smdlt_ctx* ctx, uint64_t t);
+/* applyPBC alone at the current lattice (positions already in the wrap frame). */
+int smdlt_apply_pbc(smdlt_ctx* ctx);
+/* transformCoords (inverse=0) / inverseCoords (inverse=1), X/main14.c:1514-1535. */
+int smdlt_rotate(smdlt_ctx* ctx, int inverse);
+
+/* getNID for a batch of pairs of one replica (S/main26.c:1429-1457,
+ * X/main14.c:1462-1480): out[3*k] = r_i - r_j under the reference's minimum
+ * image.  Computed on the device in FP64. */
+int smdlt_min_image(smdlt_ctx* ctx, int replica, int npairs, const int* i, const int* j, double* out);
+/* binij for a batch of pairs (S/main26.c:1518-1529) with the table geometry of
+ * initLattice (bin_size = -g). */
+int smdlt_bin_index(smdlt_ctx* ctx, int replica, double bin_size, int npairs, const int* i, const int* j,
+                    int* bins);
+
+/* checkVerlet + verletlist (S/main26.c:682-707, 858-869): rebuilds the LJ list
+ * when any accumulated displacement exceeds (rv-rc)/2, or always if force != 0.
+ * *rebuilt = number of replicas rebuilt.  The real-space mobility list is
+ * refreshed by its own displacement test inside the same call. */
+int smdlt_neighbors(smdlt_ctx* ctx, int force, int* rebuilt);
+/* The reference's list of one replica: nlist[n], list[n*stride] ascending j>i. */
+int smdlt_get_verlet(smdlt_ctx* ctx, int replica, int* nlist, int* list, int stride);
+
+/* bondforce + LJforce + coulombForce (S/main26.c:541-681, 1891-1925).  Forces
+ * stay on the device; fx/fy/fz may be NULL, else copied out (SoA doubles). */
+int smdlt_forces(smdlt_ctx* ctx, double* fx, double* fy, double* fz);
+
+/* Mobility product u = M v over all replicas (calcHI + the matvec of updateHI,
+ * S/main26.c:1530-1806, 745-857).  v, u: AoS 3N doubles, host memory. */
+int smdlt_mobility_apply(smdlt_ctx* ctx, const double* v, double* u);
+/* Same with device pointers (no copies); stream-ordered on the context stream. */
+int smdlt_mobility_apply_device(smdlt_ctx* ctx, const double* d_v, double* d_u);
+
+/* Brownian displacement y = M^(1/2) z by Lanczos (replaces the TEA product of
+ * S/main26.c:787-804).  iters[r], resid[r] per replica (may be NULL). */
+int smdlt_brownian(smdlt_ctx* ctx, const double* z, double* y, double tol, int maxit, int* iters,
+                   double* resid);
+/* The reference's TEA product y_i = C_i (.) (M_ii z_i + beta sum_j M_ij z_j) with
+ * the mobility of this build (S/main26.c:787-804); beta per replica, C 3N. */
+int smdlt_tea_apply(smdlt_ctx* ctx, const double* beta, const double* C, const double* z, double* y);
+
+/* One full timestep at step t (S/main26.c:65-159, X/main14.c:68-247, iteration
+ * >= 1): remap, Verlet check, forces, noise, M F + M^(1/2) z (or TEA), position
+ * update with the flow.  noise: NULL = device generator, else 3N host doubles. */
+int smdlt_step(smdlt_ctx* ctx, uint64_t t, const double* noise);
+/* nsteps steps from t0 with device noise.  use_graph != 0 captures one step in a
+ * hipGraph and replays it (the lattice parameters are refreshed between replays). */
+int smdlt_run(smdlt_ctx* ctx, uint64_t t0, int nsteps, int use_graph);
+/* Per-replica Lanczos iteration counts of the last step (or brownian call). */
+int smdlt_last_iters(const smdlt_ctx* ctx, int* iters);
+
+/* chargeHop (S/main26.c:1926-2390) for every replica at one MC step.  mode =
+ * SMDLT_MC_*.  uniforms: per replica a host stream of n_uniforms values (layout
+ * [replica][n_uniforms]) consumed in the reference's data-dependent order; NULL =
+ * device generator.  consumed[r] = values used.  log (may be NULL): per replica
+ * nq[r] records of 6 ints {k, from, to, kind(-1 skip,0 adj,1 adj-near,2 pop),
+ * accepted, draws}, concatenated. */
+int smdlt_mc_sweep(smdlt_ctx* ctx, int mode, const float* uniforms, int n_uniforms, int* consumed,
+                   int* log);
+/* Screened-Coulomb energy per replica (the double loop of chargeHop, S/main26.c:1955-1970). */
+int smdlt_mc_energy(smdlt_ctx* ctx, double* energy);
+
+/* Learned proposal for SMDLT_MC_MH_LEARNED: a 16-64-64-1 MLP (BF16 weights,
+ * FP32 accumulation) scoring each hop candidate.  weights: FP32 host array of
+ * smdlt_mlp_num_params() values (W1,b1,W2,b2,W3,b3).  No reference counterpart. */
+int smdlt_mlp_num_params(void);
+int smdlt_mlp_set_weights(smdlt_ctx* ctx, const float* weights);
+/* Forward + backward of the binary cross-entropy loss on a batch (features
+ * [n][16] FP32, labels [n] in {0,1}); grad gets d loss / d params (FP32). */
+int smdlt_mlp_loss_grad(smdlt_ctx* ctx, const float* weights, const float* features, const float* labels,
+                        int n, float* loss, float* grad);
+/* Proposal features of every hop candidate of every charge (16 per candidate) as
+ * the learned proposal computes them; returns the count in *n (capacity cap). */
+int smdlt_mc_features(smdlt_ctx* ctx, int replica, float* features, int* cand_from, int* cand_to, int cap,
+                      int* n);
+
+/* Per-stage device times of the last step, milliseconds (HIP events on the
+ * context stream when profiling is enabled): see smdlt_stage_name(). */
+int smdlt_set_profiling(smdlt_ctx* ctx, int on);
+int smdlt_stage_times(const smdlt_ctx* ctx, double* ms, int* counts, int max_stages);
+const char* smdlt_stage_name(int stage);
+
+/* Synchronise the context stream. */
+int smdlt_sync(smdlt_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,352 @@
+"""ctypes view of the CPU oracle (oracle/build/libsmdlt_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: the checker.  Only tests/, __graft_entry__.smoke()
+and bench.py's cpu_baseline leg import this module.  The product library never
+loads it.
+
+Each wrapper names the reference lines its C function restates
+(S/ = shear/src/, X/ = extensional/src/).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "build", "libsmdlt_oracle.so")
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+
+
+class RNG(C.Structure):
+    _fields_ = [("idum", C.c_int64), ("idum2", C.c_int64), ("iy", C.c_int64),
+                ("iv", C.c_int64 * 32), ("iset", C.c_int), ("gset", C.c_float)]
+
+
+class Lattice(C.Structure):
+    _fields_ = [("flow", C.c_int), ("L", C.c_double), ("edot", C.c_double), ("dt", C.c_double),
+                ("tp", C.c_int), ("point0", (C.c_double * 2) * 3), ("point", (C.c_double * 2) * 3),
+                ("L1", C.c_double * 2), ("L2", C.c_double * 2),
+                ("L1xp", C.c_double), ("L1yp", C.c_double), ("L2xp", C.c_double), ("L2yp", C.c_double),
+                ("detL", C.c_double), ("L2pmag", C.c_double), ("treal", C.c_int),
+                ("delx", C.c_double), ("theta", C.c_double), ("ct", C.c_double), ("st", C.c_double),
+                ("pbc_compat", C.c_int)]
+
+
+class Bins(C.Structure):
+    _fields_ = [("bin_size_x", C.c_double), ("bin_size_y", C.c_double), ("bin_size_z", C.c_double),
+                ("nbx", C.c_int), ("nby", C.c_int), ("nbz", C.c_int), ("nhx", C.c_int),
+                ("nhy", C.c_int), ("nhz", C.c_int), ("num_bins", C.c_int)]
+
+
+class Cell(C.Structure):
+    _fields_ = [("a1x", C.c_double), ("a1y", C.c_double), ("a2x", C.c_double),
+                ("a2y", C.c_double), ("Lz", C.c_double)]
+
+
+class MCState(C.Structure):
+    _fields_ = [("n", C.c_int), ("nb", C.c_int), ("nc", C.c_int), ("nq", C.c_int),
+                ("track", C.POINTER(C.c_int)), ("indices", C.POINTER(C.c_int)),
+                ("old", C.POINTER(C.c_int)), ("ivalues", C.POINTER(C.c_int)),
+                ("charge", C.POINTER(C.c_int))]
+
+
+class RefRun(C.Structure):
+    _fields_ = [("nb", C.c_int), ("nc", C.c_int), ("flow", C.c_int),
+                ("L", C.c_double), ("edot", C.c_double), ("dt", C.c_double), ("kappas", C.c_double),
+                ("qmax", C.c_double), ("epsilon", C.c_double), ("lambda_b", C.c_double),
+                ("lambda_d", C.c_double), ("barrier", C.c_double), ("barrier2", C.c_double),
+                ("spring_fene", C.c_int), ("nq", C.c_int), ("mstep", C.c_int), ("srpy", C.c_int),
+                ("nt_avg", C.c_int), ("beta", C.c_double), ("seed", C.c_int64)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle library missing: run `make -C oracle` (or __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        L.orc_rng_seed.argtypes = [C.POINTER(RNG), C.c_int64]
+        L.orc_ran1.argtypes = [C.POINTER(RNG)]
+        L.orc_ran1.restype = C.c_float
+        L.orc_gasdev.argtypes = [C.POINTER(RNG)]
+        L.orc_gasdev.restype = C.c_float
+        L.orc_ran1_fill.argtypes = [C.POINTER(RNG), C.c_int, _fp]
+        L.orc_gasdev_fill.argtypes = [C.POINTER(RNG), C.c_int, _dp]
+        L.orc_lattice_init.argtypes = [C.POINTER(Lattice), C.c_int, C.c_double, C.c_double, C.c_double, C.c_int]
+        L.orc_lattice_update.argtypes = [C.POINTER(Lattice), C.c_uint64]
+        L.orc_apply_pbc.argtypes = [C.POINTER(Lattice), C.c_int, _dp, _dp, _dp]
+        L.orc_rotate.argtypes = [C.POINTER(Lattice), C.c_int, _dp, _dp, C.c_int]
+        L.orc_nid.argtypes = [C.POINTER(Lattice), _dp, _dp, _dp, C.c_int, C.c_int, _dp]
+        L.orc_nid_bin.argtypes = [C.POINTER(Lattice), _dp, _dp, _dp, C.c_int, C.c_int, _dp]
+        L.orc_bins_init.argtypes = [C.POINTER(Bins), C.POINTER(Lattice), C.c_double]
+        L.orc_binij.argtypes = [C.POINTER(Bins), C.POINTER(Lattice), _dp, _dp, _dp, C.c_int, C.c_int]
+        L.orc_binij.restype = C.c_int
+        L.orc_verlet.argtypes = [C.POINTER(Lattice), C.c_int, _dp, _dp, _dp, C.c_double, _ip, _ip, C.c_int]
+        L.orc_check_verlet.argtypes = [C.c_int, _dp, _dp, _dp, C.c_double]
+        L.orc_check_verlet.restype = C.c_int
+        L.orc_bond_force.argtypes = [C.POINTER(Lattice), C.c_int, C.c_int, _dp, _dp, _dp, C.c_int,
+                                     C.c_double, C.c_double, C.c_double, _dp, _dp, _dp, _dp]
+        L.orc_bond_force.restype = C.c_int
+        L.orc_lj_force.argtypes = [C.POINTER(Lattice), C.c_int, _dp, _dp, _dp, _ip, _ip, C.c_int,
+                                   C.c_double, C.c_double, C.c_int, _dp, _dp, _dp, _dp]
+        L.orc_coulomb_force.argtypes = [C.POINTER(Lattice), C.c_int, _dp, _dp, _dp, _ip, C.c_double,
+                                        C.c_double, _dp, _dp, _dp]
+        L.orc_coulomb_energy.argtypes = [C.POINTER(Lattice), C.c_int, _dp, _dp, _dp, _ip, C.c_double, C.c_double]
+        L.orc_coulomb_energy.restype = C.c_double
+        L.orc_rpy_real_coeffs.argtypes = [C.c_double, C.c_double, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.orc_rpy_ewald_dense.argtypes = [C.POINTER(Cell), C.c_int, _dp, C.c_double, C.c_double, C.c_double, C.c_int, _dp]
+        L.orc_rpy_free_dense.argtypes = [C.c_int, _dp, _dp]
+        L.orc_ewald_selfconst.argtypes = [C.c_double]
+        L.orc_ewald_selfconst.restype = C.c_double
+        L.orc_calchi_ref.argtypes = [C.POINTER(Lattice), C.c_int, _dp, _dp, _dp, C.c_uint64, C.c_int,
+                                     C.POINTER(Bins), C.c_void_p, C.c_double, _dp,
+                                     C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.orc_tea_apply.argtypes = [C.c_int, _dp, C.c_double, _dp, _dp, _dp]
+        L.orc_sym_sqrt_apply.argtypes = [C.c_int, _dp, _dp, _dp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.orc_sym_sqrt_apply.restype = C.c_int
+        L.orc_lanczos_sqrt.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_int, _dp, C.POINTER(C.c_int)]
+        L.orc_lanczos_sqrt.restype = C.c_int
+        L.orc_matvec.argtypes = [C.c_int, _dp, _dp, _dp]
+        L.orc_charge_hop.argtypes = [C.POINTER(MCState), C.POINTER(Lattice), _dp, _dp, _dp, C.c_double,
+                                     C.c_double, C.c_double, C.c_double, _fp, C.c_int,
+                                     C.POINTER(C.c_int), C.c_void_p, C.POINTER(C.c_double)]
+        L.orc_charge_hop.restype = C.c_int
+        L.orc_ref_run.argtypes = [C.POINTER(RefRun), _dp, _dp, _dp, C.c_int, C.c_int, C.POINTER(C.c_int)]
+        L.orc_ref_run.restype = C.c_double
+        L.orc_num_threads.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+# ---------------------------------------------------------------- RNG
+class Ran1:
+    """ran1/gasdev stream (S/Initialization.c:601-634, S/main26.c:515-540)."""
+
+    def __init__(self, seed):
+        self.s = RNG()
+        lib().orc_rng_seed(C.byref(self.s), int(seed))
+
+    def uniforms(self, n):
+        out = np.empty(n, np.float32)
+        lib().orc_ran1_fill(C.byref(self.s), n, out)
+        return out
+
+    def gauss(self, n):
+        out = np.empty(n, np.float64)
+        lib().orc_gasdev_fill(C.byref(self.s), n, out)
+        return out
+
+
+# ---------------------------------------------------------------- lattice
+def lattice(flow, L, edot, dt, pbc_compat=1, t=0):
+    """initLattice + updateLattice(t) (S/Initialization.c:217-298, S/main26.c:1382-1408,
+    X/main14.c:1426-1450)."""
+    lat = Lattice()
+    lib().orc_lattice_init(C.byref(lat), flow, L, edot, dt, pbc_compat)
+    if t:
+        lib().orc_lattice_update(C.byref(lat), t)
+    return lat
+
+
+def lattice_update(lat, t):
+    lib().orc_lattice_update(C.byref(lat), int(t))
+
+
+def apply_pbc(lat, rx, ry, rz):
+    rx, ry, rz = (np.ascontiguousarray(a, np.float64).copy() for a in (rx, ry, rz))
+    lib().orc_apply_pbc(C.byref(lat), len(rx), rx, ry, rz)
+    return rx, ry, rz
+
+
+def rotate(lat, rx, ry, inverse):
+    rx, ry = (np.ascontiguousarray(a, np.float64).copy() for a in (rx, ry))
+    lib().orc_rotate(C.byref(lat), len(rx), rx, ry, int(inverse))
+    return rx, ry
+
+
+def nid(lat, rx, ry, rz, i, j):
+    out = np.zeros(3)
+    lib().orc_nid(C.byref(lat), rx, ry, rz, i, j, out)
+    return out
+
+
+def nid_bin(lat, rx, ry, rz, i, j):
+    out = np.zeros(3)
+    lib().orc_nid_bin(C.byref(lat), rx, ry, rz, i, j, out)
+    return out
+
+
+def bins(lat, bin_size):
+    b = Bins()
+    lib().orc_bins_init(C.byref(b), C.byref(lat), bin_size)
+    return b
+
+
+def binij(b, lat, rx, ry, rz, i, j):
+    return lib().orc_binij(C.byref(b), C.byref(lat), rx, ry, rz, i, j)
+
+
+def verlet(lat, rx, ry, rz, rv, stride=None):
+    n = len(rx)
+    stride = stride or n
+    nl = np.zeros(n, np.int32)
+    lst = np.zeros(n * stride, np.int32)
+    lib().orc_verlet(C.byref(lat), n, rx, ry, rz, rv, nl, lst, stride)
+    return nl, lst.reshape(n, stride)
+
+
+# ---------------------------------------------------------------- forces
+def bond_force(lat, nb, nc, rx, ry, rz, fene, kappas, qmax, kappab=0.0):
+    n = nb * nc
+    f = [np.zeros(n) for _ in range(3)]
+    tau = np.zeros(3)
+    st = lib().orc_bond_force(C.byref(lat), nb, nc, rx, ry, rz, int(fene), kappas, qmax, kappab,
+                              f[0], f[1], f[2], tau)
+    return st, np.stack(f, 1), tau
+
+
+def lj_force(lat, rx, ry, rz, nlist, lst, r2cut, epsilon, ext_form):
+    n = len(rx)
+    lst = np.ascontiguousarray(lst, np.int32)
+    stride = lst.shape[1]
+    f = [np.zeros(n) for _ in range(3)]
+    tau = np.zeros(3)
+    lib().orc_lj_force(C.byref(lat), n, rx, ry, rz, np.ascontiguousarray(nlist, np.int32),
+                       lst.reshape(-1), stride, r2cut, epsilon, int(ext_form), f[0], f[1], f[2], tau)
+    return np.stack(f, 1), tau
+
+
+def coulomb_force(lat, rx, ry, rz, q, lambda_b, lambda_d):
+    n = len(rx)
+    f = [np.zeros(n) for _ in range(3)]
+    lib().orc_coulomb_force(C.byref(lat), n, rx, ry, rz, np.ascontiguousarray(q, np.int32),
+                            lambda_b, lambda_d, f[0], f[1], f[2])
+    return np.stack(f, 1)
+
+
+def coulomb_energy(lat, rx, ry, rz, q, lambda_b, lambda_d):
+    return lib().orc_coulomb_energy(C.byref(lat), len(rx), rx, ry, rz,
+                                    np.ascontiguousarray(q, np.int32), lambda_b, lambda_d)
+
+
+# ---------------------------------------------------------------- mobility
+def rpy_real_coeffs(r, alpha, overlap_mode=0):
+    t1, t2 = C.c_double(), C.c_double()
+    lib().orc_rpy_real_coeffs(r, alpha, overlap_mode, C.byref(t1), C.byref(t2))
+    return t1.value, t2.value
+
+
+def cell_of(lat):
+    """lattice vectors in the frame positions are wrapped in: a1=(L1xp, 0), a2=(L2xp, L2yp)."""
+    return Cell(lat.L1xp, 0.0, lat.L2xp, lat.L2yp, lat.L)
+
+
+def ewald_dense(cell, pos, alpha, rcut=None, kcut=None, overlap_mode=0):
+    """Exact Ewald-summed RPY matrix (3n x 3n).  Defaults put the truncation
+    error near FP64 round-off."""
+    pos = np.ascontiguousarray(pos, np.float64).reshape(-1)
+    n = len(pos) // 3
+    rcut = rcut if rcut is not None else 6.2 / alpha
+    kcut = kcut if kcut is not None else 12.4 * alpha
+    M = np.zeros((3 * n) * (3 * n))
+    lib().orc_rpy_ewald_dense(C.byref(cell), n, pos, alpha, rcut, kcut, overlap_mode, M)
+    return M.reshape(3 * n, 3 * n)
+
+
+def rpy_free_dense(pos):
+    pos = np.ascontiguousarray(pos, np.float64).reshape(-1)
+    n = len(pos) // 3
+    M = np.zeros((3 * n) * (3 * n))
+    lib().orc_rpy_free_dense(n, pos, M)
+    return M.reshape(3 * n, 3 * n)
+
+
+def selfconst(alpha):
+    return lib().orc_ewald_selfconst(alpha)
+
+
+def calchi_ref(lat, rx, ry, rz, t, nt_avg, b, table, box_volume):
+    n = len(rx)
+    M = np.zeros(9 * n * n)
+    far = C.c_int()
+    kc = C.c_int()
+    tp = None
+    if table is not None:
+        table = np.ascontiguousarray(table, np.float64)
+        tp = table.ctypes.data_as(C.c_void_p)
+    lib().orc_calchi_ref(C.byref(lat), n, rx, ry, rz, int(t), nt_avg, C.byref(b), tp, box_volume, M,
+                         C.byref(far), C.byref(kc))
+    return M.reshape(3 * n, 3 * n), far.value, kc.value
+
+
+def tea_apply(M, beta, Cv, R):
+    n = M.shape[0] // 3
+    y = np.zeros(3 * n)
+    lib().orc_tea_apply(n, np.ascontiguousarray(M), beta, np.ascontiguousarray(Cv, np.float64),
+                        np.ascontiguousarray(R, np.float64), y)
+    return y
+
+
+def sym_sqrt_apply(A, z):
+    n3 = A.shape[0]
+    y = np.zeros(n3)
+    lo, hi = C.c_double(), C.c_double()
+    st = lib().orc_sym_sqrt_apply(n3, np.ascontiguousarray(A), np.ascontiguousarray(z, np.float64), y,
+                                  C.byref(lo), C.byref(hi))
+    return y, st, lo.value, hi.value
+
+
+def lanczos_sqrt(A, z, tol, maxit):
+    n3 = A.shape[0]
+    y = np.zeros(n3)
+    it = C.c_int()
+    st = lib().orc_lanczos_sqrt(n3, np.ascontiguousarray(A), np.ascontiguousarray(z, np.float64), tol,
+                                maxit, y, C.byref(it))
+    return y, st, it.value
+
+
+# ---------------------------------------------------------------- MC
+class ChargeHop:
+    """chargeHop state (S/main26.c:1926-2390): Charge_track, Charge_indices,
+    Charge_old, i_values, Charge."""
+
+    def __init__(self, nb, nc, track):
+        n = nb * nc
+        self.track = np.ascontiguousarray(track, np.int32).copy()
+        self.indices = self.track.copy()
+        self.old = self.track.copy()
+        self.ivalues = np.zeros(n, np.int32)
+        self.charge = np.zeros(n, np.int32)
+        self.charge[self.track] = 1
+        P = C.POINTER(C.c_int)
+        self.st = MCState(n, nb, nc, len(self.track), self.track.ctypes.data_as(P),
+                          self.indices.ctypes.data_as(P), self.old.ctypes.data_as(P),
+                          self.ivalues.ctypes.data_as(P), self.charge.ctypes.data_as(P))
+
+    def sweep(self, lat, rx, ry, rz, lambda_b, lambda_d, barrier, barrier2, u):
+        u = np.ascontiguousarray(u, np.float32)
+        cur = C.c_int(0)
+        log = np.zeros(6 * max(1, len(self.track)), np.int32)
+        e0 = C.c_double()
+        st = lib().orc_charge_hop(C.byref(self.st), C.byref(lat), rx, ry, rz, lambda_b, lambda_d,
+                                  barrier, barrier2, u, len(u), C.byref(cur),
+                                  log.ctypes.data_as(C.c_void_p), C.byref(e0))
+        return st, cur.value, log.reshape(-1, 6), e0.value
+
+
+def ref_run(params: dict, rx, ry, rz, nsteps, nthreads=0):
+    p = RefRun(**params)
+    rx, ry, rz = (np.ascontiguousarray(a, np.float64).copy() for a in (rx, ry, rz))
+    st = C.c_int()
+    el = lib().orc_ref_run(C.byref(p), rx, ry, rz, nsteps, nthreads, C.byref(st))
+    return el, st.value, (rx, ry, rz)
+
+
+def num_threads():
+    return lib().orc_num_threads()
