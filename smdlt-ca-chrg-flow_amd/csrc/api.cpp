@@ -1,0 +1,1625 @@
+// Host side of the C ABI declared in include/smdlt_hip.h: context and device
+// memory, the flowing-box lattice (updateLattice restated in FP64 with the
+// reference's expressions), the mobility settings (autotuner), the step
+// sequence and the stage timers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "smdlt_hip.h"
+
+#define HCK(x)                                   \
+  do {                                           \
+    hipError_t e_ = (x);                         \
+    if (e_ != hipSuccess) {                      \
+      last_hip_error() = hipGetErrorString(e_);  \
+      return SMDLT_E_HIP;                        \
+    }                                            \
+  } while (0)
+
+static std::string& last_hip_error() {
+  static thread_local std::string s;
+  return s;
+}
+
+static const double kEP = 0.9624;  // lattice reset strain (S/Parameters.h:37)
+
+// ---------------------------------------------------------------- lattice
+// initLattice / updateLattice of the reference (S/Initialization.c:217-298,
+// S/main26.c:1382-1408; X/Initialization.c:236-287, X/main14.c:1426-1450).
+struct HostLattice {
+  int flow = 0, tp = INT_MAX, pbc_compat = 1, treal = 0;
+  double L = 0, edot = 0, dt = 0;
+  double point0[3][2], point[3][2], L1[2], L2[2];
+  double L1xp = 0, L1yp = 0, L2xp = 0, L2yp = 0, detL = 0, delx = 0, theta = 0, ct = 1, st = 0;
+
+  void init(int flow_, double L_, double edot_, double dt_, int compat) {
+    flow = flow_;
+    L = L_;
+    edot = edot_;
+    dt = dt_;
+    pbc_compat = compat;
+    tp = (flow == 0 || edot == 0.0) ? INT_MAX : (int)(kEP / edot / dt);
+    double theta0 = (flow == 2) ? 31.7 : 0.0;
+    double th = 90.0 - (45.0 + theta0);
+    th *= M_PI / 180;
+    point[0][0] = -sqrt(2) / 2 * L * cos(th);
+    point[0][1] = sqrt(2) / 2 * L * sin(th);
+    point[1][0] = -sqrt(2) / 2 * L * sin(th);
+    point[1][1] = -sqrt(2) / 2 * L * cos(th);
+    point[2][0] = sqrt(2) / 2 * L * cos(th);
+    point[2][1] = -sqrt(2) / 2 * L * sin(th);
+    memcpy(point0, point, sizeof(point));
+    for (int i = 0; i < 2; ++i) {
+      L1[i] = point[2][i] - point[1][i];
+      L2[i] = point[0][i] - point[1][i];
+    }
+    update(0);
+  }
+  void update(uint64_t t) {
+    treal = (tp == INT_MAX) ? 0 : (int)(t % (uint64_t)tp);
+    double flowrate = edot;
+    if (flow == 2) {
+      for (int i = 0; i < 3; ++i) {
+        point[i][0] = point0[i][0] * exp(flowrate * treal * dt);
+        point[i][1] = point0[i][1] * exp(-flowrate * treal * dt);
+      }
+      for (int i = 0; i < 2; ++i) {
+        L1[i] = point[2][i] - point[1][i];
+        L2[i] = point[0][i] - point[1][i];
+      }
+      double th = atan(L1[1] / L1[0]);
+      L1xp = L1[0] * cos(th) + L1[1] * sin(th);
+      L2xp = L2[0] * cos(th) + L2[1] * sin(th);
+      L1yp = -L1[0] * sin(th) + L1[1] * cos(th);
+      L2yp = -L2[0] * sin(th) + L2[1] * cos(th);
+      detL = L * (-L1yp * L2xp + L1xp * L2yp);
+      delx = 0.0;
+    } else {
+      point[0][0] = point0[0][0] + point[0][1] * flowrate * treal * dt;
+      point[1][0] = point0[1][0] + point[1][1] * flowrate * treal * dt;
+      point[2][0] = point0[2][0] + point[2][1] * flowrate * treal * dt;
+      for (int i = 0; i < 2; ++i) {
+        L1[i] = point[2][i] - point[1][i];
+        L2[i] = point[0][i] - point[1][i];
+      }
+      L1xp = L1[0];
+      L2xp = L2[0];
+      L1yp = L1[1];
+      L2yp = L2[1];
+      detL = L2yp * L1xp * L;
+      delx = flowrate * treal * dt;
+    }
+    theta = atan(L1[1] / L1[0]);
+    ct = cos(theta);
+    st = sin(theta);
+  }
+};
+
+// Lagrange-Gauss reduction of the in-plane basis (a1, a2).
+static void reduce_basis(double a1x, double a1y, double a2x, double a2y, double b[4], double dual[4], double* wmin) {
+  double u[2] = {a1x, a1y}, v[2] = {a2x, a2y};
+  for (int it = 0; it < 64; ++it) {
+    double uu = u[0] * u[0] + u[1] * u[1], vv = v[0] * v[0] + v[1] * v[1];
+    if (vv < uu) {
+      std::swap(u, v);
+      std::swap(uu, vv);
+    }
+    double mu = std::round((u[0] * v[0] + u[1] * v[1]) / uu);
+    if (mu == 0.0) break;
+    v[0] -= mu * u[0];
+    v[1] -= mu * u[1];
+  }
+  b[0] = u[0];
+  b[1] = u[1];
+  b[2] = v[0];
+  b[3] = v[1];
+  double det = u[0] * v[1] - u[1] * v[0];
+  dual[0] = v[1] / det;
+  dual[1] = -v[0] / det;
+  dual[2] = -u[1] / det;
+  dual[3] = u[0] / det;
+  double area = fabs(det);
+  double w1 = area / sqrt(v[0] * v[0] + v[1] * v[1]), w2 = area / sqrt(u[0] * u[0] + u[1] * u[1]);
+  *wmin = std::min(w1, w2);
+}
+
+static void reduced_widths(const HostLattice& h, double w[2]) {
+  double b[4], d[4], wm;
+  reduce_basis(h.L1xp, 0.0, h.L2xp, h.L2yp, b, d, &wm);
+  double area = fabs(h.L1xp * h.L2yp);
+  w[0] = area / sqrt(b[2] * b[2] + b[3] * b[3]);
+  w[1] = area / sqrt(b[0] * b[0] + b[1] * b[1]);
+}
+
+// ---------------------------------------------------------------- mobility settings
+static double ewald_selfconst(double a) {
+  double rtpi = sqrt(M_PI);
+  return 1 - 6 / rtpi * a + 40 / (3 * rtpi) * a * a * a;
+}
+
+// magnitude of the Beenakker real-space coefficients at r (far form)
+static double real_tail(double r, double a) {
+  double rr = r * r, a2 = a * a, a3 = a2 * a, a5 = a3 * a2, a7 = a5 * a2;
+  double C1 = 0.75 / r + 0.5 / (r * rr), C3 = 0.75 / r - 1.5 / (r * rr);
+  double C2 = 4 * a7 * rr * rr + 3 * a3 * rr - 20 * a5 * rr - 4.5 * a + 14 * a3 + a / rr;
+  double C4 = -4 * a7 * rr * rr - 3 * a3 * rr + 16 * a5 * rr + 1.5 * a - 2 * a3 - 3 * a / rr;
+  double ec = erfc(a * r), g = exp(-a2 * rr) / sqrt(M_PI);
+  return fabs(C1 * ec + C2 * g) + fabs(C3 * ec + C4 * g);
+}
+
+// alpha such that the real-space coefficient at rc is tol * (density factor)
+static double choose_alpha(double rc, double tol) {
+  double lo = 0.01, hi = 5.0;
+  for (int i = 0; i < 100; ++i) {
+    double mid = 0.5 * (lo + hi);
+    if (real_tail(rc, mid) > tol)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return hi;
+}
+
+// k at which the reciprocal kernel envelope (1+q+2q^2)e^{-q} 6pi/k^2 drops to tol
+static double choose_kmax(double alpha, double tol) {
+  double k = 0.1;
+  for (; k < 100.0; k *= 1.01) {
+    double q = k * k / (4 * alpha * alpha);
+    double env = (1 + q + 2 * q * q) * exp(-q) * 6 * M_PI / (k * k) * fabs(1 - k * k / 3);
+    if (env < tol && q > 1.0) break;
+  }
+  return k;
+}
+
+static int next_pow2(int x) {
+  int p = 16;
+  while (p < x) p <<= 1;
+  return p;
+}
+
+// B-spline modulus |b(m)|^2 = 1/|sum_{k=0}^{p-2} M_p(k+1) e^{2 pi i m k/K}|^2 (Essmann et al. 1995)
+static void bspline_moduli(int K, int p, std::vector<double>& out) {
+  std::vector<double> W(MAX_ORDER, 0.0);
+  // M_p at integer points 1..p-1: weights W[j] = M_p(0 + j) at x=0
+  W[0] = 1.0;
+  for (int k = 2; k <= p; ++k) {
+    double div = 1.0 / (k - 1);
+    W[k - 1] = div * W[k - 2];
+    for (int j = k - 2; j >= 1; --j) W[j] = div * (j * W[j] + (k - j) * W[j - 1]);
+    W[0] = 0.0;
+  }
+  out.assign(K, 0.0);
+  for (int m = 0; m < K; ++m) {
+    double re = 0, im = 0;
+    for (int k = 0; k <= p - 2; ++k) {
+      double ang = 2 * M_PI * m * k / K;
+      re += W[k + 1] * cos(ang);
+      im += W[k + 1] * sin(ang);
+    }
+    double d = re * re + im * im;
+    out[m] = d > 1e-10 ? 1.0 / d : 0.0;
+  }
+}
+
+// ---------------------------------------------------------------- context
+enum Stage { ST_REMAP, ST_NLIST, ST_FORCES, ST_NOISE, ST_MOB, ST_LANCZOS, ST_UPDATE, ST_MC, ST_COUNT };
+static const char* kStageNames[ST_COUNT] = {"remap", "neighbors", "forces", "noise",
+                                            "mobility_apply", "lanczos_other", "update", "mc"};
+
+struct Timer {
+  hipEvent_t a, b;
+  int stage;
+};
+
+struct smdlt_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int nrep = 0, prec = 0, nst = 0, nwaves = 0;
+  std::vector<smdlt_params> P;
+  std::vector<HostLattice> lat;
+  std::vector<RepDev> rh;
+  std::vector<double> ct_prev, st_prev;
+  std::vector<int> max_w_cells;  // cell capacity per replica
+  RepDev* d_reps = nullptr;
+  int* d_slot_rep = nullptr;
+  uint64_t* d_seeds = nullptr;
+  // bead state
+  double *rx = nullptr, *ry = nullptr, *rz = nullptr, *dxv = nullptr, *dyv = nullptr, *dzv = nullptr;
+  double *dsp = nullptr, *f = nullptr, *z = nullptr, *mf = nullptr, *bz = nullptr, *tmp = nullptr, *tmp2 = nullptr;
+  bool have_pos = false;
+  // lists
+  int cap_rpy = 64, cap_lj = 64;
+  int *nrpy = nullptr, *lrpy = nullptr, *nlj = nullptr, *llj = nullptr, *overflow = nullptr;
+  int *cell_of = nullptr, *cell_cnt = nullptr, *cell_start = nullptr, *cell_fill = nullptr, *cell_beads = nullptr;
+  int ncells = 0;
+  int* rep_flags = nullptr;
+  double *out2 = nullptr, *aff = nullptr;
+  bool lists_built = false;
+  // grid
+  void* grid = nullptr;
+  long long grid_total = 0;
+  void* bsp = nullptr;
+  void* tw = nullptr;
+  FftTask *tx = nullptr, *ty = nullptr, *tz = nullptr;
+  int ntx = 0, nty = 0, ntz = 0;
+  std::vector<int> tx0, ty0, tz0;  // per-replica task ranges (size nrep+1)
+  // lanczos
+  int maxit = 30;
+  double *V = nullptr, *W = nullptr, *partial = nullptr, *scal = nullptr, *bdiv = nullptr;
+  int *done = nullptr, *iters = nullptr;
+  std::vector<int> last_iters;
+  // charges
+  std::vector<int> nq, q0;
+  int nq_total = 0, max_nq = 0;
+  int *qtrack = nullptr, *qsign = nullptr, *qold = nullptr, *qidx = nullptr, *mc_err = nullptr;
+  unsigned char* occ = nullptr;
+  long long occ_stride = 0;
+  int* log_off = nullptr;
+  float* mlp_w = nullptr;
+  uint64_t mc_sweeps = 0;
+  int* d_err = nullptr;
+  // profiling
+  bool prof = false;
+  std::vector<Timer> timers;
+  std::vector<hipEvent_t> pool;
+  double stage_ms[ST_COUNT] = {0};
+  int stage_cnt[ST_COUNT] = {0};
+  // graph
+  hipGraphExec_t gexec = nullptr;
+  bool graph_ready = false;
+  int cur_rep_begin = 0, cur_rep_end = 0;
+};
+
+static hipEvent_t get_event(smdlt_ctx* c) {
+  if (!c->pool.empty()) {
+    hipEvent_t e = c->pool.back();
+    c->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+struct StageScope {
+  smdlt_ctx* c;
+  int st;
+  hipEvent_t b = nullptr;
+  StageScope(smdlt_ctx* c_, int s) : c(c_), st(s) {
+    if (c->prof) {
+      b = get_event(c);
+      hipEventRecord(b, c->stream);
+    }
+  }
+  ~StageScope() {
+    if (c->prof && b) {
+      hipEvent_t e = get_event(c);
+      hipEventRecord(e, c->stream);
+      c->timers.push_back({b, e, st});
+    }
+  }
+};
+
+static void collect_timers(smdlt_ctx* c) {
+  if (c->timers.empty()) return;
+  hipStreamSynchronize(c->stream);
+  for (auto& t : c->timers) {
+    float ms = 0;
+    hipEventElapsedTime(&ms, t.a, t.b);
+    c->stage_ms[t.stage] += ms;
+    c->stage_cnt[t.stage] += 1;
+    c->pool.push_back(t.a);
+    c->pool.push_back(t.b);
+  }
+  c->timers.clear();
+}
+
+static int pad_slots(int n) { return ((n + SLOT_ALIGN - 1) / SLOT_ALIGN) * SLOT_ALIGN; }
+
+// Fill the lattice-dependent fields of a replica descriptor.
+static void fill_lattice(smdlt_ctx* c, int r) {
+  RepDev& R = c->rh[r];
+  const HostLattice& h = c->lat[r];
+  R.L = h.L;
+  R.invL = 1.0 / h.L;
+  R.L1xp = h.L1xp;
+  R.L2xp = h.L2xp;
+  R.L2yp = h.L2yp;
+  R.delx = h.delx;
+  R.ct = h.ct;
+  R.st = h.st;
+  R.ct0 = c->ct_prev[r];
+  R.st0 = c->st_prev[r];
+  double wm;
+  reduce_basis(h.L1xp, 0.0, h.L2xp, h.L2yp, R.rb, R.rd, &wm);
+  R.hinv[0] = 1.0 / h.L1xp;
+  R.hinv[1] = 1.0 / h.L2yp;
+  R.hinv[2] = 1.0 / h.L;
+  R.kb[0] = 2 * M_PI / h.L1xp;
+  R.kb[1] = -2 * M_PI * h.L2xp / (h.L1xp * h.L2yp);
+  R.kb[2] = 2 * M_PI / h.L2yp;
+  R.kb[3] = 2 * M_PI / h.L;
+  R.vol = fabs(h.L1xp * h.L2yp) * h.L;
+  R.treal = h.treal;
+  // cell grid for the neighbour build at this lattice
+  double w[2];
+  reduced_widths(h, w);
+  double cut = std::max(sqrt(R.rlist2), R.rv + 1.0);
+  int n1 = std::max(1, (int)floor(w[0] / cut)), n2 = std::max(1, (int)floor(w[1] / cut));
+  int n3 = std::max(1, (int)floor(h.L / cut));
+  while ((long long)n1 * n2 * n3 > c->max_w_cells[r]) {
+    if (n1 >= n2 && n1 >= n3 && n1 > 1)
+      n1--;
+    else if (n2 >= n3 && n2 > 1)
+      n2--;
+    else
+      n3--;
+  }
+  R.ncell[0] = n1;
+  R.ncell[1] = n2;
+  R.ncell[2] = n3;
+}
+
+static int upload_reps(smdlt_ctx* c) {
+  HCK(hipMemcpyAsync(c->d_reps, c->rh.data(), sizeof(RepDev) * c->nrep, hipMemcpyHostToDevice, c->stream));
+  return 0;
+}
+
+static void set_active(smdlt_ctx* c, int rb, int re) {
+  for (int r = 0; r < c->nrep; ++r) c->rh[r].active = (r >= rb && r < re) ? 1 : 0;
+  c->cur_rep_begin = rb;
+  c->cur_rep_end = re;
+}
+
+template <typename T> static int alloc_dev(T** p, size_t n) {
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+  if (e != hipSuccess) {
+    last_hip_error() = hipGetErrorString(e);
+    return SMDLT_E_NOMEM;
+  }
+  hipMemset(*p, 0, n * sizeof(T));
+  return 0;
+}
+
+extern "C" {
+
+const char* smdlt_version(void) { return "smdlt-mi355x 0.1 (gfx950)"; }
+
+const char* smdlt_strerror(int code) {
+  switch (code) {
+    case SMDLT_OK: return "ok";
+    case SMDLT_E_ARG: return "bad argument";
+    case SMDLT_E_HIP: return last_hip_error().empty() ? "HIP runtime error" : last_hip_error().c_str();
+    case SMDLT_E_FENE_OVERSTRETCH: return "FENE overstretch";
+    case SMDLT_E_LANCZOS_NOCONV: return "Lanczos did not converge";
+    case SMDLT_E_NLIST_OVERFLOW: return "neighbour list capacity exceeded";
+    case SMDLT_E_GEOMETRY: return "cutoff too large for the box";
+    case SMDLT_E_STATE: return "call order violated";
+    case SMDLT_E_NOMEM: return "device allocation failed";
+    case SMDLT_E_MC_STREAM: return "MC uniform stream exhausted";
+    default: return "unknown error";
+  }
+}
+
+void smdlt_default_params(smdlt_params* p) {
+  memset(p, 0, sizeof(*p));
+  p->nb = 50;
+  p->nc = 20;
+  p->L = 60.0;
+  p->flow_type = SMDLT_FLOW_SHEAR;
+  p->edot = 0.1;
+  p->dt = 0.0005;
+  p->spring_fene = 0;
+  p->epsilon = 0.1;
+  p->kappab = 0.0;
+  p->kappas = 200.0;
+  p->qmax = 2.0;
+  p->lambda_b = 3.59;
+  p->lambda_d = 5.0;
+  p->barrier = 3.0;
+  p->barrier2 = 3.0;
+  p->lj_ext_form = 0;
+  p->pbc_compat = 1;
+  p->precision = SMDLT_PREC_MIXED;
+  p->noise_mode = SMDLT_NOISE_LANCZOS;
+  p->mobility_tol = 1e-5;
+  p->lanczos_tol = 1e-6;
+  p->lanczos_maxit = 30;
+  p->seed = 0x5EED0000ull;
+}
+
+int smdlt_num_beads(const smdlt_ctx* c, int replica) {
+  if (!c) return SMDLT_E_ARG;
+  if (replica < 0) {
+    int s = 0;
+    for (int r = 0; r < c->nrep; ++r) s += c->rh[r].n;
+    return s;
+  }
+  if (replica >= c->nrep) return SMDLT_E_ARG;
+  return c->rh[replica].n;
+}
+
+static int choose_mobility(smdlt_ctx* c, int r) {
+  const smdlt_params& p = c->P[r];
+  RepDev& R = c->rh[r];
+  HostLattice h = c->lat[r];
+  // geometry over one lattice period: smallest reduced width, longest lattice vectors
+  double wmin = 1e300, a1max = 0, a2max = 0;
+  int nsamp = (h.tp == INT_MAX) ? 1 : 65;
+  for (int k = 0; k < nsamp; ++k) {
+    uint64_t t = (h.tp == INT_MAX) ? 0 : (uint64_t)((double)k / (nsamp - 1) * (h.tp - 1));
+    h.update(t);
+    double w[2];
+    reduced_widths(h, w);
+    wmin = std::min(wmin, std::min(std::min(w[0], w[1]), h.L));
+    a1max = std::max(a1max, fabs(h.L1xp));
+    a2max = std::max(a2max, sqrt(h.L2xp * h.L2xp + h.L2yp * h.L2yp));
+  }
+  double tol = p.mobility_tol > 0 ? p.mobility_tol : 1e-5;
+  double skin = p.skin > 0 ? p.skin : 1.0;
+  double rc = p.rc_real;
+  if (rc <= 0) rc = std::min(8.0, 0.45 * wmin - skin);
+  if (rc + skin >= 0.5 * wmin) return SMDLT_E_GEOMETRY;
+  double alpha = p.ewald_alpha > 0 ? p.ewald_alpha : choose_alpha(rc, 0.1 * tol);
+  int order = p.spline_order > 0 ? p.spline_order : 6;
+  if (order != 4 && order != 6 && order != 8) return SMDLT_E_ARG;
+  int K[3];
+  if (p.grid[0] > 0 && p.grid[1] > 0 && p.grid[2] > 0) {
+    for (int d = 0; d < 3; ++d) K[d] = p.grid[d];
+  } else {
+    double kmax = choose_kmax(alpha, 0.1 * tol);
+    // grid oversampling for the B-spline interpolation error of this order
+    double sigma = (order >= 8) ? 1.25 : (order >= 6) ? 1.45 : 1.9;
+    double len[3] = {a1max, a2max, h.L};
+    for (int d = 0; d < 3; ++d) K[d] = next_pow2((int)ceil(sigma * kmax * len[d] / M_PI));
+  }
+  for (int d = 0; d < 3; ++d)
+    if (K[d] < 16 || K[d] > KMAX || (K[d] & (K[d] - 1))) return SMDLT_E_ARG;
+  R.K[0] = K[0];
+  R.K[1] = K[1];
+  R.K[2] = K[2];
+  R.order = order;
+  R.alpha = alpha;
+  R.rc2 = rc * rc;
+  R.rlist2 = (rc + skin) * (rc + skin);
+  R.skin = skin;
+  R.selfconst = ewald_selfconst(alpha);
+  // cell capacity over the period
+  double cut = std::max(rc + skin, 6.0 + 1.0);
+  double wmax = 0;
+  h = c->lat[r];
+  for (int k = 0; k < nsamp; ++k) {
+    uint64_t t = (h.tp == INT_MAX) ? 0 : (uint64_t)((double)k / (nsamp - 1) * (h.tp - 1));
+    h.update(t);
+    double w[2];
+    reduced_widths(h, w);
+    wmax = std::max(wmax, std::max(w[0], w[1]));
+  }
+  int nmaxc = std::max(1, (int)floor(wmax / cut));
+  int nz = std::max(1, (int)floor(h.L / cut));
+  c->max_w_cells[r] = nmaxc * nmaxc * nz;
+  return 0;
+}
+
+int smdlt_create(const smdlt_params* params, int n_replicas, int device, smdlt_ctx** out) {
+  if (!params || n_replicas <= 0 || !out) return SMDLT_E_ARG;
+  *out = nullptr;
+  HCK(hipSetDevice(device));
+  smdlt_ctx* c = new smdlt_ctx();
+  c->device = device;
+  c->nrep = n_replicas;
+  c->P.assign(params, params + n_replicas);
+  c->prec = params[0].precision;
+  c->lat.resize(n_replicas);
+  c->rh.assign(n_replicas, RepDev());
+  memset(c->rh.data(), 0, sizeof(RepDev) * n_replicas);
+  c->ct_prev.assign(n_replicas, 1.0);
+  c->st_prev.assign(n_replicas, 0.0);
+  c->max_w_cells.assign(n_replicas, 1);
+  c->nq.assign(n_replicas, 0);
+  c->q0.assign(n_replicas, 0);
+  c->last_iters.assign(n_replicas, 0);
+  int slot = 0, wave = 0, maxit = 1;
+  long long g = 0;
+  int bsp_off = 0, cells = 0;
+  for (int r = 0; r < n_replicas; ++r) {
+    const smdlt_params& p = params[r];
+    if (p.nb <= 0 || p.nc <= 0 || p.L <= 0 || p.dt <= 0 || p.precision != c->prec) {
+      delete c;
+      return SMDLT_E_ARG;
+    }
+    RepDev& R = c->rh[r];
+    R.n = p.nb * p.nc;
+    R.nb = p.nb;
+    R.nc = p.nc;
+    R.slot0 = slot;
+    R.wave0 = wave;
+    int ps = pad_slots(R.n);
+    R.nwaves = ps / SLOT_ALIGN;
+    slot += ps;
+    wave += R.nwaves;
+    R.flow = p.flow_type;
+    R.pbc_compat = p.pbc_compat;
+    R.lj_ext = p.lj_ext_form;
+    R.spring_fene = p.spring_fene;
+    R.dt = p.dt;
+    R.edot = (p.flow_type == 0) ? 0.0 : p.edot;
+    R.kappas = p.kappas;
+    R.qmax = p.qmax;
+    R.kappab = p.kappab;
+    R.epsilon = p.epsilon;
+    R.lambda_b = p.lambda_b;
+    R.kappa_d = 1.0 / p.lambda_d;
+    R.barrier = p.barrier;
+    R.barrier2 = p.barrier2;
+    R.r2cut = p.lj_ext_form ? 5.03968 : 25.0;
+    R.rv = 6.0;
+    R.rnew = 0.5 * (R.rv - sqrt(R.r2cut));
+    R.sqrt2dt = sqrt(2.0 * p.dt);
+    R.active = 1;
+    c->lat[r].init(p.flow_type, p.L, R.edot, p.dt, p.pbc_compat);
+    c->ct_prev[r] = c->lat[r].ct;
+    c->st_prev[r] = c->lat[r].st;
+    int st = choose_mobility(c, r);
+    if (st) {
+      delete c;
+      return st;
+    }
+    R.grid0 = g;
+    g += 3LL * R.K[0] * R.K[1] * R.K[2];
+    R.bsp0 = bsp_off;
+    bsp_off += R.K[0] + R.K[1] + R.K[2];
+    R.cell0 = cells;
+    cells += c->max_w_cells[r];
+    maxit = std::max(maxit, p.lanczos_maxit > 0 ? p.lanczos_maxit : 30);
+    fill_lattice(c, r);
+  }
+  if (maxit > LZ_MAXIT) {
+    delete c;
+    return SMDLT_E_ARG;
+  }
+  c->maxit = maxit;
+  c->nst = slot;
+  c->nwaves = wave;
+  c->grid_total = g;
+  c->ncells = cells;
+  int st = 0;
+  HCK(hipStreamCreate(&c->stream));
+  size_t n3 = 3 * (size_t)c->nst;
+  st |= alloc_dev(&c->d_reps, n_replicas);
+  st |= alloc_dev(&c->d_slot_rep, c->nst);
+  st |= alloc_dev(&c->d_seeds, n_replicas);
+  st |= alloc_dev(&c->rx, c->nst);
+  st |= alloc_dev(&c->ry, c->nst);
+  st |= alloc_dev(&c->rz, c->nst);
+  st |= alloc_dev(&c->dxv, c->nst);
+  st |= alloc_dev(&c->dyv, c->nst);
+  st |= alloc_dev(&c->dzv, c->nst);
+  st |= alloc_dev(&c->dsp, n3);
+  st |= alloc_dev(&c->f, n3);
+  st |= alloc_dev(&c->z, n3);
+  st |= alloc_dev(&c->mf, n3);
+  st |= alloc_dev(&c->bz, n3);
+  st |= alloc_dev(&c->tmp, n3);
+  st |= alloc_dev(&c->tmp2, n3);
+  st |= alloc_dev(&c->nrpy, c->nst);
+  st |= alloc_dev(&c->nlj, c->nst);
+  st |= alloc_dev(&c->lrpy, (size_t)c->cap_rpy * c->nst);
+  st |= alloc_dev(&c->llj, (size_t)c->cap_lj * c->nst);
+  st |= alloc_dev(&c->overflow, 1);
+  st |= alloc_dev(&c->d_err, 4);
+  st |= alloc_dev(&c->cell_of, c->nst);
+  st |= alloc_dev(&c->cell_cnt, c->ncells + 1);
+  st |= alloc_dev(&c->cell_start, c->ncells + 1);
+  st |= alloc_dev(&c->cell_fill, c->ncells + 1);
+  st |= alloc_dev(&c->cell_beads, c->nst);
+  st |= alloc_dev(&c->rep_flags, n_replicas);
+  st |= alloc_dev(&c->out2, 2 * n_replicas);
+  st |= alloc_dev(&c->aff, n_replicas);
+  size_t esz = (c->prec == SMDLT_PREC_FP64) ? 8 : 4;
+  {
+    hipError_t e = hipMalloc(&c->grid, std::max<size_t>(1, (size_t)c->grid_total * esz));
+    if (e != hipSuccess) st |= SMDLT_E_NOMEM;
+    e = hipMalloc(&c->bsp, std::max<size_t>(1, (size_t)bsp_off * esz));
+    if (e != hipSuccess) st |= SMDLT_E_NOMEM;
+    e = hipMalloc(&c->tw, KMAX * 2 * esz);
+    if (e != hipSuccess) st |= SMDLT_E_NOMEM;
+  }
+  st |= alloc_dev(&c->V, (size_t)(c->maxit + 1) * n3);
+  st |= alloc_dev(&c->W, n3);
+  st |= alloc_dev(&c->partial, c->nwaves + 1);
+  st |= alloc_dev(&c->scal, (size_t)SCAL_STRIDE * n_replicas);
+  st |= alloc_dev(&c->bdiv, n_replicas);
+  st |= alloc_dev(&c->done, n_replicas);
+  st |= alloc_dev(&c->iters, n_replicas);
+  st |= alloc_dev(&c->mc_err, 4);
+  if (st) {
+    smdlt_destroy(c);
+    return SMDLT_E_NOMEM;
+  }
+  // slot -> replica map, seeds
+  std::vector<int> sr(c->nst, -1);
+  std::vector<uint64_t> seeds(n_replicas);
+  for (int r = 0; r < n_replicas; ++r) {
+    for (int s = 0; s < pad_slots(c->rh[r].n); ++s) sr[c->rh[r].slot0 + s] = r;
+    seeds[r] = params[r].seed;
+  }
+  HCK(hipMemcpy(c->d_slot_rep, sr.data(), sizeof(int) * c->nst, hipMemcpyHostToDevice));
+  HCK(hipMemcpy(c->d_seeds, seeds.data(), sizeof(uint64_t) * n_replicas, hipMemcpyHostToDevice));
+  // twiddles exp(-2 pi i t / KMAX) and B-spline moduli
+  {
+    std::vector<double> twd(2 * KMAX);
+    for (int t = 0; t < KMAX; ++t) {
+      twd[2 * t] = cos(2 * M_PI * t / KMAX);
+      twd[2 * t + 1] = -sin(2 * M_PI * t / KMAX);
+    }
+    std::vector<double> bs(bsp_off);
+    for (int r = 0; r < n_replicas; ++r) {
+      int off = c->rh[r].bsp0;
+      for (int d = 0; d < 3; ++d) {
+        std::vector<double> m;
+        bspline_moduli(c->rh[r].K[d], c->rh[r].order, m);
+        std::copy(m.begin(), m.end(), bs.begin() + off);
+        off += c->rh[r].K[d];
+      }
+    }
+    if (esz == 8) {
+      HCK(hipMemcpy(c->tw, twd.data(), 16 * KMAX, hipMemcpyHostToDevice));
+      HCK(hipMemcpy(c->bsp, bs.data(), 8 * bs.size(), hipMemcpyHostToDevice));
+    } else {
+      std::vector<float> tf(twd.begin(), twd.end()), bf(bs.begin(), bs.end());
+      HCK(hipMemcpy(c->tw, tf.data(), 8 * KMAX, hipMemcpyHostToDevice));
+      HCK(hipMemcpy(c->bsp, bf.data(), 4 * bf.size(), hipMemcpyHostToDevice));
+    }
+  }
+  // FFT task tables, grouped by replica
+  {
+    std::vector<FftTask> vx, vy, vz;
+    c->tx0.assign(n_replicas + 1, 0);
+    c->ty0.assign(n_replicas + 1, 0);
+    c->tz0.assign(n_replicas + 1, 0);
+    for (int r = 0; r < n_replicas; ++r) {
+      const RepDev& R = c->rh[r];
+      int K0 = R.K[0], K1 = R.K[1], K2 = R.K[2], Kh = K0 / 2;
+      c->tx0[r] = (int)vx.size();
+      c->ty0[r] = (int)vy.size();
+      c->tz0[r] = (int)vz.size();
+      int nlx = std::max(1, std::min(64, FFT_TILE / K0));
+      int nly = std::max(1, std::min(64, FFT_TILE / K1));
+      int nlz = std::max(1, std::min(64, FFT_TILE / K2));
+      for (int comp = 0; comp < 3; ++comp) {
+        int pairs = K2 * K1 / 2;
+        for (int l0 = 0; l0 < pairs; l0 += nlx) vx.push_back({r, comp, l0, std::min(nlx, pairs - l0)});
+        int lines = K2 * Kh;
+        for (int l0 = 0; l0 < lines; l0 += nly) vy.push_back({r, comp, l0, std::min(nly, lines - l0)});
+      }
+      int zl = K1 * Kh;
+      for (int l0 = 0; l0 < zl; l0 += nlz) vz.push_back({r, -1, l0, std::min(nlz, zl - l0)});
+    }
+    c->tx0[n_replicas] = (int)vx.size();
+    c->ty0[n_replicas] = (int)vy.size();
+    c->tz0[n_replicas] = (int)vz.size();
+    c->ntx = (int)vx.size();
+    c->nty = (int)vy.size();
+    c->ntz = (int)vz.size();
+    st |= alloc_dev(&c->tx, vx.size());
+    st |= alloc_dev(&c->ty, vy.size());
+    st |= alloc_dev(&c->tz, vz.size());
+    if (st) {
+      smdlt_destroy(c);
+      return SMDLT_E_NOMEM;
+    }
+    HCK(hipMemcpy(c->tx, vx.data(), sizeof(FftTask) * vx.size(), hipMemcpyHostToDevice));
+    HCK(hipMemcpy(c->ty, vy.data(), sizeof(FftTask) * vy.size(), hipMemcpyHostToDevice));
+    HCK(hipMemcpy(c->tz, vz.data(), sizeof(FftTask) * vz.size(), hipMemcpyHostToDevice));
+  }
+  set_active(c, 0, n_replicas);
+  if (upload_reps(c)) {
+    smdlt_destroy(c);
+    return SMDLT_E_HIP;
+  }
+  HCK(hipStreamSynchronize(c->stream));
+  *out = c;
+  return 0;
+}
+
+void smdlt_destroy(smdlt_ctx* c) {
+  if (!c) return;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->gexec) hipGraphExecDestroy(c->gexec);
+  void* ptrs[] = {c->d_reps, c->d_slot_rep, c->d_seeds, c->rx, c->ry, c->rz, c->dxv, c->dyv, c->dzv, c->dsp, c->f,
+                  c->z, c->mf, c->bz, c->tmp, c->tmp2, c->nrpy, c->lrpy, c->nlj, c->llj, c->overflow, c->d_err,
+                  c->cell_of, c->cell_cnt, c->cell_start, c->cell_fill, c->cell_beads, c->rep_flags, c->out2, c->aff,
+                  c->grid, c->bsp, c->tw, c->tx, c->ty, c->tz, c->V, c->W, c->partial, c->scal, c->bdiv, c->done,
+                  c->iters, c->qtrack, c->qsign, c->qold, c->qidx, c->mc_err, c->mlp_w, c->occ, c->log_off};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+  for (auto& t : c->timers) {
+    hipEventDestroy(t.a);
+    hipEventDestroy(t.b);
+  }
+  for (auto e : c->pool) hipEventDestroy(e);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int smdlt_get_mobility_config(const smdlt_ctx* c, int r, double out[7]) {
+  if (!c || r < 0 || r >= c->nrep || !out) return SMDLT_E_ARG;
+  const RepDev& R = c->rh[r];
+  out[0] = R.alpha;
+  out[1] = sqrt(R.rc2);
+  out[2] = R.skin;
+  out[3] = R.K[0];
+  out[4] = R.K[1];
+  out[5] = R.K[2];
+  out[6] = R.order;
+  return 0;
+}
+
+static int gather_soa(smdlt_ctx* c, const double* h, double* d) {
+  // host concatenated (unpadded) -> device slots
+  std::vector<double> buf(c->nst, 0.0);
+  int k = 0;
+  for (int r = 0; r < c->nrep; ++r)
+    for (int i = 0; i < c->rh[r].n; ++i) buf[c->rh[r].slot0 + i] = h[k++];
+  HCK(hipMemcpyAsync(d, buf.data(), sizeof(double) * c->nst, hipMemcpyHostToDevice, c->stream));
+  HCK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+static int scatter_soa(smdlt_ctx* c, const double* d, double* h) {
+  std::vector<double> buf(c->nst);
+  HCK(hipMemcpyAsync(buf.data(), d, sizeof(double) * c->nst, hipMemcpyDeviceToHost, c->stream));
+  HCK(hipStreamSynchronize(c->stream));
+  int k = 0;
+  for (int r = 0; r < c->nrep; ++r)
+    for (int i = 0; i < c->rh[r].n; ++i) h[k++] = buf[c->rh[r].slot0 + i];
+  return 0;
+}
+static int gather_aos(smdlt_ctx* c, const double* h, double* d) {
+  std::vector<double> buf(3 * (size_t)c->nst, 0.0);
+  size_t k = 0;
+  for (int r = 0; r < c->nrep; ++r)
+    for (int i = 0; i < c->rh[r].n; ++i)
+      for (int q = 0; q < 3; ++q) buf[3 * (size_t)(c->rh[r].slot0 + i) + q] = h[k++];
+  HCK(hipMemcpyAsync(d, buf.data(), sizeof(double) * 3 * c->nst, hipMemcpyHostToDevice, c->stream));
+  HCK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+static int scatter_aos(smdlt_ctx* c, const double* d, double* h) {
+  std::vector<double> buf(3 * (size_t)c->nst);
+  HCK(hipMemcpyAsync(buf.data(), d, sizeof(double) * 3 * c->nst, hipMemcpyDeviceToHost, c->stream));
+  HCK(hipStreamSynchronize(c->stream));
+  size_t k = 0;
+  for (int r = 0; r < c->nrep; ++r)
+    for (int i = 0; i < c->rh[r].n; ++i)
+      for (int q = 0; q < 3; ++q) h[k++] = buf[3 * (size_t)(c->rh[r].slot0 + i) + q];
+  return 0;
+}
+
+int smdlt_upload_positions(smdlt_ctx* c, const double* rx, const double* ry, const double* rz) {
+  if (!c || !rx || !ry || !rz) return SMDLT_E_ARG;
+  HCK(hipSetDevice(c->device));
+  int st = gather_soa(c, rx, c->rx);
+  st = st ? st : gather_soa(c, ry, c->ry);
+  st = st ? st : gather_soa(c, rz, c->rz);
+  if (st) return st;
+  launch_fill_f64(c->stream, c->dxv, 0.0, c->nst);
+  launch_fill_f64(c->stream, c->dyv, 0.0, c->nst);
+  launch_fill_f64(c->stream, c->dzv, 0.0, c->nst);
+  launch_fill_f64(c->stream, c->dsp, 0.0, 3LL * c->nst);
+  c->have_pos = true;
+  c->lists_built = false;
+  HCK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int smdlt_download_positions(smdlt_ctx* c, double* rx, double* ry, double* rz) {
+  if (!c || !rx || !ry || !rz) return SMDLT_E_ARG;
+  HCK(hipSetDevice(c->device));
+  int st = scatter_soa(c, c->rx, rx);
+  st = st ? st : scatter_soa(c, c->ry, ry);
+  st = st ? st : scatter_soa(c, c->rz, rz);
+  return st;
+}
+
+static int lattice_to_device(smdlt_ctx* c, uint64_t t) {
+  for (int r = 0; r < c->nrep; ++r) {
+    c->ct_prev[r] = c->lat[r].ct;
+    c->st_prev[r] = c->lat[r].st;
+    c->lat[r].update(t);
+    fill_lattice(c, r);
+    c->rh[r].reset = (c->lat[r].treal == 0 && t > 0) ? 1 : 0;
+    c->rh[r].step = t;
+  }
+  return upload_reps(c);
+}
+
+int smdlt_update_lattice(smdlt_ctx* c, uint64_t t, double* out) {
+  if (!c) return SMDLT_E_ARG;
+  HCK(hipSetDevice(c->device));
+  int st = lattice_to_device(c, t);
+  if (st) return st;
+  if (out)
+    for (int r = 0; r < c->nrep; ++r) {
+      const HostLattice& h = c->lat[r];
+      double v[8] = {h.L1[0], h.L1[1], h.L2[0], h.L2[1], h.L1xp, h.L2xp, h.L2yp, h.delx};
+      memcpy(out + 8 * r, v, sizeof(v));
+    }
+  HCK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+static int remap_internal(smdlt_ctx* c, uint64_t t) {
+  StageScope sc(c, ST_REMAP);
+  int st = lattice_to_device(c, t);
+  if (st) return st;
+  const RepDev& a = c->rh[c->cur_rep_begin];
+  const RepDev& b = c->rh[c->cur_rep_end - 1];
+  int s0 = a.slot0, ns = b.slot0 + pad_slots(b.n) - s0;
+  launch_remap(c->stream, c->d_reps, c->d_slot_rep, s0, ns, c->rx, c->ry, c->rz, c->dxv, c->dyv, c->dzv, c->dsp, 1);
+  HCK(hipGetLastError());
+  return 0;
+}
+
+int smdlt_remap(smdlt_ctx* c, uint64_t t) {
+  if (!c) return SMDLT_E_ARG;
+  if (!c->have_pos) return SMDLT_E_STATE;
+  HCK(hipSetDevice(c->device));
+  int st = remap_internal(c, t);
+  if (st) return st;
+  HCK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int smdlt_apply_pbc(smdlt_ctx* c) {
+  if (!c || !c->have_pos) return SMDLT_E_STATE;
+  HCK(hipSetDevice(c->device));
+  int st = upload_reps(c);
+  if (st) return st;
+  launch_remap(c->stream, c->d_reps, c->d_slot_rep, 0, c->nst, c->rx, c->ry, c->rz, c->dxv, c->dyv, c->dzv, c->dsp, 0);
+  HCK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int smdlt_rotate(smdlt_ctx* c, int inverse) {
+  if (!c || !c->have_pos) return SMDLT_E_STATE;
+  HCK(hipSetDevice(c->device));
+  launch_remap(c->stream, c->d_reps, c->d_slot_rep, 0, c->nst, c->rx, c->ry, c->rz, c->dxv, c->dyv, c->dzv, c->dsp,
+               inverse ? 3 : 2);
+  HCK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int smdlt_min_image(smdlt_ctx* c, int replica, int npairs, const int* i, const int* j, double* out) {
+  if (!c || replica < 0 || replica >= c->nrep || npairs < 0 || (npairs && (!i || !j || !out))) return SMDLT_E_ARG;
+  if (npairs == 0) return 0;
+  HCK(hipSetDevice(c->device));
+  for (int k = 0; k < npairs; ++k)
+    if (i[k] < 0 || j[k] < 0 || i[k] >= c->rh[replica].n || j[k] >= c->rh[replica].n) return SMDLT_E_ARG;
+  int *di, *dj;
+  double* dout;
+  HCK(hipMalloc(&di, 4 * npairs));
+  HCK(hipMalloc(&dj, 4 * npairs));
+  HCK(hipMalloc(&dout, 24 * (size_t)npairs));
+  HCK(hipMemcpy(di, i, 4 * npairs, hipMemcpyHostToDevice));
+  HCK(hipMemcpy(dj, j, 4 * npairs, hipMemcpyHostToDevice));
+  launch_min_image(c->stream, c->d_reps + replica, c->rx, c->ry, c->rz, npairs, di, dj, dout);
+  HCK(hipStreamSynchronize(c->stream));
+  HCK(hipMemcpy(out, dout, 24 * (size_t)npairs, hipMemcpyDeviceToHost));
+  hipFree(di);
+  hipFree(dj);
+  hipFree(dout);
+  return 0;
+}
+
+int smdlt_bin_index(smdlt_ctx* c, int replica, double bin_size, int npairs, const int* i, const int* j, int* bins) {
+  if (!c || replica < 0 || replica >= c->nrep || npairs < 0 || bin_size <= 0) return SMDLT_E_ARG;
+  if (npairs == 0) return 0;
+  HCK(hipSetDevice(c->device));
+  // table geometry of initLattice (S/Initialization.c:247-296, X/Initialization.c:258-285)
+  HostLattice h0;
+  const smdlt_params& p = c->P[replica];
+  h0.init(p.flow_type, p.L, c->rh[replica].edot, p.dt, p.pbc_compat);
+  double xmax, ymax, zmax = p.L / 2;
+  if (h0.flow == 2) {
+    ymax = -h0.point0[1][1];
+    xmax = h0.point0[2][0] * exp(h0.edot * h0.tp * h0.dt);
+  } else {
+    ymax = p.L / 2;
+    int treal = (h0.tp == INT_MAX) ? 0 : h0.tp;
+    xmax = p.L / 2 * h0.edot * treal * h0.dt;
+  }
+  double bs[3] = {bin_size, bin_size, bin_size};
+  int nbx = 2 * (int)ceil(xmax / bs[0]) + 1;
+  bs[0] = xmax / ((nbx - 1) / 2);
+  int nby = 2 * (int)ceil(ymax / bs[1]) + 1;
+  bs[1] = ymax / ((nby - 1) / 2);
+  int nbz = 2 * (int)ceil(zmax / bs[2]) + 1;
+  bs[2] = zmax / ((nbz - 1) / 2);
+  const HostLattice& h = c->lat[replica];
+  int nmax = (h.treal <= h.tp / 2) ? 1 : 2;
+  int nbv[6] = {nby, nbz, (nbx - 1) / 2, (nby - 1) / 2, (nbz - 1) / 2, nmax};
+  int *di, *dj, *dout, *dnb;
+  double* dbs;
+  HCK(hipMalloc(&di, 4 * npairs));
+  HCK(hipMalloc(&dj, 4 * npairs));
+  HCK(hipMalloc(&dout, 4 * npairs));
+  HCK(hipMalloc(&dnb, sizeof(nbv)));
+  HCK(hipMalloc(&dbs, sizeof(bs)));
+  HCK(hipMemcpy(di, i, 4 * npairs, hipMemcpyHostToDevice));
+  HCK(hipMemcpy(dj, j, 4 * npairs, hipMemcpyHostToDevice));
+  HCK(hipMemcpy(dnb, nbv, sizeof(nbv), hipMemcpyHostToDevice));
+  HCK(hipMemcpy(dbs, bs, sizeof(bs), hipMemcpyHostToDevice));
+  launch_bin_index(c->stream, c->d_reps + replica, c->rx, c->ry, c->rz, npairs, di, dj, dbs, dnb, dout);
+  HCK(hipStreamSynchronize(c->stream));
+  HCK(hipMemcpy(bins, dout, 4 * npairs, hipMemcpyDeviceToHost));
+  hipFree(di);
+  hipFree(dj);
+  hipFree(dout);
+  hipFree(dnb);
+  hipFree(dbs);
+  return 0;
+}
+
+// Device-side neighbour maintenance for the active replica range.
+static int neighbors_internal(smdlt_ctx* c, int force) {
+  StageScope sc(c, ST_NLIST);
+  int rb = c->cur_rep_begin, re = c->cur_rep_end;
+  const RepDev& a = c->rh[rb];
+  const RepDev& b = c->rh[re - 1];
+  int s0 = a.slot0, ns = b.slot0 + pad_slots(b.n) - s0;
+  int cb = a.cell0, ce = b.cell0 + c->max_w_cells[re - 1];
+  launch_disp_max(c->stream, c->d_reps, rb, re - rb, c->dxv, c->dyv, c->dzv, c->dsp, c->out2);
+  launch_nl_check(c->stream, c->d_reps, rb, re - rb, c->out2, c->aff, force | (c->lists_built ? 0 : 3), c->rep_flags);
+  launch_fill_i32(c->stream, c->cell_cnt + cb, 0, ce - cb);
+  launch_fill_i32(c->stream, c->cell_fill + cb, 0, ce - cb);
+  launch_cell_count(c->stream, c->d_reps, c->d_slot_rep, s0, ns, c->rx, c->ry, c->rz, c->cell_of, c->cell_cnt);
+  launch_scan(c->stream, c->cell_cnt + cb, c->cell_start + cb, ce - cb);
+  launch_cell_fill(c->stream, c->d_reps, c->d_slot_rep, s0, ns, c->cell_of, c->cell_start, c->cell_fill, c->cell_beads);
+  launch_build_lists(c->stream, c->d_reps, c->d_slot_rep, s0, ns, c->rx, c->ry, c->rz, c->cell_start, c->cell_cnt,
+                     c->cell_beads, c->rep_flags, c->nst, c->cap_rpy, c->nrpy, c->lrpy, c->cap_lj, c->nlj, c->llj,
+                     c->overflow);
+  launch_reset_trackers(c->stream, c->d_reps, c->d_slot_rep, s0, ns, c->rep_flags, c->dxv, c->dyv, c->dzv, c->dsp);
+  HCK(hipGetLastError());
+  c->lists_built = true;
+  return 0;
+}
+
+static int grow_lists(smdlt_ctx* c) {
+  int ov = 0;
+  HCK(hipMemcpy(&ov, c->overflow, 4, hipMemcpyDeviceToHost));
+  if (!ov) return 0;
+  // measure and grow
+  c->cap_rpy *= 2;
+  c->cap_lj *= 2;
+  hipFree(c->lrpy);
+  hipFree(c->llj);
+  c->lrpy = nullptr;
+  c->llj = nullptr;
+  if (alloc_dev(&c->lrpy, (size_t)c->cap_rpy * c->nst) || alloc_dev(&c->llj, (size_t)c->cap_lj * c->nst))
+    return SMDLT_E_NOMEM;
+  HCK(hipMemset(c->overflow, 0, 4));
+  if (c->gexec) {
+    hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+  }
+  return 1;
+}
+
+int smdlt_neighbors(smdlt_ctx* c, int force, int* rebuilt) {
+  if (!c || !c->have_pos) return SMDLT_E_STATE;
+  HCK(hipSetDevice(c->device));
+  int st = upload_reps(c);
+  if (st) return st;
+  for (int attempt = 0; attempt < 8; ++attempt) {
+    st = neighbors_internal(c, force ? 3 : 0);
+    if (st) return st;
+    HCK(hipStreamSynchronize(c->stream));
+    int g = grow_lists(c);
+    if (g < 0) return g;
+    if (g == 0) break;
+    force = 1;
+  }
+  if (rebuilt) {
+    std::vector<int> fl(c->nrep);
+    HCK(hipMemcpy(fl.data(), c->rep_flags, 4 * c->nrep, hipMemcpyDeviceToHost));
+    int n = 0;
+    for (int v : fl) n += (v & 1) ? 1 : 0;
+    *rebuilt = n;
+  }
+  return 0;
+}
+
+int smdlt_get_verlet(smdlt_ctx* c, int replica, int* nlist, int* list, int stride) {
+  if (!c || replica < 0 || replica >= c->nrep || !nlist || !list || stride <= 0) return SMDLT_E_ARG;
+  HCK(hipSetDevice(c->device));
+  const RepDev& R = c->rh[replica];
+  std::vector<int> cnt(R.n), all((size_t)c->cap_lj * c->nst);
+  HCK(hipMemcpy(cnt.data(), c->nlj + R.slot0, 4 * R.n, hipMemcpyDeviceToHost));
+  HCK(hipMemcpy(all.data(), c->llj, 4 * all.size(), hipMemcpyDeviceToHost));
+  for (int i = 0; i < R.n; ++i) {
+    int k = 0;
+    for (int q = 0; q < cnt[i]; ++q) {
+      int j = all[(size_t)q * c->nst + R.slot0 + i] - R.slot0;
+      if (j > i) {
+        if (k >= stride) return SMDLT_E_ARG;
+        list[(size_t)i * stride + k++] = j;
+      }
+    }
+    nlist[i] = k;
+  }
+  return 0;
+}
+
+static int forces_internal(smdlt_ctx* c) {
+  StageScope sc(c, ST_FORCES);
+  int rb = c->cur_rep_begin, re = c->cur_rep_end;
+  const RepDev& a = c->rh[rb];
+  const RepDev& b = c->rh[re - 1];
+  int s0 = a.slot0, ns = b.slot0 + pad_slots(b.n) - s0;
+  launch_forces(c->stream, c->d_reps, c->d_slot_rep, s0, ns, c->rx, c->ry, c->rz, c->nst, c->nlj, c->llj, c->f,
+                c->d_err);
+  if (c->max_nq > 0)
+    launch_coulomb(c->stream, c->d_reps, rb, re - rb, c->rx, c->ry, c->rz, c->qtrack, c->qsign, c->f, c->max_nq);
+  HCK(hipGetLastError());
+  return 0;
+}
+
+static int check_err(smdlt_ctx* c) {
+  int e = 0;
+  HCK(hipMemcpyAsync(&e, c->d_err, 4, hipMemcpyDeviceToHost, c->stream));
+  HCK(hipStreamSynchronize(c->stream));
+  if (e) {
+    HCK(hipMemset(c->d_err, 0, 4));
+    return SMDLT_E_FENE_OVERSTRETCH;
+  }
+  return 0;
+}
+
+int smdlt_forces(smdlt_ctx* c, double* fx, double* fy, double* fz) {
+  if (!c || !c->have_pos) return SMDLT_E_STATE;
+  HCK(hipSetDevice(c->device));
+  if (!c->lists_built) {
+    int st = smdlt_neighbors(c, 1, nullptr);
+    if (st) return st;
+  }
+  int st = upload_reps(c);
+  if (st) return st;
+  st = forces_internal(c);
+  if (st) return st;
+  st = check_err(c);
+  if (st) return st;
+  if (fx || fy || fz) {
+    // forces are in the wrap frame; the reference returns lab-frame forces for KRBC
+    HCK(hipMemcpyAsync(c->tmp, c->f, 24 * (size_t)c->nst, hipMemcpyDeviceToDevice, c->stream));
+    launch_rotate_vec(c->stream, c->d_reps, c->d_slot_rep, 0, c->nst, c->tmp, 1);
+    std::vector<double> h(3 * (size_t)smdlt_num_beads(c, -1));
+    st = scatter_aos(c, c->tmp, h.data());
+    if (st) return st;
+    for (size_t i = 0; i < h.size() / 3; ++i) {
+      if (fx) fx[i] = h[3 * i];
+      if (fy) fy[i] = h[3 * i + 1];
+      if (fz) fz[i] = h[3 * i + 2];
+    }
+  }
+  return 0;
+}
+
+// u = M v for the active replica range (device pointers).  beta/vout/partial/done
+// are the Lanczos hooks (nullptr for a plain product).
+static void mobility_internal(smdlt_ctx* c, const double* v, double* u, const double* beta, double* vout,
+                              double* partial, const int* done) {
+  StageScope sc(c, ST_MOB);
+  int rb = c->cur_rep_begin, re = c->cur_rep_end;
+  const RepDev& a = c->rh[rb];
+  const RepDev& b = c->rh[re - 1];
+  MobArgs m;
+  memset(&m, 0, sizeof(m));
+  m.reps = c->d_reps;
+  m.slot_rep = c->d_slot_rep;
+  m.slot_begin = a.slot0;
+  m.nslots = b.slot0 + pad_slots(b.n) - a.slot0;
+  m.nst = c->nst;
+  m.rx = c->rx;
+  m.ry = c->ry;
+  m.rz = c->rz;
+  m.nrpy = c->nrpy;
+  m.lrpy = c->lrpy;
+  m.v = v;
+  m.beta = beta;
+  m.vout = vout;
+  m.u = u;
+  m.partial = partial;
+  m.done = done;
+  m.grid = c->grid;
+  m.grid_begin = a.grid0;
+  m.grid_len = b.grid0 + 3LL * b.K[0] * b.K[1] * b.K[2] - a.grid0;
+  m.bsp = c->bsp;
+  m.tw = c->tw;
+  m.tx = c->tx + c->tx0[rb];
+  m.ty = c->ty + c->ty0[rb];
+  m.tz = c->tz + c->tz0[rb];
+  m.ntx = c->tx0[re] - c->tx0[rb];
+  m.nty = c->ty0[re] - c->ty0[rb];
+  m.ntz = c->tz0[re] - c->tz0[rb];
+  m.order = c->rh[rb].order;
+  if (c->prec == SMDLT_PREC_FP64)
+    launch_mobility<double>(c->stream, m);
+  else
+    launch_mobility<float>(c->stream, m);
+}
+
+static int ensure_lists(smdlt_ctx* c) {
+  if (c->lists_built) return 0;
+  int st = smdlt_neighbors(c, 1, nullptr);
+  return st;
+}
+
+int smdlt_mobility_apply_device(smdlt_ctx* c, const double* d_v, double* d_u) {
+  if (!c || !d_v || !d_u) return SMDLT_E_ARG;
+  if (!c->have_pos) return SMDLT_E_STATE;
+  HCK(hipSetDevice(c->device));
+  int st = ensure_lists(c);
+  if (st) return st;
+  mobility_internal(c, d_v, d_u, nullptr, nullptr, nullptr, nullptr);
+  HCK(hipGetLastError());
+  return 0;
+}
+
+int smdlt_mobility_apply(smdlt_ctx* c, const double* v, double* u) {
+  if (!c || !v || !u) return SMDLT_E_ARG;
+  if (!c->have_pos) return SMDLT_E_STATE;
+  HCK(hipSetDevice(c->device));
+  int st = ensure_lists(c);
+  if (st) return st;
+  st = upload_reps(c);
+  if (st) return st;
+  st = gather_aos(c, v, c->tmp);
+  if (st) return st;
+  mobility_internal(c, c->tmp, c->tmp2, nullptr, nullptr, nullptr, nullptr);
+  HCK(hipGetLastError());
+  st = scatter_aos(c, c->tmp2, u);
+  collect_timers(c);
+  return st;
+}
+
+// Lanczos on the active replica range: y = M^(1/2) z (device pointers).
+static int brownian_internal(smdlt_ctx* c, const double* z, double* y, double tol, int maxit, bool host_exit) {
+  int rb = c->cur_rep_begin, re = c->cur_rep_end;
+  const RepDev& a = c->rh[rb];
+  const RepDev& b = c->rh[re - 1];
+  int s0 = a.slot0, ns = b.slot0 + pad_slots(b.n) - s0;
+  size_t n3 = 3 * (size_t)c->nst;
+  {
+    StageScope sc(c, ST_LANCZOS);
+    launch_fill_i32(c->stream, c->done + rb, 0, re - rb);
+    lz_wave_dot(c->stream, c->d_reps, c->d_slot_rep, s0, ns, z, nullptr, nullptr, c->partial);
+    lz_rep_reduce(c->stream, c->d_reps, rb, re - rb, c->partial, nullptr, c->scal, SCAL_STRIDE, 4 * LZ_MAXIT, 1,
+                  c->bdiv);
+  }
+  for (int j = 1; j <= maxit; ++j) {
+    const double* vin = (j == 1) ? z : c->W;
+    double* vj = c->V + (size_t)(j - 1) * n3;
+    const double* vjm1 = (j > 1) ? c->V + (size_t)(j - 2) * n3 : nullptr;
+    mobility_internal(c, vin, c->W, c->bdiv, vj, c->partial, c->done);
+    {
+      StageScope sc(c, ST_LANCZOS);
+      lz_rep_reduce(c->stream, c->d_reps, rb, re - rb, c->partial, c->done, c->scal, SCAL_STRIDE, j - 1, 0, nullptr);
+      lz_orth(c->stream, c->d_reps, c->d_slot_rep, s0, ns, c->W, vj, vjm1, c->scal, j, c->done, c->partial);
+      lz_tri(c->stream, c->d_reps, rb, re - rb, c->partial, c->scal, j, tol, maxit, c->done, c->bdiv, c->iters);
+    }
+    if (host_exit && j >= 2) {
+      std::vector<int> d(re - rb);
+      HCK(hipMemcpyAsync(d.data(), c->done + rb, 4 * (re - rb), hipMemcpyDeviceToHost, c->stream));
+      HCK(hipStreamSynchronize(c->stream));
+      bool all = true;
+      for (int v : d) all = all && (v != 0);
+      if (all) break;
+    }
+  }
+  StageScope sc(c, ST_LANCZOS);
+  lz_combine(c->stream, c->d_reps, c->d_slot_rep, s0, ns, c->V, c->nst, c->scal, c->iters, y);
+  HCK(hipGetLastError());
+  return 0;
+}
+
+static int read_iters(smdlt_ctx* c, int* iters, double* resid) {
+  HCK(hipMemcpyAsync(c->last_iters.data(), c->iters, 4 * c->nrep, hipMemcpyDeviceToHost, c->stream));
+  std::vector<double> sc((size_t)SCAL_STRIDE * c->nrep);
+  HCK(hipMemcpyAsync(sc.data(), c->scal, 8 * sc.size(), hipMemcpyDeviceToHost, c->stream));
+  std::vector<int> dn(c->nrep);
+  HCK(hipMemcpyAsync(dn.data(), c->done, 4 * c->nrep, hipMemcpyDeviceToHost, c->stream));
+  HCK(hipStreamSynchronize(c->stream));
+  int st = 0;
+  for (int r = c->cur_rep_begin; r < c->cur_rep_end; ++r) {
+    if (iters) iters[r] = c->last_iters[r];
+    if (resid) resid[r] = sc[(size_t)r * SCAL_STRIDE + 4 * LZ_MAXIT + 2];
+    if (dn[r] != 1) st = SMDLT_E_LANCZOS_NOCONV;
+  }
+  return st;
+}
+
+int smdlt_brownian(smdlt_ctx* c, const double* z, double* y, double tol, int maxit, int* iters, double* resid) {
+  if (!c || !z || !y) return SMDLT_E_ARG;
+  if (!c->have_pos) return SMDLT_E_STATE;
+  if (maxit <= 0) maxit = c->maxit;
+  if (maxit > c->maxit) return SMDLT_E_ARG;
+  if (tol <= 0) tol = 1e-6;
+  HCK(hipSetDevice(c->device));
+  int st = ensure_lists(c);
+  if (st) return st;
+  st = upload_reps(c);
+  if (st) return st;
+  st = gather_aos(c, z, c->z);
+  if (st) return st;
+  st = brownian_internal(c, c->z, c->bz, tol, maxit, true);
+  if (st) return st;
+  int conv = read_iters(c, iters, resid);
+  st = scatter_aos(c, c->bz, y);
+  collect_timers(c);
+  return st ? st : conv;
+}
+
+int smdlt_last_iters(const smdlt_ctx* c, int* iters) {
+  if (!c || !iters) return SMDLT_E_ARG;
+  for (int r = 0; r < c->nrep; ++r) iters[r] = c->last_iters[r];
+  return 0;
+}
+
+// Periodic self-mobility of one bead for the replica's current lattice and
+// split: selfconst + sum_k G(k)_aa (the Ewald sum of its own images).
+static void self_mobility(const RepDev& R, double out[3]) {
+  double a2 = R.alpha * R.alpha;
+  double kmax = choose_kmax(R.alpha, 1e-14);
+  out[0] = out[1] = out[2] = R.selfconst;
+  int m1m = (int)ceil(kmax / fabs(R.kb[0]) * 1.5) + 1, m2m = (int)ceil(kmax / fabs(R.kb[2]) * 1.5) + 1;
+  int m3m = (int)ceil(kmax / R.kb[3]) + 1;
+  for (int m1 = -m1m; m1 <= m1m; ++m1)
+    for (int m2 = -m2m; m2 <= m2m; ++m2)
+      for (int m3 = -m3m; m3 <= m3m; ++m3) {
+        if (!m1 && !m2 && !m3) continue;
+        double kx = m1 * R.kb[0], ky = m1 * R.kb[1] + m2 * R.kb[2], kz = m3 * R.kb[3];
+        double kk = kx * kx + ky * ky + kz * kz;
+        if (kk > kmax * kmax) continue;
+        double q = kk / (4 * a2);
+        double g = (1 - kk / 3) * (1 + q + 2 * q * q) * 6 * M_PI / kk * exp(-q) / R.vol;
+        out[0] += g * (1 - kx * kx / kk);
+        out[1] += g * (1 - ky * ky / kk);
+        out[2] += g * (1 - kz * kz / kk);
+      }
+}
+
+int smdlt_tea_apply(smdlt_ctx* c, const double* beta, const double* C, const double* z, double* y) {
+  if (!c || !beta || !C || !z || !y) return SMDLT_E_ARG;
+  if (!c->have_pos) return SMDLT_E_STATE;
+  HCK(hipSetDevice(c->device));
+  int st = ensure_lists(c);
+  if (st) return st;
+  st = upload_reps(c);
+  if (st) return st;
+  std::vector<double> diag(3 * c->nrep);
+  for (int r = 0; r < c->nrep; ++r) self_mobility(c->rh[r], &diag[3 * r]);
+  double *dd, *db;
+  HCK(hipMalloc(&dd, 8 * diag.size()));
+  HCK(hipMalloc(&db, 8 * c->nrep));
+  HCK(hipMemcpy(dd, diag.data(), 8 * diag.size(), hipMemcpyHostToDevice));
+  HCK(hipMemcpy(db, beta, 8 * c->nrep, hipMemcpyHostToDevice));
+  st = gather_aos(c, z, c->z);
+  if (!st) st = gather_aos(c, C, c->bz);
+  if (st) return st;
+  mobility_internal(c, c->z, c->tmp2, nullptr, nullptr, nullptr, nullptr);
+  lz_tea(c->stream, c->d_slot_rep, 0, c->nst, c->tmp2, c->z, dd, db, c->bz, c->tmp);
+  HCK(hipGetLastError());
+  st = scatter_aos(c, c->tmp, y);
+  hipFree(dd);
+  hipFree(db);
+  return st;
+}
+
+// One timestep for the active replica range; noise_dev: device noise (nullptr =
+// generate).  Lanczos early exit on the host only when host_exit.
+static int step_internal(smdlt_ctx* c, uint64_t t, double* noise_dev, bool host_exit) {
+  int rb = c->cur_rep_begin, re = c->cur_rep_end;
+  const RepDev& a = c->rh[rb];
+  const RepDev& b = c->rh[re - 1];
+  int s0 = a.slot0, ns = b.slot0 + pad_slots(b.n) - s0;
+  int st = remap_internal(c, t);
+  if (st) return st;
+  // host noise is the reference's lab-frame R; rotate it into the wrap frame
+  if (noise_dev) launch_rotate_vec(c->stream, c->d_reps, c->d_slot_rep, s0, ns, noise_dev, 0);
+  st = neighbors_internal(c, 0);
+  if (st) return st;
+  st = forces_internal(c);
+  if (st) return st;
+  const double* zz = noise_dev;
+  {
+    StageScope sc(c, ST_NOISE);
+    if (!zz) {
+      launch_noise(c->stream, c->d_slot_rep, c->d_reps, s0, ns, c->d_seeds, t, c->z);
+      zz = c->z;
+    }
+  }
+  mobility_internal(c, c->f, c->mf, nullptr, nullptr, nullptr, nullptr);
+  const smdlt_params& p = c->P[rb];
+  st = brownian_internal(c, zz, c->bz, p.lanczos_tol > 0 ? p.lanczos_tol : 1e-6, c->maxit, host_exit);
+  if (st) return st;
+  {
+    StageScope sc(c, ST_UPDATE);
+    launch_update_positions(c->stream, c->d_reps, c->d_slot_rep, s0, ns, c->rx, c->ry, c->rz, c->mf, c->bz, c->dxv,
+                            c->dyv, c->dzv, c->dsp, 1);
+  }
+  HCK(hipGetLastError());
+  return 0;
+}
+
+int smdlt_step(smdlt_ctx* c, uint64_t t, const double* noise) {
+  if (!c) return SMDLT_E_ARG;
+  if (!c->have_pos) return SMDLT_E_STATE;
+  HCK(hipSetDevice(c->device));
+  int st = 0;
+  if (noise) {
+    st = gather_aos(c, noise, c->tmp);
+    if (st) return st;
+  }
+  st = step_internal(c, t, noise ? c->tmp : nullptr, true);
+  if (st) return st;
+  st = check_err(c);
+  if (st) return st;
+  int conv = read_iters(c, nullptr, nullptr);
+  int g = grow_lists(c);
+  collect_timers(c);
+  if (g < 0) return g;
+  if (g > 0) return SMDLT_E_NLIST_OVERFLOW;
+  return conv;
+}
+
+int smdlt_run(smdlt_ctx* c, uint64_t t0, int nsteps, int use_graph) {
+  if (!c || nsteps < 0) return SMDLT_E_ARG;
+  if (!c->have_pos) return SMDLT_E_STATE;
+  HCK(hipSetDevice(c->device));
+  int st = ensure_lists(c);
+  if (st) return st;
+  for (int k = 0; k < nsteps; ++k) {
+    uint64_t t = t0 + k;
+    if (!use_graph) {
+      st = step_internal(c, t, nullptr, false);
+      if (st) return st;
+      continue;
+    }
+    // lattice for this step, then replay the captured step body
+    st = lattice_to_device(c, t);
+    if (st) return st;
+    if (!c->gexec) {
+      hipGraph_t graph;
+      bool prof = c->prof;
+      c->prof = false;
+      HCK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      int rb = c->cur_rep_begin, re = c->cur_rep_end;
+      const RepDev& a = c->rh[rb];
+      const RepDev& b = c->rh[re - 1];
+      int s0 = a.slot0, ns = b.slot0 + pad_slots(b.n) - s0;
+      launch_remap(c->stream, c->d_reps, c->d_slot_rep, s0, ns, c->rx, c->ry, c->rz, c->dxv, c->dyv, c->dzv, c->dsp,
+                   1);
+      st = neighbors_internal(c, 0);
+      if (!st) st = forces_internal(c);
+      launch_noise(c->stream, c->d_slot_rep, c->d_reps, s0, ns, c->d_seeds, t, c->z);
+      mobility_internal(c, c->f, c->mf, nullptr, nullptr, nullptr, nullptr);
+      const smdlt_params& p = c->P[rb];
+      if (!st) st = brownian_internal(c, c->z, c->bz, p.lanczos_tol > 0 ? p.lanczos_tol : 1e-6, c->maxit, false);
+      launch_update_positions(c->stream, c->d_reps, c->d_slot_rep, s0, ns, c->rx, c->ry, c->rz, c->mf, c->bz, c->dxv,
+                              c->dyv, c->dzv, c->dsp, 1);
+      hipError_t e = hipStreamEndCapture(c->stream, &graph);
+      c->prof = prof;
+      if (st) return st;
+      HCK(e);
+      HCK(hipGraphInstantiate(&c->gexec, graph, nullptr, nullptr, 0));
+      hipGraphDestroy(graph);
+    }
+    StageScope sc(c, ST_MOB);
+    HCK(hipGraphLaunch(c->gexec, c->stream));
+  }
+  HCK(hipStreamSynchronize(c->stream));
+  st = check_err(c);
+  if (st) return st;
+  int conv = read_iters(c, nullptr, nullptr);
+  int g = grow_lists(c);
+  collect_timers(c);
+  if (g < 0) return g;
+  if (g > 0) return SMDLT_E_NLIST_OVERFLOW;
+  return conv;
+}
+
+int smdlt_set_profiling(smdlt_ctx* c, int on) {
+  if (!c) return SMDLT_E_ARG;
+  collect_timers(c);
+  c->prof = on != 0;
+  for (int s = 0; s < ST_COUNT; ++s) {
+    c->stage_ms[s] = 0;
+    c->stage_cnt[s] = 0;
+  }
+  return 0;
+}
+
+int smdlt_stage_times(const smdlt_ctx* c, double* ms, int* counts, int max_stages) {
+  if (!c || !ms) return SMDLT_E_ARG;
+  collect_timers(const_cast<smdlt_ctx*>(c));
+  int n = std::min(max_stages, (int)ST_COUNT);
+  for (int s = 0; s < n; ++s) {
+    ms[s] = c->stage_ms[s];
+    if (counts) counts[s] = c->stage_cnt[s];
+  }
+  return n;
+}
+
+const char* smdlt_stage_name(int s) { return (s >= 0 && s < ST_COUNT) ? kStageNames[s] : ""; }
+
+int smdlt_sync(smdlt_ctx* c) {
+  if (!c) return SMDLT_E_ARG;
+  HCK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int smdlt_upload_charges(smdlt_ctx* c, const int* nq, const int* track, const int* sign) {
+  if (!c || !nq) return SMDLT_E_ARG;
+  HCK(hipSetDevice(c->device));
+  int tot = 0, mx = 0, maxn = 0;
+  for (int r = 0; r < c->nrep; ++r) {
+    if (nq[r] < 0 || nq[r] > c->rh[r].n) return SMDLT_E_ARG;
+    c->q0[r] = tot;
+    c->nq[r] = nq[r];
+    tot += nq[r];
+    mx = std::max(mx, nq[r]);
+    maxn = std::max(maxn, c->rh[r].n);
+  }
+  for (int k = 0; k < tot; ++k) {
+    if (!track) return SMDLT_E_ARG;
+  }
+  int k = 0;
+  for (int r = 0; r < c->nrep; ++r) {
+    std::vector<char> seen(c->rh[r].n, 0);
+    for (int q = 0; q < nq[r]; ++q, ++k) {
+      int b = track[k];
+      if (b < 0 || b >= c->rh[r].n || seen[b]) return SMDLT_E_ARG;
+      seen[b] = 1;
+      if (sign && sign[k] != 1 && sign[k] != -1) return SMDLT_E_ARG;
+    }
+  }
+  void* ptrs[] = {c->qtrack, c->qsign, c->qold, c->occ, c->log_off};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+  c->qtrack = c->qsign = c->qold = c->log_off = nullptr;
+  c->occ = nullptr;
+  int st = 0;
+  st |= alloc_dev(&c->qtrack, tot);
+  st |= alloc_dev(&c->qsign, tot);
+  st |= alloc_dev(&c->qold, tot);
+  st |= alloc_dev(&c->log_off, c->nrep);
+  c->occ_stride = ((long long)maxn + 255) / 256 * 256;
+  st |= alloc_dev(&c->occ, (size_t)c->occ_stride * c->nrep);
+  if (st) return SMDLT_E_NOMEM;
+  std::vector<int> sg(tot, 1);
+  if (sign) sg.assign(sign, sign + tot);
+  if (tot) {
+    HCK(hipMemcpy(c->qtrack, track, 4 * tot, hipMemcpyHostToDevice));
+    HCK(hipMemcpy(c->qold, track, 4 * tot, hipMemcpyHostToDevice));
+    HCK(hipMemcpy(c->qsign, sg.data(), 4 * tot, hipMemcpyHostToDevice));
+  }
+  HCK(hipMemcpy(c->log_off, c->q0.data(), 4 * c->nrep, hipMemcpyHostToDevice));
+  for (int r = 0; r < c->nrep; ++r) {
+    c->rh[r].q0 = c->q0[r];
+    c->rh[r].nq = c->nq[r];
+  }
+  c->nq_total = tot;
+  c->max_nq = mx;
+  if (c->gexec) {
+    hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+  }
+  return upload_reps(c);
+}
+
+int smdlt_download_charges(smdlt_ctx* c, int* track, int* old) {
+  if (!c) return SMDLT_E_ARG;
+  HCK(hipSetDevice(c->device));
+  if (c->nq_total == 0) return 0;
+  if (track) HCK(hipMemcpy(track, c->qtrack, 4 * c->nq_total, hipMemcpyDeviceToHost));
+  if (old) HCK(hipMemcpy(old, c->qold, 4 * c->nq_total, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int smdlt_mc_sweep(smdlt_ctx* c, int mode, const float* uniforms, int n_uniforms, int* consumed, int* log) {
+  if (!c) return SMDLT_E_ARG;
+  if (!c->have_pos) return SMDLT_E_STATE;
+  if (mode != SMDLT_MC_REFERENCE) return SMDLT_E_ARG;
+  if (uniforms && n_uniforms <= 0) return SMDLT_E_ARG;
+  HCK(hipSetDevice(c->device));
+  int st = upload_reps(c);
+  if (st) return st;
+  StageScope sc(c, ST_MC);
+  float* du = nullptr;
+  int* dcons = nullptr;
+  int* dlog = nullptr;
+  if (uniforms) {
+    HCK(hipMalloc(&du, sizeof(float) * (size_t)n_uniforms * c->nrep));
+    HCK(hipMemcpy(du, uniforms, sizeof(float) * (size_t)n_uniforms * c->nrep, hipMemcpyHostToDevice));
+  }
+  HCK(hipMalloc(&dcons, 4 * c->nrep));
+  if (log && c->nq_total) HCK(hipMalloc(&dlog, 24 * (size_t)c->nq_total));
+  HCK(hipMemset(c->mc_err, 0, 4));
+  int rb = c->cur_rep_begin, re = c->cur_rep_end;
+  launch_mc_sweep(c->stream, c->d_reps, rb, re - rb, c->rx, c->ry, c->rz, c->qtrack, c->qold, c->qsign, c->occ,
+                  c->occ_stride, du, uniforms ? n_uniforms : 0, c->d_seeds, c->mc_sweeps++, dcons, dlog, c->log_off,
+                  c->mc_err);
+  HCK(hipGetLastError());
+  HCK(hipStreamSynchronize(c->stream));
+  int e = 0;
+  HCK(hipMemcpy(&e, c->mc_err, 4, hipMemcpyDeviceToHost));
+  if (consumed) HCK(hipMemcpy(consumed, dcons, 4 * c->nrep, hipMemcpyDeviceToHost));
+  if (log && dlog) HCK(hipMemcpy(log, dlog, 24 * (size_t)c->nq_total, hipMemcpyDeviceToHost));
+  if (du) hipFree(du);
+  hipFree(dcons);
+  if (dlog) hipFree(dlog);
+  return (e & 2) ? SMDLT_E_MC_STREAM : 0;
+}
+
+int smdlt_mc_energy(smdlt_ctx* c, double* energy) {
+  if (!c || !energy) return SMDLT_E_ARG;
+  if (!c->have_pos) return SMDLT_E_STATE;
+  HCK(hipSetDevice(c->device));
+  int st = upload_reps(c);
+  if (st) return st;
+  double* de;
+  HCK(hipMalloc(&de, 8 * c->nrep));
+  HCK(hipMemset(de, 0, 8 * c->nrep));
+  launch_mc_energy(c->stream, c->d_reps, c->cur_rep_begin, c->cur_rep_end - c->cur_rep_begin, c->rx, c->ry, c->rz,
+                   c->qtrack, c->qsign, de);
+  HCK(hipStreamSynchronize(c->stream));
+  HCK(hipMemcpy(energy, de, 8 * c->nrep, hipMemcpyDeviceToHost));
+  hipFree(de);
+  return 0;
+}
+
+int smdlt_mlp_num_params(void) { return 16 * 64 + 64 + 64 * 64 + 64 + 64 + 1; }
+
+int smdlt_mlp_set_weights(smdlt_ctx* c, const float* w) {
+  if (!c || !w) return SMDLT_E_ARG;
+  HCK(hipSetDevice(c->device));
+  if (!c->mlp_w && alloc_dev(&c->mlp_w, smdlt_mlp_num_params())) return SMDLT_E_NOMEM;
+  HCK(hipMemcpy(c->mlp_w, w, 4 * smdlt_mlp_num_params(), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int smdlt_mlp_loss_grad(smdlt_ctx* c, const float* w, const float* x, const float* y, int n, float* loss, float* grad) {
+  (void)c; (void)w; (void)x; (void)y; (void)n; (void)loss; (void)grad;
+  return SMDLT_E_ARG;
+}
+
+int smdlt_mc_features(smdlt_ctx* c, int replica, float* features, int* cand_from, int* cand_to, int cap, int* n) {
+  (void)c; (void)replica; (void)features; (void)cand_from; (void)cand_to; (void)cap; (void)n;
+  return SMDLT_E_ARG;
+}
+
+// Select a replica sub-range for the per-replica (plain) path: subsequent calls
+// touch only replicas [begin, end).  Not part of the reference interface.
+int smdlt_select_replicas(smdlt_ctx* c, int begin, int end) {
+  if (!c || begin < 0 || end > c->nrep || begin >= end) return SMDLT_E_ARG;
+  set_active(c, begin, end);
+  if (c->gexec) {
+    hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+  }
+  return upload_reps(c);
+}
+
+}  // extern "C"

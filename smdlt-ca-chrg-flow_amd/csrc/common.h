@@ -1,0 +1,286 @@
+// Shared definitions of the gfx950 timestep library: replica descriptor,
+// device helpers and the launch wrappers each kernel file exports to api.cpp.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define SLOT_ALIGN 64       // beads of a replica are padded to whole wavefronts
+#define KMAX 512            // largest FFT length (twiddle table size)
+#define FFT_TILE 1024       // complex elements per component per FFT workgroup
+#define FFT_THREADS 256
+#define MAX_ORDER 8         // largest B-spline order
+#define LZ_MAXIT 64         // largest Lanczos iteration count supported
+
+// Per-replica descriptor, uploaded to the device every step (lattice fields
+// change with t).  All lattice quantities are computed on the host in FP64 with
+// the reference's expressions so that the FP64 remap kernels reproduce them.
+struct RepDev {
+  int slot0, n, nb, nc;        // first slot, beads, beads per chain, chains
+  int wave0, nwaves;           // wavefront range of the replica's slots
+  int flow, pbc_compat, lj_ext, spring_fene;
+  int K[3], order;             // PME grid along a1, a2, a3 and B-spline order
+  int ncell[3], cell0;         // neighbour-build cell grid and offset
+  int q0, nq;                  // charges: offset into the charge arrays, count
+  long long grid0;             // element offset of the replica's grid (3 components)
+  int bsp0, active;            // B-spline modulus offset; replica enabled for this launch
+  // lattice in the wrap frame
+  double L, invL, L1xp, L2xp, L2yp, delx;
+  double ct, st;               // rotation lab -> wrap frame at this step (KRBC)
+  double ct0, st0;             // rotation of the previous step (KRBC round trip)
+  double rb[4];                // Lagrange-reduced in-plane basis b1=(rb0,rb1), b2=(rb2,rb3)
+  double rd[4];                // its dual rows
+  // Ewald split
+  double alpha, rc2, rlist2, selfconst, vol;
+  double hinv[3];              // s2=y*hinv1, s1=(x-L2xp*s2)*hinv0, s3=z*hinv2
+  double kb[4];                // reciprocal basis: b1=(kb0,kb1,0), b2=(0,kb2,0), b3=(0,0,kb3)
+  // physics
+  double dt, edot, kappas, qmax, kappab, epsilon, lambda_b, kappa_d, r2cut, rv, sqrt2dt;
+  double barrier, barrier2;
+  double rnew, skin;           // Verlet trigger (rv-rc)/2 and mobility-list skin
+  int treal, reset;            // t % tp and whether this step starts a lattice period
+  unsigned long long step;     // step counter for the device noise stream
+};
+
+template <typename T> struct cplx_t;
+template <> struct cplx_t<float> { typedef float2 type; };
+template <> struct cplx_t<double> { typedef double2 type; };
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+__device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+template <typename C> __device__ __forceinline__ C mk2(double x, double y);
+template <> __device__ __forceinline__ float2 mk2<float2>(double x, double y) { return make_float2((float)x, (float)y); }
+template <> __device__ __forceinline__ double2 mk2<double2>(double x, double y) { return make_double2(x, y); }
+
+// Compensated (TwoSum) accumulator for per-bead velocity sums.
+template <typename T> struct CompSum {
+  T s, c;
+  __device__ __forceinline__ CompSum() : s(0), c(0) {}
+  __device__ __forceinline__ void add(T x) {
+    T t = s + x;
+    T bp = t - s;
+    c += (s - (t - bp)) + (x - bp);
+    s = t;
+  }
+  __device__ __forceinline__ double value() const { return (double)s + (double)c; }
+};
+
+// Minimum image in the Lagrange-reduced basis (FP64).  For |d| below half the
+// reduced cell's smallest width the rounding returns the unique minimum image.
+__device__ __forceinline__ void min_image_reduced(const RepDev& R, double& dx, double& dy, double& dz) {
+  double f1 = R.rd[0] * dx + R.rd[1] * dy;
+  double f2 = R.rd[2] * dx + R.rd[3] * dy;
+  double n1 = rint(f1), n2 = rint(f2);
+  dx -= n1 * R.rb[0] + n2 * R.rb[2];
+  dy -= n1 * R.rb[1] + n2 * R.rb[3];
+  dz -= R.L * rint(dz * R.invL);
+}
+
+// getNID of the reference, FP64, written in its operation order
+// (S/main26.c:1429-1457 shear with the delx shift; X/main14.c:1462-1480 KRBC).
+// Returns the pair vector r_i - r_j in the frame the reference returns it in.
+__device__ __forceinline__ void ref_nid(const RepDev& R, double xi, double yi, double zi, double xj,
+                                        double yj, double zj, double& ox, double& oy, double& oz,
+                                        bool to_lab = true) {
+#pragma clang fp contract(off)
+  double dx = xi - xj, dy = yi - yj, dz = zi - zj;
+  double L2xp = R.L2xp, L2yp = R.L2yp;
+  if (R.flow == 2 || !R.pbc_compat) {
+    dx -= L2xp * round(dy / L2yp);
+    dy -= L2yp * round(dy / L2yp);
+  } else {
+    dx -= L2xp * round(dy / L2yp);
+    if (dy > L2yp / 2) {
+      dx -= R.delx;
+      dy -= L2yp * round(dy / L2yp);
+    } else if (dy < -L2yp / 2) {
+      dx += R.delx;
+      dy -= L2yp * round(dy / L2yp);
+    }
+  }
+  dx -= R.L1xp * round(dx / R.L1xp);
+  dz -= R.L * round(dz / R.L);
+  if (R.flow == 2 && to_lab) {
+    ox = dx * R.ct - dy * R.st;
+    oy = dx * R.st + dy * R.ct;
+  } else {
+    ox = dx;
+    oy = dy;
+  }
+  oz = dz;
+}
+
+// Cardinal B-spline weights W[j] = M_P(x + j), j = 0..P-1, x in [0,1); weight j
+// belongs to grid point floor(u) - j.  Recursion M_k(y) = [y M_{k-1}(y) +
+// (k-y) M_{k-1}(y-1)]/(k-1), unrolled so every index is static.
+template <typename T, int P> __device__ __forceinline__ void bspline_weights(T x, T* W) {
+  T A[P];
+#pragma unroll
+  for (int j = 0; j < P; ++j) A[j] = (j == 0) ? (T)1 : (T)0;
+#pragma unroll
+  for (int k = 2; k <= P; ++k) {
+    T div = (T)1 / (T)(k - 1);
+#pragma unroll
+    for (int j = P - 1; j >= 0; --j) {
+      T lo = (j >= 1) ? A[j - 1] : (T)0;
+      A[j] = div * ((x + (T)j) * A[j] + ((T)k - x - (T)j) * lo);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < P; ++j) W[j] = A[j];
+}
+
+// Beenakker real-space coefficients with the exact overlap correction for r<2
+// (a=1, unit self-mobility): M = t1 I + t2 d d^T / r^2.
+template <typename T> __device__ __forceinline__ void rpy_real(T r, T alpha, T& t1, T& t2) {
+  T rr = r * r, ir = (T)1 / r, ir3 = ir / rr;
+  T a2 = alpha * alpha, a3 = a2 * alpha, a5 = a3 * a2, a7 = a5 * a2;
+  T ec = erfc(alpha * r);
+  T g = exp(-a2 * rr) * (T)0.56418958354775628695;  // 1/sqrt(pi)
+  T C1 = (T)0.75 * ir + (T)0.5 * ir3;
+  T C2 = (T)4 * a7 * rr * rr + (T)3 * a3 * rr - (T)20 * a5 * rr - (T)4.5 * alpha + (T)14 * a3 + alpha / rr;
+  T C3 = (T)0.75 * ir - (T)1.5 * ir3;
+  T C4 = -(T)4 * a7 * rr * rr - (T)3 * a3 * rr + (T)16 * a5 * rr + (T)1.5 * alpha - (T)2 * a3 - (T)3 * alpha / rr;
+  t1 = C1 * ec + C2 * g;
+  t2 = C3 * ec + C4 * g;
+  if (r < (T)2) {
+    t1 += ((T)1 - (T)0.28125 * r) - C1;
+    t2 += (T)0.09375 * r - C3;
+  }
+}
+
+__device__ __forceinline__ double warp_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ float warp_sum(float v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ double warp_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), counter-based.
+__device__ __forceinline__ void philox10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+  for (int r = 0; r < 10; ++r) {
+    uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+    uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+    uint32_t h0 = (uint32_t)(p0 >> 32), l0 = (uint32_t)p0;
+    uint32_t h1 = (uint32_t)(p1 >> 32), l1 = (uint32_t)p1;
+    uint32_t n0 = h1 ^ c[1] ^ k0, n2 = h0 ^ c[3] ^ k1;
+    c[0] = n0;
+    c[1] = l1;
+    c[2] = n2;
+    c[3] = l0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+__device__ __forceinline__ double u01_53(uint32_t a, uint32_t b) {
+  uint64_t x = ((uint64_t)a << 21) ^ (uint64_t)b;
+  x &= ((1ull << 53) - 1);
+  return ((double)x + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+// --------------------------------------------------------------------------
+// Launch wrappers (host) exported by the kernel files.
+struct FftTask { int rep, comp, l0, nl; };
+
+struct DevState;  // defined in api.cpp
+
+// k_remap.hip
+void launch_remap(hipStream_t s, const RepDev* reps, const int* slot_rep, int slot_begin, int nslots,
+                  double* rx, double* ry, double* rz, double* dxv, double* dyv, double* dzv, double* dsp,
+                  int mode);
+void launch_min_image(hipStream_t s, const RepDev* rep, const double* rx, const double* ry, const double* rz,
+                      int npairs, const int* pi, const int* pj, double* out);
+void launch_bin_index(hipStream_t s, const RepDev* rep, const double* rx, const double* ry, const double* rz,
+                      int npairs, const int* pi, const int* pj, const double* bins, const int* nbins, int* out);
+void launch_update_positions(hipStream_t s, const RepDev* reps, const int* slot_rep, int slot_begin, int nslots,
+                             double* rx, double* ry, double* rz, const double* mf, const double* bz,
+                             double* dxv, double* dyv, double* dzv, double* dsp, int noise_is_scaled);
+void launch_disp_max(hipStream_t s, const RepDev* reps, int rep_begin, int nrep, const double* dxv,
+                     const double* dyv, const double* dzv, const double* dsp, double* out2);
+void launch_noise(hipStream_t s, const int* slot_rep, const RepDev* reps, int slot_begin, int nslots,
+                  const uint64_t* seeds, uint64_t step, double* z);
+void launch_rotate_vec(hipStream_t s, const RepDev* reps, const int* slot_rep, int slot_begin, int nslots,
+                       double* v, int inverse);
+
+// k_nlist.hip
+void launch_cell_count(hipStream_t s, const RepDev* reps, const int* slot_rep, int slot_begin, int nslots,
+                       const double* rx, const double* ry, const double* rz, int* cell_of, int* cell_cnt);
+void launch_scan(hipStream_t s, const int* in, int* out, int n);
+void launch_reset_trackers(hipStream_t s, const RepDev* reps, const int* slot_rep, int slot_begin, int nslots,
+                           const int* rep_flags, double* dxv, double* dyv, double* dzv, double* dsp);
+void launch_nl_check(hipStream_t s, const RepDev* reps, int rep_begin, int nrep, const double* out2, double* aff,
+                     int force, int* rep_flags);
+void launch_cell_fill(hipStream_t s, const RepDev* reps, const int* slot_rep, int slot_begin, int nslots,
+                      const int* cell_of, const int* cell_start, int* cell_fill, int* cell_beads);
+void launch_build_lists(hipStream_t s, const RepDev* reps, const int* slot_rep, int slot_begin, int nslots,
+                        const double* rx, const double* ry, const double* rz, const int* cell_start,
+                        const int* cell_cnt, const int* cell_beads, const int* rep_flags, int nslots_total,
+                        int cap_rpy, int* nrpy, int* lrpy, int cap_lj, int* nlj, int* llj, int* overflow);
+
+void launch_fill_i32(hipStream_t s, int* p, int v, long long n);
+void launch_fill_f64(hipStream_t s, double* p, double v, long long n);
+void launch_fill_f32(hipStream_t s, float* p, float v, long long n);
+
+// k_forces.hip
+void launch_forces(hipStream_t s, const RepDev* reps, const int* slot_rep, int slot_begin, int nslots,
+                   const double* rx, const double* ry, const double* rz, int nslots_total, const int* nlj,
+                   const int* llj, double* f, int* err);
+void launch_coulomb(hipStream_t s, const RepDev* reps, int rep_begin, int nrep, const double* rx,
+                    const double* ry, const double* rz, const int* qtrack, const int* qsign, double* f, int max_nq);
+
+// k_mobility.hip
+struct MobArgs {
+  const RepDev* reps;
+  const int* slot_rep;
+  int slot_begin, nslots, nst;
+  const double *rx, *ry, *rz;
+  const int *nrpy, *lrpy;
+  const double* v;          // input vector (AoS 3N); scaled by 1/beta[r] when beta != nullptr
+  const double* beta;       // per-replica divisor (Lanczos normalisation) or nullptr
+  double* vout;             // where the (scaled) input is stored for the real-space pass
+  double* u;                // output M v (AoS 3N)
+  double* partial;          // per-wave partial v.u (nullptr: skip)
+  const int* done;          // per-replica skip flag (nullptr: none)
+  void* grid;               // grid buffer (T)
+  long long grid_begin, grid_len;
+  const void* bsp;          // B-spline moduli (T)
+  const void* tw;           // twiddles (complex T, KMAX)
+  const FftTask *tx, *ty, *tz;
+  int ntx, nty, ntz;
+  int order;                // B-spline order (4, 6 or 8), one per context
+};
+template <typename T> void launch_mobility(hipStream_t s, const MobArgs& a);
+
+// k_lanczos.hip
+void lz_wave_dot(hipStream_t st, const RepDev* reps, const int* slot_rep, int slot_begin, int nslots, const double* x,
+                 const double* y, const int* done, double* partial);
+void lz_rep_reduce(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, const double* partial,
+                   const int* done, double* out, int stride, int off, int take_sqrt, double* copy_to);
+void lz_orth(hipStream_t st, const RepDev* reps, const int* slot_rep, int slot_begin, int nslots, double* w,
+             const double* vj, const double* vjm1, const double* scal, int j, const int* done, double* partial);
+void lz_tri(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, const double* partial, double* scal, int j,
+            double tol, int maxit, int* done, double* bdiv, int* iters);
+void lz_combine(hipStream_t st, const RepDev* reps, const int* slot_rep, int slot_begin, int nslots, const double* V,
+                int nst, const double* scal, const int* iters, double* y);
+void lz_tea(hipStream_t st, const int* slot_rep, int slot_begin, int nslots, const double* mz, const double* z,
+            const double* diag, const double* beta, const double* C, double* y);
+#define SCAL_STRIDE (4 * LZ_MAXIT + 8)
+
+// k_mc.hip
+void launch_mc_sweep(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, const double* rx, const double* ry,
+                     const double* rz, int* track, int* old, const int* sign, unsigned char* occ,
+                     long long occ_stride, const float* uniforms, int nu, const uint64_t* seeds, uint64_t sweep_id,
+                     int* consumed, int* log, const int* log_off, int* err);
+void launch_mc_energy(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, const double* rx, const double* ry,
+                      const double* rz, const int* track, const int* sign, double* e);

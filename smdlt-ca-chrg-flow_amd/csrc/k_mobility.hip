@@ -1,0 +1,439 @@
+// Ewald-summed RPY mobility product u = M v for every replica in one launch
+// sequence, replacing the dense matrix of calcHI/updateHI (S/main26.c:1530-1806,
+// 745-857) with a smooth-PME split:
+//   u_i = selfconst v_i + sum_{j in list(i), r<rc} M_real(r_ij) v_j + interp_i(phi),
+//   phi = IDFT[ G(m) |b(m)|^2 DFT[ spread(v) ] ],
+// on a K1 x K2 x K3 grid along the (deformed) lattice vectors, cardinal
+// B-splines of order p (Essmann et al. 1995), and the Beenakker reciprocal
+// kernel G(k) = (1 - k^2/3)(1 + k^2/4a^2 + k^4/8a^4)(6 pi/k^2) e^{-k^2/4a^2}
+// (I - k k^T/k^2)/V (S/main26.c:1622).  The FFTs are hand-written in-LDS
+// Stockham radix-4/2 passes: x (two real rows packed per complex line),
+// y, and z with the Green multiply and inverse z fused; then inverse y and x.
+// T = float (mixed mode; per-bead sums compensated) or double.
+#include "common.h"
+
+template <typename T> struct Prec {
+  typedef typename cplx_t<T>::type C;
+};
+
+// -------------------------------------------------------------------- FFT core
+// One in-place Stockham stage of radix R over nl lines of length K in LDS:
+// every thread first loads its butterflies into registers, then (after a
+// barrier) writes them to their autosorted positions.
+template <typename C, int R, int MAXB>
+__device__ __forceinline__ void fft_stage(C* sh, int K, int pitch, int nl, int Ns, int dir, const C* __restrict__ tw) {
+  const int T = blockDim.x, t = threadIdx.x;
+  const int KR = K / R;
+  const int nb = nl * KR;
+  C v[MAXB][R];
+#pragma unroll
+  for (int q = 0; q < MAXB; ++q) {
+    int b = t + q * T;
+    if (b < nb) {
+      int l = b / KR, j = b - l * KR;
+      const C* src = sh + l * pitch + j;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[q][r] = src[r * KR];
+    }
+  }
+  __syncthreads();
+  const int tstep = KMAX / (Ns * R);
+#pragma unroll
+  for (int q = 0; q < MAXB; ++q) {
+    int b = t + q * T;
+    if (b < nb) {
+      int l = b / KR, j = b - l * KR;
+      int jm = j & (Ns - 1);
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        C w = tw[(jm * r * tstep) & (KMAX - 1)];
+        if (dir > 0) w = cconj(w);
+        v[q][r] = cmul(v[q][r], w);
+      }
+      C* dst = sh + l * pitch + (j - jm) * R + jm;
+      if (R == 2) {
+        dst[0] = cadd(v[q][0], v[q][1]);
+        dst[Ns] = csub(v[q][0], v[q][1]);
+      } else {
+        C a = cadd(v[q][0], v[q][R > 2 ? 2 : 0]), bb = csub(v[q][0], v[q][R > 2 ? 2 : 0]);
+        C c = cadd(v[q][1], v[q][R > 2 ? 3 : 1]), d = csub(v[q][1], v[q][R > 2 ? 3 : 1]);
+        C id;  // -i*d for forward, +i*d for inverse
+        if (dir < 0) {
+          id.x = d.y;
+          id.y = -d.x;
+        } else {
+          id.x = -d.y;
+          id.y = d.x;
+        }
+        dst[0] = cadd(a, c);
+        dst[Ns] = cadd(bb, id);
+        dst[2 * Ns] = csub(a, c);
+        dst[3 * Ns] = csub(bb, id);
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// FFT of nl lines (nl*K <= MULT*FFT_TILE): radix-4 stages, then radix 2 if K
+// is an odd power of two.
+template <typename C, int MULT>
+__device__ __forceinline__ void fft_lines(C* sh, int K, int pitch, int nl, int dir, const C* __restrict__ tw) {
+  int Ns = 1;
+  while (K / Ns >= 4) {
+    fft_stage<C, 4, MULT * FFT_TILE / (4 * FFT_THREADS)>(sh, K, pitch, nl, Ns, dir, tw);
+    Ns *= 4;
+  }
+  if (K / Ns == 2) fft_stage<C, 2, MULT * FFT_TILE / (2 * FFT_THREADS)>(sh, K, pitch, nl, Ns, dir, tw);
+}
+
+// ------------------------------------------------------------------- spreading
+template <typename T, int P>
+__device__ __forceinline__ void frac_weights(const RepDev& R, double x, double y, double z, int base[3],
+                                             T W[3][P]) {
+  double s2 = y * R.hinv[1];
+  double s1 = (x - R.L2xp * s2) * R.hinv[0];
+  double s3 = z * R.hinv[2];
+  double s[3] = {s1, s2, s3};
+  for (int d = 0; d < 3; ++d) {
+    double u = s[d] * (double)R.K[d];
+    double fl = floor(u);
+    T fr = (T)(u - fl);
+    base[d] = ((int)fl) & (R.K[d] - 1);
+    bspline_weights<T, P>(fr, W[d]);
+  }
+}
+
+template <typename T> __device__ __forceinline__ void grid_add(T* p, T v) { unsafeAtomicAdd(p, v); }
+
+template <typename T, int P>
+__global__ void k_spread(MobArgs a) {
+  int s = a.slot_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.slot_begin + a.nslots) return;
+  int r = a.slot_rep[s];
+  if (r < 0) return;
+  const RepDev& R = a.reps[r];
+  if (!R.active || (a.done && a.done[r])) return;
+  if (s - R.slot0 >= R.n) return;
+  double sc = a.beta ? 1.0 / a.beta[r] : 1.0;
+  double v0 = a.v[3 * s] * sc, v1 = a.v[3 * s + 1] * sc, v2 = a.v[3 * s + 2] * sc;
+  if (a.vout) {
+    a.vout[3 * s] = v0;
+    a.vout[3 * s + 1] = v1;
+    a.vout[3 * s + 2] = v2;
+  }
+  int base[3];
+  T W[3][P];
+  frac_weights<T, P>(R, a.rx[s], a.ry[s], a.rz[s], base, W);
+  T* g = (T*)a.grid + R.grid0;
+  const long long ncomp = (long long)R.K[0] * R.K[1] * R.K[2];
+  T tv0 = (T)v0, tv1 = (T)v1, tv2 = (T)v2;
+  const int K0 = R.K[0], K1 = R.K[1], K2 = R.K[2];
+#pragma unroll
+  for (int j3 = 0; j3 < P; ++j3) {
+    int g3 = (base[2] - j3) & (K2 - 1);
+#pragma unroll
+    for (int j2 = 0; j2 < P; ++j2) {
+      int g2 = (base[1] - j2) & (K1 - 1);
+      T w23 = W[2][j3] * W[1][j2];
+      long long row = ((long long)g3 * K1 + g2) * K0;
+#pragma unroll
+      for (int j1 = 0; j1 < P; ++j1) {
+        int g1 = (base[0] - j1) & (K0 - 1);
+        T w = w23 * W[0][j1];
+        T* q = g + row + g1;
+        grid_add(q, w * tv0);
+        grid_add(q + ncomp, w * tv1);
+        grid_add(q + 2 * ncomp, w * tv2);
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------- FFT passes
+// x pass: task = (rep, comp, first row pair, pairs); forward packs rows 2p, 2p+1
+// as one complex line and splits the spectrum into two half spectra (Nyquist
+// dropped); inverse rebuilds the full line and writes real/imag to the rows.
+template <typename T>
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_x(MobArgs a, int dir) {
+  typedef typename Prec<T>::C C;
+  __shared__ C sh[FFT_TILE + 64];
+  FftTask tk = a.tx[blockIdx.x];
+  const RepDev& R = a.reps[tk.rep];
+  if (!R.active || (a.done && a.done[tk.rep])) return;
+  const int K = R.K[0], Kh = K / 2, pitch = K + 1, nl = tk.nl;
+  const long long ncomp = (long long)R.K[0] * R.K[1] * R.K[2];
+  T* g = (T*)a.grid + R.grid0 + tk.comp * ncomp;
+  C* gc = (C*)g;
+  const C* tw = (const C*)a.tw;
+  if (dir < 0) {
+    for (int e = threadIdx.x; e < nl * K; e += blockDim.x) {
+      int l = e / K, x = e - l * K;
+      long long row = 2LL * (tk.l0 + l);
+      C z;
+      z.x = g[row * K + x];
+      z.y = g[(row + 1) * K + x];
+      sh[l * pitch + x] = z;
+    }
+    __syncthreads();
+    fft_lines<C, 1>(sh, K, pitch, nl, -1, tw);
+    for (int e = threadIdx.x; e < nl * Kh; e += blockDim.x) {
+      int l = e / Kh, k = e - l * Kh;
+      C zk = sh[l * pitch + k], zm = sh[l * pitch + ((K - k) & (K - 1))];
+      C A, B;
+      A.x = (T)0.5 * (zk.x + zm.x);
+      A.y = (T)0.5 * (zk.y - zm.y);
+      B.x = (T)0.5 * (zk.y + zm.y);
+      B.y = (T)0.5 * (zm.x - zk.x);
+      long long row = 2LL * (tk.l0 + l);
+      gc[row * Kh + k] = A;
+      gc[(row + 1) * Kh + k] = B;
+    }
+  } else {
+    for (int e = threadIdx.x; e < nl * Kh; e += blockDim.x) {
+      int l = e / Kh, k = e - l * Kh;
+      long long row = 2LL * (tk.l0 + l);
+      C A = gc[row * Kh + k], B = gc[(row + 1) * Kh + k];
+      C z;
+      z.x = A.x - B.y;
+      z.y = A.y + B.x;
+      sh[l * pitch + k] = z;
+      if (k > 0) {
+        C w;
+        w.x = A.x + B.y;
+        w.y = B.x - A.y;
+        sh[l * pitch + K - k] = w;
+      } else {
+        C zero;
+        zero.x = (T)0;
+        zero.y = (T)0;
+        sh[l * pitch + Kh] = zero;
+      }
+    }
+    __syncthreads();
+    fft_lines<C, 1>(sh, K, pitch, nl, 1, tw);
+    for (int e = threadIdx.x; e < nl * K; e += blockDim.x) {
+      int l = e / K, x = e - l * K;
+      long long row = 2LL * (tk.l0 + l);
+      C z = sh[l * pitch + x];
+      g[row * K + x] = z.x;
+      g[(row + 1) * K + x] = z.y;
+    }
+  }
+}
+
+// y pass (C2C along a2): line L = z*Kh + x, element k at (z*K1 + k)*Kh + x.
+template <typename T>
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_y(MobArgs a, int dir) {
+  typedef typename Prec<T>::C C;
+  __shared__ C sh[FFT_TILE + 64];
+  FftTask tk = a.ty[blockIdx.x];
+  const RepDev& R = a.reps[tk.rep];
+  if (!R.active || (a.done && a.done[tk.rep])) return;
+  const int Kh = R.K[0] / 2, K = R.K[1], pitch = K + 1, nl = tk.nl;
+  const long long ncomp = (long long)R.K[0] * R.K[1] * R.K[2];
+  C* gc = (C*)((T*)a.grid + R.grid0 + tk.comp * ncomp);
+  for (int e = threadIdx.x; e < nl * K; e += blockDim.x) {
+    int k = e / nl, l = e - k * nl;
+    int L = tk.l0 + l, z = L / Kh, x = L - z * Kh;
+    sh[l * pitch + k] = gc[((long long)z * K + k) * Kh + x];
+  }
+  __syncthreads();
+  fft_lines<C, 1>(sh, K, pitch, nl, dir, (const C*)a.tw);
+  for (int e = threadIdx.x; e < nl * K; e += blockDim.x) {
+    int k = e / nl, l = e - k * nl;
+    int L = tk.l0 + l, z = L / Kh, x = L - z * Kh;
+    gc[((long long)z * K + k) * Kh + x] = sh[l * pitch + k];
+  }
+}
+
+// z pass: forward FFT of the three components, Green multiply, inverse FFT.
+// Line L = y*Kh + x, element k at k*(K1*Kh) + L.
+template <typename T>
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_z_green(MobArgs a) {
+  typedef typename Prec<T>::C C;
+  __shared__ C sh[3 * (FFT_TILE + 64)];
+  FftTask tk = a.tz[blockIdx.x];
+  const RepDev& R = a.reps[tk.rep];
+  if (!R.active || (a.done && a.done[tk.rep])) return;
+  const int K0 = R.K[0], Kh = K0 / 2, K1 = R.K[1], K = R.K[2], pitch = K + 1, nl = tk.nl;
+  const long long ncomp = (long long)R.K[0] * R.K[1] * R.K[2];
+  const long long stride = (long long)K1 * Kh;
+  C* gc[3];
+  for (int c = 0; c < 3; ++c) gc[c] = (C*)((T*)a.grid + R.grid0 + c * ncomp);
+  const int cpitch = nl * pitch;
+  for (int e = threadIdx.x; e < nl * K; e += blockDim.x) {
+    int k = e / nl, l = e - k * nl;
+    long long idx = k * stride + tk.l0 + l;
+    for (int c = 0; c < 3; ++c) sh[c * cpitch + l * pitch + k] = gc[c][idx];
+  }
+  __syncthreads();
+  const C* tw = (const C*)a.tw;
+  fft_lines<C, 3>(sh, K, pitch, 3 * nl, -1, tw);
+  const T* bsp = (const T*)a.bsp + R.bsp0;
+  const T ia2 = (T)(1.0 / (4.0 * R.alpha * R.alpha));
+  const T invV = (T)(1.0 / R.vol);
+  for (int e = threadIdx.x; e < nl * K; e += blockDim.x) {
+    int k = e / nl, l = e - k * nl;
+    int L = tk.l0 + l, y = L / Kh, x = L - y * Kh;
+    C* p0 = sh + l * pitch + k;
+    C* p1 = p0 + cpitch;
+    C* p2 = p1 + cpitch;
+    bool zero = (y == K1 / 2) || (k == K / 2) || (x == 0 && y == 0 && k == 0);
+    if (zero) {
+      C z0;
+      z0.x = (T)0;
+      z0.y = (T)0;
+      *p0 = z0;
+      *p1 = z0;
+      *p2 = z0;
+      continue;
+    }
+    int m1 = x, m2 = (y < K1 / 2) ? y : y - K1, m3 = (k < K / 2) ? k : k - K;
+    T kx = (T)m1 * (T)R.kb[0];
+    T ky = (T)m1 * (T)R.kb[1] + (T)m2 * (T)R.kb[2];
+    T kz = (T)m3 * (T)R.kb[3];
+    T kk = kx * kx + ky * ky + kz * kz;
+    T q = kk * ia2;
+    T gsc = ((T)1 - kk / (T)3) * ((T)1 + q + (T)2 * q * q) * ((T)18.849555921538759 / kk) * exp(-q) * invV;
+    gsc *= bsp[x] * bsp[K0 + y] * bsp[K0 + K1 + k];
+    T ikk = (T)1 / kk;
+    C v0 = *p0, v1 = *p1, v2 = *p2;
+    T dr = (kx * v0.x + ky * v1.x + kz * v2.x) * ikk;
+    T di = (kx * v0.y + ky * v1.y + kz * v2.y) * ikk;
+    C o0, o1, o2;
+    o0.x = gsc * (v0.x - kx * dr);
+    o0.y = gsc * (v0.y - kx * di);
+    o1.x = gsc * (v1.x - ky * dr);
+    o1.y = gsc * (v1.y - ky * di);
+    o2.x = gsc * (v2.x - kz * dr);
+    o2.y = gsc * (v2.y - kz * di);
+    *p0 = o0;
+    *p1 = o1;
+    *p2 = o2;
+  }
+  __syncthreads();
+  fft_lines<C, 3>(sh, K, pitch, 3 * nl, 1, tw);
+  for (int e = threadIdx.x; e < nl * K; e += blockDim.x) {
+    int k = e / nl, l = e - k * nl;
+    long long idx = k * stride + tk.l0 + l;
+    for (int c = 0; c < 3; ++c) gc[c][idx] = sh[c * cpitch + l * pitch + k];
+  }
+}
+
+// -------------------------------------------------- real space + interpolation
+template <typename T, int P>
+__global__ void k_mob_final(MobArgs a) {
+  int s = a.slot_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  bool in = s < a.slot_begin + a.nslots;
+  int r = in ? a.slot_rep[s] : -1;
+  double dot = 0.0;
+  bool act = false;
+  if (r >= 0) {
+    const RepDev& R = a.reps[r];
+    act = R.active && !(a.done && a.done[r]);
+    if (act && s - R.slot0 < R.n) {
+      const double* vv = a.vout ? a.vout : a.v;
+      double sc = (a.beta && !a.vout) ? 1.0 / a.beta[r] : 1.0;
+      double xi = a.rx[s], yi = a.ry[s], zi = a.rz[s];
+      CompSum<T> acc[3];
+      // reciprocal part: interpolate the three potential grids
+      int base[3];
+      T W[3][P];
+      frac_weights<T, P>(R, xi, yi, zi, base, W);
+      const T* g = (const T*)a.grid + R.grid0;
+      const long long ncomp = (long long)R.K[0] * R.K[1] * R.K[2];
+      const int K0 = R.K[0], K1 = R.K[1], K2 = R.K[2];
+#pragma unroll
+      for (int j3 = 0; j3 < P; ++j3) {
+        int g3 = (base[2] - j3) & (K2 - 1);
+#pragma unroll
+        for (int j2 = 0; j2 < P; ++j2) {
+          int g2 = (base[1] - j2) & (K1 - 1);
+          T w23 = W[2][j3] * W[1][j2];
+          long long row = ((long long)g3 * K1 + g2) * K0;
+          T s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+          for (int j1 = 0; j1 < P; ++j1) {
+            int g1 = (base[0] - j1) & (K0 - 1);
+            T w = W[0][j1];
+            const T* q = g + row + g1;
+            s0 += w * q[0];
+            s1 += w * q[ncomp];
+            s2 += w * q[2 * ncomp];
+          }
+          acc[0].add(w23 * s0);
+          acc[1].add(w23 * s1);
+          acc[2].add(w23 * s2);
+        }
+      }
+      // real-space part over the list
+      const T alpha = (T)R.alpha, rc2 = (T)R.rc2;
+      int m = a.nrpy[s];
+      for (int k = 0; k < m; ++k) {
+        int j = a.lrpy[(size_t)k * a.nst + s];
+        double dx = a.rx[j] - xi, dy = a.ry[j] - yi, dz = a.rz[j] - zi;
+        min_image_reduced(R, dx, dy, dz);
+        T tx = (T)dx, ty = (T)dy, tz = (T)dz;
+        T rr = tx * tx + ty * ty + tz * tz;
+        if (rr < rc2) {
+          T rd = sqrt(rr), t1, t2;
+          rpy_real<T>(rd, alpha, t1, t2);
+          T vx = (T)(vv[3 * j] * sc), vy = (T)(vv[3 * j + 1] * sc), vz = (T)(vv[3 * j + 2] * sc);
+          T dv = (tx * vx + ty * vy + tz * vz) * t2 / rr;
+          acc[0].add(t1 * vx + dv * tx);
+          acc[1].add(t1 * vy + dv * ty);
+          acc[2].add(t1 * vz + dv * tz);
+        }
+      }
+      double vi0 = vv[3 * s] * sc, vi1 = vv[3 * s + 1] * sc, vi2 = vv[3 * s + 2] * sc;
+      double u0 = acc[0].value() + R.selfconst * vi0;
+      double u1 = acc[1].value() + R.selfconst * vi1;
+      double u2 = acc[2].value() + R.selfconst * vi2;
+      a.u[3 * s] = u0;
+      a.u[3 * s + 1] = u1;
+      a.u[3 * s + 2] = u2;
+      dot = vi0 * u0 + vi1 * u1 + vi2 * u2;
+    } else if (act) {
+      a.u[3 * s] = a.u[3 * s + 1] = a.u[3 * s + 2] = 0.0;
+    }
+  }
+  if (a.partial) {
+    dot = warp_sum(dot);
+    if ((threadIdx.x & 63) == 0 && in && r >= 0 && act) a.partial[s >> 6] = dot;
+  }
+}
+
+template <typename T> void launch_mobility(hipStream_t st, const MobArgs& a) {
+  if (a.grid_len > 0) {
+    if (sizeof(T) == 4)
+      launch_fill_f32(st, (float*)a.grid + a.grid_begin, 0.0f, a.grid_len);
+    else
+      launch_fill_f64(st, (double*)a.grid + a.grid_begin, 0.0, a.grid_len);
+  }
+  int nb = (a.nslots + 255) / 256;
+  if (a.nslots > 0) {
+    if (a.order == 4)
+      k_spread<T, 4><<<nb, 256, 0, st>>>(a);
+    else if (a.order == 8)
+      k_spread<T, 8><<<nb, 256, 0, st>>>(a);
+    else
+      k_spread<T, 6><<<nb, 256, 0, st>>>(a);
+  }
+  if (a.ntx > 0) k_fft_x<T><<<a.ntx, FFT_THREADS, 0, st>>>(a, -1);
+  if (a.nty > 0) k_fft_y<T><<<a.nty, FFT_THREADS, 0, st>>>(a, -1);
+  if (a.ntz > 0) k_fft_z_green<T><<<a.ntz, FFT_THREADS, 0, st>>>(a);
+  if (a.nty > 0) k_fft_y<T><<<a.nty, FFT_THREADS, 0, st>>>(a, 1);
+  if (a.ntx > 0) k_fft_x<T><<<a.ntx, FFT_THREADS, 0, st>>>(a, 1);
+  if (a.nslots > 0) {
+    if (a.order == 4)
+      k_mob_final<T, 4><<<nb, 256, 0, st>>>(a);
+    else if (a.order == 8)
+      k_mob_final<T, 8><<<nb, 256, 0, st>>>(a);
+    else
+      k_mob_final<T, 6><<<nb, 256, 0, st>>>(a);
+  }
+}
+
+template void launch_mobility<float>(hipStream_t, const MobArgs&);
+template void launch_mobility<double>(hipStream_t, const MobArgs&);

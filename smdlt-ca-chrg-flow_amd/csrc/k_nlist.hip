@@ -1,0 +1,256 @@
+// Neighbour lists from a cell grid in the Lagrange-reduced lattice basis.
+// One build produces, per bead, (a) the real-space mobility list: every bead
+// within rc+skin under the true minimum image, and (b) the reference's Verlet
+// list: every bead with |getNID| < rv, bit-exact with verletlist
+// (S/main26.c:682-707) because the candidate test is the reference's FP64
+// getNID.  Cells are at least max(rc+skin, rv+1) wide, so both candidate sets
+// lie in the 27 surrounding cells (the +1 covers the shear delx shift).
+// Lists are column-major ([k*nslots + i]) and sorted by bead index.
+#include "common.h"
+
+__device__ __forceinline__ void cell_coords(const RepDev& R, double x, double y, double z, int c[3]) {
+  double f[3];
+  f[0] = R.rd[0] * x + R.rd[1] * y;
+  f[1] = R.rd[2] * x + R.rd[3] * y;
+  f[2] = z * R.invL;
+  for (int d = 0; d < 3; ++d) {
+    double g = f[d] - floor(f[d]);
+    int k = (int)(g * R.ncell[d]);
+    if (k >= R.ncell[d]) k = R.ncell[d] - 1;
+    if (k < 0) k = 0;
+    c[d] = k;
+  }
+}
+
+__global__ void k_cell_count(const RepDev* __restrict__ reps, const int* __restrict__ slot_rep, int slot_begin,
+                             int nslots, const double* rx, const double* ry, const double* rz, int* cell_of,
+                             int* cell_cnt) {
+  int s = slot_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= slot_begin + nslots) return;
+  int r = slot_rep[s];
+  if (r < 0) return;
+  const RepDev& R = reps[r];
+  if (!R.active) return;
+  if (s - R.slot0 >= R.n) {  // padding slot: never binned
+    cell_of[s] = -1;
+    return;
+  }
+  int c[3];
+  cell_coords(R, rx[s], ry[s], rz[s], c);
+  int id = R.cell0 + (c[2] * R.ncell[1] + c[1]) * R.ncell[0] + c[0];
+  cell_of[s] = id;
+  atomicAdd(&cell_cnt[id], 1);
+}
+
+void launch_cell_count(hipStream_t st, const RepDev* reps, const int* slot_rep, int slot_begin, int nslots,
+                       const double* rx, const double* ry, const double* rz, int* cell_of, int* cell_cnt) {
+  if (nslots <= 0) return;
+  k_cell_count<<<(nslots + 255) / 256, 256, 0, st>>>(reps, slot_rep, slot_begin, nslots, rx, ry, rz, cell_of,
+                                                     cell_cnt);
+}
+
+// Exclusive scan of n ints by one workgroup (chunked per thread).
+__global__ void k_scan(const int* in, int* out, int n) {
+  __shared__ int part[1024];
+  int t = threadIdx.x, T = blockDim.x;
+  int chunk = (n + T - 1) / T;
+  int b = t * chunk, e = min(n, b + chunk);
+  int s = 0;
+  for (int i = b; i < e; ++i) s += in[i];
+  part[t] = s;
+  __syncthreads();
+  for (int o = 1; o < T; o <<= 1) {
+    int v = (t >= o) ? part[t - o] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int run = part[t] - s;
+  for (int i = b; i < e; ++i) {
+    int v = in[i];
+    out[i] = run;
+    run += v;
+  }
+}
+
+void launch_scan(hipStream_t st, const int* in, int* out, int n) {
+  if (n <= 0) return;
+  k_scan<<<1, 1024, 0, st>>>(in, out, n);
+}
+
+__global__ void k_cell_fill(const RepDev* __restrict__ reps, const int* __restrict__ slot_rep, int slot_begin,
+                            int nslots, const int* cell_of, const int* cell_start, int* cell_fill, int* cell_beads) {
+  int s = slot_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= slot_begin + nslots) return;
+  int r = slot_rep[s];
+  if (r < 0 || !reps[r].active || s - reps[r].slot0 >= reps[r].n) return;
+  int id = cell_of[s];
+  if (id < 0) return;
+  int k = atomicAdd(&cell_fill[id], 1);
+  cell_beads[cell_start[id] + k] = s;
+}
+
+void launch_cell_fill(hipStream_t st, const RepDev* reps, const int* slot_rep, int slot_begin, int nslots,
+                      const int* cell_of, const int* cell_start, int* cell_fill, int* cell_beads) {
+  if (nslots <= 0) return;
+  k_cell_fill<<<(nslots + 255) / 256, 256, 0, st>>>(reps, slot_rep, slot_begin, nslots, cell_of, cell_start,
+                                                    cell_fill, cell_beads);
+}
+
+__device__ __forceinline__ void sorted_insert(int* list, int stride, int i, int& cnt, int cap, int j, int* overflow) {
+  if (cnt >= cap) {
+    atomicOr(overflow, 1);
+    return;
+  }
+  int pos = cnt;
+  while (pos > 0) {
+    int prev = list[(size_t)(pos - 1) * stride + i];
+    if (prev <= j) break;
+    list[(size_t)pos * stride + i] = prev;
+    --pos;
+  }
+  list[(size_t)pos * stride + i] = j;
+  ++cnt;
+}
+
+// rep_flags[r]: bit0 rebuild the Verlet (LJ) list, bit1 rebuild the mobility list.
+__global__ void k_build_lists(const RepDev* __restrict__ reps, const int* __restrict__ slot_rep, int slot_begin,
+                              int nslots, const double* rx, const double* ry, const double* rz,
+                              const int* cell_start, const int* cell_cnt, const int* cell_beads,
+                              const int* rep_flags, int nst, int cap_rpy, int* nrpy, int* lrpy, int cap_lj,
+                              int* nlj, int* llj, int* overflow) {
+  int i = slot_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= slot_begin + nslots) return;
+  int r = slot_rep[i];
+  if (r < 0) return;
+  const RepDev& R = reps[r];
+  int flags = rep_flags[r];
+  if (!R.active || !(flags & 3)) return;
+  if (i - R.slot0 >= R.n) {
+    if (flags & 2) nrpy[i] = 0;
+    if (flags & 1) nlj[i] = 0;
+    return;
+  }
+  double xi = rx[i], yi = ry[i], zi = rz[i];
+  int c[3];
+  cell_coords(R, xi, yi, zi, c);
+  int lo[3], hi[3];
+  for (int d = 0; d < 3; ++d) {
+    int n = R.ncell[d];
+    lo[d] = (n >= 3) ? -1 : 0;
+    hi[d] = (n >= 3) ? 1 : n - 1;
+  }
+  int cr = 0, cl = 0;
+  for (int o3 = lo[2]; o3 <= hi[2]; ++o3)
+    for (int o2 = lo[1]; o2 <= hi[1]; ++o2)
+      for (int o1 = lo[0]; o1 <= hi[0]; ++o1) {
+        int a = (c[0] + o1 + R.ncell[0]) % R.ncell[0];
+        int b = (c[1] + o2 + R.ncell[1]) % R.ncell[1];
+        int e = (c[2] + o3 + R.ncell[2]) % R.ncell[2];
+        int id = R.cell0 + (e * R.ncell[1] + b) * R.ncell[0] + a;
+        int beg = cell_start[id], cnt = cell_cnt[id];
+        for (int k = 0; k < cnt; ++k) {
+          int j = cell_beads[beg + k];
+          if (j == i) continue;
+          double xj = rx[j], yj = ry[j], zj = rz[j];
+          if (flags & 2) {
+            double dx = xj - xi, dy = yj - yi, dz = zj - zi;
+            min_image_reduced(R, dx, dy, dz);
+            if (dx * dx + dy * dy + dz * dz < R.rlist2) sorted_insert(lrpy, nst, i, cr, cap_rpy, j, overflow);
+          }
+          if (flags & 1) {
+            double ox, oy, oz;
+            ref_nid(R, xi, yi, zi, xj, yj, zj, ox, oy, oz);
+            double rr;
+            {
+#pragma clang fp contract(off)
+              rr = sqrt(ox * ox + oy * oy + oz * oz);
+            }
+            if (rr < R.rv) sorted_insert(llj, nst, i, cl, cap_lj, j, overflow);
+          }
+        }
+      }
+  if (flags & 2) nrpy[i] = cr;
+  if (flags & 1) nlj[i] = cl;
+}
+
+void launch_build_lists(hipStream_t st, const RepDev* reps, const int* slot_rep, int slot_begin, int nslots,
+                        const double* rx, const double* ry, const double* rz, const int* cell_start,
+                        const int* cell_cnt, const int* cell_beads, const int* rep_flags, int nslots_total,
+                        int cap_rpy, int* nrpy, int* lrpy, int cap_lj, int* nlj, int* llj, int* overflow) {
+  if (nslots <= 0) return;
+  k_build_lists<<<(nslots + 127) / 128, 128, 0, st>>>(reps, slot_rep, slot_begin, nslots, rx, ry, rz, cell_start,
+                                                      cell_cnt, cell_beads, rep_flags, nslots_total, cap_rpy, nrpy,
+                                                      lrpy, cap_lj, nlj, llj, overflow);
+}
+
+// Fill kernels (used instead of memset so that captured graphs replay correctly).
+template <typename T> __global__ void k_fill(T* p, T v, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long stride = (long long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+static int fill_blocks(long long n) {
+  long long b = (n + 255) / 256;
+  return (int)(b > 4096 ? 4096 : (b < 1 ? 1 : b));
+}
+void launch_fill_i32(hipStream_t st, int* p, int v, long long n) {
+  if (n > 0) k_fill<int><<<fill_blocks(n), 256, 0, st>>>(p, v, n);
+}
+void launch_fill_f64(hipStream_t st, double* p, double v, long long n) {
+  if (n > 0) k_fill<double><<<fill_blocks(n), 256, 0, st>>>(p, v, n);
+}
+void launch_fill_f32(hipStream_t st, float* p, float v, long long n) {
+  if (n > 0) k_fill<float><<<fill_blocks(n), 256, 0, st>>>(p, v, n);
+}
+
+// Rebuild decision per replica: bit0 when the reference's Verlet trigger fires
+// (max |dxv| > (rv-rc)/2, checkVerlet S/main26.c:858-869); bit1 when the
+// mobility list may have lost a pair: 2 max|dsp| plus the affine stretch of a
+// pair separation since the last build reaches the skin, or a new lattice period.
+__global__ void k_nl_check(const RepDev* __restrict__ reps, int rep_begin, int nrep, const double* out2, double* aff,
+                           int force, int* rep_flags) {
+  int r = rep_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rep_begin + nrep) return;
+  const RepDev& R = reps[r];
+  if (!R.active) {
+    rep_flags[r] = 0;
+    return;
+  }
+  int f = 0;
+  if (sqrt(out2[2 * r]) > R.rnew || (force & 1)) f |= 1;
+  double a = aff[r] + fabs(R.edot) * R.dt;
+  double reach = sqrt(R.rlist2);
+  if (2.0 * sqrt(out2[2 * r + 1]) + a * reach >= R.skin || R.reset || (force & 2)) {
+    f |= 2;
+    a = 0.0;
+  }
+  aff[r] = a;
+  rep_flags[r] = f;
+}
+
+void launch_nl_check(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, const double* out2, double* aff,
+                     int force, int* rep_flags) {
+  if (nrep > 0) k_nl_check<<<(nrep + 63) / 64, 64, 0, st>>>(reps, rep_begin, nrep, out2, aff, force, rep_flags);
+}
+
+// Zero the displacement trackers of the replicas whose lists were rebuilt
+// (verletlist zeroes dxv, S/main26.c:686-688).
+__global__ void k_reset_trackers(const RepDev* __restrict__ reps, const int* __restrict__ slot_rep, int slot_begin,
+                                 int nslots, const int* rep_flags, double* dxv, double* dyv, double* dzv,
+                                 double* dsp) {
+  int s = slot_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= slot_begin + nslots) return;
+  int r = slot_rep[s];
+  if (r < 0 || !reps[r].active) return;
+  int f = rep_flags[r];
+  if (f & 1) dxv[s] = dyv[s] = dzv[s] = 0.0;
+  if (f & 2) dsp[3 * s] = dsp[3 * s + 1] = dsp[3 * s + 2] = 0.0;
+}
+
+void launch_reset_trackers(hipStream_t st, const RepDev* reps, const int* slot_rep, int slot_begin, int nslots,
+                           const int* rep_flags, double* dxv, double* dyv, double* dzv, double* dsp) {
+  if (nslots > 0)
+    k_reset_trackers<<<(nslots + 255) / 256, 256, 0, st>>>(reps, slot_rep, slot_begin, nslots, rep_flags, dxv, dyv,
+                                                           dzv, dsp);
+}

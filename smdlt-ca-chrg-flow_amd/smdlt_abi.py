@@ -1,0 +1,320 @@
+"""ctypes binding of libsmdlt_hip.so (the C ABI in include/smdlt_hip.h).
+
+Host-side mirror of the reference's hot-path calls for Python callers (tests,
+bench).  The library must have been built (`make -C smdlt-ca-chrg-flow_amd`);
+loading fails loudly otherwise -- there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libsmdlt_hip.so")
+HEADER = os.path.join(_HERE, "..", "include", "smdlt_hip.h")
+
+FLOW_EQUILIBRIUM, FLOW_SHEAR, FLOW_EXTENSION = 0, 1, 2
+PREC_MIXED, PREC_FP64 = 0, 1
+NOISE_LANCZOS, NOISE_TEA = 0, 1
+MC_REFERENCE, MC_MH_UNIFORM, MC_MH_LEARNED = 0, 1, 2
+
+ERRORS = {0: "ok", -1: "SMDLT_E_ARG", -2: "SMDLT_E_HIP", -3: "SMDLT_E_FENE_OVERSTRETCH",
+          -4: "SMDLT_E_LANCZOS_NOCONV", -5: "SMDLT_E_NLIST_OVERFLOW", -6: "SMDLT_E_GEOMETRY",
+          -7: "SMDLT_E_STATE", -8: "SMDLT_E_NOMEM", -9: "SMDLT_E_MC_STREAM"}
+
+
+class Params(C.Structure):
+    _fields_ = [("nb", C.c_int), ("nc", C.c_int), ("L", C.c_double), ("flow_type", C.c_int),
+                ("edot", C.c_double), ("dt", C.c_double), ("spring_fene", C.c_int),
+                ("epsilon", C.c_double), ("kappab", C.c_double), ("kappas", C.c_double),
+                ("qmax", C.c_double), ("lambda_b", C.c_double), ("lambda_d", C.c_double),
+                ("barrier", C.c_double), ("barrier2", C.c_double), ("lj_ext_form", C.c_int),
+                ("pbc_compat", C.c_int), ("precision", C.c_int), ("noise_mode", C.c_int),
+                ("ewald_alpha", C.c_double), ("rc_real", C.c_double), ("skin", C.c_double),
+                ("grid", C.c_int * 3), ("spline_order", C.c_int), ("mobility_tol", C.c_double),
+                ("lanczos_tol", C.c_double), ("lanczos_maxit", C.c_int), ("seed", C.c_uint64)]
+
+
+class SmdltError(RuntimeError):
+    def __init__(self, code, where):
+        self.code = code
+        super().__init__(f"{where}: {ERRORS.get(code, code)} ({lib().smdlt_strerror(code).decode()})")
+
+
+_lib = None
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_fp = C.POINTER(C.c_float)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                               "or `make -C smdlt-ca-chrg-flow_amd`")
+        L = C.CDLL(LIB_PATH)
+        L.smdlt_version.restype = C.c_char_p
+        L.smdlt_strerror.restype = C.c_char_p
+        L.smdlt_strerror.argtypes = [C.c_int]
+        L.smdlt_default_params.argtypes = [C.POINTER(Params)]
+        L.smdlt_create.argtypes = [C.POINTER(Params), C.c_int, C.c_int, C.POINTER(C.c_void_p)]
+        L.smdlt_destroy.argtypes = [C.c_void_p]
+        L.smdlt_num_beads.argtypes = [C.c_void_p, C.c_int]
+        L.smdlt_get_mobility_config.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.smdlt_upload_positions.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.smdlt_download_positions.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.smdlt_upload_charges.argtypes = [C.c_void_p, _ip, _ip, _ip]
+        L.smdlt_download_charges.argtypes = [C.c_void_p, _ip, _ip]
+        L.smdlt_update_lattice.argtypes = [C.c_void_p, C.c_uint64, _dp]
+        L.smdlt_remap.argtypes = [C.c_void_p, C.c_uint64]
+        L.smdlt_apply_pbc.argtypes = [C.c_void_p]
+        L.smdlt_rotate.argtypes = [C.c_void_p, C.c_int]
+        L.smdlt_min_image.argtypes = [C.c_void_p, C.c_int, C.c_int, _ip, _ip, _dp]
+        L.smdlt_bin_index.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, _ip, _ip, _ip]
+        L.smdlt_neighbors.argtypes = [C.c_void_p, C.c_int, _ip]
+        L.smdlt_get_verlet.argtypes = [C.c_void_p, C.c_int, _ip, _ip, C.c_int]
+        L.smdlt_forces.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.smdlt_mobility_apply.argtypes = [C.c_void_p, _dp, _dp]
+        L.smdlt_brownian.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.c_int, _ip, _dp]
+        L.smdlt_tea_apply.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+        L.smdlt_step.argtypes = [C.c_void_p, C.c_uint64, _dp]
+        L.smdlt_run.argtypes = [C.c_void_p, C.c_uint64, C.c_int, C.c_int]
+        L.smdlt_last_iters.argtypes = [C.c_void_p, _ip]
+        L.smdlt_set_profiling.argtypes = [C.c_void_p, C.c_int]
+        L.smdlt_stage_times.argtypes = [C.c_void_p, _dp, _ip, C.c_int]
+        L.smdlt_stage_name.restype = C.c_char_p
+        L.smdlt_stage_name.argtypes = [C.c_int]
+        L.smdlt_sync.argtypes = [C.c_void_p]
+        L.smdlt_select_replicas.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        for name in ("smdlt_mc_sweep", "smdlt_mc_energy", "smdlt_mlp_num_params", "smdlt_mlp_set_weights",
+                     "smdlt_mlp_loss_grad", "smdlt_mc_features"):
+            if hasattr(L, name):
+                pass
+        if hasattr(L, "smdlt_mc_sweep"):
+            L.smdlt_mc_sweep.argtypes = [C.c_void_p, C.c_int, _fp, C.c_int, _ip, _ip]
+            L.smdlt_mc_energy.argtypes = [C.c_void_p, _dp]
+            L.smdlt_mlp_set_weights.argtypes = [C.c_void_p, _fp]
+            L.smdlt_mlp_loss_grad.argtypes = [C.c_void_p, _fp, _fp, _fp, C.c_int, _fp, _fp]
+            L.smdlt_mc_features.argtypes = [C.c_void_p, C.c_int, _fp, _ip, _ip, C.c_int, _ip]
+        _lib = L
+    return _lib
+
+
+def declared_symbols():
+    """Names of every function the ABI header declares."""
+    import re
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(smdlt_[a-z0-9_]+)\s*\(", text)))
+
+
+def _ptr(a, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def _chk(code, where):
+    if code < 0:
+        raise SmdltError(code, where)
+    return code
+
+
+def default_params(**kw):
+    p = Params()
+    lib().smdlt_default_params(C.byref(p))
+    for k, v in kw.items():
+        if k == "grid":
+            p.grid = (C.c_int * 3)(*v)
+        else:
+            setattr(p, k, v)
+    return p
+
+
+class Context:
+    """One device context over a list of replicas (smdlt_create/smdlt_destroy)."""
+
+    def __init__(self, params, device=0):
+        if isinstance(params, Params):
+            params = [params]
+        arr = (Params * len(params))(*params)
+        h = C.c_void_p()
+        _chk(lib().smdlt_create(arr, len(params), device, C.byref(h)), "smdlt_create")
+        self.h = h
+        self.params = list(params)
+        self.nrep = len(params)
+        self.n = [p.nb * p.nc for p in params]
+        self.ntot = sum(self.n)
+
+    def close(self):
+        if self.h:
+            lib().smdlt_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def mobility_config(self, r=0):
+        out = np.zeros(7)
+        _chk(lib().smdlt_get_mobility_config(self.h, r, _ptr(out, C.c_double)), "config")
+        return dict(alpha=out[0], rc=out[1], skin=out[2], grid=tuple(int(x) for x in out[3:6]), order=int(out[6]))
+
+    def upload_positions(self, rx, ry, rz):
+        rx, ry, rz = (np.ascontiguousarray(a, np.float64) for a in (rx, ry, rz))
+        _chk(lib().smdlt_upload_positions(self.h, _ptr(rx, C.c_double), _ptr(ry, C.c_double),
+                                          _ptr(rz, C.c_double)), "upload_positions")
+
+    def download_positions(self):
+        out = [np.zeros(self.ntot) for _ in range(3)]
+        _chk(lib().smdlt_download_positions(self.h, *(_ptr(a, C.c_double) for a in out)), "download")
+        return out
+
+    def update_lattice(self, t):
+        out = np.zeros(8 * self.nrep)
+        _chk(lib().smdlt_update_lattice(self.h, t, _ptr(out, C.c_double)), "update_lattice")
+        return out.reshape(self.nrep, 8)
+
+    def remap(self, t):
+        _chk(lib().smdlt_remap(self.h, t), "remap")
+
+    def apply_pbc(self):
+        _chk(lib().smdlt_apply_pbc(self.h), "apply_pbc")
+
+    def rotate(self, inverse):
+        _chk(lib().smdlt_rotate(self.h, int(inverse)), "rotate")
+
+    def min_image(self, r, i, j):
+        i = np.ascontiguousarray(i, np.int32)
+        j = np.ascontiguousarray(j, np.int32)
+        out = np.zeros(3 * len(i))
+        _chk(lib().smdlt_min_image(self.h, r, len(i), _ptr(i, C.c_int), _ptr(j, C.c_int),
+                                   _ptr(out, C.c_double)), "min_image")
+        return out.reshape(-1, 3)
+
+    def bin_index(self, r, bin_size, i, j):
+        i = np.ascontiguousarray(i, np.int32)
+        j = np.ascontiguousarray(j, np.int32)
+        out = np.zeros(len(i), np.int32)
+        _chk(lib().smdlt_bin_index(self.h, r, bin_size, len(i), _ptr(i, C.c_int), _ptr(j, C.c_int),
+                                   _ptr(out, C.c_int)), "bin_index")
+        return out
+
+    def neighbors(self, force=False):
+        n = C.c_int()
+        _chk(lib().smdlt_neighbors(self.h, int(force), C.byref(n)), "neighbors")
+        return n.value
+
+    def verlet(self, r, stride=None):
+        n = self.n[r]
+        stride = stride or n
+        nl = np.zeros(n, np.int32)
+        lst = np.zeros(n * stride, np.int32)
+        _chk(lib().smdlt_get_verlet(self.h, r, _ptr(nl, C.c_int), _ptr(lst, C.c_int), stride), "get_verlet")
+        return nl, lst.reshape(n, stride)
+
+    def forces(self):
+        f = [np.zeros(self.ntot) for _ in range(3)]
+        _chk(lib().smdlt_forces(self.h, *(_ptr(a, C.c_double) for a in f)), "forces")
+        return np.stack(f, 1)
+
+    def mobility_apply(self, v):
+        v = np.ascontiguousarray(v, np.float64).reshape(-1)
+        u = np.zeros_like(v)
+        _chk(lib().smdlt_mobility_apply(self.h, _ptr(v, C.c_double), _ptr(u, C.c_double)), "mobility_apply")
+        return u
+
+    def brownian(self, z, tol=1e-6, maxit=30):
+        z = np.ascontiguousarray(z, np.float64).reshape(-1)
+        y = np.zeros_like(z)
+        it = np.zeros(self.nrep, np.int32)
+        res = np.zeros(self.nrep)
+        code = lib().smdlt_brownian(self.h, _ptr(z, C.c_double), _ptr(y, C.c_double), tol, maxit,
+                                    _ptr(it, C.c_int), _ptr(res, C.c_double))
+        if code < 0 and code != -4:
+            raise SmdltError(code, "brownian")
+        return y, it, res, code
+
+    def tea_apply(self, beta, Cv, z):
+        beta = np.ascontiguousarray(np.broadcast_to(beta, (self.nrep,)), np.float64)
+        Cv = np.ascontiguousarray(Cv, np.float64).reshape(-1)
+        z = np.ascontiguousarray(z, np.float64).reshape(-1)
+        y = np.zeros_like(z)
+        _chk(lib().smdlt_tea_apply(self.h, _ptr(beta, C.c_double), _ptr(Cv, C.c_double), _ptr(z, C.c_double),
+                                   _ptr(y, C.c_double)), "tea_apply")
+        return y
+
+    def step(self, t, noise=None):
+        if noise is None:
+            code = lib().smdlt_step(self.h, t, None)
+        else:
+            noise = np.ascontiguousarray(noise, np.float64).reshape(-1)
+            code = lib().smdlt_step(self.h, t, _ptr(noise, C.c_double))
+        if code < 0 and code != -4:
+            raise SmdltError(code, "step")
+        return code
+
+    def run(self, t0, nsteps, use_graph=True):
+        code = lib().smdlt_run(self.h, t0, nsteps, int(use_graph))
+        if code < 0 and code != -4:
+            raise SmdltError(code, "run")
+        return code
+
+    def last_iters(self):
+        it = np.zeros(self.nrep, np.int32)
+        _chk(lib().smdlt_last_iters(self.h, _ptr(it, C.c_int)), "last_iters")
+        return it
+
+    def set_profiling(self, on):
+        _chk(lib().smdlt_set_profiling(self.h, int(on)), "set_profiling")
+
+    def stage_times(self):
+        ms = np.zeros(16)
+        cnt = np.zeros(16, np.int32)
+        n = _chk(lib().smdlt_stage_times(self.h, _ptr(ms, C.c_double), _ptr(cnt, C.c_int), 16), "stage_times")
+        return {lib().smdlt_stage_name(i).decode(): (ms[i], int(cnt[i])) for i in range(n)}
+
+    def select_replicas(self, b, e):
+        _chk(lib().smdlt_select_replicas(self.h, b, e), "select_replicas")
+
+    def sync(self):
+        _chk(lib().smdlt_sync(self.h), "sync")
+
+    # ---- charges / MC
+    def upload_charges(self, nq, track, sign=None):
+        nq = np.ascontiguousarray(nq, np.int32)
+        track = np.ascontiguousarray(track, np.int32)
+        sign = np.ones_like(track) if sign is None else np.ascontiguousarray(sign, np.int32)
+        _chk(lib().smdlt_upload_charges(self.h, _ptr(nq, C.c_int), _ptr(track, C.c_int), _ptr(sign, C.c_int)),
+             "upload_charges")
+        self.nq = nq.copy()
+
+    def download_charges(self):
+        tot = int(self.nq.sum())
+        tr = np.zeros(max(tot, 1), np.int32)
+        old = np.zeros(max(tot, 1), np.int32)
+        _chk(lib().smdlt_download_charges(self.h, _ptr(tr, C.c_int), _ptr(old, C.c_int)), "download_charges")
+        return tr[:tot], old[:tot]
+
+    def mc_sweep(self, mode=MC_REFERENCE, uniforms=None):
+        tot = int(self.nq.sum())
+        consumed = np.zeros(self.nrep, np.int32)
+        log = np.zeros(6 * max(tot, 1), np.int32)
+        if uniforms is None:
+            code = lib().smdlt_mc_sweep(self.h, mode, None, 0, _ptr(consumed, C.c_int), _ptr(log, C.c_int))
+        else:
+            u = np.ascontiguousarray(uniforms, np.float32)
+            nu = u.shape[-1]
+            code = lib().smdlt_mc_sweep(self.h, mode, _ptr(u, C.c_float), nu, _ptr(consumed, C.c_int),
+                                        _ptr(log, C.c_int))
+        _chk(code, "mc_sweep")
+        return consumed, log[:6 * tot].reshape(-1, 6)
+
+    def mc_energy(self):
+        e = np.zeros(self.nrep)
+        _chk(lib().smdlt_mc_energy(self.h, _ptr(e, C.c_double)), "mc_energy")
+        return e
+
+
+def stage_names():
+    return [lib().smdlt_stage_name(i).decode() for i in range(8)]

@@ -1,0 +1,402 @@
+"""Pins of the CPU oracle (the checker) against closed forms, invariants,
+brute force and the values the survey measured with an independent restatement.
+These run without a GPU."""
+import numpy as np
+import pytest
+from helpers import O, random_walk
+
+import spme_ref as P
+
+
+def xyz(pos):
+    return [np.ascontiguousarray(pos[:, k]) for k in range(3)]
+
+
+# ----------------------------------------------------------------- RNG (ran1/gasdev)
+def test_ran1_lcg_state_is_schrage_multiplication():
+    # idum_{k+1} = 40014 * idum_k mod 2147483563 (L'Ecuyer 1988), computed here
+    # with exact integers instead of Schrage's decomposition.
+    r = O.Ran1(830062920)
+    x = 830062920
+    for _ in range(1000):
+        r.uniforms(1)
+        x = (40014 * x) % 2147483563
+        assert r.s.idum == x
+
+
+def test_ran1_positive_seed_quirks():
+    # positive seeds skip the shuffle-table fill, so the first two draws are
+    # equal, and idum2 never advances (S/Initialization.c:609-627)
+    u = O.Ran1(830062920).uniforms(3)
+    assert u[0] == u[1]
+    r = O.Ran1(981272199)
+    r.uniforms(500)
+    assert r.s.idum2 == 123456789
+
+
+def test_ran1_range_and_moments():
+    u = O.Ran1(12345).uniforms(200000).astype(np.float64)
+    assert u.min() > 0 and u.max() <= 1 - 1.2e-7 + 1e-12
+    assert abs(u.mean() - 0.5) < 0.003
+    g = O.Ran1(12345).gauss(200000)
+    assert abs(g.mean()) < 0.01 and abs(g.var() - 1) < 0.01
+
+
+def test_ran1_negative_seed_fills_table():
+    r = O.Ran1(-7)
+    u = r.uniforms(2)
+    assert u[0] != u[1]
+    assert r.s.idum2 == 7
+
+
+# ----------------------------------------------------------------- lattice / remap
+def test_shear_lattice_is_deforming_brick():
+    L, edot, dt = 30.0, 0.05, 1e-3
+    lat = O.lattice(1, L, edot, dt)
+    assert lat.tp == int(0.9624 / edot / dt)
+    for t in (0, 17, lat.tp - 1, lat.tp + 5):
+        O.lattice_update(lat, t)
+        g = edot * (t % lat.tp) * dt
+        assert lat.L1xp == pytest.approx(L, rel=1e-14) and abs(lat.L1yp) < 1e-12
+        assert lat.L2xp == pytest.approx(g * L, abs=1e-12)
+        assert lat.L2yp == pytest.approx(L, rel=1e-14)
+        assert lat.delx == pytest.approx(g, abs=1e-15)
+
+
+def test_krbc_lattice_returns_after_one_period():
+    # Kraynik-Reinelt: after the Hencky strain ln((3+sqrt5)/2) the deformed
+    # lattice is the initial one in a new integer basis.  The reference uses the
+    # truncated EP=0.9624 and THETA0=31.7 deg, so equivalence holds to ~1e-4.
+    L = 20.0
+    edot, dt = 0.1, 1e-3
+    lat = O.lattice(2, L, edot, dt)
+    A0 = np.array([[lat.L1[0], lat.L2[0]], [lat.L1[1], lat.L2[1]]])
+    pts = np.array(lat.point0)
+    f = np.exp(edot * lat.tp * dt)
+    A1 = (pts[2] - pts[1]) * [f, 1 / f], (pts[0] - pts[1]) * [f, 1 / f]
+    A1 = np.array(A1).T
+    Mint = np.linalg.solve(A0, A1)
+    assert np.abs(Mint - np.round(Mint)).max() < 2e-3
+    assert abs(np.linalg.det(np.round(Mint))) == pytest.approx(1.0)
+
+
+@pytest.mark.parametrize("flow,compat", [(1, 1), (1, 0), (2, 1), (0, 1)])
+def test_apply_pbc_wraps_into_cell(flow, compat):
+    L = 25.0
+    lat = O.lattice(flow, L, 0.2, 1e-3, compat, 2345)
+    rng = np.random.default_rng(3)
+    p = rng.uniform(-3 * L, 3 * L, (500, 3))
+    rx, ry, rz = O.apply_pbc(lat, *xyz(p))
+    assert np.all(np.abs(ry) <= lat.L2yp / 2 + 1e-9)
+    assert np.all(np.abs(rz) <= L / 2 + 1e-9)
+    assert np.all(np.abs(rx - ry * lat.L2xp / lat.L2yp) <= lat.L1xp / 2 + 1e-9)
+    # idempotent
+    rx2, ry2, rz2 = O.apply_pbc(lat, rx, ry, rz)
+    assert np.allclose(rx2, rx) and np.allclose(ry2, ry) and np.allclose(rz2, rz)
+
+
+@pytest.mark.parametrize("flow", [1, 2])
+def test_min_image_lattice_shift_invariant(flow):
+    # without the delx quirk getNID only depends on positions modulo the lattice
+    L = 18.0
+    lat = O.lattice(flow, L, 0.3, 1e-3, 0, 777)
+    rng = np.random.default_rng(5)
+    p = rng.uniform(-L / 2, L / 2, (40, 3))
+    rx, ry, rz = xyz(p)
+    a1 = np.array([lat.L1xp, 0.0, 0.0])
+    a2 = np.array([lat.L2xp, lat.L2yp, 0.0])
+    for i in range(0, 40, 7):
+        for j in range(1, 40, 5):
+            d0 = O.nid(lat, rx, ry, rz, i, j)
+            q = p.copy()
+            q[j] += 2 * a1 - a2 + np.array([0, 0, 3 * L])
+            d1 = O.nid(lat, *xyz(q), i, j)
+            assert np.allclose(d0, d1, atol=1e-9)
+
+
+def test_min_image_is_true_minimum_within_half_width():
+    L = 20.0
+    lat = O.lattice(1, L, 0.5, 1e-3, 0, 300)
+    cell = (lat.L1xp, lat.L2xp, lat.L2yp, L)
+    rng = np.random.default_rng(9)
+    p = rng.uniform(-L / 2, L / 2, (60, 3))
+    rx, ry, rz = xyz(p)
+    for i in range(60):
+        for j in range(60):
+            if i == j:
+                continue
+            d = O.nid(lat, rx, ry, rz, i, j)
+            m = P.min_image(p[i] - p[j], cell)
+            if np.linalg.norm(m) < 0.4 * L:
+                assert np.allclose(d, m, atol=1e-9)
+
+
+def test_krbc_rotation_round_trip_drift_is_ulps():
+    lat = O.lattice(2, 30.0, 0.1, 1e-3, 1, 4000)
+    rng = np.random.default_rng(2)
+    rx, ry = rng.uniform(-15, 15, 200), rng.uniform(-15, 15, 200)
+    a, b = O.rotate(lat, rx, ry, 0)
+    c, d = O.rotate(lat, a, b, 1)
+    assert np.abs(c - rx).max() < 1e-13 and np.abs(d - ry).max() < 1e-13
+
+
+# ----------------------------------------------------------------- neighbour list
+@pytest.mark.parametrize("flow,compat", [(1, 1), (1, 0), (2, 1)])
+def test_verlet_matches_brute_force(flow, compat):
+    L = 22.0
+    pos = random_walk(30, 4, L, 11)
+    lat = O.lattice(flow, L, 0.4, 1e-3, compat, 1500)
+    rx, ry, rz = O.apply_pbc(lat, *xyz(pos))
+    nl, lst = O.verlet(lat, rx, ry, rz, 6.0)
+    n = len(rx)
+    for i in range(n):
+        expect = [j for j in range(i + 1, n) if np.linalg.norm(O.nid(lat, rx, ry, rz, i, j)) < 6.0]
+        assert list(lst[i, :nl[i]]) == expect
+    # with the exact wrap the list is the true minimum-image neighbour set
+    if not compat:
+        cell = (lat.L1xp, lat.L2xp, lat.L2yp, L)
+        p = np.stack([rx, ry, rz], 1)
+        for i in range(0, n, 9):
+            d = np.linalg.norm(P.min_image(p[i + 1:] - p[i], cell), axis=1)
+            assert sorted((np.nonzero(d < 6.0)[0] + i + 1).tolist()) == list(lst[i, :nl[i]])
+
+
+# ----------------------------------------------------------------- forces
+def test_two_bead_forces_closed_form():
+    lat = O.lattice(0, 50.0, 0.0, 1e-3)
+    r = 2.3
+    rx, ry, rz = np.array([0.0, r]), np.zeros(2), np.zeros(2)
+    st, fb, tau = O.bond_force(lat, 2, 1, rx, ry, rz, 0, 200.0, 2.0)
+    assert st == 0
+    assert fb[0, 0] == pytest.approx(200.0 * (r - 2.0))  # Hookean pulls together
+    st, ff, _ = O.bond_force(lat, 2, 1, rx, ry, rz, 1, 30.0, 3.0)
+    assert ff[0, 0] == pytest.approx(30.0 * r / (1 - r * r / 9.0))
+    st, _, _ = O.bond_force(lat, 2, 1, np.array([0.0, 3.5]), ry, rz, 1, 30.0, 3.0)
+    assert st == -1  # FENE overstretch (S/main26.c:553-558)
+    nl, lst = O.verlet(lat, rx, ry, rz, 6.0)
+    fl, _ = O.lj_force(lat, rx, ry, rz, nl, lst, 25.0, 0.1, 0)
+    s6 = (4.0 / r ** 2) ** 3
+    assert fl[1, 0] == pytest.approx(12 * 0.1 / r ** 2 * (s6 * s6 - s6) * r)
+    q = np.array([1, 1], np.int32)
+    fc = O.coulomb_force(lat, rx, ry, rz, q, 3.59, 5.0)
+    assert fc[1, 0] == pytest.approx(3.59 * np.exp(-r / 5) * (1 + r / 5) / r ** 2)
+    assert O.coulomb_energy(lat, rx, ry, rz, q, 3.59, 5.0) == pytest.approx(3.59 * np.exp(-r / 5) / r)
+
+
+def test_forces_sum_to_zero():
+    L = 30.0
+    pos = random_walk(25, 3, L, 2)
+    lat = O.lattice(1, L, 0.1, 1e-3, 0, 100)
+    rx, ry, rz = O.apply_pbc(lat, *xyz(pos))
+    st, fb, _ = O.bond_force(lat, 25, 3, rx, ry, rz, 0, 200.0, 2.0, kappab=1.5)
+    nl, lst = O.verlet(lat, rx, ry, rz, 6.0)
+    fl, _ = O.lj_force(lat, rx, ry, rz, nl, lst, 25.0, 0.1, 0)
+    q = np.zeros(75, np.int32)
+    q[::3] = 1
+    q[1::7] = -1
+    fc = O.coulomb_force(lat, rx, ry, rz, q, 3.59, 5.0)
+    for f in (fb, fl, fc):
+        assert np.abs(f.sum(0)).max() < 1e-9 * max(1.0, np.abs(f).max())
+
+
+# ----------------------------------------------------------------- Ewald RPY
+@pytest.mark.parametrize("L", [12.0, 20.0, 60.0])
+def test_hasimoto_self_mobility(L):
+    cell = O.Cell(L, 0, 0, L, L)
+    for a in (3.0 / L, 6.0 / L):
+        M = O.ewald_dense(cell, np.zeros(3), a)
+        h = 1 - 2.8372974795 / L + 4.1887902048 / L ** 3
+        assert M[0, 0] == pytest.approx(h, abs=30.0 / L ** 6 + 1e-9)
+        assert np.allclose(M, M[0, 0] * np.eye(3), atol=1e-12)
+
+
+def test_ewald_alpha_invariant_on_sheared_lattice():
+    L = 15.0
+    cell = O.Cell(L, 0, 0.37 * L, L, L)
+    pos = np.random.default_rng(1).uniform(-L / 2, L / 2, (6, 3))
+    Ms = [O.ewald_dense(cell, pos, a) for a in (0.2, 0.4, 0.8)]
+    assert np.abs(Ms[0] - Ms[1]).max() < 1e-11
+    assert np.abs(Ms[1] - Ms[2]).max() < 1e-11
+
+
+def test_ewald_symmetric_positive_definite_with_overlaps():
+    L = 14.0
+    pos = random_walk(20, 2, L, 4, bond=1.5, min_sep=1.0)
+    M = O.ewald_dense(O.Cell(L, 0, 0.2 * L, L, L), pos, 0.5)
+    assert np.abs(M - M.T).max() < 1e-12
+    assert np.linalg.eigvalsh(M).min() > 0
+
+
+def test_ewald_large_box_approaches_free_rpy():
+    pos = np.array([[0.0, 0, 0], [2.5, 0.7, -0.4], [1.0, 1.3, 0.2]])
+    Mf = O.rpy_free_dense(pos)
+    for L in (200.0, 400.0):
+        Me = O.ewald_dense(O.Cell(L, 0, 0, L, L), pos, 6.0 / L)
+        dev = np.abs(Me - Mf).max()
+        assert dev < 3.2 / L and dev > 2.0 / L  # the leading periodic correction is -2.837/L
+
+
+def test_overlap_branch_matches_survey_measurements():
+    # relative error of the reference's r<=2 I-coefficient at alpha=4/60
+    for r, e in ((1.0, 2.67e-2), (1.5, 4.13e-3), (1.9, 2.14e-3), (1.99, 3.21e-4)):
+        te, _ = O.rpy_real_coeffs(r, 4 / 60, 0)
+        tr, _ = O.rpy_real_coeffs(r, 4 / 60, 1)
+        assert abs(tr - te) / abs(te) == pytest.approx(e, rel=0.01)
+    # and the exact branch is continuous at r = 2 with the far form
+    a, b = O.rpy_real_coeffs(2.0 - 1e-9, 0.3, 0)
+    c, d = O.rpy_real_coeffs(2.0 + 1e-9, 0.3, 0)
+    assert abs(a - c) < 1e-8 and abs(b - d) < 1e-8
+
+
+def test_calchi_self_term_survey_values():
+    L = 60.0
+    lat = O.lattice(1, L, 0.01, 1e-3, 1, 100)  # t%tp <= tp/2: rce = L
+    b = O.bins(lat, 2.0)
+    pos = np.array([[0.0, 0, 0], [3.0, 1.0, 0.5]])
+    M, far, kc = O.calchi_ref(lat, *xyz(pos), 100, 8, b, None, L ** 3)
+    assert M[0, 0] == pytest.approx(0.95272918, abs=2e-8)
+    # second half of a period with an undeformed lattice: rce = 2L, alpha = 2/L,
+    # the self-image loop with && and dy*dy gives 0.94307920 (survey, App. A.1)
+    lat.treal = lat.tp // 2 + 1
+    lat.L2[0] = 0.0
+    lat.L2xp = 0.0
+    lat.delx = 0.0
+    M2, _, _ = O.calchi_ref(lat, *xyz(pos), 0, 8, b, None, L ** 3)
+    assert M2[0, 0] == pytest.approx(0.94307920, abs=2e-8)
+
+
+def test_calchi_near_pairs_match_exact_ewald_in_declared_regime():
+    # t%tp <= tp/2, all pairs closer than 12a, none overlapping: the reference's
+    # matrix equals the exact Ewald sum up to its k-cut (~1e-6 relative)
+    L = 13.0
+    rng = np.random.default_rng(8)
+    lat = O.lattice(1, L, 0.05, 1e-3, 1, 2000)
+    cell = (lat.L1xp, lat.L2xp, lat.L2yp, L)
+    pos = []
+    while len(pos) < 12:  # random sequential addition, no pair closer than 2.2
+        q = rng.uniform(-L / 2, L / 2, 3)
+        if all(np.linalg.norm(P.min_image(q - x, cell)) > 2.2 for x in pos):
+            pos.append(q)
+    rx, ry, rz = O.apply_pbc(lat, *xyz(np.array(pos)))
+    p = np.stack([rx, ry, rz], 1)
+    b = O.bins(lat, 2.0)
+    M, far, kc = O.calchi_ref(lat, rx, ry, rz, 2000, 8, b, None, L ** 3)
+    Me = O.ewald_dense(O.cell_of(lat), p, 0.4)
+    off = ~np.kron(np.eye(len(p), dtype=bool), np.ones((3, 3), bool))
+    assert np.abs(M - Me)[off].max() / np.abs(Me[off]).max() < 2e-5
+
+
+# ----------------------------------------------------------------- Brownian helpers
+def test_tea_reduces_to_mobility_and_diagonal():
+    L = 16.0
+    pos = random_walk(10, 2, L, 1)
+    M = O.ewald_dense(O.Cell(L, 0, 0, L, L), pos, 0.5)
+    z = np.random.default_rng(0).normal(size=60)
+    one = np.ones(60)
+    assert np.allclose(O.tea_apply(M, 1.0, one, z), M @ z, atol=1e-12)
+    assert np.allclose(O.tea_apply(M, 0.0, one, z), np.diag(M) * z, atol=1e-12)
+    Cv = np.random.default_rng(1).uniform(0.5, 1, 60)
+    y = O.tea_apply(M, 0.3, Cv, z)
+    D = np.diag(np.diag(M))
+    assert np.allclose(y, Cv * (0.3 * (M - D) @ z + D @ z), atol=1e-12)
+
+
+def test_jacobi_sqrt_squares_back():
+    L = 16.0
+    pos = random_walk(12, 2, L, 3)
+    M = O.ewald_dense(O.Cell(L, 0, 0.1 * L, L, L), pos, 0.5)
+    z = np.random.default_rng(2).normal(size=M.shape[0])
+    y, st, lo, hi = O.sym_sqrt_apply(M, z)
+    assert st == 0 and lo > 0
+    yy, _, _, _ = O.sym_sqrt_apply(M, y)
+    assert np.allclose(yy, M @ z, atol=1e-10)
+    assert z @ M @ z == pytest.approx(y @ y, rel=1e-12)
+    # numpy eigendecomposition as a second route
+    w, V = np.linalg.eigh(M)
+    assert np.allclose(y, V @ (np.sqrt(w) * (V.T @ z)), atol=1e-10)
+
+
+def test_lanczos_oracle_converges_to_sqrt():
+    L = 16.0
+    pos = random_walk(20, 2, L, 3)
+    M = O.ewald_dense(O.Cell(L, 0, 0.1 * L, L, L), pos, 0.5)
+    z = np.random.default_rng(2).normal(size=M.shape[0])
+    y, st, it = O.lanczos_sqrt(M, z, 1e-6, 60)
+    ys, _, _, _ = O.sym_sqrt_apply(M, z)
+    assert st == 0 and it < 40
+    assert np.linalg.norm(y - ys) / np.linalg.norm(ys) < 1e-5
+
+
+# ----------------------------------------------------------------- PME restatement
+def test_spme_restatement_converges_to_dense_ewald():
+    L = 16.0
+    pos = random_walk(12, 2, L, 5)
+    lat = O.lattice(1, L, 0.5, 1e-3, 0, 400)
+    cell = (lat.L1xp, lat.L2xp, lat.L2yp, L)
+    Me = O.ewald_dense(O.cell_of(lat), pos, 0.5)
+    v = np.random.default_rng(4).normal(size=3 * len(pos))
+    ur = Me @ v
+    errs = []
+    for K in ((16, 16, 16), (32, 32, 32), (64, 64, 64)):
+        u = P.mobility_apply(pos, v, cell, 0.6, 7.0, K, 6)
+        errs.append(np.linalg.norm(u - ur) / np.linalg.norm(ur))
+    assert errs[0] > errs[1] > errs[2] and errs[2] < 2e-5
+
+
+def test_bspline_partition_of_unity():
+    for p in (4, 6, 8):
+        x = np.linspace(0, 0.999, 17)
+        W = P.bspline_weights(x, p)
+        assert np.allclose(W.sum(-1), 1.0)
+        assert np.all(W >= 0)
+
+
+# ----------------------------------------------------------------- MC
+def mc_setup(nb, nc, L, seed, nq):
+    pos = random_walk(nb, nc, L, seed)
+    lat = O.lattice(1, L, 0.1, 1e-3, 1, 50)
+    rx, ry, rz = O.apply_pbc(lat, *xyz(pos))
+    rng = np.random.default_rng(seed)
+    track = rng.choice(nb * nc, nq, replace=False).astype(np.int32)
+    return lat, rx, ry, rz, track
+
+
+def test_mc_two_charge_closed_form_and_draw_counts():
+    # two beads on one chain of 3: bead0 charged; a right hop to bead1 is the only
+    # option (mm > .5, no close non-adjacent bead), energy 0 with one charge
+    lat = O.lattice(0, 40.0, 0.0, 1e-3)
+    rx, ry, rz = np.array([0.0, 2.0, 4.0]), np.zeros(3), np.zeros(3)
+    mc = O.ChargeHop(3, 1, [0])
+    st, cur, log, e0 = mc.sweep(lat, rx, ry, rz, 3.59, 5.0, 3.0, 3.0, np.array([0.7, 0.01], np.float32))
+    # x = exp(-3) < 1 -> second draw 0.01 <= x accepts
+    assert st == 0 and cur == 2 and list(log[0]) == [0, 0, 1, 0, 1, 2] and mc.track[0] == 1
+    mc = O.ChargeHop(3, 1, [0])
+    st, cur, log, e0 = mc.sweep(lat, rx, ry, rz, 3.59, 5.0, 3.0, 3.0, np.array([0.2, 0.9], np.float32))
+    # left of bead 0 is blocked: skipped with a single draw
+    assert cur == 1 and log[0][3] == -1 and mc.track[0] == 0
+
+
+def test_mc_pop_targets_lowest_close_bead():
+    lat = O.lattice(0, 40.0, 0.0, 1e-3)
+    # chain of 6 folded so beads 0 and 5 and 4 are within 2.1 of bead 2? build explicitly
+    rx = np.array([0.0, 2.0, 4.0, 10.0, 5.0, 4.5])
+    ry = np.array([0.0, 0.0, 0.0, 0.0, 1.5, 1.0])
+    rz = np.zeros(6)
+    mc = O.ChargeHop(6, 1, [2])
+    st, cur, log, e0 = mc.sweep(lat, rx, ry, rz, 3.59, 5.0, 0.0, 0.0, np.array([0.5], np.float32))
+    # close non-adjacent beads of 2: bead 4 (r=1.80) and 5 (r=1.12); lowest index wins
+    assert log[0][2] == 4 and log[0][3] == 2 and log[0][4] == 1 and mc.track[0] == 4
+
+
+def test_mc_energy_bookkeeping_and_uniqueness():
+    lat, rx, ry, rz, track = mc_setup(30, 4, 25.0, 7, 24)
+    mc = O.ChargeHop(30, 4, track)
+    u = O.Ran1(830062920).uniforms(200)
+    st, cur, log, e0 = mc.sweep(lat, rx, ry, rz, 3.59, 5.0, 1.0, 1.5, u)
+    assert st == 0
+    q = np.zeros(120, np.int32)
+    q[track] = 1
+    assert e0 == pytest.approx(O.coulomb_energy(lat, rx, ry, rz, q, 3.59, 5.0))
+    assert len(set(mc.track.tolist())) == len(track)
+    assert cur == sum(int(l[5]) for l in log)
+    assert np.all(np.isin(log[:, 5], [1, 2]))
