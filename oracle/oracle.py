@@ -117,7 +117,8 @@ def lib():
                                      C.c_double, C.c_double, C.c_double, _fp, C.c_int,
                                      C.POINTER(C.c_int), C.c_void_p, C.POINTER(C.c_double)]
         L.orc_charge_hop.restype = C.c_int
-        L.orc_ref_run.argtypes = [C.POINTER(RefRun), _dp, _dp, _dp, C.c_int, C.c_int, C.POINTER(C.c_int)]
+        L.orc_ref_run.argtypes = [C.POINTER(RefRun), _dp, _dp, _dp, C.c_int, C.c_int, C.POINTER(C.c_int),
+                                  C.POINTER(C.c_double)]
         L.orc_ref_run.restype = C.c_double
         L.orc_num_threads.restype = C.c_int
         _lib = L
@@ -344,8 +345,9 @@ def ref_run(params: dict, rx, ry, rz, nsteps, nthreads=0):
     p = RefRun(**params)
     rx, ry, rz = (np.ascontiguousarray(a, np.float64).copy() for a in (rx, ry, rz))
     st = C.c_int()
-    el = lib().orc_ref_run(C.byref(p), rx, ry, rz, nsteps, nthreads, C.byref(st))
-    return el, st.value, (rx, ry, rz)
+    thi = C.c_double(0.0)
+    el = lib().orc_ref_run(C.byref(p), rx, ry, rz, nsteps, nthreads, C.byref(st), C.byref(thi))
+    return el, st.value, (rx, ry, rz), thi.value
 
 
 def num_threads():

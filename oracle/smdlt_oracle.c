@@ -821,7 +821,10 @@ void orc_calchi_ref(const orc_lattice_t* lat, int N, const double* rx, const dou
     M[(3 * i + 2) * n3 + 3 * i + 2] = Dtr[2];
   }
   int nfar = 0;
+#pragma omp parallel for schedule(dynamic) reduction(+ : nfar)
   for (int i = 0; i < N; i++) {
+    int kx, ky, kz, start;
+    double rkx, rky, rkz;
     double rxi = krbc ? rx[i] * cos(theta) - ry[i] * sin(theta) : rx[i];
     double ryi = krbc ? rx[i] * sin(theta) + ry[i] * cos(theta) : ry[i];
     for (int j = i + 1; j < N; j++) {
@@ -862,6 +865,10 @@ void orc_calchi_ref(const orc_lattice_t* lat, int N, const double* rx, const dou
         }
       } else {
         int bin = orc_binij(bins, lat, rx, ry, rz, i, j);
+        /* the shear table spans |dx| <= L/2*edot*tp*dt only (S/Initialization.c:269); the
+         * reference reads past it for wider pairs.  Clamp so the restatement stays defined. */
+        if (bin < 0) bin = 0;
+        if (bin >= bins->num_bins) bin = bins->num_bins - 1;
         memcpy(Dt, &table[(size_t)nt * 9 * bins->num_bins + 9 * (size_t)bin], 9 * sizeof(double));
       }
       for (int a = 0; a < 3; ++a)
@@ -1252,7 +1259,7 @@ int orc_charge_hop(orc_mc_state_t* s, const orc_lattice_t* lat, const double* rx
  * S/main26.c:745-857): lattice, PBC, Verlet check, forces, noise, dense M every
  * srpy steps (calcHI with a far-field table lookup), TEA matvec, MC every mstep. */
 double orc_ref_run(const orc_ref_run_t* p, double* rx, double* ry, double* rz, int nsteps, int nthreads,
-                   int* status) {
+                   int* status, double* t_hi) {
 #ifdef _OPENMP
   if (nthreads > 0) omp_set_num_threads(nthreads);
   double t0 = omp_get_wtime();
@@ -1323,7 +1330,15 @@ double orc_ref_run(const orc_ref_run_t* p, double* rx, double* ry, double* rz, i
     orc_lj_force(&lat, N, rx, ry, rz, nlist, list, stride, 25.0, p->epsilon, 0, fx, fy, fz, NULL);
     orc_coulomb_force(&lat, N, rx, ry, rz, q, p->lambda_b, p->lambda_d, fx, fy, fz);
     for (int i = 0; i < n3; ++i) R[i] = (double)orc_gasdev(&rng);
-    if (t % p->srpy == 0) orc_calchi_ref(&lat, N, rx, ry, rz, (uint64_t)t, p->nt_avg, &bins, table, box_volume, M, NULL, NULL);
+    if (t % p->srpy == 0) {
+#ifdef _OPENMP
+      double h0 = omp_get_wtime();
+#endif
+      orc_calchi_ref(&lat, N, rx, ry, rz, (uint64_t)t, p->nt_avg, &bins, table, box_volume, M, NULL, NULL);
+#ifdef _OPENMP
+      if (t_hi) *t_hi += omp_get_wtime() - h0;
+#endif
+    }
     double bt = p->beta;
 #pragma omp parallel for schedule(dynamic)
     for (int i = 0; i < n3; ++i) {

@@ -146,7 +146,7 @@ typedef struct {
   int64_t seed;
 } orc_ref_run_t;
 double orc_ref_run(const orc_ref_run_t* p, double* rx, double* ry, double* rz, int nsteps,
-                   int nthreads, int* status);
+                   int nthreads, int* status, double* t_hi);
 
 int orc_num_threads(void);
 

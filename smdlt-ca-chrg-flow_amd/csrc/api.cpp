@@ -211,13 +211,19 @@ static void bspline_moduli(int K, int p, std::vector<double>& out) {
 }
 
 // ---------------------------------------------------------------- context
-enum Stage { ST_REMAP, ST_NLIST, ST_FORCES, ST_NOISE, ST_MOB, ST_LANCZOS, ST_UPDATE, ST_MC, ST_COUNT };
-static const char* kStageNames[ST_COUNT] = {"remap", "neighbors", "forces", "noise",
-                                            "mobility_apply", "lanczos_other", "update", "mc"};
+enum Stage {
+  ST_REMAP, ST_NLIST, ST_FORCES, ST_NOISE, ST_MOB, ST_LANCZOS, ST_UPDATE, ST_MC,
+  ST_K_ZERO, ST_K_SPREAD, ST_K_FFTX, ST_K_FFTY, ST_K_FFTZ_GREEN, ST_K_IFFTY, ST_K_IFFTX, ST_K_FINAL, ST_COUNT
+};
+static const char* kStageNames[ST_COUNT] = {
+    "remap", "neighbors", "forces", "noise", "mobility_apply", "lanczos_other", "update", "mc",
+    "k_fill(grid)", "k_spread", "k_fft_x(fwd)", "k_fft_y(fwd)", "k_fft_z_green", "k_fft_y(inv)", "k_fft_x(inv)",
+    "k_mob_final"};
 
 struct Timer {
   hipEvent_t a, b;
   int stage;
+  bool own_a = true;  // return a to the pool after reading (shared events are returned once)
 };
 
 struct smdlt_ctx {
@@ -238,6 +244,7 @@ struct smdlt_ctx {
   bool have_pos = false;
   // lists
   int cap_rpy = 64, cap_lj = 64;
+  double nbr_est = 0;
   int *nrpy = nullptr, *lrpy = nullptr, *nlj = nullptr, *llj = nullptr, *overflow = nullptr;
   int *cell_of = nullptr, *cell_cnt = nullptr, *cell_start = nullptr, *cell_fill = nullptr, *cell_beads = nullptr;
   int ncells = 0;
@@ -304,7 +311,7 @@ struct StageScope {
     if (c->prof && b) {
       hipEvent_t e = get_event(c);
       hipEventRecord(e, c->stream);
-      c->timers.push_back({b, e, st});
+      c->timers.push_back({b, e, st, true});
     }
   }
 };
@@ -317,7 +324,7 @@ static void collect_timers(smdlt_ctx* c) {
     hipEventElapsedTime(&ms, t.a, t.b);
     c->stage_ms[t.stage] += ms;
     c->stage_cnt[t.stage] += 1;
-    c->pool.push_back(t.a);
+    if (t.own_a) c->pool.push_back(t.a);
     c->pool.push_back(t.b);
   }
   c->timers.clear();
@@ -449,6 +456,43 @@ int smdlt_num_beads(const smdlt_ctx* c, int replica) {
   return c->rh[replica].n;
 }
 
+// Mobility settings.  Error model (calibrated against the dense Ewald sum):
+// the real-space tail |t1|+|t2| at rc is held at 0.1 tol, and the B-spline
+// interpolation error of order p reaches tol = 1e-5 at grid spacing
+// h*alpha = 0.10 (p=4), 0.18 (p=6), 0.25 (p=8), scaling as tol^(1/p).
+// Among the (rc, p) candidates that fit the box, the cheapest by a cost model
+// of the M.v kernels (pairs, grid passes, spline points) is chosen.
+static double grid_h_alpha(int p, double tol) {
+  double c = (p == 4) ? 0.10 : (p == 6) ? 0.18 : 0.25;
+  return c * pow(tol / 1e-5, 1.0 / p);
+}
+
+struct MobChoice {
+  double rc, alpha;
+  int K[3], order;
+  double cost, nbr;
+};
+
+static MobChoice mob_candidate(double rc, int order, double tol, double skin, const double len[3], int n, double vol,
+                               int nb) {
+  MobChoice m;
+  m.rc = rc;
+  m.order = order;
+  m.alpha = choose_alpha(rc, 0.1 * tol);
+  double h = grid_h_alpha(order, tol) / m.alpha;
+  for (int d = 0; d < 3; ++d) m.K[d] = next_pow2((int)ceil(len[d] / h));
+  double rl = rc + skin;
+  double chain = std::min((double)nb, 2.0 * pow(rl / 2.0, 1.7));
+  m.nbr = (double)n / vol * (4.0 / 3.0) * M_PI * rl * rl * rl + chain;
+  double G = (double)m.K[0] * m.K[1] * m.K[2];
+  // relative costs per pair, per grid point (5 FFT passes + fill + interp) and per spline point
+  m.cost = 0.003 * n * m.nbr + 0.03 * G + 0.002 * n * 3.0 * order * order * order;
+  bool bad = false;
+  for (int d = 0; d < 3; ++d) bad = bad || m.K[d] > KMAX;
+  if (bad || m.nbr > 4000) m.cost = 1e300;
+  return m;
+}
+
 static int choose_mobility(smdlt_ctx* c, int r) {
   const smdlt_params& p = c->P[r];
   RepDev& R = c->rh[r];
@@ -467,33 +511,50 @@ static int choose_mobility(smdlt_ctx* c, int r) {
   }
   double tol = p.mobility_tol > 0 ? p.mobility_tol : 1e-5;
   double skin = p.skin > 0 ? p.skin : 1.0;
-  double rc = p.rc_real;
-  if (rc <= 0) rc = std::min(8.0, 0.45 * wmin - skin);
-  if (rc + skin >= 0.5 * wmin) return SMDLT_E_GEOMETRY;
-  double alpha = p.ewald_alpha > 0 ? p.ewald_alpha : choose_alpha(rc, 0.1 * tol);
-  int order = p.spline_order > 0 ? p.spline_order : 6;
-  if (order != 4 && order != 6 && order != 8) return SMDLT_E_ARG;
-  int K[3];
-  if (p.grid[0] > 0 && p.grid[1] > 0 && p.grid[2] > 0) {
-    for (int d = 0; d < 3; ++d) K[d] = p.grid[d];
-  } else {
-    double kmax = choose_kmax(alpha, 0.1 * tol);
-    // grid oversampling for the B-spline interpolation error of this order
-    double sigma = (order >= 8) ? 1.25 : (order >= 6) ? 1.45 : 1.9;
-    double len[3] = {a1max, a2max, h.L};
-    for (int d = 0; d < 3; ++d) K[d] = next_pow2((int)ceil(sigma * kmax * len[d] / M_PI));
+  double rcmax = 0.5 * wmin - skin - 1e-6;
+  double len[3] = {a1max, a2max, h.L};
+  int n = R.n;
+  double vol = p.L * p.L * p.L;
+  MobChoice best;
+  best.cost = 1e301;
+  const double rcs[] = {5, 6, 7, 8, 10, 12, 14, 16, 20, 24, 28, 32};
+  const int orders[] = {4, 6, 8};
+  for (double rc : rcs) {
+    if (p.rc_real > 0) rc = p.rc_real;
+    if (rc > rcmax) continue;
+    for (int o : orders) {
+      if (p.spline_order > 0 && o != p.spline_order) continue;
+      MobChoice m = mob_candidate(rc, o, tol, skin, len, n, vol, p.nb);
+      if (m.cost < best.cost) best = m;
+    }
+    if (p.rc_real > 0) break;
   }
+  if (best.cost >= 1e300) {
+    if (p.rc_real > 0 && p.rc_real > rcmax) return SMDLT_E_GEOMETRY;
+    if (rcmax < 2.0) return SMDLT_E_GEOMETRY;
+    best = mob_candidate(std::min(rcmax, 8.0), p.spline_order > 0 ? p.spline_order : 8, tol, skin, len, n, vol, p.nb);
+  }
+  if (p.ewald_alpha > 0) {
+    best.alpha = p.ewald_alpha;
+    double hh = grid_h_alpha(best.order, tol) / best.alpha;
+    for (int d = 0; d < 3; ++d) best.K[d] = next_pow2((int)ceil(len[d] / hh));
+  }
+  if (p.grid[0] > 0 && p.grid[1] > 0 && p.grid[2] > 0)
+    for (int d = 0; d < 3; ++d) best.K[d] = p.grid[d];
   for (int d = 0; d < 3; ++d)
-    if (K[d] < 16 || K[d] > KMAX || (K[d] & (K[d] - 1))) return SMDLT_E_ARG;
-  R.K[0] = K[0];
-  R.K[1] = K[1];
-  R.K[2] = K[2];
-  R.order = order;
-  R.alpha = alpha;
+    if (best.K[d] < 16 || best.K[d] > KMAX || (best.K[d] & (best.K[d] - 1))) return SMDLT_E_ARG;
+  if (best.order != 4 && best.order != 6 && best.order != 8) return SMDLT_E_ARG;
+  double rc = best.rc;
+  R.K[0] = best.K[0];
+  R.K[1] = best.K[1];
+  R.K[2] = best.K[2];
+  R.order = best.order;
+  R.alpha = best.alpha;
   R.rc2 = rc * rc;
   R.rlist2 = (rc + skin) * (rc + skin);
   R.skin = skin;
-  R.selfconst = ewald_selfconst(alpha);
+  R.selfconst = ewald_selfconst(best.alpha);
+  c->nbr_est = std::max(c->nbr_est, best.nbr);
   // cell capacity over the period
   double cut = std::max(rc + skin, 6.0 + 1.0);
   double wmax = 0;
@@ -570,6 +631,7 @@ int smdlt_create(const smdlt_params* params, int n_replicas, int device, smdlt_c
     c->lat[r].init(p.flow_type, p.L, R.edot, p.dt, p.pbc_compat);
     c->ct_prev[r] = c->lat[r].ct;
     c->st_prev[r] = c->lat[r].st;
+    if (r > 0 && c->P[r].spline_order == 0) c->P[r].spline_order = c->rh[0].order;  // one order per context
     int st = choose_mobility(c, r);
     if (st) {
       delete c;
@@ -589,6 +651,7 @@ int smdlt_create(const smdlt_params* params, int n_replicas, int device, smdlt_c
     return SMDLT_E_ARG;
   }
   c->maxit = maxit;
+  c->cap_rpy = std::max(64, (int)(1.5 * c->nbr_est) + 32);
   c->nst = slot;
   c->nwaves = wave;
   c->grid_total = g;
@@ -1154,10 +1217,18 @@ static void mobility_internal(smdlt_ctx* c, const double* v, double* u, const do
   m.nty = c->ty0[re] - c->ty0[rb];
   m.ntz = c->tz0[re] - c->tz0[rb];
   m.order = c->rh[rb].order;
+  hipEvent_t ev[9];
+  m.ev = nullptr;
+  if (c->prof) {
+    for (int k = 0; k < 9; ++k) ev[k] = get_event(c);
+    m.ev = ev;
+  }
   if (c->prec == SMDLT_PREC_FP64)
     launch_mobility<double>(c->stream, m);
   else
     launch_mobility<float>(c->stream, m);
+  if (m.ev)  // kernel k runs between ev[k] and ev[k+1]; each event is returned once
+    for (int k = 0; k < 8; ++k) c->timers.push_back({ev[k], ev[k + 1], ST_K_ZERO + k, k == 0});
 }
 
 static int ensure_lists(smdlt_ctx* c) {
@@ -1212,9 +1283,10 @@ static int brownian_internal(smdlt_ctx* c, const double* z, double* y, double to
     const double* vin = (j == 1) ? z : c->W;
     double* vj = c->V + (size_t)(j - 1) * n3;
     const double* vjm1 = (j > 1) ? c->V + (size_t)(j - 2) * n3 : nullptr;
-    mobility_internal(c, vin, c->W, c->bdiv, vj, c->partial, c->done);
+    mobility_internal(c, vin, c->W, c->bdiv, vj, nullptr, c->done);
     {
       StageScope sc(c, ST_LANCZOS);
+      lz_wave_dot(c->stream, c->d_reps, c->d_slot_rep, s0, ns, vj, c->W, c->done, c->partial);
       lz_rep_reduce(c->stream, c->d_reps, rb, re - rb, c->partial, c->done, c->scal, SCAL_STRIDE, j - 1, 0, nullptr);
       lz_orth(c->stream, c->d_reps, c->d_slot_rep, s0, ns, c->W, vj, vjm1, c->scal, j, c->done, c->partial);
       lz_tri(c->stream, c->d_reps, rb, re - rb, c->partial, c->scal, j, tol, maxit, c->done, c->bdiv, c->iters);
@@ -1283,7 +1355,9 @@ static void self_mobility(const RepDev& R, double out[3]) {
   double a2 = R.alpha * R.alpha;
   double kmax = choose_kmax(R.alpha, 1e-14);
   out[0] = out[1] = out[2] = R.selfconst;
-  int m1m = (int)ceil(kmax / fabs(R.kb[0]) * 1.5) + 1, m2m = (int)ceil(kmax / fabs(R.kb[2]) * 1.5) + 1;
+  // |m1| <= kmax/|b1x|; |m2 b2y| <= kmax + |m1 b1y|
+  int m1m = (int)ceil(kmax / fabs(R.kb[0])) + 1;
+  int m2m = (int)ceil((kmax + m1m * fabs(R.kb[1])) / fabs(R.kb[2])) + 1;
   int m3m = (int)ceil(kmax / R.kb[3]) + 1;
   for (int m1 = -m1m; m1 <= m1m; ++m1)
     for (int m2 = -m2m; m2 <= m2m; ++m2)
@@ -1340,6 +1414,17 @@ static int step_internal(smdlt_ctx* c, uint64_t t, double* noise_dev, bool host_
   if (noise_dev) launch_rotate_vec(c->stream, c->d_reps, c->d_slot_rep, s0, ns, noise_dev, 0);
   st = neighbors_internal(c, 0);
   if (st) return st;
+  if (host_exit) {  // grow the lists now rather than finish the step with truncated ones
+    HCK(hipStreamSynchronize(c->stream));
+    for (int attempt = 0; attempt < 8; ++attempt) {
+      int g = grow_lists(c);
+      if (g < 0) return g;
+      if (g == 0) break;
+      st = neighbors_internal(c, 3);
+      if (st) return st;
+      HCK(hipStreamSynchronize(c->stream));
+    }
+  }
   st = forces_internal(c);
   if (st) return st;
   const double* zz = noise_dev;
@@ -1367,7 +1452,8 @@ int smdlt_step(smdlt_ctx* c, uint64_t t, const double* noise) {
   if (!c) return SMDLT_E_ARG;
   if (!c->have_pos) return SMDLT_E_STATE;
   HCK(hipSetDevice(c->device));
-  int st = 0;
+  int st = ensure_lists(c);
+  if (st) return st;
   if (noise) {
     st = gather_aos(c, noise, c->tmp);
     if (st) return st;

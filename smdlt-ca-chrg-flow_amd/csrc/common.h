@@ -259,6 +259,7 @@ struct MobArgs {
   const FftTask *tx, *ty, *tz;
   int ntx, nty, ntz;
   int order;                // B-spline order (4, 6 or 8), one per context
+  hipEvent_t* ev;           // 9 events bracketing the 8 launches (profiling) or nullptr
 };
 template <typename T> void launch_mobility(hipStream_t s, const MobArgs& a);
 

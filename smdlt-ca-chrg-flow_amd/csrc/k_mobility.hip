@@ -106,9 +106,20 @@ __device__ __forceinline__ void frac_weights(const RepDev& R, double x, double y
 
 template <typename T> __device__ __forceinline__ void grid_add(T* p, T v) { unsafeAtomicAdd(p, v); }
 
+// select W[j] for a runtime j without dynamic register indexing
+template <typename T, int P> __device__ __forceinline__ T pick(const T* W, int j) {
+  T w = (T)0;
+#pragma unroll
+  for (int k = 0; k < P; ++k) w = (k == j) ? W[k] : w;
+  return w;
+}
+
+// One wavefront per bead: every lane computes the bead's weights, lane l < P^2
+// owns stencil row (j2, j3) = (l % P, l / P) and adds its P points x 3 components.
 template <typename T, int P>
-__global__ void k_spread(MobArgs a) {
-  int s = a.slot_begin + blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_spread(MobArgs a) {
+  const int lane = threadIdx.x & 63;
+  int s = a.slot_begin + blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (s >= a.slot_begin + a.nslots) return;
   int r = a.slot_rep[s];
   if (r < 0) return;
@@ -117,36 +128,31 @@ __global__ void k_spread(MobArgs a) {
   if (s - R.slot0 >= R.n) return;
   double sc = a.beta ? 1.0 / a.beta[r] : 1.0;
   double v0 = a.v[3 * s] * sc, v1 = a.v[3 * s + 1] * sc, v2 = a.v[3 * s + 2] * sc;
-  if (a.vout) {
+  if (a.vout && lane == 0) {
     a.vout[3 * s] = v0;
     a.vout[3 * s + 1] = v1;
     a.vout[3 * s + 2] = v2;
   }
+  if (lane >= P * P) return;
   int base[3];
   T W[3][P];
   frac_weights<T, P>(R, a.rx[s], a.ry[s], a.rz[s], base, W);
+  const int j2 = lane % P, j3 = lane / P;
   T* g = (T*)a.grid + R.grid0;
   const long long ncomp = (long long)R.K[0] * R.K[1] * R.K[2];
-  T tv0 = (T)v0, tv1 = (T)v1, tv2 = (T)v2;
   const int K0 = R.K[0], K1 = R.K[1], K2 = R.K[2];
+  int g3 = (base[2] - j3) & (K2 - 1);
+  int g2 = (base[1] - j2) & (K1 - 1);
+  T w23 = pick<T, P>(W[2], j3) * pick<T, P>(W[1], j2);
+  T tv0 = (T)v0 * w23, tv1 = (T)v1 * w23, tv2 = (T)v2 * w23;
+  T* row = g + ((long long)g3 * K1 + g2) * K0;
 #pragma unroll
-  for (int j3 = 0; j3 < P; ++j3) {
-    int g3 = (base[2] - j3) & (K2 - 1);
-#pragma unroll
-    for (int j2 = 0; j2 < P; ++j2) {
-      int g2 = (base[1] - j2) & (K1 - 1);
-      T w23 = W[2][j3] * W[1][j2];
-      long long row = ((long long)g3 * K1 + g2) * K0;
-#pragma unroll
-      for (int j1 = 0; j1 < P; ++j1) {
-        int g1 = (base[0] - j1) & (K0 - 1);
-        T w = w23 * W[0][j1];
-        T* q = g + row + g1;
-        grid_add(q, w * tv0);
-        grid_add(q + ncomp, w * tv1);
-        grid_add(q + 2 * ncomp, w * tv2);
-      }
-    }
+  for (int j1 = 0; j1 < P; ++j1) {
+    int g1 = (base[0] - j1) & (K0 - 1);
+    T w = W[0][j1];
+    grid_add(row + g1, w * tv0);
+    grid_add(row + g1 + ncomp, w * tv1);
+    grid_add(row + g1 + 2 * ncomp, w * tv2);
   }
 }
 
@@ -322,96 +328,93 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_z_green(MobArgs a) {
 }
 
 // -------------------------------------------------- real space + interpolation
+// One wavefront per bead.  Lanes split the interpolation stencil rows and the
+// real-space list; each lane keeps compensated FP32 (or FP64) sums, which are
+// combined across the wavefront in FP64.
 template <typename T, int P>
-__global__ void k_mob_final(MobArgs a) {
-  int s = a.slot_begin + blockIdx.x * blockDim.x + threadIdx.x;
-  bool in = s < a.slot_begin + a.nslots;
-  int r = in ? a.slot_rep[s] : -1;
-  double dot = 0.0;
-  bool act = false;
-  if (r >= 0) {
-    const RepDev& R = a.reps[r];
-    act = R.active && !(a.done && a.done[r]);
-    if (act && s - R.slot0 < R.n) {
-      const double* vv = a.vout ? a.vout : a.v;
-      double sc = (a.beta && !a.vout) ? 1.0 / a.beta[r] : 1.0;
-      double xi = a.rx[s], yi = a.ry[s], zi = a.rz[s];
-      CompSum<T> acc[3];
-      // reciprocal part: interpolate the three potential grids
-      int base[3];
-      T W[3][P];
-      frac_weights<T, P>(R, xi, yi, zi, base, W);
-      const T* g = (const T*)a.grid + R.grid0;
-      const long long ncomp = (long long)R.K[0] * R.K[1] * R.K[2];
-      const int K0 = R.K[0], K1 = R.K[1], K2 = R.K[2];
+__global__ __launch_bounds__(256) void k_mob_final(MobArgs a) {
+  const int lane = threadIdx.x & 63;
+  int s = a.slot_begin + blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (s >= a.slot_begin + a.nslots) return;
+  int r = a.slot_rep[s];
+  if (r < 0) return;
+  const RepDev& R = a.reps[r];
+  if (!R.active || (a.done && a.done[r])) return;
+  if (s - R.slot0 >= R.n) {
+    if (lane == 0) a.u[3 * s] = a.u[3 * s + 1] = a.u[3 * s + 2] = 0.0;
+    return;
+  }
+  const double* vv = a.vout ? a.vout : a.v;
+  const double sc = (a.beta && !a.vout) ? 1.0 / a.beta[r] : 1.0;
+  const double xi = a.rx[s], yi = a.ry[s], zi = a.rz[s];
+  CompSum<T> acc[3];
+  // reciprocal part: lane l < P^2 interpolates stencil row (l % P, l / P)
+  if (lane < P * P) {
+    int base[3];
+    T W[3][P];
+    frac_weights<T, P>(R, xi, yi, zi, base, W);
+    const int j2 = lane % P, j3 = lane / P;
+    const T* g = (const T*)a.grid + R.grid0;
+    const long long ncomp = (long long)R.K[0] * R.K[1] * R.K[2];
+    const int K0 = R.K[0], K1 = R.K[1], K2 = R.K[2];
+    int g3 = (base[2] - j3) & (K2 - 1);
+    int g2 = (base[1] - j2) & (K1 - 1);
+    T w23 = pick<T, P>(W[2], j3) * pick<T, P>(W[1], j2);
+    const T* row = g + ((long long)g3 * K1 + g2) * K0;
+    T s0 = 0, s1 = 0, s2 = 0;
 #pragma unroll
-      for (int j3 = 0; j3 < P; ++j3) {
-        int g3 = (base[2] - j3) & (K2 - 1);
-#pragma unroll
-        for (int j2 = 0; j2 < P; ++j2) {
-          int g2 = (base[1] - j2) & (K1 - 1);
-          T w23 = W[2][j3] * W[1][j2];
-          long long row = ((long long)g3 * K1 + g2) * K0;
-          T s0 = 0, s1 = 0, s2 = 0;
-#pragma unroll
-          for (int j1 = 0; j1 < P; ++j1) {
-            int g1 = (base[0] - j1) & (K0 - 1);
-            T w = W[0][j1];
-            const T* q = g + row + g1;
-            s0 += w * q[0];
-            s1 += w * q[ncomp];
-            s2 += w * q[2 * ncomp];
-          }
-          acc[0].add(w23 * s0);
-          acc[1].add(w23 * s1);
-          acc[2].add(w23 * s2);
-        }
-      }
-      // real-space part over the list
-      const T alpha = (T)R.alpha, rc2 = (T)R.rc2;
-      int m = a.nrpy[s];
-      for (int k = 0; k < m; ++k) {
-        int j = a.lrpy[(size_t)k * a.nst + s];
-        double dx = a.rx[j] - xi, dy = a.ry[j] - yi, dz = a.rz[j] - zi;
-        min_image_reduced(R, dx, dy, dz);
-        T tx = (T)dx, ty = (T)dy, tz = (T)dz;
-        T rr = tx * tx + ty * ty + tz * tz;
-        if (rr < rc2) {
-          T rd = sqrt(rr), t1, t2;
-          rpy_real<T>(rd, alpha, t1, t2);
-          T vx = (T)(vv[3 * j] * sc), vy = (T)(vv[3 * j + 1] * sc), vz = (T)(vv[3 * j + 2] * sc);
-          T dv = (tx * vx + ty * vy + tz * vz) * t2 / rr;
-          acc[0].add(t1 * vx + dv * tx);
-          acc[1].add(t1 * vy + dv * ty);
-          acc[2].add(t1 * vz + dv * tz);
-        }
-      }
-      double vi0 = vv[3 * s] * sc, vi1 = vv[3 * s + 1] * sc, vi2 = vv[3 * s + 2] * sc;
-      double u0 = acc[0].value() + R.selfconst * vi0;
-      double u1 = acc[1].value() + R.selfconst * vi1;
-      double u2 = acc[2].value() + R.selfconst * vi2;
-      a.u[3 * s] = u0;
-      a.u[3 * s + 1] = u1;
-      a.u[3 * s + 2] = u2;
-      dot = vi0 * u0 + vi1 * u1 + vi2 * u2;
-    } else if (act) {
-      a.u[3 * s] = a.u[3 * s + 1] = a.u[3 * s + 2] = 0.0;
+    for (int j1 = 0; j1 < P; ++j1) {
+      int g1 = (base[0] - j1) & (K0 - 1);
+      T w = W[0][j1];
+      s0 += w * row[g1];
+      s1 += w * row[g1 + ncomp];
+      s2 += w * row[g1 + 2 * ncomp];
+    }
+    acc[0].add(w23 * s0);
+    acc[1].add(w23 * s1);
+    acc[2].add(w23 * s2);
+  }
+  // real-space part over the list
+  const T alpha = (T)R.alpha, rc2 = (T)R.rc2;
+  const int m = a.nrpy[s];
+  for (int k = lane; k < m; k += 64) {
+    int j = a.lrpy[(size_t)k * a.nst + s];
+    double dx = a.rx[j] - xi, dy = a.ry[j] - yi, dz = a.rz[j] - zi;
+    min_image_reduced(R, dx, dy, dz);
+    T tx = (T)dx, ty = (T)dy, tz = (T)dz;
+    T rr = tx * tx + ty * ty + tz * tz;
+    if (rr < rc2) {
+      T rd = sqrt(rr), t1, t2;
+      rpy_real<T>(rd, alpha, t1, t2);
+      T vx = (T)(vv[3 * j] * sc), vy = (T)(vv[3 * j + 1] * sc), vz = (T)(vv[3 * j + 2] * sc);
+      T dv = (tx * vx + ty * vy + tz * vz) * t2 / rr;
+      acc[0].add(t1 * vx + dv * tx);
+      acc[1].add(t1 * vy + dv * ty);
+      acc[2].add(t1 * vz + dv * tz);
     }
   }
-  if (a.partial) {
-    dot = warp_sum(dot);
-    if ((threadIdx.x & 63) == 0 && in && r >= 0 && act) a.partial[s >> 6] = dot;
+  double u0 = warp_sum(acc[0].value()), u1 = warp_sum(acc[1].value()), u2 = warp_sum(acc[2].value());
+  if (lane == 0) {
+    double vi0 = vv[3 * s] * sc, vi1 = vv[3 * s + 1] * sc, vi2 = vv[3 * s + 2] * sc;
+    a.u[3 * s] = u0 + R.selfconst * vi0;
+    a.u[3 * s + 1] = u1 + R.selfconst * vi1;
+    a.u[3 * s + 2] = u2 + R.selfconst * vi2;
   }
 }
 
 template <typename T> void launch_mobility(hipStream_t st, const MobArgs& a) {
+  auto mark = [&](int k) {
+    if (a.ev) (void)hipEventRecord(a.ev[k], st);
+  };
+  mark(0);
   if (a.grid_len > 0) {
     if (sizeof(T) == 4)
       launch_fill_f32(st, (float*)a.grid + a.grid_begin, 0.0f, a.grid_len);
     else
       launch_fill_f64(st, (double*)a.grid + a.grid_begin, 0.0, a.grid_len);
   }
-  int nb = (a.nslots + 255) / 256;
+  mark(1);
+  int nb = (a.nslots + 3) / 4;  // one wavefront per bead, four per block
   if (a.nslots > 0) {
     if (a.order == 4)
       k_spread<T, 4><<<nb, 256, 0, st>>>(a);
@@ -420,11 +423,17 @@ template <typename T> void launch_mobility(hipStream_t st, const MobArgs& a) {
     else
       k_spread<T, 6><<<nb, 256, 0, st>>>(a);
   }
+  mark(2);
   if (a.ntx > 0) k_fft_x<T><<<a.ntx, FFT_THREADS, 0, st>>>(a, -1);
+  mark(3);
   if (a.nty > 0) k_fft_y<T><<<a.nty, FFT_THREADS, 0, st>>>(a, -1);
+  mark(4);
   if (a.ntz > 0) k_fft_z_green<T><<<a.ntz, FFT_THREADS, 0, st>>>(a);
+  mark(5);
   if (a.nty > 0) k_fft_y<T><<<a.nty, FFT_THREADS, 0, st>>>(a, 1);
+  mark(6);
   if (a.ntx > 0) k_fft_x<T><<<a.ntx, FFT_THREADS, 0, st>>>(a, 1);
+  mark(7);
   if (a.nslots > 0) {
     if (a.order == 4)
       k_mob_final<T, 4><<<nb, 256, 0, st>>>(a);
@@ -433,6 +442,7 @@ template <typename T> void launch_mobility(hipStream_t st, const MobArgs& a) {
     else
       k_mob_final<T, 6><<<nb, 256, 0, st>>>(a);
   }
+  mark(8);
 }
 
 template void launch_mobility<float>(hipStream_t, const MobArgs&);

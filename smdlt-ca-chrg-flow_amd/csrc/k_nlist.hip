@@ -113,34 +113,37 @@ __device__ __forceinline__ void sorted_insert(int* list, int stride, int i, int&
   ++cnt;
 }
 
-// rep_flags[r]: bit0 rebuild the Verlet (LJ) list, bit1 rebuild the mobility list.
-__global__ void k_build_lists(const RepDev* __restrict__ reps, const int* __restrict__ slot_rep, int slot_begin,
-                              int nslots, const double* rx, const double* ry, const double* rz,
-                              const int* cell_start, const int* cell_cnt, const int* cell_beads,
-                              const int* rep_flags, int nst, int cap_rpy, int* nrpy, int* lrpy, int cap_lj,
-                              int* nlj, int* llj, int* overflow) {
-  int i = slot_begin + blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void neighbour_cell_range(const RepDev& R, int lo[3], int hi[3]) {
+  for (int d = 0; d < 3; ++d) {
+    int n = R.ncell[d];
+    lo[d] = (n >= 3) ? -1 : 0;  // with fewer than 3 cells visit each distinct cell once
+    hi[d] = (n >= 3) ? 1 : n - 1;
+  }
+}
+
+// Mobility list (rep_flags bit1): one wavefront per bead, lanes test the
+// candidates of each neighbouring cell and append hits in wave order (ballot).
+__global__ __launch_bounds__(256) void k_build_rpy(const RepDev* __restrict__ reps, const int* __restrict__ slot_rep,
+                                                   int slot_begin, int nslots, const double* rx, const double* ry,
+                                                   const double* rz, const int* cell_start, const int* cell_cnt,
+                                                   const int* cell_beads, const int* rep_flags, int nst, int cap,
+                                                   int* nrpy, int* lrpy, int* overflow) {
+  const int lane = threadIdx.x & 63;
+  int i = slot_begin + blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (i >= slot_begin + nslots) return;
   int r = slot_rep[i];
   if (r < 0) return;
   const RepDev& R = reps[r];
-  int flags = rep_flags[r];
-  if (!R.active || !(flags & 3)) return;
+  if (!R.active || !(rep_flags[r] & 2)) return;
   if (i - R.slot0 >= R.n) {
-    if (flags & 2) nrpy[i] = 0;
-    if (flags & 1) nlj[i] = 0;
+    if (lane == 0) nrpy[i] = 0;
     return;
   }
-  double xi = rx[i], yi = ry[i], zi = rz[i];
-  int c[3];
+  const double xi = rx[i], yi = ry[i], zi = rz[i];
+  int c[3], lo[3], hi[3];
   cell_coords(R, xi, yi, zi, c);
-  int lo[3], hi[3];
-  for (int d = 0; d < 3; ++d) {
-    int n = R.ncell[d];
-    lo[d] = (n >= 3) ? -1 : 0;
-    hi[d] = (n >= 3) ? 1 : n - 1;
-  }
-  int cr = 0, cl = 0;
+  neighbour_cell_range(R, lo, hi);
+  int cr = 0;
   for (int o3 = lo[2]; o3 <= hi[2]; ++o3)
     for (int o2 = lo[1]; o2 <= hi[1]; ++o2)
       for (int o1 = lo[0]; o1 <= hi[0]; ++o1) {
@@ -149,29 +152,82 @@ __global__ void k_build_lists(const RepDev* __restrict__ reps, const int* __rest
         int e = (c[2] + o3 + R.ncell[2]) % R.ncell[2];
         int id = R.cell0 + (e * R.ncell[1] + b) * R.ncell[0] + a;
         int beg = cell_start[id], cnt = cell_cnt[id];
-        for (int k = 0; k < cnt; ++k) {
-          int j = cell_beads[beg + k];
-          if (j == i) continue;
-          double xj = rx[j], yj = ry[j], zj = rz[j];
-          if (flags & 2) {
-            double dx = xj - xi, dy = yj - yi, dz = zj - zi;
-            min_image_reduced(R, dx, dy, dz);
-            if (dx * dx + dy * dy + dz * dz < R.rlist2) sorted_insert(lrpy, nst, i, cr, cap_rpy, j, overflow);
-          }
-          if (flags & 1) {
-            double ox, oy, oz;
-            ref_nid(R, xi, yi, zi, xj, yj, zj, ox, oy, oz);
-            double rr;
-            {
-#pragma clang fp contract(off)
-              rr = sqrt(ox * ox + oy * oy + oz * oz);
+        for (int k0 = 0; k0 < cnt; k0 += 64) {
+          int k = k0 + lane;
+          bool hit = false;
+          int j = -1;
+          if (k < cnt) {
+            j = cell_beads[beg + k];
+            if (j != i) {
+              double dx = rx[j] - xi, dy = ry[j] - yi, dz = rz[j] - zi;
+              min_image_reduced(R, dx, dy, dz);
+              hit = dx * dx + dy * dy + dz * dz < R.rlist2;
             }
-            if (rr < R.rv) sorted_insert(llj, nst, i, cl, cap_lj, j, overflow);
           }
+          unsigned long long m = __ballot(hit);
+          int pos = cr + __popcll(m & ((1ull << lane) - 1ull));
+          if (hit) {
+            if (pos < cap)
+              lrpy[(size_t)pos * nst + i] = j;
+            else
+              atomicOr(overflow, 1);
+          }
+          cr += __popcll(m);
         }
       }
-  if (flags & 2) nrpy[i] = cr;
-  if (flags & 1) nlj[i] = cl;
+  if (lane == 0) nrpy[i] = min(cr, cap);
+}
+
+// The reference's Verlet list (rep_flags bit0): all j with |getNID(i,j)| < rv,
+// sorted.  Candidates come from the fresh-enough mobility list when its cutoff
+// covers rv + 1 (the pair distance under the delx shift differs from the true
+// one by less than 1), otherwise from the cell grid.
+__global__ void k_build_lj(const RepDev* __restrict__ reps, const int* __restrict__ slot_rep, int slot_begin,
+                           int nslots, const double* rx, const double* ry, const double* rz, const int* cell_start,
+                           const int* cell_cnt, const int* cell_beads, const int* rep_flags, int nst,
+                           const int* nrpy, const int* lrpy, int cap_lj, int* nlj, int* llj, int* overflow) {
+  int i = slot_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= slot_begin + nslots) return;
+  int r = slot_rep[i];
+  if (r < 0) return;
+  const RepDev& R = reps[r];
+  if (!R.active || !(rep_flags[r] & 1)) return;
+  if (i - R.slot0 >= R.n) {
+    nlj[i] = 0;
+    return;
+  }
+  const double xi = rx[i], yi = ry[i], zi = rz[i];
+  int cl = 0;
+  auto test = [&](int j) {
+    if (j == i) return;
+    double ox, oy, oz;
+    ref_nid(R, xi, yi, zi, rx[j], ry[j], rz[j], ox, oy, oz);
+    double rr;
+    {
+#pragma clang fp contract(off)
+      rr = sqrt(ox * ox + oy * oy + oz * oz);
+    }
+    if (rr < R.rv) sorted_insert(llj, nst, i, cl, cap_lj, j, overflow);
+  };
+  if (R.rc2 >= (R.rv + 1.0) * (R.rv + 1.0)) {
+    int m = nrpy[i];
+    for (int k = 0; k < m; ++k) test(lrpy[(size_t)k * nst + i]);
+  } else {
+    int c[3], lo[3], hi[3];
+    cell_coords(R, xi, yi, zi, c);
+    neighbour_cell_range(R, lo, hi);
+    for (int o3 = lo[2]; o3 <= hi[2]; ++o3)
+      for (int o2 = lo[1]; o2 <= hi[1]; ++o2)
+        for (int o1 = lo[0]; o1 <= hi[0]; ++o1) {
+          int a = (c[0] + o1 + R.ncell[0]) % R.ncell[0];
+          int b = (c[1] + o2 + R.ncell[1]) % R.ncell[1];
+          int e = (c[2] + o3 + R.ncell[2]) % R.ncell[2];
+          int id = R.cell0 + (e * R.ncell[1] + b) * R.ncell[0] + a;
+          int beg = cell_start[id], cnt = cell_cnt[id];
+          for (int k = 0; k < cnt; ++k) test(cell_beads[beg + k]);
+        }
+  }
+  nlj[i] = cl;
 }
 
 void launch_build_lists(hipStream_t st, const RepDev* reps, const int* slot_rep, int slot_begin, int nslots,
@@ -179,9 +235,11 @@ void launch_build_lists(hipStream_t st, const RepDev* reps, const int* slot_rep,
                         const int* cell_cnt, const int* cell_beads, const int* rep_flags, int nslots_total,
                         int cap_rpy, int* nrpy, int* lrpy, int cap_lj, int* nlj, int* llj, int* overflow) {
   if (nslots <= 0) return;
-  k_build_lists<<<(nslots + 127) / 128, 128, 0, st>>>(reps, slot_rep, slot_begin, nslots, rx, ry, rz, cell_start,
-                                                      cell_cnt, cell_beads, rep_flags, nslots_total, cap_rpy, nrpy,
-                                                      lrpy, cap_lj, nlj, llj, overflow);
+  k_build_rpy<<<(nslots + 3) / 4, 256, 0, st>>>(reps, slot_rep, slot_begin, nslots, rx, ry, rz, cell_start, cell_cnt,
+                                                cell_beads, rep_flags, nslots_total, cap_rpy, nrpy, lrpy, overflow);
+  k_build_lj<<<(nslots + 127) / 128, 128, 0, st>>>(reps, slot_rep, slot_begin, nslots, rx, ry, rz, cell_start,
+                                                   cell_cnt, cell_beads, rep_flags, nslots_total, nrpy, lrpy, cap_lj,
+                                                   nlj, llj, overflow);
 }
 
 // Fill kernels (used instead of memset so that captured graphs replay correctly).

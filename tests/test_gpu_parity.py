@@ -1,0 +1,373 @@
+"""GPU parity tests: the HIP library through its C ABI against the CPU oracle.
+Bit-exact for integer and index results (remap, min image, bins, lists, MC
+sequence); tolerances stated per test for floating point."""
+import numpy as np
+import pytest
+from helpers import O, S, random_walk
+
+import spme_ref as P
+
+pytestmark = pytest.mark.gpu
+
+
+def xyz(pos):
+    return [np.ascontiguousarray(pos[:, k]) for k in range(3)]
+
+
+def ctx_for(flow, L, nb, nc, prec=S.PREC_MIXED, compat=1, edot=0.3, dt=1e-3, **kw):
+    p = S.default_params(nb=nb, nc=nc, L=L, flow_type=flow, edot=edot, dt=dt, precision=prec,
+                         pbc_compat=compat, **kw)
+    return S.Context(p), p
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def oracle_remap(lat, t, rx, ry, rz):
+    """The reference's per-step remap sequence (X/main14.c:69-78, S/main26.c:71-73)."""
+    if lat.flow == 2:
+        rx, ry = O.rotate(lat, rx, ry, 0)
+        rx, ry = O.rotate(lat, rx, ry, 1)
+    O.lattice_update(lat, t)
+    if lat.flow == 2:
+        rx, ry = O.rotate(lat, rx, ry, 0)
+    return O.apply_pbc(lat, rx, ry, rz)
+
+
+# ----------------------------------------------------------------- remap / images / lists
+@pytest.mark.parametrize("flow,compat", [(1, 1), (1, 0), (2, 1), (0, 1)])
+def test_remap_bit_exact(flow, compat):
+    L = 26.0
+    ctx, p = ctx_for(flow, L, 30, 5, compat=compat, edot=0.4)
+    lat = O.lattice(flow, L, 0.4 if flow else 0.0, 1e-3, compat)
+    pos = random_walk(30, 5, L, 3) * 1.7  # some beads well outside the box
+    rx, ry, rz = xyz(pos)
+    ctx.upload_positions(rx, ry, rz)
+    rng = np.random.default_rng(0)
+    for t in (0, 1, 2, 700, 701, lat.tp - 1 if lat.tp < 10**8 else 3, lat.tp if lat.tp < 10**8 else 4):
+        ctx.remap(t)
+        rx, ry, rz = oracle_remap(lat, t, rx, ry, rz)
+        gx, gy, gz = ctx.download_positions()
+        assert np.array_equal(gx, rx) and np.array_equal(gy, ry) and np.array_equal(gz, rz), t
+        # move beads (as a step would) and continue from identical state
+        d = rng.normal(scale=0.7, size=(len(rx), 3))
+        rx, ry, rz = rx + d[:, 0], ry + d[:, 1], rz + d[:, 2]
+        ctx.upload_positions(rx, ry, rz)
+        if flow == 2:  # positions return to the lab frame between steps
+            rx, ry = O.rotate(lat, rx, ry, 1)
+            ctx.rotate(1)
+            gx, gy, _ = ctx.download_positions()
+            assert np.array_equal(gx, rx) and np.array_equal(gy, ry)
+
+
+@pytest.mark.parametrize("flow,compat", [(1, 1), (2, 1), (1, 0)])
+def test_min_image_and_bins_bit_exact(flow, compat):
+    L = 30.0
+    ctx, p = ctx_for(flow, L, 40, 5, compat=compat, edot=0.2)
+    lat = O.lattice(flow, L, 0.2, 1e-3, compat)
+    binsz = O.bins(lat, 2.0)
+    rx, ry, rz = xyz(random_walk(40, 5, L, 4))
+    ctx.upload_positions(rx, ry, rz)
+    t = lat.tp // 2 + 37  # second half of a period: nmax = 2 in getNIDbin
+    for tt in (5, t):
+        ctx.remap(tt)
+        rx, ry, rz = oracle_remap(lat, tt, rx, ry, rz)
+        rng = np.random.default_rng(tt)
+        i = rng.integers(0, 200, 500).astype(np.int32)
+        j = rng.integers(0, 200, 500).astype(np.int32)
+        g = ctx.min_image(0, i, j)
+        o = np.array([O.nid(lat, rx, ry, rz, a, b) for a, b in zip(i, j)])
+        assert np.array_equal(g, o)
+        gb = ctx.bin_index(0, 2.0, i, j)
+        ob = np.array([O.binij(binsz, lat, rx, ry, rz, a, b) for a, b in zip(i, j)])
+        assert np.array_equal(gb, ob)
+
+
+@pytest.mark.parametrize("flow,compat,nb,nc,L", [(1, 1, 40, 6, 24.0), (1, 0, 50, 4, 20.0), (2, 1, 40, 6, 24.0),
+                                                 (1, 1, 100, 1, 60.0)])
+def test_verlet_list_bit_exact(flow, compat, nb, nc, L):
+    ctx, p = ctx_for(flow, L, nb, nc, compat=compat, edot=0.3)
+    lat = O.lattice(flow, L, 0.3, 1e-3, compat)
+    rx, ry, rz = xyz(random_walk(nb, nc, L, 5))
+    ctx.upload_positions(rx, ry, rz)
+    for t in (0, 333):
+        ctx.remap(t)
+        rx, ry, rz = oracle_remap(lat, t, rx, ry, rz)
+        ctx.neighbors(True)
+        nl, lst = ctx.verlet(0)
+        onl, olst = O.verlet(lat, rx, ry, rz, 6.0)
+        assert np.array_equal(nl, onl)
+        for a in range(len(nl)):
+            assert np.array_equal(lst[a, :nl[a]], olst[a, :onl[a]])
+
+
+def test_verlet_trigger_follows_reference():
+    # rebuild only when some accumulated displacement exceeds (rv - rc)/2 = 0.5
+    L = 30.0
+    ctx, p = ctx_for(1, L, 30, 3, edot=0.0)
+    rx, ry, rz = xyz(random_walk(30, 3, L, 6))
+    ctx.upload_positions(rx, ry, rz)
+    assert ctx.neighbors(False) == 1  # first call builds
+    assert ctx.neighbors(False) == 0
+
+
+# ----------------------------------------------------------------- forces
+@pytest.mark.parametrize("flow,fene", [(1, 0), (2, 0), (1, 1)])
+def test_forces_match_oracle(flow, fene):
+    L = 28.0
+    nb, nc = 40, 5
+    kw = dict(spring_fene=fene, kappas=30.0 if fene else 200.0, qmax=3.0 if fene else 2.0, kappab=0.7,
+              lj_ext_form=1 if flow == 2 else 0)
+    ctx, p = ctx_for(flow, L, nb, nc, edot=0.2, **kw)
+    lat = O.lattice(flow, L, 0.2, 1e-3, 1)
+    pos = random_walk(nb, nc, L, 7, bond=2.0 if not fene else 2.4)
+    rx, ry, rz = xyz(pos)
+    ctx.upload_positions(rx, ry, rz)
+    rng = np.random.default_rng(1)
+    track = rng.choice(nb * nc, 60, replace=False).astype(np.int32)
+    sign = np.where(np.arange(60) % 2 == 0, 1, -1).astype(np.int32)
+    ctx.upload_charges([60], track, sign)
+    ctx.remap(50)
+    rx, ry, rz = oracle_remap(lat, 50, rx, ry, rz)
+    ctx.neighbors(True)
+    f = ctx.forces()
+    st, fb, _ = O.bond_force(lat, nb, nc, rx, ry, rz, fene, kw["kappas"], kw["qmax"], kw["kappab"])
+    assert st == 0
+    nl, lst = O.verlet(lat, rx, ry, rz, 6.0)
+    fl, _ = O.lj_force(lat, rx, ry, rz, nl, lst, 5.03968 if flow == 2 else 25.0, 0.1, flow == 2)
+    q = np.zeros(nb * nc, np.int32)
+    q[track] = sign
+    fc = O.coulomb_force(lat, rx, ry, rz, q, 3.59, 5.0)
+    ref = fb + fl + fc
+    assert np.abs(f - ref).max() <= 1e-10 * np.abs(ref).max()
+
+
+def test_fene_overstretch_is_reported():
+    L = 30.0
+    ctx, p = ctx_for(1, L, 3, 1, spring_fene=1, kappas=30.0, qmax=2.2)
+    ctx.upload_positions(np.array([0.0, 2.0, 6.0]), np.zeros(3), np.zeros(3))
+    with pytest.raises(S.SmdltError) as e:
+        ctx.forces()
+    assert e.value.code == -3
+
+
+# ----------------------------------------------------------------- mobility
+def mob_case(flow=1, L=24.0, nb=40, nc=3, seed=1, t=300, prec=S.PREC_MIXED, **kw):
+    ctx, p = ctx_for(flow, L, nb, nc, prec=prec, edot=0.5, **kw)
+    lat = O.lattice(flow, L, 0.5, 1e-3, 1, t)
+    pos = random_walk(nb, nc, L, seed)
+    rx, ry, rz = xyz(pos)
+    if flow == 2:  # the mobility works in the wrap frame
+        rx, ry = O.rotate(lat, rx, ry, 0)
+    ctx.upload_positions(rx, ry, rz)
+    ctx.update_lattice(t)
+    return ctx, lat, np.stack([rx, ry, rz], 1)
+
+
+@pytest.mark.parametrize("flow", [1, 2])
+def test_mobility_fp64_equals_pme_restatement(flow):
+    ctx, lat, pos = mob_case(flow, prec=S.PREC_FP64)
+    cfg = ctx.mobility_config()
+    v = np.random.default_rng(2).normal(size=pos.size)
+    u = ctx.mobility_apply(v)
+    un = P.mobility_apply(pos, v, (lat.L1xp, lat.L2xp, lat.L2yp, lat.L), cfg["alpha"], cfg["rc"], cfg["grid"],
+                          cfg["order"])
+    assert rel(u, un) < 1e-12
+
+
+@pytest.mark.parametrize("flow,prec", [(1, S.PREC_MIXED), (1, S.PREC_FP64), (2, S.PREC_MIXED), (0, S.PREC_MIXED)])
+def test_mobility_matches_dense_ewald(flow, prec):
+    # M.F against the exact dense FP64 Ewald sum: <= 1e-5 relative
+    ctx, lat, pos = mob_case(flow, prec=prec)
+    Md = O.ewald_dense(O.cell_of(lat), pos, 0.5)
+    for seed in (2, 3):
+        v = np.random.default_rng(seed).normal(size=pos.size)
+        assert rel(ctx.mobility_apply(v), Md @ v) < 1e-5
+
+
+def test_mobility_split_invariance():
+    vals = []
+    v = np.random.default_rng(5).normal(size=3 * 120)
+    for rc in (6.0, 9.0):
+        ctx, lat, pos = mob_case(1, rc_real=rc)
+        vals.append(ctx.mobility_apply(v))
+    assert rel(vals[0], vals[1]) < 2e-5
+
+
+def test_mobility_symmetric():
+    ctx, lat, pos = mob_case(1)
+    rng = np.random.default_rng(6)
+    v, w = rng.normal(size=pos.size), rng.normal(size=pos.size)
+    a, b = w @ ctx.mobility_apply(v), v @ ctx.mobility_apply(w)
+    assert abs(a - b) <= 1e-6 * np.linalg.norm(v) * np.linalg.norm(w)
+
+
+def test_single_bead_hasimoto():
+    L = 20.0
+    ctx, p = ctx_for(0, L, 1, 1, prec=S.PREC_FP64)
+    ctx.upload_positions(np.array([0.3]), np.array([-1.2]), np.array([2.7]))
+    for k in range(3):
+        e = np.zeros(3)
+        e[k] = 1
+        u = ctx.mobility_apply(e)
+        assert u[k] == pytest.approx(1 - 2.8372974795 / L + 4.1887902048 / L ** 3, rel=1e-5)
+
+
+# ----------------------------------------------------------------- Brownian
+@pytest.mark.parametrize("prec", [S.PREC_MIXED, S.PREC_FP64])
+def test_lanczos_matches_dense_sqrt(prec):
+    ctx, lat, pos = mob_case(1, prec=prec)
+    Md = O.ewald_dense(O.cell_of(lat), pos, 0.5)
+    z = np.random.default_rng(3).normal(size=pos.size)
+    y, it, res, code = ctx.brownian(z, 1e-6, 30)
+    assert code == 0 and 2 <= it[0] <= 30
+    ys, st, lo, hi = O.sym_sqrt_apply(Md, z)
+    assert rel(y, ys) < 2e-5
+    assert y @ y == pytest.approx(z @ Md @ z, rel=2e-5)
+
+
+def test_tea_matches_oracle():
+    ctx, lat, pos = mob_case(1)
+    Md = O.ewald_dense(O.cell_of(lat), pos, 0.5)
+    rng = np.random.default_rng(4)
+    z = rng.normal(size=pos.size)
+    Cv = rng.uniform(0.6, 1.0, pos.size)
+    y = ctx.tea_apply(0.37, Cv, z)
+    assert rel(y, O.tea_apply(Md, 0.37, Cv, z)) < 1e-5
+
+
+# ----------------------------------------------------------------- timestep
+@pytest.mark.parametrize("flow", [1, 2])
+def test_step_matches_dense_update(flow):
+    L, nb, nc, dt, edot = 24.0, 30, 3, 1e-3, 0.5
+    ctx, p = ctx_for(flow, L, nb, nc, prec=S.PREC_FP64, edot=edot, dt=dt, kappas=200.0)
+    lat = O.lattice(flow, L, edot, dt, 1)
+    rx, ry, rz = xyz(random_walk(nb, nc, L, 9))
+    ctx.upload_positions(rx, ry, rz)
+    z = np.random.default_rng(8).normal(size=3 * nb * nc)
+    t = 10
+    ctx.step(t, z)
+    gx, gy, gz = ctx.download_positions()
+    # oracle: remap, forces, dense M, M F, M^(1/2) z (Jacobi), flow, back to lab frame
+    wx, wy, wz = oracle_remap(lat, t, rx, ry, rz)
+    nl, lst = O.verlet(lat, wx, wy, wz, 6.0)
+    st, fb, _ = O.bond_force(lat, nb, nc, wx, wy, wz, 0, 200.0, 2.0)
+    fl, _ = O.lj_force(lat, wx, wy, wz, nl, lst, 25.0, 0.1, 0)
+    F = fb + fl  # lab frame
+    pos = np.stack([wx, wy, wz], 1)
+    M = O.ewald_dense(O.cell_of(lat), pos, 0.5)
+    if flow == 2:
+        c, s = lat.ct, lat.st
+        Rot = np.array([[c, s, 0], [-s, c, 0], [0, 0, 1]])  # lab -> wrap
+        Fw = F @ Rot.T
+        zw = (z.reshape(-1, 3) @ Rot.T).reshape(-1)
+    else:
+        Fw, zw = F, z
+    ys, _, _, _ = O.sym_sqrt_apply(M, zw)
+    d = (dt * (M @ Fw.reshape(-1)) + np.sqrt(2 * dt) * ys).reshape(-1, 3)
+    if flow == 2:
+        RotI = Rot.T
+        lab = pos @ RotI.T
+        d = d @ RotI.T
+        ex = lab[:, 0] + d[:, 0] + dt * edot * lab[:, 0]
+        ey = lab[:, 1] + d[:, 1] - dt * edot * lab[:, 1]
+        ez = lab[:, 2] + d[:, 2]
+    else:
+        ex = wx + d[:, 0] + dt * edot * wy
+        ey = wy + d[:, 1]
+        ez = wz + d[:, 2]
+    disp_ref = np.stack([ex - (lab[:, 0] if flow == 2 else wx), ey - (lab[:, 1] if flow == 2 else wy),
+                         ez - wz], 1)
+    disp_gpu = np.stack([gx - (lab[:, 0] if flow == 2 else wx), gy - (lab[:, 1] if flow == 2 else wy),
+                         gz - wz], 1)
+    assert rel(disp_gpu, disp_ref) < 2e-5
+
+
+def test_graph_replay_matches_eager():
+    L, nb, nc = 30.0, 50, 4
+    out = []
+    for graph in (False, True):
+        ctx, p = ctx_for(1, L, nb, nc, prec=S.PREC_FP64, edot=0.3, dt=5e-4)
+        rx, ry, rz = xyz(random_walk(nb, nc, L, 10))
+        ctx.upload_positions(rx, ry, rz)
+        ctx.run(0, 6, use_graph=graph)
+        out.append(np.stack(ctx.download_positions(), 1))
+        ctx.close()
+    assert np.abs(out[0] - out[1]).max() < 1e-9
+
+
+def test_batched_replicas_equal_single_contexts():
+    specs = [(1, 26.0, 30, 2), (2, 30.0, 50, 3), (1, 22.0, 17, 5)]
+    params = []
+    poss = []
+    for k, (flow, L, nb, nc) in enumerate(specs):
+        params.append(S.default_params(nb=nb, nc=nc, L=L, flow_type=flow, edot=0.4, dt=1e-3,
+                                       precision=S.PREC_FP64, rc_real=6.0, grid=(32, 64, 32)))
+        poss.append(random_walk(nb, nc, L, 20 + k))
+    big = S.Context(params)
+    allp = np.concatenate(poss)
+    big.upload_positions(*xyz(allp))
+    big.remap(120)
+    v = np.random.default_rng(0).normal(size=3 * len(allp))
+    ub = big.mobility_apply(v)
+    off = 0
+    for k, prm in enumerate(params):
+        one = S.Context(prm)
+        one.upload_positions(*xyz(poss[k]))
+        one.remap(120)
+        n = prm.nb * prm.nc
+        u1 = one.mobility_apply(v[3 * off:3 * (off + n)])
+        assert rel(ub[3 * off:3 * (off + n)], u1) < 1e-12
+        off += n
+    # the per-replica (plain) path on the same context gives the same numbers
+    big.select_replicas(1, 2)
+    u_plain = big.mobility_apply(v)
+    n0 = params[0].nb * params[0].nc
+    n1 = params[1].nb * params[1].nc
+    assert rel(u_plain[3 * n0:3 * (n0 + n1)], ub[3 * n0:3 * (n0 + n1)]) < 1e-12
+
+
+# ----------------------------------------------------------------- MC
+@pytest.mark.parametrize("flow,nq,b1,b2", [(1, 24, 1.0, 1.5), (2, 40, 0.5, 0.2), (1, 60, 3.0, 3.0)])
+def test_mc_reference_sequence_bit_exact(flow, nq, b1, b2):
+    L, nb, nc = 22.0, 30, 4
+    ctx, p = ctx_for(flow, L, nb, nc, edot=0.3, lambda_d=5.0, lambda_b=3.59, barrier=b1, barrier2=b2)
+    lat = O.lattice(flow, L, 0.3, 1e-3, 1)
+    rx, ry, rz = xyz(random_walk(nb, nc, L, 12))
+    ctx.upload_positions(rx, ry, rz)
+    ctx.remap(40)
+    rx, ry, rz = oracle_remap(lat, 40, rx, ry, rz)
+    track = np.random.default_rng(3).choice(nb * nc, nq, replace=False).astype(np.int32)
+    ctx.upload_charges([nq], track)
+    mc = O.ChargeHop(nb, nc, track)
+    rng = O.Ran1(830062920)
+    for sweep in range(4):
+        u = rng.uniforms(3 * nq)
+        cons, log = ctx.mc_sweep(S.MC_REFERENCE, u[None, :])
+        st, cur, olog, e0 = mc.sweep(lat, rx, ry, rz, 3.59, 5.0, b1, b2, u)
+        tr, old = ctx.download_charges()
+        assert cons[0] == cur
+        assert np.array_equal(tr, mc.track)
+        # per-trial records: from, to (skips: -1), kind, accepted, draws
+        assert np.array_equal(log[:, 1], olog[:, 1])
+        assert np.array_equal(log[:, 4], olog[:, 4]) and np.array_equal(log[:, 5], olog[:, 5])
+        keep = olog[:, 3] >= 0
+        assert np.array_equal(log[keep, 2], olog[keep, 2]) and np.array_equal(log[keep, 3], olog[keep, 3])
+
+
+def test_mc_energy_matches_oracle():
+    L, nb, nc = 25.0, 40, 3
+    ctx, p = ctx_for(2, L, nb, nc, lambda_d=3.0)
+    lat = O.lattice(2, L, 0.3, 1e-3, 1)
+    rx, ry, rz = xyz(random_walk(nb, nc, L, 13))
+    ctx.upload_positions(rx, ry, rz)
+    ctx.remap(9)
+    rx, ry, rz = oracle_remap(lat, 9, rx, ry, rz)
+    track = np.arange(0, 120, 3, dtype=np.int32)
+    sign = np.where(np.arange(len(track)) % 3 == 0, -1, 1).astype(np.int32)
+    ctx.upload_charges([len(track)], track, sign)
+    q = np.zeros(120, np.int32)
+    q[track] = sign
+    e = ctx.mc_energy()[0]
+    assert e == pytest.approx(O.coulomb_energy(lat, rx, ry, rz, q, 3.59, 3.0), rel=1e-12)
