@@ -30,24 +30,32 @@ import smdlt_abi as S  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 
 
-def chains(nb, nc, L, seed, bond=2.0):
-    """Seeded random-walk chains (bond 2a, no immediate back-steps), wrapped."""
+def chains(nb, nc, L, seed, bond=2.05, min_sep=2.0, tries=1000):
+    """Seeded self-avoiding random walks like initChains (S/main26.c:420-470): bond
+    2.05a, a new bead is redrawn while it lies within min_sep of any earlier bead
+    (minimum image in the cubic box); positions wrapped into the box."""
     rng = np.random.default_rng(seed)
     pos = np.zeros((nb * nc, 3))
+    k = 0
     for c in range(nc):
-        p = rng.uniform(-L / 2, L / 2, 3)
-        prev = None
         for j in range(nb):
-            if j:
-                while True:
-                    d = rng.normal(size=3)
-                    d *= bond / np.linalg.norm(d)
-                    if prev is None or np.dot(d, prev) > -0.5 * bond * bond:
-                        break
-                p = p + d
-                prev = d
-            pos[c * nb + j] = p
-    return pos - L * np.round(pos / L)
+            for _ in range(tries):
+                if j == 0:
+                    p = rng.uniform(-L / 2, L / 2, 3)
+                else:
+                    th = rng.uniform(0, 2 * np.pi)
+                    ph = np.arccos(rng.uniform(-1, 1))
+                    p = pos[k - 1] + bond * np.array([np.sin(ph) * np.cos(th), np.sin(ph) * np.sin(th), np.cos(ph)])
+                p = p - L * np.round(p / L)
+                if k == 0:
+                    break
+                d = pos[:k] - p
+                d -= L * np.round(d / L)
+                if np.min(np.einsum("ij,ij->i", d, d)) >= min_sep * min_sep:
+                    break
+            pos[k] = p
+            k += 1
+    return pos
 
 
 def c2_params(prec):
@@ -65,11 +73,12 @@ def c2_charges(n, seed=7):
 
 # Algorithmic HBM bytes per launch of each mobility kernel (one M.v):
 # bead arrays are read/written once, grids once per pass (FP32 mixed path).
-def kernel_bytes(n, G, nbr, esz=4):
+def kernel_bytes(n, G, nbr, order=8, esz=4):
     g = 3 * G * esz
+    BW = 3 * order * esz
     return {
-        "k_fill(grid)": g,
-        "k_spread": n * 48 + g,
+        "k_bead_prep": n * (24 + 24 + 12 + BW) ,
+        "k_spread_gather": n * (12 + 24 + BW) + g,
         "k_fft_x(fwd)": 2 * g, "k_fft_y(fwd)": 2 * g, "k_fft_z_green": 2 * g,
         "k_fft_y(inv)": 2 * g, "k_fft_x(inv)": 2 * g,
         "k_mob_final": n * (24 + 24 + 24) + 4 * nbr + g,
@@ -116,7 +125,7 @@ def main():
     ctx.set_profiling(False)
     G = cfg["grid"][0] * cfg["grid"][1] * cfg["grid"][2]
     nbr = n * 120  # mobility-list entries (estimated from the density within rc+skin)
-    kb = kernel_bytes(n, G, nbr)
+    kb = kernel_bytes(n, G, nbr, cfg["order"])
     kern = {k: v for k, v in st.items() if k.startswith("k_") and v[1] > 0}
     dom = max(kern, key=lambda k: kern[k][0])
     avg_ms = kern[dom][0] / kern[dom][1]

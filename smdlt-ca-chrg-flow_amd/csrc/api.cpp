@@ -217,8 +217,8 @@ enum Stage {
 };
 static const char* kStageNames[ST_COUNT] = {
     "remap", "neighbors", "forces", "noise", "mobility_apply", "lanczos_other", "update", "mc",
-    "k_fill(grid)", "k_spread", "k_fft_x(fwd)", "k_fft_y(fwd)", "k_fft_z_green", "k_fft_y(inv)", "k_fft_x(inv)",
-    "k_mob_final"};
+    "k_bead_prep", "k_spread_gather", "k_fft_x(fwd)", "k_fft_y(fwd)", "k_fft_z_green", "k_fft_y(inv)",
+    "k_fft_x(inv)", "k_mob_final"};
 
 struct Timer {
   hipEvent_t a, b;
@@ -258,6 +258,13 @@ struct smdlt_ctx {
   void* tw = nullptr;
   FftTask *tx = nullptr, *ty = nullptr, *tz = nullptr;
   int ntx = 0, nty = 0, ntz = 0;
+  BrickTask* bricks = nullptr;
+  int nbricks = 0;
+  std::vector<int> bk0;  // per-replica brick task ranges (size nrep+1)
+  int *bbase = nullptr, *brick_cnt = nullptr, *brick_start = nullptr, *brick_fill = nullptr, *brick_beads = nullptr;
+  void* bw = nullptr;
+  uint64_t pos_version = 1, prep_version = 0;
+  long long prep_range = -1;
   std::vector<int> tx0, ty0, tz0;  // per-replica task ranges (size nrep+1)
   // lanczos
   int maxit = 30;
@@ -485,8 +492,9 @@ static MobChoice mob_candidate(double rc, int order, double tol, double skin, co
   double chain = std::min((double)nb, 2.0 * pow(rl / 2.0, 1.7));
   m.nbr = (double)n / vol * (4.0 / 3.0) * M_PI * rl * rl * rl + chain;
   double G = (double)m.K[0] * m.K[1] * m.K[2];
-  // relative costs per pair, per grid point (5 FFT passes + fill + interp) and per spline point
-  m.cost = 0.003 * n * m.nbr + 0.03 * G + 0.002 * n * 3.0 * order * order * order;
+  // measured costs (ns) on MI355X: per real-space pair, per grid point (gather +
+  // five FFT passes) and per interpolated stencil point
+  m.cost = 0.02 * n * m.nbr + 0.15 * G + 0.01 * n * order * order * order;
   bool bad = false;
   for (int d = 0; d < 3; ++d) bad = bad || m.K[d] > KMAX;
   if (bad || m.nbr > 4000) m.cost = 1e300;
@@ -786,6 +794,34 @@ int smdlt_create(const smdlt_params* params, int n_replicas, int device, smdlt_c
     HCK(hipMemcpy(c->ty, vy.data(), sizeof(FftTask) * vy.size(), hipMemcpyHostToDevice));
     HCK(hipMemcpy(c->tz, vz.data(), sizeof(FftTask) * vz.size(), hipMemcpyHostToDevice));
   }
+  // spreading bricks (8x8x8 grid points), grouped by replica
+  {
+    std::vector<BrickTask> vb;
+    c->bk0.assign(n_replicas + 1, 0);
+    for (int r = 0; r < n_replicas; ++r) {
+      RepDev& R = c->rh[r];
+      c->bk0[r] = (int)vb.size();
+      R.brick0 = (int)vb.size();
+      for (int d = 0; d < 3; ++d) R.nbrick[d] = R.K[d] / BRICK;
+      for (int b2 = 0; b2 < R.nbrick[2]; ++b2)
+        for (int b1 = 0; b1 < R.nbrick[1]; ++b1)
+          for (int b0 = 0; b0 < R.nbrick[0]; ++b0) vb.push_back({r, b0, b1, b2});
+    }
+    c->bk0[n_replicas] = (int)vb.size();
+    c->nbricks = (int)vb.size();
+    st |= alloc_dev(&c->bricks, vb.size());
+    st |= alloc_dev(&c->brick_cnt, vb.size() + 1);
+    st |= alloc_dev(&c->brick_start, vb.size() + 1);
+    st |= alloc_dev(&c->brick_fill, vb.size() + 1);
+    st |= alloc_dev(&c->brick_beads, c->nst);
+    st |= alloc_dev(&c->bbase, 3 * (size_t)c->nst);
+    if (hipMalloc(&c->bw, (size_t)BW_STRIDE * c->nst * esz) != hipSuccess) st |= SMDLT_E_NOMEM;
+    if (st) {
+      smdlt_destroy(c);
+      return SMDLT_E_NOMEM;
+    }
+    HCK(hipMemcpy(c->bricks, vb.data(), sizeof(BrickTask) * vb.size(), hipMemcpyHostToDevice));
+  }
   set_active(c, 0, n_replicas);
   if (upload_reps(c)) {
     smdlt_destroy(c);
@@ -804,7 +840,8 @@ void smdlt_destroy(smdlt_ctx* c) {
   void* ptrs[] = {c->d_reps, c->d_slot_rep, c->d_seeds, c->rx, c->ry, c->rz, c->dxv, c->dyv, c->dzv, c->dsp, c->f,
                   c->z, c->mf, c->bz, c->tmp, c->tmp2, c->nrpy, c->lrpy, c->nlj, c->llj, c->overflow, c->d_err,
                   c->cell_of, c->cell_cnt, c->cell_start, c->cell_fill, c->cell_beads, c->rep_flags, c->out2, c->aff,
-                  c->grid, c->bsp, c->tw, c->tx, c->ty, c->tz, c->V, c->W, c->partial, c->scal, c->bdiv, c->done,
+                  c->grid, c->bsp, c->tw, c->tx, c->ty, c->tz, c->bricks, c->bbase, c->brick_cnt, c->brick_start,
+                  c->brick_fill, c->brick_beads, c->bw, c->V, c->W, c->partial, c->scal, c->bdiv, c->done,
                   c->iters, c->qtrack, c->qsign, c->qold, c->qidx, c->mc_err, c->mlp_w, c->occ, c->log_off};
   for (void* p : ptrs)
     if (p) hipFree(p);
@@ -883,6 +920,7 @@ int smdlt_upload_positions(smdlt_ctx* c, const double* rx, const double* ry, con
   launch_fill_f64(c->stream, c->dsp, 0.0, 3LL * c->nst);
   c->have_pos = true;
   c->lists_built = false;
+  c->pos_version++;
   HCK(hipStreamSynchronize(c->stream));
   return 0;
 }
@@ -897,6 +935,7 @@ int smdlt_download_positions(smdlt_ctx* c, double* rx, double* ry, double* rz) {
 }
 
 static int lattice_to_device(smdlt_ctx* c, uint64_t t) {
+  c->pos_version++;
   for (int r = 0; r < c->nrep; ++r) {
     c->ct_prev[r] = c->lat[r].ct;
     c->st_prev[r] = c->lat[r].st;
@@ -951,6 +990,7 @@ int smdlt_apply_pbc(smdlt_ctx* c) {
   int st = upload_reps(c);
   if (st) return st;
   launch_remap(c->stream, c->d_reps, c->d_slot_rep, 0, c->nst, c->rx, c->ry, c->rz, c->dxv, c->dyv, c->dzv, c->dsp, 0);
+  c->pos_version++;
   HCK(hipStreamSynchronize(c->stream));
   return 0;
 }
@@ -960,6 +1000,7 @@ int smdlt_rotate(smdlt_ctx* c, int inverse) {
   HCK(hipSetDevice(c->device));
   launch_remap(c->stream, c->d_reps, c->d_slot_rep, 0, c->nst, c->rx, c->ry, c->rz, c->dxv, c->dyv, c->dzv, c->dsp,
                inverse ? 3 : 2);
+  c->pos_version++;
   HCK(hipStreamSynchronize(c->stream));
   return 0;
 }
@@ -1207,7 +1248,7 @@ static void mobility_internal(smdlt_ctx* c, const double* v, double* u, const do
   m.done = done;
   m.grid = c->grid;
   m.grid_begin = a.grid0;
-  m.grid_len = b.grid0 + 3LL * b.K[0] * b.K[1] * b.K[2] - a.grid0;
+  m.grid_len = 0;  // the gather writes every grid point
   m.bsp = c->bsp;
   m.tw = c->tw;
   m.tx = c->tx + c->tx0[rb];
@@ -1217,6 +1258,19 @@ static void mobility_internal(smdlt_ctx* c, const double* v, double* u, const do
   m.nty = c->ty0[re] - c->ty0[rb];
   m.ntz = c->tz0[re] - c->tz0[rb];
   m.order = c->rh[rb].order;
+  m.bbase = c->bbase;
+  m.bw = c->bw;
+  m.brick_cnt = c->brick_cnt;
+  m.brick_start = c->brick_start;
+  m.brick_fill = c->brick_fill;
+  m.brick_beads = c->brick_beads;
+  m.bricks = c->bricks + c->bk0[rb];
+  m.nbricks = c->bk0[re] - c->bk0[rb];
+  m.brick_begin = c->bk0[rb];
+  m.brick_len = c->bk0[re] - c->bk0[rb];
+  m.prep = (c->prep_version != c->pos_version || c->prep_range != rb * 100003LL + re) ? 1 : 0;
+  c->prep_version = c->pos_version;
+  c->prep_range = rb * 100003LL + re;
   hipEvent_t ev[9];
   m.ev = nullptr;
   if (c->prof) {
@@ -1443,6 +1497,7 @@ static int step_internal(smdlt_ctx* c, uint64_t t, double* noise_dev, bool host_
     StageScope sc(c, ST_UPDATE);
     launch_update_positions(c->stream, c->d_reps, c->d_slot_rep, s0, ns, c->rx, c->ry, c->rz, c->mf, c->bz, c->dxv,
                             c->dyv, c->dzv, c->dsp, 1);
+    c->pos_version++;
   }
   HCK(hipGetLastError());
   return 0;
@@ -1452,8 +1507,7 @@ int smdlt_step(smdlt_ctx* c, uint64_t t, const double* noise) {
   if (!c) return SMDLT_E_ARG;
   if (!c->have_pos) return SMDLT_E_STATE;
   HCK(hipSetDevice(c->device));
-  int st = ensure_lists(c);
-  if (st) return st;
+  int st = 0;  // lists are (re)built after the remap, in the wrap frame
   if (noise) {
     st = gather_aos(c, noise, c->tmp);
     if (st) return st;
@@ -1474,11 +1528,10 @@ int smdlt_run(smdlt_ctx* c, uint64_t t0, int nsteps, int use_graph) {
   if (!c || nsteps < 0) return SMDLT_E_ARG;
   if (!c->have_pos) return SMDLT_E_STATE;
   HCK(hipSetDevice(c->device));
-  int st = ensure_lists(c);
-  if (st) return st;
+  int st = 0;
   for (int k = 0; k < nsteps; ++k) {
     uint64_t t = t0 + k;
-    if (!use_graph) {
+    if (!use_graph || !c->lists_built) {  // the first step builds the lists after its remap
       st = step_internal(c, t, nullptr, false);
       if (st) return st;
       continue;
@@ -1505,6 +1558,7 @@ int smdlt_run(smdlt_ctx* c, uint64_t t0, int nsteps, int use_graph) {
       if (!st) st = brownian_internal(c, c->z, c->bz, p.lanczos_tol > 0 ? p.lanczos_tol : 1e-6, c->maxit, false);
       launch_update_positions(c->stream, c->d_reps, c->d_slot_rep, s0, ns, c->rx, c->ry, c->rz, c->mf, c->bz, c->dxv,
                               c->dyv, c->dzv, c->dsp, 1);
+      c->pos_version++;
       hipError_t e = hipStreamEndCapture(c->stream, &graph);
       c->prof = prof;
       if (st) return st;

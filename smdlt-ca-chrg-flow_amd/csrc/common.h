@@ -23,6 +23,7 @@ struct RepDev {
   int q0, nq;                  // charges: offset into the charge arrays, count
   long long grid0;             // element offset of the replica's grid (3 components)
   int bsp0, active;            // B-spline modulus offset; replica enabled for this launch
+  int brick0, nbrick[3];       // 8x8x8-point grid bricks of the spreading gather
   // lattice in the wrap frame
   double L, invL, L1xp, L2xp, L2yp, delx;
   double ct, st;               // rotation lab -> wrap frame at this step (KRBC)
@@ -192,6 +193,9 @@ __device__ __forceinline__ double u01_53(uint32_t a, uint32_t b) {
 // --------------------------------------------------------------------------
 // Launch wrappers (host) exported by the kernel files.
 struct FftTask { int rep, comp, l0, nl; };
+struct BrickTask { int rep, b0, b1, b2; };
+#define BRICK 8
+#define BW_STRIDE (3 * MAX_ORDER)  // B-spline weights stored per bead
 
 struct DevState;  // defined in api.cpp
 
@@ -260,6 +264,14 @@ struct MobArgs {
   int ntx, nty, ntz;
   int order;                // B-spline order (4, 6 or 8), one per context
   hipEvent_t* ev;           // 9 events bracketing the 8 launches (profiling) or nullptr
+  // spreading gather: per-bead stencil base and weights, beads binned by brick
+  int* bbase;               // 3 ints per slot
+  void* bw;                 // BW_STRIDE weights (T) per slot
+  int *brick_cnt, *brick_start, *brick_fill, *brick_beads;
+  const BrickTask* bricks;
+  int nbricks, brick_begin, brick_len;
+  int prep;                 // recompute base/weights/binning (positions or lattice changed)
+  int* scan_tmp;
 };
 template <typename T> void launch_mobility(hipStream_t s, const MobArgs& a);
 

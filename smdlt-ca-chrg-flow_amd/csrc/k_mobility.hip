@@ -104,8 +104,6 @@ __device__ __forceinline__ void frac_weights(const RepDev& R, double x, double y
   }
 }
 
-template <typename T> __device__ __forceinline__ void grid_add(T* p, T v) { unsafeAtomicAdd(p, v); }
-
 // select W[j] for a runtime j without dynamic register indexing
 template <typename T, int P> __device__ __forceinline__ T pick(const T* W, int j) {
   T w = (T)0;
@@ -114,46 +112,115 @@ template <typename T, int P> __device__ __forceinline__ T pick(const T* W, int j
   return w;
 }
 
-// One wavefront per bead: every lane computes the bead's weights, lane l < P^2
-// owns stencil row (j2, j3) = (l % P, l / P) and adds its P points x 3 components.
+__device__ __forceinline__ int brick_of(const RepDev& R, const int* base) {
+  return R.brick0 + ((base[2] / BRICK) * R.nbrick[1] + base[1] / BRICK) * R.nbrick[0] + base[0] / BRICK;
+}
+
+// Per bead: stencil base point and B-spline weights along each lattice vector,
+// and a count in the brick holding the base point (for the gather).
 template <typename T, int P>
-__global__ __launch_bounds__(256) void k_spread(MobArgs a) {
-  const int lane = threadIdx.x & 63;
-  int s = a.slot_begin + blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+__global__ void k_bead_prep(MobArgs a) {
+  int s = a.slot_begin + blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= a.slot_begin + a.nslots) return;
   int r = a.slot_rep[s];
   if (r < 0) return;
   const RepDev& R = a.reps[r];
-  if (!R.active || (a.done && a.done[r])) return;
-  if (s - R.slot0 >= R.n) return;
-  double sc = a.beta ? 1.0 / a.beta[r] : 1.0;
-  double v0 = a.v[3 * s] * sc, v1 = a.v[3 * s + 1] * sc, v2 = a.v[3 * s + 2] * sc;
-  if (a.vout && lane == 0) {
-    a.vout[3 * s] = v0;
-    a.vout[3 * s + 1] = v1;
-    a.vout[3 * s + 2] = v2;
-  }
-  if (lane >= P * P) return;
+  if (!R.active || s - R.slot0 >= R.n) return;
   int base[3];
   T W[3][P];
   frac_weights<T, P>(R, a.rx[s], a.ry[s], a.rz[s], base, W);
-  const int j2 = lane % P, j3 = lane / P;
-  T* g = (T*)a.grid + R.grid0;
-  const long long ncomp = (long long)R.K[0] * R.K[1] * R.K[2];
-  const int K0 = R.K[0], K1 = R.K[1], K2 = R.K[2];
-  int g3 = (base[2] - j3) & (K2 - 1);
-  int g2 = (base[1] - j2) & (K1 - 1);
-  T w23 = pick<T, P>(W[2], j3) * pick<T, P>(W[1], j2);
-  T tv0 = (T)v0 * w23, tv1 = (T)v1 * w23, tv2 = (T)v2 * w23;
-  T* row = g + ((long long)g3 * K1 + g2) * K0;
+  T* bw = (T*)a.bw + (size_t)BW_STRIDE * s;
 #pragma unroll
-  for (int j1 = 0; j1 < P; ++j1) {
-    int g1 = (base[0] - j1) & (K0 - 1);
-    T w = W[0][j1];
-    grid_add(row + g1, w * tv0);
-    grid_add(row + g1 + ncomp, w * tv1);
-    grid_add(row + g1 + 2 * ncomp, w * tv2);
+  for (int d = 0; d < 3; ++d) {
+    a.bbase[3 * s + d] = base[d];
+#pragma unroll
+    for (int j = 0; j < P; ++j) bw[d * MAX_ORDER + j] = W[d][j];
   }
+  atomicAdd(&a.brick_cnt[brick_of(R, base)], 1);
+}
+
+__global__ void k_brick_fill(MobArgs a) {
+  int s = a.slot_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.slot_begin + a.nslots) return;
+  int r = a.slot_rep[s];
+  if (r < 0) return;
+  const RepDev& R = a.reps[r];
+  if (!R.active || s - R.slot0 >= R.n) return;
+  int bi = brick_of(R, a.bbase + 3 * s);
+  int k = atomicAdd(&a.brick_fill[bi], 1);
+  a.brick_beads[a.brick_start[bi] + k] = s;
+}
+
+// Lanczos normalisation: vout = v / beta[r] (the gather and the real-space pass read vout)
+__global__ void k_scale_v(MobArgs a) {
+  int s = a.slot_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.slot_begin + a.nslots) return;
+  int r = a.slot_rep[s];
+  if (r < 0) return;
+  const RepDev& R = a.reps[r];
+  if (!R.active || (a.done && a.done[r]) || s - R.slot0 >= R.n) return;
+  double sc = a.beta ? 1.0 / a.beta[r] : 1.0;
+  for (int c = 0; c < 3; ++c) a.vout[3 * s + c] = a.v[3 * s + c] * sc;
+}
+
+// Spreading as a gather: one workgroup per 8x8x8 brick of one replica's grid,
+// one thread per grid point.  A bead with stencil base b touches points b-P+1..b,
+// so the beads reaching this brick live in it or in the next brick along each
+// axis (P <= 8); they are staged through LDS in chunks and every thread adds
+// the ones whose stencil covers its point.  No atomics: the grid needs no
+// zeroing and the result does not depend on scheduling.
+#define GATHER_CHUNK 128
+template <typename T, int P>
+__global__ __launch_bounds__(512) void k_spread_gather(MobArgs a) {
+  __shared__ int sb[GATHER_CHUNK * 3];
+  __shared__ T sw[GATHER_CHUNK * 3 * P];
+  __shared__ T sv[GATHER_CHUNK * 3];
+  const BrickTask bt = a.bricks[blockIdx.x];
+  const RepDev& R = a.reps[bt.rep];
+  if (!R.active || (a.done && a.done[bt.rep])) return;
+  const int tid = threadIdx.x;
+  const int K0 = R.K[0], K1 = R.K[1], K2 = R.K[2];
+  const int g0 = bt.b0 * BRICK + (tid & 7), g1 = bt.b1 * BRICK + ((tid >> 3) & 7), g2 = bt.b2 * BRICK + (tid >> 6);
+  const double* vin = a.vout ? a.vout : a.v;
+  const T* bw = (const T*)a.bw;
+  T acc0 = 0, acc1 = 0, acc2 = 0;
+  for (int nbk = 0; nbk < 8; ++nbk) {
+    int c0 = (bt.b0 + (nbk & 1)) % R.nbrick[0];
+    int c1 = (bt.b1 + ((nbk >> 1) & 1)) % R.nbrick[1];
+    int c2 = (bt.b2 + (nbk >> 2)) % R.nbrick[2];
+    int bi = R.brick0 + (c2 * R.nbrick[1] + c1) * R.nbrick[0] + c0;
+    int beg = a.brick_start[bi], cnt = a.brick_cnt[bi];
+    for (int k0 = 0; k0 < cnt; k0 += GATHER_CHUNK) {
+      int m = min(GATHER_CHUNK, cnt - k0);
+      __syncthreads();
+      for (int k = tid; k < m * 3; k += blockDim.x) {
+        int kk = k / 3, d = k - 3 * kk;
+        int s = a.brick_beads[beg + k0 + kk];
+        sb[k] = a.bbase[3 * s + d];
+        sv[k] = (T)vin[3 * s + d];
+#pragma unroll
+        for (int j = 0; j < P; ++j) sw[k * P + j] = bw[(size_t)BW_STRIDE * s + d * MAX_ORDER + j];
+      }
+      __syncthreads();
+      for (int k = 0; k < m; ++k) {
+        int d0 = (sb[3 * k] - g0) & (K0 - 1);
+        int d1 = (sb[3 * k + 1] - g1) & (K1 - 1);
+        int d2 = (sb[3 * k + 2] - g2) & (K2 - 1);
+        if (d0 < P && d1 < P && d2 < P) {
+          T w = sw[(3 * k) * P + d0] * sw[(3 * k + 1) * P + d1] * sw[(3 * k + 2) * P + d2];
+          acc0 += w * sv[3 * k];
+          acc1 += w * sv[3 * k + 1];
+          acc2 += w * sv[3 * k + 2];
+        }
+      }
+    }
+  }
+  T* g = (T*)a.grid + R.grid0;
+  const long long ncomp = (long long)K0 * K1 * K2;
+  long long idx = ((long long)g2 * K1 + g1) * K0 + g0;
+  g[idx] = acc0;
+  g[idx + ncomp] = acc1;
+  g[idx + 2 * ncomp] = acc2;
 }
 
 // ------------------------------------------------------------------- FFT passes
@@ -352,7 +419,13 @@ __global__ __launch_bounds__(256) void k_mob_final(MobArgs a) {
   if (lane < P * P) {
     int base[3];
     T W[3][P];
-    frac_weights<T, P>(R, xi, yi, zi, base, W);
+    const T* bw = (const T*)a.bw + (size_t)BW_STRIDE * s;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      base[d] = a.bbase[3 * s + d];
+#pragma unroll
+      for (int j = 0; j < P; ++j) W[d][j] = bw[d * MAX_ORDER + j];
+    }
     const int j2 = lane % P, j3 = lane / P;
     const T* g = (const T*)a.grid + R.grid0;
     const long long ncomp = (long long)R.K[0] * R.K[1] * R.K[2];
@@ -383,7 +456,17 @@ __global__ __launch_bounds__(256) void k_mob_final(MobArgs a) {
     min_image_reduced(R, dx, dy, dz);
     T tx = (T)dx, ty = (T)dy, tz = (T)dz;
     T rr = tx * tx + ty * ty + tz * tz;
-    if (rr < rc2) {
+    if (rr < (T)4) {
+      // overlapping pair: the overlap correction cancels 1/r^3 terms, so
+      // evaluate it in FP64 whatever T is (rare in a physical configuration)
+      double r2 = dx * dx + dy * dy + dz * dz, rd = sqrt(r2), t1, t2;
+      rpy_real<double>(rd, R.alpha, t1, t2);
+      double vx = vv[3 * j] * sc, vy = vv[3 * j + 1] * sc, vz = vv[3 * j + 2] * sc;
+      double dv = (dx * vx + dy * vy + dz * vz) * t2 / r2;
+      acc[0].add((T)(t1 * vx + dv * dx));
+      acc[1].add((T)(t1 * vy + dv * dy));
+      acc[2].add((T)(t1 * vz + dv * dz));
+    } else if (rr < rc2) {
       T rd = sqrt(rr), t1, t2;
       rpy_real<T>(rd, alpha, t1, t2);
       T vx = (T)(vv[3 * j] * sc), vy = (T)(vv[3 * j + 1] * sc), vz = (T)(vv[3 * j + 2] * sc);
@@ -402,27 +485,22 @@ __global__ __launch_bounds__(256) void k_mob_final(MobArgs a) {
   }
 }
 
-template <typename T> void launch_mobility(hipStream_t st, const MobArgs& a) {
+template <typename T, int P> static void launch_mob_p(hipStream_t st, const MobArgs& a) {
   auto mark = [&](int k) {
     if (a.ev) (void)hipEventRecord(a.ev[k], st);
   };
   mark(0);
-  if (a.grid_len > 0) {
-    if (sizeof(T) == 4)
-      launch_fill_f32(st, (float*)a.grid + a.grid_begin, 0.0f, a.grid_len);
-    else
-      launch_fill_f64(st, (double*)a.grid + a.grid_begin, 0.0, a.grid_len);
+  int nt = (a.nslots + 255) / 256;
+  if (a.prep && a.nslots > 0) {
+    launch_fill_i32(st, a.brick_cnt + a.brick_begin, 0, a.brick_len);
+    launch_fill_i32(st, a.brick_fill + a.brick_begin, 0, a.brick_len);
+    k_bead_prep<T, P><<<nt, 256, 0, st>>>(a);
+    launch_scan(st, a.brick_cnt + a.brick_begin, a.brick_start + a.brick_begin, a.brick_len);
+    k_brick_fill<<<nt, 256, 0, st>>>(a);
   }
+  if (a.vout && a.nslots > 0) k_scale_v<<<nt, 256, 0, st>>>(a);
   mark(1);
-  int nb = (a.nslots + 3) / 4;  // one wavefront per bead, four per block
-  if (a.nslots > 0) {
-    if (a.order == 4)
-      k_spread<T, 4><<<nb, 256, 0, st>>>(a);
-    else if (a.order == 8)
-      k_spread<T, 8><<<nb, 256, 0, st>>>(a);
-    else
-      k_spread<T, 6><<<nb, 256, 0, st>>>(a);
-  }
+  if (a.nbricks > 0) k_spread_gather<T, P><<<a.nbricks, 512, 0, st>>>(a);
   mark(2);
   if (a.ntx > 0) k_fft_x<T><<<a.ntx, FFT_THREADS, 0, st>>>(a, -1);
   mark(3);
@@ -434,15 +512,17 @@ template <typename T> void launch_mobility(hipStream_t st, const MobArgs& a) {
   mark(6);
   if (a.ntx > 0) k_fft_x<T><<<a.ntx, FFT_THREADS, 0, st>>>(a, 1);
   mark(7);
-  if (a.nslots > 0) {
-    if (a.order == 4)
-      k_mob_final<T, 4><<<nb, 256, 0, st>>>(a);
-    else if (a.order == 8)
-      k_mob_final<T, 8><<<nb, 256, 0, st>>>(a);
-    else
-      k_mob_final<T, 6><<<nb, 256, 0, st>>>(a);
-  }
+  if (a.nslots > 0) k_mob_final<T, P><<<(a.nslots + 3) / 4, 256, 0, st>>>(a);
   mark(8);
+}
+
+template <typename T> void launch_mobility(hipStream_t st, const MobArgs& a) {
+  if (a.order == 4)
+    launch_mob_p<T, 4>(st, a);
+  else if (a.order == 8)
+    launch_mob_p<T, 8>(st, a);
+  else
+    launch_mob_p<T, 6>(st, a);
 }
 
 template void launch_mobility<float>(hipStream_t, const MobArgs&);

@@ -303,7 +303,7 @@ def test_batched_replicas_equal_single_contexts():
     poss = []
     for k, (flow, L, nb, nc) in enumerate(specs):
         params.append(S.default_params(nb=nb, nc=nc, L=L, flow_type=flow, edot=0.4, dt=1e-3,
-                                       precision=S.PREC_FP64, rc_real=6.0, grid=(32, 64, 32)))
+                                       precision=S.PREC_FP64, rc_real=6.0, grid=(32, 64, 32), spline_order=6))
         poss.append(random_walk(nb, nc, L, 20 + k))
     big = S.Context(params)
     allp = np.concatenate(poss)
