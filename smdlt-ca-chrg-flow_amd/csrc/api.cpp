@@ -268,7 +268,7 @@ struct smdlt_ctx {
   std::vector<int> tx0, ty0, tz0;  // per-replica task ranges (size nrep+1)
   // lanczos
   int maxit = 30;
-  double *V = nullptr, *W = nullptr, *partial = nullptr, *scal = nullptr, *bdiv = nullptr;
+  double *V = nullptr, *W = nullptr, *partial = nullptr, *partial2 = nullptr, *scal = nullptr, *bdiv = nullptr;
   int *done = nullptr, *iters = nullptr;
   std::vector<int> last_iters;
   // charges
@@ -709,6 +709,7 @@ int smdlt_create(const smdlt_params* params, int n_replicas, int device, smdlt_c
   st |= alloc_dev(&c->V, (size_t)(c->maxit + 1) * n3);
   st |= alloc_dev(&c->W, n3);
   st |= alloc_dev(&c->partial, c->nwaves + 1);
+  st |= alloc_dev(&c->partial2, c->nwaves + 1);
   st |= alloc_dev(&c->scal, (size_t)SCAL_STRIDE * n_replicas);
   st |= alloc_dev(&c->bdiv, n_replicas);
   st |= alloc_dev(&c->done, n_replicas);
@@ -840,7 +841,7 @@ void smdlt_destroy(smdlt_ctx* c) {
   void* ptrs[] = {c->d_reps, c->d_slot_rep, c->d_seeds, c->rx, c->ry, c->rz, c->dxv, c->dyv, c->dzv, c->dsp, c->f,
                   c->z, c->mf, c->bz, c->tmp, c->tmp2, c->nrpy, c->lrpy, c->nlj, c->llj, c->overflow, c->d_err,
                   c->cell_of, c->cell_cnt, c->cell_start, c->cell_fill, c->cell_beads, c->rep_flags, c->out2, c->aff,
-                  c->grid, c->bsp, c->tw, c->tx, c->ty, c->tz, c->bricks, c->bbase, c->brick_cnt, c->brick_start,
+                  c->partial2, c->grid, c->bsp, c->tw, c->tx, c->ty, c->tz, c->bricks, c->bbase, c->brick_cnt, c->brick_start,
                   c->brick_fill, c->brick_beads, c->bw, c->V, c->W, c->partial, c->scal, c->bdiv, c->done,
                   c->iters, c->qtrack, c->qsign, c->qold, c->qidx, c->mc_err, c->mlp_w, c->occ, c->log_off};
   for (void* p : ptrs)
@@ -1240,6 +1241,7 @@ static void mobility_internal(smdlt_ctx* c, const double* v, double* u, const do
   m.rz = c->rz;
   m.nrpy = c->nrpy;
   m.lrpy = c->lrpy;
+  m.cap_rpy = c->cap_rpy;
   m.v = v;
   m.beta = beta;
   m.vout = vout;
@@ -1329,6 +1331,7 @@ static int brownian_internal(smdlt_ctx* c, const double* z, double* y, double to
   {
     StageScope sc(c, ST_LANCZOS);
     launch_fill_i32(c->stream, c->done + rb, 0, re - rb);
+    lz_reset_checks(c->stream, c->scal, rb, re - rb);
     lz_wave_dot(c->stream, c->d_reps, c->d_slot_rep, s0, ns, z, nullptr, nullptr, c->partial);
     lz_rep_reduce(c->stream, c->d_reps, rb, re - rb, c->partial, nullptr, c->scal, SCAL_STRIDE, 4 * LZ_MAXIT, 1,
                   c->bdiv);
@@ -1340,9 +1343,8 @@ static int brownian_internal(smdlt_ctx* c, const double* z, double* y, double to
     mobility_internal(c, vin, c->W, c->bdiv, vj, nullptr, c->done);
     {
       StageScope sc(c, ST_LANCZOS);
-      lz_wave_dot(c->stream, c->d_reps, c->d_slot_rep, s0, ns, vj, c->W, c->done, c->partial);
-      lz_rep_reduce(c->stream, c->d_reps, rb, re - rb, c->partial, c->done, c->scal, SCAL_STRIDE, j - 1, 0, nullptr);
-      lz_orth(c->stream, c->d_reps, c->d_slot_rep, s0, ns, c->W, vj, vjm1, c->scal, j, c->done, c->partial);
+      lz_wave_dot(c->stream, c->d_reps, c->d_slot_rep, s0, ns, vj, c->W, c->done, c->partial2);
+      lz_orth(c->stream, c->d_reps, c->d_slot_rep, s0, ns, c->W, vj, vjm1, c->scal, j, c->done, c->partial2, c->partial);
       lz_tri(c->stream, c->d_reps, rb, re - rb, c->partial, c->scal, j, tol, maxit, c->done, c->bdiv, c->iters);
     }
     if (host_exit && j >= 2) {

@@ -249,7 +249,8 @@ struct MobArgs {
   const int* slot_rep;
   int slot_begin, nslots, nst;
   const double *rx, *ry, *rz;
-  const int *nrpy, *lrpy;
+  const int *nrpy, *lrpy;   // mobility list, row-major: lrpy[slot * cap_rpy + k]
+  int cap_rpy;
   const double* v;          // input vector (AoS 3N); scaled by 1/beta[r] when beta != nullptr
   const double* beta;       // per-replica divisor (Lanczos normalisation) or nullptr
   double* vout;             // where the (scaled) input is stored for the real-space pass
@@ -281,7 +282,8 @@ void lz_wave_dot(hipStream_t st, const RepDev* reps, const int* slot_rep, int sl
 void lz_rep_reduce(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, const double* partial,
                    const int* done, double* out, int stride, int off, int take_sqrt, double* copy_to);
 void lz_orth(hipStream_t st, const RepDev* reps, const int* slot_rep, int slot_begin, int nslots, double* w,
-             const double* vj, const double* vjm1, const double* scal, int j, const int* done, double* partial);
+             const double* vj, const double* vjm1, double* scal, int j, const int* done, const double* partial_alpha,
+             double* partial);
 void lz_tri(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, const double* partial, double* scal, int j,
             double tol, int maxit, int* done, double* bdiv, int* iters);
 void lz_combine(hipStream_t st, const RepDev* reps, const int* slot_rep, int slot_begin, int nslots, const double* V,
@@ -289,6 +291,7 @@ void lz_combine(hipStream_t st, const RepDev* reps, const int* slot_rep, int slo
 void lz_tea(hipStream_t st, const int* slot_rep, int slot_begin, int nslots, const double* mz, const double* z,
             const double* diag, const double* beta, const double* C, double* y);
 #define SCAL_STRIDE (4 * LZ_MAXIT + 8)
+void lz_reset_checks(hipStream_t st, double* scal, int rep_begin, int nrep);
 
 // k_mc.hip
 void launch_mc_sweep(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, const double* rx, const double* ry,

@@ -105,63 +105,44 @@ void launch_forces(hipStream_t st, const RepDev* reps, const int* slot_rep, int 
                                                  llj, f, err);
 }
 
-// Screened Coulomb between all charged beads of a replica: thread per charge,
-// partner positions staged through LDS.  qtrack holds bead indices local to the
-// replica (Charge_track); qsign the charge of each.
-__global__ void k_coulomb(const RepDev* __restrict__ reps, int rep_begin, const double* __restrict__ rx,
-                          const double* __restrict__ ry, const double* __restrict__ rz, const int* __restrict__ qtrack,
-                          const int* __restrict__ qsign, double* f) {
+// Screened Coulomb between all charged beads of a replica: one wavefront per
+// charge, lanes stride over the partners, FP64 sums combined across the
+// wavefront.  qtrack holds bead indices local to the replica (Charge_track).
+__global__ __launch_bounds__(256) void k_coulomb(const RepDev* __restrict__ reps, int rep_begin,
+                                                 const double* __restrict__ rx, const double* __restrict__ ry,
+                                                 const double* __restrict__ rz, const int* __restrict__ qtrack,
+                                                 const int* __restrict__ qsign, double* f) {
   const RepDev& R = reps[rep_begin + blockIdx.y];
   if (!R.active) return;
-  int nq = R.nq;
-  int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if ((int)(blockIdx.x * blockDim.x) >= nq) return;
-  __shared__ double sx[256], sy[256], sz[256];
-  __shared__ int sq[256];
-  int ia = (a < nq) ? R.slot0 + qtrack[R.q0 + a] : -1;
-  double xa = 0, ya = 0, za = 0;
-  int qa = 0;
-  if (a < nq) {
-    xa = rx[ia];
-    ya = ry[ia];
-    za = rz[ia];
-    qa = qsign[R.q0 + a];
-  }
+  const int lane = threadIdx.x & 63;
+  const int a = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int nq = R.nq;
+  if (a >= nq) return;
+  const int ia = R.slot0 + qtrack[R.q0 + a];
+  const double xa = rx[ia], ya = ry[ia], za = rz[ia];
+  const int qa = qsign[R.q0 + a];
   double F[3] = {0, 0, 0};
-  for (int b0 = 0; b0 < nq; b0 += 256) {
-    int b = b0 + threadIdx.x;
-    if (b < nq) {
-      int ib = R.slot0 + qtrack[R.q0 + b];
-      sx[threadIdx.x] = rx[ib];
-      sy[threadIdx.x] = ry[ib];
-      sz[threadIdx.x] = rz[ib];
-      sq[threadIdx.x] = qsign[R.q0 + b];
-    }
-    __syncthreads();
-    int nb = min(256, nq - b0);
-    if (a < nq) {
-      for (int k = 0; k < nb; ++k) {
-        if (b0 + k == a) continue;
-        double d[3];
-        ref_nid(R, xa, ya, za, sx[k], sy[k], sz[k], d[0], d[1], d[2], false);
-        double rr = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-        double rd = sqrt(rr);
-        double Fc = R.lambda_b * (double)(qa * sq[k]) * exp(-R.kappa_d * rd) * (1 + rd * R.kappa_d) / rr;
-        for (int q = 0; q < 3; ++q) F[q] += Fc * d[q] / rd;
-      }
-    }
-    __syncthreads();
+  for (int b = lane; b < nq; b += 64) {
+    if (b == a) continue;
+    int ib = R.slot0 + qtrack[R.q0 + b];
+    double d[3];
+    ref_nid(R, xa, ya, za, rx[ib], ry[ib], rz[ib], d[0], d[1], d[2], false);
+    double rr = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    double rd = sqrt(rr);
+    double Fc = R.lambda_b * (double)(qa * qsign[R.q0 + b]) * exp(-R.kappa_d * rd) * (1 + rd * R.kappa_d) / rr;
+    for (int q = 0; q < 3; ++q) F[q] += Fc * d[q] / rd;
   }
-  if (a < nq) {
-    f[3 * ia] += F[0];
-    f[3 * ia + 1] += F[1];
-    f[3 * ia + 2] += F[2];
+  double fx = warp_sum(F[0]), fy = warp_sum(F[1]), fz = warp_sum(F[2]);
+  if (lane == 0) {
+    f[3 * ia] += fx;
+    f[3 * ia + 1] += fy;
+    f[3 * ia + 2] += fz;
   }
 }
 
 void launch_coulomb(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, const double* rx, const double* ry,
                     const double* rz, const int* qtrack, const int* qsign, double* f, int max_nq) {
   if (nrep <= 0 || max_nq <= 0) return;
-  dim3 grid((max_nq + 255) / 256, nrep);
+  dim3 grid((max_nq + 3) / 4, nrep);
   k_coulomb<<<grid, 256, 0, st>>>(reps, rep_begin, rx, ry, rz, qtrack, qsign, f);
 }

@@ -53,36 +53,46 @@ __global__ void k_rep_reduce(const RepDev* __restrict__ reps, int rep_begin, con
   }
 }
 
-// w <- w - a_j v_j - b_{j-1} v_{j-1}; partial |w|^2
+// w <- w - a_j v_j - b_{j-1} v_{j-1}; partial |w|^2 out.  a_j = v_j.w is
+// reduced by every wavefront from its replica's per-wave partials in wave order
+// (a wavefront's 64 slots always belong to one replica), so all wavefronts of a
+// replica use the identical value.
 __global__ void k_lz_orth(const RepDev* __restrict__ reps, const int* __restrict__ slot_rep, int slot_begin,
-                          int nslots, double* w, const double* vj, const double* vjm1, const double* scal, int j,
-                          const int* done, double* partial) {
+                          int nslots, double* w, const double* vj, const double* vjm1, double* scal, int j,
+                          const int* done, const double* partial_alpha, double* partial) {
+  const int lane = threadIdx.x & 63;
   int s = slot_begin + blockIdx.x * blockDim.x + threadIdx.x;
-  bool in = s < slot_begin + nslots;
-  int r = in ? slot_rep[s] : -1;
+  int s_w = s - lane;  // first slot of this wavefront
+  if (s_w >= slot_begin + nslots) return;
+  int r = slot_rep[s_w];
+  if (r < 0) return;
+  const RepDev& R = reps[r];
+  if (!R.active || (done && done[r])) return;
+  double a = 0.0;
+  for (int q = lane; q < R.nwaves; q += 64) a += partial_alpha[R.wave0 + q];
+  a = warp_sum(a);
+  double* sc = scal + (size_t)r * SCAL_STRIDE;
+  if (lane == 0 && s_w == R.slot0) sc[S_AL + j - 1] = a;
   double v = 0.0;
-  bool act = false;
-  if (r >= 0) {
-    const RepDev& R = reps[r];
-    act = R.active && !(done && done[r]);
-    if (act && s - R.slot0 < R.n) {
-      const double* sc = scal + (size_t)r * SCAL_STRIDE;
-      double a = sc[S_AL + j - 1];
-      double b = (j > 1) ? sc[S_BE + j - 2] : 0.0;
-      for (int c = 0; c < 3; ++c) {
-        double x = w[3 * s + c] - a * vj[3 * s + c];
-        if (j > 1) x -= b * vjm1[3 * s + c];
-        w[3 * s + c] = x;
-        v += x * x;
-      }
+  if (s - R.slot0 < R.n) {
+    double b = (j > 1) ? sc[S_BE + j - 2] : 0.0;
+    for (int c = 0; c < 3; ++c) {
+      double x = w[3 * s + c] - a * vj[3 * s + c];
+      if (j > 1) x -= b * vjm1[3 * s + c];
+      w[3 * s + c] = x;
+      v += x * x;
     }
   }
   v = warp_sum(v);
-  if ((threadIdx.x & 63) == 0 && act) partial[s >> 6] = v;
+  if (lane == 0) partial[s >> 6] = v;
 }
 
-// b_j from the partials, then c_j = b0 T_j^(1/2) e1 by implicit QL and the
-// stopping test.  Block = one wavefront (64 lanes) per replica.
+// b_j from the partials; then, on check iterations (j even and >= 4, the last
+// iteration, or a breakdown), c_j = b0 T_j^(1/2) e1 by implicit-shift QL and the
+// stopping test |c_j - c_prev| < tol |c_j| against the previous check.  Block =
+// one wavefront per replica; every lane runs the scalar QL recurrence on its
+// own copy (registers / LDS written with identical values) and rotates its own
+// row of the eigenvector matrix, so the recurrence needs no barriers.
 __global__ __launch_bounds__(64) void k_lz_tri(const RepDev* __restrict__ reps, int rep_begin, const double* partial,
                                                double* scal, int j, double tol, int maxit, int* done, double* bdiv,
                                                int* iters) {
@@ -91,22 +101,29 @@ __global__ __launch_bounds__(64) void k_lz_tri(const RepDev* __restrict__ reps, 
   if (!R.active || done[r]) return;
   const int lane = threadIdx.x;
   double* sc = scal + (size_t)r * SCAL_STRIDE;
-  // b_j
   double v = 0.0;
   for (int w = lane; w < R.nwaves; w += 64) v += partial[R.wave0 + w];
   v = warp_sum(v);
-  double bj = sqrt(v);
+  const double bj = sqrt(v);
+  const bool breakdown = !(bj > 1e-300);
+  const bool check = (j >= 4 && (j % 2) == 0) || j >= maxit || breakdown;
+  if (lane == 0) {
+    sc[S_BE + j - 1] = bj;
+    bdiv[r] = bj;
+    iters[r] = j;
+    sc[S_IT] = (double)j;
+  }
+  if (!check) return;
   __shared__ double d[LZ_MAXIT], e[LZ_MAXIT];
   __shared__ double Q[LZ_MAXIT * 64];
   const int n = j;
+  __syncthreads();
   if (lane < n) {
     d[lane] = sc[S_AL + lane];
     e[lane] = (lane < n - 1) ? sc[S_BE + lane] : 0.0;
   }
   for (int k = 0; k < n; ++k) Q[k * 64 + lane] = (k == lane) ? 1.0 : 0.0;
   __syncthreads();
-  // implicit-shift QL on (d, e); every lane runs the scalar recurrence on the
-  // shared copies and rotates its own row of Q.
   for (int l = 0; l < n; ++l) {
     for (int iter = 0; iter < 64; ++iter) {
       int m;
@@ -116,22 +133,17 @@ __global__ __launch_bounds__(64) void k_lz_tri(const RepDev* __restrict__ reps, 
       }
       if (m == l) break;
       double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-      double rr = hypot(g, 1.0);
+      double rr = sqrt(g * g + 1.0);
       g = d[m] - d[l] + e[l] / (g + copysign(rr, g));
       double s = 1.0, c = 1.0, p = 0.0;
       bool early = false;
       for (int i = m - 1; i >= l; --i) {
         double f = s * e[i], b = c * e[i];
-        rr = hypot(f, g);
-        __syncthreads();
-        if (lane == 0) e[i + 1] = rr;
-        __syncthreads();
+        rr = sqrt(f * f + g * g);
+        e[i + 1] = rr;
         if (rr == 0.0) {
-          if (lane == 0) {
-            d[i + 1] -= p;
-            e[m] = 0.0;
-          }
-          __syncthreads();
+          d[i + 1] -= p;
+          e[m] = 0.0;
           early = true;
           break;
         }
@@ -140,9 +152,7 @@ __global__ __launch_bounds__(64) void k_lz_tri(const RepDev* __restrict__ reps, 
         g = d[i + 1] - p;
         rr = (d[i] - g) * s + 2.0 * c * b;
         p = s * rr;
-        __syncthreads();
-        if (lane == 0) d[i + 1] = g + p;
-        __syncthreads();
+        d[i + 1] = g + p;
         g = c * rr - b;
         if (lane < n) {
           double qa = Q[(i + 1) * 64 + lane], qb = Q[i * 64 + lane];
@@ -151,37 +161,31 @@ __global__ __launch_bounds__(64) void k_lz_tri(const RepDev* __restrict__ reps, 
         }
       }
       if (early) continue;
-      __syncthreads();
-      if (lane == 0) {
-        d[l] -= p;
-        e[l] = g;
-        e[m] = 0.0;
-      }
-      __syncthreads();
+      d[l] -= p;
+      e[l] = g;
+      e[m] = 0.0;
     }
   }
   __syncthreads();
-  double b0 = sc[S_B0];
+  const double b0 = sc[S_B0];
   double ci = 0.0;
   if (lane < n) {
     for (int k = 0; k < n; ++k) ci += Q[k * 64 + lane] * sqrt(fmax(d[k], 0.0)) * Q[k * 64 + 0];
     ci *= b0;
   }
-  double cprev = (lane < n - 1) ? sc[S_CP + lane] : 0.0;
+  const int nprev = (int)sc[S_ERR + 1];
+  double cprev = (lane < nprev) ? sc[S_CP + lane] : 0.0;
   double num = warp_sum((ci - cprev) * (ci - cprev));
   double den = warp_sum(ci * ci);
-  double err = sqrt(num / den);
+  double err = (nprev > 0) ? sqrt(num / den) : 1.0;
   if (lane < n) {
     sc[S_C + lane] = ci;
     sc[S_CP + lane] = ci;
   }
   if (lane == 0) {
-    sc[S_BE + j - 1] = bj;
-    sc[S_ERR] = (j > 1) ? err : 1.0;
-    sc[S_IT] = (double)j;
-    iters[r] = j;
-    bdiv[r] = bj;
-    if ((j > 1 && err < tol) || j >= maxit || !(bj > 0.0)) done[r] = (j > 1 && err < tol) ? 1 : 2;
+    sc[S_ERR] = err;
+    sc[S_ERR + 1] = (double)n;
+    if ((nprev > 0 && err < tol) || j >= maxit || breakdown) done[r] = (nprev > 0 && err < tol) ? 1 : 2;
   }
 }
 
@@ -237,10 +241,11 @@ void lz_rep_reduce(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, 
     k_rep_reduce<<<nrep, 64, 0, st>>>(reps, rep_begin, partial, done, out, stride, off, take_sqrt, copy_to);
 }
 void lz_orth(hipStream_t st, const RepDev* reps, const int* slot_rep, int slot_begin, int nslots, double* w,
-             const double* vj, const double* vjm1, const double* scal, int j, const int* done, double* partial) {
+             const double* vj, const double* vjm1, double* scal, int j, const int* done, const double* partial_alpha,
+             double* partial) {
   if (nslots > 0)
     k_lz_orth<<<(nslots + 255) / 256, 256, 0, st>>>(reps, slot_rep, slot_begin, nslots, w, vj, vjm1, scal, j, done,
-                                                     partial);
+                                                     partial_alpha, partial);
 }
 void lz_tri(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, const double* partial, double* scal, int j,
             double tol, int maxit, int* done, double* bdiv, int* iters) {
@@ -255,4 +260,13 @@ void lz_tea(hipStream_t st, const int* slot_rep, int slot_begin, int nslots, con
             const double* diag, const double* beta, const double* C, double* y) {
   if (nslots > 0)
     k_tea<<<(nslots + 255) / 256, 256, 0, st>>>(slot_rep, slot_begin, nslots, mz, z, diag, beta, C, y);
+}
+
+// zero the "previous check" length before a new solve
+__global__ void k_lz_reset(double* scal, int rep_begin, int nrep) {
+  int r = rep_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < rep_begin + nrep) scal[(size_t)r * SCAL_STRIDE + S_ERR + 1] = 0.0;
+}
+void lz_reset_checks(hipStream_t st, double* scal, int rep_begin, int nrep) {
+  if (nrep > 0) k_lz_reset<<<(nrep + 63) / 64, 64, 0, st>>>(scal, rep_begin, nrep);
 }

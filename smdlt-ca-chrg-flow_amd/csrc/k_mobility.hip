@@ -451,7 +451,7 @@ __global__ __launch_bounds__(256) void k_mob_final(MobArgs a) {
   const T alpha = (T)R.alpha, rc2 = (T)R.rc2;
   const int m = a.nrpy[s];
   for (int k = lane; k < m; k += 64) {
-    int j = a.lrpy[(size_t)k * a.nst + s];
+    int j = a.lrpy[(size_t)s * a.cap_rpy + k];
     double dx = a.rx[j] - xi, dy = a.ry[j] - yi, dz = a.rz[j] - zi;
     min_image_reduced(R, dx, dy, dz);
     T tx = (T)dx, ty = (T)dy, tz = (T)dz;

@@ -168,7 +168,7 @@ __global__ __launch_bounds__(256) void k_build_rpy(const RepDev* __restrict__ re
           int pos = cr + __popcll(m & ((1ull << lane) - 1ull));
           if (hit) {
             if (pos < cap)
-              lrpy[(size_t)pos * nst + i] = j;
+              lrpy[(size_t)i * cap + pos] = j;
             else
               atomicOr(overflow, 1);
           }
@@ -178,40 +178,66 @@ __global__ __launch_bounds__(256) void k_build_rpy(const RepDev* __restrict__ re
   if (lane == 0) nrpy[i] = min(cr, cap);
 }
 
+// ascending bitonic sort of one value per lane across a wavefront
+__device__ __forceinline__ int wave_sort(int v, int lane) {
+  for (int k = 2; k <= 64; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      int o = __shfl_xor(v, j, 64);
+      bool keep_min = ((lane & j) == 0) == ((lane & k) == 0);
+      v = keep_min ? min(v, o) : max(v, o);
+    }
+  return v;
+}
+
 // The reference's Verlet list (rep_flags bit0): all j with |getNID(i,j)| < rv,
-// sorted.  Candidates come from the fresh-enough mobility list when its cutoff
-// covers rv + 1 (the pair distance under the delx shift differs from the true
-// one by less than 1), otherwise from the cell grid.
-__global__ void k_build_lj(const RepDev* __restrict__ reps, const int* __restrict__ slot_rep, int slot_begin,
-                           int nslots, const double* rx, const double* ry, const double* rz, const int* cell_start,
-                           const int* cell_cnt, const int* cell_beads, const int* rep_flags, int nst,
-                           const int* nrpy, const int* lrpy, int cap_lj, int* nlj, int* llj, int* overflow) {
-  int i = slot_begin + blockIdx.x * blockDim.x + threadIdx.x;
+// ascending.  One wavefront per bead: lanes test candidates (from the mobility
+// list when its cutoff covers rv + 1 -- the delx shift changes a pair distance
+// by less than 1 -- otherwise from the cell grid), hits are compacted in LDS
+// and sorted (bitonic across the wavefront, or serially past 64 hits).
+#define LJ_BUF 256
+__global__ __launch_bounds__(256) void k_build_lj(const RepDev* __restrict__ reps, const int* __restrict__ slot_rep,
+                                                  int slot_begin, int nslots, const double* rx, const double* ry,
+                                                  const double* rz, const int* cell_start, const int* cell_cnt,
+                                                  const int* cell_beads, const int* rep_flags, int nst,
+                                                  const int* nrpy, const int* lrpy, int cap_rpy, int cap_lj,
+                                                  int* nlj, int* llj, int* overflow) {
+  __shared__ int buf[4][LJ_BUF];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int i = slot_begin + blockIdx.x * 4 + wv;
   if (i >= slot_begin + nslots) return;
   int r = slot_rep[i];
   if (r < 0) return;
   const RepDev& R = reps[r];
   if (!R.active || !(rep_flags[r] & 1)) return;
   if (i - R.slot0 >= R.n) {
-    nlj[i] = 0;
+    if (lane == 0) nlj[i] = 0;
     return;
   }
   const double xi = rx[i], yi = ry[i], zi = rz[i];
   int cl = 0;
-  auto test = [&](int j) {
-    if (j == i) return;
-    double ox, oy, oz;
-    ref_nid(R, xi, yi, zi, rx[j], ry[j], rz[j], ox, oy, oz);
-    double rr;
-    {
+  auto test = [&](int j, bool valid) {
+    bool hit = false;
+    if (valid && j != i) {
+      double ox, oy, oz;
+      ref_nid(R, xi, yi, zi, rx[j], ry[j], rz[j], ox, oy, oz);
+      double rr;
+      {
 #pragma clang fp contract(off)
-      rr = sqrt(ox * ox + oy * oy + oz * oz);
+        rr = sqrt(ox * ox + oy * oy + oz * oz);
+      }
+      hit = rr < R.rv;
     }
-    if (rr < R.rv) sorted_insert(llj, nst, i, cl, cap_lj, j, overflow);
+    unsigned long long m = __ballot(hit);
+    int pos = cl + __popcll(m & ((1ull << lane) - 1ull));
+    if (hit && pos < LJ_BUF) buf[wv][pos] = j;
+    cl += __popcll(m);
   };
   if (R.rc2 >= (R.rv + 1.0) * (R.rv + 1.0)) {
     int m = nrpy[i];
-    for (int k = 0; k < m; ++k) test(lrpy[(size_t)k * nst + i]);
+    for (int k0 = 0; k0 < m; k0 += 64) {
+      int k = k0 + lane;
+      test(k < m ? lrpy[(size_t)i * cap_rpy + k] : -1, k < m);
+    }
   } else {
     int c[3], lo[3], hi[3];
     cell_coords(R, xi, yi, zi, c);
@@ -224,10 +250,33 @@ __global__ void k_build_lj(const RepDev* __restrict__ reps, const int* __restric
           int e = (c[2] + o3 + R.ncell[2]) % R.ncell[2];
           int id = R.cell0 + (e * R.ncell[1] + b) * R.ncell[0] + a;
           int beg = cell_start[id], cnt = cell_cnt[id];
-          for (int k = 0; k < cnt; ++k) test(cell_beads[beg + k]);
+          for (int k0 = 0; k0 < cnt; k0 += 64) {
+            int k = k0 + lane;
+            test(k < cnt ? cell_beads[beg + k] : -1, k < cnt);
+          }
         }
   }
-  nlj[i] = cl;
+  if (cl > LJ_BUF || cl > cap_lj) {
+    if (lane == 0) atomicOr(overflow, 1);
+    cl = min(cl, min(LJ_BUF, cap_lj));
+  }
+  if (cl <= 64) {
+    int v = (lane < cl) ? buf[wv][lane] : INT_MAX;
+    v = wave_sort(v, lane);
+    if (lane < cl) llj[(size_t)lane * nst + i] = v;
+  } else if (lane == 0) {
+    int* q = buf[wv];
+    for (int a = 1; a < cl; ++a) {
+      int x = q[a], b = a - 1;
+      while (b >= 0 && q[b] > x) {
+        q[b + 1] = q[b];
+        --b;
+      }
+      q[b + 1] = x;
+    }
+    for (int a = 0; a < cl; ++a) llj[(size_t)a * nst + i] = q[a];
+  }
+  if (lane == 0) nlj[i] = cl;
 }
 
 void launch_build_lists(hipStream_t st, const RepDev* reps, const int* slot_rep, int slot_begin, int nslots,
@@ -237,9 +286,9 @@ void launch_build_lists(hipStream_t st, const RepDev* reps, const int* slot_rep,
   if (nslots <= 0) return;
   k_build_rpy<<<(nslots + 3) / 4, 256, 0, st>>>(reps, slot_rep, slot_begin, nslots, rx, ry, rz, cell_start, cell_cnt,
                                                 cell_beads, rep_flags, nslots_total, cap_rpy, nrpy, lrpy, overflow);
-  k_build_lj<<<(nslots + 127) / 128, 128, 0, st>>>(reps, slot_rep, slot_begin, nslots, rx, ry, rz, cell_start,
-                                                   cell_cnt, cell_beads, rep_flags, nslots_total, nrpy, lrpy, cap_lj,
-                                                   nlj, llj, overflow);
+  k_build_lj<<<(nslots + 3) / 4, 256, 0, st>>>(reps, slot_rep, slot_begin, nslots, rx, ry, rz, cell_start, cell_cnt,
+                                               cell_beads, rep_flags, nslots_total, nrpy, lrpy, cap_rpy, cap_lj, nlj,
+                                               llj, overflow);
 }
 
 // Fill kernels (used instead of memset so that captured graphs replay correctly).
