@@ -6,7 +6,7 @@ Workload (the metric's configuration): semidilute polyelectrolyte under shear,
 notebooks' Wi = 175 edot), dt = 5e-4, 50 % of the beads charged +-1 with zero
 net charge, Debye length 5a, Bjerrum length 1a, Hookean springs of the reference
 defaults.  One step = remap + neighbour check + forces + device noise +
-M.F + Lanczos M^(1/2) z (tol 1e-6, <= 30 iterations) + position update,
+M.F + Lanczos M^(1/2) z (tol 1e-6, <= 128 iterations) + position update,
 mixed precision (FP32 pair/grid/FFT sums, FP64 Lanczos), replayed as a hipGraph.
 
 Also reported: the same workload on the plain per-replica FP64 path (no graph,
@@ -30,38 +30,199 @@ import smdlt_abi as S  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 
 
-def chains(nb, nc, L, seed, bond=2.05, min_sep=2.0, tries=1000):
-    """Seeded self-avoiding random walks like initChains (S/main26.c:420-470): bond
-    2.05a, a new bead is redrawn while it lies within min_sep of any earlier bead
-    (minimum image in the cubic box); positions wrapped into the box."""
+def grow_chains(specs, seed, bond=2.05, min_sep=2.0, back=5, patience=40):
+    """Seeded self-avoiding random walks for several boxes at once, like the
+    reference's initChains (S/main26.c:398-470): bond 2.05a, a bead is redrawn
+    while it lies within min_sep of any bead already placed in its box (minimum
+    image, cubic box), positions wrapped into [-L/2, L/2).  specs: list of
+    (nb, nc, L); returns one (nb*nc, 3) array per box.  All chains grow in step
+    (one proposal per unfinished chain per round, vectorised); placed beads sit
+    in a cell grid of side >= min_sep, so a proposal is checked against 27 cells.
+    A chain that fails `patience` times in a row takes back its last `back`
+    beads (self-trapping)."""
     rng = np.random.default_rng(seed)
-    pos = np.zeros((nb * nc, 3))
-    k = 0
-    for c in range(nc):
-        for j in range(nb):
-            for _ in range(tries):
-                if j == 0:
-                    p = rng.uniform(-L / 2, L / 2, 3)
-                else:
-                    th = rng.uniform(0, 2 * np.pi)
-                    ph = np.arccos(rng.uniform(-1, 1))
-                    p = pos[k - 1] + bond * np.array([np.sin(ph) * np.cos(th), np.sin(ph) * np.sin(th), np.cos(ph)])
-                p = p - L * np.round(p / L)
-                if k == 0:
-                    break
-                d = pos[:k] - p
-                d -= L * np.round(d / L)
-                if np.min(np.einsum("ij,ij->i", d, d)) >= min_sep * min_sep:
-                    break
-            pos[k] = p
-            k += 1
-    return pos
+    ch_rep, ch_nb, ch_first, rep_off = [], [], [], []
+    off = 0
+    for r, (nb, nc, L) in enumerate(specs):
+        rep_off.append(off)
+        for c in range(nc):
+            ch_rep.append(r)
+            ch_nb.append(nb)
+            ch_first.append(off + c * nb)
+        off += nb * nc
+    ch_rep, ch_nb, ch_first = map(np.array, (ch_rep, ch_nb, ch_first))
+    pos = np.zeros((off, 3))
+    Ls = np.array([s[2] for s in specs], float)
+    ncell = np.maximum(3, np.floor(Ls / min_sep).astype(np.int64))
+    cbase = np.concatenate([[0], np.cumsum(ncell ** 3)])
+    CAP = 5  # points >= min_sep apart in a cell of side >= min_sep: at most 4
+    occ = np.full((cbase[-1], CAP), -1, np.int32)
+    cnt = np.zeros(cbase[-1], np.int64)
+    bead_cell = np.full(off, -1, np.int64)
+    offs = np.array([(a, b, c) for a in (-1, 0, 1) for b in (-1, 0, 1) for c in (-1, 0, 1)])
+    jn = np.zeros(len(ch_nb), np.int64)
+    fails = np.zeros(len(ch_nb), np.int64)
+    while True:
+        act = np.nonzero(jn < ch_nb)[0]
+        if not act.size:
+            break
+        rep = ch_rep[act]
+        L = Ls[rep][:, None]
+        j = jn[act]
+        v = rng.normal(size=(act.size, 3))
+        v /= np.linalg.norm(v, axis=1)[:, None]
+        prev = pos[ch_first[act] + np.maximum(j - 1, 0)]
+        p = np.where((j == 0)[:, None], (rng.random((act.size, 3)) - 0.5) * L, prev + bond * v)
+        p -= L * np.round(p / L)
+        n = ncell[rep]
+        f = np.floor((p / L + 0.5) * n[:, None]).astype(np.int64) % n[:, None]
+        nbc = (f[:, None, :] + offs[None]) % n[:, None, None]
+        flat = cbase[rep][:, None] + (nbc[..., 0] * n[:, None] + nbc[..., 1]) * n[:, None] + nbc[..., 2]
+        cand = occ[flat].reshape(act.size, -1)
+        valid = cand >= 0
+        d = pos[np.where(valid, cand, 0)] - p[:, None, :]
+        d -= L[:, None] * np.round(d / L[:, None])
+        ok = ~(valid & (np.einsum("ijk,ijk->ij", d, d) < min_sep * min_sep)).any(axis=1)
+        # proposals of one box against each other (act is grouped by box)
+        pos_in = np.arange(act.size) - np.searchsorted(rep, rep, side="left")
+        G = int(pos_in.max()) + 1
+        tab = np.full((len(specs), G), -1)
+        tab[rep, pos_in] = np.arange(act.size)
+        mate = tab[rep]
+        mv = (mate >= 0) & (np.arange(G)[None, :] < pos_in[:, None])
+        dd = p[np.where(mv, mate, 0)] - p[:, None, :]
+        dd -= L[:, None] * np.round(dd / L[:, None])
+        ok &= ~(mv & (np.einsum("ijk,ijk->ij", dd, dd) < min_sep * min_sep)).any(axis=1)
+        qs = np.nonzero(ok)[0]
+        b = ch_first[act[qs]] + j[qs]
+        c = flat[qs, 13]
+        pos[b] = p[qs]
+        bead_cell[b] = c
+        o = np.argsort(c, kind="stable")
+        cs = c[o]
+        slot = cnt[cs] + np.arange(len(cs)) - np.searchsorted(cs, cs, side="left")
+        if slot.size and slot.max() >= CAP:
+            raise RuntimeError("grow_chains: cell capacity exceeded")
+        occ[cs, slot] = b[o]
+        np.add.at(cnt, cs, 1)
+        jn[act[ok]] += 1
+        fails[act[ok]] = 0
+        bad = act[~ok]
+        fails[bad] += 1
+        for g in bad[fails[bad] > patience]:
+            k = min(back, jn[g])
+            for bb in range(ch_first[g] + jn[g] - k, ch_first[g] + jn[g]):
+                cc = bead_cell[bb]
+                row = occ[cc]
+                i = int(np.nonzero(row[:cnt[cc]] == bb)[0][0])
+                row[i] = row[cnt[cc] - 1]
+                row[cnt[cc] - 1] = -1
+                cnt[cc] -= 1
+                bead_cell[bb] = -1
+            jn[g] -= k
+            fails[g] = 0
+    return [pos[o:o + s[0] * s[1]] for o, s in zip(rep_off, specs)]
+
+
+def chains(nb, nc, L, seed):
+    return grow_chains([(nb, nc, L)], seed)[0]
 
 
 def c2_params(prec):
     return S.default_params(nb=200, nc=40, L=80.0, flow_type=S.FLOW_SHEAR, edot=50.0 / 175.0, dt=5e-4,
                             spring_fene=0, kappas=200.0, qmax=2.0, epsilon=0.1, lambda_b=1.0, lambda_d=5.0,
-                            precision=prec, lanczos_tol=1e-6, lanczos_maxit=30, seed=0x5EED0000)
+                            precision=prec, lanczos_tol=1e-6, lanczos_maxit=128, seed=0x5EED0000)
+
+
+def c4_specs(nrep, seed=11):
+    """Uneven replicas: chain length U[50, 1000], chain count U{1..20}, box at
+    volume fraction 0.02 (L = (N 4 pi / 3 / 0.02)^(1/3)), shear or planar
+    extension with probability 1/2, Wi U[1, 100] (edot = Wi / 175)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for r in range(nrep):
+        nb = int(rng.integers(50, 1001))
+        nc = int(rng.integers(1, 21))
+        L = (nb * nc * 4.0 * np.pi / 3.0 / 0.02) ** (1.0 / 3.0)
+        flow = S.FLOW_SHEAR if rng.random() < 0.5 else S.FLOW_EXTENSION
+        wi = rng.uniform(1.0, 100.0)
+        out.append(dict(nb=nb, nc=nc, L=L, flow=flow, edot=wi / 175.0))
+    return out
+
+
+def to_lab_frame(ctx, specs, pos):
+    """Chains are grown in a cube with cubic minimum image.  At t = 0 the
+    Kraynik-Reinelt lattice is that cube rotated by theta0 (the reference's
+    31.7 degrees), so a planar-extension replica's conformation is the grown one
+    rotated into the lab frame; tiled with the cubic lattice instead it would
+    overlap its own images across the rotated faces."""
+    lat = ctx.update_lattice(0)
+    out = []
+    for r, sp in enumerate(specs):
+        p = pos[r]
+        if sp["flow"] == S.FLOW_EXTENSION:
+            l1x, l1y = lat[r, 0], lat[r, 1]
+            ct, st = l1x / np.hypot(l1x, l1y), l1y / np.hypot(l1x, l1y)
+            p = np.stack([p[:, 0] * ct - p[:, 1] * st, p[:, 0] * st + p[:, 1] * ct, p[:, 2]], 1)
+        out.append(p)
+    return out
+
+
+def c4_params(specs, prec):
+    return [S.default_params(nb=sp["nb"], nc=sp["nc"], L=sp["L"], flow_type=sp["flow"], edot=sp["edot"], dt=5e-4,
+                             spring_fene=0, kappas=200.0, qmax=2.0, epsilon=0.1, precision=prec, lanczos_tol=1e-6,
+                             lanczos_maxit=128, seed=0x5EED0000 + r) for r, sp in enumerate(specs)]
+
+
+def batched_leg(nrep, steps, warmup, plain_reps):
+    """Batched replicas (one launch per stage, graph-replayed) against the plain
+    per-replica FP64 path (same kernels in FP64, one replica at a time, eager,
+    host-checked Lanczos exit) timed on the first plain_reps replicas."""
+    specs = c4_specs(nrep)
+    t0 = time.perf_counter()
+    pos = grow_chains([(sp["nb"], sp["nc"], sp["L"]) for sp in specs], 2)
+    ctx = S.Context(c4_params(specs, S.PREC_MIXED))
+    pos = to_lab_frame(ctx, specs, pos)
+    allp = np.concatenate(pos)
+    ntot = len(allp)
+    print(f"[bench] batched: {nrep} replicas, {ntot} beads, chains grown in {time.perf_counter() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    ctx.upload_positions(allp[:, 0], allp[:, 1], allp[:, 2])
+    ctx.run(0, warmup, use_graph=True)
+    ctx.sync()
+    t1 = time.perf_counter()
+    code = ctx.run(warmup, steps, use_graph=True)
+    ctx.sync()
+    el = time.perf_counter() - t1
+    iters = ctx.last_iters()
+    batched = ntot * steps / el
+    print(f"[bench] batched {batched:.4g} bead-steps/s ({1e3 * el / steps:.1f} ms/step, Lanczos iters "
+          f"{int(iters.min())}..{int(iters.max())})", file=sys.stderr, flush=True)
+    ctx.close()
+    # plain path on a sample of the replicas
+    sub = list(range(min(plain_reps, nrep)))
+    c64 = S.Context(c4_params(specs, S.PREC_FP64))
+    c64.upload_positions(allp[:, 0], allp[:, 1], allp[:, 2])
+    nsub = sum(len(pos[r]) for r in sub)
+    for r in sub:  # first step builds the lists
+        c64.select_replicas(r, r + 1)
+        c64.step(0)
+    c64.sync()
+    t2 = time.perf_counter()
+    for r in sub:
+        c64.select_replicas(r, r + 1)
+        c64.step(1)
+    c64.sync()
+    plain = nsub / (time.perf_counter() - t2)
+    c64.close()
+    print(f"[bench] plain per-replica fp64 {plain:.4g} bead-steps/s over {len(sub)} replicas ({nsub} beads)",
+          file=sys.stderr, flush=True)
+    return {"workload": f"{nrep} uneven replicas (chains U[50,1000] x U{{1..20}}, volume fraction 0.02, "
+                        f"shear or planar extension, Wi U[1,100]), no charges",
+            "replicas": nrep, "beads": ntot, "steps": steps, "ms_per_step": 1e3 * el / steps, "value": batched,
+            "lanczos_iters_last_step": [int(iters.min()), int(iters.max())], "lanczos_converged": code == 0,
+            "plain_fp64_value": plain, "plain_sample": f"{len(sub)} replicas, {nsub} beads, 1 step each",
+            "speedup_vs_plain": batched / plain}
 
 
 def c2_charges(n, seed=7):
@@ -93,6 +254,9 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--batched-replicas", type=int, default=64)
+    ap.add_argument("--batched-steps", type=int, default=3)
+    ap.add_argument("--plain-replicas", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -110,7 +274,7 @@ def main():
     ctx.run(0, args.warmup, use_graph=True)
     ctx.sync()
     t0 = time.perf_counter()
-    ctx.run(args.warmup, args.steps, use_graph=True)
+    code = ctx.run(args.warmup, args.steps, use_graph=True)
     ctx.sync()
     el = time.perf_counter() - t0
     value = n * args.steps / el
@@ -165,6 +329,10 @@ def main():
         c64.close()
         print(f"[bench] plain fp64 path {plain:.4g} bead-steps/s", file=sys.stderr, flush=True)
 
+    batched = None
+    if args.batched_replicas > 0:
+        batched = batched_leg(args.batched_replicas, args.batched_steps, 1, args.plain_replicas)
+
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -189,7 +357,8 @@ def main():
         "config": {"workload": "semidilute polyelectrolyte under shear: 40x200 beads, L=80, Wi=50, dt=5e-4, "
                                "50% charged, lambda_D=5, lambda_B=1",
                    "grid": list(cfg["grid"]), "spline_order": cfg["order"], "rc": cfg["rc"],
-                   "ewald_alpha": round(cfg["alpha"], 5), "lanczos_iters": iters, "parallelism": "1 GPU"},
+                   "ewald_alpha": round(cfg["alpha"], 5), "lanczos_tol": 1e-6, "lanczos_iters_last_step": iters,
+                   "lanczos_converged": code == 0, "parallelism": "1 GPU"},
         "mf_rel_err_vs_fp64": mf_err,
         "plain_fp64_value": plain,
         "speedup_vs_plain": (value / plain) if plain else None,
@@ -197,6 +366,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "bytes_per_launch": kb[dom]},
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     print(json.dumps(out))
 

@@ -171,6 +171,10 @@ int smdlt_step(smdlt_ctx* ctx, uint64_t t, const double* noise);
 int smdlt_run(smdlt_ctx* ctx, uint64_t t0, int nsteps, int use_graph);
 /* Per-replica Lanczos iteration counts of the last step (or brownian call). */
 int smdlt_last_iters(const smdlt_ctx* ctx, int* iters);
+/* Per-replica outcome of the last Lanczos solve: iters, status (1 converged,
+ * 2 stopped at the iteration limit or on a breakdown, 0 not run) and the last
+ * relative change |c_j - c_(j-1)| / |c_j|.  Any argument may be NULL. */
+int smdlt_lanczos_status(smdlt_ctx* ctx, int* iters, int* status, double* change);
 
 /* chargeHop (S/main26.c:1926-2390) for every replica at one MC step.  mode =
  * SMDLT_MC_*.  uniforms: per replica a host stream of n_uniforms values (layout

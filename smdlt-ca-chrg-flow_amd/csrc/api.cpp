@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <functional>
 #include <vector>
 
 #include "common.h"
@@ -288,10 +289,27 @@ struct smdlt_ctx {
   double stage_ms[ST_COUNT] = {0};
   int stage_cnt[ST_COUNT] = {0};
   // graph
+  // one step = head graph (remap .. M.F, Lanczos iterations 1..LZ_CHUNK), then
+  // chunk graphs of LZ_CHUNK further iterations launched while any replica is
+  // unconverged, then the tail graph (combination, position update)
   hipGraphExec_t gexec = nullptr;
+  hipGraphExec_t gtail = nullptr;
+  std::vector<hipGraphExec_t> gchunks;
+  int* h_flags = nullptr;  // pinned: done[nrep], iters[nrep], overflow
+  double* saved = nullptr; // positions and displacement trackers at the start of a graph step
+  bool force_rebuild = false;
+  int expect_iters = 0;    // largest iteration count of the previous graph step
   bool graph_ready = false;
   int cur_rep_begin = 0, cur_rep_end = 0;
 };
+
+static void destroy_graphs(smdlt_ctx* c) {
+  if (c->gexec) hipGraphExecDestroy(c->gexec);
+  if (c->gtail) hipGraphExecDestroy(c->gtail);
+  for (hipGraphExec_t g : c->gchunks) hipGraphExecDestroy(g);
+  c->gexec = c->gtail = nullptr;
+  c->gchunks.clear();
+}
 
 static hipEvent_t get_event(smdlt_ctx* c) {
   if (!c->pool.empty()) {
@@ -715,6 +733,11 @@ int smdlt_create(const smdlt_params* params, int n_replicas, int device, smdlt_c
   st |= alloc_dev(&c->done, n_replicas);
   st |= alloc_dev(&c->iters, n_replicas);
   st |= alloc_dev(&c->mc_err, 4);
+  st |= alloc_dev(&c->saved, 9 * (size_t)c->nst);
+  if (hipHostMalloc((void**)&c->h_flags, 8 * (size_t)n_replicas + 4, hipHostMallocDefault) != hipSuccess) {
+    c->h_flags = nullptr;
+    st |= SMDLT_E_NOMEM;
+  }
   if (st) {
     smdlt_destroy(c);
     return SMDLT_E_NOMEM;
@@ -837,8 +860,9 @@ void smdlt_destroy(smdlt_ctx* c) {
   if (!c) return;
   hipSetDevice(c->device);
   if (c->stream) hipStreamSynchronize(c->stream);
-  if (c->gexec) hipGraphExecDestroy(c->gexec);
-  void* ptrs[] = {c->d_reps, c->d_slot_rep, c->d_seeds, c->rx, c->ry, c->rz, c->dxv, c->dyv, c->dzv, c->dsp, c->f,
+  destroy_graphs(c);
+  if (c->h_flags) hipHostFree(c->h_flags);
+  void* ptrs[] = {c->saved, c->d_reps, c->d_slot_rep, c->d_seeds, c->rx, c->ry, c->rz, c->dxv, c->dyv, c->dzv, c->dsp, c->f,
                   c->z, c->mf, c->bz, c->tmp, c->tmp2, c->nrpy, c->lrpy, c->nlj, c->llj, c->overflow, c->d_err,
                   c->cell_of, c->cell_cnt, c->cell_start, c->cell_fill, c->cell_beads, c->rep_flags, c->out2, c->aff,
                   c->partial2, c->grid, c->bsp, c->tw, c->tx, c->ty, c->tz, c->bricks, c->bbase, c->brick_cnt, c->brick_start,
@@ -1086,6 +1110,8 @@ static int neighbors_internal(smdlt_ctx* c, int force) {
   int s0 = a.slot0, ns = b.slot0 + pad_slots(b.n) - s0;
   int cb = a.cell0, ce = b.cell0 + c->max_w_cells[re - 1];
   launch_disp_max(c->stream, c->d_reps, rb, re - rb, c->dxv, c->dyv, c->dzv, c->dsp, c->out2);
+  if (c->force_rebuild) force = 3;
+  c->force_rebuild = false;
   launch_nl_check(c->stream, c->d_reps, rb, re - rb, c->out2, c->aff, force | (c->lists_built ? 0 : 3), c->rep_flags);
   launch_fill_i32(c->stream, c->cell_cnt + cb, 0, ce - cb);
   launch_fill_i32(c->stream, c->cell_fill + cb, 0, ce - cb);
@@ -1115,10 +1141,7 @@ static int grow_lists(smdlt_ctx* c) {
   if (alloc_dev(&c->lrpy, (size_t)c->cap_rpy * c->nst) || alloc_dev(&c->llj, (size_t)c->cap_lj * c->nst))
     return SMDLT_E_NOMEM;
   HCK(hipMemset(c->overflow, 0, 4));
-  if (c->gexec) {
-    hipGraphExecDestroy(c->gexec);
-    c->gexec = nullptr;
-  }
+  destroy_graphs(c);
   return 1;
 }
 
@@ -1322,42 +1345,69 @@ int smdlt_mobility_apply(smdlt_ctx* c, const double* v, double* u) {
 }
 
 // Lanczos on the active replica range: y = M^(1/2) z (device pointers).
-static int brownian_internal(smdlt_ctx* c, const double* z, double* y, double tol, int maxit, bool host_exit) {
+// Lanczos pieces: start (|z|, flags), iteration j, combination y = V c.
+static void lz_start(smdlt_ctx* c, const double* z) {
   int rb = c->cur_rep_begin, re = c->cur_rep_end;
-  const RepDev& a = c->rh[rb];
-  const RepDev& b = c->rh[re - 1];
-  int s0 = a.slot0, ns = b.slot0 + pad_slots(b.n) - s0;
+  int s0 = c->rh[rb].slot0, ns = c->rh[re - 1].slot0 + pad_slots(c->rh[re - 1].n) - s0;
+  StageScope sc(c, ST_LANCZOS);
+  launch_fill_i32(c->stream, c->done + rb, 0, re - rb);
+  lz_reset_checks(c->stream, c->scal, rb, re - rb);
+  lz_wave_dot(c->stream, c->d_reps, c->d_slot_rep, s0, ns, z, nullptr, nullptr, c->partial);
+  lz_rep_reduce(c->stream, c->d_reps, rb, re - rb, c->partial, nullptr, c->scal, SCAL_STRIDE, 4 * LZ_MAXIT, 1,
+                c->bdiv);
+}
+
+static void lz_iteration(smdlt_ctx* c, const double* z, int j, double tol, int maxit) {
+  int rb = c->cur_rep_begin, re = c->cur_rep_end;
+  int s0 = c->rh[rb].slot0, ns = c->rh[re - 1].slot0 + pad_slots(c->rh[re - 1].n) - s0;
   size_t n3 = 3 * (size_t)c->nst;
-  {
-    StageScope sc(c, ST_LANCZOS);
-    launch_fill_i32(c->stream, c->done + rb, 0, re - rb);
-    lz_reset_checks(c->stream, c->scal, rb, re - rb);
-    lz_wave_dot(c->stream, c->d_reps, c->d_slot_rep, s0, ns, z, nullptr, nullptr, c->partial);
-    lz_rep_reduce(c->stream, c->d_reps, rb, re - rb, c->partial, nullptr, c->scal, SCAL_STRIDE, 4 * LZ_MAXIT, 1,
-                  c->bdiv);
+  const double* vin = (j == 1) ? z : c->W;
+  double* vj = c->V + (size_t)(j - 1) * n3;
+  const double* vjm1 = (j > 1) ? c->V + (size_t)(j - 2) * n3 : nullptr;
+  mobility_internal(c, vin, c->W, c->bdiv, vj, nullptr, c->done);
+  StageScope sc(c, ST_LANCZOS);
+  lz_wave_dot(c->stream, c->d_reps, c->d_slot_rep, s0, ns, vj, c->W, c->done, c->partial2);
+  lz_orth(c->stream, c->d_reps, c->d_slot_rep, s0, ns, c->W, vj, vjm1, c->scal, j, c->done, c->partial2, c->partial);
+  lz_tri(c->stream, c->d_reps, rb, re - rb, c->partial, c->scal, j, tol, maxit, c->done, c->bdiv, c->iters);
+}
+
+static void lz_finish(smdlt_ctx* c, double* y) {
+  int rb = c->cur_rep_begin, re = c->cur_rep_end;
+  int s0 = c->rh[rb].slot0, ns = c->rh[re - 1].slot0 + pad_slots(c->rh[re - 1].n) - s0;
+  StageScope sc(c, ST_LANCZOS);
+  lz_combine(c->stream, c->d_reps, c->d_slot_rep, s0, ns, c->V, c->nst, c->scal, c->iters, y);
+}
+
+// copy done flags and iteration counts of the current replica range to the
+// host; returns 1 when every replica has stopped
+static int lz_poll(smdlt_ctx* c, int* max_iters, int* overflow = nullptr) {
+  int rb = c->cur_rep_begin, re = c->cur_rep_end, nr = re - rb;
+  HCK(hipMemcpyAsync(c->h_flags, c->done + rb, 4 * nr, hipMemcpyDeviceToHost, c->stream));
+  HCK(hipMemcpyAsync(c->h_flags + c->nrep, c->iters + rb, 4 * nr, hipMemcpyDeviceToHost, c->stream));
+  if (overflow) HCK(hipMemcpyAsync(c->h_flags + 2 * c->nrep, c->overflow, 4, hipMemcpyDeviceToHost, c->stream));
+  HCK(hipStreamSynchronize(c->stream));
+  if (overflow) *overflow = c->h_flags[2 * c->nrep];
+  bool all = true;
+  int mx = 0;
+  for (int r = 0; r < nr; ++r) {
+    all = all && c->h_flags[r] != 0;
+    mx = std::max(mx, c->h_flags[c->nrep + r]);
   }
+  if (max_iters) *max_iters = mx;
+  return all ? 1 : 0;
+}
+
+static int brownian_internal(smdlt_ctx* c, const double* z, double* y, double tol, int maxit, bool host_exit) {
+  lz_start(c, z);
   for (int j = 1; j <= maxit; ++j) {
-    const double* vin = (j == 1) ? z : c->W;
-    double* vj = c->V + (size_t)(j - 1) * n3;
-    const double* vjm1 = (j > 1) ? c->V + (size_t)(j - 2) * n3 : nullptr;
-    mobility_internal(c, vin, c->W, c->bdiv, vj, nullptr, c->done);
-    {
-      StageScope sc(c, ST_LANCZOS);
-      lz_wave_dot(c->stream, c->d_reps, c->d_slot_rep, s0, ns, vj, c->W, c->done, c->partial2);
-      lz_orth(c->stream, c->d_reps, c->d_slot_rep, s0, ns, c->W, vj, vjm1, c->scal, j, c->done, c->partial2, c->partial);
-      lz_tri(c->stream, c->d_reps, rb, re - rb, c->partial, c->scal, j, tol, maxit, c->done, c->bdiv, c->iters);
-    }
+    lz_iteration(c, z, j, tol, maxit);
     if (host_exit && j >= 2) {
-      std::vector<int> d(re - rb);
-      HCK(hipMemcpyAsync(d.data(), c->done + rb, 4 * (re - rb), hipMemcpyDeviceToHost, c->stream));
-      HCK(hipStreamSynchronize(c->stream));
-      bool all = true;
-      for (int v : d) all = all && (v != 0);
+      int all = lz_poll(c, nullptr);
+      if (all < 0) return all;
       if (all) break;
     }
   }
-  StageScope sc(c, ST_LANCZOS);
-  lz_combine(c->stream, c->d_reps, c->d_slot_rep, s0, ns, c->V, c->nst, c->scal, c->iters, y);
+  lz_finish(c, y);
   HCK(hipGetLastError());
   return 0;
 }
@@ -1397,6 +1447,23 @@ int smdlt_brownian(smdlt_ctx* c, const double* z, double* y, double tol, int max
   st = scatter_aos(c, c->bz, y);
   collect_timers(c);
   return st ? st : conv;
+}
+
+int smdlt_lanczos_status(smdlt_ctx* c, int* iters, int* status, double* change) {
+  if (!c) return SMDLT_E_ARG;
+  HCK(hipSetDevice(c->device));
+  std::vector<int> it(c->nrep), dn(c->nrep);
+  std::vector<double> sc((size_t)SCAL_STRIDE * c->nrep);
+  HCK(hipMemcpyAsync(it.data(), c->iters, 4 * c->nrep, hipMemcpyDeviceToHost, c->stream));
+  HCK(hipMemcpyAsync(dn.data(), c->done, 4 * c->nrep, hipMemcpyDeviceToHost, c->stream));
+  HCK(hipMemcpyAsync(sc.data(), c->scal, 8 * sc.size(), hipMemcpyDeviceToHost, c->stream));
+  HCK(hipStreamSynchronize(c->stream));
+  for (int r = 0; r < c->nrep; ++r) {
+    if (iters) iters[r] = it[r];
+    if (status) status[r] = dn[r];
+    if (change) change[r] = sc[(size_t)r * SCAL_STRIDE + 4 * LZ_MAXIT + 2];
+  }
+  return 0;
 }
 
 int smdlt_last_iters(const smdlt_ctx* c, int* iters) {
@@ -1526,6 +1593,81 @@ int smdlt_step(smdlt_ctx* c, uint64_t t, const double* noise) {
   return conv;
 }
 
+// capture the work that body() enqueues on the context stream as a graph
+static int capture_graph(smdlt_ctx* c, hipGraphExec_t* out, const std::function<int()>& body) {
+  hipGraph_t graph;
+  bool prof = c->prof;
+  c->prof = false;
+  HCK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  int st = body();
+  hipError_t e = hipStreamEndCapture(c->stream, &graph);
+  c->prof = prof;
+  if (st) {
+    if (e == hipSuccess) hipGraphDestroy(graph);
+    return st;
+  }
+  HCK(e);
+  e = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  HCK(e);
+  return 0;
+}
+
+#define LZ_CHUNK 8
+
+// positions and displacement trackers <-> the saved copy (a graph step whose
+// neighbour lists overflowed is undone and redone eagerly with larger lists)
+static void save_state(smdlt_ctx* c, bool restore) {
+  double* cur[] = {c->rx, c->ry, c->rz, c->dxv, c->dyv, c->dzv, c->dsp};
+  long long len[] = {c->nst, c->nst, c->nst, c->nst, c->nst, c->nst, 3LL * c->nst};
+  long long off = 0;
+  for (int k = 0; k < 7; ++k) {
+    if (restore)
+      launch_copy_f64(c->stream, cur[k], c->saved + off, len[k]);
+    else
+      launch_copy_f64(c->stream, c->saved + off, cur[k], len[k]);
+    off += len[k];
+  }
+}
+
+// The graphs of one step for the current replica range (see smdlt_ctx).
+static int build_step_graphs(smdlt_ctx* c, uint64_t t) {
+  int rb = c->cur_rep_begin, re = c->cur_rep_end;
+  int s0 = c->rh[rb].slot0, ns = c->rh[re - 1].slot0 + pad_slots(c->rh[re - 1].n) - s0;
+  const smdlt_params& p = c->P[rb];
+  const double tol = p.lanczos_tol > 0 ? p.lanczos_tol : 1e-6;
+  const int first = std::min(c->maxit, LZ_CHUNK);
+  int st = capture_graph(c, &c->gexec, [&]() -> int {
+    save_state(c, false);
+    launch_remap(c->stream, c->d_reps, c->d_slot_rep, s0, ns, c->rx, c->ry, c->rz, c->dxv, c->dyv, c->dzv, c->dsp, 1);
+    int s = neighbors_internal(c, 0);
+    if (!s) s = forces_internal(c);
+    if (s) return s;
+    launch_noise(c->stream, c->d_slot_rep, c->d_reps, s0, ns, c->d_seeds, t, c->z);
+    mobility_internal(c, c->f, c->mf, nullptr, nullptr, nullptr, nullptr);
+    lz_start(c, c->z);
+    for (int j = 1; j <= first; ++j) lz_iteration(c, c->z, j, tol, c->maxit);
+    return 0;
+  });
+  if (st) return st;
+  for (int j0 = first; j0 < c->maxit; j0 += LZ_CHUNK) {
+    hipGraphExec_t g = nullptr;
+    st = capture_graph(c, &g, [&]() -> int {
+      for (int j = j0 + 1; j <= std::min(c->maxit, j0 + LZ_CHUNK); ++j) lz_iteration(c, c->z, j, tol, c->maxit);
+      return 0;
+    });
+    if (st) return st;
+    c->gchunks.push_back(g);
+  }
+  return capture_graph(c, &c->gtail, [&]() -> int {
+    lz_finish(c, c->bz);
+    launch_update_positions(c->stream, c->d_reps, c->d_slot_rep, s0, ns, c->rx, c->ry, c->rz, c->mf, c->bz, c->dxv,
+                            c->dyv, c->dzv, c->dsp, 1);
+    c->pos_version++;
+    return 0;
+  });
+}
+
 int smdlt_run(smdlt_ctx* c, uint64_t t0, int nsteps, int use_graph) {
   if (!c || nsteps < 0) return SMDLT_E_ARG;
   if (!c->have_pos) return SMDLT_E_STATE;
@@ -1534,42 +1676,50 @@ int smdlt_run(smdlt_ctx* c, uint64_t t0, int nsteps, int use_graph) {
   for (int k = 0; k < nsteps; ++k) {
     uint64_t t = t0 + k;
     if (!use_graph || !c->lists_built) {  // the first step builds the lists after its remap
-      st = step_internal(c, t, nullptr, false);
+      st = step_internal(c, t, nullptr, !c->lists_built);
       if (st) return st;
       continue;
     }
-    // lattice for this step, then replay the captured step body
+    // lattice for this step, then replay the captured step
+    std::vector<HostLattice> lat_before = c->lat;
+    std::vector<double> ct_before = c->ct_prev, st_before = c->st_prev;
     st = lattice_to_device(c, t);
     if (st) return st;
     if (!c->gexec) {
-      hipGraph_t graph;
-      bool prof = c->prof;
-      c->prof = false;
-      HCK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-      int rb = c->cur_rep_begin, re = c->cur_rep_end;
-      const RepDev& a = c->rh[rb];
-      const RepDev& b = c->rh[re - 1];
-      int s0 = a.slot0, ns = b.slot0 + pad_slots(b.n) - s0;
-      launch_remap(c->stream, c->d_reps, c->d_slot_rep, s0, ns, c->rx, c->ry, c->rz, c->dxv, c->dyv, c->dzv, c->dsp,
-                   1);
-      st = neighbors_internal(c, 0);
-      if (!st) st = forces_internal(c);
-      launch_noise(c->stream, c->d_slot_rep, c->d_reps, s0, ns, c->d_seeds, t, c->z);
-      mobility_internal(c, c->f, c->mf, nullptr, nullptr, nullptr, nullptr);
-      const smdlt_params& p = c->P[rb];
-      if (!st) st = brownian_internal(c, c->z, c->bz, p.lanczos_tol > 0 ? p.lanczos_tol : 1e-6, c->maxit, false);
-      launch_update_positions(c->stream, c->d_reps, c->d_slot_rep, s0, ns, c->rx, c->ry, c->rz, c->mf, c->bz, c->dxv,
-                              c->dyv, c->dzv, c->dsp, 1);
-      c->pos_version++;
-      hipError_t e = hipStreamEndCapture(c->stream, &graph);
-      c->prof = prof;
-      if (st) return st;
-      HCK(e);
-      HCK(hipGraphInstantiate(&c->gexec, graph, nullptr, nullptr, 0));
-      hipGraphDestroy(graph);
+      st = build_step_graphs(c, t);
+      if (st) {
+        destroy_graphs(c);
+        return st;
+      }
     }
     StageScope sc(c, ST_MOB);
     HCK(hipGraphLaunch(c->gexec, c->stream));
+    // chunks the previous step needed go in unchecked; then poll between chunks
+    int j0 = std::min(c->maxit, LZ_CHUNK), ch = 0, mx = 0, ov = 0;
+    while (ch < (int)c->gchunks.size() && j0 < c->expect_iters) {
+      HCK(hipGraphLaunch(c->gchunks[ch++], c->stream));
+      j0 += LZ_CHUNK;
+    }
+    for (;;) {
+      int all = lz_poll(c, &mx, &ov);
+      if (all < 0) return all;
+      if (ov || all || ch >= (int)c->gchunks.size()) break;
+      HCK(hipGraphLaunch(c->gchunks[ch++], c->stream));
+    }
+    if (ov) {  // truncated lists: undo, grow, redo this step eagerly
+      save_state(c, true);
+      int g = grow_lists(c);
+      if (g < 0) return g;
+      c->force_rebuild = true;
+      c->lat = lat_before;
+      c->ct_prev = ct_before;
+      c->st_prev = st_before;
+      st = step_internal(c, t, nullptr, true);
+      if (st) return st;
+      continue;
+    }
+    c->expect_iters = mx;
+    HCK(hipGraphLaunch(c->gtail, c->stream));
   }
   HCK(hipStreamSynchronize(c->stream));
   st = check_err(c);
@@ -1664,10 +1814,7 @@ int smdlt_upload_charges(smdlt_ctx* c, const int* nq, const int* track, const in
   }
   c->nq_total = tot;
   c->max_nq = mx;
-  if (c->gexec) {
-    hipGraphExecDestroy(c->gexec);
-    c->gexec = nullptr;
-  }
+  destroy_graphs(c);
   return upload_reps(c);
 }
 
@@ -1757,10 +1904,7 @@ int smdlt_mc_features(smdlt_ctx* c, int replica, float* features, int* cand_from
 int smdlt_select_replicas(smdlt_ctx* c, int begin, int end) {
   if (!c || begin < 0 || end > c->nrep || begin >= end) return SMDLT_E_ARG;
   set_active(c, begin, end);
-  if (c->gexec) {
-    hipGraphExecDestroy(c->gexec);
-    c->gexec = nullptr;
-  }
+  destroy_graphs(c);
   return upload_reps(c);
 }
 

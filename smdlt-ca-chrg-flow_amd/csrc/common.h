@@ -9,7 +9,7 @@
 #define FFT_TILE 1024       // complex elements per component per FFT workgroup
 #define FFT_THREADS 256
 #define MAX_ORDER 8         // largest B-spline order
-#define LZ_MAXIT 64         // largest Lanczos iteration count supported
+#define LZ_MAXIT 128        // largest Lanczos iteration count supported
 
 // Per-replica descriptor, uploaded to the device every step (lattice fields
 // change with t).  All lattice quantities are computed on the host in FP64 with
@@ -234,6 +234,7 @@ void launch_build_lists(hipStream_t s, const RepDev* reps, const int* slot_rep, 
 
 void launch_fill_i32(hipStream_t s, int* p, int v, long long n);
 void launch_fill_f64(hipStream_t s, double* p, double v, long long n);
+void launch_copy_f64(hipStream_t s, double* dst, const double* src, long long n);
 void launch_fill_f32(hipStream_t s, float* p, float v, long long n);
 
 // k_forces.hip

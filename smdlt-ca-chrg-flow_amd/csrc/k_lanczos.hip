@@ -4,9 +4,9 @@
 //   y = V_j c_j.
 // This replaces the TEA product of updateHI (S/main26.c:787-804).  Reductions
 // are two-level and ordered (per-wavefront partials, then one wavefront per
-// replica), so results do not depend on scheduling.  The small tridiagonal
-// eigenproblem is solved by implicit-shift QL on one wavefront per replica,
-// each lane owning one row of the eigenvector matrix in LDS.
+// replica), so results do not depend on scheduling.  The tridiagonal square
+// root is evaluated by a quadrature of the inverse-square-root integral on one
+// wavefront per replica (k_lz_tri).
 #include "common.h"
 
 #define SCAL_STRIDE (4 * LZ_MAXIT + 8)
@@ -87,12 +87,22 @@ __global__ void k_lz_orth(const RepDev* __restrict__ reps, const int* __restrict
   if (lane == 0) partial[s >> 6] = v;
 }
 
-// b_j from the partials; then, on check iterations (j even and >= 4, the last
-// iteration, or a breakdown), c_j = b0 T_j^(1/2) e1 by implicit-shift QL and the
-// stopping test |c_j - c_prev| < tol |c_j| against the previous check.  Block =
-// one wavefront per replica; every lane runs the scalar QL recurrence on its
-// own copy (registers / LDS written with identical values) and rotates its own
-// row of the eigenvector matrix, so the recurrence needs no barriers.
+// b_j from the partials, then c_j = b0 T_j^(1/2) e1 and the stopping test
+// |c_j - [c_{j-1}; 0]| < tol |c_j|.  One wavefront per replica.
+//
+// T^(1/2) e1 = T T^(-1/2) e1 with T^(-1/2) = (2/pi) int_R e^s (T + e^(2s) I)^(-1) ds,
+// the substitution t = e^s of the classical integral (2/pi) int_0^inf (T + t^2)^(-1) dt.
+// For one eigenvalue the integrand is lambda^(-1/2) sech(s - ln(lambda)/2)/pi, so
+// the trapezoid rule with step h has relative error ~exp(-pi^2/h) for every
+// eigenvalue alike (poles of sech at +-i pi/2), and truncating at U beyond the
+// spectrum costs ~(2/pi) e^-U.  h = 0.4, U = 24 give ~1e-10.  The spectrum is
+// bracketed by Gershgorin above and by a 64-point Sturm multisection below.
+// Each node is one SPD tridiagonal solve (T + sigma I) x = e1 (backward pivots,
+// then the first column of the inverse forwards); lanes take nodes round-robin
+// and the weighted columns are summed in lane order, so the result does not
+// depend on scheduling.  About 130 nodes: two or three solves per lane.
+#define LZ_QH 0.4
+#define LZ_QU 24.0
 __global__ __launch_bounds__(64) void k_lz_tri(const RepDev* __restrict__ reps, int rep_begin, const double* partial,
                                                double* scal, int j, double tol, int maxit, int* done, double* bdiv,
                                                int* iters) {
@@ -106,86 +116,100 @@ __global__ __launch_bounds__(64) void k_lz_tri(const RepDev* __restrict__ reps, 
   v = warp_sum(v);
   const double bj = sqrt(v);
   const bool breakdown = !(bj > 1e-300);
-  const bool check = (j >= 4 && (j % 2) == 0) || j >= maxit || breakdown;
   if (lane == 0) {
     sc[S_BE + j - 1] = bj;
     bdiv[r] = bj;
     iters[r] = j;
     sc[S_IT] = (double)j;
   }
-  if (!check) return;
-  __shared__ double d[LZ_MAXIT], e[LZ_MAXIT];
-  __shared__ double Q[LZ_MAXIT * 64];
-  const int n = j;
-  __syncthreads();
-  if (lane < n) {
-    d[lane] = sc[S_AL + lane];
-    e[lane] = (lane < n - 1) ? sc[S_BE + lane] : 0.0;
+  if (!(sc[S_B0] > 0.0)) {  // z = 0: y = 0 exactly
+    if (lane == 0) {
+      sc[S_C] = 0.0;
+      sc[S_ERR] = 0.0;
+      done[r] = 1;
+    }
+    return;
   }
-  for (int k = 0; k < n; ++k) Q[k * 64 + lane] = (k == lane) ? 1.0 : 0.0;
+  __shared__ double d[LZ_MAXIT], e[LZ_MAXIT], e2[LZ_MAXIT], u[LZ_MAXIT];
+  __shared__ double ipiv[LZ_MAXIT * 64];  // 1 / backward pivot, [i][lane]
+  __shared__ double acc[LZ_MAXIT * 64];   // weighted first columns, [i][lane]
+  const int n = j;
+  for (int i = lane; i < n; i += 64) {
+    d[i] = sc[S_AL + i];
+    e[i] = (i < n - 1) ? sc[S_BE + i] : 0.0;
+    e2[i] = e[i] * e[i];
+  }
   __syncthreads();
-  for (int l = 0; l < n; ++l) {
-    for (int iter = 0; iter < 64; ++iter) {
-      int m;
-      for (m = l; m < n - 1; ++m) {
-        double dd = fabs(d[m]) + fabs(d[m + 1]);
-        if (fabs(e[m]) <= 2.220446049250313e-16 * dd) break;
-      }
-      if (m == l) break;
-      double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-      double rr = sqrt(g * g + 1.0);
-      g = d[m] - d[l] + e[l] / (g + copysign(rr, g));
-      double s = 1.0, c = 1.0, p = 0.0;
-      bool early = false;
-      for (int i = m - 1; i >= l; --i) {
-        double f = s * e[i], b = c * e[i];
-        rr = sqrt(f * f + g * g);
-        e[i + 1] = rr;
-        if (rr == 0.0) {
-          d[i + 1] -= p;
-          e[m] = 0.0;
-          early = true;
-          break;
-        }
-        s = f / rr;
-        c = g / rr;
-        g = d[i + 1] - p;
-        rr = (d[i] - g) * s + 2.0 * c * b;
-        p = s * rr;
-        d[i + 1] = g + p;
-        g = c * rr - b;
-        if (lane < n) {
-          double qa = Q[(i + 1) * 64 + lane], qb = Q[i * 64 + lane];
-          Q[(i + 1) * 64 + lane] = s * qb + c * qa;
-          Q[i * 64 + lane] = c * qb - s * qa;
-        }
-      }
-      if (early) continue;
-      d[l] -= p;
-      e[l] = g;
-      e[m] = 0.0;
+  // Gershgorin upper bound of the spectrum
+  double hi = 0.0;
+  for (int i = lane; i < n; i += 64) hi = fmax(hi, d[i] + fabs(e[i]) + (i > 0 ? fabs(e[i - 1]) : 0.0));
+  hi = warp_max(hi);
+  // lower bound: largest of hi * 10^(-12 (l+1)/64) with no eigenvalue below it
+  const double x = hi * exp(-27.631021115928547 * (lane + 1) / 64.0);
+  int neg = 0;
+  {
+    double p = d[0] - x;
+    neg += p < 0.0;
+    for (int i = 1; i < n; ++i) {
+      if (p == 0.0) p = 1e-300;
+      p = d[i] - x - e2[i - 1] / p;
+      neg += p < 0.0;
+    }
+  }
+  unsigned long long zero = __ballot(neg == 0);
+  double lo = zero ? hi * exp(-27.631021115928547 * (__ffsll((long long)zero)) / 64.0) : hi * 1e-12;
+  const double slo = 0.5 * log(lo) - LZ_QU;
+  const int nodes = (int)ceil((0.5 * log(hi) + LZ_QU - slo) / LZ_QH) + 1;
+  for (int i = 0; i < n; ++i) acc[i * 64 + lane] = 0.0;
+  for (int k = lane; k < nodes; k += 64) {
+    const double s = slo + k * LZ_QH;
+    const double sig = exp(2.0 * s);
+    const double wk = LZ_QH * 0.6366197723675814 * exp(s);
+    double rp = d[n - 1] + sig;
+    double ip = 1.0 / rp;
+    ipiv[(n - 1) * 64 + lane] = ip;
+    for (int i = n - 2; i >= 0; --i) {
+      rp = d[i] + sig - e2[i] * ip;
+      ip = 1.0 / rp;
+      ipiv[i * 64 + lane] = ip;
+    }
+    double xi = ip;  // (T + sig)^-1 e1, first entry
+    acc[lane] += wk * xi;
+    for (int i = 0; i < n - 1; ++i) {
+      xi = -e[i] * xi * ipiv[(i + 1) * 64 + lane];
+      acc[(i + 1) * 64 + lane] += wk * xi;
     }
   }
   __syncthreads();
+  for (int i = lane; i < n; i += 64) {
+    double t = 0.0;
+    for (int l = 0; l < 64; ++l) t += acc[i * 64 + l];
+    u[i] = t;
+  }
+  __syncthreads();
   const double b0 = sc[S_B0];
-  double ci = 0.0;
-  if (lane < n) {
-    for (int k = 0; k < n; ++k) ci += Q[k * 64 + lane] * sqrt(fmax(d[k], 0.0)) * Q[k * 64 + 0];
-    ci *= b0;
-  }
   const int nprev = (int)sc[S_ERR + 1];
-  double cprev = (lane < nprev) ? sc[S_CP + lane] : 0.0;
-  double num = warp_sum((ci - cprev) * (ci - cprev));
-  double den = warp_sum(ci * ci);
-  double err = (nprev > 0) ? sqrt(num / den) : 1.0;
-  if (lane < n) {
-    sc[S_C + lane] = ci;
-    sc[S_CP + lane] = ci;
+  double num = 0.0, den = 0.0;
+  for (int i = lane; i < n; i += 64) {
+    double ci = d[i] * u[i];
+    if (i > 0) ci += e[i - 1] * u[i - 1];
+    if (i < n - 1) ci += e[i] * u[i + 1];
+    ci *= b0;
+    double cp = (i < nprev) ? sc[S_C + i] : 0.0;
+    num += (ci - cp) * (ci - cp);
+    den += ci * ci;
+    sc[S_CP + i] = ci;
   }
+  num = warp_sum(num);
+  den = warp_sum(den);
+  __syncthreads();
+  for (int i = lane; i < n; i += 64) sc[S_C + i] = sc[S_CP + i];
+  double err = (nprev > 0) ? sqrt(num / den) : 1.0;
   if (lane == 0) {
     sc[S_ERR] = err;
     sc[S_ERR + 1] = (double)n;
-    if ((nprev > 0 && err < tol) || j >= maxit || breakdown) done[r] = (nprev > 0 && err < tol) ? 1 : 2;
+    bool conv = nprev > 0 && err < tol;
+    if (conv || j >= maxit || breakdown) done[r] = conv ? 1 : 2;
   }
 }
 

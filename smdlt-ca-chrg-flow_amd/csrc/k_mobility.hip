@@ -159,7 +159,7 @@ __global__ void k_scale_v(MobArgs a) {
   if (r < 0) return;
   const RepDev& R = a.reps[r];
   if (!R.active || (a.done && a.done[r]) || s - R.slot0 >= R.n) return;
-  double sc = a.beta ? 1.0 / a.beta[r] : 1.0;
+  double sc = a.beta ? (a.beta[r] > 0.0 ? 1.0 / a.beta[r] : 0.0) : 1.0;  // z = 0 gives y = 0
   for (int c = 0; c < 3; ++c) a.vout[3 * s + c] = a.v[3 * s + c] * sc;
 }
 
@@ -412,7 +412,7 @@ __global__ __launch_bounds__(256) void k_mob_final(MobArgs a) {
     return;
   }
   const double* vv = a.vout ? a.vout : a.v;
-  const double sc = (a.beta && !a.vout) ? 1.0 / a.beta[r] : 1.0;
+  const double sc = (a.beta && !a.vout) ? (a.beta[r] > 0.0 ? 1.0 / a.beta[r] : 0.0) : 1.0;
   const double xi = a.rx[s], yi = a.ry[s], zi = a.rz[s];
   CompSum<T> acc[3];
   // reciprocal part: lane l < P^2 interpolates stencil row (l % P, l / P)

@@ -301,6 +301,13 @@ static int fill_blocks(long long n) {
   long long b = (n + 255) / 256;
   return (int)(b > 4096 ? 4096 : (b < 1 ? 1 : b));
 }
+__global__ void k_copy_f64(double* dst, const double* src, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+void launch_copy_f64(hipStream_t st, double* dst, const double* src, long long n) {
+  if (n > 0) k_copy_f64<<<fill_blocks(n), 256, 0, st>>>(dst, src, n);
+}
 void launch_fill_i32(hipStream_t st, int* p, int v, long long n) {
   if (n > 0) k_fill<int><<<fill_blocks(n), 256, 0, st>>>(p, v, n);
 }

@@ -81,6 +81,7 @@ def lib():
         L.smdlt_step.argtypes = [C.c_void_p, C.c_uint64, _dp]
         L.smdlt_run.argtypes = [C.c_void_p, C.c_uint64, C.c_int, C.c_int]
         L.smdlt_last_iters.argtypes = [C.c_void_p, _ip]
+        L.smdlt_lanczos_status.argtypes = [C.c_void_p, _ip, _ip, _dp]
         L.smdlt_set_profiling.argtypes = [C.c_void_p, C.c_int]
         L.smdlt_stage_times.argtypes = [C.c_void_p, _dp, _ip, C.c_int]
         L.smdlt_stage_name.restype = C.c_char_p
@@ -264,6 +265,14 @@ class Context:
         it = np.zeros(self.nrep, np.int32)
         _chk(lib().smdlt_last_iters(self.h, _ptr(it, C.c_int)), "last_iters")
         return it
+
+    def lanczos_status(self):
+        it = np.zeros(self.nrep, np.int32)
+        stt = np.zeros(self.nrep, np.int32)
+        ch = np.zeros(self.nrep)
+        _chk(lib().smdlt_lanczos_status(self.h, _ptr(it, C.c_int), _ptr(stt, C.c_int), _ptr(ch, C.c_double)),
+             "lanczos_status")
+        return it, stt, ch
 
     def set_profiling(self, on):
         _chk(lib().smdlt_set_profiling(self.h, int(on)), "set_profiling")

@@ -227,6 +227,26 @@ def test_lanczos_matches_dense_sqrt(prec):
     assert y @ y == pytest.approx(z @ Md @ z, rel=2e-5)
 
 
+def test_lanczos_long_solve_matches_dense_sqrt():
+    # an unreachable tolerance runs the solve to 100 iterations (tridiagonal
+    # square roots of order > 64); the result still equals the dense square root
+    # (300 overlapping-walk beads in L = 20: condition number ~130)
+    ctx, lat, pos = mob_case(1, L=20.0, nb=60, nc=5, prec=S.PREC_FP64, lanczos_maxit=100)
+    Md = O.ewald_dense(O.cell_of(lat), pos, 0.5)
+    z = np.random.default_rng(13).normal(size=pos.size)
+    y, it, res, code = ctx.brownian(z, 1e-15, 100)
+    assert code == -4 and it[0] == 100
+    w, Q = np.linalg.eigh(Md)
+    ys = Q @ (np.sqrt(w) * (Q.T @ z))
+    assert rel(y, ys) < 2e-5  # the PME product itself is accurate to ~1e-5 (mobility_tol)
+
+
+def test_lanczos_zero_noise_gives_zero():
+    ctx, lat, pos = mob_case(1)
+    y, it, res, code = ctx.brownian(np.zeros(pos.size), 1e-6, 30)
+    assert code == 0 and np.all(y == 0.0)
+
+
 def test_tea_matches_oracle():
     ctx, lat, pos = mob_case(1)
     Md = O.ewald_dense(O.cell_of(lat), pos, 0.5)
