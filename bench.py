@@ -199,6 +199,12 @@ def batched_leg(nrep, steps, warmup, plain_reps):
     print(f"[bench] batched {batched:.4g} bead-steps/s ({1e3 * el / steps:.1f} ms/step, Lanczos iters "
           f"{int(iters.min())}..{int(iters.max())})", file=sys.stderr, flush=True)
     ctx.close()
+    out = {"workload": f"{nrep} uneven replicas (chains U[50,1000] x U{{1..20}}, volume fraction 0.02, "
+                       f"shear or planar extension, Wi U[1,100]), no charges",
+           "replicas": nrep, "beads": ntot, "steps": steps, "ms_per_step": 1e3 * el / steps, "value": batched,
+           "lanczos_iters_last_step": [int(iters.min()), int(iters.max())], "lanczos_converged": code == 0}
+    if plain_reps <= 0:
+        return out
     # plain path on a sample of the replicas
     sub = list(range(min(plain_reps, nrep)))
     c64 = S.Context(c4_params(specs, S.PREC_FP64))
@@ -217,12 +223,9 @@ def batched_leg(nrep, steps, warmup, plain_reps):
     c64.close()
     print(f"[bench] plain per-replica fp64 {plain:.4g} bead-steps/s over {len(sub)} replicas ({nsub} beads)",
           file=sys.stderr, flush=True)
-    return {"workload": f"{nrep} uneven replicas (chains U[50,1000] x U{{1..20}}, volume fraction 0.02, "
-                        f"shear or planar extension, Wi U[1,100]), no charges",
-            "replicas": nrep, "beads": ntot, "steps": steps, "ms_per_step": 1e3 * el / steps, "value": batched,
-            "lanczos_iters_last_step": [int(iters.min()), int(iters.max())], "lanczos_converged": code == 0,
-            "plain_fp64_value": plain, "plain_sample": f"{len(sub)} replicas, {nsub} beads, 1 step each",
-            "speedup_vs_plain": batched / plain}
+    out.update({"plain_fp64_value": plain, "plain_sample": f"{len(sub)} replicas, {nsub} beads, 1 step each",
+                "speedup_vs_plain": batched / plain})
+    return out
 
 
 def c2_charges(n, seed=7):
@@ -254,6 +257,10 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--workload", choices=["all", "c2", "batched"], default="all",
+                    help="c2: only the 8000-bead shear line; batched: only the replica ensemble")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only copy 256 MiB device-to-device three times (HBM counter calibration)")
     ap.add_argument("--batched-replicas", type=int, default=64)
     ap.add_argument("--batched-steps", type=int, default=3)
     ap.add_argument("--plain-replicas", type=int, default=8)
@@ -262,6 +269,19 @@ def main():
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "replicas shard one per GPU by design but no multi-GPU path is built", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        import torch
+        a = torch.empty(64 << 20, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1.0)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": a.numel() * 4, "copies": 3}))
+        return
+    if args.workload == "batched":
+        print(json.dumps({"batched": batched_leg(args.batched_replicas, args.batched_steps, 1, args.plain_replicas)}))
+        return
 
     pos = chains(200, 40, 80.0, 1)
     n = len(pos)
@@ -288,17 +308,17 @@ def main():
     st = ctx.stage_times()
     ctx.set_profiling(False)
     G = cfg["grid"][0] * cfg["grid"][1] * cfg["grid"][2]
-    nbr = n * 120  # mobility-list entries (estimated from the density within rc+skin)
+    nbr = ctx.mobility_config()["list_entries"]  # real-space pair-list entries of the current list
     kb = kernel_bytes(n, G, nbr, cfg["order"])
     kern = {k: v for k, v in st.items() if k.startswith("k_") and v[1] > 0}
     dom = max(kern, key=lambda k: kern[k][0])
     avg_ms = kern[dom][0] / kern[dom][1]
     achieved = kb[dom] / (avg_ms * 1e-3) / 1e9
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "c2_traffic.json")
-    if os.path.exists(tf):
+    tf = os.path.join(ROOT, "profiles", "hbm_traffic_pmc.json")
+    if os.path.exists(tf):  # bytes per dispatch from the FETCH_SIZE / WRITE_SIZE runs
         with open(tf) as fh:
-            traffic = json.load(fh).get(dom)
+            traffic = json.load(fh).get("c2", {}).get(dom)
 
     # M.F accuracy of the mixed path against a tight FP64 evaluation
     rng = np.random.default_rng(3)
@@ -330,7 +350,7 @@ def main():
         print(f"[bench] plain fp64 path {plain:.4g} bead-steps/s", file=sys.stderr, flush=True)
 
     batched = None
-    if args.batched_replicas > 0:
+    if args.batched_replicas > 0 and args.workload == "all":
         batched = batched_leg(args.batched_replicas, args.batched_steps, 1, args.plain_replicas)
 
     cpu = None
@@ -364,7 +384,8 @@ def main():
         "speedup_vs_plain": (value / plain) if plain else None,
         "kernel_ms_avg": {k: v[0] / v[1] for k, v in kern.items()},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "bytes_per_launch": kb[dom]},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "bytes_per_launch": kb[dom],
+                     "avg_launch_ms": avg_ms, "pair_list_entries": nbr},
         "cpu_baseline": cpu,
         "batched": batched,
     }

@@ -103,8 +103,9 @@ int smdlt_create(const smdlt_params* params, int n_replicas, int device, smdlt_c
 void smdlt_destroy(smdlt_ctx* ctx);
 int smdlt_num_beads(const smdlt_ctx* ctx, int replica); /* replica = -1: total */
 
-/* Effective mobility settings chosen for a replica: alpha, rc, skin, K1, K2, K3, order. */
-int smdlt_get_mobility_config(const smdlt_ctx* ctx, int replica, double out[7]);
+/* Effective mobility settings chosen for a replica: alpha, rc, skin, K1, K2, K3,
+ * order, and the number of real-space pair-list entries after the last build. */
+int smdlt_get_mobility_config(const smdlt_ctx* ctx, int replica, double out[8]);
 
 /* Host <-> device state.  Positions SoA over all beads (reference rx, ry, rz). */
 int smdlt_upload_positions(smdlt_ctx* ctx, const double* rx, const double* ry, const double* rz);

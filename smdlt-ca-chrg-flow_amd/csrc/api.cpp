@@ -879,9 +879,18 @@ void smdlt_destroy(smdlt_ctx* c) {
   delete c;
 }
 
-int smdlt_get_mobility_config(const smdlt_ctx* c, int r, double out[7]) {
+int smdlt_get_mobility_config(const smdlt_ctx* c, int r, double out[8]) {
   if (!c || r < 0 || r >= c->nrep || !out) return SMDLT_E_ARG;
   const RepDev& R = c->rh[r];
+  out[7] = 0.0;  // real-space list entries of the replica (after the last build)
+  if (c->lists_built && R.n > 0) {
+    HCK(hipSetDevice(c->device));
+    std::vector<int> cnt(R.n);
+    HCK(hipMemcpy(cnt.data(), c->nrpy + R.slot0, 4 * (size_t)R.n, hipMemcpyDeviceToHost));
+    double e = 0.0;
+    for (int v : cnt) e += v;
+    out[7] = e;
+  }
   out[0] = R.alpha;
   out[1] = sqrt(R.rc2);
   out[2] = R.skin;
