@@ -86,6 +86,10 @@ typedef struct smdlt_params {
   double lanczos_tol;      /* 0 = 1e-6 */
   int lanczos_maxit;       /* 0 = 30 */
   uint64_t seed;           /* device noise stream seed for this replica */
+  /* Metropolis-Hastings hop modes (SMDLT_MC_MH_*) */
+  double hop_radius;       /* candidate beads within this distance [a]; 0 = 3.0 */
+  double feature_radius;   /* neighbourhood of the proposal features [a]; 0 = 4.0 */
+  double proposal_floor;   /* learned mode: weight of the uniform part of q; 0 = 0.05 */
 } smdlt_params;
 
 typedef struct smdlt_ctx smdlt_ctx;
@@ -188,18 +192,26 @@ int smdlt_mc_sweep(smdlt_ctx* ctx, int mode, const float* uniforms, int n_unifor
 /* Screened-Coulomb energy per replica (the double loop of chargeHop, S/main26.c:1955-1970). */
 int smdlt_mc_energy(smdlt_ctx* ctx, double* energy);
 
-/* Learned proposal for SMDLT_MC_MH_LEARNED: a 16-64-64-1 MLP (BF16 weights,
- * FP32 accumulation) scoring each hop candidate.  weights: FP32 host array of
- * smdlt_mlp_num_params() values (W1,b1,W2,b2,W3,b3).  No reference counterpart. */
+/* Learned proposal for SMDLT_MC_MH_LEARNED: a 16-64-64-1 MLP with ReLU scoring
+ * each hop candidate; weights are rounded to BF16 (nearest even), arithmetic is
+ * FP32.  weights: FP32 host array of smdlt_mlp_num_params() values laid out
+ * W1[64][16], b1[64], W2[64][64], b2[64], W3[64], b3.  No reference counterpart
+ * (the reference's chargeHop proposes by a fixed rule, S/main26.c:1926-2390). */
 int smdlt_mlp_num_params(void);
 int smdlt_mlp_set_weights(smdlt_ctx* ctx, const float* weights);
-/* Forward + backward of the binary cross-entropy loss on a batch (features
- * [n][16] FP32, labels [n] in {0,1}); grad gets d loss / d params (FP32). */
+/* Forward + backward of the mean binary cross-entropy (with logits) on a batch:
+ * features [n][16] FP32, labels [n] in {0,1}; the weights are rounded to BF16 as
+ * in the proposal, the layer products run on the matrix cores (BF16 in, FP32
+ * accumulate).  grad gets d loss / d weights (FP32, smdlt_mlp_num_params()).
+ * features = labels = NULL reuses the batch of the previous call (same n), which
+ * stays on the device: a training loop uploads its data once. */
 int smdlt_mlp_loss_grad(smdlt_ctx* ctx, const float* weights, const float* features, const float* labels,
                         int n, float* loss, float* grad);
-/* Proposal features of every hop candidate of every charge (16 per candidate) as
- * the learned proposal computes them; returns the count in *n (capacity cap). */
-int smdlt_mc_features(smdlt_ctx* ctx, int replica, float* features, int* cand_from, int* cand_to, int cap,
+/* The proposed moves of the last SMDLT_MC_MH_* sweep of one replica (one per
+ * attempt, nq[replica] attempts): their 16 features, source and destination bead
+ * (destination -1: no candidate) and whether they were accepted.  *n = count;
+ * returns SMDLT_E_ARG if it exceeds cap.  Any output may be NULL. */
+int smdlt_mc_features(smdlt_ctx* ctx, int replica, float* features, int* from, int* to, int* accepted, int cap,
                       int* n);
 
 /* Per-stage device times of the last step, milliseconds (HIP events on the

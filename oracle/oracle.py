@@ -72,6 +72,14 @@ def lib():
             raise RuntimeError("oracle library missing: run `make -C oracle` (or __graft_entry__.build())")
         L = C.CDLL(LIB_PATH)
         L.orc_rng_seed.argtypes = [C.POINTER(RNG), C.c_int64]
+        L.orc_bf16_round.restype = C.c_float
+        L.orc_bf16_round.argtypes = [C.c_float]
+        L.orc_mh_features.argtypes = [C.POINTER(Lattice), C.POINTER(MHParams), _dp, _dp, _dp, C.c_void_p, C.c_int,
+                                      C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _dp]
+        L.orc_mlp_forward.restype = C.c_double
+        L.orc_mlp_forward.argtypes = [C.c_void_p, _dp]
+        L.orc_mh_sweep.argtypes = [C.POINTER(Lattice), C.POINTER(MHParams), _dp, _dp, _dp, C.c_int, C.c_void_p,
+                                   C.c_void_p, C.c_void_p, C.c_void_p, _fp, C.c_int, C.c_void_p, C.c_void_p]
         L.orc_ran1.argtypes = [C.POINTER(RNG)]
         L.orc_ran1.restype = C.c_float
         L.orc_gasdev.argtypes = [C.POINTER(RNG)]
@@ -339,6 +347,52 @@ class ChargeHop:
                                   barrier, barrier2, u, len(u), C.byref(cur),
                                   log.ctypes.data_as(C.c_void_p), C.byref(e0))
         return st, cur.value, log.reshape(-1, 6), e0.value
+
+
+# ---------------------------------------------------------------- MH charge hopping
+class MHParams(C.Structure):
+    _fields_ = [("n", C.c_int), ("nb", C.c_int), ("hop_radius", C.c_double), ("feature_radius", C.c_double),
+                ("floor_eps", C.c_double), ("lambda_b", C.c_double), ("lambda_d", C.c_double),
+                ("barrier", C.c_double), ("barrier2", C.c_double)]
+
+
+def mh_params(n, nb, hop_radius=3.0, feature_radius=4.0, floor_eps=0.05, lambda_b=1.0, lambda_d=5.0,
+              barrier=0.0, barrier2=0.0):
+    return MHParams(n, nb, hop_radius, feature_radius, floor_eps, lambda_b, lambda_d, barrier, barrier2)
+
+
+def bf16_round(w):
+    w = np.ascontiguousarray(w, np.float32)
+    return np.array([lib().orc_bf16_round(float(x)) for x in w], np.float32)
+
+
+def mh_features(lat, p, rx, ry, rz, qbead, src, dst, qk, clr=-1, set_=-1, ncand=0):
+    f = np.zeros(16)
+    qb = np.ascontiguousarray(qbead, np.int32)
+    lib().orc_mh_features(C.byref(lat), C.byref(p), rx, ry, rz, qb.ctypes.data_as(C.c_void_p), src, dst, qk, clr,
+                          set_, ncand, f)
+    return f
+
+
+def mlp_forward(w, f):
+    w = np.ascontiguousarray(w, np.float32)
+    return lib().orc_mlp_forward(w.ctypes.data_as(C.c_void_p), np.ascontiguousarray(f, np.float64))
+
+
+def mh_sweep(lat, p, rx, ry, rz, track, sign, qbead, w, u, features=False):
+    """Runs len(u)//3 attempts in place on track and qbead; returns (accepted, log, feats)."""
+    u = np.ascontiguousarray(u, np.float32)
+    na = len(u) // 3
+    log = np.zeros(6 * max(1, na), np.int32)
+    feats = np.zeros(16 * max(1, na), np.float32) if features else None
+    wp = None if w is None else np.ascontiguousarray(w, np.float32)
+    acc = lib().orc_mh_sweep(C.byref(lat), C.byref(p), rx, ry, rz, len(track), track.ctypes.data_as(C.c_void_p),
+                             np.ascontiguousarray(sign, np.int32).ctypes.data_as(C.c_void_p),
+                             qbead.ctypes.data_as(C.c_void_p),
+                             None if wp is None else wp.ctypes.data_as(C.c_void_p), u, na,
+                             log.ctypes.data_as(C.c_void_p),
+                             None if feats is None else feats.ctypes.data_as(C.c_void_p))
+    return acc, log.reshape(-1, 6)[:na], (feats.reshape(-1, 16)[:na] if features else None)
 
 
 def ref_run(params: dict, rx, ry, rz, nsteps, nthreads=0):

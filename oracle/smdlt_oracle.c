@@ -1398,3 +1398,207 @@ double orc_ref_run(const orc_ref_run_t* p, double* rx, double* ry, double* rz, i
   if (status) *status = st;
   return el;
 }
+
+/* ============================================================ Metropolis-Hastings hops */
+float orc_bf16_round(float x) {
+  uint32_t u;
+  memcpy(&u, &x, 4);
+  if ((u & 0x7f800000u) != 0x7f800000u) u += 0x7fffu + ((u >> 16) & 1u);
+  u &= 0xffff0000u;
+  float r;
+  memcpy(&r, &u, 4);
+  return r;
+}
+
+static double mh_dist(const orc_lattice_t* lat, const double* rx, const double* ry, const double* rz, int i, int j) {
+  double d[3];
+  orc_nid(lat, rx, ry, rz, i, j, d);
+  return sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+}
+
+static double mh_cos_at(const orc_lattice_t* lat, const double* rx, const double* ry, const double* rz, int b,
+                        int nb) {
+  int pos = b % nb;
+  if (pos == 0 || pos == nb - 1) return 0.0;
+  double u[3], v[3];
+  orc_nid(lat, rx, ry, rz, b - 1, b, u);
+  orc_nid(lat, rx, ry, rz, b + 1, b, v);
+  double uu = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]), vv = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  return (u[0] * v[0] + u[1] * v[1] + u[2] * v[2]) / (uu * vv);
+}
+
+void orc_mh_features(const orc_lattice_t* lat, const orc_mh_params_t* p, const double* rx, const double* ry,
+                     const double* rz, const int* qbead, int src, int dst, int qk, int clr, int set, int ncand,
+                     double f[16]) {
+  const int nb = p->nb;
+  const double R4 = p->feature_radius, kap = 1.0 / p->lambda_d;
+  for (int k = 0; k < 16; ++k) f[k] = 0.0;
+  f[0] = mh_dist(lat, rx, ry, rz, src, dst);
+  int same = (src / nb) == (dst / nb);
+  int gap = abs(dst - src);
+  f[1] = (same && gap == 1) ? 1.0 : 0.0;
+  f[2] = same ? (double)(gap < 20 ? gap : 20) / 20.0 : 1.0;
+  for (int m = 0; m < p->n; ++m) {
+    if (m == src || m == dst) continue;
+    double rdm = mh_dist(lat, rx, ry, rz, dst, m), rsm = mh_dist(lat, rx, ry, rz, src, m);
+    int qm = (m == clr) ? 0 : (m == set) ? qk : qbead[m];
+    if (qm != 0 && (rdm < R4 || rsm < R4))
+      f[3] += p->lambda_b * (double)(qk * qm) * (exp(-kap * rdm) / rdm - exp(-kap * rsm) / rsm);
+    if (rdm < R4) {
+      f[8] += 0.1;
+      if (qm != 0) {
+        f[4] += 1.0;
+        f[6] += (double)(qk * qm);
+      }
+    }
+    if (rsm < R4) {
+      f[9] += 0.1;
+      if (qm != 0) {
+        f[5] += 1.0;
+        f[7] += (double)(qk * qm);
+      }
+    }
+  }
+  f[10] = mh_cos_at(lat, rx, ry, rz, dst, nb);
+  f[11] = mh_cos_at(lat, rx, ry, rz, src, nb);
+  f[12] = (dst % nb == 0 || dst % nb == nb - 1) ? 1.0 : 0.0;
+  f[13] = (src % nb == 0 || src % nb == nb - 1) ? 1.0 : 0.0;
+  f[14] = (double)qk;
+  f[15] = (double)ncand / 10.0;
+}
+
+double orc_mlp_forward(const float* w, const double f[16]) {
+  const float *W1 = w, *b1 = W1 + 64 * 16, *W2 = b1 + 64, *b2 = W2 + 64 * 64, *W3 = b2 + 64, *b3 = W3 + 64;
+  double h1[64], h2[64];
+  for (int j = 0; j < 64; ++j) {
+    double a = b1[j];
+    for (int i = 0; i < 16; ++i) a += (double)W1[j * 16 + i] * f[i];
+    h1[j] = a > 0 ? a : 0;
+  }
+  for (int j = 0; j < 64; ++j) {
+    double a = b2[j];
+    for (int i = 0; i < 64; ++i) a += (double)W2[j * 64 + i] * h1[i];
+    h2[j] = a > 0 ? a : 0;
+  }
+  double s = b3[0];
+  for (int j = 0; j < 64; ++j) s += (double)W3[j] * h2[j];
+  return s;
+}
+
+/* candidates of src: uncharged beads (under clr/set) other than src within hop radius */
+static int mh_candidates(const orc_lattice_t* lat, const orc_mh_params_t* p, const double* rx, const double* ry,
+                         const double* rz, const int* qbead, int src, int clr, int set, int* out) {
+  int nc = 0;
+  double R2 = p->hop_radius * p->hop_radius;
+  for (int b = 0; b < p->n; ++b) {
+    if (b == src) continue;
+    int qb = (b == clr) ? 0 : (b == set) ? 1 : qbead[b];
+    if (qb != 0) continue;
+    double d[3];
+    orc_nid(lat, rx, ry, rz, src, b, d);
+    if (d[0] * d[0] + d[1] * d[1] + d[2] * d[2] < R2) out[nc++] = b;
+  }
+  return nc;
+}
+
+/* proposal probabilities over the candidates (FP64) */
+static void mh_proposal(const orc_lattice_t* lat, const orc_mh_params_t* p, const double* rx, const double* ry,
+                        const double* rz, const int* qbead, const float* w, int src, int qk, int clr, int set,
+                        const int* cand, int nc, double* q) {
+  if (!w) {
+    for (int c = 0; c < nc; ++c) q[c] = 1.0 / nc;
+    return;
+  }
+  double z = 0.0;
+  for (int c = 0; c < nc; ++c) {
+    double f[16];
+    orc_mh_features(lat, p, rx, ry, rz, qbead, src, cand[c], qk, clr, set, nc, f);
+    q[c] = 1.0 / (1.0 + exp(-orc_mlp_forward(w, f)));  /* predicted acceptance: Barker-type weight */
+    z += q[c];
+  }
+  for (int c = 0; c < nc; ++c) q[c] = (1.0 - p->floor_eps) * q[c] / z + p->floor_eps / nc;
+}
+
+int orc_mh_sweep(const orc_lattice_t* lat, const orc_mh_params_t* p, const double* rx, const double* ry,
+                 const double* rz, int nq, int* track, const int* sign, int* qbead, const float* w,
+                 const float* u, int nattempt, int* mlog, float* feats) {
+  int* cand = (int*)malloc(sizeof(int) * p->n);
+  double* q = (double*)malloc(sizeof(double) * p->n);
+  float* wr = NULL;
+  int nacc = 0;
+  const double kap = 1.0 / p->lambda_d;
+  if (w) {
+    int np = 16 * 64 + 64 + 64 * 64 + 64 + 64 + 1;
+    wr = (float*)malloc(sizeof(float) * np);
+    for (int i = 0; i < np; ++i) wr[i] = orc_bf16_round(w[i]);
+  }
+  for (int a = 0; a < nattempt; ++a) {
+    float u0 = u[3 * a], u1 = u[3 * a + 1], u2 = u[3 * a + 2];
+    int k = (int)(u0 * (float)nq);
+    if (k > nq - 1) k = nq - 1;
+    int x = track[k], qk = sign[k];
+    int* lg = mlog ? mlog + 6 * a : NULL;
+    float* fa = feats ? feats + 16 * a : NULL;
+    if (fa)
+      for (int i = 0; i < 16; ++i) fa[i] = 0.0f;
+    int nc = mh_candidates(lat, p, rx, ry, rz, qbead, x, -1, -1, cand);
+    if (nc == 0) {
+      if (lg) {
+        lg[0] = k; lg[1] = x; lg[2] = -1; lg[3] = -1; lg[4] = 0; lg[5] = 3;
+      }
+      continue;
+    }
+    mh_proposal(lat, p, rx, ry, rz, qbead, wr, x, qk, -1, -1, cand, nc, q);
+    double tot = 0.0;
+    for (int c = 0; c < nc; ++c) tot += q[c];
+    double t = (double)u1 * tot, cum = 0.0;
+    int pick = nc - 1;
+    for (int c = 0; c < nc; ++c) {
+      cum += q[c];
+      if (cum >= t) {
+        pick = c;
+        break;
+      }
+    }
+    int y = cand[pick];
+    double qf = q[pick];
+    if (fa) {
+      double f[16];
+      orc_mh_features(lat, p, rx, ry, rz, qbead, x, y, qk, -1, -1, nc, f);
+      for (int i = 0; i < 16; ++i) fa[i] = (float)f[i];
+    }
+    double de = 0.0;
+    for (int m = 0; m < nq; ++m) {
+      if (m == k) continue;
+      int bm = track[m];
+      double rym = mh_dist(lat, rx, ry, rz, y, bm), rxm = mh_dist(lat, rx, ry, rz, x, bm);
+      de += p->lambda_b * (double)(qk * sign[m]) * (exp(-kap * rym) / rym - exp(-kap * rxm) / rxm);
+    }
+    /* reverse proposal in the state after the move */
+    int nc2 = mh_candidates(lat, p, rx, ry, rz, qbead, y, x, y, cand);
+    mh_proposal(lat, p, rx, ry, rz, qbead, wr, y, qk, x, y, cand, nc2, q);
+    double qr = 0.0;
+    for (int c = 0; c < nc2; ++c)
+      if (cand[c] == x) qr = q[c];
+    double lnrho = -de + log(qr) - log(qf);
+    int adj = ((x / p->nb) == (y / p->nb)) && abs(y - x) == 1;
+    double ab = exp(-(adj ? p->barrier : p->barrier2));
+    double sr = exp(0.5 * lnrho);
+    double m1 = ab * sr > ab / sr ? ab * sr : ab / sr;
+    double A = ab * sr / (m1 > 1.0 ? m1 : 1.0);
+    int acc = (double)u2 < A;
+    if (acc) {
+      track[k] = y;
+      qbead[x] = 0;
+      qbead[y] = qk;
+      ++nacc;
+    }
+    if (lg) {
+      lg[0] = k; lg[1] = x; lg[2] = y; lg[3] = adj ? 0 : 2; lg[4] = acc; lg[5] = 3;
+    }
+  }
+  free(cand);
+  free(q);
+  free(wr);
+  return nacc;
+}

@@ -135,6 +135,45 @@ int orc_charge_hop(orc_mc_state_t* s, const orc_lattice_t* lat, const double* rx
                    double barrier2, const float* u, int nu, int* cursor, int* log,
                    double* e_initial_out);
 
+/* ---------------- Metropolis-Hastings charge hopping (no reference counterpart: an
+ * exactly detailed-balanced alternative to chargeHop's proposal, S/main26.c:1926-2390,
+ * with the same screened-Coulomb energy and barriers).
+ * A move takes charge k (sign qk) from bead x to an uncharged bead y with
+ * |getNID(x,y)| < hop_radius, drawn from q(x->.) (uniform, or learned: a
+ * (1-floor)/floor mixture of the normalised predicted acceptances sigmoid(MLP
+ * score), a Barker-type locally balanced weight, and uniform).  With
+ * rho = exp(-dE) q(y->x)/q(x->y) and a = exp(-b) (b = barrier for chain
+ * neighbours, barrier2 otherwise) the move is accepted with probability
+ * A = a sqrt(rho) / max(1, a sqrt(rho), a / sqrt(rho)), so A(x->y)/A(y->x) = rho
+ * (detailed balance for exp(-E)) and A = exp(-b - dE/2), the reference's rule,
+ * whenever |ln rho| <= 2b under a symmetric proposal.  Three uniforms per attempt:
+ * u0 picks k = min(floor(u0 nq), nq-1), u1 picks y by inverse CDF over the
+ * candidates in ascending bead order, u2 is the acceptance dice (accept if u2 < A). */
+typedef struct {
+  int n, nb;                /* beads, beads per chain */
+  double hop_radius, feature_radius, floor_eps;
+  double lambda_b, lambda_d, barrier, barrier2;
+} orc_mh_params_t;
+/* round a float to bfloat16 (round to nearest even), returned as float */
+float orc_bf16_round(float x);
+/* The 16 proposal features of moving the charge at bead src to bead dst.  qbead[n]:
+ * charge sign per bead (0 = none) with the moving charge at src; clr / set (or -1):
+ * a bead whose charge is taken as 0 / a bead taken to carry the moving charge
+ * (the state after a move, for the reverse proposal).  ncand: candidates of src. */
+void orc_mh_features(const orc_lattice_t* lat, const orc_mh_params_t* p, const double* rx, const double* ry,
+                     const double* rz, const int* qbead, int src, int dst, int qk, int clr, int set, int ncand,
+                     double f[16]);
+/* MLP 16-64-64-1 with ReLU, FP64 arithmetic on the given (already rounded) weights
+ * laid out W1[64][16], b1[64], W2[64][64], b2[64], W3[64], b3. */
+double orc_mlp_forward(const float* w, const double f[16]);
+/* nattempt attempts; w = NULL: uniform proposal.  u: 3 per attempt.  log: 6 ints per
+ * attempt {k, x, y, kind (0 chain neighbour, 2 other, -1 no candidate), accepted, 3};
+ * feats (may be NULL): 16 features of each proposed move (zeros when skipped).
+ * track/sign: charges; qbead: per-bead sign, kept consistent.  Returns accepted count. */
+int orc_mh_sweep(const orc_lattice_t* lat, const orc_mh_params_t* p, const double* rx, const double* ry,
+                 const double* rz, int nq, int* track, const int* sign, int* qbead, const float* w,
+                 const float* u, int nattempt, int* log, float* feats);
+
 /* ---------------- The reference's HI timestep, restated for CPU timing.
  * Dense Ewald M rebuilt every srpy steps, TEA matvec, forces, MC every mstep.
  * OpenMP inside; returns elapsed seconds. */

@@ -279,7 +279,16 @@ struct smdlt_ctx {
   unsigned char* occ = nullptr;
   long long occ_stride = 0;
   int* log_off = nullptr;
-  float* mlp_w = nullptr;
+  float* mlp_w = nullptr;     // BF16-rounded proposal network weights (FP32 storage)
+  float* mh_feats = nullptr;  // features of each MH attempt of the last sweep [nq_total][16]
+  int* mh_log = nullptr;      // its log [nq_total][6]
+  float* mlp_ws = nullptr;    // training: weights, batch, per-block partial gradients
+  float* mlp_x = nullptr;
+  float* mlp_y = nullptr;
+  float* mlp_part = nullptr;
+  float* mlp_out = nullptr;
+  int mlp_cap = 0;
+  int mlp_n = 0;              // size of the batch resident on the device
   uint64_t mc_sweeps = 0;
   int* d_err = nullptr;
   // profiling
@@ -468,6 +477,9 @@ void smdlt_default_params(smdlt_params* p) {
   p->lanczos_tol = 1e-6;
   p->lanczos_maxit = 30;
   p->seed = 0x5EED0000ull;
+  p->hop_radius = 3.0;
+  p->feature_radius = 4.0;
+  p->proposal_floor = 0.05;
 }
 
 int smdlt_num_beads(const smdlt_ctx* c, int replica) {
@@ -649,6 +661,13 @@ int smdlt_create(const smdlt_params* params, int n_replicas, int device, smdlt_c
     R.kappa_d = 1.0 / p.lambda_d;
     R.barrier = p.barrier;
     R.barrier2 = p.barrier2;
+    {
+      double hop = p.hop_radius > 0 ? p.hop_radius : 3.0, fr = p.feature_radius > 0 ? p.feature_radius : 4.0;
+      R.hop2 = hop * hop;
+      R.feat_r = fr;
+      R.nbhd2 = (2 * hop + fr + 0.5) * (2 * hop + fr + 0.5);  // every bead any reverse feature can see
+      R.pfloor = (p.proposal_floor > 0 && p.proposal_floor <= 1) ? p.proposal_floor : 0.05;
+    }
     R.r2cut = p.lj_ext_form ? 5.03968 : 25.0;
     R.rv = 6.0;
     R.rnew = 0.5 * (R.rv - sqrt(R.r2cut));
@@ -862,7 +881,7 @@ void smdlt_destroy(smdlt_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   destroy_graphs(c);
   if (c->h_flags) hipHostFree(c->h_flags);
-  void* ptrs[] = {c->saved, c->d_reps, c->d_slot_rep, c->d_seeds, c->rx, c->ry, c->rz, c->dxv, c->dyv, c->dzv, c->dsp, c->f,
+  void* ptrs[] = {c->mh_feats, c->mh_log, c->mlp_ws, c->mlp_x, c->mlp_y, c->mlp_part, c->mlp_out, c->saved, c->d_reps, c->d_slot_rep, c->d_seeds, c->rx, c->ry, c->rz, c->dxv, c->dyv, c->dzv, c->dsp, c->f,
                   c->z, c->mf, c->bz, c->tmp, c->tmp2, c->nrpy, c->lrpy, c->nlj, c->llj, c->overflow, c->d_err,
                   c->cell_of, c->cell_cnt, c->cell_start, c->cell_fill, c->cell_beads, c->rep_flags, c->out2, c->aff,
                   c->partial2, c->grid, c->bsp, c->tw, c->tx, c->ty, c->tz, c->bricks, c->bbase, c->brick_cnt, c->brick_start,
@@ -1796,11 +1815,12 @@ int smdlt_upload_charges(smdlt_ctx* c, const int* nq, const int* track, const in
       if (sign && sign[k] != 1 && sign[k] != -1) return SMDLT_E_ARG;
     }
   }
-  void* ptrs[] = {c->qtrack, c->qsign, c->qold, c->occ, c->log_off};
+  void* ptrs[] = {c->qtrack, c->qsign, c->qold, c->occ, c->log_off, c->mh_feats, c->mh_log};
   for (void* p : ptrs)
     if (p) hipFree(p);
-  c->qtrack = c->qsign = c->qold = c->log_off = nullptr;
+  c->qtrack = c->qsign = c->qold = c->log_off = c->mh_log = nullptr;
   c->occ = nullptr;
+  c->mh_feats = nullptr;
   int st = 0;
   st |= alloc_dev(&c->qtrack, tot);
   st |= alloc_dev(&c->qsign, tot);
@@ -1839,7 +1859,8 @@ int smdlt_download_charges(smdlt_ctx* c, int* track, int* old) {
 int smdlt_mc_sweep(smdlt_ctx* c, int mode, const float* uniforms, int n_uniforms, int* consumed, int* log) {
   if (!c) return SMDLT_E_ARG;
   if (!c->have_pos) return SMDLT_E_STATE;
-  if (mode != SMDLT_MC_REFERENCE) return SMDLT_E_ARG;
+  if (mode != SMDLT_MC_REFERENCE && mode != SMDLT_MC_MH_UNIFORM && mode != SMDLT_MC_MH_LEARNED) return SMDLT_E_ARG;
+  if (mode == SMDLT_MC_MH_LEARNED && !c->mlp_w) return SMDLT_E_STATE;
   if (uniforms && n_uniforms <= 0) return SMDLT_E_ARG;
   HCK(hipSetDevice(c->device));
   int st = upload_reps(c);
@@ -1848,17 +1869,32 @@ int smdlt_mc_sweep(smdlt_ctx* c, int mode, const float* uniforms, int n_uniforms
   float* du = nullptr;
   int* dcons = nullptr;
   int* dlog = nullptr;
+  const bool mh = mode != SMDLT_MC_REFERENCE;
   if (uniforms) {
     HCK(hipMalloc(&du, sizeof(float) * (size_t)n_uniforms * c->nrep));
     HCK(hipMemcpy(du, uniforms, sizeof(float) * (size_t)n_uniforms * c->nrep, hipMemcpyHostToDevice));
   }
   HCK(hipMalloc(&dcons, 4 * c->nrep));
-  if (log && c->nq_total) HCK(hipMalloc(&dlog, 24 * (size_t)c->nq_total));
+  if (mh) {
+    if (!c->mh_feats && c->nq_total) {
+      if (alloc_dev(&c->mh_feats, 16 * (size_t)c->nq_total) || alloc_dev(&c->mh_log, 6 * (size_t)c->nq_total))
+        return SMDLT_E_NOMEM;
+    }
+    dlog = c->mh_log;
+  } else if (log && c->nq_total) {
+    HCK(hipMalloc(&dlog, 24 * (size_t)c->nq_total));
+  }
   HCK(hipMemset(c->mc_err, 0, 4));
   int rb = c->cur_rep_begin, re = c->cur_rep_end;
-  launch_mc_sweep(c->stream, c->d_reps, rb, re - rb, c->rx, c->ry, c->rz, c->qtrack, c->qold, c->qsign, c->occ,
-                  c->occ_stride, du, uniforms ? n_uniforms : 0, c->d_seeds, c->mc_sweeps++, dcons, dlog, c->log_off,
-                  c->mc_err);
+  if (mh)
+    launch_mh_sweep(c->stream, c->d_reps, rb, re - rb, c->rx, c->ry, c->rz, c->qtrack, c->qold, c->qsign,
+                    (signed char*)c->occ, c->occ_stride, du, uniforms ? n_uniforms : 0, c->d_seeds, c->mc_sweeps++,
+                    mode == SMDLT_MC_MH_LEARNED ? c->mlp_w : nullptr, dcons, dlog, c->mh_feats, c->log_off,
+                    c->mc_err);
+  else
+    launch_mc_sweep(c->stream, c->d_reps, rb, re - rb, c->rx, c->ry, c->rz, c->qtrack, c->qold, c->qsign, c->occ,
+                    c->occ_stride, du, uniforms ? n_uniforms : 0, c->d_seeds, c->mc_sweeps++, dcons, dlog, c->log_off,
+                    c->mc_err);
   HCK(hipGetLastError());
   HCK(hipStreamSynchronize(c->stream));
   int e = 0;
@@ -1867,7 +1903,8 @@ int smdlt_mc_sweep(smdlt_ctx* c, int mode, const float* uniforms, int n_uniforms
   if (log && dlog) HCK(hipMemcpy(log, dlog, 24 * (size_t)c->nq_total, hipMemcpyDeviceToHost));
   if (du) hipFree(du);
   hipFree(dcons);
-  if (dlog) hipFree(dlog);
+  if (dlog && !mh) hipFree(dlog);
+  if (e & 8) return SMDLT_E_GEOMETRY;
   return (e & 2) ? SMDLT_E_MC_STREAM : 0;
 }
 
@@ -1890,22 +1927,86 @@ int smdlt_mc_energy(smdlt_ctx* c, double* energy) {
 
 int smdlt_mlp_num_params(void) { return 16 * 64 + 64 + 64 * 64 + 64 + 64 + 1; }
 
+static float bf16_round(float x) {
+  uint32_t u;
+  memcpy(&u, &x, 4);
+  if ((u & 0x7f800000u) != 0x7f800000u) u += 0x7fffu + ((u >> 16) & 1u);
+  u &= 0xffff0000u;
+  float r;
+  memcpy(&r, &u, 4);
+  return r;
+}
+
 int smdlt_mlp_set_weights(smdlt_ctx* c, const float* w) {
   if (!c || !w) return SMDLT_E_ARG;
   HCK(hipSetDevice(c->device));
-  if (!c->mlp_w && alloc_dev(&c->mlp_w, smdlt_mlp_num_params())) return SMDLT_E_NOMEM;
-  HCK(hipMemcpy(c->mlp_w, w, 4 * smdlt_mlp_num_params(), hipMemcpyHostToDevice));
+  const int np = smdlt_mlp_num_params();
+  if (!c->mlp_w && alloc_dev(&c->mlp_w, np)) return SMDLT_E_NOMEM;
+  std::vector<float> r(np);
+  for (int i = 0; i < np; ++i) r[i] = bf16_round(w[i]);
+  HCK(hipMemcpy(c->mlp_w, r.data(), 4 * np, hipMemcpyHostToDevice));
   return 0;
 }
 
 int smdlt_mlp_loss_grad(smdlt_ctx* c, const float* w, const float* x, const float* y, int n, float* loss, float* grad) {
-  (void)c; (void)w; (void)x; (void)y; (void)n; (void)loss; (void)grad;
-  return SMDLT_E_ARG;
+  if (!c || !w || n <= 0 || !loss || !grad || (!x) != (!y)) return SMDLT_E_ARG;
+  if (!x && n != c->mlp_n) return SMDLT_E_STATE;  // reuse needs the batch of the previous call
+  HCK(hipSetDevice(c->device));
+  const int np = smdlt_mlp_num_params();
+  const int nblk = (n + MLP_TILE - 1) / MLP_TILE;
+  if (n > c->mlp_cap) {
+    void* old[] = {c->mlp_x, c->mlp_y, c->mlp_part};
+    for (void* p : old)
+      if (p) hipFree(p);
+    c->mlp_x = c->mlp_y = c->mlp_part = nullptr;
+    c->mlp_n = 0;
+    if (!x) return SMDLT_E_STATE;
+    int cap = std::max(n, 1024);
+    int cb = (cap + MLP_TILE - 1) / MLP_TILE;
+    if (alloc_dev(&c->mlp_x, 16 * (size_t)cap) || alloc_dev(&c->mlp_y, cap) ||
+        alloc_dev(&c->mlp_part, (size_t)(np + 1) * cb))
+      return SMDLT_E_NOMEM;
+    c->mlp_cap = cap;
+  }
+  if (!c->mlp_ws && (alloc_dev(&c->mlp_ws, np) || alloc_dev(&c->mlp_out, np + 1))) return SMDLT_E_NOMEM;
+  std::vector<float> r(np);
+  for (int i = 0; i < np; ++i) r[i] = bf16_round(w[i]);
+  HCK(hipMemcpyAsync(c->mlp_ws, r.data(), 4 * np, hipMemcpyHostToDevice, c->stream));
+  if (x) {
+    HCK(hipMemcpyAsync(c->mlp_x, x, 64 * (size_t)n, hipMemcpyHostToDevice, c->stream));
+    HCK(hipMemcpyAsync(c->mlp_y, y, 4 * (size_t)n, hipMemcpyHostToDevice, c->stream));
+    c->mlp_n = n;
+  }
+  launch_mlp_loss_grad(c->stream, c->mlp_ws, c->mlp_x, c->mlp_y, n, c->mlp_part, c->mlp_out);
+  HCK(hipGetLastError());
+  std::vector<float> out(np + 1);
+  HCK(hipMemcpyAsync(out.data(), c->mlp_out, 4 * (size_t)(np + 1), hipMemcpyDeviceToHost, c->stream));
+  HCK(hipStreamSynchronize(c->stream));
+  memcpy(grad, out.data(), 4 * (size_t)np);
+  *loss = out[np];
+  (void)nblk;
+  return 0;
 }
 
-int smdlt_mc_features(smdlt_ctx* c, int replica, float* features, int* cand_from, int* cand_to, int cap, int* n) {
-  (void)c; (void)replica; (void)features; (void)cand_from; (void)cand_to; (void)cap; (void)n;
-  return SMDLT_E_ARG;
+int smdlt_mc_features(smdlt_ctx* c, int replica, float* features, int* from, int* to, int* accepted, int cap,
+                      int* n) {
+  if (!c || replica < 0 || replica >= c->nrep || !n) return SMDLT_E_ARG;
+  if (!c->mh_feats) return SMDLT_E_STATE;
+  HCK(hipSetDevice(c->device));
+  const int nq = c->nq[replica], off = c->q0[replica];
+  *n = nq;
+  if (nq > cap) return SMDLT_E_ARG;
+  if (features) HCK(hipMemcpy(features, c->mh_feats + 16 * (size_t)off, 64 * (size_t)nq, hipMemcpyDeviceToHost));
+  if (from || to || accepted) {
+    std::vector<int> lg(6 * (size_t)nq);
+    HCK(hipMemcpy(lg.data(), c->mh_log + 6 * (size_t)off, 24 * (size_t)nq, hipMemcpyDeviceToHost));
+    for (int k = 0; k < nq; ++k) {
+      if (from) from[k] = lg[6 * k + 1];
+      if (to) to[k] = lg[6 * k + 2];
+      if (accepted) accepted[k] = lg[6 * k + 4];
+    }
+  }
+  return 0;
 }
 
 // Select a replica sub-range for the per-replica (plain) path: subsequent calls

@@ -37,6 +37,7 @@ struct RepDev {
   // physics
   double dt, edot, kappas, qmax, kappab, epsilon, lambda_b, kappa_d, r2cut, rv, sqrt2dt;
   double barrier, barrier2;
+  double hop2, nbhd2, feat_r, pfloor;  // MH hops: hop radius^2, neighbourhood radius^2, feature radius, floor
   double rnew, skin;           // Verlet trigger (rv-rc)/2 and mobility-list skin
   int treal, reset;            // t % tp and whether this step starts a lattice period
   unsigned long long step;     // step counter for the device noise stream
@@ -295,6 +296,13 @@ void lz_tea(hipStream_t st, const int* slot_rep, int slot_begin, int nslots, con
 void lz_reset_checks(hipStream_t st, double* scal, int rep_begin, int nrep);
 
 // k_mc.hip
+#define MLP_TILE 64
+void launch_mlp_loss_grad(hipStream_t st, const float* w, const float* X, const float* Y, int n, float* part,
+                          float* out);
+void launch_mh_sweep(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, const double* rx, const double* ry,
+                     const double* rz, int* track, int* old, const int* sign, signed char* occ, long long occ_stride,
+                     const float* uniforms, int nu, const uint64_t* seeds, uint64_t sweep_id, const float* weights,
+                     int* consumed, int* log, float* feats, const int* log_off, int* err);
 void launch_mc_sweep(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, const double* rx, const double* ry,
                      const double* rz, int* track, int* old, const int* sign, unsigned char* occ,
                      long long occ_stride, const float* uniforms, int nu, const uint64_t* seeds, uint64_t sweep_id,

@@ -279,3 +279,388 @@ void launch_mc_energy(hipStream_t st, const RepDev* reps, int rep_begin, int nre
                       const double* rz, const int* track, const int* sign, double* e) {
   if (nrep > 0) k_mc_energy<<<nrep, MC_THREADS, 0, st>>>(reps, rep_begin, rx, ry, rz, track, sign, e);
 }
+
+// ============================================================================
+// Exactly detailed-balanced hops (no reference counterpart; same energy and
+// barriers as chargeHop).  Attempt a: u0 picks charge k (at bead x), u1 picks a
+// destination y among the uncharged beads within the hop radius of x by the
+// inverse CDF of q(x->.) in ascending bead order, u2 is the acceptance dice:
+//   rho = exp(-dE) q(y->x) / q(x->y),  a = exp(-b) (b = barrier for a chain
+//   neighbour, barrier2 otherwise),  A = a sqrt(rho) / max(1, a sqrt(rho), a/sqrt(rho)).
+// q is uniform, or (learned) (1-eps) w_c / sum(w) + eps/ncand with w_c =
+// sigmoid(MLP score), the network (trained to predict acceptance of uniform
+// proposals) scoring 16 features of each candidate move (BF16 weights, FP32
+// arithmetic).
+// q(y->x) is evaluated in the state after the move.  The beads within
+// 2 hop + feature radius of x (all any forward or reverse feature can see) are
+// gathered once per attempt in ascending order; features are summed by one
+// wavefront per candidate.  dE is exact FP64 over all charges.
+#define MH_MAXL 1024
+#define MH_MAXC 128
+#define MLP_NP (16 * 64 + 64 + 64 * 64 + 64 + 64 + 1)
+
+struct MhShared {
+  int loc[MH_MAXL];
+  int cand[MH_MAXC];
+  double q[MH_MAXC];
+  float feat[MH_MAXC][16];
+  float fsel[16];
+  float w[MLP_NP];
+  double dsum[MC_THREADS / 64];
+  int wcnt[MC_THREADS / 64];
+  int nloc, ncand, k, x, qk, y, pick, accept, bad;
+  float u[3];
+  double qf;
+};
+
+__device__ __forceinline__ double mh_dist(const RepDev& R, const double* rx, const double* ry, const double* rz, int a,
+                                          int b) {
+  const int sa = R.slot0 + a, sb = R.slot0 + b;
+  double d[3];
+  ref_nid(R, rx[sa], ry[sa], rz[sa], rx[sb], ry[sb], rz[sb], d[0], d[1], d[2]);
+  return sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+}
+
+__device__ __forceinline__ double mh_dist2(const RepDev& R, const double* rx, const double* ry, const double* rz,
+                                           int a, int b) {
+  const int sa = R.slot0 + a, sb = R.slot0 + b;
+  double d[3];
+  ref_nid(R, rx[sa], ry[sa], rz[sa], rx[sb], ry[sb], rz[sb], d[0], d[1], d[2]);
+  return d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+}
+
+__device__ __forceinline__ double mh_cos_at(const RepDev& R, const double* rx, const double* ry, const double* rz,
+                                            int b) {
+  const int pos = b % R.nb;
+  if (pos == 0 || pos == R.nb - 1) return 0.0;
+  const int s = R.slot0 + b;
+  double u[3], v[3];
+  ref_nid(R, rx[s - 1], ry[s - 1], rz[s - 1], rx[s], ry[s], rz[s], u[0], u[1], u[2]);
+  ref_nid(R, rx[s + 1], ry[s + 1], rz[s + 1], rx[s], ry[s], rz[s], v[0], v[1], v[2]);
+  double uu = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]), vv = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  return (u[0] * v[0] + u[1] * v[1] + u[2] * v[2]) / (uu * vv);
+}
+
+// Append the flagged values of one block-wide round in ascending thread order.
+__device__ __forceinline__ void block_append(bool flag, int value, int* list, int* cnt, int cap, int* wcnt) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  unsigned long long m = __ballot(flag);
+  if (lane == 0) wcnt[w] = __popcll(m);
+  __syncthreads();
+  int off = *cnt;
+  for (int k = 0; k < w; ++k) off += wcnt[k];
+  int pos = off + __popcll(m & ((1ull << lane) - 1ull));
+  if (flag && pos < cap) list[pos] = value;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) t += wcnt[k];
+    *cnt += t;
+  }
+  __syncthreads();
+}
+
+// 16 features of moving the charge at src (sign qk) to dst, summed by one
+// wavefront over the gathered neighbourhood; clr / set (or -1) give the state
+// after a move.  Lane 0 writes out[16].
+__device__ void mh_features_wave(const RepDev& R, const double* rx, const double* ry, const double* rz,
+                                 const signed char* qb, const int* loc, int nloc, int src, int dst, int qk, int clr,
+                                 int set, int ncand, float* out) {
+  const int lane = threadIdx.x & 63;
+  double e = 0, n4d = 0, n4s = 0, s4d = 0, s4s = 0, b4d = 0, b4s = 0;
+  const double R4 = R.feat_r, kap = R.kappa_d;
+  for (int i = lane; i < nloc; i += 64) {
+    const int m = loc[i];
+    if (m == src || m == dst) continue;
+    const double rdm = mh_dist(R, rx, ry, rz, dst, m), rsm = mh_dist(R, rx, ry, rz, src, m);
+    const int qm = (m == clr) ? 0 : (m == set) ? qk : (int)qb[m];
+    if (qm != 0 && (rdm < R4 || rsm < R4))
+      e += R.lambda_b * (double)(qk * qm) * (exp(-kap * rdm) / rdm - exp(-kap * rsm) / rsm);
+    if (rdm < R4) {
+      b4d += 0.1;
+      if (qm != 0) {
+        n4d += 1.0;
+        s4d += (double)(qk * qm);
+      }
+    }
+    if (rsm < R4) {
+      b4s += 0.1;
+      if (qm != 0) {
+        n4s += 1.0;
+        s4s += (double)(qk * qm);
+      }
+    }
+  }
+  e = warp_sum(e);
+  n4d = warp_sum(n4d);
+  n4s = warp_sum(n4s);
+  s4d = warp_sum(s4d);
+  s4s = warp_sum(s4s);
+  b4d = warp_sum(b4d);
+  b4s = warp_sum(b4s);
+  if (lane == 0) {
+    const int nb = R.nb, gap = abs(dst - src);
+    const bool same = (src / nb) == (dst / nb);
+    out[0] = (float)mh_dist(R, rx, ry, rz, src, dst);
+    out[1] = (same && gap == 1) ? 1.f : 0.f;
+    out[2] = same ? (float)((double)(gap < 20 ? gap : 20) / 20.0) : 1.f;
+    out[3] = (float)e;
+    out[4] = (float)n4d;
+    out[5] = (float)n4s;
+    out[6] = (float)s4d;
+    out[7] = (float)s4s;
+    out[8] = (float)b4d;
+    out[9] = (float)b4s;
+    out[10] = (float)mh_cos_at(R, rx, ry, rz, dst);
+    out[11] = (float)mh_cos_at(R, rx, ry, rz, src);
+    out[12] = (dst % nb == 0 || dst % nb == nb - 1) ? 1.f : 0.f;
+    out[13] = (src % nb == 0 || src % nb == nb - 1) ? 1.f : 0.f;
+    out[14] = (float)qk;
+    out[15] = (float)((double)ncand / 10.0);
+  }
+}
+
+// MLP score of one feature vector by one wavefront: lane j owns hidden unit j.
+__device__ float mlp_score_wave(const float* w, const float* f) {
+  const int j = threadIdx.x & 63;
+  const float *W1 = w, *b1 = W1 + 64 * 16, *W2 = b1 + 64, *b2 = W2 + 64 * 64, *W3 = b2 + 64, *b3 = W3 + 64;
+  float a = b1[j];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) a = fmaf(W1[j * 16 + i], f[i], a);
+  const float h1 = fmaxf(a, 0.f);
+  float a2 = b2[j];
+  for (int i = 0; i < 64; ++i) a2 = fmaf(W2[j * 64 + i], __shfl(h1, i, 64), a2);
+  const float h2 = fmaxf(a2, 0.f);
+  float s = W3[j] * h2;
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  return s + b3[0];
+}
+
+// proposal over the scored candidates (thread 0): the network's predicted
+// acceptance sigmoid(score) normalised (a Barker-type locally balanced weight),
+// mixed with the uniform proposal at weight eps
+__device__ void mh_proposal_probs(double* q, int nc, double eps) {
+  double z = 0.0;
+  for (int c = 0; c < nc; ++c) {
+    q[c] = 1.0 / (1.0 + exp(-q[c]));
+    z += q[c];
+  }
+  for (int c = 0; c < nc; ++c) q[c] = (1.0 - eps) * q[c] / z + eps / nc;
+}
+
+__global__ __launch_bounds__(MC_THREADS) void k_mh_sweep(const RepDev* __restrict__ reps, int rep_begin,
+                                                         const double* __restrict__ rx, const double* __restrict__ ry,
+                                                         const double* __restrict__ rz, int* track, int* old,
+                                                         const int* sign, signed char* occ, long long occ_stride,
+                                                         const float* uniforms, int nu, const uint64_t* seeds,
+                                                         uint64_t sweep_id, const float* weights, int* consumed,
+                                                         int* mlog, float* feats, const int* log_off, int* err) {
+  const int r = rep_begin + blockIdx.x;
+  const RepDev& R = reps[r];
+  if (!R.active) return;
+  __shared__ MhShared S;
+  const int N = R.n, nq = R.nq, tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+  const int nwv = blockDim.x >> 6;
+  const bool learned = weights != nullptr;
+  signed char* qb = occ + (size_t)blockIdx.x * occ_stride;
+  for (int i = tid; i < N; i += blockDim.x) qb[i] = 0;
+  if (learned)
+    for (int i = tid; i < MLP_NP; i += blockDim.x) S.w[i] = weights[i];
+  __syncthreads();
+  for (int k = tid; k < nq; k += blockDim.x) qb[track[R.q0 + k]] = (signed char)sign[R.q0 + k];
+  __syncthreads();
+  const float* ur = uniforms ? uniforms + (size_t)r * nu : nullptr;
+  int done_att = 0;
+  for (int a = 0; a < nq; ++a) {
+    if (tid == 0) {
+      S.bad = 0;
+      for (int i = 0; i < 3; ++i) {
+        if (ur) {
+          if (3 * a + i < nu) {
+            S.u[i] = ur[3 * a + i];
+          } else {
+            S.bad = 1;
+            S.u[i] = 0.f;
+          }
+        } else {
+          S.u[i] = dev_uniform(seeds[r], sweep_id, (uint32_t)(3 * a + i));
+        }
+      }
+      int k = (int)(S.u[0] * (float)nq);
+      S.k = k > nq - 1 ? nq - 1 : k;
+      S.x = track[R.q0 + S.k];
+      S.qk = sign[R.q0 + S.k];
+      S.nloc = 0;
+      S.ncand = 0;
+    }
+    __syncthreads();
+    if (S.bad) {
+      if (tid == 0) atomicOr(err, 2);
+      break;
+    }
+    const int k = S.k, x = S.x, qk = S.qk;
+    int* lg = mlog ? mlog + 6 * (size_t)(log_off[r] + a) : nullptr;
+    float* fa = feats ? feats + 16 * (size_t)(log_off[r] + a) : nullptr;
+    // neighbourhood of x (x included), ascending
+    for (int base = 0; base < N; base += blockDim.x) {
+      const int b = base + tid;
+      const bool f = b < N && mh_dist2(R, rx, ry, rz, x, b) < R.nbhd2;
+      block_append(f, b, S.loc, &S.nloc, MH_MAXL, S.wcnt);
+    }
+    const int nloc = S.nloc;
+    if (nloc > MH_MAXL) {
+      if (tid == 0) atomicOr(err, 8);
+      break;
+    }
+    // candidates of x
+    for (int base = 0; base < nloc; base += blockDim.x) {
+      const int i = base + tid;
+      const int b = i < nloc ? S.loc[i] : -1;
+      const bool f = b >= 0 && b != x && qb[b] == 0 && mh_dist2(R, rx, ry, rz, x, b) < R.hop2;
+      block_append(f, b, S.cand, &S.ncand, MH_MAXC, S.wcnt);
+    }
+    const int nc = S.ncand;
+    if (nc > MH_MAXC) {
+      if (tid == 0) atomicOr(err, 8);
+      break;
+    }
+    if (nc == 0) {
+      if (tid == 0) {
+        if (lg) {
+          lg[0] = k;
+          lg[1] = x;
+          lg[2] = -1;
+          lg[3] = -1;
+          lg[4] = 0;
+          lg[5] = 3;
+        }
+        if (fa)
+          for (int i = 0; i < 16; ++i) fa[i] = 0.f;
+      }
+      done_att = a + 1;
+      __syncthreads();
+      continue;
+    }
+    // forward proposal
+    if (learned) {
+      for (int c = wv; c < nc; c += nwv) {
+        mh_features_wave(R, rx, ry, rz, qb, S.loc, nloc, x, S.cand[c], qk, -1, -1, nc, S.feat[c]);
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        const float s = mlp_score_wave(S.w, S.feat[c]);
+        if (lane == 0) S.q[c] = (double)s;
+      }
+      __syncthreads();
+      if (tid == 0) mh_proposal_probs(S.q, nc, R.pfloor);
+    } else if (tid == 0) {
+      for (int c = 0; c < nc; ++c) S.q[c] = 1.0 / nc;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double tot = 0.0;
+      for (int c = 0; c < nc; ++c) tot += S.q[c];
+      const double t = (double)S.u[1] * tot;
+      double cum = 0.0;
+      int pick = nc - 1;
+      for (int c = 0; c < nc; ++c) {
+        cum += S.q[c];
+        if (cum >= t) {
+          pick = c;
+          break;
+        }
+      }
+      S.pick = pick;
+      S.y = S.cand[pick];
+      S.qf = S.q[pick];
+    }
+    __syncthreads();
+    const int y = S.y;
+    // features of the proposed move (training record)
+    if (fa || learned) {
+      if (learned) {
+        if (tid < 16) S.fsel[tid] = S.feat[S.pick][tid];
+      } else if (wv == 0) {
+        mh_features_wave(R, rx, ry, rz, qb, S.loc, nloc, x, y, qk, -1, -1, nc, S.fsel);
+      }
+    }
+    // exact energy change over all charges
+    double de = 0.0;
+    for (int m = tid; m < nq; m += blockDim.x) {
+      if (m == k) continue;
+      const int bm = track[R.q0 + m], qm = sign[R.q0 + m];
+      const double rym = mh_dist(R, rx, ry, rz, y, bm), rxm = mh_dist(R, rx, ry, rz, x, bm);
+      de += R.lambda_b * (double)(qk * qm) * (exp(-R.kappa_d * rym) / rym - exp(-R.kappa_d * rxm) / rxm);
+    }
+    de = block_sum(de, S.dsum);
+    // reverse proposal from y in the state after the move
+    if (tid == 0) S.ncand = 0;
+    __syncthreads();
+    for (int base = 0; base < nloc; base += blockDim.x) {
+      const int i = base + tid;
+      const int b = i < nloc ? S.loc[i] : -1;
+      bool f = false;
+      if (b >= 0 && b != y) {
+        const int qbb = (b == x) ? 0 : (int)qb[b];
+        f = qbb == 0 && mh_dist2(R, rx, ry, rz, y, b) < R.hop2;
+      }
+      block_append(f, b, S.cand, &S.ncand, MH_MAXC, S.wcnt);
+    }
+    const int nc2 = S.ncand;
+    if (nc2 > MH_MAXC) {
+      if (tid == 0) atomicOr(err, 8);
+      break;
+    }
+    if (learned) {
+      for (int c = wv; c < nc2; c += nwv) {
+        mh_features_wave(R, rx, ry, rz, qb, S.loc, nloc, y, S.cand[c], qk, x, y, nc2, S.feat[c]);
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        const float s = mlp_score_wave(S.w, S.feat[c]);
+        if (lane == 0) S.q[c] = (double)s;
+      }
+      __syncthreads();
+      if (tid == 0) mh_proposal_probs(S.q, nc2, R.pfloor);
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double qr = 0.0;
+      for (int c = 0; c < nc2; ++c)
+        if (S.cand[c] == x) qr = learned ? S.q[c] : 1.0 / nc2;
+      const double lnrho = -de + log(qr) - log(S.qf);
+      const bool adj = ((x / R.nb) == (y / R.nb)) && abs(y - x) == 1;
+      const double ab = exp(-(adj ? R.barrier : R.barrier2));
+      const double sr = exp(0.5 * lnrho);
+      const double m1 = fmax(ab * sr, ab / sr);
+      const double A = ab * sr / fmax(m1, 1.0);
+      const int acc = (double)S.u[2] < A;
+      S.accept = acc;
+      old[R.q0 + k] = x;
+      if (acc) {
+        track[R.q0 + k] = y;
+        qb[x] = 0;
+        qb[y] = (signed char)qk;
+      }
+      if (lg) {
+        lg[0] = k;
+        lg[1] = x;
+        lg[2] = y;
+        lg[3] = adj ? 0 : 2;
+        lg[4] = acc;
+        lg[5] = 3;
+      }
+    }
+    if (fa && tid < 16) fa[tid] = S.fsel[tid];
+    done_att = a + 1;
+    __syncthreads();
+  }
+  if (tid == 0 && consumed) consumed[r] = 3 * done_att;
+}
+
+void launch_mh_sweep(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, const double* rx, const double* ry,
+                     const double* rz, int* track, int* old, const int* sign, signed char* occ, long long occ_stride,
+                     const float* uniforms, int nu, const uint64_t* seeds, uint64_t sweep_id, const float* weights,
+                     int* consumed, int* log, float* feats, const int* log_off, int* err) {
+  if (nrep <= 0) return;
+  k_mh_sweep<<<nrep, MC_THREADS, 0, st>>>(reps, rep_begin, rx, ry, rz, track, old, sign, occ, occ_stride, uniforms, nu,
+                                          seeds, sweep_id, weights, consumed, log, feats, log_off, err);
+}

@@ -32,7 +32,8 @@ class Params(C.Structure):
                 ("pbc_compat", C.c_int), ("precision", C.c_int), ("noise_mode", C.c_int),
                 ("ewald_alpha", C.c_double), ("rc_real", C.c_double), ("skin", C.c_double),
                 ("grid", C.c_int * 3), ("spline_order", C.c_int), ("mobility_tol", C.c_double),
-                ("lanczos_tol", C.c_double), ("lanczos_maxit", C.c_int), ("seed", C.c_uint64)]
+                ("lanczos_tol", C.c_double), ("lanczos_maxit", C.c_int), ("seed", C.c_uint64),
+                ("hop_radius", C.c_double), ("feature_radius", C.c_double), ("proposal_floor", C.c_double)]
 
 
 class SmdltError(RuntimeError):
@@ -88,16 +89,11 @@ def lib():
         L.smdlt_stage_name.argtypes = [C.c_int]
         L.smdlt_sync.argtypes = [C.c_void_p]
         L.smdlt_select_replicas.argtypes = [C.c_void_p, C.c_int, C.c_int]
-        for name in ("smdlt_mc_sweep", "smdlt_mc_energy", "smdlt_mlp_num_params", "smdlt_mlp_set_weights",
-                     "smdlt_mlp_loss_grad", "smdlt_mc_features"):
-            if hasattr(L, name):
-                pass
-        if hasattr(L, "smdlt_mc_sweep"):
-            L.smdlt_mc_sweep.argtypes = [C.c_void_p, C.c_int, _fp, C.c_int, _ip, _ip]
-            L.smdlt_mc_energy.argtypes = [C.c_void_p, _dp]
-            L.smdlt_mlp_set_weights.argtypes = [C.c_void_p, _fp]
-            L.smdlt_mlp_loss_grad.argtypes = [C.c_void_p, _fp, _fp, _fp, C.c_int, _fp, _fp]
-            L.smdlt_mc_features.argtypes = [C.c_void_p, C.c_int, _fp, _ip, _ip, C.c_int, _ip]
+        L.smdlt_mc_sweep.argtypes = [C.c_void_p, C.c_int, _fp, C.c_int, _ip, _ip]
+        L.smdlt_mc_energy.argtypes = [C.c_void_p, _dp]
+        L.smdlt_mlp_set_weights.argtypes = [C.c_void_p, _fp]
+        L.smdlt_mlp_loss_grad.argtypes = [C.c_void_p, _fp, C.c_void_p, C.c_void_p, C.c_int, _fp, _fp]
+        L.smdlt_mc_features.argtypes = [C.c_void_p, C.c_int, _fp, _ip, _ip, _ip, C.c_int, _ip]
         _lib = L
     return _lib
 
@@ -319,6 +315,54 @@ class Context:
                                         _ptr(log, C.c_int))
         _chk(code, "mc_sweep")
         return consumed, log[:6 * tot].reshape(-1, 6)
+
+    def mlp_set_weights(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        _chk(lib().smdlt_mlp_set_weights(self.h, _ptr(w, C.c_float)), "mlp_set_weights")
+
+    def mlp_loss_grad(self, w, X=None, y=None, n=None):
+        """mean BCE loss and gradient; X = y = None reuses the device-resident batch of size n."""
+        w = np.ascontiguousarray(w, np.float32)
+        g = np.zeros(len(w), np.float32)
+        loss = np.zeros(1, np.float32)
+        if X is None:
+            code = lib().smdlt_mlp_loss_grad(self.h, _ptr(w, C.c_float), None, None, int(n), _ptr(loss, C.c_float),
+                                             _ptr(g, C.c_float))
+        else:
+            X = np.ascontiguousarray(X, np.float32).reshape(-1, 16)
+            y = np.ascontiguousarray(y, np.float32).reshape(-1)
+            code = lib().smdlt_mlp_loss_grad(self.h, _ptr(w, C.c_float), X.ctypes.data_as(C.c_void_p),
+                                             y.ctypes.data_as(C.c_void_p), len(y), _ptr(loss, C.c_float),
+                                             _ptr(g, C.c_float))
+        _chk(code, "mlp_loss_grad")
+        return float(loss[0]), g
+
+    def mlp_train(self, X, y, w0, iters=600, lr=3e-3):
+        """Adam on the GPU loss gradient (the batch is uploaded once); returns (w, first loss, last loss)."""
+        w = np.asarray(w0, np.float64).copy()
+        m = np.zeros_like(w)
+        v = np.zeros_like(w)
+        l0, g = self.mlp_loss_grad(w, X, y)
+        loss = l0
+        for it in range(1, iters + 1):
+            if it > 1:
+                loss, g = self.mlp_loss_grad(w, n=len(y))
+            g = g.astype(np.float64)
+            m = 0.9 * m + 0.1 * g
+            v = 0.999 * v + 0.001 * g * g
+            w -= lr * (m / (1 - 0.9 ** it)) / (np.sqrt(v / (1 - 0.999 ** it)) + 1e-8)
+        return w.astype(np.float32), l0, loss
+
+    def mc_features(self, replica=0):
+        """features [nq][16], source, destination, accepted of the last MH sweep's moves."""
+        cap = int(self.nq[replica])
+        f = np.zeros((max(cap, 1), 16), np.float32)
+        fr, to, acc = (np.zeros(max(cap, 1), np.int32) for _ in range(3))
+        n = np.zeros(1, np.int32)
+        _chk(lib().smdlt_mc_features(self.h, replica, _ptr(f, C.c_float), _ptr(fr, C.c_int), _ptr(to, C.c_int),
+                                     _ptr(acc, C.c_int), cap, _ptr(n, C.c_int)), "mc_features")
+        k = int(n[0])
+        return f[:k], fr[:k], to[:k], acc[:k]
 
     def mc_energy(self):
         e = np.zeros(self.nrep)
