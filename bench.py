@@ -228,6 +228,155 @@ def batched_leg(nrep, steps, warmup, plain_reps):
     return out
 
 
+def tau_int_multi(E, c=5.0, per_column_mean=False, groups=1):
+    """Integrated autocorrelation time from the autocorrelation of the columns of E
+    (each column centred on its own mean), summed within `groups` consecutive
+    column blocks (one block per independent replica) and averaged over blocks,
+    with Sokal's automatic window."""
+    x = E - E.mean(0)
+    n = len(x)
+    f = np.fft.rfft(x, 2 * n, axis=0)
+    acf = np.fft.irfft(f * np.conj(f), axis=0)[:n]
+    acf = acf.reshape(n, groups, -1).sum(2).mean(1) if groups > 1 else acf.reshape(n, -1).mean(1)
+    acf /= acf[0]
+    tau = 1.0
+    for W in range(1, n):
+        tau = 1.0 + 2.0 * acf[1:W + 1].sum()
+        if W >= c * tau:
+            break
+    return tau
+
+
+def mh_study(ctx, R, nq, n, track, sign, train_moves, sweeps, burn, iters, hops_learned=True, tag=""):
+    """Train the proposal on the uniform chains' moves of ctx (R replicas), then run
+    uniform and learned chains; tau_int of the energy and of the charge configuration."""
+    t0 = time.perf_counter()
+    ctx.upload_charges([nq] * R, np.tile(track, R), np.tile(sign, R))
+    F, Y = [], []
+    while hops_learned and sum(len(v) for v in Y) < train_moves:
+        ctx.mc_sweep(S.MC_MH_UNIFORM)
+        for r in range(R):
+            f, fr, to, a = ctx.mc_features(r)
+            k = to >= 0
+            F.append(f[k])
+            Y.append(a[k])
+    w, l0, l1 = None, None, None
+    if hops_learned:
+        F = np.concatenate(F)
+        Y = np.concatenate(Y).astype(np.float32)
+        t1 = time.perf_counter()
+        w, l0, l1 = ctx.mlp_train(F, Y, S.mlp_init(0), iters=iters)
+        ctx.mlp_set_weights(w)
+        t2 = time.perf_counter()
+        print(f"[bench] {tag}: {len(Y)} training moves in {t1 - t0:.1f}s, loss {l0:.3f} -> {l1:.3f} in "
+              f"{t2 - t1:.1f}s", file=sys.stderr, flush=True)
+
+    def chain(mode):
+        ctx.upload_charges([nq] * R, np.tile(track, R), np.tile(sign, R))
+        E = np.zeros((sweeps, R))
+        occ = np.zeros((sweeps, R, nq), np.int32)
+        acc = 0
+        ts = time.perf_counter()
+        for s in range(sweeps):
+            _, log = ctx.mc_sweep(mode)
+            acc += int(log[:, 4].sum())
+            E[s] = ctx.mc_energy()
+            tr, _ = ctx.download_charges()
+            occ[s] = tr.reshape(R, nq)
+        el = time.perf_counter() - ts
+        sub = occ[burn:]
+        ns = len(sub)
+        O3 = np.zeros((ns, R, n), np.float32)
+        for r in range(R):
+            O3[np.arange(ns)[:, None], r, sub[:, r, :]] = sign[None, :]
+        out = {"acceptance": acc / (sweeps * nq * R), "tau_E": float(tau_int_multi(E[burn:])),
+               "tau_config": float(tau_int_multi(O3.reshape(ns, -1), per_column_mean=True, groups=R)),
+               "mean_E": float(E[burn:].mean()), "mean_E_se": float(E[burn:].mean(0).std(ddof=1) / np.sqrt(R)),
+               "sweep_ms": 1e3 * el / sweeps}
+        print(f"[bench] {tag} {'learned' if mode == S.MC_MH_LEARNED else 'uniform'}: {out}", file=sys.stderr,
+              flush=True)
+        return out
+    res = {"uniform": chain(S.MC_MH_UNIFORM)}
+    if hops_learned:
+        res["learned"] = chain(S.MC_MH_LEARNED)
+        res.update({"train_moves": int(len(Y)), "train_accept": float(Y.mean()), "loss": [l0, l1]})
+    return res, w
+
+
+def learned_leg(replicas=16, train_moves=1_000_000, sweeps=150, burn=30, iters=1000):
+    """Charge hopping with the learned proposal.
+    (1) 20 chains x 300 beads (L = 60, 40 % charged, lambda_B = 3.59, lambda_D = 5,
+    hop radius 3a): bead-steps/s of Brownian dynamics with one learned-proposal MC
+    sweep every 10 steps, and tau_int per sweep of the Coulomb energy and of the
+    charge configuration, uniform against learned proposal (trained on 1e6
+    uniform-proposal moves).
+    (2) A strongly coupled small system (4 x 30 beads, L = 18, 40 charges,
+    lambda_B = 4) where the uniform proposal mixes slowly: uniform at 3a and 8a
+    against the learned proposal at 8a."""
+    nb, nc, L = 300, 20, 60.0
+    n = nb * nc
+    nq = int(0.4 * n)
+    pos = grow_chains([(nb, nc, L)], 5)[0]
+    rng = np.random.default_rng(11)
+    track = np.sort(rng.choice(n, nq, replace=False)).astype(np.int32)
+    sign = np.where(np.arange(nq) % 2 == 0, 1, -1).astype(np.int32)
+
+    def params(r, hop, nb=nb, nc=nc, L=L, lb=3.59):
+        return S.default_params(nb=nb, nc=nc, L=L, flow_type=S.FLOW_SHEAR, edot=50.0 / 175.0, dt=5e-4,
+                                spring_fene=0, kappas=200.0, qmax=2.0, epsilon=0.1, lambda_b=lb, lambda_d=5.0,
+                                barrier=0.0, barrier2=0.0, precision=S.PREC_MIXED, lanczos_tol=1e-6,
+                                lanczos_maxit=128, seed=0x5EED0000 + r, hop_radius=hop)
+    out = {"workload": f"{nc}x{nb} beads, L={L}, {nq} charges (+-1), lambda_B=3.59, lambda_D=5, shear Wi=50, "
+                       f"hop radius 3",
+           "mlp": "16-64-64-1 ReLU, BF16 weights, FP32 accumulation; trained by Adam on the GPU gradient"}
+    bd = S.Context(params(0, 3.0))
+    bd.upload_positions(pos[:, 0], pos[:, 1], pos[:, 2])
+    bd.upload_charges([nq], track, sign)
+    bd.run(0, 2, use_graph=True)
+    x, y, z = bd.download_positions()
+    R = replicas
+    ctx = S.Context([params(r, 3.0) for r in range(R)])
+    ctx.upload_positions(np.tile(x, R), np.tile(y, R), np.tile(z, R))
+    ctx.remap(2)
+    c5, w = mh_study(ctx, R, nq, n, track, sign, train_moves, sweeps, burn, iters, tag="20x300 hop 3")
+    ctx.close()
+    bd.mlp_set_weights(w)
+    bd.sync()
+    t0 = time.perf_counter()
+    nblk = 3
+    for b in range(nblk):
+        bd.run(2 + 10 * b, 10, use_graph=True)
+        bd.mc_sweep(S.MC_MH_LEARNED)
+    bd.sync()
+    el = time.perf_counter() - t0
+    bd.close()
+    out.update({"bead_steps_per_s_bd_plus_mc": n * 10 * nblk / el, "mc_every": 10, "c5": c5,
+                "c5_tau_E_ratio": c5["uniform"]["tau_E"] / c5["learned"]["tau_E"],
+                "c5_tau_config_ratio": c5["uniform"]["tau_config"] / c5["learned"]["tau_config"]})
+    # strongly coupled small system
+    nb2, nc2, L2, nq2, R2 = 30, 4, 18.0, 40, 64
+    pos2 = grow_chains([(nb2, nc2, L2)], 6)[0]
+    tr2 = np.sort(np.random.default_rng(3).choice(nb2 * nc2, nq2, replace=False)).astype(np.int32)
+    sg2 = np.where(np.arange(nq2) % 2 == 0, 1, -1).astype(np.int32)
+    small = {}
+    for hop, learn in ((3.0, False), (8.0, True)):
+        ctx = S.Context([params(r, hop, nb2, nc2, L2, 4.0) for r in range(R2)])
+        ctx.upload_positions(np.tile(pos2[:, 0], R2), np.tile(pos2[:, 1], R2), np.tile(pos2[:, 2], R2))
+        ctx.remap(0)
+        small[hop], _ = mh_study(ctx, R2, nq2, nb2 * nc2, tr2, sg2, 250_000, 600, 100, 1500, learn,
+                                 tag=f"4x30 lambda_B=4 hop {hop:g}")
+        ctx.close()
+    out.update({"strong_coupling": {"workload": "4x30 beads, L=18, 40 charges, lambda_B=4, lambda_D=5, 64 replicas",
+                                    "hop3": small[3.0], "hop8": small[8.0],
+                                    "tau_E_ratio_uniform3_over_learned8":
+                                        small[3.0]["uniform"]["tau_E"] / small[8.0]["learned"]["tau_E"],
+                                    "tau_E_ratio_same_radius8":
+                                        small[8.0]["uniform"]["tau_E"] / small[8.0]["learned"]["tau_E"],
+                                    "tau_config_ratio_uniform3_over_learned8":
+                                        small[3.0]["uniform"]["tau_config"] / small[8.0]["learned"]["tau_config"]}})
+    return out
+
+
 def c2_charges(n, seed=7):
     rng = np.random.default_rng(seed)
     track = rng.choice(n, n // 2, replace=False).astype(np.int32)
@@ -257,13 +406,13 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--workload", choices=["all", "c2", "batched"], default="all",
+    ap.add_argument("--workload", choices=["all", "c2", "batched", "learned"], default="all",
                     help="c2: only the 8000-bead shear line; batched: only the replica ensemble")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 256 MiB device-to-device three times (HBM counter calibration)")
     ap.add_argument("--batched-replicas", type=int, default=64)
     ap.add_argument("--batched-steps", type=int, default=3)
-    ap.add_argument("--plain-replicas", type=int, default=8)
+    ap.add_argument("--plain-replicas", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -278,6 +427,9 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes": a.numel() * 4, "copies": 3}))
+        return
+    if args.workload == "learned":
+        print(json.dumps({"learned": learned_leg()}, default=float))
         return
     if args.workload == "batched":
         print(json.dumps({"batched": batched_leg(args.batched_replicas, args.batched_steps, 1, args.plain_replicas)}))

@@ -279,6 +279,9 @@ struct smdlt_ctx {
   unsigned char* occ = nullptr;
   long long occ_stride = 0;
   int* log_off = nullptr;
+  float4* p32 = nullptr;      // mixed precision: FP32 positions and scaled input vector
+  float4* v32 = nullptr;
+  float4* rtab = nullptr;     // mixed precision: real-space coefficient tables
   float* mlp_w = nullptr;     // BF16-rounded proposal network weights (FP32 storage)
   float* mh_feats = nullptr;  // features of each MH attempt of the last sweep [nq_total][16]
   int* mh_log = nullptr;      // its log [nq_total][6]
@@ -610,6 +613,65 @@ static int choose_mobility(smdlt_ctx* c, int r) {
   return 0;
 }
 
+// Beenakker real-space coefficients for r >= 2 (a = 1): M = t1 I + t2 d d^T / r^2
+// (S/main26.c:1727-1745), FP64 on the host for the mixed-precision tables.
+static void rpy_real_host(double r, double alpha, double& t1, double& t2) {
+  const double rr = r * r, ir = 1.0 / r, ir3 = ir / rr;
+  const double a2 = alpha * alpha, a3 = a2 * alpha, a5 = a3 * a2, a7 = a5 * a2;
+  const double ec = erfc(alpha * r), g = exp(-a2 * rr) / sqrt(M_PI);
+  const double C1 = 0.75 * ir + 0.5 * ir3, C3 = 0.75 * ir - 1.5 * ir3;
+  const double C2 = 4 * a7 * rr * rr + 3 * a3 * rr - 20 * a5 * rr - 4.5 * alpha + 14 * a3 + alpha / rr;
+  const double C4 = -4 * a7 * rr * rr - 3 * a3 * rr + 16 * a5 * rr + 1.5 * alpha - 2 * a3 - 3 * alpha / rr;
+  t1 = C1 * ec + C2 * g;
+  t2 = C3 * ec + C4 * g;
+}
+
+// Cubic coefficients per interval [2 + i/RTAB_INV, 2 + (i+1)/RTAB_INV) of t1(r) and
+// t2(r)/r^2, interpolating at the 4 Chebyshev nodes of the interval (FP64 solve).
+static void build_rtab(double alpha, int ntab, float* out) {
+  const double h = 1.0 / RTAB_INV;
+  double tn[4], V[4][4];
+  for (int k = 0; k < 4; ++k) {
+    tn[k] = 0.5 * (1.0 - cos((2 * k + 1) * M_PI / 8.0));
+    for (int p = 0; p < 4; ++p) V[k][p] = pow(tn[k], p);
+  }
+  // invert the 4x4 Vandermonde once (Gauss-Jordan)
+  double A[4][8];
+  for (int k = 0; k < 4; ++k)
+    for (int p = 0; p < 8; ++p) A[k][p] = p < 4 ? V[k][p] : (p - 4 == k ? 1.0 : 0.0);
+  for (int col = 0; col < 4; ++col) {
+    int piv = col;
+    for (int k = col + 1; k < 4; ++k)
+      if (fabs(A[k][col]) > fabs(A[piv][col])) piv = k;
+    for (int p = 0; p < 8; ++p) std::swap(A[col][p], A[piv][p]);
+    double d = A[col][col];
+    for (int p = 0; p < 8; ++p) A[col][p] /= d;
+    for (int k = 0; k < 4; ++k)
+      if (k != col) {
+        double f = A[k][col];
+        for (int p = 0; p < 8; ++p) A[k][p] -= f * A[col][p];
+      }
+  }
+  for (int i = 0; i < ntab; ++i) {
+    double f1[4], f2[4];
+    for (int k = 0; k < 4; ++k) {
+      double r = 2.0 + (i + tn[k]) * h, t1, t2;
+      rpy_real_host(r, alpha, t1, t2);
+      f1[k] = t1;
+      f2[k] = t2 / (r * r);
+    }
+    for (int p = 0; p < 4; ++p) {
+      double c1 = 0, c2 = 0;
+      for (int k = 0; k < 4; ++k) {
+        c1 += A[p][4 + k] * f1[k];
+        c2 += A[p][4 + k] * f2[k];
+      }
+      out[8 * i + p] = (float)c1;
+      out[8 * i + 4 + p] = (float)c2;
+    }
+  }
+}
+
 int smdlt_create(const smdlt_params* params, int n_replicas, int device, smdlt_ctx** out) {
   if (!params || n_replicas <= 0 || !out) return SMDLT_E_ARG;
   *out = nullptr;
@@ -859,9 +921,28 @@ int smdlt_create(const smdlt_params* params, int n_replicas, int device, smdlt_c
     st |= alloc_dev(&c->brick_beads, c->nst);
     st |= alloc_dev(&c->bbase, 3 * (size_t)c->nst);
     if (hipMalloc(&c->bw, (size_t)BW_STRIDE * c->nst * esz) != hipSuccess) st |= SMDLT_E_NOMEM;
+    if (esz == 4 && (hipMalloc(&c->p32, 16 * (size_t)c->nst) != hipSuccess ||
+                     hipMalloc(&c->v32, 16 * (size_t)c->nst) != hipSuccess))
+      st |= SMDLT_E_NOMEM;
     if (st) {
       smdlt_destroy(c);
       return SMDLT_E_NOMEM;
+    }
+    if (esz == 4) {  // real-space coefficient tables, one per replica
+      int tot = 0;
+      for (int r = 0; r < n_replicas; ++r) {
+        RepDev& R = c->rh[r];
+        R.ntab = std::max(1, (int)ceil((sqrt(R.rc2) - 2.0) * RTAB_INV) + 1);
+        R.tab0 = tot;
+        tot += 2 * R.ntab;
+      }
+      std::vector<float> tab(4 * (size_t)tot);
+      for (int r = 0; r < n_replicas; ++r) build_rtab(c->rh[r].alpha, c->rh[r].ntab, tab.data() + 4 * c->rh[r].tab0);
+      if (hipMalloc(&c->rtab, 16 * (size_t)tot) != hipSuccess) {
+        smdlt_destroy(c);
+        return SMDLT_E_NOMEM;
+      }
+      HCK(hipMemcpy(c->rtab, tab.data(), 16 * (size_t)tot, hipMemcpyHostToDevice));
     }
     HCK(hipMemcpy(c->bricks, vb.data(), sizeof(BrickTask) * vb.size(), hipMemcpyHostToDevice));
   }
@@ -881,7 +962,7 @@ void smdlt_destroy(smdlt_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   destroy_graphs(c);
   if (c->h_flags) hipHostFree(c->h_flags);
-  void* ptrs[] = {c->mh_feats, c->mh_log, c->mlp_ws, c->mlp_x, c->mlp_y, c->mlp_part, c->mlp_out, c->saved, c->d_reps, c->d_slot_rep, c->d_seeds, c->rx, c->ry, c->rz, c->dxv, c->dyv, c->dzv, c->dsp, c->f,
+  void* ptrs[] = {c->rtab, c->p32, c->v32, c->mh_feats, c->mh_log, c->mlp_ws, c->mlp_x, c->mlp_y, c->mlp_part, c->mlp_out, c->saved, c->d_reps, c->d_slot_rep, c->d_seeds, c->rx, c->ry, c->rz, c->dxv, c->dyv, c->dzv, c->dsp, c->f,
                   c->z, c->mf, c->bz, c->tmp, c->tmp2, c->nrpy, c->lrpy, c->nlj, c->llj, c->overflow, c->d_err,
                   c->cell_of, c->cell_cnt, c->cell_start, c->cell_fill, c->cell_beads, c->rep_flags, c->out2, c->aff,
                   c->partial2, c->grid, c->bsp, c->tw, c->tx, c->ty, c->tz, c->bricks, c->bbase, c->brick_cnt, c->brick_start,
@@ -1321,6 +1402,9 @@ static void mobility_internal(smdlt_ctx* c, const double* v, double* u, const do
   m.nbricks = c->bk0[re] - c->bk0[rb];
   m.brick_begin = c->bk0[rb];
   m.brick_len = c->bk0[re] - c->bk0[rb];
+  m.p32 = c->p32;
+  m.v32 = c->v32;
+  m.rtab = c->rtab;
   m.prep = (c->prep_version != c->pos_version || c->prep_range != rb * 100003LL + re) ? 1 : 0;
   c->prep_version = c->pos_version;
   c->prep_range = rb * 100003LL + re;

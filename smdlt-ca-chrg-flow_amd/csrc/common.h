@@ -24,6 +24,7 @@ struct RepDev {
   long long grid0;             // element offset of the replica's grid (3 components)
   int bsp0, active;            // B-spline modulus offset; replica enabled for this launch
   int brick0, nbrick[3];       // 8x8x8-point grid bricks of the spreading gather
+  int tab0, ntab;              // mixed precision: real-space coefficient table (float4 pairs) and its intervals
   // lattice in the wrap frame
   double L, invL, L1xp, L2xp, L2yp, delx;
   double ct, st;               // rotation lab -> wrap frame at this step (KRBC)
@@ -246,6 +247,14 @@ void launch_coulomb(hipStream_t s, const RepDev* reps, int rep_begin, int nrep, 
                     const double* ry, const double* rz, const int* qtrack, const int* qsign, double* f, int max_nq);
 
 // k_mobility.hip
+// Workgroups are dealt to the 8 XCDs round-robin by workgroup id.  This maps
+// the ids so that XCD x receives one contiguous range of logical blocks (its
+// beads' neighbours then live in its own L2).
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+  const int x = b & 7, i = b >> 3, q = nb >> 3, rem = nb & 7;
+  return x * q + (x < rem ? x : rem) + i;
+}
+
 struct MobArgs {
   const RepDev* reps;
   const int* slot_rep;
@@ -275,7 +284,15 @@ struct MobArgs {
   int nbricks, brick_begin, brick_len;
   int prep;                 // recompute base/weights/binning (positions or lattice changed)
   int* scan_tmp;
+  // mixed precision: FP32 copies of the positions (per position version) and of
+  // the scaled input vector (per product), 16-byte aligned for one load per bead
+  float4* p32;
+  float4* v32;
+  // mixed precision: per replica, for r in [2, rc) on intervals of 1/RTAB_INV,
+  // cubic coefficients (in t = frac) of t1(r) and of t2(r)/r^2: rtab[2i], rtab[2i+1]
+  const float4* rtab;
 };
+#define RTAB_INV 16
 template <typename T> void launch_mobility(hipStream_t s, const MobArgs& a);
 
 // k_lanczos.hip

@@ -295,7 +295,7 @@ void launch_mc_energy(hipStream_t st, const RepDev* reps, int rep_begin, int nre
 // 2 hop + feature radius of x (all any forward or reverse feature can see) are
 // gathered once per attempt in ascending order; features are summed by one
 // wavefront per candidate.  dE is exact FP64 over all charges.
-#define MH_MAXL 1024
+#define MH_MAXL 4096
 #define MH_MAXC 128
 #define MLP_NP (16 * 64 + 64 + 64 * 64 + 64 + 64 + 1)
 

@@ -136,6 +136,7 @@ __global__ void k_bead_prep(MobArgs a) {
 #pragma unroll
     for (int j = 0; j < P; ++j) bw[d * MAX_ORDER + j] = W[d][j];
   }
+  if (a.p32) a.p32[s] = make_float4((float)a.rx[s], (float)a.ry[s], (float)a.rz[s], 0.f);
   atomicAdd(&a.brick_cnt[brick_of(R, base)], 1);
 }
 
@@ -151,7 +152,8 @@ __global__ void k_brick_fill(MobArgs a) {
   a.brick_beads[a.brick_start[bi] + k] = s;
 }
 
-// Lanczos normalisation: vout = v / beta[r] (the gather and the real-space pass read vout)
+// Lanczos normalisation vout = v / beta[r] (the gather and the real-space pass
+// read vout) and, in mixed precision, the FP32 copy v32 of the scaled vector.
 __global__ void k_scale_v(MobArgs a) {
   int s = a.slot_begin + blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= a.slot_begin + a.nslots) return;
@@ -160,7 +162,13 @@ __global__ void k_scale_v(MobArgs a) {
   const RepDev& R = a.reps[r];
   if (!R.active || (a.done && a.done[r]) || s - R.slot0 >= R.n) return;
   double sc = a.beta ? (a.beta[r] > 0.0 ? 1.0 / a.beta[r] : 0.0) : 1.0;  // z = 0 gives y = 0
-  for (int c = 0; c < 3; ++c) a.vout[3 * s + c] = a.v[3 * s + c] * sc;
+  double v0 = a.v[3 * s] * sc, v1 = a.v[3 * s + 1] * sc, v2 = a.v[3 * s + 2] * sc;
+  if (a.vout) {
+    a.vout[3 * s] = v0;
+    a.vout[3 * s + 1] = v1;
+    a.vout[3 * s + 2] = v2;
+  }
+  if (a.v32) a.v32[s] = make_float4((float)v0, (float)v1, (float)v2, 0.f);
 }
 
 // Spreading as a gather: one workgroup per 8x8x8 brick of one replica's grid,
@@ -183,35 +191,52 @@ __global__ __launch_bounds__(512) void k_spread_gather(MobArgs a) {
   const int g0 = bt.b0 * BRICK + (tid & 7), g1 = bt.b1 * BRICK + ((tid >> 3) & 7), g2 = bt.b2 * BRICK + (tid >> 6);
   const double* vin = a.vout ? a.vout : a.v;
   const T* bw = (const T*)a.bw;
-  T acc0 = 0, acc1 = 0, acc2 = 0;
-  for (int nbk = 0; nbk < 8; ++nbk) {
-    int c0 = (bt.b0 + (nbk & 1)) % R.nbrick[0];
-    int c1 = (bt.b1 + ((nbk >> 1) & 1)) % R.nbrick[1];
-    int c2 = (bt.b2 + (nbk >> 2)) % R.nbrick[2];
+  // the 8 source bricks (this one and its +1 neighbours): starts and counts in one step
+  __shared__ int nbeg[8], noff[9];
+  if (tid < 8) {
+    int c0 = (bt.b0 + (tid & 1)) % R.nbrick[0];
+    int c1 = (bt.b1 + ((tid >> 1) & 1)) % R.nbrick[1];
+    int c2 = (bt.b2 + (tid >> 2)) % R.nbrick[2];
     int bi = R.brick0 + (c2 * R.nbrick[1] + c1) * R.nbrick[0] + c0;
-    int beg = a.brick_start[bi], cnt = a.brick_cnt[bi];
-    for (int k0 = 0; k0 < cnt; k0 += GATHER_CHUNK) {
-      int m = min(GATHER_CHUNK, cnt - k0);
-      __syncthreads();
-      for (int k = tid; k < m * 3; k += blockDim.x) {
-        int kk = k / 3, d = k - 3 * kk;
-        int s = a.brick_beads[beg + k0 + kk];
-        sb[k] = a.bbase[3 * s + d];
+    nbeg[tid] = a.brick_start[bi];
+    noff[tid + 1] = a.brick_cnt[bi];
+  }
+  __syncthreads();
+  if (tid == 0) {
+    noff[0] = 0;
+    for (int k = 0; k < 8; ++k) noff[k + 1] += noff[k];
+  }
+  __syncthreads();
+  const int total = noff[8];
+  T acc0 = 0, acc1 = 0, acc2 = 0;
+  for (int k0 = 0; k0 < total; k0 += GATHER_CHUNK) {
+    const int m = min(GATHER_CHUNK, total - k0);
+    if (k0 > 0) __syncthreads();
+    for (int k = tid; k < m * 3; k += blockDim.x) {
+      int kk = k / 3, d = k - 3 * kk;
+      int gi = k0 + kk, nbk = 0;
+      while (gi >= noff[nbk + 1]) ++nbk;
+      int s = a.brick_beads[nbeg[nbk] + gi - noff[nbk]];
+      sb[k] = a.bbase[3 * s + d];
+      if (a.v32) {
+        const float4 v4 = a.v32[s];
+        sv[k] = (T)(d == 0 ? v4.x : d == 1 ? v4.y : v4.z);
+      } else {
         sv[k] = (T)vin[3 * s + d];
-#pragma unroll
-        for (int j = 0; j < P; ++j) sw[k * P + j] = bw[(size_t)BW_STRIDE * s + d * MAX_ORDER + j];
       }
-      __syncthreads();
-      for (int k = 0; k < m; ++k) {
-        int d0 = (sb[3 * k] - g0) & (K0 - 1);
-        int d1 = (sb[3 * k + 1] - g1) & (K1 - 1);
-        int d2 = (sb[3 * k + 2] - g2) & (K2 - 1);
-        if (d0 < P && d1 < P && d2 < P) {
-          T w = sw[(3 * k) * P + d0] * sw[(3 * k + 1) * P + d1] * sw[(3 * k + 2) * P + d2];
-          acc0 += w * sv[3 * k];
-          acc1 += w * sv[3 * k + 1];
-          acc2 += w * sv[3 * k + 2];
-        }
+#pragma unroll
+      for (int j = 0; j < P; ++j) sw[k * P + j] = bw[(size_t)BW_STRIDE * s + d * MAX_ORDER + j];
+    }
+    __syncthreads();
+    for (int k = 0; k < m; ++k) {
+      int d0 = (sb[3 * k] - g0) & (K0 - 1);
+      int d1 = (sb[3 * k + 1] - g1) & (K1 - 1);
+      int d2 = (sb[3 * k + 2] - g2) & (K2 - 1);
+      if (d0 < P && d1 < P && d2 < P) {
+        T w = sw[(3 * k) * P + d0] * sw[(3 * k + 1) * P + d1] * sw[(3 * k + 2) * P + d2];
+        acc0 += w * sv[3 * k];
+        acc1 += w * sv[3 * k + 1];
+        acc2 += w * sv[3 * k + 2];
       }
     }
   }
@@ -398,10 +423,13 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_z_green(MobArgs a) {
 // One wavefront per bead.  Lanes split the interpolation stencil rows and the
 // real-space list; each lane keeps compensated FP32 (or FP64) sums, which are
 // combined across the wavefront in FP64.
+#ifndef PAIR_UNROLL
+#define PAIR_UNROLL 4
+#endif
 template <typename T, int P>
 __global__ __launch_bounds__(256) void k_mob_final(MobArgs a) {
   const int lane = threadIdx.x & 63;
-  int s = a.slot_begin + blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  int s = a.slot_begin + xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (s >= a.slot_begin + a.nslots) return;
   int r = a.slot_rep[s];
   if (r < 0) return;
@@ -450,6 +478,77 @@ __global__ __launch_bounds__(256) void k_mob_final(MobArgs a) {
   // real-space part over the list
   const T alpha = (T)R.alpha, rc2 = (T)R.rc2;
   const int m = a.nrpy[s];
+  if constexpr (sizeof(T) == 4) {  // mixed precision: FP32 positions and vector, one 16-byte load each per partner
+    const float4 pi = a.p32[s];
+    const float rd0 = (float)R.rd[0], rd1 = (float)R.rd[1], rd2 = (float)R.rd[2], rd3 = (float)R.rd[3];
+    const float rb0 = (float)R.rb[0], rb1 = (float)R.rb[1], rb2 = (float)R.rb[2], rb3 = (float)R.rb[3];
+    const float Lz = (float)R.L, iLz = (float)R.invL, rcf = (float)R.rc2;
+    const int* row = a.lrpy + (size_t)s * a.cap_rpy;
+    bool overlap = false;
+    // PAIR_UNROLL partners per lane in flight: all loads issued before any arithmetic
+    for (int k0 = lane; k0 < m; k0 += 64 * PAIR_UNROLL) {
+      int jj[PAIR_UNROLL];
+      float4 pj[PAIR_UNROLL], vj[PAIR_UNROLL];
+#pragma unroll
+      for (int u = 0; u < PAIR_UNROLL; ++u) jj[u] = (k0 + 64 * u < m) ? row[k0 + 64 * u] : -1;
+#pragma unroll
+      for (int u = 0; u < PAIR_UNROLL; ++u) {
+        const int j = jj[u] < 0 ? s : jj[u];
+        pj[u] = a.p32[j];
+        vj[u] = a.v32[j];
+      }
+#pragma unroll
+      for (int u = 0; u < PAIR_UNROLL; ++u) {
+        if (jj[u] < 0) continue;
+        const int j = jj[u];
+        float dx = pj[u].x - pi.x, dy = pj[u].y - pi.y, dz = pj[u].z - pi.z;
+        const float n1 = rintf(rd0 * dx + rd1 * dy), n2 = rintf(rd2 * dx + rd3 * dy);
+        dx -= n1 * rb0 + n2 * rb2;
+        dy -= n1 * rb1 + n2 * rb3;
+        dz -= Lz * rintf(dz * iLz);
+        const float rr = dx * dx + dy * dy + dz * dz;
+        if (rr < 4.f) {
+          overlap = true;  // evaluated below in FP64
+        } else if (rr < rcf) {
+          // t1 and t2/r^2 from the replica's cubic table (FP64-fitted, ~1e-7 relative)
+          const float x = (sqrtf(rr) - 2.f) * (float)RTAB_INV;
+          const int it = min((int)x, R.ntab - 1);
+          const float t = x - (float)it;
+          const float4 ca = a.rtab[R.tab0 + 2 * it], cb = a.rtab[R.tab0 + 2 * it + 1];
+          const float t1 = ca.x + t * (ca.y + t * (ca.z + t * ca.w));
+          const float b2 = cb.x + t * (cb.y + t * (cb.z + t * cb.w));
+          const float dv = (dx * vj[u].x + dy * vj[u].y + dz * vj[u].z) * b2;
+          acc[0].add((T)(t1 * vj[u].x + dv * dx));
+          acc[1].add((T)(t1 * vj[u].y + dv * dy));
+          acc[2].add((T)(t1 * vj[u].z + dv * dz));
+        }
+      }
+    }
+    // overlapping pairs (r < 2, rare in a physical configuration): the overlap
+    // correction cancels 1/r^3 terms, so they are evaluated in FP64 from the FP64
+    // positions, in a second pass only by the wavefronts that met one
+    if (__ballot(overlap)) {
+      for (int k = lane; k < m; k += 64) {
+        const int j = row[k];
+        double ex = a.rx[j] - xi, ey = a.ry[j] - yi, ez = a.rz[j] - zi;
+        float fx = a.p32[j].x - pi.x, fy = a.p32[j].y - pi.y, fz = a.p32[j].z - pi.z;
+        const float n1 = rintf(rd0 * fx + rd1 * fy), n2 = rintf(rd2 * fx + rd3 * fy);
+        fx -= n1 * rb0 + n2 * rb2;
+        fy -= n1 * rb1 + n2 * rb3;
+        fz -= Lz * rintf(fz * iLz);
+        if (!(fx * fx + fy * fy + fz * fz < 4.f)) continue;  // the same test as the first pass
+        min_image_reduced(R, ex, ey, ez);
+        double r2 = ex * ex + ey * ey + ez * ez, rd = sqrt(r2), t1, t2;
+        rpy_real<double>(rd, R.alpha, t1, t2);
+        double vx = vv[3 * j] * sc, vy = vv[3 * j + 1] * sc, vz = vv[3 * j + 2] * sc;
+        double dv = (ex * vx + ey * vy + ez * vz) * t2 / r2;
+        acc[0].add((T)(t1 * vx + dv * ex));
+        acc[1].add((T)(t1 * vy + dv * ey));
+        acc[2].add((T)(t1 * vz + dv * ez));
+      }
+    }
+  }
+  if constexpr (sizeof(T) == 8)
   for (int k = lane; k < m; k += 64) {
     int j = a.lrpy[(size_t)s * a.cap_rpy + k];
     double dx = a.rx[j] - xi, dy = a.ry[j] - yi, dz = a.rz[j] - zi;
@@ -498,7 +597,7 @@ template <typename T, int P> static void launch_mob_p(hipStream_t st, const MobA
     launch_scan(st, a.brick_cnt + a.brick_begin, a.brick_start + a.brick_begin, a.brick_len);
     k_brick_fill<<<nt, 256, 0, st>>>(a);
   }
-  if (a.vout && a.nslots > 0) k_scale_v<<<nt, 256, 0, st>>>(a);
+  if ((a.vout || a.v32) && a.nslots > 0) k_scale_v<<<nt, 256, 0, st>>>(a);
   mark(1);
   if (a.nbricks > 0) k_spread_gather<T, P><<<a.nbricks, 512, 0, st>>>(a);
   mark(2);

@@ -370,5 +370,15 @@ class Context:
         return e
 
 
+def mlp_init(seed=0):
+    """He-style initial weights of the hop-proposal network, packed W1[64][16], b1,
+    W2[64][64], b2, W3[64], b3 (FP32)."""
+    rng = np.random.default_rng(seed)
+    W1 = rng.normal(0, np.sqrt(2 / 16), (64, 16))
+    W2 = rng.normal(0, np.sqrt(2 / 64), (64, 64))
+    W3 = rng.normal(0, np.sqrt(1 / 64), 64)
+    return np.concatenate([W1.ravel(), np.zeros(64), W2.ravel(), np.zeros(64), W3, [0.0]]).astype(np.float32)
+
+
 def stage_names():
     return [lib().smdlt_stage_name(i).decode() for i in range(8)]
