@@ -181,10 +181,17 @@ static double choose_kmax(double alpha, double tol) {
   return k;
 }
 
-static int next_pow2(int x) {
-  int p = 16;
-  while (p < x) p <<= 1;
-  return p;
+// FFT lengths: multiples of 8 (grid bricks) of the form 2^a 3^b 5^c, 16..KMAX
+static bool good_fft_len(int k) {
+  if (k < 16 || k > KMAX || k % 8) return false;
+  for (int f : {2, 3, 5})
+    while (k % f == 0) k /= f;
+  return k == 1;
+}
+static int next_fft_len(int x) {
+  int k = std::max(16, x);
+  while (k <= KMAX && !good_fft_len(k)) ++k;
+  return k;  // > KMAX when x is too large (rejected by the caller)
 }
 
 // B-spline modulus |b(m)|^2 = 1/|sum_{k=0}^{p-2} M_p(k+1) e^{2 pi i m k/K}|^2 (Essmann et al. 1995)
@@ -520,7 +527,7 @@ static MobChoice mob_candidate(double rc, int order, double tol, double skin, co
   m.order = order;
   m.alpha = choose_alpha(rc, 0.1 * tol);
   double h = grid_h_alpha(order, tol) / m.alpha;
-  for (int d = 0; d < 3; ++d) m.K[d] = next_pow2((int)ceil(len[d] / h));
+  for (int d = 0; d < 3; ++d) m.K[d] = next_fft_len((int)ceil(len[d] / h));
   double rl = rc + skin;
   double chain = std::min((double)nb, 2.0 * pow(rl / 2.0, 1.7));
   m.nbr = (double)n / vol * (4.0 / 3.0) * M_PI * rl * rl * rl + chain;
@@ -578,12 +585,12 @@ static int choose_mobility(smdlt_ctx* c, int r) {
   if (p.ewald_alpha > 0) {
     best.alpha = p.ewald_alpha;
     double hh = grid_h_alpha(best.order, tol) / best.alpha;
-    for (int d = 0; d < 3; ++d) best.K[d] = next_pow2((int)ceil(len[d] / hh));
+    for (int d = 0; d < 3; ++d) best.K[d] = next_fft_len((int)ceil(len[d] / hh));
   }
   if (p.grid[0] > 0 && p.grid[1] > 0 && p.grid[2] > 0)
     for (int d = 0; d < 3; ++d) best.K[d] = p.grid[d];
   for (int d = 0; d < 3; ++d)
-    if (best.K[d] < 16 || best.K[d] > KMAX || (best.K[d] & (best.K[d] - 1))) return SMDLT_E_ARG;
+    if (!good_fft_len(best.K[d])) return SMDLT_E_ARG;
   if (best.order != 4 && best.order != 6 && best.order != 8) return SMDLT_E_ARG;
   double rc = best.rc;
   R.K[0] = best.K[0];
@@ -802,8 +809,6 @@ int smdlt_create(const smdlt_params* params, int n_replicas, int device, smdlt_c
     if (e != hipSuccess) st |= SMDLT_E_NOMEM;
     e = hipMalloc(&c->bsp, std::max<size_t>(1, (size_t)bsp_off * esz));
     if (e != hipSuccess) st |= SMDLT_E_NOMEM;
-    e = hipMalloc(&c->tw, KMAX * 2 * esz);
-    if (e != hipSuccess) st |= SMDLT_E_NOMEM;
   }
   st |= alloc_dev(&c->V, (size_t)(c->maxit + 1) * n3);
   st |= alloc_dev(&c->W, n3);
@@ -832,12 +837,32 @@ int smdlt_create(const smdlt_params* params, int n_replicas, int device, smdlt_c
   }
   HCK(hipMemcpy(c->d_slot_rep, sr.data(), sizeof(int) * c->nst, hipMemcpyHostToDevice));
   HCK(hipMemcpy(c->d_seeds, seeds.data(), sizeof(uint64_t) * n_replicas, hipMemcpyHostToDevice));
-  // twiddles exp(-2 pi i t / KMAX) and B-spline moduli
+  // twiddles exp(-2 pi i t / K), one table per distinct line length, and B-spline moduli
   {
-    std::vector<double> twd(2 * KMAX);
-    for (int t = 0; t < KMAX; ++t) {
-      twd[2 * t] = cos(2 * M_PI * t / KMAX);
-      twd[2 * t + 1] = -sin(2 * M_PI * t / KMAX);
+    std::vector<int> lens;
+    std::vector<double> twd;
+    for (int r = 0; r < n_replicas; ++r)
+      for (int d = 0; d < 3; ++d) {
+        const int K = c->rh[r].K[d];
+        int off = -1, acc = 0;
+        for (int L : lens) {
+          if (L == K) off = acc;
+          acc += L;
+        }
+        if (off < 0) {
+          off = acc;
+          lens.push_back(K);
+          for (int t = 0; t < K; ++t) {
+            twd.push_back(cos(2 * M_PI * t / K));
+            twd.push_back(-sin(2 * M_PI * t / K));
+          }
+        }
+        c->rh[r].twoff[d] = off;
+      }
+    const size_t ntw = twd.size() / 2;
+    if (hipMalloc(&c->tw, ntw * 2 * esz) != hipSuccess) {
+      smdlt_destroy(c);
+      return SMDLT_E_NOMEM;
     }
     std::vector<double> bs(bsp_off);
     for (int r = 0; r < n_replicas; ++r) {
@@ -850,11 +875,11 @@ int smdlt_create(const smdlt_params* params, int n_replicas, int device, smdlt_c
       }
     }
     if (esz == 8) {
-      HCK(hipMemcpy(c->tw, twd.data(), 16 * KMAX, hipMemcpyHostToDevice));
+      HCK(hipMemcpy(c->tw, twd.data(), 16 * ntw, hipMemcpyHostToDevice));
       HCK(hipMemcpy(c->bsp, bs.data(), 8 * bs.size(), hipMemcpyHostToDevice));
     } else {
       std::vector<float> tf(twd.begin(), twd.end()), bf(bs.begin(), bs.end());
-      HCK(hipMemcpy(c->tw, tf.data(), 8 * KMAX, hipMemcpyHostToDevice));
+      HCK(hipMemcpy(c->tw, tf.data(), 8 * ntw, hipMemcpyHostToDevice));
       HCK(hipMemcpy(c->bsp, bf.data(), 4 * bf.size(), hipMemcpyHostToDevice));
     }
   }
@@ -1355,10 +1380,8 @@ int smdlt_forces(smdlt_ctx* c, double* fx, double* fy, double* fz) {
 
 // u = M v for the active replica range (device pointers).  beta/vout/partial/done
 // are the Lanczos hooks (nullptr for a plain product).
-static void mobility_internal(smdlt_ctx* c, const double* v, double* u, const double* beta, double* vout,
-                              double* partial, const int* done) {
-  StageScope sc(c, ST_MOB);
-  int rb = c->cur_rep_begin, re = c->cur_rep_end;
+static MobArgs mob_args(smdlt_ctx* c, int rb, int re, const double* v, double* u, const double* beta, double* vout,
+                        double* partial, const int* done) {
   const RepDev& a = c->rh[rb];
   const RepDev& b = c->rh[re - 1];
   MobArgs m;
@@ -1405,9 +1428,10 @@ static void mobility_internal(smdlt_ctx* c, const double* v, double* u, const do
   m.p32 = c->p32;
   m.v32 = c->v32;
   m.rtab = c->rtab;
-  m.prep = (c->prep_version != c->pos_version || c->prep_range != rb * 100003LL + re) ? 1 : 0;
-  c->prep_version = c->pos_version;
-  c->prep_range = rb * 100003LL + re;
+  return m;
+}
+
+static void mob_launch(smdlt_ctx* c, MobArgs& m) {
   hipEvent_t ev[9];
   m.ev = nullptr;
   if (c->prof) {
@@ -1420,6 +1444,22 @@ static void mobility_internal(smdlt_ctx* c, const double* v, double* u, const do
     launch_mobility<float>(c->stream, m);
   if (m.ev)  // kernel k runs between ev[k] and ev[k+1]; each event is returned once
     for (int k = 0; k < 8; ++k) c->timers.push_back({ev[k], ev[k + 1], ST_K_ZERO + k, k == 0});
+}
+
+// One launch sequence for every replica of the range.  (Processing replicas in
+// groups whose grids fit the 256 MB last-level cache was measured slower on
+// 64 replicas, 380k beads: the smaller launches lose more to tails than the
+// grid passes gain from the cache.)
+static void mobility_internal(smdlt_ctx* c, const double* v, double* u, const double* beta, double* vout,
+                              double* partial, const int* done) {
+  StageScope sc(c, ST_MOB);
+  const int rb = c->cur_rep_begin, re = c->cur_rep_end;
+  MobArgs m = mob_args(c, rb, re, v, u, beta, vout, partial, done);
+  m.prep = (c->prep_version != c->pos_version || c->prep_range != rb * 100003LL + re) ? 1 : 0;
+  c->prep_version = c->pos_version;
+  c->prep_range = rb * 100003LL + re;
+  m.phase = 3;
+  mob_launch(c, m);
 }
 
 static int ensure_lists(smdlt_ctx* c) {

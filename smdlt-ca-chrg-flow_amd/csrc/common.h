@@ -5,7 +5,7 @@
 #include <stdint.h>
 
 #define SLOT_ALIGN 64       // beads of a replica are padded to whole wavefronts
-#define KMAX 512            // largest FFT length (twiddle table size)
+#define KMAX 512            // largest FFT length (lengths are 2^a 3^b 5^c, multiples of 8)
 #define FFT_TILE 1024       // complex elements per component per FFT workgroup
 #define FFT_THREADS 256
 #define MAX_ORDER 8         // largest B-spline order
@@ -25,6 +25,7 @@ struct RepDev {
   int bsp0, active;            // B-spline modulus offset; replica enabled for this launch
   int brick0, nbrick[3];       // 8x8x8-point grid bricks of the spreading gather
   int tab0, ntab;              // mixed precision: real-space coefficient table (float4 pairs) and its intervals
+  int twoff[3];                // offsets of the twiddle tables exp(-2 pi i t / K[d]) (complex elements)
   // lattice in the wrap frame
   double L, invL, L1xp, L2xp, L2yp, delx;
   double ct, st;               // rotation lab -> wrap frame at this step (KRBC)
@@ -291,6 +292,7 @@ struct MobArgs {
   // mixed precision: per replica, for r in [2, rc) on intervals of 1/RTAB_INV,
   // cubic coefficients (in t = frac) of t1(r) and of t2(r)/r^2: rtab[2i], rtab[2i+1]
   const float4* rtab;
+  int phase;                // bit 0: bead preparation and input scaling; bit 1: spread .. final
 };
 #define RTAB_INV 16
 template <typename T> void launch_mobility(hipStream_t s, const MobArgs& a);

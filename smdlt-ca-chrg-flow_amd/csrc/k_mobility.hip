@@ -17,18 +17,39 @@ template <typename T> struct Prec {
 };
 
 // -------------------------------------------------------------------- FFT core
-// One in-place Stockham stage of radix R over nl lines of length K in LDS:
-// every thread first loads its butterflies into registers, then (after a
-// barrier) writes them to their autosorted positions.
+// One in-place Stockham stage of radix R (2, 3, 4 or 5) over nl lines of length
+// K in LDS: every thread first loads its butterflies into registers, then
+// (after a barrier) writes them to their autosorted positions.  tw holds
+// exp(-2 pi i t / K), t < K, for this line length.
+template <typename C> __device__ __forceinline__ C cscale(C a, decltype(a.x) s) {
+  C r;
+  r.x = a.x * s;
+  r.y = a.y * s;
+  return r;
+}
+// multiply by -i (forward) or +i (inverse)
+template <typename C> __device__ __forceinline__ C crot(C d, int dir) {
+  C r;
+  if (dir < 0) {
+    r.x = d.y;
+    r.y = -d.x;
+  } else {
+    r.x = -d.y;
+    r.y = d.x;
+  }
+  return r;
+}
+
 template <typename C, int R, int MAXB>
 __device__ __forceinline__ void fft_stage(C* sh, int K, int pitch, int nl, int Ns, int dir, const C* __restrict__ tw) {
-  const int T = blockDim.x, t = threadIdx.x;
+  typedef decltype(C().x) T;
+  const int TH = blockDim.x, t = threadIdx.x;
   const int KR = K / R;
   const int nb = nl * KR;
   C v[MAXB][R];
 #pragma unroll
   for (int q = 0; q < MAXB; ++q) {
-    int b = t + q * T;
+    int b = t + q * TH;
     if (b < nb) {
       int l = b / KR, j = b - l * KR;
       const C* src = sh + l * pitch + j;
@@ -37,16 +58,16 @@ __device__ __forceinline__ void fft_stage(C* sh, int K, int pitch, int nl, int N
     }
   }
   __syncthreads();
-  const int tstep = KMAX / (Ns * R);
+  const int tstep = K / (Ns * R);
 #pragma unroll
   for (int q = 0; q < MAXB; ++q) {
-    int b = t + q * T;
+    int b = t + q * TH;
     if (b < nb) {
       int l = b / KR, j = b - l * KR;
-      int jm = j & (Ns - 1);
+      int jm = j % Ns;
 #pragma unroll
       for (int r = 1; r < R; ++r) {
-        C w = tw[(jm * r * tstep) & (KMAX - 1)];
+        C w = tw[jm * r * tstep];
         if (dir > 0) w = cconj(w);
         v[q][r] = cmul(v[q][r], w);
       }
@@ -54,37 +75,70 @@ __device__ __forceinline__ void fft_stage(C* sh, int K, int pitch, int nl, int N
       if (R == 2) {
         dst[0] = cadd(v[q][0], v[q][1]);
         dst[Ns] = csub(v[q][0], v[q][1]);
-      } else {
+      } else if (R == 4) {
         C a = cadd(v[q][0], v[q][R > 2 ? 2 : 0]), bb = csub(v[q][0], v[q][R > 2 ? 2 : 0]);
-        C c = cadd(v[q][1], v[q][R > 2 ? 3 : 1]), d = csub(v[q][1], v[q][R > 2 ? 3 : 1]);
-        C id;  // -i*d for forward, +i*d for inverse
-        if (dir < 0) {
-          id.x = d.y;
-          id.y = -d.x;
-        } else {
-          id.x = -d.y;
-          id.y = d.x;
-        }
+        C c = cadd(v[q][1], v[q][R > 3 ? 3 : 1]), d = csub(v[q][1], v[q][R > 3 ? 3 : 1]);
+        C id = crot(d, dir);  // -i*d for forward, +i*d for inverse
         dst[0] = cadd(a, c);
         dst[Ns] = cadd(bb, id);
         dst[2 * Ns] = csub(a, c);
         dst[3 * Ns] = csub(bb, id);
+      } else if (R == 3) {
+        // y0 = a + (b + c); y1,2 = a - (b + c)/2 -+ i sin(2pi/3) (b - c) (forward)
+        const T s3 = (T)0.86602540378443864676;
+        C x0 = v[q][0], x1 = v[q][1], x2 = v[q][R > 2 ? 2 : 0];
+        C sm = cadd(x1, x2), df = csub(x1, x2);
+        C m = csub(x0, cscale(sm, (T)0.5));
+        C rt = cscale(crot(df, dir), s3);
+        dst[0] = cadd(x0, sm);
+        dst[Ns] = cadd(m, rt);
+        dst[2 * Ns] = csub(m, rt);
+      } else if (R == 5) {
+        const T c1 = (T)0.30901699437494742410, c2 = (T)-0.80901699437494742410;
+        const T s1 = (T)0.95105651629515357212, s2 = (T)0.58778525229247312917;
+        C x0 = v[q][0], x1 = v[q][1], x2 = v[q][R > 2 ? 2 : 0], x3 = v[q][R > 3 ? 3 : 0], x4 = v[q][R > 4 ? 4 : 0];
+        C a1 = cadd(x1, x4), b1 = csub(x1, x4), a2 = cadd(x2, x3), b2 = csub(x2, x3);
+        C m1 = cadd(x0, cadd(cscale(a1, c1), cscale(a2, c2)));
+        C m2 = cadd(x0, cadd(cscale(a1, c2), cscale(a2, c1)));
+        // forward: y_k = m_k - i (s-combination); crot gives -i for forward
+        C r1 = crot(cadd(cscale(b1, s1), cscale(b2, s2)), dir);
+        C r2 = crot(csub(cscale(b1, s2), cscale(b2, s1)), dir);
+        dst[0] = cadd(x0, cadd(a1, a2));
+        dst[Ns] = cadd(m1, r1);
+        dst[2 * Ns] = cadd(m2, r2);
+        dst[3 * Ns] = csub(m2, r2);
+        dst[4 * Ns] = csub(m1, r1);
       }
     }
   }
   __syncthreads();
 }
 
-// FFT of nl lines (nl*K <= MULT*FFT_TILE): radix-4 stages, then radix 2 if K
-// is an odd power of two.
+// FFT of nl lines of length K = 2^a 3^b 5^c (nl*K <= MULT*FFT_TILE): radix-4
+// stages, a radix-2 stage for an odd power of two, then radix 3 and radix 5.
 template <typename C, int MULT>
 __device__ __forceinline__ void fft_lines(C* sh, int K, int pitch, int nl, int dir, const C* __restrict__ tw) {
-  int Ns = 1;
-  while (K / Ns >= 4) {
+  int Ns = 1, rest = K;
+  while (rest % 4 == 0) {
     fft_stage<C, 4, MULT * FFT_TILE / (4 * FFT_THREADS)>(sh, K, pitch, nl, Ns, dir, tw);
     Ns *= 4;
+    rest /= 4;
   }
-  if (K / Ns == 2) fft_stage<C, 2, MULT * FFT_TILE / (2 * FFT_THREADS)>(sh, K, pitch, nl, Ns, dir, tw);
+  if (rest % 2 == 0) {
+    fft_stage<C, 2, MULT * FFT_TILE / (2 * FFT_THREADS)>(sh, K, pitch, nl, Ns, dir, tw);
+    Ns *= 2;
+    rest /= 2;
+  }
+  while (rest % 3 == 0) {
+    fft_stage<C, 3, (MULT * FFT_TILE + 3 * FFT_THREADS - 1) / (3 * FFT_THREADS)>(sh, K, pitch, nl, Ns, dir, tw);
+    Ns *= 3;
+    rest /= 3;
+  }
+  while (rest % 5 == 0) {
+    fft_stage<C, 5, (MULT * FFT_TILE + 5 * FFT_THREADS - 1) / (5 * FFT_THREADS)>(sh, K, pitch, nl, Ns, dir, tw);
+    Ns *= 5;
+    rest /= 5;
+  }
 }
 
 // ------------------------------------------------------------------- spreading
@@ -99,7 +153,8 @@ __device__ __forceinline__ void frac_weights(const RepDev& R, double x, double y
     double u = s[d] * (double)R.K[d];
     double fl = floor(u);
     T fr = (T)(u - fl);
-    base[d] = ((int)fl) & (R.K[d] - 1);
+    int bi = ((int)fl) % R.K[d];
+    base[d] = bi < 0 ? bi + R.K[d] : bi;
     bspline_weights<T, P>(fr, W[d]);
   }
 }
@@ -150,6 +205,23 @@ __global__ void k_brick_fill(MobArgs a) {
   int bi = brick_of(R, a.bbase + 3 * s);
   int k = atomicAdd(&a.brick_fill[bi], 1);
   a.brick_beads[a.brick_start[bi] + k] = s;
+}
+
+// Bead lists of the bricks in ascending slot order (k_brick_fill appends in
+// arrival order), so the spreading sums do not depend on scheduling.
+__global__ void k_brick_sort(MobArgs a) {
+  int bi = a.brick_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (bi >= a.brick_begin + a.brick_len) return;
+  int* L = a.brick_beads + a.brick_start[bi];
+  const int n = a.brick_cnt[bi];
+  for (int i = 1; i < n; ++i) {
+    int v = L[i], k = i - 1;
+    while (k >= 0 && L[k] > v) {
+      L[k + 1] = L[k];
+      --k;
+    }
+    L[k + 1] = v;
+  }
 }
 
 // Lanczos normalisation vout = v / beta[r] (the gather and the real-space pass
@@ -229,9 +301,10 @@ __global__ __launch_bounds__(512) void k_spread_gather(MobArgs a) {
     }
     __syncthreads();
     for (int k = 0; k < m; ++k) {
-      int d0 = (sb[3 * k] - g0) & (K0 - 1);
-      int d1 = (sb[3 * k + 1] - g1) & (K1 - 1);
-      int d2 = (sb[3 * k + 2] - g2) & (K2 - 1);
+      int d0 = sb[3 * k] - g0, d1 = sb[3 * k + 1] - g1, d2 = sb[3 * k + 2] - g2;
+      d0 += d0 < 0 ? K0 : 0;
+      d1 += d1 < 0 ? K1 : 0;
+      d2 += d2 < 0 ? K2 : 0;
       if (d0 < P && d1 < P && d2 < P) {
         T w = sw[(3 * k) * P + d0] * sw[(3 * k + 1) * P + d1] * sw[(3 * k + 2) * P + d2];
         acc0 += w * sv[3 * k];
@@ -263,7 +336,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_x(MobArgs a, int dir) {
   const long long ncomp = (long long)R.K[0] * R.K[1] * R.K[2];
   T* g = (T*)a.grid + R.grid0 + tk.comp * ncomp;
   C* gc = (C*)g;
-  const C* tw = (const C*)a.tw;
+  const C* tw = (const C*)a.tw + R.twoff[0];
   if (dir < 0) {
     for (int e = threadIdx.x; e < nl * K; e += blockDim.x) {
       int l = e / K, x = e - l * K;
@@ -277,7 +350,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_x(MobArgs a, int dir) {
     fft_lines<C, 1>(sh, K, pitch, nl, -1, tw);
     for (int e = threadIdx.x; e < nl * Kh; e += blockDim.x) {
       int l = e / Kh, k = e - l * Kh;
-      C zk = sh[l * pitch + k], zm = sh[l * pitch + ((K - k) & (K - 1))];
+      C zk = sh[l * pitch + k], zm = sh[l * pitch + (k == 0 ? 0 : K - k)];
       C A, B;
       A.x = (T)0.5 * (zk.x + zm.x);
       A.y = (T)0.5 * (zk.y - zm.y);
@@ -337,7 +410,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_y(MobArgs a, int dir) {
     sh[l * pitch + k] = gc[((long long)z * K + k) * Kh + x];
   }
   __syncthreads();
-  fft_lines<C, 1>(sh, K, pitch, nl, dir, (const C*)a.tw);
+  fft_lines<C, 1>(sh, K, pitch, nl, dir, (const C*)a.tw + R.twoff[1]);
   for (int e = threadIdx.x; e < nl * K; e += blockDim.x) {
     int k = e / nl, l = e - k * nl;
     int L = tk.l0 + l, z = L / Kh, x = L - z * Kh;
@@ -366,7 +439,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_z_green(MobArgs a) {
     for (int c = 0; c < 3; ++c) sh[c * cpitch + l * pitch + k] = gc[c][idx];
   }
   __syncthreads();
-  const C* tw = (const C*)a.tw;
+  const C* tw = (const C*)a.tw + R.twoff[2];
   fft_lines<C, 3>(sh, K, pitch, 3 * nl, -1, tw);
   const T* bsp = (const T*)a.bsp + R.bsp0;
   const T ia2 = (T)(1.0 / (4.0 * R.alpha * R.alpha));
@@ -458,14 +531,16 @@ __global__ __launch_bounds__(256) void k_mob_final(MobArgs a) {
     const T* g = (const T*)a.grid + R.grid0;
     const long long ncomp = (long long)R.K[0] * R.K[1] * R.K[2];
     const int K0 = R.K[0], K1 = R.K[1], K2 = R.K[2];
-    int g3 = (base[2] - j3) & (K2 - 1);
-    int g2 = (base[1] - j2) & (K1 - 1);
+    int g3 = base[2] - j3, g2 = base[1] - j2;
+    g3 += g3 < 0 ? K2 : 0;
+    g2 += g2 < 0 ? K1 : 0;
     T w23 = pick<T, P>(W[2], j3) * pick<T, P>(W[1], j2);
     const T* row = g + ((long long)g3 * K1 + g2) * K0;
     T s0 = 0, s1 = 0, s2 = 0;
 #pragma unroll
     for (int j1 = 0; j1 < P; ++j1) {
-      int g1 = (base[0] - j1) & (K0 - 1);
+      int g1 = base[0] - j1;
+      g1 += g1 < 0 ? K0 : 0;
       T w = W[0][j1];
       s0 += w * row[g1];
       s1 += w * row[g1 + ncomp];
@@ -590,15 +665,19 @@ template <typename T, int P> static void launch_mob_p(hipStream_t st, const MobA
   };
   mark(0);
   int nt = (a.nslots + 255) / 256;
-  if (a.prep && a.nslots > 0) {
-    launch_fill_i32(st, a.brick_cnt + a.brick_begin, 0, a.brick_len);
-    launch_fill_i32(st, a.brick_fill + a.brick_begin, 0, a.brick_len);
-    k_bead_prep<T, P><<<nt, 256, 0, st>>>(a);
-    launch_scan(st, a.brick_cnt + a.brick_begin, a.brick_start + a.brick_begin, a.brick_len);
-    k_brick_fill<<<nt, 256, 0, st>>>(a);
+  if (a.phase & 1) {  // per-bead preparation over the whole replica range
+    if (a.prep && a.nslots > 0) {
+      launch_fill_i32(st, a.brick_cnt + a.brick_begin, 0, a.brick_len);
+      launch_fill_i32(st, a.brick_fill + a.brick_begin, 0, a.brick_len);
+      k_bead_prep<T, P><<<nt, 256, 0, st>>>(a);
+      launch_scan(st, a.brick_cnt + a.brick_begin, a.brick_start + a.brick_begin, a.brick_len);
+      k_brick_fill<<<nt, 256, 0, st>>>(a);
+      if (a.brick_len > 0) k_brick_sort<<<(a.brick_len + 255) / 256, 256, 0, st>>>(a);
+    }
+    if ((a.vout || a.v32) && a.nslots > 0) k_scale_v<<<nt, 256, 0, st>>>(a);
   }
-  if ((a.vout || a.v32) && a.nslots > 0) k_scale_v<<<nt, 256, 0, st>>>(a);
   mark(1);
+  if (!(a.phase & 2)) return;
   if (a.nbricks > 0) k_spread_gather<T, P><<<a.nbricks, 512, 0, st>>>(a);
   mark(2);
   if (a.ntx > 0) k_fft_x<T><<<a.ntx, FFT_THREADS, 0, st>>>(a, -1);

@@ -176,6 +176,19 @@ def test_mobility_fp64_equals_pme_restatement(flow):
     assert rel(u, un) < 1e-12
 
 
+@pytest.mark.parametrize("grid", [(40, 48, 72), (24, 80, 120), (96, 32, 40)])
+def test_mobility_mixed_radix_grids_equal_pme_restatement(grid):
+    # line lengths 2^a 3^b 5^c exercise the radix-3 and radix-5 Stockham stages
+    ctx, lat, pos = mob_case(1, prec=S.PREC_FP64, grid=grid, spline_order=6)
+    cfg = ctx.mobility_config()
+    assert cfg["grid"] == grid
+    v = np.random.default_rng(12).normal(size=pos.size)
+    u = ctx.mobility_apply(v)
+    un = P.mobility_apply(pos, v, (lat.L1xp, lat.L2xp, lat.L2yp, lat.L), cfg["alpha"], cfg["rc"], cfg["grid"],
+                          cfg["order"])
+    assert rel(u, un) < 1e-12
+
+
 @pytest.mark.parametrize("flow,prec", [(1, S.PREC_MIXED), (1, S.PREC_FP64), (2, S.PREC_MIXED), (0, S.PREC_MIXED)])
 def test_mobility_matches_dense_ewald(flow, prec):
     # M.F against the exact dense FP64 Ewald sum: <= 1e-5 relative
@@ -346,6 +359,33 @@ def test_batched_replicas_equal_single_contexts():
     n0 = params[0].nb * params[0].nc
     n1 = params[1].nb * params[1].nc
     assert rel(u_plain[3 * n0:3 * (n0 + n1)], ub[3 * n0:3 * (n0 + n1)]) < 1e-12
+
+
+def test_large_grid_replicas_equal_single_contexts():
+    # four mixed-precision replicas on 128^3 grids (100 MB of grids in one launch
+    # sequence) give exactly the products of one context per replica
+    specs = [(1, 40.0, 50, 4), (2, 44.0, 40, 5), (1, 36.0, 60, 3), (0, 40.0, 30, 6)]
+    params, poss = [], []
+    for k, (flow, L, nb, nc) in enumerate(specs):
+        params.append(S.default_params(nb=nb, nc=nc, L=L, flow_type=flow, edot=0.4, dt=1e-3,
+                                       precision=S.PREC_MIXED, rc_real=8.0, grid=(128, 128, 128), spline_order=6))
+        poss.append(random_walk(nb, nc, L, 40 + k))
+    big = S.Context(params)
+    allp = np.concatenate(poss)
+    big.upload_positions(*xyz(allp))
+    big.remap(60)
+    v = np.random.default_rng(1).normal(size=3 * len(allp))
+    ub = big.mobility_apply(v)
+    off = 0
+    for k, prm in enumerate(params):
+        one = S.Context(prm)
+        one.upload_positions(*xyz(poss[k]))
+        one.remap(60)
+        n = prm.nb * prm.nc
+        u1 = one.mobility_apply(v[3 * off:3 * (off + n)])
+        assert rel(ub[3 * off:3 * (off + n)], u1) < 1e-12
+        off += n
+        one.close()
 
 
 # ----------------------------------------------------------------- MC
