@@ -377,6 +377,52 @@ def learned_leg(replicas=16, train_moves=1_000_000, sweeps=150, burn=30, iters=1
     return out
 
 
+def c3_leg(steps=5, warmup=2):
+    """Planar extensional flow with Kraynik-Reinelt remaps: 200 chains x 250 beads
+    (50000 beads), L = 160, Wi = 20 (edot = 20/175), 30 % charged (+-1, zero net),
+    Debye length 3, dt = 2e-4, mixed precision, Lanczos tol 1e-6."""
+    nb, nc, L = 250, 200, 160.0
+    n = nb * nc
+    p = S.default_params(nb=nb, nc=nc, L=L, flow_type=S.FLOW_EXTENSION, edot=20.0 / 175.0, dt=2e-4, spring_fene=0,
+                         kappas=200.0, qmax=2.0, epsilon=0.1, lambda_b=1.0, lambda_d=3.0, lj_ext_form=1,
+                         precision=S.PREC_MIXED, lanczos_tol=1e-6, lanczos_maxit=128, seed=0x5EED0003)
+    t0 = time.perf_counter()
+    pos = grow_chains([(nb, nc, L)], 4)
+    ctx = S.Context(p)
+    pos = to_lab_frame(ctx, [{"flow": S.FLOW_EXTENSION}], pos)[0]
+    rng = np.random.default_rng(13)
+    nq = int(0.3 * n) // 2 * 2
+    track = rng.choice(n, nq, replace=False).astype(np.int32)
+    sign = np.where(np.arange(nq) < nq // 2, 1, -1).astype(np.int32)
+    ctx.upload_positions(pos[:, 0], pos[:, 1], pos[:, 2])
+    ctx.upload_charges([nq], track, sign)
+    cfg = ctx.mobility_config()
+    print(f"[bench] c3: {n} beads, {nq} charges, setup {time.perf_counter() - t0:.1f}s, {cfg}", file=sys.stderr,
+          flush=True)
+    ctx.run(0, warmup, use_graph=True)
+    ctx.sync()
+    t1 = time.perf_counter()
+    code = ctx.run(warmup, steps, use_graph=True)
+    ctx.sync()
+    el = time.perf_counter() - t1
+    it, stt, ch = ctx.lanczos_status()
+    ctx.set_profiling(True)
+    ctx.run(warmup + steps, 1, use_graph=False)
+    stages = {k: round(v[0], 4) for k, v in ctx.stage_times().items() if v[1]}
+    ctx.set_profiling(False)
+    x, y, z = ctx.download_positions()
+    ctx.close()
+    out = {"workload": "planar extension (KRBC), 200x250 beads, L=160, Wi=20, 15000 charges, lambda_D=3, dt=2e-4",
+           "value": n * steps / el, "unit": "bead-steps/s", "ms_per_step": 1e3 * el / steps,
+           "lanczos_iters_last_step": int(it[0]), "lanczos_converged": bool(stt[0] == 1),
+           "grid": list(cfg["grid"]), "rc": cfg["rc"], "ewald_alpha": round(cfg["alpha"], 5),
+           "finite": bool(np.all(np.isfinite(x)) and np.all(np.isfinite(y)) and np.all(np.isfinite(z))),
+           "stage_ms_one_eager_step": stages}
+    print(f"[bench] c3: {out['value']:.4g} bead-steps/s ({out['ms_per_step']:.1f} ms/step)", file=sys.stderr,
+          flush=True)
+    return out
+
+
 def c2_charges(n, seed=7):
     rng = np.random.default_rng(seed)
     track = rng.choice(n, n // 2, replace=False).astype(np.int32)
@@ -406,7 +452,7 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--workload", choices=["all", "c2", "batched", "learned"], default="all",
+    ap.add_argument("--workload", choices=["all", "c2", "batched", "learned", "c3"], default="all",
                     help="c2: only the 8000-bead shear line; batched: only the replica ensemble")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 256 MiB device-to-device three times (HBM counter calibration)")
@@ -427,6 +473,9 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes": a.numel() * 4, "copies": 3}))
+        return
+    if args.workload == "c3":
+        print(json.dumps({"c3": c3_leg()}))
         return
     if args.workload == "learned":
         print(json.dumps({"learned": learned_leg()}, default=float))
