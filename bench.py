@@ -377,6 +377,30 @@ def learned_leg(replicas=16, train_moves=1_000_000, sweeps=150, burn=30, iters=1
     return out
 
 
+def c1_leg(steps=200, warmup=5):
+    """The reference's own small case: one FENE chain of 100 beads (b_max = 10,
+    H = 30), L = 60, shear Wi = 10 (edot = 10/175), dt = 1e-3, no charges, FP64,
+    Lanczos tol 1e-6 (<= 30 iterations), mobility settings auto-tuned to 1e-5."""
+    p = S.default_params(nb=100, nc=1, L=60.0, flow_type=S.FLOW_SHEAR, edot=10.0 / 175.0, dt=1e-3, spring_fene=1,
+                         kappas=30.0, qmax=10.0, precision=S.PREC_FP64, lanczos_tol=1e-6, lanczos_maxit=30,
+                         seed=0x5EED0001)
+    pos = grow_chains([(100, 1, 60.0)], 8)[0]
+    ctx = S.Context(p)
+    ctx.upload_positions(pos[:, 0], pos[:, 1], pos[:, 2])
+    ctx.run(0, warmup, use_graph=True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    code = ctx.run(warmup, steps, use_graph=True)
+    ctx.sync()
+    el = time.perf_counter() - t0
+    it, stt, ch = ctx.lanczos_status()
+    cfg = ctx.mobility_config()
+    ctx.close()
+    return {"workload": "1 FENE chain x 100 beads, L=60, shear Wi=10, dt=1e-3, FP64", "value": 100 * steps / el,
+            "unit": "bead-steps/s", "ms_per_step": 1e3 * el / steps, "lanczos_iters_last_step": int(it[0]),
+            "grid": list(cfg["grid"]), "rc": cfg["rc"], "ewald_alpha": round(cfg["alpha"], 5), "return_code": code}
+
+
 def c3_leg(steps=5, warmup=2):
     """Planar extensional flow with Kraynik-Reinelt remaps: 200 chains x 250 beads
     (50000 beads), L = 160, Wi = 20 (edot = 20/175), 30 % charged (+-1, zero net),
@@ -430,17 +454,21 @@ def c2_charges(n, seed=7):
     return track, sign
 
 
-# Algorithmic HBM bytes per launch of each mobility kernel (one M.v):
-# bead arrays are read/written once, grids once per pass (FP32 mixed path).
-def kernel_bytes(n, G, nbr, order=8, esz=4):
+# Algorithmic HBM bytes per launch of each mobility kernel (one M.v, mixed
+# precision): bead arrays read or written once, grids once per pass, the pair
+# list once.  The real-space pass ("k_mob_final" stage) reads the FP32 position
+# and vector copies (16 + 16 B), the FP64 input (24 B) for the self term, the
+# stencil data (12 + 12p B), writes u (24 B), reads the grid once and its list:
+# 4 B per cluster pair in mixed precision (4 B per bead pair in FP64).
+def kernel_bytes(n, G, list_entries, order=8, esz=4):
     g = 3 * G * esz
     BW = 3 * order * esz
     return {
-        "k_bead_prep": n * (24 + 24 + 12 + BW) ,
-        "k_spread_gather": n * (12 + 24 + BW) + g,
+        "k_bead_prep": n * (24 + 16 + 12 + BW),
+        "k_spread_gather": n * (12 + 16 + BW) + g,
         "k_fft_x(fwd)": 2 * g, "k_fft_y(fwd)": 2 * g, "k_fft_z_green": 2 * g,
         "k_fft_y(inv)": 2 * g, "k_fft_x(inv)": 2 * g,
-        "k_mob_final": n * (24 + 24 + 24) + 4 * nbr + g,
+        "k_mob_final": n * (16 + 16 + 24 + 24 + 12 + BW) + 4 * list_entries + g,
     }
 
 
@@ -452,7 +480,7 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--workload", choices=["all", "c2", "batched", "learned", "c3"], default="all",
+    ap.add_argument("--workload", choices=["all", "c2", "batched", "learned", "c3", "c1"], default="all",
                     help="c2: only the 8000-bead shear line; batched: only the replica ensemble")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 256 MiB device-to-device three times (HBM counter calibration)")
@@ -473,6 +501,9 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes": a.numel() * 4, "copies": 3}))
+        return
+    if args.workload == "c1":
+        print(json.dumps({"c1": c1_leg()}))
         return
     if args.workload == "c3":
         print(json.dumps({"c3": c3_leg()}))
@@ -509,7 +540,8 @@ def main():
     st = ctx.stage_times()
     ctx.set_profiling(False)
     G = cfg["grid"][0] * cfg["grid"][1] * cfg["grid"][2]
-    nbr = ctx.mobility_config()["list_entries"]  # real-space pair-list entries of the current list
+    mc = ctx.mobility_config()
+    nbr = mc["cluster_pairs"] or mc["list_entries"]  # the list the real-space pass reads
     kb = kernel_bytes(n, G, nbr, cfg["order"])
     kern = {k: v for k, v in st.items() if k.startswith("k_") and v[1] > 0}
     dom = max(kern, key=lambda k: kern[k][0])
@@ -586,7 +618,8 @@ def main():
         "kernel_ms_avg": {k: v[0] / v[1] for k, v in kern.items()},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "bytes_per_launch": kb[dom],
-                     "avg_launch_ms": avg_ms, "pair_list_entries": nbr},
+                     "avg_launch_ms": avg_ms, "list_entries": nbr, "bead_pair_entries": mc["list_entries"],
+                     "note": "k_mob_final is latency/L2-bound (gathers, interpolation), not HBM-bound; see traffic"},
         "cpu_baseline": cpu,
         "batched": batched,
     }

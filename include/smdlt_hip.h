@@ -108,8 +108,9 @@ void smdlt_destroy(smdlt_ctx* ctx);
 int smdlt_num_beads(const smdlt_ctx* ctx, int replica); /* replica = -1: total */
 
 /* Effective mobility settings chosen for a replica: alpha, rc, skin, K1, K2, K3,
- * order, and the number of real-space pair-list entries after the last build. */
-int smdlt_get_mobility_config(const smdlt_ctx* ctx, int replica, double out[8]);
+ * order, the number of real-space bead-pair list entries after the last build
+ * and (mixed precision) the number of cluster-pair entries. */
+int smdlt_get_mobility_config(const smdlt_ctx* ctx, int replica, double out[9]);
 
 /* Host <-> device state.  Positions SoA over all beads (reference rx, ry, rz). */
 int smdlt_upload_positions(smdlt_ctx* ctx, const double* rx, const double* ry, const double* rz);

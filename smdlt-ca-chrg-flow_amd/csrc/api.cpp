@@ -289,6 +289,9 @@ struct smdlt_ctx {
   float4* p32 = nullptr;      // mixed precision: FP32 positions and scaled input vector
   float4* v32 = nullptr;
   float4* rtab = nullptr;     // mixed precision: real-space coefficient tables
+  // bead clusters and the cluster-pair list (mixed precision)
+  int *cl_first = nullptr, *cl_cnt = nullptr, *slot_cl = nullptr, *lcl = nullptr, *nclp = nullptr;
+  int ncl_total = 0, capc = 0, cl_words = 0;
   float* mlp_w = nullptr;     // BF16-rounded proposal network weights (FP32 storage)
   float* mh_feats = nullptr;  // features of each MH attempt of the last sweep [nq_total][16]
   int* mh_log = nullptr;      // its log [nq_total][6]
@@ -968,6 +971,34 @@ int smdlt_create(const smdlt_params* params, int n_replicas, int device, smdlt_c
         return SMDLT_E_NOMEM;
       }
       HCK(hipMemcpy(c->rtab, tab.data(), 16 * (size_t)tot, hipMemcpyHostToDevice));
+      // clusters: runs of <= 8 consecutive beads of one chain
+      std::vector<int> cf, cn, scl(c->nst, -1);
+      int maxncl = 0;
+      for (int r = 0; r < n_replicas; ++r) {
+        RepDev& R = c->rh[r];
+        R.cl0 = (int)cf.size();
+        for (int ch = 0; ch < R.nc; ++ch)
+          for (int j0 = 0; j0 < R.nb; j0 += 8) {
+            int f = R.slot0 + ch * R.nb + j0, k = std::min(8, R.nb - j0);
+            for (int q = 0; q < k; ++q) scl[f + q] = (int)cf.size();
+            cf.push_back(f);
+            cn.push_back(k);
+          }
+        R.ncl = (int)cf.size() - R.cl0;
+        maxncl = std::max(maxncl, R.ncl);
+      }
+      c->ncl_total = (int)cf.size();
+      c->cl_words = (maxncl + 31) / 32;
+      c->capc = std::min(maxncl, c->cap_rpy / 2 + 64);
+      if (4LL * c->cl_words > 65536 || alloc_dev(&c->cl_first, c->ncl_total) || alloc_dev(&c->cl_cnt, c->ncl_total) ||
+          alloc_dev(&c->slot_cl, c->nst) || alloc_dev(&c->nclp, c->ncl_total) ||
+          alloc_dev(&c->lcl, (size_t)c->capc * c->ncl_total)) {
+        smdlt_destroy(c);
+        return SMDLT_E_NOMEM;
+      }
+      HCK(hipMemcpy(c->cl_first, cf.data(), 4 * cf.size(), hipMemcpyHostToDevice));
+      HCK(hipMemcpy(c->cl_cnt, cn.data(), 4 * cn.size(), hipMemcpyHostToDevice));
+      HCK(hipMemcpy(c->slot_cl, scl.data(), 4 * scl.size(), hipMemcpyHostToDevice));
     }
     HCK(hipMemcpy(c->bricks, vb.data(), sizeof(BrickTask) * vb.size(), hipMemcpyHostToDevice));
   }
@@ -987,7 +1018,7 @@ void smdlt_destroy(smdlt_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   destroy_graphs(c);
   if (c->h_flags) hipHostFree(c->h_flags);
-  void* ptrs[] = {c->rtab, c->p32, c->v32, c->mh_feats, c->mh_log, c->mlp_ws, c->mlp_x, c->mlp_y, c->mlp_part, c->mlp_out, c->saved, c->d_reps, c->d_slot_rep, c->d_seeds, c->rx, c->ry, c->rz, c->dxv, c->dyv, c->dzv, c->dsp, c->f,
+  void* ptrs[] = {c->cl_first, c->cl_cnt, c->slot_cl, c->lcl, c->nclp, c->rtab, c->p32, c->v32, c->mh_feats, c->mh_log, c->mlp_ws, c->mlp_x, c->mlp_y, c->mlp_part, c->mlp_out, c->saved, c->d_reps, c->d_slot_rep, c->d_seeds, c->rx, c->ry, c->rz, c->dxv, c->dyv, c->dzv, c->dsp, c->f,
                   c->z, c->mf, c->bz, c->tmp, c->tmp2, c->nrpy, c->lrpy, c->nlj, c->llj, c->overflow, c->d_err,
                   c->cell_of, c->cell_cnt, c->cell_start, c->cell_fill, c->cell_beads, c->rep_flags, c->out2, c->aff,
                   c->partial2, c->grid, c->bsp, c->tw, c->tx, c->ty, c->tz, c->bricks, c->bbase, c->brick_cnt, c->brick_start,
@@ -1004,10 +1035,11 @@ void smdlt_destroy(smdlt_ctx* c) {
   delete c;
 }
 
-int smdlt_get_mobility_config(const smdlt_ctx* c, int r, double out[8]) {
+int smdlt_get_mobility_config(const smdlt_ctx* c, int r, double out[9]) {
   if (!c || r < 0 || r >= c->nrep || !out) return SMDLT_E_ARG;
   const RepDev& R = c->rh[r];
   out[7] = 0.0;  // real-space list entries of the replica (after the last build)
+  out[8] = 0.0;  // cluster-pair list entries (mixed precision)
   if (c->lists_built && R.n > 0) {
     HCK(hipSetDevice(c->device));
     std::vector<int> cnt(R.n);
@@ -1015,6 +1047,13 @@ int smdlt_get_mobility_config(const smdlt_ctx* c, int r, double out[8]) {
     double e = 0.0;
     for (int v : cnt) e += v;
     out[7] = e;
+    if (c->nclp && R.ncl > 0) {
+      std::vector<int> cc(R.ncl);
+      HCK(hipMemcpy(cc.data(), c->nclp + R.cl0, 4 * (size_t)R.ncl, hipMemcpyDeviceToHost));
+      double f = 0.0;
+      for (int v : cc) f += v;
+      out[8] = f;
+    }
   }
   out[0] = R.alpha;
   out[1] = sqrt(R.rc2);
@@ -1255,6 +1294,12 @@ static int neighbors_internal(smdlt_ctx* c, int force) {
   launch_build_lists(c->stream, c->d_reps, c->d_slot_rep, s0, ns, c->rx, c->ry, c->rz, c->cell_start, c->cell_cnt,
                      c->cell_beads, c->rep_flags, c->nst, c->cap_rpy, c->nrpy, c->lrpy, c->cap_lj, c->nlj, c->llj,
                      c->overflow);
+  if (c->lcl) {
+    const int clb = c->rh[rb].cl0, cle = c->rh[re - 1].cl0 + c->rh[re - 1].ncl;
+    launch_build_clusters(c->stream, c->d_reps, c->d_slot_rep, clb, cle - clb, c->cl_words, c->cl_first, c->cl_cnt,
+                          c->slot_cl, c->rep_flags, c->nrpy, c->lrpy, c->cap_rpy, c->capc, c->nclp, c->lcl,
+                          c->overflow);
+  }
   launch_reset_trackers(c->stream, c->d_reps, c->d_slot_rep, s0, ns, c->rep_flags, c->dxv, c->dyv, c->dzv, c->dsp);
   HCK(hipGetLastError());
   c->lists_built = true;
@@ -1274,6 +1319,12 @@ static int grow_lists(smdlt_ctx* c) {
   c->llj = nullptr;
   if (alloc_dev(&c->lrpy, (size_t)c->cap_rpy * c->nst) || alloc_dev(&c->llj, (size_t)c->cap_lj * c->nst))
     return SMDLT_E_NOMEM;
+  if (c->lcl) {
+    hipFree(c->lcl);
+    c->lcl = nullptr;
+    c->capc *= 2;
+    if (alloc_dev(&c->lcl, (size_t)c->capc * c->ncl_total)) return SMDLT_E_NOMEM;
+  }
   HCK(hipMemset(c->overflow, 0, 4));
   destroy_graphs(c);
   return 1;
@@ -1428,6 +1479,13 @@ static MobArgs mob_args(smdlt_ctx* c, int rb, int re, const double* v, double* u
   m.p32 = c->p32;
   m.v32 = c->v32;
   m.rtab = c->rtab;
+  m.cl_first = c->cl_first;
+  m.cl_cnt = c->cl_cnt;
+  m.lcl = c->lcl;
+  m.nclp = c->nclp;
+  m.capc = c->capc;
+  m.cl_begin = a.cl0;
+  m.ncl = b.cl0 + b.ncl - a.cl0;
   return m;
 }
 

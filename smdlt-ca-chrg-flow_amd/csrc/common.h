@@ -26,6 +26,7 @@ struct RepDev {
   int brick0, nbrick[3];       // 8x8x8-point grid bricks of the spreading gather
   int tab0, ntab;              // mixed precision: real-space coefficient table (float4 pairs) and its intervals
   int twoff[3];                // offsets of the twiddle tables exp(-2 pi i t / K[d]) (complex elements)
+  int cl0, ncl;                // bead clusters (<= 8 consecutive beads of one chain): first id, count
   // lattice in the wrap frame
   double L, invL, L1xp, L2xp, L2yp, delx;
   double ct, st;               // rotation lab -> wrap frame at this step (KRBC)
@@ -293,9 +294,18 @@ struct MobArgs {
   // cubic coefficients (in t = frac) of t1(r) and of t2(r)/r^2: rtab[2i], rtab[2i+1]
   const float4* rtab;
   int phase;                // bit 0: bead preparation and input scaling; bit 1: spread .. final
+  // mixed precision real space by cluster pairs: cluster c holds beads cl_first[c] ..
+  // + cl_cnt[c] - 1; lcl[c * capc + k], k < nclp[c]: clusters with a bead pair in the list
+  const int *cl_first, *cl_cnt, *lcl, *nclp;
+  int capc, cl_begin, ncl;
 };
 #define RTAB_INV 16
 template <typename T> void launch_mobility(hipStream_t s, const MobArgs& a);
+
+void launch_build_clusters(hipStream_t st, const RepDev* reps, const int* slot_rep, int cl_begin, int ncl,
+                           int max_words, const int* cl_first, const int* cl_cnt, const int* slot_cl,
+                           const int* rep_flags, const int* nrpy, const int* lrpy, int cap_rpy, int capc, int* nclp,
+                           int* lcl, int* overflow);
 
 // k_lanczos.hip
 void lz_wave_dot(hipStream_t st, const RepDev* reps, const int* slot_rep, int slot_begin, int nslots, const double* x,

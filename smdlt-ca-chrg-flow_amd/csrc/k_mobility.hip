@@ -659,6 +659,170 @@ __global__ __launch_bounds__(256) void k_mob_final(MobArgs a) {
   }
 }
 
+// Mixed precision, one workgroup (4 wavefronts) per bead cluster I (<= 8 beads):
+// the interpolation of each bead (one wavefront, a lane per stencil row), then
+// the real-space pairs by cluster pairs: lane 8*ii + jj evaluates bead ii of I
+// against bead jj of the listed cluster J, the wavefronts taking J in turn.  The
+// partner data of one J is 8 positions and 8 vectors shared by the wave, so list
+// and gather traffic fall by ~the cluster size against a bead-pair list.
+// Overlapping pairs (r < 2) are evaluated in FP64 in a second pass.  Sums are
+// compensated per lane and combined in FP64 in a fixed order.
+#define CL_WAVES 8
+template <int P>
+__global__ __launch_bounds__(64 * CL_WAVES) void k_mob_cl(MobArgs a) {
+  typedef float T;
+  const int I = a.cl_begin + xcd_block(blockIdx.x, gridDim.x);
+  const int first = a.cl_first[I], cnt = a.cl_cnt[I];
+  const int r = a.slot_rep[first];
+  const RepDev& R = a.reps[r];
+  if (!R.active || (a.done && a.done[r])) return;
+  __shared__ double part[CL_WAVES][8][3];
+  __shared__ double uin[8][3];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const double* vv = a.vout ? a.vout : a.v;
+  const double sc = (a.beta && !a.vout) ? (a.beta[r] > 0.0 ? 1.0 / a.beta[r] : 0.0) : 1.0;
+  // reciprocal part, bead ii by wavefront ii % CL_WAVES
+  for (int ii = w; ii < cnt; ii += CL_WAVES) {
+    const int s = first + ii;
+    CompSum<T> ai[3];
+    if (lane < P * P) {
+      int base[3];
+      T W[3][P];
+      const T* bw = (const T*)a.bw + (size_t)BW_STRIDE * s;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        base[d] = a.bbase[3 * s + d];
+#pragma unroll
+        for (int j = 0; j < P; ++j) W[d][j] = bw[d * MAX_ORDER + j];
+      }
+      const int j2 = lane % P, j3 = lane / P;
+      const T* g = (const T*)a.grid + R.grid0;
+      const long long ncomp = (long long)R.K[0] * R.K[1] * R.K[2];
+      const int K0 = R.K[0], K1 = R.K[1], K2 = R.K[2];
+      int g3 = base[2] - j3, g2 = base[1] - j2;
+      g3 += g3 < 0 ? K2 : 0;
+      g2 += g2 < 0 ? K1 : 0;
+      T w23 = pick<T, P>(W[2], j3) * pick<T, P>(W[1], j2);
+      const T* row = g + ((long long)g3 * K1 + g2) * K0;
+      T s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+      for (int j1 = 0; j1 < P; ++j1) {
+        int g1 = base[0] - j1;
+        g1 += g1 < 0 ? K0 : 0;
+        T wt = W[0][j1];
+        s0 += wt * row[g1];
+        s1 += wt * row[g1 + ncomp];
+        s2 += wt * row[g1 + 2 * ncomp];
+      }
+      ai[0].add(w23 * s0);
+      ai[1].add(w23 * s1);
+      ai[2].add(w23 * s2);
+    }
+    double x0 = warp_sum(ai[0].value()), x1 = warp_sum(ai[1].value()), x2 = warp_sum(ai[2].value());
+    if (lane == 0) {
+      uin[ii][0] = x0;
+      uin[ii][1] = x1;
+      uin[ii][2] = x2;
+    }
+  }
+  // real-space part by cluster pairs
+  const int ii = lane >> 3, jj = lane & 7;
+  const bool ivalid = ii < cnt;
+  const int si = first + (ivalid ? ii : 0);
+  const float4 pi = a.p32[si];
+  const float rd0 = (float)R.rd[0], rd1 = (float)R.rd[1], rd2 = (float)R.rd[2], rd3 = (float)R.rd[3];
+  const float rb0 = (float)R.rb[0], rb1 = (float)R.rb[1], rb2 = (float)R.rb[2], rb3 = (float)R.rb[3];
+  const float Lz = (float)R.L, iLz = (float)R.invL, rcf = (float)R.rc2;
+  CompSum<T> acc[3];
+  bool overlap = false;
+  const int m = a.nclp[I];
+  const int* row = a.lcl + (size_t)I * a.capc;
+  for (int k0 = w; k0 < m; k0 += CL_WAVES * PAIR_UNROLL) {
+    int sj[PAIR_UNROLL];
+    bool ok[PAIR_UNROLL];
+    float4 pj[PAIR_UNROLL], vj[PAIR_UNROLL];
+#pragma unroll
+    for (int u = 0; u < PAIR_UNROLL; ++u) {
+      const int k = k0 + CL_WAVES * u;
+      const int J = k < m ? row[k] : I;
+      const int fj = a.cl_first[J], cj = k < m ? a.cl_cnt[J] : 0;
+      ok[u] = ivalid && jj < cj && !(J == I && jj == ii);
+      sj[u] = fj + (jj < cj ? jj : 0);
+    }
+#pragma unroll
+    for (int u = 0; u < PAIR_UNROLL; ++u) {
+      pj[u] = a.p32[sj[u]];
+      vj[u] = a.v32[sj[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < PAIR_UNROLL; ++u) {
+      float dx = pj[u].x - pi.x, dy = pj[u].y - pi.y, dz = pj[u].z - pi.z;
+      const float n1 = rintf(rd0 * dx + rd1 * dy), n2 = rintf(rd2 * dx + rd3 * dy);
+      dx -= n1 * rb0 + n2 * rb2;
+      dy -= n1 * rb1 + n2 * rb3;
+      dz -= Lz * rintf(dz * iLz);
+      const float rr = dx * dx + dy * dy + dz * dz;
+      if (!ok[u]) continue;
+      if (rr < 4.f) {
+        overlap = true;
+      } else if (rr < rcf) {
+        const float x = (sqrtf(rr) - 2.f) * (float)RTAB_INV;
+        const int it = min((int)x, R.ntab - 1);
+        const float t = x - (float)it;
+        const float4 ca = a.rtab[R.tab0 + 2 * it], cb = a.rtab[R.tab0 + 2 * it + 1];
+        const float t1 = ca.x + t * (ca.y + t * (ca.z + t * ca.w));
+        const float b2 = cb.x + t * (cb.y + t * (cb.z + t * cb.w));
+        const float dv = (dx * vj[u].x + dy * vj[u].y + dz * vj[u].z) * b2;
+        acc[0].add(t1 * vj[u].x + dv * dx);
+        acc[1].add(t1 * vj[u].y + dv * dy);
+        acc[2].add(t1 * vj[u].z + dv * dz);
+      }
+    }
+  }
+  // overlapping pairs in FP64, only by wavefronts that met one
+  if (__ballot(overlap)) {
+    const double xi = a.rx[si], yi = a.ry[si], zi = a.rz[si];
+    for (int k = w; k < m; k += CL_WAVES) {
+      const int J = row[k], fj = a.cl_first[J], cj = a.cl_cnt[J];
+      if (!(ivalid && jj < cj && !(J == I && jj == ii))) continue;
+      const int j = fj + jj;
+      float fx = a.p32[j].x - pi.x, fy = a.p32[j].y - pi.y, fz = a.p32[j].z - pi.z;
+      const float n1 = rintf(rd0 * fx + rd1 * fy), n2 = rintf(rd2 * fx + rd3 * fy);
+      fx -= n1 * rb0 + n2 * rb2;
+      fy -= n1 * rb1 + n2 * rb3;
+      fz -= Lz * rintf(fz * iLz);
+      if (!(fx * fx + fy * fy + fz * fz < 4.f)) continue;  // the same test as the first pass
+      double ex = a.rx[j] - xi, ey = a.ry[j] - yi, ez = a.rz[j] - zi;
+      min_image_reduced(R, ex, ey, ez);
+      double r2 = ex * ex + ey * ey + ez * ez, rd = sqrt(r2), t1, t2;
+      rpy_real<double>(rd, R.alpha, t1, t2);
+      double vx = vv[3 * j] * sc, vy = vv[3 * j + 1] * sc, vz = vv[3 * j + 2] * sc;
+      double dv = (ex * vx + ey * vy + ez * vz) * t2 / r2;
+      acc[0].add((T)(t1 * vx + dv * ex));
+      acc[1].add((T)(t1 * vy + dv * ey));
+      acc[2].add((T)(t1 * vz + dv * ez));
+    }
+  }
+  double u0 = acc[0].value(), u1 = acc[1].value(), u2 = acc[2].value();
+  for (int o = 1; o < 8; o <<= 1) {
+    u0 += __shfl_xor(u0, o, 64);
+    u1 += __shfl_xor(u1, o, 64);
+    u2 += __shfl_xor(u2, o, 64);
+  }
+  if (jj == 0) {
+    part[w][ii][0] = u0;
+    part[w][ii][1] = u1;
+    part[w][ii][2] = u2;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3 * cnt) {
+    const int b = threadIdx.x / 3, c = threadIdx.x - 3 * b, s = first + b;
+    double t = uin[b][c];
+    for (int q = 0; q < CL_WAVES; ++q) t += part[q][b][c];
+    a.u[3 * s + c] = t + R.selfconst * (vv[3 * s + c] * sc);
+  }
+}
+
 template <typename T, int P> static void launch_mob_p(hipStream_t st, const MobArgs& a) {
   auto mark = [&](int k) {
     if (a.ev) (void)hipEventRecord(a.ev[k], st);
@@ -690,7 +854,11 @@ template <typename T, int P> static void launch_mob_p(hipStream_t st, const MobA
   mark(6);
   if (a.ntx > 0) k_fft_x<T><<<a.ntx, FFT_THREADS, 0, st>>>(a, 1);
   mark(7);
-  if (a.nslots > 0) k_mob_final<T, P><<<(a.nslots + 3) / 4, 256, 0, st>>>(a);
+  if (sizeof(T) == 4 && a.lcl) {
+    if (a.ncl > 0) k_mob_cl<P><<<a.ncl, 64 * CL_WAVES, 0, st>>>(a);
+  } else if (a.nslots > 0) {
+    k_mob_final<T, P><<<(a.nslots + 3) / 4, 256, 0, st>>>(a);
+  }
   mark(8);
 }
 

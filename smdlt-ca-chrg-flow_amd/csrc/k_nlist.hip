@@ -291,6 +291,69 @@ void launch_build_lists(hipStream_t st, const RepDev* reps, const int* slot_rep,
                                                llj, overflow);
 }
 
+// Cluster-pair list for the mixed-precision real-space pass: cluster J is listed
+// for cluster I when some bead of J is in the mobility list of some bead of I
+// (pairs within rc + skin at build time, so every pair within rc until the next
+// rebuild is covered), plus I itself.  One wavefront per cluster marks the
+// clusters in an LDS bitmap, then emits them in ascending order.
+__global__ __launch_bounds__(64) void k_build_clusters(const RepDev* __restrict__ reps,
+                                                        const int* __restrict__ slot_rep, int cl_begin, int ncl,
+                                                        const int* cl_first, const int* cl_cnt, const int* slot_cl,
+                                                        const int* rep_flags, const int* nrpy, const int* lrpy,
+                                                        int cap_rpy, int capc, int* nclp, int* lcl, int* overflow) {
+  extern __shared__ unsigned bm[];
+  const int I = cl_begin + blockIdx.x;
+  if (I >= cl_begin + ncl) return;
+  const int first = cl_first[I], cnt = cl_cnt[I];
+  const int r = slot_rep[first];
+  const RepDev& R = reps[r];
+  if (!R.active || !(rep_flags[r] & 2)) return;
+  const int lane = threadIdx.x;
+  const int nw = (R.ncl + 31) >> 5;
+  for (int w = lane; w < nw; w += 64) bm[w] = 0u;
+  __syncthreads();
+  if (lane == 0) atomicOr(&bm[(I - R.cl0) >> 5], 1u << ((I - R.cl0) & 31));
+  for (int ii = 0; ii < cnt; ++ii) {
+    const int i = first + ii, m = nrpy[i];
+    for (int k = lane; k < m; k += 64) {
+      const int cj = slot_cl[lrpy[(size_t)i * cap_rpy + k]] - R.cl0;
+      atomicOr(&bm[cj >> 5], 1u << (cj & 31));
+    }
+  }
+  __syncthreads();
+  int base = 0;
+  for (int w0 = 0; w0 < nw; w0 += 64) {
+    const int w = w0 + lane;
+    unsigned bits = w < nw ? bm[w] : 0u;
+    int c = __popc(bits), incl = c;
+    for (int o = 1; o < 64; o <<= 1) {
+      int t = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += t;
+    }
+    int pos = base + incl - c;
+    while (bits) {
+      const int b = __ffs(bits) - 1;
+      bits &= bits - 1u;
+      if (pos < capc)
+        lcl[(size_t)I * capc + pos] = R.cl0 + w * 32 + b;
+      else
+        atomicOr(overflow, 1);
+      ++pos;
+    }
+    base += __shfl(incl, 63, 64);
+  }
+  if (lane == 0) nclp[I] = min(base, capc);
+}
+
+void launch_build_clusters(hipStream_t st, const RepDev* reps, const int* slot_rep, int cl_begin, int ncl,
+                           int max_words, const int* cl_first, const int* cl_cnt, const int* slot_cl,
+                           const int* rep_flags, const int* nrpy, const int* lrpy, int cap_rpy, int capc, int* nclp,
+                           int* lcl, int* overflow) {
+  if (ncl <= 0) return;
+  k_build_clusters<<<ncl, 64, 4 * (size_t)max_words, st>>>(reps, slot_rep, cl_begin, ncl, cl_first, cl_cnt, slot_cl,
+                                                          rep_flags, nrpy, lrpy, cap_rpy, capc, nclp, lcl, overflow);
+}
+
 // Fill kernels (used instead of memset so that captured graphs replay correctly).
 template <typename T> __global__ void k_fill(T* p, T v, long long n) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;

@@ -153,10 +153,10 @@ class Context:
             pass
 
     def mobility_config(self, r=0):
-        out = np.zeros(8)
+        out = np.zeros(9)
         _chk(lib().smdlt_get_mobility_config(self.h, r, _ptr(out, C.c_double)), "config")
         return dict(alpha=float(out[0]), rc=float(out[1]), skin=float(out[2]), grid=tuple(int(x) for x in out[3:6]),
-                    order=int(out[6]), list_entries=int(out[7]))
+                    order=int(out[6]), list_entries=int(out[7]), cluster_pairs=int(out[8]))
 
     def upload_positions(self, rx, ry, rz):
         rx, ry, rz = (np.ascontiguousarray(a, np.float64) for a in (rx, ry, rz))
