@@ -987,6 +987,10 @@ int smdlt_create(const smdlt_params* params, int n_replicas, int device, smdlt_c
         R.ncl = (int)cf.size() - R.cl0;
         maxncl = std::max(maxncl, R.ncl);
       }
+      if (c->nst >= (1 << 27)) {  // cluster-list entries pack the first slot in 27 bits
+        smdlt_destroy(c);
+        return SMDLT_E_ARG;
+      }
       c->ncl_total = (int)cf.size();
       c->cl_words = (maxncl + 31) / 32;
       c->capc = std::min(maxncl, c->cap_rpy / 2 + 64);

@@ -295,7 +295,8 @@ struct MobArgs {
   const float4* rtab;
   int phase;                // bit 0: bead preparation and input scaling; bit 1: spread .. final
   // mixed precision real space by cluster pairs: cluster c holds beads cl_first[c] ..
-  // + cl_cnt[c] - 1; lcl[c * capc + k], k < nclp[c]: clusters with a bead pair in the list
+  // + cl_cnt[c] - 1; lcl[c * capc + k], k < nclp[c]: clusters with a bead pair in the
+  // list, as (first slot << 4) | count
   const int *cl_first, *cl_cnt, *lcl, *nclp;
   int capc, cl_begin, ncl;
 };

@@ -744,9 +744,9 @@ __global__ __launch_bounds__(64 * CL_WAVES) void k_mob_cl(MobArgs a) {
 #pragma unroll
     for (int u = 0; u < PAIR_UNROLL; ++u) {
       const int k = k0 + CL_WAVES * u;
-      const int J = k < m ? row[k] : I;
-      const int fj = a.cl_first[J], cj = k < m ? a.cl_cnt[J] : 0;
-      ok[u] = ivalid && jj < cj && !(J == I && jj == ii);
+      const int e = k < m ? row[k] : (first << 4);
+      const int fj = e >> 4, cj = e & 15;
+      ok[u] = ivalid && jj < cj && !(fj == first && jj == ii);
       sj[u] = fj + (jj < cj ? jj : 0);
     }
 #pragma unroll
@@ -783,8 +783,8 @@ __global__ __launch_bounds__(64 * CL_WAVES) void k_mob_cl(MobArgs a) {
   if (__ballot(overlap)) {
     const double xi = a.rx[si], yi = a.ry[si], zi = a.rz[si];
     for (int k = w; k < m; k += CL_WAVES) {
-      const int J = row[k], fj = a.cl_first[J], cj = a.cl_cnt[J];
-      if (!(ivalid && jj < cj && !(J == I && jj == ii))) continue;
+      const int e = row[k], fj = e >> 4, cj = e & 15;
+      if (!(ivalid && jj < cj && !(fj == first && jj == ii))) continue;
       const int j = fj + jj;
       float fx = a.p32[j].x - pi.x, fy = a.p32[j].y - pi.y, fz = a.p32[j].z - pi.z;
       const float n1 = rintf(rd0 * fx + rd1 * fy), n2 = rintf(rd2 * fx + rd3 * fy);

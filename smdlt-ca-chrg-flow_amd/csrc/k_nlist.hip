@@ -295,7 +295,8 @@ void launch_build_lists(hipStream_t st, const RepDev* reps, const int* slot_rep,
 // for cluster I when some bead of J is in the mobility list of some bead of I
 // (pairs within rc + skin at build time, so every pair within rc until the next
 // rebuild is covered), plus I itself.  One wavefront per cluster marks the
-// clusters in an LDS bitmap, then emits them in ascending order.
+// clusters in an LDS bitmap, then emits them in ascending order, each entry
+// packed as (first bead slot << 4) | bead count.
 __global__ __launch_bounds__(64) void k_build_clusters(const RepDev* __restrict__ reps,
                                                         const int* __restrict__ slot_rep, int cl_begin, int ncl,
                                                         const int* cl_first, const int* cl_cnt, const int* slot_cl,
@@ -334,9 +335,10 @@ __global__ __launch_bounds__(64) void k_build_clusters(const RepDev* __restrict_
     while (bits) {
       const int b = __ffs(bits) - 1;
       bits &= bits - 1u;
-      if (pos < capc)
-        lcl[(size_t)I * capc + pos] = R.cl0 + w * 32 + b;
-      else
+      if (pos < capc) {
+        const int J = R.cl0 + w * 32 + b;
+        lcl[(size_t)I * capc + pos] = (cl_first[J] << 4) | cl_cnt[J];  // first bead, count: one load per partner
+      } else
         atomicOr(overflow, 1);
       ++pos;
     }
