@@ -48,7 +48,8 @@ extern "C" {
 
 /* Brownian term. */
 #define SMDLT_NOISE_LANCZOS 0 /* M^(1/2) z by Krylov (Lanczos) */
-#define SMDLT_NOISE_TEA 1     /* the reference's CA-TEA product (S/main26.c:787-804) */
+#define SMDLT_NOISE_TEA 1     /* the reference's CA-TEA product (S/main26.c:787-804); see smdlt_set_tea */
+#define SMDLT_NOISE_FREE_DRAINING 2 /* no hydrodynamics: updateFD (S/main26.c:731-744), M = I */
 
 /* MC modes. */
 #define SMDLT_MC_REFERENCE 0 /* chargeHop's state machine and acceptance (S/main26.c:1926-2390) */
@@ -167,6 +168,10 @@ int smdlt_brownian(smdlt_ctx* ctx, const double* z, double* y, double tol, int m
 /* The reference's TEA product y_i = C_i (.) (M_ii z_i + beta sum_j M_ij z_j) with
  * the mobility of this build (S/main26.c:787-804); beta per replica, C 3N. */
 int smdlt_tea_apply(smdlt_ctx* ctx, const double* beta, const double* C, const double* z, double* y);
+
+/* TEA parameters for SMDLT_NOISE_TEA steps (the output of ewaldBin/CATEA,
+ * S/main26.c:975-1381): beta per replica, C per bead component (3N AoS). */
+int smdlt_set_tea(smdlt_ctx* ctx, const double* beta, const double* C);
 
 /* One full timestep at step t (S/main26.c:65-159, X/main14.c:68-247, iteration
  * >= 1): remap, Verlet check, forces, noise, M F + M^(1/2) z (or TEA), position

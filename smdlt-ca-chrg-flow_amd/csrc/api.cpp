@@ -254,7 +254,8 @@ struct smdlt_ctx {
   int cap_rpy = 64, cap_lj = 64;
   double nbr_est = 0;
   int *nrpy = nullptr, *lrpy = nullptr, *nlj = nullptr, *llj = nullptr, *overflow = nullptr;
-  int *cell_of = nullptr, *cell_cnt = nullptr, *cell_start = nullptr, *cell_fill = nullptr, *cell_beads = nullptr;
+  int *cell_of = nullptr, *cell_cnt = nullptr, *cell_start = nullptr, *cell_fill = nullptr, *cell_beads = nullptr,
+      *cell_tmp = nullptr;
   int ncells = 0;
   int* rep_flags = nullptr;
   double *out2 = nullptr, *aff = nullptr;
@@ -318,6 +319,7 @@ struct smdlt_ctx {
   hipGraphExec_t gtail = nullptr;
   std::vector<hipGraphExec_t> gchunks;
   int* h_flags = nullptr;  // pinned: done[nrep], iters[nrep], overflow
+  double *tea_beta = nullptr, *tea_C = nullptr, *tea_diag = nullptr;  // SMDLT_NOISE_TEA steps
   double* saved = nullptr; // positions and displacement trackers at the start of a graph step
   bool force_rebuild = false;
   int expect_iters = 0;    // largest iteration count of the previous graph step
@@ -803,6 +805,7 @@ int smdlt_create(const smdlt_params* params, int n_replicas, int device, smdlt_c
   st |= alloc_dev(&c->cell_start, c->ncells + 1);
   st |= alloc_dev(&c->cell_fill, c->ncells + 1);
   st |= alloc_dev(&c->cell_beads, c->nst);
+  st |= alloc_dev(&c->cell_tmp, c->nst);
   st |= alloc_dev(&c->rep_flags, n_replicas);
   st |= alloc_dev(&c->out2, 2 * n_replicas);
   st |= alloc_dev(&c->aff, n_replicas);
@@ -1022,9 +1025,9 @@ void smdlt_destroy(smdlt_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   destroy_graphs(c);
   if (c->h_flags) hipHostFree(c->h_flags);
-  void* ptrs[] = {c->cl_first, c->cl_cnt, c->slot_cl, c->lcl, c->nclp, c->rtab, c->p32, c->v32, c->mh_feats, c->mh_log, c->mlp_ws, c->mlp_x, c->mlp_y, c->mlp_part, c->mlp_out, c->saved, c->d_reps, c->d_slot_rep, c->d_seeds, c->rx, c->ry, c->rz, c->dxv, c->dyv, c->dzv, c->dsp, c->f,
+  void* ptrs[] = {c->tea_beta, c->tea_C, c->tea_diag, c->cl_first, c->cl_cnt, c->slot_cl, c->lcl, c->nclp, c->rtab, c->p32, c->v32, c->mh_feats, c->mh_log, c->mlp_ws, c->mlp_x, c->mlp_y, c->mlp_part, c->mlp_out, c->saved, c->d_reps, c->d_slot_rep, c->d_seeds, c->rx, c->ry, c->rz, c->dxv, c->dyv, c->dzv, c->dsp, c->f,
                   c->z, c->mf, c->bz, c->tmp, c->tmp2, c->nrpy, c->lrpy, c->nlj, c->llj, c->overflow, c->d_err,
-                  c->cell_of, c->cell_cnt, c->cell_start, c->cell_fill, c->cell_beads, c->rep_flags, c->out2, c->aff,
+                  c->cell_of, c->cell_cnt, c->cell_start, c->cell_fill, c->cell_beads, c->cell_tmp, c->rep_flags, c->out2, c->aff,
                   c->partial2, c->grid, c->bsp, c->tw, c->tx, c->ty, c->tz, c->bricks, c->bbase, c->brick_cnt, c->brick_start,
                   c->brick_fill, c->brick_beads, c->bw, c->V, c->W, c->partial, c->scal, c->bdiv, c->done,
                   c->iters, c->qtrack, c->qsign, c->qold, c->qidx, c->mc_err, c->mlp_w, c->occ, c->log_off};
@@ -1294,7 +1297,8 @@ static int neighbors_internal(smdlt_ctx* c, int force) {
   launch_fill_i32(c->stream, c->cell_fill + cb, 0, ce - cb);
   launch_cell_count(c->stream, c->d_reps, c->d_slot_rep, s0, ns, c->rx, c->ry, c->rz, c->cell_of, c->cell_cnt);
   launch_scan(c->stream, c->cell_cnt + cb, c->cell_start + cb, ce - cb);
-  launch_cell_fill(c->stream, c->d_reps, c->d_slot_rep, s0, ns, c->cell_of, c->cell_start, c->cell_fill, c->cell_beads);
+  launch_cell_fill(c->stream, c->d_reps, c->d_slot_rep, s0, ns, c->cell_of, c->cell_start, c->cell_fill, c->cell_tmp);
+  launch_cell_sort(c->stream, c->cell_start, c->cell_cnt, cb, ce - cb, c->cell_tmp, c->cell_beads);
   launch_build_lists(c->stream, c->d_reps, c->d_slot_rep, s0, ns, c->rx, c->ry, c->rz, c->cell_start, c->cell_cnt,
                      c->cell_beads, c->rep_flags, c->nst, c->cap_rpy, c->nrpy, c->lrpy, c->cap_lj, c->nlj, c->llj,
                      c->overflow);
@@ -1738,6 +1742,19 @@ int smdlt_tea_apply(smdlt_ctx* c, const double* beta, const double* C, const dou
   return st;
 }
 
+int smdlt_set_tea(smdlt_ctx* c, const double* beta, const double* C) {
+  if (!c || !beta || !C) return SMDLT_E_ARG;
+  HCK(hipSetDevice(c->device));
+  if (!c->tea_beta && (alloc_dev(&c->tea_beta, c->nrep) || alloc_dev(&c->tea_C, 3 * (size_t)c->nst) ||
+                       alloc_dev(&c->tea_diag, 3 * (size_t)c->nrep)))
+    return SMDLT_E_NOMEM;
+  std::vector<double> diag(3 * c->nrep);
+  for (int r = 0; r < c->nrep; ++r) self_mobility(c->rh[r], &diag[3 * r]);
+  HCK(hipMemcpy(c->tea_diag, diag.data(), 8 * diag.size(), hipMemcpyHostToDevice));
+  HCK(hipMemcpy(c->tea_beta, beta, 8 * (size_t)c->nrep, hipMemcpyHostToDevice));
+  return gather_aos(c, C, c->tea_C);
+}
+
 // One timestep for the active replica range; noise_dev: device noise (nullptr =
 // generate).  Lanczos early exit on the host only when host_exit.
 static int step_internal(smdlt_ctx* c, uint64_t t, double* noise_dev, bool host_exit) {
@@ -1745,6 +1762,8 @@ static int step_internal(smdlt_ctx* c, uint64_t t, double* noise_dev, bool host_
   const RepDev& a = c->rh[rb];
   const RepDev& b = c->rh[re - 1];
   int s0 = a.slot0, ns = b.slot0 + pad_slots(b.n) - s0;
+  const smdlt_params& p = c->P[rb];
+  if (p.noise_mode == SMDLT_NOISE_TEA && !c->tea_beta) return SMDLT_E_STATE;  // smdlt_set_tea first
   int st = remap_internal(c, t);
   if (st) return st;
   // host noise is the reference's lab-frame R; rotate it into the wrap frame
@@ -1772,10 +1791,18 @@ static int step_internal(smdlt_ctx* c, uint64_t t, double* noise_dev, bool host_
       zz = c->z;
     }
   }
-  mobility_internal(c, c->f, c->mf, nullptr, nullptr, nullptr, nullptr);
-  const smdlt_params& p = c->P[rb];
-  st = brownian_internal(c, zz, c->bz, p.lanczos_tol > 0 ? p.lanczos_tol : 1e-6, c->maxit, host_exit);
-  if (st) return st;
+  if (p.noise_mode == SMDLT_NOISE_FREE_DRAINING) {  // updateFD: M = I (S/main26.c:731-744)
+    launch_copy_f64(c->stream, c->mf, c->f, 3LL * c->nst);
+    launch_copy_f64(c->stream, c->bz, zz, 3LL * c->nst);
+  } else if (p.noise_mode == SMDLT_NOISE_TEA) {  // updateHI's TEA product (S/main26.c:787-804)
+    mobility_internal(c, c->f, c->mf, nullptr, nullptr, nullptr, nullptr);
+    mobility_internal(c, zz, c->tmp2, nullptr, nullptr, nullptr, nullptr);
+    lz_tea(c->stream, c->d_slot_rep, s0, ns, c->tmp2, zz, c->tea_diag, c->tea_beta, c->tea_C, c->bz);
+  } else {
+    mobility_internal(c, c->f, c->mf, nullptr, nullptr, nullptr, nullptr);
+    st = brownian_internal(c, zz, c->bz, p.lanczos_tol > 0 ? p.lanczos_tol : 1e-6, c->maxit, host_exit);
+    if (st) return st;
+  }
   {
     StageScope sc(c, ST_UPDATE);
     launch_update_positions(c->stream, c->d_reps, c->d_slot_rep, s0, ns, c->rx, c->ry, c->rz, c->mf, c->bz, c->dxv,
@@ -1799,7 +1826,7 @@ int smdlt_step(smdlt_ctx* c, uint64_t t, const double* noise) {
   if (st) return st;
   st = check_err(c);
   if (st) return st;
-  int conv = read_iters(c, nullptr, nullptr);
+  int conv = c->P[c->cur_rep_begin].noise_mode == SMDLT_NOISE_LANCZOS ? read_iters(c, nullptr, nullptr) : 0;
   int g = grow_lists(c);
   collect_timers(c);
   if (g < 0) return g;
@@ -1889,8 +1916,10 @@ int smdlt_run(smdlt_ctx* c, uint64_t t0, int nsteps, int use_graph) {
   int st = 0;
   for (int k = 0; k < nsteps; ++k) {
     uint64_t t = t0 + k;
-    if (!use_graph || !c->lists_built) {  // the first step builds the lists after its remap
-      st = step_internal(c, t, nullptr, !c->lists_built);
+    if (!use_graph || !c->lists_built || c->P[c->cur_rep_begin].noise_mode != SMDLT_NOISE_LANCZOS) {
+      // the first step builds the lists after its remap; TEA and free-draining steps run eagerly.
+      // An eager step checks for list overflow on the host and grows the lists before its forces.
+      st = step_internal(c, t, nullptr, true);
       if (st) return st;
       continue;
     }
@@ -1938,7 +1967,7 @@ int smdlt_run(smdlt_ctx* c, uint64_t t0, int nsteps, int use_graph) {
   HCK(hipStreamSynchronize(c->stream));
   st = check_err(c);
   if (st) return st;
-  int conv = read_iters(c, nullptr, nullptr);
+  int conv = c->P[c->cur_rep_begin].noise_mode == SMDLT_NOISE_LANCZOS ? read_iters(c, nullptr, nullptr) : 0;
   int g = grow_lists(c);
   collect_timers(c);
   if (g < 0) return g;

@@ -231,6 +231,8 @@ void launch_nl_check(hipStream_t s, const RepDev* reps, int rep_begin, int nrep,
                      int force, int* rep_flags);
 void launch_cell_fill(hipStream_t s, const RepDev* reps, const int* slot_rep, int slot_begin, int nslots,
                       const int* cell_of, const int* cell_start, int* cell_fill, int* cell_beads);
+void launch_cell_sort(hipStream_t s, const int* cell_start, const int* cell_cnt, int cell_begin, int ncells,
+                      const int* unsorted, int* cell_beads);
 void launch_build_lists(hipStream_t s, const RepDev* reps, const int* slot_rep, int slot_begin, int nslots,
                         const double* rx, const double* ry, const double* rz, const int* cell_start,
                         const int* cell_cnt, const int* cell_beads, const int* rep_flags, int nslots_total,

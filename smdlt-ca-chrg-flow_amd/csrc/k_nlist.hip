@@ -97,6 +97,43 @@ void launch_cell_fill(hipStream_t st, const RepDev* reps, const int* slot_rep, i
                                                     cell_fill, cell_beads);
 }
 
+// Writes each cell's members in ascending slot order: a member's place is the number of
+// members with a smaller slot.  k_cell_fill's atomics leave the order in which they landed,
+// and list rows, with the FP64 sums that follow them, inherit the cell order, so this makes
+// every result independent of that order.  One workgroup per cell; members are staged
+// through LDS in chunks, so any cell size works.
+constexpr int CELL_SORT_CHUNK = 2048;
+
+__global__ __launch_bounds__(256) void k_cell_sort(const int* __restrict__ cell_start,
+                                                   const int* __restrict__ cell_cnt, int cell_begin,
+                                                   const int* __restrict__ unsorted, int* __restrict__ cell_beads) {
+  __shared__ int v[CELL_SORT_CHUNK];
+  const int id = cell_begin + blockIdx.x;
+  const int m = cell_cnt[id], base = cell_start[id];
+  const int* src = unsorted + base;
+  if (m == 1 && threadIdx.x == 0) cell_beads[base] = src[0];
+  if (m <= 1) return;
+  for (int k0 = 0; k0 < m; k0 += blockDim.x) {
+    const int k = k0 + threadIdx.x;
+    const int x = k < m ? src[k] : 0;
+    int rank = 0;
+    for (int q0 = 0; q0 < m; q0 += CELL_SORT_CHUNK) {
+      const int qe = min(CELL_SORT_CHUNK, m - q0);
+      __syncthreads();
+      for (int q = threadIdx.x; q < qe; q += blockDim.x) v[q] = src[q0 + q];
+      __syncthreads();
+      for (int q = 0; q < qe; ++q) rank += v[q] < x;
+    }
+    if (k < m) cell_beads[base + rank] = x;
+  }
+}
+
+void launch_cell_sort(hipStream_t st, const int* cell_start, const int* cell_cnt, int cell_begin, int ncells,
+                      const int* unsorted, int* cell_beads) {
+  if (ncells <= 0) return;
+  k_cell_sort<<<ncells, 256, 0, st>>>(cell_start, cell_cnt, cell_begin, unsorted, cell_beads);
+}
+
 __device__ __forceinline__ void sorted_insert(int* list, int stride, int i, int& cnt, int cap, int j, int* overflow) {
   if (cnt >= cap) {
     atomicOr(overflow, 1);

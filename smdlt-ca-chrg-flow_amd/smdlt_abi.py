@@ -15,7 +15,7 @@ HEADER = os.path.join(_HERE, "..", "include", "smdlt_hip.h")
 
 FLOW_EQUILIBRIUM, FLOW_SHEAR, FLOW_EXTENSION = 0, 1, 2
 PREC_MIXED, PREC_FP64 = 0, 1
-NOISE_LANCZOS, NOISE_TEA = 0, 1
+NOISE_LANCZOS, NOISE_TEA, NOISE_FREE_DRAINING = 0, 1, 2
 MC_REFERENCE, MC_MH_UNIFORM, MC_MH_LEARNED = 0, 1, 2
 
 ERRORS = {0: "ok", -1: "SMDLT_E_ARG", -2: "SMDLT_E_HIP", -3: "SMDLT_E_FENE_OVERSTRETCH",
@@ -83,6 +83,7 @@ def lib():
         L.smdlt_run.argtypes = [C.c_void_p, C.c_uint64, C.c_int, C.c_int]
         L.smdlt_last_iters.argtypes = [C.c_void_p, _ip]
         L.smdlt_lanczos_status.argtypes = [C.c_void_p, _ip, _ip, _dp]
+        L.smdlt_set_tea.argtypes = [C.c_void_p, _dp, _dp]
         L.smdlt_set_profiling.argtypes = [C.c_void_p, C.c_int]
         L.smdlt_stage_times.argtypes = [C.c_void_p, _dp, _ip, C.c_int]
         L.smdlt_stage_name.restype = C.c_char_p
@@ -262,6 +263,11 @@ class Context:
         it = np.zeros(self.nrep, np.int32)
         _chk(lib().smdlt_last_iters(self.h, _ptr(it, C.c_int)), "last_iters")
         return it
+
+    def set_tea(self, beta, Cv):
+        beta = np.ascontiguousarray(np.broadcast_to(beta, (self.nrep,)), np.float64)
+        Cv = np.ascontiguousarray(Cv, np.float64).reshape(-1)
+        _chk(lib().smdlt_set_tea(self.h, _ptr(beta, C.c_double), _ptr(Cv, C.c_double)), "set_tea")
 
     def lanczos_status(self):
         it = np.zeros(self.nrep, np.int32)

@@ -271,14 +271,21 @@ def test_tea_matches_oracle():
 
 
 # ----------------------------------------------------------------- timestep
-@pytest.mark.parametrize("flow", [1, 2])
-def test_step_matches_dense_update(flow):
+@pytest.mark.parametrize("flow,mode", [(1, S.NOISE_LANCZOS), (2, S.NOISE_LANCZOS), (1, S.NOISE_FREE_DRAINING),
+                                       (2, S.NOISE_FREE_DRAINING), (1, S.NOISE_TEA), (2, S.NOISE_TEA)])
+def test_step_matches_dense_update(flow, mode):
+    """One step of updateHI (S/main26.c:745-857) with Lanczos or TEA noise, or of updateFD
+    (S/main26.c:731-744), against the dense-matrix update."""
     L, nb, nc, dt, edot = 24.0, 30, 3, 1e-3, 0.5
-    ctx, p = ctx_for(flow, L, nb, nc, prec=S.PREC_FP64, edot=edot, dt=dt, kappas=200.0)
+    ctx, p = ctx_for(flow, L, nb, nc, prec=S.PREC_FP64, edot=edot, dt=dt, kappas=200.0, noise_mode=mode)
     lat = O.lattice(flow, L, edot, dt, 1)
     rx, ry, rz = xyz(random_walk(nb, nc, L, 9))
     ctx.upload_positions(rx, ry, rz)
-    z = np.random.default_rng(8).normal(size=3 * nb * nc)
+    rng = np.random.default_rng(8)
+    z = rng.normal(size=3 * nb * nc)
+    beta, Cv = 0.37, rng.uniform(0.6, 1.0, 3 * nb * nc)
+    if mode == S.NOISE_TEA:
+        ctx.set_tea(beta, Cv)
     t = 10
     ctx.step(t, z)
     gx, gy, gz = ctx.download_positions()
@@ -289,7 +296,10 @@ def test_step_matches_dense_update(flow):
     fl, _ = O.lj_force(lat, wx, wy, wz, nl, lst, 25.0, 0.1, 0)
     F = fb + fl  # lab frame
     pos = np.stack([wx, wy, wz], 1)
-    M = O.ewald_dense(O.cell_of(lat), pos, 0.5)
+    if mode == S.NOISE_FREE_DRAINING:
+        M = np.eye(pos.size)
+    else:
+        M = O.ewald_dense(O.cell_of(lat), pos, 0.5)
     if flow == 2:
         c, s = lat.ct, lat.st
         Rot = np.array([[c, s, 0], [-s, c, 0], [0, 0, 1]])  # lab -> wrap
@@ -297,7 +307,10 @@ def test_step_matches_dense_update(flow):
         zw = (z.reshape(-1, 3) @ Rot.T).reshape(-1)
     else:
         Fw, zw = F, z
-    ys, _, _, _ = O.sym_sqrt_apply(M, zw)
+    if mode == S.NOISE_TEA:  # C and beta act on wrap-frame components
+        ys = O.tea_apply(M, beta, Cv, zw)
+    else:
+        ys, _, _, _ = O.sym_sqrt_apply(M, zw)
     d = (dt * (M @ Fw.reshape(-1)) + np.sqrt(2 * dt) * ys).reshape(-1, 3)
     if flow == 2:
         RotI = Rot.T
@@ -327,7 +340,38 @@ def test_graph_replay_matches_eager():
         ctx.run(0, 6, use_graph=graph)
         out.append(np.stack(ctx.download_positions(), 1))
         ctx.close()
+    print("graph vs eager max diff", np.abs(out[0] - out[1]).max())
     assert np.abs(out[0] - out[1]).max() < 1e-9
+
+
+@pytest.mark.parametrize("mode", [S.NOISE_FREE_DRAINING, S.NOISE_TEA])
+def test_run_in_other_noise_modes_equals_single_steps(mode):
+    """smdlt_run in the free-draining and TEA modes steps eagerly; with device noise seeded by
+    the step index it equals the same steps taken one by one."""
+    L, nb, nc = 30.0, 50, 4
+    out = []
+    for use_run in (False, True):
+        ctx, p = ctx_for(1, L, nb, nc, prec=S.PREC_FP64, edot=0.3, dt=5e-4, noise_mode=mode)
+        if mode == S.NOISE_TEA:
+            ctx.set_tea(0.4, np.full(3 * nb * nc, 0.8))
+        rx, ry, rz = xyz(random_walk(nb, nc, L, 10))
+        ctx.upload_positions(rx, ry, rz)
+        if use_run:
+            assert ctx.run(0, 5, use_graph=True) == 0
+        else:
+            for t in range(5):
+                assert ctx.step(t) == 0
+        out.append(np.stack(ctx.download_positions(), 1))
+        ctx.close()
+    assert np.array_equal(out[0], out[1])  # cells are sorted, so list order and sums are fixed
+
+
+def test_tea_step_without_coefficients_is_a_state_error():
+    ctx, p = ctx_for(1, 30.0, 20, 2, prec=S.PREC_FP64, noise_mode=S.NOISE_TEA)
+    rx, ry, rz = xyz(random_walk(20, 2, 30.0, 3))
+    ctx.upload_positions(rx, ry, rz)
+    with pytest.raises(S.SmdltError):
+        ctx.step(0)
 
 
 def test_batched_replicas_equal_single_contexts():
