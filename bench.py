@@ -560,8 +560,8 @@ def main():
     p64.mobility_tol = 1e-9
     ctx_t = S.Context(p64)
     ctx_t.upload_positions(*ctx.download_positions())
-    ctx.update_lattice(args.warmup + args.steps + nprof)
-    ctx_t.update_lattice(args.warmup + args.steps + nprof)
+    ctx.remap(args.warmup + args.steps + nprof)  # M is taken in the lattice frame
+    ctx_t.remap(args.warmup + args.steps + nprof)
     u_mix = ctx.mobility_apply(v)
     u_ref = ctx_t.mobility_apply(v)
     mf_err = float(np.linalg.norm(u_mix - u_ref) / np.linalg.norm(u_ref))

@@ -156,7 +156,12 @@ int smdlt_get_verlet(smdlt_ctx* ctx, int replica, int* nlist, int* list, int str
 int smdlt_forces(smdlt_ctx* ctx, double* fx, double* fy, double* fz);
 
 /* Mobility product u = M v over all replicas (calcHI + the matvec of updateHI,
- * S/main26.c:1530-1806, 745-857).  v, u: AoS 3N doubles, host memory. */
+ * S/main26.c:1530-1806, 745-857).  v, u: AoS 3N doubles, host memory.
+ * M is taken at the device positions in the frame the current lattice
+ * describes, as calcHI runs after transformCoords/applyPBC (X/main14.c:77-79):
+ * between steps a planar-extension replica's positions are in the lab frame,
+ * so call smdlt_remap(t) first (a shear replica's frames coincide).  The
+ * lists built by the last step or smdlt_neighbors are reused. */
 int smdlt_mobility_apply(smdlt_ctx* ctx, const double* v, double* u);
 /* Same with device pointers (no copies); stream-ordered on the context stream. */
 int smdlt_mobility_apply_device(smdlt_ctx* ctx, const double* d_v, double* d_u);

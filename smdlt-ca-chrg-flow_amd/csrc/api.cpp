@@ -508,15 +508,22 @@ int smdlt_num_beads(const smdlt_ctx* c, int replica) {
   return c->rh[replica].n;
 }
 
-// Mobility settings.  Error model (calibrated against the dense Ewald sum):
-// the real-space tail |t1|+|t2| at rc is held at 0.1 tol, and the B-spline
-// interpolation error of order p reaches tol = 1e-5 at grid spacing
-// h*alpha = 0.10 (p=4), 0.18 (p=6), 0.25 (p=8), scaling as tol^(1/p).
+// Mobility settings.  Error model: the real-space tail |t1|+|t2| at rc is held
+// at 0.1 tol, and the grid part is given a quarter of tol: that is its RMS over
+// beads, and one bead's self term can be 2-3 times the RMS.  The grid error was
+// measured as the relative norm error of the FP64 product at a coarse grid
+// against a fine one (h alpha = 0.12, p = 8; same alpha and rc; random v;
+// chain systems of 1000-8000 beads at 0.005-0.04 beads per unit volume,
+// alpha 0.12-0.7): it reaches 5e-6 at h alpha = 0.32 (p=8), 0.235 (p=6),
+// 0.095 (p=4) for alpha <= 0.4, grows as (h alpha)^(1.6p - 2.5), and at larger
+// alpha (small boxes) the admissible h alpha shrinks about as alpha^-0.45.
 // Among the (rc, p) candidates that fit the box, the cheapest by a cost model
 // of the M.v kernels (pairs, grid passes, spline points) is chosen.
-static double grid_h_alpha(int p, double tol) {
-  double c = (p == 4) ? 0.10 : (p == 6) ? 0.18 : 0.25;
-  return c * pow(tol / 1e-5, 1.0 / p);
+static double grid_h_alpha(int p, double tol, double alpha) {
+  double c = (p == 4) ? 0.095 : (p == 6) ? 0.235 : 0.32;
+  double s = pow(0.25 * tol / 5e-6, 1.0 / (1.6 * p - 2.5));
+  if (alpha > 0.4) s *= pow(0.4 / alpha, 0.45);
+  return c * s;
 }
 
 struct MobChoice {
@@ -531,7 +538,7 @@ static MobChoice mob_candidate(double rc, int order, double tol, double skin, co
   m.rc = rc;
   m.order = order;
   m.alpha = choose_alpha(rc, 0.1 * tol);
-  double h = grid_h_alpha(order, tol) / m.alpha;
+  double h = grid_h_alpha(order, tol, m.alpha) / m.alpha;
   for (int d = 0; d < 3; ++d) m.K[d] = next_fft_len((int)ceil(len[d] / h));
   double rl = rc + skin;
   double chain = std::min((double)nb, 2.0 * pow(rl / 2.0, 1.7));
@@ -589,7 +596,7 @@ static int choose_mobility(smdlt_ctx* c, int r) {
   }
   if (p.ewald_alpha > 0) {
     best.alpha = p.ewald_alpha;
-    double hh = grid_h_alpha(best.order, tol) / best.alpha;
+    double hh = grid_h_alpha(best.order, tol, best.alpha) / best.alpha;
     for (int d = 0; d < 3; ++d) best.K[d] = next_fft_len((int)ceil(len[d] / hh));
   }
   if (p.grid[0] > 0 && p.grid[1] > 0 && p.grid[2] > 0)

@@ -493,6 +493,46 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_z_green(MobArgs a) {
 }
 
 // -------------------------------------------------- real space + interpolation
+// Interpolation of one bead's velocity from the three component grids by one
+// wavefront.  Lane j1 + P*j2 (< P^2) owns stencil column (j1, j2) and walks the
+// P planes j3, so each load instruction reads P runs of P consecutive x points
+// (P rows of one plane) rather than P^2 scattered rows.
+template <typename T, int P>
+__device__ __forceinline__ void interp_bead(const MobArgs& a, const RepDev& R, int s, int lane, CompSum<T> acc[3]) {
+  if (lane >= P * P) return;
+  int base[3];
+  T W[3][P];
+  const T* bw = (const T*)a.bw + (size_t)BW_STRIDE * s;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    base[d] = a.bbase[3 * s + d];
+#pragma unroll
+    for (int j = 0; j < P; ++j) W[d][j] = bw[d * MAX_ORDER + j];
+  }
+  const int j1 = lane % P, j2 = lane / P;
+  const int K0 = R.K[0], K1 = R.K[1], K2 = R.K[2];
+  const long long plane = (long long)K1 * K0, ncomp = plane * K2;
+  int g1 = base[0] - j1, g2 = base[1] - j2;
+  g1 += g1 < 0 ? K0 : 0;
+  g2 += g2 < 0 ? K1 : 0;
+  const T w12 = pick<T, P>(W[0], j1) * pick<T, P>(W[1], j2);
+  const T* col = (const T*)a.grid + R.grid0 + (long long)g2 * K0 + g1;
+  T s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+  for (int j3 = 0; j3 < P; ++j3) {
+    int g3 = base[2] - j3;
+    g3 += g3 < 0 ? K2 : 0;
+    const T* q = col + g3 * plane;
+    const T wt = W[2][j3];
+    s0 += wt * q[0];
+    s1 += wt * q[ncomp];
+    s2 += wt * q[2 * ncomp];
+  }
+  acc[0].add(w12 * s0);
+  acc[1].add(w12 * s1);
+  acc[2].add(w12 * s2);
+}
+
 // One wavefront per bead.  Lanes split the interpolation stencil rows and the
 // real-space list; each lane keeps compensated FP32 (or FP64) sums, which are
 // combined across the wavefront in FP64.
@@ -516,40 +556,7 @@ __global__ __launch_bounds__(256) void k_mob_final(MobArgs a) {
   const double sc = (a.beta && !a.vout) ? (a.beta[r] > 0.0 ? 1.0 / a.beta[r] : 0.0) : 1.0;
   const double xi = a.rx[s], yi = a.ry[s], zi = a.rz[s];
   CompSum<T> acc[3];
-  // reciprocal part: lane l < P^2 interpolates stencil row (l % P, l / P)
-  if (lane < P * P) {
-    int base[3];
-    T W[3][P];
-    const T* bw = (const T*)a.bw + (size_t)BW_STRIDE * s;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      base[d] = a.bbase[3 * s + d];
-#pragma unroll
-      for (int j = 0; j < P; ++j) W[d][j] = bw[d * MAX_ORDER + j];
-    }
-    const int j2 = lane % P, j3 = lane / P;
-    const T* g = (const T*)a.grid + R.grid0;
-    const long long ncomp = (long long)R.K[0] * R.K[1] * R.K[2];
-    const int K0 = R.K[0], K1 = R.K[1], K2 = R.K[2];
-    int g3 = base[2] - j3, g2 = base[1] - j2;
-    g3 += g3 < 0 ? K2 : 0;
-    g2 += g2 < 0 ? K1 : 0;
-    T w23 = pick<T, P>(W[2], j3) * pick<T, P>(W[1], j2);
-    const T* row = g + ((long long)g3 * K1 + g2) * K0;
-    T s0 = 0, s1 = 0, s2 = 0;
-#pragma unroll
-    for (int j1 = 0; j1 < P; ++j1) {
-      int g1 = base[0] - j1;
-      g1 += g1 < 0 ? K0 : 0;
-      T w = W[0][j1];
-      s0 += w * row[g1];
-      s1 += w * row[g1 + ncomp];
-      s2 += w * row[g1 + 2 * ncomp];
-    }
-    acc[0].add(w23 * s0);
-    acc[1].add(w23 * s1);
-    acc[2].add(w23 * s2);
-  }
+  interp_bead<T, P>(a, R, s, lane, acc);  // reciprocal part
   // real-space part over the list
   const T alpha = (T)R.alpha, rc2 = (T)R.rc2;
   const int m = a.nrpy[s];
@@ -685,39 +692,7 @@ __global__ __launch_bounds__(64 * CL_WAVES) void k_mob_cl(MobArgs a) {
   for (int ii = w; ii < cnt; ii += CL_WAVES) {
     const int s = first + ii;
     CompSum<T> ai[3];
-    if (lane < P * P) {
-      int base[3];
-      T W[3][P];
-      const T* bw = (const T*)a.bw + (size_t)BW_STRIDE * s;
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        base[d] = a.bbase[3 * s + d];
-#pragma unroll
-        for (int j = 0; j < P; ++j) W[d][j] = bw[d * MAX_ORDER + j];
-      }
-      const int j2 = lane % P, j3 = lane / P;
-      const T* g = (const T*)a.grid + R.grid0;
-      const long long ncomp = (long long)R.K[0] * R.K[1] * R.K[2];
-      const int K0 = R.K[0], K1 = R.K[1], K2 = R.K[2];
-      int g3 = base[2] - j3, g2 = base[1] - j2;
-      g3 += g3 < 0 ? K2 : 0;
-      g2 += g2 < 0 ? K1 : 0;
-      T w23 = pick<T, P>(W[2], j3) * pick<T, P>(W[1], j2);
-      const T* row = g + ((long long)g3 * K1 + g2) * K0;
-      T s0 = 0, s1 = 0, s2 = 0;
-#pragma unroll
-      for (int j1 = 0; j1 < P; ++j1) {
-        int g1 = base[0] - j1;
-        g1 += g1 < 0 ? K0 : 0;
-        T wt = W[0][j1];
-        s0 += wt * row[g1];
-        s1 += wt * row[g1 + ncomp];
-        s2 += wt * row[g1 + 2 * ncomp];
-      }
-      ai[0].add(w23 * s0);
-      ai[1].add(w23 * s1);
-      ai[2].add(w23 * s2);
-    }
+    interp_bead<T, P>(a, R, s, lane, ai);
     double x0 = warp_sum(ai[0].value()), x1 = warp_sum(ai[1].value()), x2 = warp_sum(ai[2].value());
     if (lane == 0) {
       uin[ii][0] = x0;
