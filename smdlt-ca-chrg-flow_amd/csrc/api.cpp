@@ -265,8 +265,8 @@ struct smdlt_ctx {
   long long grid_total = 0;
   void* bsp = nullptr;
   void* tw = nullptr;
-  FftTask *tx = nullptr, *ty = nullptr, *tz = nullptr;
-  int ntx = 0, nty = 0, ntz = 0;
+  FftTask *tx = nullptr, *ty = nullptr, *tz = nullptr, *txy = nullptr;
+  int ntx = 0, nty = 0, ntz = 0, ntxy = 0;
   BrickTask* bricks = nullptr;
   int nbricks = 0;
   std::vector<int> bk0;  // per-replica brick task ranges (size nrep+1)
@@ -274,7 +274,7 @@ struct smdlt_ctx {
   void* bw = nullptr;
   uint64_t pos_version = 1, prep_version = 0;
   long long prep_range = -1;
-  std::vector<int> tx0, ty0, tz0;  // per-replica task ranges (size nrep+1)
+  std::vector<int> tx0, ty0, tz0, txy0;  // per-replica task ranges (size nrep+1)
   // lanczos
   int maxit = 30;
   double *V = nullptr, *W = nullptr, *partial = nullptr, *partial2 = nullptr, *scal = nullptr, *bdiv = nullptr;
@@ -898,20 +898,32 @@ int smdlt_create(const smdlt_params* params, int n_replicas, int device, smdlt_c
   }
   // FFT task tables, grouped by replica
   {
-    std::vector<FftTask> vx, vy, vz;
+    std::vector<FftTask> vx, vy, vz, vxy;
+    // The fused x-y pass saves two launches and a grid round trip, which pays on
+    // small grids; on large batches the separate passes keep more waves per CU
+    // and were measured ~5% faster, so fusing stops above XY_MAX_POINTS in total.
+    double gtot = 0;
+    for (int r = 0; r < n_replicas; ++r) gtot += (double)c->rh[r].K[0] * c->rh[r].K[1] * c->rh[r].K[2];
     c->tx0.assign(n_replicas + 1, 0);
     c->ty0.assign(n_replicas + 1, 0);
     c->tz0.assign(n_replicas + 1, 0);
+    c->txy0.assign(n_replicas + 1, 0);
     for (int r = 0; r < n_replicas; ++r) {
       const RepDev& R = c->rh[r];
       int K0 = R.K[0], K1 = R.K[1], K2 = R.K[2], Kh = K0 / 2;
       c->tx0[r] = (int)vx.size();
       c->ty0[r] = (int)vy.size();
       c->tz0[r] = (int)vz.size();
-      int nlx = std::max(1, std::min(64, FFT_TILE / K0));
-      int nly = std::max(1, std::min(64, FFT_TILE / K1));
-      int nlz = std::max(1, std::min(64, FFT_TILE / K2));
+      c->txy0[r] = (int)vxy.size();
+      int nlx = std::max(1, std::min(FFT_MAXL, FFT_TILE_X / K0));
+      int nly = std::max(1, std::min(FFT_MAXL, FFT_TILE_Y / K1));
+      int nlz = std::max(1, std::min(FFT_MAXL, FFT_TILE_Z / K2));
+      const bool fused = K0 * K1 / 2 <= XY_ELEMS && gtot <= XY_MAX_POINTS;  // the plane fits one workgroup's LDS
       for (int comp = 0; comp < 3; ++comp) {
+        if (fused) {
+          for (int zz = 0; zz < K2; ++zz) vxy.push_back({r, comp, zz, 1});
+          continue;
+        }
         int pairs = K2 * K1 / 2;
         for (int l0 = 0; l0 < pairs; l0 += nlx) vx.push_back({r, comp, l0, std::min(nlx, pairs - l0)});
         int lines = K2 * Kh;
@@ -923,19 +935,23 @@ int smdlt_create(const smdlt_params* params, int n_replicas, int device, smdlt_c
     c->tx0[n_replicas] = (int)vx.size();
     c->ty0[n_replicas] = (int)vy.size();
     c->tz0[n_replicas] = (int)vz.size();
+    c->txy0[n_replicas] = (int)vxy.size();
     c->ntx = (int)vx.size();
     c->nty = (int)vy.size();
     c->ntz = (int)vz.size();
-    st |= alloc_dev(&c->tx, vx.size());
-    st |= alloc_dev(&c->ty, vy.size());
+    c->ntxy = (int)vxy.size();
+    st |= alloc_dev(&c->tx, std::max<size_t>(1, vx.size()));
+    st |= alloc_dev(&c->ty, std::max<size_t>(1, vy.size()));
     st |= alloc_dev(&c->tz, vz.size());
+    st |= alloc_dev(&c->txy, std::max<size_t>(1, vxy.size()));
     if (st) {
       smdlt_destroy(c);
       return SMDLT_E_NOMEM;
     }
-    HCK(hipMemcpy(c->tx, vx.data(), sizeof(FftTask) * vx.size(), hipMemcpyHostToDevice));
-    HCK(hipMemcpy(c->ty, vy.data(), sizeof(FftTask) * vy.size(), hipMemcpyHostToDevice));
+    if (!vx.empty()) HCK(hipMemcpy(c->tx, vx.data(), sizeof(FftTask) * vx.size(), hipMemcpyHostToDevice));
+    if (!vy.empty()) HCK(hipMemcpy(c->ty, vy.data(), sizeof(FftTask) * vy.size(), hipMemcpyHostToDevice));
     HCK(hipMemcpy(c->tz, vz.data(), sizeof(FftTask) * vz.size(), hipMemcpyHostToDevice));
+    if (!vxy.empty()) HCK(hipMemcpy(c->txy, vxy.data(), sizeof(FftTask) * vxy.size(), hipMemcpyHostToDevice));
   }
   // spreading bricks (8x8x8 grid points), grouped by replica
   {
@@ -1035,7 +1051,7 @@ void smdlt_destroy(smdlt_ctx* c) {
   void* ptrs[] = {c->tea_beta, c->tea_C, c->tea_diag, c->cl_first, c->cl_cnt, c->slot_cl, c->lcl, c->nclp, c->rtab, c->p32, c->v32, c->mh_feats, c->mh_log, c->mlp_ws, c->mlp_x, c->mlp_y, c->mlp_part, c->mlp_out, c->saved, c->d_reps, c->d_slot_rep, c->d_seeds, c->rx, c->ry, c->rz, c->dxv, c->dyv, c->dzv, c->dsp, c->f,
                   c->z, c->mf, c->bz, c->tmp, c->tmp2, c->nrpy, c->lrpy, c->nlj, c->llj, c->overflow, c->d_err,
                   c->cell_of, c->cell_cnt, c->cell_start, c->cell_fill, c->cell_beads, c->cell_tmp, c->rep_flags, c->out2, c->aff,
-                  c->partial2, c->grid, c->bsp, c->tw, c->tx, c->ty, c->tz, c->bricks, c->bbase, c->brick_cnt, c->brick_start,
+                  c->partial2, c->grid, c->bsp, c->tw, c->tx, c->ty, c->tz, c->txy, c->bricks, c->bbase, c->brick_cnt, c->brick_start,
                   c->brick_fill, c->brick_beads, c->bw, c->V, c->W, c->partial, c->scal, c->bdiv, c->done,
                   c->iters, c->qtrack, c->qsign, c->qold, c->qidx, c->mc_err, c->mlp_w, c->occ, c->log_off};
   for (void* p : ptrs)
@@ -1480,6 +1496,8 @@ static MobArgs mob_args(smdlt_ctx* c, int rb, int re, const double* v, double* u
   m.ntx = c->tx0[re] - c->tx0[rb];
   m.nty = c->ty0[re] - c->ty0[rb];
   m.ntz = c->tz0[re] - c->tz0[rb];
+  m.txy = c->txy + c->txy0[rb];
+  m.ntxy = c->txy0[re] - c->txy0[rb];
   m.order = c->rh[rb].order;
   m.bbase = c->bbase;
   m.bw = c->bw;

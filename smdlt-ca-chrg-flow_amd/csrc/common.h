@@ -6,7 +6,24 @@
 
 #define SLOT_ALIGN 64       // beads of a replica are padded to whole wavefronts
 #define KMAX 512            // largest FFT length (lengths are 2^a 3^b 5^c, multiples of 8)
-#define FFT_TILE 1024       // complex elements per component per FFT workgroup
+// complex elements per component per FFT workgroup, by pass; at most FFT_MAXL lines
+#ifndef FFT_TILE_X
+#define FFT_TILE_X 1024
+#endif
+#ifndef FFT_TILE_Y
+#define FFT_TILE_Y 1024
+#endif
+#ifndef FFT_TILE_Z
+#define FFT_TILE_Z 1024
+#endif
+#define FFT_MAXL 64
+// fused x-y pass: one workgroup of XY_THREADS per (replica, component, z plane)
+// when the plane's half spectrum K0*K1/2 fits XY_ELEMS complex values in LDS
+#define XY_THREADS 512
+#ifndef XY_ELEMS
+#define XY_ELEMS 8192
+#endif
+#define XY_MAX_POINTS 8.0e6  // fused only while the context's grids total at most this many points
 #define FFT_THREADS 256
 #define MAX_ORDER 8         // largest B-spline order
 #define LZ_MAXIT 128        // largest Lanczos iteration count supported
@@ -276,8 +293,8 @@ struct MobArgs {
   long long grid_begin, grid_len;
   const void* bsp;          // B-spline moduli (T)
   const void* tw;           // twiddles (complex T, KMAX)
-  const FftTask *tx, *ty, *tz;
-  int ntx, nty, ntz;
+  const FftTask *tx, *ty, *tz, *txy;
+  int ntx, nty, ntz, ntxy;
   int order;                // B-spline order (4, 6 or 8), one per context
   hipEvent_t* ev;           // 9 events bracketing the 8 launches (profiling) or nullptr
   // spreading gather: per-bead stencil base and weights, beads binned by brick

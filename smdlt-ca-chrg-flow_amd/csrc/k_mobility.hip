@@ -114,28 +114,28 @@ __device__ __forceinline__ void fft_stage(C* sh, int K, int pitch, int nl, int N
   __syncthreads();
 }
 
-// FFT of nl lines of length K = 2^a 3^b 5^c (nl*K <= MULT*FFT_TILE): radix-4
+// FFT of nl lines of length K = 2^a 3^b 5^c (nl*K <= TILE): radix-4
 // stages, a radix-2 stage for an odd power of two, then radix 3 and radix 5.
-template <typename C, int MULT>
+template <typename C, int TILE, int TH = FFT_THREADS>
 __device__ __forceinline__ void fft_lines(C* sh, int K, int pitch, int nl, int dir, const C* __restrict__ tw) {
   int Ns = 1, rest = K;
   while (rest % 4 == 0) {
-    fft_stage<C, 4, MULT * FFT_TILE / (4 * FFT_THREADS)>(sh, K, pitch, nl, Ns, dir, tw);
+    fft_stage<C, 4, (TILE + 4 * TH - 1) / (4 * TH)>(sh, K, pitch, nl, Ns, dir, tw);
     Ns *= 4;
     rest /= 4;
   }
   if (rest % 2 == 0) {
-    fft_stage<C, 2, MULT * FFT_TILE / (2 * FFT_THREADS)>(sh, K, pitch, nl, Ns, dir, tw);
+    fft_stage<C, 2, (TILE + 2 * TH - 1) / (2 * TH)>(sh, K, pitch, nl, Ns, dir, tw);
     Ns *= 2;
     rest /= 2;
   }
   while (rest % 3 == 0) {
-    fft_stage<C, 3, (MULT * FFT_TILE + 3 * FFT_THREADS - 1) / (3 * FFT_THREADS)>(sh, K, pitch, nl, Ns, dir, tw);
+    fft_stage<C, 3, (TILE + 3 * TH - 1) / (3 * TH)>(sh, K, pitch, nl, Ns, dir, tw);
     Ns *= 3;
     rest /= 3;
   }
   while (rest % 5 == 0) {
-    fft_stage<C, 5, (MULT * FFT_TILE + 5 * FFT_THREADS - 1) / (5 * FFT_THREADS)>(sh, K, pitch, nl, Ns, dir, tw);
+    fft_stage<C, 5, (TILE + 5 * TH - 1) / (5 * TH)>(sh, K, pitch, nl, Ns, dir, tw);
     Ns *= 5;
     rest /= 5;
   }
@@ -322,13 +322,20 @@ __global__ __launch_bounds__(512) void k_spread_gather(MobArgs a) {
 }
 
 // ------------------------------------------------------------------- FFT passes
+// Copies a line length's twiddle table into LDS, so the Stockham stages read
+// their twiddles there; the barrier that follows the grid load orders it.
+template <typename C> __device__ __forceinline__ void stage_twiddles(C* dst, const C* __restrict__ src, int K) {
+  for (int i = threadIdx.x; i < K; i += blockDim.x) dst[i] = src[i];
+}
+
 // x pass: task = (rep, comp, first row pair, pairs); forward packs rows 2p, 2p+1
 // as one complex line and splits the spectrum into two half spectra (Nyquist
 // dropped); inverse rebuilds the full line and writes real/imag to the rows.
 template <typename T>
 __global__ __launch_bounds__(FFT_THREADS) void k_fft_x(MobArgs a, int dir) {
   typedef typename Prec<T>::C C;
-  __shared__ C sh[FFT_TILE + 64];
+  __shared__ C sh[FFT_TILE_X + FFT_MAXL];
+  __shared__ C tw[KMAX];
   FftTask tk = a.tx[blockIdx.x];
   const RepDev& R = a.reps[tk.rep];
   if (!R.active || (a.done && a.done[tk.rep])) return;
@@ -336,7 +343,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_x(MobArgs a, int dir) {
   const long long ncomp = (long long)R.K[0] * R.K[1] * R.K[2];
   T* g = (T*)a.grid + R.grid0 + tk.comp * ncomp;
   C* gc = (C*)g;
-  const C* tw = (const C*)a.tw + R.twoff[0];
+  stage_twiddles(tw, (const C*)a.tw + R.twoff[0], K);
   if (dir < 0) {
     for (int e = threadIdx.x; e < nl * K; e += blockDim.x) {
       int l = e / K, x = e - l * K;
@@ -347,7 +354,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_x(MobArgs a, int dir) {
       sh[l * pitch + x] = z;
     }
     __syncthreads();
-    fft_lines<C, 1>(sh, K, pitch, nl, -1, tw);
+    fft_lines<C, FFT_TILE_X>(sh, K, pitch, nl, -1, tw);
     for (int e = threadIdx.x; e < nl * Kh; e += blockDim.x) {
       int l = e / Kh, k = e - l * Kh;
       C zk = sh[l * pitch + k], zm = sh[l * pitch + (k == 0 ? 0 : K - k)];
@@ -382,7 +389,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_x(MobArgs a, int dir) {
       }
     }
     __syncthreads();
-    fft_lines<C, 1>(sh, K, pitch, nl, 1, tw);
+    fft_lines<C, FFT_TILE_X>(sh, K, pitch, nl, 1, tw);
     for (int e = threadIdx.x; e < nl * K; e += blockDim.x) {
       int l = e / K, x = e - l * K;
       long long row = 2LL * (tk.l0 + l);
@@ -393,24 +400,141 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_x(MobArgs a, int dir) {
   }
 }
 
+// Fused x-y pass over one z plane of one component (planes with K0*K1/2 <=
+// XY_ELEMS).  Forward: the K1 real rows are packed in pairs as K1/2 complex
+// lines and transformed along x; each pair is split into the two rows' half
+// spectra (Nyquist dropped), which are written transposed, one line per x
+// wavenumber, and transformed along y; the result goes out in the layout the
+// y pass writes, (z*K1 + y)*Kh + x.  Inverse: the same steps backwards.  The
+// two LDS layouts share one buffer: the split and the rebuild go through
+// registers, all reads before a barrier, all writes after it.
+template <typename T>
+__global__ __launch_bounds__(XY_THREADS) void k_fft_xy(MobArgs a, int dir) {
+  typedef typename Prec<T>::C C;
+  constexpr int MAXP = (XY_ELEMS / 2 + XY_THREADS - 1) / XY_THREADS;  // row pairs x wavenumbers per thread
+  __shared__ C sh[XY_ELEMS + KMAX / 2];
+  __shared__ C twx[KMAX], twy[KMAX];
+  FftTask tk = a.txy[blockIdx.x];
+  const RepDev& R = a.reps[tk.rep];
+  if (!R.active || (a.done && a.done[tk.rep])) return;
+  const int K0 = R.K[0], K1 = R.K[1], Kh = K0 / 2, P2 = K1 / 2;
+  const int pa = K0 + 1, pb = K1 + 1;  // x-line and y-line pitches
+  const long long ncomp = (long long)K0 * K1 * R.K[2];
+  const int z = tk.l0;
+  T* g = (T*)a.grid + R.grid0 + tk.comp * ncomp + (long long)z * K1 * K0;  // this plane's real rows
+  C* gc = (C*)((T*)a.grid + R.grid0 + tk.comp * ncomp) + (long long)z * K1 * Kh;  // its half spectrum
+  stage_twiddles(twx, (const C*)a.tw + R.twoff[0], K0);
+  stage_twiddles(twy, (const C*)a.tw + R.twoff[1], K1);
+  const int t = threadIdx.x, TH = blockDim.x, np = P2 * Kh;
+  C ra[MAXP], rb[MAXP];
+  if (dir < 0) {
+    for (int e = t; e < P2 * K0; e += TH) {
+      int p = e / K0, x = e - p * K0;
+      C v;
+      v.x = g[(2 * p) * K0 + x];
+      v.y = g[(2 * p + 1) * K0 + x];
+      sh[p * pa + x] = v;
+    }
+    __syncthreads();
+    fft_lines<C, XY_ELEMS, XY_THREADS>(sh, K0, pa, P2, -1, twx);
+#pragma unroll
+    for (int q = 0; q < MAXP; ++q) {
+      int e = t + q * TH;
+      if (e < np) {
+        int p = e / Kh, k = e - p * Kh;
+        C zk = sh[p * pa + k], zm = sh[p * pa + (k == 0 ? 0 : K0 - k)];
+        ra[q].x = (T)0.5 * (zk.x + zm.x);
+        ra[q].y = (T)0.5 * (zk.y - zm.y);
+        rb[q].x = (T)0.5 * (zk.y + zm.y);
+        rb[q].y = (T)0.5 * (zm.x - zk.x);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < MAXP; ++q) {
+      int e = t + q * TH;
+      if (e < np) {
+        int p = e / Kh, k = e - p * Kh;
+        sh[k * pb + 2 * p] = ra[q];
+        sh[k * pb + 2 * p + 1] = rb[q];
+      }
+    }
+    __syncthreads();
+    fft_lines<C, XY_ELEMS, XY_THREADS>(sh, K1, pb, Kh, -1, twy);
+    for (int e = t; e < K1 * Kh; e += TH) {
+      int y = e / Kh, k = e - y * Kh;
+      gc[(long long)y * Kh + k] = sh[k * pb + y];
+    }
+  } else {
+    for (int e = t; e < K1 * Kh; e += TH) {
+      int y = e / Kh, k = e - y * Kh;
+      sh[k * pb + y] = gc[(long long)y * Kh + k];
+    }
+    __syncthreads();
+    fft_lines<C, XY_ELEMS, XY_THREADS>(sh, K1, pb, Kh, 1, twy);
+#pragma unroll
+    for (int q = 0; q < MAXP; ++q) {
+      int e = t + q * TH;
+      if (e < np) {
+        int p = e / Kh, k = e - p * Kh;
+        ra[q] = sh[k * pb + 2 * p];
+        rb[q] = sh[k * pb + 2 * p + 1];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < MAXP; ++q) {
+      int e = t + q * TH;
+      if (e < np) {
+        int p = e / Kh, k = e - p * Kh;
+        C A = ra[q], B = rb[q], v;
+        v.x = A.x - B.y;
+        v.y = A.y + B.x;
+        sh[p * pa + k] = v;
+        if (k > 0) {
+          C w;
+          w.x = A.x + B.y;
+          w.y = B.x - A.y;
+          sh[p * pa + K0 - k] = w;
+        } else {
+          C zero;
+          zero.x = (T)0;
+          zero.y = (T)0;
+          sh[p * pa + Kh] = zero;
+        }
+      }
+    }
+    __syncthreads();
+    fft_lines<C, XY_ELEMS, XY_THREADS>(sh, K0, pa, P2, 1, twx);
+    for (int e = t; e < P2 * K0; e += TH) {
+      int p = e / K0, x = e - p * K0;
+      C v = sh[p * pa + x];
+      g[(2 * p) * K0 + x] = v.x;
+      g[(2 * p + 1) * K0 + x] = v.y;
+    }
+  }
+}
+
 // y pass (C2C along a2): line L = z*Kh + x, element k at (z*K1 + k)*Kh + x.
 template <typename T>
 __global__ __launch_bounds__(FFT_THREADS) void k_fft_y(MobArgs a, int dir) {
   typedef typename Prec<T>::C C;
-  __shared__ C sh[FFT_TILE + 64];
+  __shared__ C sh[FFT_TILE_Y + FFT_MAXL];
+  __shared__ C tw[KMAX];
   FftTask tk = a.ty[blockIdx.x];
   const RepDev& R = a.reps[tk.rep];
   if (!R.active || (a.done && a.done[tk.rep])) return;
   const int Kh = R.K[0] / 2, K = R.K[1], pitch = K + 1, nl = tk.nl;
   const long long ncomp = (long long)R.K[0] * R.K[1] * R.K[2];
   C* gc = (C*)((T*)a.grid + R.grid0 + tk.comp * ncomp);
+  stage_twiddles(tw, (const C*)a.tw + R.twoff[1], K);
   for (int e = threadIdx.x; e < nl * K; e += blockDim.x) {
     int k = e / nl, l = e - k * nl;
     int L = tk.l0 + l, z = L / Kh, x = L - z * Kh;
     sh[l * pitch + k] = gc[((long long)z * K + k) * Kh + x];
   }
   __syncthreads();
-  fft_lines<C, 1>(sh, K, pitch, nl, dir, (const C*)a.tw + R.twoff[1]);
+  fft_lines<C, FFT_TILE_Y>(sh, K, pitch, nl, dir, tw);
   for (int e = threadIdx.x; e < nl * K; e += blockDim.x) {
     int k = e / nl, l = e - k * nl;
     int L = tk.l0 + l, z = L / Kh, x = L - z * Kh;
@@ -423,10 +547,12 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_y(MobArgs a, int dir) {
 template <typename T>
 __global__ __launch_bounds__(FFT_THREADS) void k_fft_z_green(MobArgs a) {
   typedef typename Prec<T>::C C;
-  __shared__ C sh[3 * (FFT_TILE + 64)];
+  __shared__ C sh[3 * (FFT_TILE_Z + FFT_MAXL)];
+  __shared__ C tw[KMAX];
   FftTask tk = a.tz[blockIdx.x];
   const RepDev& R = a.reps[tk.rep];
   if (!R.active || (a.done && a.done[tk.rep])) return;
+  stage_twiddles(tw, (const C*)a.tw + R.twoff[2], R.K[2]);
   const int K0 = R.K[0], Kh = K0 / 2, K1 = R.K[1], K = R.K[2], pitch = K + 1, nl = tk.nl;
   const long long ncomp = (long long)R.K[0] * R.K[1] * R.K[2];
   const long long stride = (long long)K1 * Kh;
@@ -439,8 +565,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_z_green(MobArgs a) {
     for (int c = 0; c < 3; ++c) sh[c * cpitch + l * pitch + k] = gc[c][idx];
   }
   __syncthreads();
-  const C* tw = (const C*)a.tw + R.twoff[2];
-  fft_lines<C, 3>(sh, K, pitch, 3 * nl, -1, tw);
+  fft_lines<C, 3 * FFT_TILE_Z>(sh, K, pitch, 3 * nl, -1, tw);
   const T* bsp = (const T*)a.bsp + R.bsp0;
   const T ia2 = (T)(1.0 / (4.0 * R.alpha * R.alpha));
   const T invV = (T)(1.0 / R.vol);
@@ -484,7 +609,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_z_green(MobArgs a) {
     *p2 = o2;
   }
   __syncthreads();
-  fft_lines<C, 3>(sh, K, pitch, 3 * nl, 1, tw);
+  fft_lines<C, 3 * FFT_TILE_Z>(sh, K, pitch, 3 * nl, 1, tw);
   for (int e = threadIdx.x; e < nl * K; e += blockDim.x) {
     int k = e / nl, l = e - k * nl;
     long long idx = k * stride + tk.l0 + l;
@@ -819,6 +944,7 @@ template <typename T, int P> static void launch_mob_p(hipStream_t st, const MobA
   if (!(a.phase & 2)) return;
   if (a.nbricks > 0) k_spread_gather<T, P><<<a.nbricks, 512, 0, st>>>(a);
   mark(2);
+  if (a.ntxy > 0) k_fft_xy<T><<<a.ntxy, XY_THREADS, 0, st>>>(a, -1);
   if (a.ntx > 0) k_fft_x<T><<<a.ntx, FFT_THREADS, 0, st>>>(a, -1);
   mark(3);
   if (a.nty > 0) k_fft_y<T><<<a.nty, FFT_THREADS, 0, st>>>(a, -1);
@@ -827,6 +953,7 @@ template <typename T, int P> static void launch_mob_p(hipStream_t st, const MobA
   mark(5);
   if (a.nty > 0) k_fft_y<T><<<a.nty, FFT_THREADS, 0, st>>>(a, 1);
   mark(6);
+  if (a.ntxy > 0) k_fft_xy<T><<<a.ntxy, XY_THREADS, 0, st>>>(a, 1);
   if (a.ntx > 0) k_fft_x<T><<<a.ntx, FFT_THREADS, 0, st>>>(a, 1);
   mark(7);
   if (sizeof(T) == 4 && a.lcl) {

@@ -176,9 +176,11 @@ def test_mobility_fp64_equals_pme_restatement(flow):
     assert rel(u, un) < 1e-12
 
 
-@pytest.mark.parametrize("grid", [(40, 48, 72), (24, 80, 120), (96, 32, 40)])
+@pytest.mark.parametrize("grid", [(40, 48, 72), (24, 80, 120), (96, 32, 40), (160, 144, 16), (16, 40, 160)])
 def test_mobility_mixed_radix_grids_equal_pme_restatement(grid):
-    # line lengths 2^a 3^b 5^c exercise the radix-3 and radix-5 Stockham stages
+    # line lengths 2^a 3^b 5^c exercise the radix-3 and radix-5 Stockham stages; planes
+    # with K0*K1/2 <= 8192 take the fused x-y pass, (160, 144, 16) the separate x and y passes
+    # (as do the four 128^3 grids of the large-grid batch test: over 8e6 points in total)
     ctx, lat, pos = mob_case(1, prec=S.PREC_FP64, grid=grid, spline_order=6)
     cfg = ctx.mobility_config()
     assert cfg["grid"] == grid
