@@ -6,7 +6,7 @@
 // are two-level and ordered (per-wavefront partials, then one wavefront per
 // replica), so results do not depend on scheduling.  The tridiagonal square
 // root is evaluated by a quadrature of the inverse-square-root integral on one
-// wavefront per replica (k_lz_tri).
+// workgroup per replica (k_lz_tri).
 #include "common.h"
 
 #define SCAL_STRIDE (4 * LZ_MAXIT + 8)
@@ -88,66 +88,79 @@ __global__ void k_lz_orth(const RepDev* __restrict__ reps, const int* __restrict
 }
 
 // b_j from the partials, then c_j = b0 T_j^(1/2) e1 and the stopping test
-// |c_j - [c_{j-1}; 0]| < tol |c_j|.  One wavefront per replica.
+// |c_j - [c_{j-1}; 0]| < tol |c_j|.  One workgroup of LZ_TRI_TH threads per replica.
 //
 // T^(1/2) e1 = T T^(-1/2) e1 with T^(-1/2) = (2/pi) int_R e^s (T + e^(2s) I)^(-1) ds,
 // the substitution t = e^s of the classical integral (2/pi) int_0^inf (T + t^2)^(-1) dt.
 // For one eigenvalue the integrand is lambda^(-1/2) sech(s - ln(lambda)/2)/pi, so
-// the trapezoid rule with step h has relative error ~exp(-pi^2/h) for every
+// the trapezoid rule with step h has relative error ~4 exp(-pi^2/h) for every
 // eigenvalue alike (poles of sech at +-i pi/2), and truncating at U beyond the
-// spectrum costs ~(2/pi) e^-U.  h = 0.4, U = 24 give ~1e-10.  The spectrum is
-// bracketed by Gershgorin above and by a 64-point Sturm multisection below.
-// Each node is one SPD tridiagonal solve (T + sigma I) x = e1 (backward pivots,
-// then the first column of the inverse forwards); lanes take nodes round-robin
-// and the weighted columns are summed in lane order, so the result does not
-// depend on scheduling.  About 130 nodes: two or three solves per lane.
-#define LZ_QH 0.4
-#define LZ_QU 24.0
-__global__ __launch_bounds__(64) void k_lz_tri(const RepDev* __restrict__ reps, int rep_begin, const double* partial,
-                                               double* scal, int j, double tol, int maxit, int* done, double* bdiv,
-                                               int* iters) {
+// spectrum costs ~(2/pi) e^-U.  h = 0.5, U = 20 give ~1e-8, two orders below the
+// stopping tolerance.  The spectrum is bracketed by Gershgorin above and by a
+// 64-point Sturm multisection below.  Each node is one SPD tridiagonal solve
+// (T + sigma I) x = e1 (backward pivots, then the first column of the inverse
+// forwards) by one thread, so the ~90 nodes run side by side; the reciprocals are
+// v_rcp_f64 refined by two Newton steps.  The weighted columns are summed in
+// thread order, so the result does not depend on scheduling.
+#define LZ_QH 0.5
+#define LZ_QU 20.0
+#define LZ_TRI_TH 128
+
+__device__ __forceinline__ double rcp_refined(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  y = fma(fma(-x, y, 1.0), y, y);
+  return fma(fma(-x, y, 1.0), y, y);
+}
+
+__global__ __launch_bounds__(LZ_TRI_TH) void k_lz_tri(const RepDev* __restrict__ reps, int rep_begin,
+                                                      const double* partial, double* scal, int j, double tol,
+                                                      int maxit, int* done, double* bdiv, int* iters) {
   int r = rep_begin + blockIdx.x;
   const RepDev& R = reps[r];
   if (!R.active || done[r]) return;
-  const int lane = threadIdx.x;
+  const int t = threadIdx.x, lane = t & 63;
+  const bool w0 = t < 64;
   double* sc = scal + (size_t)r * SCAL_STRIDE;
-  double v = 0.0;
-  for (int w = lane; w < R.nwaves; w += 64) v += partial[R.wave0 + w];
-  v = warp_sum(v);
-  const double bj = sqrt(v);
-  const bool breakdown = !(bj > 1e-300);
-  if (lane == 0) {
-    sc[S_BE + j - 1] = bj;
-    bdiv[r] = bj;
-    iters[r] = j;
-    sc[S_IT] = (double)j;
+  __shared__ double d[LZ_MAXIT], e[LZ_MAXIT], e2[LZ_MAXIT], u[LZ_MAXIT];
+  __shared__ double col[LZ_MAXIT * LZ_TRI_TH];  // backward pivots, then weighted first columns: [i][thread]
+  __shared__ double bnd[2];
+  __shared__ int brk;
+  if (w0) {
+    double v = 0.0;
+    for (int w = lane; w < R.nwaves; w += 64) v += partial[R.wave0 + w];
+    v = warp_sum(v);
+    const double bj = sqrt(v);
+    if (lane == 0) {
+      sc[S_BE + j - 1] = bj;
+      bdiv[r] = bj;
+      iters[r] = j;
+      sc[S_IT] = (double)j;
+      brk = !(bj > 1e-300);
+    }
   }
   if (!(sc[S_B0] > 0.0)) {  // z = 0: y = 0 exactly
-    if (lane == 0) {
+    if (t == 0) {
       sc[S_C] = 0.0;
       sc[S_ERR] = 0.0;
       done[r] = 1;
     }
     return;
   }
-  __shared__ double d[LZ_MAXIT], e[LZ_MAXIT], e2[LZ_MAXIT], u[LZ_MAXIT];
-  __shared__ double ipiv[LZ_MAXIT * 64];  // 1 / backward pivot, [i][lane]
-  __shared__ double acc[LZ_MAXIT * 64];   // weighted first columns, [i][lane]
   const int n = j;
-  for (int i = lane; i < n; i += 64) {
+  for (int i = t; i < n; i += LZ_TRI_TH) {
     d[i] = sc[S_AL + i];
     e[i] = (i < n - 1) ? sc[S_BE + i] : 0.0;
     e2[i] = e[i] * e[i];
   }
   __syncthreads();
-  // Gershgorin upper bound of the spectrum
-  double hi = 0.0;
-  for (int i = lane; i < n; i += 64) hi = fmax(hi, d[i] + fabs(e[i]) + (i > 0 ? fabs(e[i - 1]) : 0.0));
-  hi = warp_max(hi);
-  // lower bound: largest of hi * 10^(-12 (l+1)/64) with no eigenvalue below it
-  const double x = hi * exp(-27.631021115928547 * (lane + 1) / 64.0);
-  int neg = 0;
-  {
+  if (w0) {
+    // Gershgorin upper bound of the spectrum
+    double hi = 0.0;
+    for (int i = lane; i < n; i += 64) hi = fmax(hi, d[i] + fabs(e[i]) + (i > 0 ? fabs(e[i - 1]) : 0.0));
+    hi = warp_max(hi);
+    // lower bound: largest of hi * 10^(-12 (l+1)/64) with no eigenvalue below it
+    const double x = hi * exp(-27.631021115928547 * (lane + 1) / 64.0);
+    int neg = 0;
     double p = d[0] - x;
     neg += p < 0.0;
     for (int i = 1; i < n; ++i) {
@@ -155,38 +168,49 @@ __global__ __launch_bounds__(64) void k_lz_tri(const RepDev* __restrict__ reps, 
       p = d[i] - x - e2[i - 1] / p;
       neg += p < 0.0;
     }
+    unsigned long long zero = __ballot(neg == 0);
+    if (lane == 0) {
+      bnd[0] = zero ? hi * exp(-27.631021115928547 * (__ffsll((long long)zero)) / 64.0) : hi * 1e-12;
+      bnd[1] = hi;
+    }
   }
-  unsigned long long zero = __ballot(neg == 0);
-  double lo = zero ? hi * exp(-27.631021115928547 * (__ffsll((long long)zero)) / 64.0) : hi * 1e-12;
-  const double slo = 0.5 * log(lo) - LZ_QU;
-  const int nodes = (int)ceil((0.5 * log(hi) + LZ_QU - slo) / LZ_QH) + 1;
-  for (int i = 0; i < n; ++i) acc[i * 64 + lane] = 0.0;
-  for (int k = lane; k < nodes; k += 64) {
-    const double s = slo + k * LZ_QH;
+  __syncthreads();
+  const double slo = 0.5 * log(bnd[0]) - LZ_QU, shi = 0.5 * log(bnd[1]) + LZ_QU;
+  double h = LZ_QH;
+  int nodes = (int)ceil((shi - slo) / h) + 1;
+  if (nodes > LZ_TRI_TH) {  // a spectrum wider than e^48: keep one node per thread
+    nodes = LZ_TRI_TH;
+    h = (shi - slo) / (LZ_TRI_TH - 1);
+  }
+  if (t < nodes) {
+    const double s = slo + t * h;
     const double sig = exp(2.0 * s);
-    const double wk = LZ_QH * 0.6366197723675814 * exp(s);
+    const double wk = h * 0.6366197723675814 * exp(s);
     double rp = d[n - 1] + sig;
-    double ip = 1.0 / rp;
-    ipiv[(n - 1) * 64 + lane] = ip;
+    double ip = rcp_refined(rp);
+    col[(n - 1) * LZ_TRI_TH + t] = ip;
     for (int i = n - 2; i >= 0; --i) {
       rp = d[i] + sig - e2[i] * ip;
-      ip = 1.0 / rp;
-      ipiv[i * 64 + lane] = ip;
+      ip = rcp_refined(rp);
+      col[i * LZ_TRI_TH + t] = ip;
     }
     double xi = ip;  // (T + sig)^-1 e1, first entry
-    acc[lane] += wk * xi;
+    col[t] = wk * xi;
     for (int i = 0; i < n - 1; ++i) {
-      xi = -e[i] * xi * ipiv[(i + 1) * 64 + lane];
-      acc[(i + 1) * 64 + lane] += wk * xi;
+      xi = -e[i] * xi * col[(i + 1) * LZ_TRI_TH + t];
+      col[(i + 1) * LZ_TRI_TH + t] = wk * xi;
     }
+  } else {
+    for (int i = 0; i < n; ++i) col[i * LZ_TRI_TH + t] = 0.0;
   }
   __syncthreads();
-  for (int i = lane; i < n; i += 64) {
-    double t = 0.0;
-    for (int l = 0; l < 64; ++l) t += acc[i * 64 + l];
-    u[i] = t;
+  for (int i = t; i < n; i += LZ_TRI_TH) {
+    double a = 0.0;
+    for (int q = 0; q < LZ_TRI_TH; ++q) a += col[i * LZ_TRI_TH + q];
+    u[i] = a;
   }
   __syncthreads();
+  if (!w0) return;
   const double b0 = sc[S_B0];
   const int nprev = (int)sc[S_ERR + 1];
   double num = 0.0, den = 0.0;
@@ -202,14 +226,13 @@ __global__ __launch_bounds__(64) void k_lz_tri(const RepDev* __restrict__ reps, 
   }
   num = warp_sum(num);
   den = warp_sum(den);
-  __syncthreads();
   for (int i = lane; i < n; i += 64) sc[S_C + i] = sc[S_CP + i];
   double err = (nprev > 0) ? sqrt(num / den) : 1.0;
   if (lane == 0) {
     sc[S_ERR] = err;
     sc[S_ERR + 1] = (double)n;
     bool conv = nprev > 0 && err < tol;
-    if (conv || j >= maxit || breakdown) done[r] = conv ? 1 : 2;
+    if (conv || j >= maxit || brk) done[r] = conv ? 1 : 2;
   }
 }
 
@@ -273,7 +296,7 @@ void lz_orth(hipStream_t st, const RepDev* reps, const int* slot_rep, int slot_b
 }
 void lz_tri(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, const double* partial, double* scal, int j,
             double tol, int maxit, int* done, double* bdiv, int* iters) {
-  if (nrep > 0) k_lz_tri<<<nrep, 64, 0, st>>>(reps, rep_begin, partial, scal, j, tol, maxit, done, bdiv, iters);
+  if (nrep > 0) k_lz_tri<<<nrep, LZ_TRI_TH, 0, st>>>(reps, rep_begin, partial, scal, j, tol, maxit, done, bdiv, iters);
 }
 void lz_combine(hipStream_t st, const RepDev* reps, const int* slot_rep, int slot_begin, int nslots, const double* V,
                 int nst, const double* scal, const int* iters, double* y) {
