@@ -174,6 +174,16 @@ int smdlt_brownian(smdlt_ctx* ctx, const double* z, double* y, double tol, int m
  * the mobility of this build (S/main26.c:787-804); beta per replica, C 3N. */
 int smdlt_tea_apply(smdlt_ctx* ctx, const double* beta, const double* C, const double* z, double* y);
 
+/* ewaldBin + CATEA (S/main26.c:975-1381, X/main14.c:1009-1420): one sample of
+ * the conformation-averaged TEA parameters for each active replica, from the
+ * device positions and the current lattice (the last smdlt_step, smdlt_remap
+ * or smdlt_update_lattice), with the reference's Ewald sums (M_HI = 3.5, rce =
+ * L2yp or 2 L2yp over the period, its self-term image sum and r <= 2 branch).
+ * beta[nrep] (the TEA beta), C: AoS 3N (Ct), eps (optional) [nrep]: the pair
+ * average eps / (9N^2 - 3N).  Entries of inactive replicas are left as they
+ * are.  The strain-bin running averages of CATEA are the caller's; beta and C
+ * feed smdlt_set_tea.  O(N^2) pair work per replica. */
+int smdlt_tea_sample(smdlt_ctx* ctx, double* beta, double* C, double* eps);
 /* TEA parameters for SMDLT_NOISE_TEA steps (the output of ewaldBin/CATEA,
  * S/main26.c:975-1381): beta per replica, C per bead component (3N AoS). */
 int smdlt_set_tea(smdlt_ctx* ctx, const double* beta, const double* C);

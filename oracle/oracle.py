@@ -108,6 +108,10 @@ def lib():
         L.orc_coulomb_energy.argtypes = [C.POINTER(Lattice), C.c_int, _dp, _dp, _dp, _ip, C.c_double, C.c_double]
         L.orc_coulomb_energy.restype = C.c_double
         L.orc_rpy_real_coeffs.argtypes = [C.c_double, C.c_double, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.orc_ewald_bin.argtypes = [C.POINTER(Lattice), C.c_int, C.c_int, _dp, _dp, _dp, C.c_double, C.c_double,
+                                    C.POINTER(C.c_double), _dp, _dp, C.POINTER(C.c_int)]
+        L.orc_catea.argtypes = [C.c_int, C.c_double, _dp, _dp]
+        L.orc_catea.restype = C.c_double
         L.orc_rpy_ewald_dense.argtypes = [C.POINTER(Cell), C.c_int, _dp, C.c_double, C.c_double, C.c_double, C.c_int, _dp]
         L.orc_rpy_free_dense.argtypes = [C.c_int, _dp, _dp]
         L.orc_ewald_selfconst.argtypes = [C.c_double]
@@ -292,6 +296,26 @@ def calchi_ref(lat, rx, ry, rz, t, nt_avg, b, table, box_volume):
     lib().orc_calchi_ref(C.byref(lat), n, rx, ry, rz, int(t), nt_avg, C.byref(b), tp, box_volume, M,
                          C.byref(far), C.byref(kc))
     return M.reshape(3 * n, 3 * n), far.value, kc.value
+
+
+def ewald_bin(lat, rx, ry, rz, nb, box_volume, m_hi=3.5):
+    """ewaldBin (S/main26.c:975-1260): pair sum eps, rowsum[3N], self block Dtr[3], k count.
+    m_hi = 3.5 is the reference's split."""
+    rx, ry, rz = (np.ascontiguousarray(a, np.float64) for a in (rx, ry, rz))
+    n = len(rx)
+    eps = C.c_double()
+    rowsum = np.zeros(3 * n)
+    dtr = np.zeros(3)
+    kc = C.c_int()
+    lib().orc_ewald_bin(C.byref(lat), n, nb, rx, ry, rz, box_volume, m_hi, C.byref(eps), rowsum, dtr, C.byref(kc))
+    return eps.value, rowsum, dtr, kc.value
+
+
+def catea(n, eps, rowsum):
+    """CATEA (S/main26.c:1317-1381) for one sample: beta and C[3N]."""
+    Cv = np.zeros(3 * n)
+    beta = lib().orc_catea(n, eps, np.ascontiguousarray(rowsum, np.float64), Cv)
+    return beta, Cv
 
 
 def tea_apply(M, beta, Cv, R):

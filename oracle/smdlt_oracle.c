@@ -649,51 +649,32 @@ void orc_rpy_ewald_dense(const orc_cell_t* c, int n, const double* pos, double a
   free(kv);
 }
 
-/* calcHI restated with its quirks (S/main26.c:1530-1806; X/main14.c:1548-1824). */
-void orc_calchi_ref(const orc_lattice_t* lat, int N, const double* rx, const double* ry, const double* rz,
-                    uint64_t t, int nt_avg, const orc_bins_t* bins, const double* table, double box_volume,
-                    double* M, int* far_count, int* kcount_out) {
-  (void)t;
-  int kx, ky, kz, start, kxmax, kymax, kzmax, nmax, kcount, ylower, zlower, yupper, zupper, treal, nt;
-  double alpha, rkx, rky, rkz, rkk, m2, kcoeff, rkxy, rtpi, selfconst, alpha2, alpha3, alpha4, alpha5,
-      alpha7, rce, rkcut, rkkcut, M_HI, tf;
-  double Dtr[3];
+/* The reciprocal-space sum of calcHI and ewaldBin, restated with its range logic
+ * (S/main26.c:1593-1652 and 1000-1097, X/main14.c:1034-1131): the wave vectors
+ * (kx >= 0 half space in the reference's index names, shear rkx from ky and rky
+ * from kx, KRBC from the deformed lattice with kcoeff = 2 pi/detL), the first/last
+ * in-cut bounds along ky and kz, the while-loop that fixes kxmax, M2(k) per vector,
+ * the kx = 0 plane halved and k = 0 removed.  Returns the count; rk[3q] and
+ * M2[9q] are malloc'ed. */
+static int ref_kspace(const orc_lattice_t* lat, double M_HI, double rce, double box_volume, double** rk_out,
+                      double** M2_out) {
+  int kx, ky, kz, kxmax, kymax, kzmax, ylower, zlower, yupper, zupper;
+  double rkx = 0, rky = 0, rkz, rkk, rkxy, m2;
   const double* L1 = lat->L1;
   const double* L2 = lat->L2;
-  double L = lat->L, L2yp = lat->L2yp, detL = lat->detL;
-  int tp = lat->tp, krbc = (lat->flow == 2);
-  double theta = lat->theta;
-  treal = lat->treal;
-  if (treal <= tp / 2) {
-    rce = 1.0 * L2yp;
-    M_HI = 4.0;
-    nmax = 1;
-  } else {
-    rce = 2.0 * L2yp;
-    M_HI = 4.0;
-    nmax = 2;
-  }
-  tf = (double)treal / tp;
-  nt = (int)round(tf * nt_avg) % nt_avg;
-  alpha = M_HI / rce;
-  rkcut = 2.0 * M_HI * M_HI / rce;
-  rkkcut = rkcut * rkcut;
-  rtpi = sqrt(M_PI);
-  alpha2 = alpha * alpha;
-  alpha3 = alpha2 * alpha;
-  alpha4 = alpha2 * alpha2;
-  alpha5 = alpha4 * alpha;
-  alpha7 = alpha4 * alpha2 * alpha;
-  selfconst = 1 - 6 / rtpi * alpha + 40 / (3 * rtpi) * alpha3;
+  const double L = lat->L, L2yp = lat->L2yp, detL = lat->detL;
+  const int krbc = (lat->flow == 2);
+  const double alpha = M_HI / rce, alpha2 = alpha * alpha, alpha4 = alpha2 * alpha2;
+  const double rkcut = 2.0 * M_HI * M_HI / rce, rkkcut = rkcut * rkcut;
   kxmax = (int)ceil(rkcut * detL / (M_PI * 2.0 * L * L2[1]));
   kymax = (int)ceil(rkcut * detL / (M_PI * 2.0 * L * L1[0]));
   kzmax = (int)ceil(rkcut * L / (M_PI * 2.0));
-  int kxalloc = kxmax + 1;
+  const int kxalloc = kxmax + 1;
   int* kyupper = (int*)calloc((size_t)kxalloc + 2, sizeof(int));
   int* kylower = (int*)calloc((size_t)kxalloc + 2, sizeof(int));
   int* kzupper = (int*)calloc((size_t)(kxalloc + 2) * (2 * kymax + 1), sizeof(int));
   int* kzlower = (int*)calloc((size_t)(kxalloc + 2) * (2 * kymax + 1), sizeof(int));
-  kcoeff = krbc ? M_PI * 2.0 / detL : M_PI * 2.0;
+  const double kcoeff = krbc ? M_PI * 2.0 / detL : M_PI * 2.0;
 #define RK_SET()                                             \
   do {                                                       \
     if (krbc) {                                              \
@@ -741,13 +722,14 @@ void orc_calchi_ref(const orc_lattice_t* lat, int N, const double* rx, const dou
   }
   kxmax++;
   if (kxmax > kxalloc - 1) kxmax = kxalloc - 1; /* stay inside the arrays the reference allocated */
-  kcount = 0;
+  int kcount = 0;
   for (kx = 0; kx < kxmax + 1; kx++)
     for (ky = kylower[kx]; ky < kyupper[kx] + 1; ky++)
       for (kz = kzlower[(2 * kymax + 1) * kx + ky + kymax]; kz < kzupper[(2 * kymax + 1) * kx + ky + kymax] + 1; kz++)
         kcount++;
   double* M2 = (double*)calloc((size_t)9 * (kcount + 1), sizeof(double));
-  kcount = 0;
+  double* rk = (double*)calloc((size_t)3 * (kcount + 1), sizeof(double));
+  int q = 0;
   for (kx = 0; kx < kxmax + 1; kx++) {
     for (ky = kylower[kx]; ky < kyupper[kx] + 1; ky++) {
       RK_SET();
@@ -757,41 +739,56 @@ void orc_calchi_ref(const orc_lattice_t* lat, int N, const double* rx, const dou
         rkk = rkxy + rkz * rkz;
         m2 = 2.0 * (1.0 - rkk / 3.0) * (1.0 + rkk / (4.0 * alpha2) + rkk * rkk / (8.0 * alpha4)) * 6.0 * M_PI / rkk *
              exp(-rkk / (4.0 * alpha2));
-        start = 9 * kcount;
-        M2[start] = m2 * (1 - rkx * rkx / rkk) / box_volume;
-        M2[start + 1] = m2 * (-rkx * rky / rkk) / box_volume;
-        M2[start + 2] = m2 * (-rkx * rkz / rkk) / box_volume;
-        M2[start + 3] = m2 * (-rky * rkx / rkk) / box_volume;
-        M2[start + 4] = m2 * (1 - rky * rky / rkk) / box_volume;
-        M2[start + 5] = m2 * (-rky * rkz / rkk) / box_volume;
-        M2[start + 6] = m2 * (-rkz * rkx / rkk) / box_volume;
-        M2[start + 7] = m2 * (-rkz * rky / rkk) / box_volume;
-        M2[start + 8] = m2 * (1 - rkz * rkz / rkk) / box_volume;
-        kcount++;
+        double* m = M2 + 9 * q;
+        m[0] = m2 * (1 - rkx * rkx / rkk) / box_volume;
+        m[1] = m2 * (-rkx * rky / rkk) / box_volume;
+        m[2] = m2 * (-rkx * rkz / rkk) / box_volume;
+        m[3] = m2 * (-rky * rkx / rkk) / box_volume;
+        m[4] = m2 * (1 - rky * rky / rkk) / box_volume;
+        m[5] = m2 * (-rky * rkz / rkk) / box_volume;
+        m[6] = m2 * (-rkz * rkx / rkk) / box_volume;
+        m[7] = m2 * (-rkz * rky / rkk) / box_volume;
+        m[8] = m2 * (1 - rkz * rkz / rkk) / box_volume;
+        rk[3 * q] = rkx;
+        rk[3 * q + 1] = rky;
+        rk[3 * q + 2] = rkz;
+        q++;
       }
     }
   }
-  kcount = 0;
+  q = 0;
   for (ky = kylower[0]; ky < kyupper[0] + 1; ky++) {
     for (kz = kzlower[ky + kymax]; kz < kzupper[ky + kymax] + 1; kz++) {
-      start = 9 * kcount;
-      for (int i = 0; i < 9; i++) M2[start + i] /= 2.0;
+      for (int i = 0; i < 9; i++) M2[9 * q + i] /= 2.0;
       if (ky == 0 && kz == 0)
-        for (int i = 0; i < 9; i++) M2[start + i] = 0.0;
-      kcount++;
+        for (int i = 0; i < 9; i++) M2[9 * q + i] = 0.0;
+      q++;
     }
   }
-  int ktotal = 0;
-  for (kx = 0; kx < kxmax + 1; kx++)
-    for (ky = kylower[kx]; ky < kyupper[kx] + 1; ky++)
-      for (kz = kzlower[(2 * kymax + 1) * kx + ky + kymax]; kz < kzupper[(2 * kymax + 1) * kx + ky + kymax] + 1; kz++)
-        ktotal++;
-  if (kcount_out) *kcount_out = ktotal;
-  /* self term with the reference's image loop (all three indices nonzero, dy*dy in z) */
+#undef RK_SET
+  free(kyupper);
+  free(kylower);
+  free(kzupper);
+  free(kzlower);
+  *rk_out = rk;
+  *M2_out = M2;
+  return kcount;
+}
+
+/* The self block of calcHI and ewaldBin (S/main26.c:1653-1686, 1097-1130): selfconst,
+ * the reciprocal diagonal and the image sum with its quirks (only images with all
+ * three indices nonzero; the z entry adds dy*dy; no r < rce test). */
+static void ref_self(const orc_lattice_t* lat, double alpha, int nmax, int kcount, const double* M2, double Dtr[3]) {
+  const double* L1 = lat->L1;
+  const double* L2 = lat->L2;
+  const double L = lat->L, rtpi = sqrt(M_PI);
+  const double alpha2 = alpha * alpha, alpha3 = alpha2 * alpha, alpha4 = alpha2 * alpha2, alpha5 = alpha4 * alpha,
+               alpha7 = alpha4 * alpha2 * alpha;
+  const double selfconst = 1 - 6 / rtpi * alpha + 40 / (3 * rtpi) * alpha3;
   Dtr[0] = selfconst;
   Dtr[1] = selfconst;
   Dtr[2] = selfconst;
-  for (int q = 0; q < ktotal; ++q) {
+  for (int q = 0; q < kcount; ++q) {
     Dtr[0] += M2[9 * q];
     Dtr[1] += M2[9 * q + 4];
     Dtr[2] += M2[9 * q + 8];
@@ -813,6 +810,58 @@ void orc_calchi_ref(const orc_lattice_t* lat, int N, const double* rx, const dou
           Dtr[2] += term1 + term2 * dy * dy / rr;
         }
       }
+}
+
+/* The pair block Dt of calcHI and ewaldBin for the raw difference (dxo, dyo, dzo):
+ * every image within rce with the reference's r <= 2 branch, plus the k-sum with
+ * cos(k.r) of the raw difference (S/main26.c:1717-1788, 1150-1219). */
+static void ref_pair(const orc_lattice_t* lat, double alpha, double rce, int nmax, int kcount, const double* rk,
+                     const double* M2, double dxo, double dyo, double dzo, double Dt[9]) {
+  const double* L1 = lat->L1;
+  const double* L2 = lat->L2;
+  const double L = lat->L;
+  for (int q = 0; q < 9; ++q) Dt[q] = 0.0;
+  for (int nx = -nmax; nx < nmax + 1; nx++)
+    for (int ny = -nmax; ny < nmax + 1; ny++)
+      for (int nz = -nmax; nz < nmax + 1; nz++) {
+        double dx = dxo + nx * L1[0] + ny * L2[0], dy = dyo + nx * L1[1] + ny * L2[1], dz = dzo + nz * L;
+        double rr = dx * dx + dy * dy + dz * dz, r = sqrt(rr);
+        if (r < rce) {
+          double t1, t2;
+          orc_rpy_real_coeffs(r, alpha, 1, &t1, &t2);
+          double d[3] = {dx, dy, dz};
+          for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) Dt[3 * a + b] += (a == b ? t1 : 0.0) + t2 * d[a] * d[b] / rr;
+        }
+      }
+  for (int q = 0; q < kcount; ++q) {
+    double coskr = cos(rk[3 * q] * dxo + rk[3 * q + 1] * dyo + rk[3 * q + 2] * dzo);
+    for (int a = 0; a < 9; ++a) Dt[a] += M2[9 * q + a] * coskr;
+  }
+}
+
+/* calcHI restated with its quirks (S/main26.c:1530-1806; X/main14.c:1548-1824). */
+void orc_calchi_ref(const orc_lattice_t* lat, int N, const double* rx, const double* ry, const double* rz,
+                    uint64_t t, int nt_avg, const orc_bins_t* bins, const double* table, double box_volume,
+                    double* M, int* far_count, int* kcount_out) {
+  (void)t;
+  const int tp = lat->tp, krbc = (lat->flow == 2), treal = lat->treal;
+  const double theta = lat->theta, L2yp = lat->L2yp;
+  double rce, M_HI = 4.0;
+  int nmax;
+  if (treal <= tp / 2) {
+    rce = 1.0 * L2yp;
+    nmax = 1;
+  } else {
+    rce = 2.0 * L2yp;
+    nmax = 2;
+  }
+  const int nt = (int)round((double)treal / tp * nt_avg) % nt_avg;
+  const double alpha = M_HI / rce;
+  double *rk, *M2, Dtr[3];
+  const int ktotal = ref_kspace(lat, M_HI, rce, box_volume, &rk, &M2);
+  if (kcount_out) *kcount_out = ktotal;
+  ref_self(lat, alpha, nmax, ktotal, M2, Dtr);
   size_t n3 = (size_t)3 * N;
   memset(M, 0, sizeof(double) * n3 * n3);
   for (int i = 0; i < N; i++) {
@@ -823,8 +872,6 @@ void orc_calchi_ref(const orc_lattice_t* lat, int N, const double* rx, const dou
   int nfar = 0;
 #pragma omp parallel for schedule(dynamic) reduction(+ : nfar)
   for (int i = 0; i < N; i++) {
-    int kx, ky, kz, start;
-    double rkx, rky, rkz;
     double rxi = krbc ? rx[i] * cos(theta) - ry[i] * sin(theta) : rx[i];
     double ryi = krbc ? rx[i] * sin(theta) + ry[i] * cos(theta) : ry[i];
     for (int j = i + 1; j < N; j++) {
@@ -836,33 +883,7 @@ void orc_calchi_ref(const orc_lattice_t* lat, int N, const double* rx, const dou
       double drrbmin = sqrt(dmin[0] * dmin[0] + dmin[1] * dmin[1] + dmin[2] * dmin[2]);
       if (drrbmin < 12.0 || !table) {
         if (!(drrbmin < 12.0)) nfar++;
-        double dxo = rxj - rxi, dyo = ryj - ryi, dzo = rz[j] - rz[i];
-        for (int nx = -nmax; nx < nmax + 1; nx++)
-          for (int ny = -nmax; ny < nmax + 1; ny++)
-            for (int nz = -nmax; nz < nmax + 1; nz++) {
-              double dx = dxo + nx * L1[0] + ny * L2[0], dy = dyo + nx * L1[1] + ny * L2[1], dz = dzo + nz * L;
-              double rr = dx * dx + dy * dy + dz * dz, r = sqrt(rr);
-              if (r < rce) {
-                double t1, t2;
-                orc_rpy_real_coeffs(r, alpha, 1, &t1, &t2);
-                double d[3] = {dx, dy, dz};
-                for (int a = 0; a < 3; ++a)
-                  for (int b = 0; b < 3; ++b) Dt[3 * a + b] += (a == b ? t1 : 0.0) + t2 * d[a] * d[b] / rr;
-              }
-            }
-        int kc = 0;
-        for (kx = 0; kx < kxmax + 1; kx++) {
-          for (ky = kylower[kx]; ky < kyupper[kx] + 1; ky++) {
-            RK_SET();
-            for (kz = kzlower[(2 * kymax + 1) * kx + ky + kymax]; kz < kzupper[(2 * kymax + 1) * kx + ky + kymax] + 1; kz++) {
-              rkz = kz * M_PI * 2.0 / L;
-              start = 9 * kc;
-              double coskr = cos(rkx * dxo + rky * dyo + rkz * dzo);
-              for (int q = 0; q < 9; ++q) Dt[q] += M2[start + q] * coskr;
-              kc++;
-            }
-          }
-        }
+        ref_pair(lat, alpha, rce, nmax, ktotal, rk, M2, rxj - rxi, ryj - ryi, rz[j] - rz[i], Dt);
       } else {
         int bin = orc_binij(bins, lat, rx, ry, rz, i, j);
         /* the shear table spans |dx| <= L/2*edot*tp*dt only (S/Initialization.c:269); the
@@ -879,12 +900,67 @@ void orc_calchi_ref(const orc_lattice_t* lat, int N, const double* rx, const dou
     }
   }
   if (far_count) *far_count = nfar;
-#undef RK_SET
-  free(kyupper);
-  free(kylower);
-  free(kzupper);
-  free(kzlower);
+  free(rk);
   free(M2);
+}
+
+/* ewaldBin restated (S/main26.c:975-1260, X/main14.c:1009-1383): the pair blocks
+ * of every pair i < j (chains after i's own first, then within each chain) from
+ * the raw position differences, no rotation; incr = Dt[a]/Dtr[a/3], eps += 2 incr
+ * and rowsum[3i + a/3], rowsum[3j + a/3] += incr^2 (both updates, as the serial
+ * program makes them).  eps is the sum before CATEA's normalisation.  The
+ * reference's split is M_HI = 3.5; M_HI is a parameter so that tests can drive
+ * the same code to a converged Ewald sum. */
+void orc_ewald_bin(const orc_lattice_t* lat, int N, int Nb, const double* rx, const double* ry, const double* rz,
+                   double box_volume, double M_HI, double* eps, double* rowsum, double Dtr[3], int* kcount_out) {
+  const int tp = lat->tp, treal = lat->treal;
+  const double L2yp = lat->L2yp;
+  double rce;
+  int nmax;
+  if (treal <= tp / 2) {
+    rce = 1.0 * L2yp;
+    nmax = 1;
+  } else {
+    rce = 2.0 * L2yp;
+    nmax = 2;
+  }
+  const double alpha = M_HI / rce;
+  double *rk, *M2;
+  const int kc = ref_kspace(lat, M_HI, rce, box_volume, &rk, &M2);
+  if (kcount_out) *kcount_out = kc;
+  ref_self(lat, alpha, nmax, kc, M2, Dtr);
+  double e = 0.0, Dt[9];
+  for (int i = 0; i < 3 * N; ++i) rowsum[i] = 0.0;
+  for (int pass = 0; pass < 2; ++pass)
+    for (int i = 0; i < N; i++) {
+      const int jb = pass == 0 ? (i / Nb + 1) * Nb : i + 1, je = pass == 0 ? N : (i / Nb + 1) * Nb;
+      for (int j = jb; j < je; j++) {
+        ref_pair(lat, alpha, rce, nmax, kc, rk, M2, rx[j] - rx[i], ry[j] - ry[i], rz[j] - rz[i], Dt);
+        for (int a = 0; a < 9; a++) {
+          double incr = Dt[a] / Dtr[a / 3];
+          e += 2.0 * incr;
+          rowsum[3 * i + a / 3] += incr * incr;
+          rowsum[3 * j + a / 3] += incr * incr;
+        }
+      }
+    }
+  *eps = e;
+  free(rk);
+  free(M2);
+}
+
+/* CATEA restated (S/main26.c:1317-1381): eps/(9N^2 - 3N), beta from the quadratic
+ * root (1 - sqrt(1 - (3N eps^2 - 3N eps)))/(3N eps^2 - 3N eps), and
+ * C_i = sqrt(1/(1 + rowsum_i beta^2)) for one sample (the strain-bin running sums
+ * are the caller's).  9N^2 - 3N is formed in FP64: the reference's int expression
+ * overflows above N = 15,447. */
+double orc_catea(int N, double eps_sum, const double* rowsum, double* C) {
+  const int ROW = 3 * N;
+  double eps = eps_sum / (9.0 * N * N - 3.0 * N);
+  double eps2 = eps * eps;
+  double btemp = (1.0 - sqrt(1.0 - (ROW * eps2 - ROW * eps))) / (ROW * eps2 - ROW * eps);
+  for (int i = 0; i < 3 * N; i++) C[i] = sqrt(1.0 / (1.0 + rowsum[i] * btemp * btemp));
+  return btemp;
 }
 
 /* ========================================================== Brownian helpers */

@@ -105,6 +105,9 @@ void orc_calchi_ref(const orc_lattice_t* lat, int n, const double* rx, const dou
                     const double* rz, uint64_t t, int nt_avg, const orc_bins_t* bins,
                     const double* table /* nt_avg*9*num_bins or NULL */, double box_volume,
                     double* M, int* far_count, int* kcount_out);
+void orc_ewald_bin(const orc_lattice_t* lat, int N, int Nb, const double* rx, const double* ry, const double* rz,
+                   double box_volume, double M_HI, double* eps, double* rowsum, double Dtr[3], int* kcount_out);
+double orc_catea(int N, double eps_sum, const double* rowsum, double* C);
 
 /* ---------------- Brownian displacement helpers */
 /* TEA product (S/main26.c:787-804): y_i = C_i (.) (M_ii R_i + beta sum_{j!=i} M_ij R_j),

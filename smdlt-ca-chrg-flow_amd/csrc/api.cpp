@@ -1780,6 +1780,203 @@ int smdlt_set_tea(smdlt_ctx* c, const double* beta, const double* C) {
   return gather_aos(c, C, c->tea_C);
 }
 
+// ---------------------------------------------------------------- TEA sample
+// The reciprocal sum of ewaldBin (S/main26.c:1000-1097, X/main14.c:1034-1131) as
+// the reference bounds it: for kx = 0..kxmax the ky range runs from the first to
+// the last in-cut value (first exit ends it), likewise kz within each (kx, ky);
+// kxmax is then fixed by the reference's while-loop; M2(k) per vector, the kx = 0
+// plane halved and k = 0 removed.  Shear names: rkx from ky, rky from kx; KRBC:
+// kcoeff = 2 pi/detL over the deformed lattice.  Appends 12 doubles per vector
+// (rkx, rky, rkz, M2[9]) and returns the count.
+static int tea_kspace(const HostLattice& h, double M_HI, double rce, double vol, std::vector<double>& tab) {
+  const double L = h.L, detL = h.detL, L2yp = h.L2yp;
+  const double* L1 = h.L1;
+  const double* L2 = h.L2;
+  const bool krbc = h.flow == 2;
+  const double alpha = M_HI / rce, a2 = alpha * alpha, a4 = a2 * a2;
+  const double rkcut = 2.0 * M_HI * M_HI / rce, rkkcut = rkcut * rkcut;
+  int kxmax = (int)ceil(rkcut * detL / (M_PI * 2.0 * L * L2[1]));
+  const int kymax = (int)ceil(rkcut * detL / (M_PI * 2.0 * L * L1[0]));
+  const int kzmax = (int)ceil(rkcut * L / (M_PI * 2.0));
+  const int kxalloc = kxmax + 1, ny = 2 * kymax + 1;
+  std::vector<int> kyl(kxalloc + 2, 0), kyu(kxalloc + 2, 0), kzl((size_t)(kxalloc + 2) * ny, 0),
+      kzu((size_t)(kxalloc + 2) * ny, 0);
+  const double kc = krbc ? 2.0 * M_PI / detL : 2.0 * M_PI;
+  auto rk = [&](int kx, int ky, double& rkx, double& rky) {
+    if (krbc) {
+      rkx = kc * L * (kx * L2[1] - ky * L1[1]);
+      rky = kc * L * (-kx * L2[0] + ky * L1[0]);
+    } else {
+      rky = kc * (kx - ky * (L2[0] / L1[0])) / L2[1];
+      rkx = kc * ky / L1[0];
+    }
+  };
+  for (int kx = 0; kx <= kxmax; ++kx) {
+    bool ylo = false, yhi = false;
+    for (int ky = -kymax; ky <= kymax; ++ky) {
+      double rkx, rky;
+      rk(kx, ky, rkx, rky);
+      const double rkxy = rkx * rkx + rky * rky;
+      bool zlo = false, zhi = false;
+      for (int kz = -kzmax; kz <= kzmax; ++kz) {
+        const double rkz = kz * M_PI * 2.0 / L, rkk = rkxy + rkz * rkz;
+        if (rkk < rkkcut && !zlo) {
+          kzl[(size_t)ny * kx + ky + kymax] = kz;
+          zlo = true;
+        }
+        if (rkk > rkkcut && zlo && !zhi) {
+          kzu[(size_t)ny * kx + ky + kymax] = kz - 1;
+          zhi = true;
+        }
+      }
+      if (rkxy < rkkcut && !ylo) {
+        kyl[kx] = ky;
+        ylo = true;
+      }
+      if (rkxy > rkkcut && ylo && !yhi) {
+        kyu[kx] = ky - 1;
+        yhi = true;
+      }
+    }
+  }
+  int kx = 0;
+  while (kx < kxalloc && kyu[kx] != -kymax && kx <= (int)(rkcut * detL / (M_PI * 2.0 * L * L2yp))) {
+    kxmax = kx;
+    ++kx;
+  }
+  kxmax = std::min(kxmax + 1, kxalloc - 1);
+  int count = 0;
+  for (int kx = 0; kx <= kxmax; ++kx)
+    for (int ky = kyl[kx]; ky <= kyu[kx]; ++ky) {
+      double rkx, rky;
+      rk(kx, ky, rkx, rky);
+      const double rkxy = rkx * rkx + rky * rky;
+      for (int kz = kzl[(size_t)ny * kx + ky + kymax]; kz <= kzu[(size_t)ny * kx + ky + kymax]; ++kz) {
+        const double rkz = kz * M_PI * 2.0 / L, rkk = rkxy + rkz * rkz;
+        const double m2 = 2.0 * (1.0 - rkk / 3.0) * (1.0 + rkk / (4.0 * a2) + rkk * rkk / (8.0 * a4)) * 6.0 * M_PI /
+                          rkk * exp(-rkk / (4.0 * a2));
+        double m[9] = {m2 * (1 - rkx * rkx / rkk) / vol,  m2 * (-rkx * rky / rkk) / vol, m2 * (-rkx * rkz / rkk) / vol,
+                       m2 * (-rky * rkx / rkk) / vol,     m2 * (1 - rky * rky / rkk) / vol, m2 * (-rky * rkz / rkk) / vol,
+                       m2 * (-rkz * rkx / rkk) / vol,     m2 * (-rkz * rky / rkk) / vol, m2 * (1 - rkz * rkz / rkk) / vol};
+        if (kx == 0)
+          for (double& v : m) v = (ky == 0 && kz == 0) ? 0.0 : v / 2.0;
+        tab.push_back(rkx);
+        tab.push_back(rky);
+        tab.push_back(rkz);
+        for (double v : m) tab.push_back(v);
+        ++count;
+      }
+    }
+  return count;
+}
+
+int smdlt_tea_sample(smdlt_ctx* c, double* beta, double* C, double* eps) {
+  if (!c || !beta || !C) return SMDLT_E_ARG;
+  if (!c->have_pos) return SMDLT_E_STATE;
+  HCK(hipSetDevice(c->device));
+  const int rb = c->cur_rep_begin, re = c->cur_rep_end;
+  std::vector<TeaRep> tr;
+  std::vector<double> tab;
+  std::vector<int> brep;
+  const double M_HI = 3.5;
+  for (int r = rb; r < re; ++r) {
+    const HostLattice& h = c->lat[r];
+    const RepDev& R = c->rh[r];
+    TeaRep t{};
+    const bool first_half = h.treal <= h.tp / 2;
+    t.rce = first_half ? h.L2yp : 2.0 * h.L2yp;
+    t.nmax = first_half ? 1 : 2;
+    t.alpha = M_HI / t.rce;
+    t.alpha2 = t.alpha * t.alpha;
+    t.alpha3 = t.alpha2 * t.alpha;
+    t.alpha5 = t.alpha3 * t.alpha2;
+    t.alpha7 = t.alpha5 * t.alpha2;
+    t.L1[0] = h.L1[0];
+    t.L1[1] = h.L1[1];
+    t.L2[0] = h.L2[0];
+    t.L2[1] = h.L2[1];
+    t.L = h.L;
+    t.slot0 = R.slot0;
+    t.n = R.n;
+    t.bead0 = (int)brep.size();
+    t.k0 = (int)(tab.size() / 12);
+    t.nk = tea_kspace(h, M_HI, t.rce, h.L * h.L * h.L, tab);
+    if (t.nk > TEA_MAXK) return SMDLT_E_GEOMETRY;
+    // self block (S/main26.c:1097-1130): selfconst, reciprocal diagonal, image sum
+    // over images with all three indices nonzero (z entry with dy*dy), no rce test
+    const double a = t.alpha, rtpi = sqrt(M_PI);
+    double sc = 1 - 6 / rtpi * a + 40 / (3 * rtpi) * t.alpha3;
+    t.dtr[0] = t.dtr[1] = t.dtr[2] = sc;
+    for (int q = 0; q < t.nk; ++q) {
+      const double* m = &tab[12 * (size_t)(t.k0 + q) + 3];
+      t.dtr[0] += m[0];
+      t.dtr[1] += m[4];
+      t.dtr[2] += m[8];
+    }
+    for (int nx = -t.nmax; nx <= t.nmax; ++nx)
+      for (int ny = -t.nmax; ny <= t.nmax; ++ny)
+        for (int nz = -t.nmax; nz <= t.nmax; ++nz) {
+          if (!(nx != 0 && ny != 0 && nz != 0)) continue;
+          double dx = nx * h.L1[0] + ny * h.L2[0], dy = nx * h.L1[1] + ny * h.L2[1], dz = nz * h.L;
+          double rr = dx * dx + dy * dy + dz * dz, rd = sqrt(rr);
+          double C1 = 0.75 / rd + 0.5 / (rd * rr);
+          double C2 = 4.0 * t.alpha7 * rr * rr + 3.0 * t.alpha3 * rr - 20.0 * t.alpha5 * rr - 4.5 * a +
+                      14.0 * t.alpha3 + a / rr;
+          double C3 = 0.75 / rd - 1.5 / (rd * rr);
+          double C4 = -4.0 * t.alpha7 * rr * rr - 3.0 * t.alpha3 * rr + 16.0 * t.alpha5 * rr + 1.5 * a -
+                      2.0 * t.alpha3 - 3.0 * a / rr;
+          double term1 = C1 * erfc(a * rd) + C2 * exp(-t.alpha2 * rr) / rtpi;
+          double term2 = C3 * erfc(a * rd) + C4 * exp(-t.alpha2 * rr) / rtpi;
+          t.dtr[0] += term1 + term2 * dx * dx / rr;
+          t.dtr[1] += term1 + term2 * dy * dy / rr;
+          t.dtr[2] += term1 + term2 * dy * dy / rr;
+        }
+    for (int i = 0; i < R.n; ++i) brep.push_back(r - rb);
+    tr.push_back(t);
+  }
+  const int nb = (int)brep.size();
+  TeaRep* d_tr = nullptr;
+  int* d_brep = nullptr;
+  double *d_tab = nullptr, *d_eps = nullptr, *d_rs = nullptr;
+  int st = alloc_dev(&d_tr, tr.size());
+  st |= alloc_dev(&d_brep, brep.size());
+  st |= alloc_dev(&d_tab, tab.size());
+  st |= alloc_dev(&d_eps, (size_t)c->nst);
+  st |= alloc_dev(&d_rs, 3 * (size_t)c->nst);
+  std::vector<double> he(c->nst), hr(3 * (size_t)c->nst);
+  if (!st) {
+    HCK(hipMemcpy(d_tr, tr.data(), sizeof(TeaRep) * tr.size(), hipMemcpyHostToDevice));
+    HCK(hipMemcpy(d_brep, brep.data(), sizeof(int) * brep.size(), hipMemcpyHostToDevice));
+    if (!tab.empty()) HCK(hipMemcpy(d_tab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice));
+    launch_tea_rows(c->stream, d_tr, d_brep, nb, c->rx, c->ry, c->rz, d_tab, d_eps, d_rs);
+    HCK(hipGetLastError());
+    HCK(hipMemcpyAsync(he.data(), d_eps, 8 * he.size(), hipMemcpyDeviceToHost, c->stream));
+    HCK(hipMemcpyAsync(hr.data(), d_rs, 8 * hr.size(), hipMemcpyDeviceToHost, c->stream));
+    HCK(hipStreamSynchronize(c->stream));
+  }
+  for (void* p : {(void*)d_tr, (void*)d_brep, (void*)d_tab, (void*)d_eps, (void*)d_rs})
+    if (p) hipFree(p);
+  if (st) return SMDLT_E_NOMEM;
+  // CATEA (S/main26.c:1317-1381) per replica: eps/(9N^2 - 3N) (in FP64: the
+  // reference's int expression overflows above N = 15,447), beta, C
+  size_t off = 0;
+  for (int r = 0; r < c->nrep; ++r) {
+    const RepDev& R = c->rh[r];
+    if (r >= rb && r < re) {
+      const double N = R.n, ROW = 3.0 * R.n;
+      double es = 0.0;
+      for (int i = 0; i < R.n; ++i) es += he[R.slot0 + i];
+      const double e = es / (9.0 * N * N - 3.0 * N), e2 = e * e;
+      const double b = (1.0 - sqrt(1.0 - (ROW * e2 - ROW * e))) / (ROW * e2 - ROW * e);
+      beta[r] = b;
+      if (eps) eps[r] = e;
+      for (int i = 0; i < 3 * R.n; ++i) C[off + i] = sqrt(1.0 / (1.0 + hr[3 * (size_t)R.slot0 + i] * b * b));
+    }
+    off += 3 * (size_t)R.n;
+  }
+  return 0;
+}
+
 // One timestep for the active replica range; noise_dev: device noise (nullptr =
 // generate).  Lanczos early exit on the host only when host_exit.
 static int step_internal(smdlt_ctx* c, uint64_t t, double* noise_dev, bool host_exit) {

@@ -322,6 +322,18 @@ struct MobArgs {
 #define RTAB_INV 16
 template <typename T> void launch_mobility(hipStream_t s, const MobArgs& a);
 
+// One replica's parameters for the TEA sample (ewaldBin): lattice vectors as the
+// reference holds them, the Ewald split it uses (M_HI = 3.5) and its k table range.
+#define TEA_THREADS 256
+#define TEA_MAXK 512
+struct TeaRep {
+  int slot0, n, bead0, k0, nk, nmax;
+  double L1[2], L2[2], L, rce, alpha, alpha2, alpha3, alpha5, alpha7;
+  double dtr[3];
+};
+void launch_tea_rows(hipStream_t s, const TeaRep* reps, const int* bead_rep, int nbeads, const double* rx,
+                     const double* ry, const double* rz, const double* ktab, double* eps_out, double* rowsum_out);
+
 void launch_build_clusters(hipStream_t st, const RepDev* reps, const int* slot_rep, int cl_begin, int ncl,
                            int max_words, const int* cl_first, const int* cl_cnt, const int* slot_cl,
                            const int* rep_flags, const int* nrpy, const int* lrpy, int cap_rpy, int capc, int* nclp,

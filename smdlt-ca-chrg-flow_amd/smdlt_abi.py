@@ -84,6 +84,7 @@ def lib():
         L.smdlt_last_iters.argtypes = [C.c_void_p, _ip]
         L.smdlt_lanczos_status.argtypes = [C.c_void_p, _ip, _ip, _dp]
         L.smdlt_set_tea.argtypes = [C.c_void_p, _dp, _dp]
+        L.smdlt_tea_sample.argtypes = [C.c_void_p, _dp, _dp, _dp]
         L.smdlt_set_profiling.argtypes = [C.c_void_p, C.c_int]
         L.smdlt_stage_times.argtypes = [C.c_void_p, _dp, _ip, C.c_int]
         L.smdlt_stage_name.restype = C.c_char_p
@@ -263,6 +264,15 @@ class Context:
         it = np.zeros(self.nrep, np.int32)
         _chk(lib().smdlt_last_iters(self.h, _ptr(it, C.c_int)), "last_iters")
         return it
+
+    def tea_sample(self):
+        """smdlt_tea_sample: (beta[nrep], C[3N], eps[nrep]) from the current positions and lattice."""
+        beta = np.zeros(self.nrep)
+        eps = np.zeros(self.nrep)
+        Cv = np.zeros(3 * self.ntot)
+        _chk(lib().smdlt_tea_sample(self.h, _ptr(beta, C.c_double), _ptr(Cv, C.c_double), _ptr(eps, C.c_double)),
+             "tea_sample")
+        return beta, Cv, eps
 
     def set_tea(self, beta, Cv):
         beta = np.ascontiguousarray(np.broadcast_to(beta, (self.nrep,)), np.float64)

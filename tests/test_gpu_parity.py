@@ -272,6 +272,53 @@ def test_tea_matches_oracle():
     assert rel(y, O.tea_apply(Md, 0.37, Cv, z)) < 1e-5
 
 
+@pytest.mark.parametrize("flow,t", [(1, 100), (1, 15000), (0, 0), (2, 300)])
+def test_tea_sample_matches_ewald_bin_catea(flow, t):
+    """ewaldBin + CATEA (S/main26.c:975-1381): beta, C and eps of one sample against
+    the oracle, in both halves of the shear period, at equilibrium and under KRBC;
+    the random walks overlap, so the reference's r <= 2 branch is exercised."""
+    L, nb, nc = 20.0, 20, 4
+    n = nb * nc
+    edot = 0.05 if flow else 0.0
+    ctx, p = ctx_for(flow, L, nb, nc, prec=S.PREC_FP64, edot=edot, dt=1e-3)
+    rx, ry, rz = xyz(random_walk(nb, nc, L, 21))
+    ctx.upload_positions(rx, ry, rz)
+    ctx.update_lattice(t)
+    beta, Cv, eps = ctx.tea_sample()
+    lat = O.lattice(flow, L, edot, 1e-3, 1, t)
+    e, rs, dtr, kc = O.ewald_bin(lat, rx, ry, rz, nb, L ** 3)
+    b, Co = O.catea(n, e, rs)
+    assert beta[0] == pytest.approx(b, rel=1e-10)
+    assert eps[0] == pytest.approx(e / (9 * n * n - 3 * n), rel=1e-10)
+    assert np.abs(Cv - Co).max() < 1e-10
+
+
+def test_tea_sample_batched_equals_single_contexts():
+    specs = [(1, 22.0, 20, 3), (2, 18.0, 15, 4), (0, 16.0, 10, 2)]
+    params, poss = [], []
+    for k, (flow, L, nb, nc) in enumerate(specs):
+        params.append(S.default_params(nb=nb, nc=nc, L=L, flow_type=flow, edot=0.05 if flow else 0.0, dt=1e-3,
+                                       precision=S.PREC_FP64))
+        poss.append(random_walk(nb, nc, L, 50 + k))
+    big = S.Context(params)
+    allp = np.concatenate(poss)
+    big.upload_positions(*xyz(allp))
+    big.update_lattice(400)
+    bb, Cb, eb = big.tea_sample()
+    off = 0
+    for k, pr in enumerate(params):
+        one = S.Context(pr)
+        one.upload_positions(*xyz(poss[k]))
+        one.update_lattice(400)
+        b1, C1, e1 = one.tea_sample()
+        m = 3 * len(poss[k])
+        assert bb[k] == pytest.approx(b1[0], rel=1e-12) and eb[k] == pytest.approx(e1[0], rel=1e-12)
+        assert np.abs(Cb[off:off + m] - C1).max() < 1e-12
+        off += m
+        one.close()
+    big.close()
+
+
 # ----------------------------------------------------------------- timestep
 @pytest.mark.parametrize("flow,mode", [(1, S.NOISE_LANCZOS), (2, S.NOISE_LANCZOS), (1, S.NOISE_FREE_DRAINING),
                                        (2, S.NOISE_FREE_DRAINING), (1, S.NOISE_TEA), (2, S.NOISE_TEA)])

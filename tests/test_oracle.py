@@ -286,6 +286,82 @@ def test_calchi_near_pairs_match_exact_ewald_in_declared_regime():
     assert np.abs(M - Me)[off].max() / np.abs(Me[off]).max() < 2e-5
 
 
+# ----------------------------------------------------------------- ewaldBin / CATEA
+def _separated(lat, L, n, seed, sep=2.2):
+    """random sequential addition with no pair closer than sep, wrapped by applyPBC"""
+    rng = np.random.default_rng(seed)
+    cell = (lat.L1xp, lat.L2xp, lat.L2yp, L)
+    pos = []
+    while len(pos) < n:
+        q = rng.uniform(-L / 2, L / 2, 3)
+        if all(np.linalg.norm(P.min_image(q - x, cell)) > sep for x in pos):
+            pos.append(q)
+    return O.apply_pbc(lat, *xyz(np.array(pos)))
+
+
+def test_ewald_bin_self_block_is_the_ewald_diagonal():
+    # first half of a period, rce = L: the images the reference's && drops lie at
+    # r >= L = rce (erfc(3.5) ~ 7e-7), so Dtr is the Ewald self-mobility truncated
+    # at ewaldBin's split (alpha = 3.5/L, k-cut 2*3.5^2/L): the dense Ewald sum with
+    # the same split and cuts (its k range edge differs from the reference's by
+    # ~1e-6), and within the M_HI = 3.5 truncation (~3e-5) of the converged value,
+    # the survey's 0.95272918 at L = 60 (Hasimoto 1 - 2.837297/L + 4.18879/L^3)
+    L = 60.0
+    pos = np.array([[0.0, 0.0, 0.0], [3.0, 1.0, 0.5]])
+    for flow, t in ((0, 0), (1, 100)):
+        lat = O.lattice(flow, L, 0.01 if flow else 0.0, 1e-3, 1, t)
+        eps, rs, dtr, kc = O.ewald_bin(lat, *xyz(pos), 2, L ** 3)
+        Mt = O.ewald_dense(O.cell_of(lat), pos, 3.5 / L, rcut=L, kcut=2 * 3.5 ** 2 / L)
+        assert np.abs(dtr - np.diag(Mt)[:3]).max() < 2e-6
+        assert np.abs(dtr - 0.95272918).max() < 5e-5
+        assert abs(0.95272918 - (1 - 2.837297 / L + 4.18879 / L ** 3)) < 3e-6
+
+
+@pytest.mark.parametrize("flow,t", [(1, 2000), (1, 15000), (2, 700)])
+def test_ewald_bin_pair_sums_match_exact_ewald(flow, t):
+    # no pair closer than 2, so the pair blocks are the alpha-independent Ewald
+    # sum.  Driven to convergence (M_HI = 6: erfc(6) and the k-cut are ~1e-16) the
+    # restated sums (reference k range and its bounds, images within rce, raw
+    # differences, both halves of the period and the KRBC lattice in the lab
+    # frame) equal eps and rowsum rebuilt from the dense Ewald matrix with
+    # ewaldBin's own Dtr; at the reference's M_HI = 3.5 they carry its ~5e-4
+    # truncation.
+    L, n = 13.0, 12
+    lat = O.lattice(flow, L, 0.05, 1e-3, 1, t)
+    p = np.stack(_separated(lat, L, n, 8), 1)  # in the cell frame (a1 along x)
+    c, s_ = lat.ct, lat.st
+    Rm = np.array([[c, -s_, 0], [s_, c, 0], [0, 0, 1.0]]) if flow == 2 else np.eye(3)  # cell frame -> lab
+    R9 = np.kron(np.eye(n), Rm)
+    Me = R9 @ O.ewald_dense(O.cell_of(lat), p, 0.4) @ R9.T
+    off = ~np.kron(np.eye(n, dtype=bool), np.ones((3, 3), bool))
+    for m_hi, tol in ((6.0, 1e-11), (3.5, 2e-3)):
+        eps, rs, dtr, kc = O.ewald_bin(lat, *xyz(p @ Rm.T), 4, L ** 3, m_hi=m_hi)
+        W = np.where(off, Me, 0.0) / np.repeat(dtr[None, :], n, 0).reshape(-1)[:, None]
+        assert abs(eps - W.sum()) / np.abs(W).sum() < tol
+        rs_dense = (W ** 2).sum(1)
+        assert np.abs(rs - rs_dense).max() / rs_dense.max() < tol
+
+
+def test_catea_beta_is_the_small_root_and_dilute_limit():
+    # beta solves X b^2 - 2b + 1 = 0 (X = 3N(e^2 - e), e = eps/(9N^2 - 3N)) with the
+    # smaller root, and tends to 1/2 as the coupling vanishes; C decreases with rowsum
+    L, n = 13.0, 12
+    lat = O.lattice(1, L, 0.05, 1e-3, 1, 2000)
+    rx, ry, rz = _separated(lat, L, n, 9)
+    eps, rs, dtr, kc = O.ewald_bin(lat, rx, ry, rz, 4, L ** 3)
+    beta, Cv = O.catea(n, eps, rs)
+    e = eps / (9 * n * n - 3 * n)
+    X = 3 * n * (e * e - e)
+    assert abs(X * beta * beta - 2 * beta + 1) < 1e-12
+    assert beta < 1.0 / X * (1 + np.sqrt(1 - X)) if X > 0 else beta > 0
+    order = np.argsort(rs)
+    assert np.all(np.diff(Cv[order]) <= 1e-15) and Cv.max() <= 1.0
+    lat2 = O.lattice(0, 400.0, 0.0, 1e-3, 1, 0)
+    eps2, rs2, _, _ = O.ewald_bin(lat2, np.array([0.0, 60.0]), np.zeros(2), np.zeros(2), 1, 400.0 ** 3)
+    b2, _ = O.catea(2, eps2, rs2)
+    assert abs(b2 - 0.5) < 0.01
+
+
 # ----------------------------------------------------------------- Brownian helpers
 def test_tea_reduces_to_mobility_and_diagonal():
     L = 16.0
