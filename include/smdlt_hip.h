@@ -184,6 +184,21 @@ int smdlt_tea_apply(smdlt_ctx* ctx, const double* beta, const double* C, const d
  * are.  The strain-bin running averages of CATEA are the caller's; beta and C
  * feed smdlt_set_tea.  O(N^2) pair work per replica. */
 int smdlt_tea_sample(smdlt_ctx* ctx, double* beta, double* C, double* eps);
+/* Far-pair table of gridHI (S/gridHI.c:3-278, X/gridHI.c:3-270): the reference's
+ * Ewald pair block (M_HI = 6, rce = L2yp or 2 L2yp over the period) at every bin
+ * displacement of the table geometry of initLattice (bin_size = -g; the origin
+ * bin at the reference's placeholder (0.6, 0.6, 0.6)), for nt_avg strain phases
+ * t = nt*tp/nt_avg, from replica's L, flow, rate and dt.  table: nt_avg * 9 *
+ * dims[0]*dims[1]*dims[2] doubles, [nt][bin][3x3 row-major], bin = (i*dims[1] +
+ * j)*dims[2] + k; dims from smdlt_far_table_dims (sizes: the stretched bin
+ * sizes, optional).  Flowing replicas only (SMDLT_E_ARG at equilibrium). */
+int smdlt_far_table_dims(smdlt_ctx* ctx, int replica, double bin_size, int dims[3], double sizes[3]);
+int smdlt_far_table(smdlt_ctx* ctx, int replica, int nt_avg, double bin_size, double* table);
+/* Writes a table in the reference's mm/G text format (S/gridHI.c:247-263, read
+ * by S/Initialization.c:497-520): per phase and bin, "i j k 1" then three rows
+ * of three %.6e values. */
+int smdlt_write_far_table(const char* path, const double* table, int nt_avg, const int dims[3]);
+
 /* TEA parameters for SMDLT_NOISE_TEA steps (the output of ewaldBin/CATEA,
  * S/main26.c:975-1381): beta per replica, C per bead component (3N AoS). */
 int smdlt_set_tea(smdlt_ctx* ctx, const double* beta, const double* C);

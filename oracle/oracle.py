@@ -112,6 +112,7 @@ def lib():
                                     C.POINTER(C.c_double), _dp, _dp, C.POINTER(C.c_int)]
         L.orc_catea.argtypes = [C.c_int, C.c_double, _dp, _dp]
         L.orc_catea.restype = C.c_double
+        L.orc_far_table.argtypes = [C.POINTER(Lattice), C.c_int, C.POINTER(Bins), C.c_double, _dp]
         L.orc_rpy_ewald_dense.argtypes = [C.POINTER(Cell), C.c_int, _dp, C.c_double, C.c_double, C.c_double, C.c_int, _dp]
         L.orc_rpy_free_dense.argtypes = [C.c_int, _dp, _dp]
         L.orc_ewald_selfconst.argtypes = [C.c_double]
@@ -309,6 +310,14 @@ def ewald_bin(lat, rx, ry, rz, nb, box_volume, m_hi=3.5):
     kc = C.c_int()
     lib().orc_ewald_bin(C.byref(lat), n, nb, rx, ry, rz, box_volume, m_hi, C.byref(eps), rowsum, dtr, C.byref(kc))
     return eps.value, rowsum, dtr, kc.value
+
+
+def far_table(lat, nt_avg, b, box_volume):
+    """gridHI (S/gridHI.c:3-244): table[nt_avg, nbins, 3, 3]; lat is the lattice at t = 0."""
+    nb = b.nbx * b.nby * b.nbz
+    table = np.zeros(nt_avg * 9 * nb)
+    lib().orc_far_table(C.byref(lat), nt_avg, C.byref(b), box_volume, table)
+    return table.reshape(nt_avg, nb, 3, 3)
 
 
 def catea(n, eps, rowsum):

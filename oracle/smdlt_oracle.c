@@ -949,6 +949,51 @@ void orc_ewald_bin(const orc_lattice_t* lat, int N, int Nb, const double* rx, co
   free(M2);
 }
 
+/* gridHI restated (S/gridHI.c:3-244, X/gridHI.c:3-240): for nt = 0..nt_avg-1 the
+ * lattice at t = nt*tp/nt_avg, the split M_HI = 6 with rce = L2yp or 2 L2yp, and
+ * the pair block at every bin displacement of invbin (the origin bin at the
+ * placeholder (0.6, 0.6, 0.6)).  table[nt][9*bin + a]. */
+void orc_far_table(const orc_lattice_t* lat0, int nt_avg, const orc_bins_t* b, double box_volume, double* table) {
+  orc_lattice_t lat = *lat0;
+  const int nbins = b->nbx * b->nby * b->nbz;
+  const int origin = b->nhx * b->nby * b->nbz + b->nhy * b->nbz + b->nhz;
+  for (int nt = 0; nt < nt_avg; nt++) {
+    uint64_t t = (uint64_t)(nt * lat.tp / nt_avg);
+    orc_lattice_update(&lat, t);
+    int nmax;
+    double rce;
+    if (lat.treal <= lat.tp / 2) {
+      rce = lat.L2yp;
+      nmax = 1;
+    } else {
+      rce = 2.0 * lat.L2yp;
+      nmax = 2;
+    }
+    const double M_HI = 6.0, alpha = M_HI / rce;
+    double *rk, *M2;
+    const int kc = ref_kspace(&lat, M_HI, rce, box_volume, &rk, &M2);
+    for (int i = 0; i < nbins; i++) {
+      double dr[3] = {0.6, 0.6, 0.6};
+      if (i != origin) {
+        int bin = i;
+        int bx = (int)(round(bin / (b->nby * b->nbz)));
+        bin -= bx * b->nby * b->nbz;
+        bx -= b->nhx;
+        int by = (int)(round(bin / (b->nbz)));
+        bin -= by * b->nbz;
+        by -= b->nhy;
+        int bz = bin - b->nhz;
+        dr[0] = bx * b->bin_size_x;
+        dr[1] = by * b->bin_size_y;
+        dr[2] = bz * b->bin_size_z;
+      }
+      ref_pair(&lat, alpha, rce, nmax, kc, rk, M2, dr[0], dr[1], dr[2], table + (size_t)nt * 9 * nbins + 9 * (size_t)i);
+    }
+    free(rk);
+    free(M2);
+  }
+}
+
 /* CATEA restated (S/main26.c:1317-1381): eps/(9N^2 - 3N), beta from the quadratic
  * root (1 - sqrt(1 - (3N eps^2 - 3N eps)))/(3N eps^2 - 3N eps), and
  * C_i = sqrt(1/(1 + rowsum_i beta^2)) for one sample (the strain-bin running sums

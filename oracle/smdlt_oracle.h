@@ -108,6 +108,7 @@ void orc_calchi_ref(const orc_lattice_t* lat, int n, const double* rx, const dou
 void orc_ewald_bin(const orc_lattice_t* lat, int N, int Nb, const double* rx, const double* ry, const double* rz,
                    double box_volume, double M_HI, double* eps, double* rowsum, double Dtr[3], int* kcount_out);
 double orc_catea(int N, double eps_sum, const double* rowsum, double* C);
+void orc_far_table(const orc_lattice_t* lat0, int nt_avg, const orc_bins_t* b, double box_volume, double* table);
 
 /* ---------------- Brownian displacement helpers */
 /* TEA product (S/main26.c:787-804): y_i = C_i (.) (M_ii R_i + beta sum_{j!=i} M_ij R_j),

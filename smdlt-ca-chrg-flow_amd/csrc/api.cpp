@@ -1255,11 +1255,10 @@ int smdlt_min_image(smdlt_ctx* c, int replica, int npairs, const int* i, const i
   return 0;
 }
 
-int smdlt_bin_index(smdlt_ctx* c, int replica, double bin_size, int npairs, const int* i, const int* j, int* bins) {
-  if (!c || replica < 0 || replica >= c->nrep || npairs < 0 || bin_size <= 0) return SMDLT_E_ARG;
-  if (npairs == 0) return 0;
-  HCK(hipSetDevice(c->device));
-  // table geometry of initLattice (S/Initialization.c:247-296, X/Initialization.c:258-285)
+// Far-pair table geometry of initLattice and gridHI (S/Initialization.c:247-296,
+// X/Initialization.c:258-285, S/gridHI.c:350-398): half-widths from the lattice
+// at the end of a period, 2*ceil(half/bin)+1 bins per axis, sizes stretched to fit.
+static void table_geometry(const smdlt_ctx* c, int replica, double bin_size, int nb[3], double bs[3]) {
   HostLattice h0;
   const smdlt_params& p = c->P[replica];
   h0.init(p.flow_type, p.L, c->rh[replica].edot, p.dt, p.pbc_compat);
@@ -1272,13 +1271,21 @@ int smdlt_bin_index(smdlt_ctx* c, int replica, double bin_size, int npairs, cons
     int treal = (h0.tp == INT_MAX) ? 0 : h0.tp;
     xmax = p.L / 2 * h0.edot * treal * h0.dt;
   }
-  double bs[3] = {bin_size, bin_size, bin_size};
-  int nbx = 2 * (int)ceil(xmax / bs[0]) + 1;
-  bs[0] = xmax / ((nbx - 1) / 2);
-  int nby = 2 * (int)ceil(ymax / bs[1]) + 1;
-  bs[1] = ymax / ((nby - 1) / 2);
-  int nbz = 2 * (int)ceil(zmax / bs[2]) + 1;
-  bs[2] = zmax / ((nbz - 1) / 2);
+  const double half[3] = {xmax, ymax, zmax};
+  for (int d = 0; d < 3; ++d) {
+    nb[d] = 2 * (int)ceil(half[d] / bin_size) + 1;
+    bs[d] = half[d] / ((nb[d] - 1) / 2);
+  }
+}
+
+int smdlt_bin_index(smdlt_ctx* c, int replica, double bin_size, int npairs, const int* i, const int* j, int* bins) {
+  if (!c || replica < 0 || replica >= c->nrep || npairs < 0 || bin_size <= 0) return SMDLT_E_ARG;
+  if (npairs == 0) return 0;
+  HCK(hipSetDevice(c->device));
+  int nbg[3];
+  double bs[3];
+  table_geometry(c, replica, bin_size, nbg, bs);
+  const int nbx = nbg[0], nby = nbg[1], nbz = nbg[2];
   const HostLattice& h = c->lat[replica];
   int nmax = (h.treal <= h.tp / 2) ? 1 : 2;
   int nbv[6] = {nby, nbz, (nbx - 1) / 2, (nby - 1) / 2, (nbz - 1) / 2, nmax};
@@ -1870,6 +1877,58 @@ static int tea_kspace(const HostLattice& h, double M_HI, double rce, double vol,
   return count;
 }
 
+// The reference's Ewald split for one lattice state (ewaldBin M_HI = 3.5, gridHI
+// M_HI = 6): rce and the image range over the period, alpha powers, lattice
+// vectors, its k table appended to tab, and the self block Dtr with its quirks
+// (S/main26.c:1097-1130: images with all three indices nonzero, z entry with
+// dy*dy, no rce test).
+static TeaRep tea_rep_for(const HostLattice& h, double M_HI, std::vector<double>& tab) {
+  TeaRep t{};
+  const bool first_half = h.treal <= h.tp / 2;
+  t.rce = first_half ? h.L2yp : 2.0 * h.L2yp;
+  t.nmax = first_half ? 1 : 2;
+  t.alpha = M_HI / t.rce;
+  t.alpha2 = t.alpha * t.alpha;
+  t.alpha3 = t.alpha2 * t.alpha;
+  t.alpha5 = t.alpha3 * t.alpha2;
+  t.alpha7 = t.alpha5 * t.alpha2;
+  t.L1[0] = h.L1[0];
+  t.L1[1] = h.L1[1];
+  t.L2[0] = h.L2[0];
+  t.L2[1] = h.L2[1];
+  t.L = h.L;
+  t.k0 = (int)(tab.size() / 12);
+  t.nk = tea_kspace(h, M_HI, t.rce, h.L * h.L * h.L, tab);
+  const double a = t.alpha, rtpi = sqrt(M_PI);
+  double sc = 1 - 6 / rtpi * a + 40 / (3 * rtpi) * t.alpha3;
+  t.dtr[0] = t.dtr[1] = t.dtr[2] = sc;
+  for (int q = 0; q < t.nk; ++q) {
+    const double* m = &tab[12 * (size_t)(t.k0 + q) + 3];
+    t.dtr[0] += m[0];
+    t.dtr[1] += m[4];
+    t.dtr[2] += m[8];
+  }
+  for (int nx = -t.nmax; nx <= t.nmax; ++nx)
+    for (int ny = -t.nmax; ny <= t.nmax; ++ny)
+      for (int nz = -t.nmax; nz <= t.nmax; ++nz) {
+        if (!(nx != 0 && ny != 0 && nz != 0)) continue;
+        double dx = nx * h.L1[0] + ny * h.L2[0], dy = nx * h.L1[1] + ny * h.L2[1], dz = nz * h.L;
+        double rr = dx * dx + dy * dy + dz * dz, rd = sqrt(rr);
+        double C1 = 0.75 / rd + 0.5 / (rd * rr);
+        double C2 =
+            4.0 * t.alpha7 * rr * rr + 3.0 * t.alpha3 * rr - 20.0 * t.alpha5 * rr - 4.5 * a + 14.0 * t.alpha3 + a / rr;
+        double C3 = 0.75 / rd - 1.5 / (rd * rr);
+        double C4 = -4.0 * t.alpha7 * rr * rr - 3.0 * t.alpha3 * rr + 16.0 * t.alpha5 * rr + 1.5 * a -
+                    2.0 * t.alpha3 - 3.0 * a / rr;
+        double term1 = C1 * erfc(a * rd) + C2 * exp(-t.alpha2 * rr) / rtpi;
+        double term2 = C3 * erfc(a * rd) + C4 * exp(-t.alpha2 * rr) / rtpi;
+        t.dtr[0] += term1 + term2 * dx * dx / rr;
+        t.dtr[1] += term1 + term2 * dy * dy / rr;
+        t.dtr[2] += term1 + term2 * dy * dy / rr;
+      }
+  return t;
+}
+
 int smdlt_tea_sample(smdlt_ctx* c, double* beta, double* C, double* eps) {
   if (!c || !beta || !C) return SMDLT_E_ARG;
   if (!c->have_pos) return SMDLT_E_STATE;
@@ -1882,55 +1941,11 @@ int smdlt_tea_sample(smdlt_ctx* c, double* beta, double* C, double* eps) {
   for (int r = rb; r < re; ++r) {
     const HostLattice& h = c->lat[r];
     const RepDev& R = c->rh[r];
-    TeaRep t{};
-    const bool first_half = h.treal <= h.tp / 2;
-    t.rce = first_half ? h.L2yp : 2.0 * h.L2yp;
-    t.nmax = first_half ? 1 : 2;
-    t.alpha = M_HI / t.rce;
-    t.alpha2 = t.alpha * t.alpha;
-    t.alpha3 = t.alpha2 * t.alpha;
-    t.alpha5 = t.alpha3 * t.alpha2;
-    t.alpha7 = t.alpha5 * t.alpha2;
-    t.L1[0] = h.L1[0];
-    t.L1[1] = h.L1[1];
-    t.L2[0] = h.L2[0];
-    t.L2[1] = h.L2[1];
-    t.L = h.L;
+    TeaRep t = tea_rep_for(h, M_HI, tab);
+    if (t.nk > TEA_MAXK) return SMDLT_E_GEOMETRY;
     t.slot0 = R.slot0;
     t.n = R.n;
     t.bead0 = (int)brep.size();
-    t.k0 = (int)(tab.size() / 12);
-    t.nk = tea_kspace(h, M_HI, t.rce, h.L * h.L * h.L, tab);
-    if (t.nk > TEA_MAXK) return SMDLT_E_GEOMETRY;
-    // self block (S/main26.c:1097-1130): selfconst, reciprocal diagonal, image sum
-    // over images with all three indices nonzero (z entry with dy*dy), no rce test
-    const double a = t.alpha, rtpi = sqrt(M_PI);
-    double sc = 1 - 6 / rtpi * a + 40 / (3 * rtpi) * t.alpha3;
-    t.dtr[0] = t.dtr[1] = t.dtr[2] = sc;
-    for (int q = 0; q < t.nk; ++q) {
-      const double* m = &tab[12 * (size_t)(t.k0 + q) + 3];
-      t.dtr[0] += m[0];
-      t.dtr[1] += m[4];
-      t.dtr[2] += m[8];
-    }
-    for (int nx = -t.nmax; nx <= t.nmax; ++nx)
-      for (int ny = -t.nmax; ny <= t.nmax; ++ny)
-        for (int nz = -t.nmax; nz <= t.nmax; ++nz) {
-          if (!(nx != 0 && ny != 0 && nz != 0)) continue;
-          double dx = nx * h.L1[0] + ny * h.L2[0], dy = nx * h.L1[1] + ny * h.L2[1], dz = nz * h.L;
-          double rr = dx * dx + dy * dy + dz * dz, rd = sqrt(rr);
-          double C1 = 0.75 / rd + 0.5 / (rd * rr);
-          double C2 = 4.0 * t.alpha7 * rr * rr + 3.0 * t.alpha3 * rr - 20.0 * t.alpha5 * rr - 4.5 * a +
-                      14.0 * t.alpha3 + a / rr;
-          double C3 = 0.75 / rd - 1.5 / (rd * rr);
-          double C4 = -4.0 * t.alpha7 * rr * rr - 3.0 * t.alpha3 * rr + 16.0 * t.alpha5 * rr + 1.5 * a -
-                      2.0 * t.alpha3 - 3.0 * a / rr;
-          double term1 = C1 * erfc(a * rd) + C2 * exp(-t.alpha2 * rr) / rtpi;
-          double term2 = C3 * erfc(a * rd) + C4 * exp(-t.alpha2 * rr) / rtpi;
-          t.dtr[0] += term1 + term2 * dx * dx / rr;
-          t.dtr[1] += term1 + term2 * dy * dy / rr;
-          t.dtr[2] += term1 + term2 * dy * dy / rr;
-        }
     for (int i = 0; i < R.n; ++i) brep.push_back(r - rb);
     tr.push_back(t);
   }
@@ -1975,6 +1990,70 @@ int smdlt_tea_sample(smdlt_ctx* c, double* beta, double* C, double* eps) {
     off += 3 * (size_t)R.n;
   }
   return 0;
+}
+
+int smdlt_far_table_dims(smdlt_ctx* c, int replica, double bin_size, int dims[3], double sizes[3]) {
+  if (!c || !dims || replica < 0 || replica >= c->nrep || bin_size <= 0) return SMDLT_E_ARG;
+  if (c->P[replica].flow_type == SMDLT_FLOW_EQUILIBRIUM || c->rh[replica].edot == 0.0) return SMDLT_E_ARG;
+  double bs[3];
+  table_geometry(c, replica, bin_size, dims, bs);
+  if (sizes)
+    for (int d = 0; d < 3; ++d) sizes[d] = bs[d];
+  return 0;
+}
+
+int smdlt_far_table(smdlt_ctx* c, int replica, int nt_avg, double bin_size, double* table) {
+  int nb[3];
+  double bs[3];
+  int st = smdlt_far_table_dims(c, replica, bin_size, nb, bs);
+  if (st) return st;
+  if (!table || nt_avg <= 0) return SMDLT_E_ARG;
+  HCK(hipSetDevice(c->device));
+  const smdlt_params& p = c->P[replica];
+  HostLattice h;
+  h.init(p.flow_type, p.L, c->rh[replica].edot, p.dt, p.pbc_compat);
+  std::vector<TeaRep> ph;
+  std::vector<double> tab;
+  for (int nt = 0; nt < nt_avg; ++nt) {  // t = nt*tp/nt_avg in the reference's int arithmetic
+    h.update((uint64_t)((long long)nt * h.tp / nt_avg));
+    ph.push_back(tea_rep_for(h, 6.0, tab));
+  }
+  const size_t nbins = (size_t)nb[0] * nb[1] * nb[2], n = 9 * nbins * nt_avg;
+  TeaRep* d_ph = nullptr;
+  double *d_tab = nullptr, *d_out = nullptr;
+  st = alloc_dev(&d_ph, ph.size());
+  st |= alloc_dev(&d_tab, tab.size());
+  st |= alloc_dev(&d_out, n);
+  if (!st) {
+    HCK(hipMemcpy(d_ph, ph.data(), sizeof(TeaRep) * ph.size(), hipMemcpyHostToDevice));
+    if (!tab.empty()) HCK(hipMemcpy(d_tab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice));
+    launch_far_table(c->stream, d_ph, nt_avg, d_tab, nb, bs, d_out);
+    HCK(hipGetLastError());
+    HCK(hipMemcpyAsync(table, d_out, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    HCK(hipStreamSynchronize(c->stream));
+  }
+  for (void* q : {(void*)d_ph, (void*)d_tab, (void*)d_out})
+    if (q) hipFree(q);
+  return st ? SMDLT_E_NOMEM : 0;
+}
+
+int smdlt_write_far_table(const char* path, const double* table, int nt_avg, const int dims[3]) {
+  if (!path || !table || !dims || nt_avg <= 0) return SMDLT_E_ARG;
+  FILE* f = fopen(path, "w");
+  if (!f) return SMDLT_E_ARG;
+  const size_t nbins = (size_t)dims[0] * dims[1] * dims[2];
+  for (int nt = 0; nt < nt_avg; ++nt)
+    for (int i = 0; i < dims[0]; ++i)
+      for (int j = 0; j < dims[1]; ++j)
+        for (int k = 0; k < dims[2]; ++k) {
+          const double* D = table + 9 * (nt * nbins + ((size_t)i * dims[1] + j) * dims[2] + k);
+          fprintf(f, "%d %d %d 1\n", i, j, k);
+          for (int a = 0; a < 3; ++a) {
+            for (int b = 0; b < 3; ++b) fprintf(f, "%.6e ", D[3 * a + b]);
+            fprintf(f, "\n");
+          }
+        }
+  return fclose(f) == 0 ? 0 : SMDLT_E_ARG;
 }
 
 // One timestep for the active replica range; noise_dev: device noise (nullptr =

@@ -333,6 +333,8 @@ struct TeaRep {
 };
 void launch_tea_rows(hipStream_t s, const TeaRep* reps, const int* bead_rep, int nbeads, const double* rx,
                      const double* ry, const double* rz, const double* ktab, double* eps_out, double* rowsum_out);
+void launch_far_table(hipStream_t s, const TeaRep* phases, int nphase, const double* ktab, const int nb[3],
+                      const double bs[3], double* table);
 
 void launch_build_clusters(hipStream_t st, const RepDev* reps, const int* slot_rep, int cl_begin, int ncl,
                            int max_words, const int* cl_first, const int* cl_cnt, const int* slot_cl,

@@ -33,6 +33,38 @@ __device__ __forceinline__ void ref_real_coeffs(double r, double rr, const TeaRe
   t2 = C3 * ec + C4 * g;
 }
 
+// The reference's Ewald pair block Dt for the raw difference (dxo, dyo, dzo):
+// every image within rce with ref_real_coeffs, then the k-sum with cos(k.r) of
+// the raw difference (S/main26.c:1150-1219, S/gridHI.c:148-244).  kt holds 12
+// doubles per wave vector (rkx, rky, rkz, M2[9]).
+__device__ __forceinline__ void tea_pair_block(const TeaRep& R, const double* kt, double dxo, double dyo, double dzo,
+                                               double Dt[9]) {
+#pragma unroll
+  for (int q = 0; q < 9; ++q) Dt[q] = 0.0;
+  for (int nx = -R.nmax; nx <= R.nmax; ++nx)
+    for (int ny = -R.nmax; ny <= R.nmax; ++ny)
+      for (int nz = -R.nmax; nz <= R.nmax; ++nz) {
+        const double dx = dxo + nx * R.L1[0] + ny * R.L2[0], dy = dyo + nx * R.L1[1] + ny * R.L2[1],
+                     dz = dzo + nz * R.L;
+        const double rr = dx * dx + dy * dy + dz * dz, rd = sqrt(rr);
+        if (rd < R.rce) {
+          double t1, t2;
+          ref_real_coeffs(rd, rr, R, t1, t2);
+          const double d[3] = {dx, dy, dz};
+#pragma unroll
+          for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) Dt[3 * a + b] += (a == b ? t1 : 0.0) + t2 * d[a] * d[b] / rr;
+        }
+      }
+  for (int q = 0; q < R.nk; ++q) {
+    const double* k = kt + 12 * q;
+    const double c = cos(k[0] * dxo + k[1] * dyo + k[2] * dzo);
+#pragma unroll
+    for (int a = 0; a < 9; ++a) Dt[a] += k[3 + a] * c;
+  }
+}
+
 __global__ __launch_bounds__(TEA_THREADS) void k_tea_rows(const TeaRep* __restrict__ reps, const int* __restrict__ bead_rep,
                                                           const double* __restrict__ rx, const double* __restrict__ ry,
                                                           const double* __restrict__ rz, const double* __restrict__ ktab,
@@ -50,32 +82,8 @@ __global__ __launch_bounds__(TEA_THREADS) void k_tea_rows(const TeaRep* __restri
   for (int jj = t; jj < R.n; jj += TEA_THREADS) {
     const int j = R.slot0 + jj;
     if (j == i) continue;
-    const double dxo = rx[j] - xi, dyo = ry[j] - yi, dzo = rz[j] - zi;
     double Dt[9];
-#pragma unroll
-    for (int q = 0; q < 9; ++q) Dt[q] = 0.0;
-    for (int nx = -R.nmax; nx <= R.nmax; ++nx)
-      for (int ny = -R.nmax; ny <= R.nmax; ++ny)
-        for (int nz = -R.nmax; nz <= R.nmax; ++nz) {
-          const double dx = dxo + nx * R.L1[0] + ny * R.L2[0], dy = dyo + nx * R.L1[1] + ny * R.L2[1],
-                       dz = dzo + nz * R.L;
-          const double rr = dx * dx + dy * dy + dz * dz, rd = sqrt(rr);
-          if (rd < R.rce) {
-            double t1, t2;
-            ref_real_coeffs(rd, rr, R, t1, t2);
-            const double d[3] = {dx, dy, dz};
-#pragma unroll
-            for (int a = 0; a < 3; ++a)
-#pragma unroll
-              for (int b = 0; b < 3; ++b) Dt[3 * a + b] += (a == b ? t1 : 0.0) + t2 * d[a] * d[b] / rr;
-          }
-        }
-    for (int q = 0; q < R.nk; ++q) {
-      const double* k = kt + 12 * q;
-      const double c = cos(k[0] * dxo + k[1] * dyo + k[2] * dzo);
-#pragma unroll
-      for (int a = 0; a < 9; ++a) Dt[a] += k[3 + a] * c;
-    }
+    tea_pair_block(R, kt, rx[j] - xi, ry[j] - yi, rz[j] - zi, Dt);
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       double rs = 0.0;
@@ -114,4 +122,41 @@ void launch_tea_rows(hipStream_t st, const TeaRep* reps, const int* bead_rep, in
                      const double* ry, const double* rz, const double* ktab, double* eps_out, double* rowsum_out) {
   if (nbeads <= 0) return;
   k_tea_rows<<<nbeads, TEA_THREADS, 0, st>>>(reps, bead_rep, rx, ry, rz, ktab, eps_out, rowsum_out);
+}
+
+// Far-field table of the reference's gridHI (S/gridHI.c:3-244, X/gridHI.c): the
+// pair block at every bin displacement of the table geometry, one thread per
+// bin, one table per strain phase (blockIdx.y); the origin bin takes the
+// reference's placeholder displacement (0.6, 0.6, 0.6).  The k table is read
+// from global memory (it can exceed LDS at M_HI = 6); every thread of a
+// wavefront reads the same entry.
+__global__ __launch_bounds__(256) void k_far_table(const TeaRep* __restrict__ phases, const double* __restrict__ ktab,
+                                                   int nbx, int nby, int nbz, double bsx, double bsy, double bsz,
+                                                   double* __restrict__ table) {
+  const TeaRep& R = phases[blockIdx.y];
+  const int nbins = nbx * nby * nbz;
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbins) return;
+  const int nhx = (nbx - 1) / 2, nhy = (nby - 1) / 2, nhz = (nbz - 1) / 2;
+  const int origin = nhx * nby * nbz + nhy * nbz + nhz;
+  double dr[3] = {0.6, 0.6, 0.6};
+  if (b != origin) {
+    const int bx = b / (nby * nbz), rem = b - bx * nby * nbz, by = rem / nbz, bz = rem - by * nbz;
+    dr[0] = (bx - nhx) * bsx;
+    dr[1] = (by - nhy) * bsy;
+    dr[2] = (bz - nhz) * bsz;
+  }
+  double Dt[9];
+  tea_pair_block(R, ktab + 12 * (size_t)R.k0, dr[0], dr[1], dr[2], Dt);
+  double* out = table + (size_t)blockIdx.y * 9 * nbins + 9 * (size_t)b;
+#pragma unroll
+  for (int a = 0; a < 9; ++a) out[a] = Dt[a];
+}
+
+void launch_far_table(hipStream_t st, const TeaRep* phases, int nphase, const double* ktab, const int nb[3],
+                      const double bs[3], double* table) {
+  const int nbins = nb[0] * nb[1] * nb[2];
+  if (nbins <= 0 || nphase <= 0) return;
+  dim3 grid((nbins + 255) / 256, nphase);
+  k_far_table<<<grid, 256, 0, st>>>(phases, ktab, nb[0], nb[1], nb[2], bs[0], bs[1], bs[2], table);
 }

@@ -85,6 +85,9 @@ def lib():
         L.smdlt_lanczos_status.argtypes = [C.c_void_p, _ip, _ip, _dp]
         L.smdlt_set_tea.argtypes = [C.c_void_p, _dp, _dp]
         L.smdlt_tea_sample.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.smdlt_far_table_dims.argtypes = [C.c_void_p, C.c_int, C.c_double, _ip, _dp]
+        L.smdlt_far_table.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, _dp]
+        L.smdlt_write_far_table.argtypes = [C.c_char_p, _dp, C.c_int, _ip]
         L.smdlt_set_profiling.argtypes = [C.c_void_p, C.c_int]
         L.smdlt_stage_times.argtypes = [C.c_void_p, _dp, _ip, C.c_int]
         L.smdlt_stage_name.restype = C.c_char_p
@@ -274,6 +277,17 @@ class Context:
              "tea_sample")
         return beta, Cv, eps
 
+    def far_table(self, replica=0, nt_avg=8, bin_size=2.0):
+        """smdlt_far_table: (table[nt_avg, nbins, 3, 3], dims, bin sizes) of gridHI."""
+        dims = np.zeros(3, np.int32)
+        sizes = np.zeros(3)
+        _chk(lib().smdlt_far_table_dims(self.h, replica, bin_size, _ptr(dims, C.c_int), _ptr(sizes, C.c_double)),
+             "far_table_dims")
+        nb = int(np.prod(dims))
+        table = np.zeros(nt_avg * 9 * nb)
+        _chk(lib().smdlt_far_table(self.h, replica, nt_avg, bin_size, _ptr(table, C.c_double)), "far_table")
+        return table.reshape(nt_avg, nb, 3, 3), tuple(int(d) for d in dims), sizes
+
     def set_tea(self, beta, Cv):
         beta = np.ascontiguousarray(np.broadcast_to(beta, (self.nrep,)), np.float64)
         Cv = np.ascontiguousarray(Cv, np.float64).reshape(-1)
@@ -384,6 +398,15 @@ class Context:
         e = np.zeros(self.nrep)
         _chk(lib().smdlt_mc_energy(self.h, _ptr(e, C.c_double)), "mc_energy")
         return e
+
+
+def write_far_table(path, table, dims):
+    """smdlt_write_far_table: the reference's mm/G text format."""
+    table = np.ascontiguousarray(table, np.float64).reshape(-1)
+    nt = table.size // (9 * int(np.prod(dims)))
+    d = np.ascontiguousarray(dims, np.int32)
+    _chk(lib().smdlt_write_far_table(str(path).encode(), _ptr(table, C.c_double), nt, _ptr(d, C.c_int)),
+         "write_far_table")
 
 
 def mlp_init(seed=0):

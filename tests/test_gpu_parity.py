@@ -319,6 +319,32 @@ def test_tea_sample_batched_equals_single_contexts():
     big.close()
 
 
+@pytest.mark.parametrize("flow", [1, 2])
+def test_far_table_matches_gridhi(flow, tmp_path):
+    """gridHI (S/gridHI.c:3-278): every phase and bin against the restatement, and
+    the mm/G text written by smdlt_write_far_table reads back in the reference's
+    layout (S/Initialization.c:497-520)."""
+    L, nt_avg = 14.0, 4
+    ctx, p = ctx_for(flow, L, 10, 2, prec=S.PREC_FP64, edot=0.05, dt=1e-3)
+    tab, dims, sizes = ctx.far_table(0, nt_avg, 2.0)
+    lat = O.lattice(flow, L, 0.05, 1e-3, 1, 0)
+    b = O.bins(lat, 2.0)
+    assert dims == (b.nbx, b.nby, b.nbz)
+    assert np.allclose(sizes, [b.bin_size_x, b.bin_size_y, b.bin_size_z], rtol=0, atol=0)
+    ref = O.far_table(lat, nt_avg, b, L ** 3)
+    assert np.abs(tab - ref).max() < 1e-11 * np.abs(ref).max()
+    path = tmp_path / "G.txt"
+    S.write_far_table(path, tab, dims)
+    lines = path.read_text().splitlines()
+    nb = int(np.prod(dims))
+    assert len(lines) == 4 * nt_avg * nb
+    k = 4 * (nt_avg - 1) * nb + 4 * 5  # phase nt_avg-1, bin 5
+    i, j, kz = 5 // (dims[1] * dims[2]), (5 // dims[2]) % dims[1], 5 % dims[2]
+    assert lines[k] == f"{i} {j} {kz} 1"
+    back = np.array([[float(x) for x in lines[k + 1 + a].split()] for a in range(3)])
+    assert np.allclose(back, tab[nt_avg - 1, 5], rtol=1e-6, atol=1e-12)
+
+
 # ----------------------------------------------------------------- timestep
 @pytest.mark.parametrize("flow,mode", [(1, S.NOISE_LANCZOS), (2, S.NOISE_LANCZOS), (1, S.NOISE_FREE_DRAINING),
                                        (2, S.NOISE_FREE_DRAINING), (1, S.NOISE_TEA), (2, S.NOISE_TEA)])

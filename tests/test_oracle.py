@@ -342,6 +342,55 @@ def test_ewald_bin_pair_sums_match_exact_ewald(flow, t):
         assert np.abs(rs - rs_dense).max() / rs_dense.max() < tol
 
 
+@pytest.mark.parametrize("flow", [1, 2])
+def test_far_table_equals_converged_ewald_pair_blocks(flow):
+    # gridHI's split (M_HI = 6) is converged, so away from overlap every table entry
+    # is the exact Ewald pair block of the displacement at the lattice of its
+    # strain phase (t = nt*tp/nt_avg), where the reference's image range holds every
+    # image within rce.  Shear: all phases, both halves of the period.  KRBC: the
+    # first half; later phases are so skewed that the reference's contiguous
+    # ky/kz ranges drop in-cut wave vectors (1e-8..4e-5), which the restatement keeps.
+    L, nt_avg = 12.0, 4
+    lat = O.lattice(flow, L, 0.05, 1e-3, 1, 0)
+    b = O.bins(lat, 2.0)
+    tab = O.far_table(lat, nt_avg, b, L ** 3)
+    nb = b.nbx * b.nby * b.nbz
+    assert tab.shape == (nt_avg, nb, 3, 3)
+    idx = np.arange(nb)
+    bx, rem = idx // (b.nby * b.nbz), idx % (b.nby * b.nbz)
+    dr = np.stack([(bx - b.nhx) * b.bin_size_x, (rem // b.nbz - b.nhy) * b.bin_size_y,
+                   (rem % b.nbz - b.nhz) * b.bin_size_z], 1)
+    origin = b.nhx * b.nby * b.nbz + b.nhy * b.nbz + b.nhz
+    dr[origin] = 0.6
+    checked = 0
+    g = np.arange(-6, 7)
+    ns = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    for nt in range(nt_avg if flow == 1 else 2):
+        latn = O.lattice(flow, L, 0.05, 1e-3, 1, nt * lat.tp // nt_avg)
+        c, s_ = latn.ct, latn.st
+        Rm = np.array([[c, -s_, 0], [s_, c, 0], [0, 0, 1.0]]) if flow == 2 else np.eye(3)  # cell frame -> lab
+        cell = O.cell_of(latn)
+        cellt = (cell.a1x, cell.a2x, cell.a2y, cell.Lz)
+        first = latn.treal <= latn.tp // 2
+        rce, nmax = (latn.L2yp, 1) if first else (2 * latn.L2yp, 2)
+        shifts = ns[:, :1] * np.array([latn.L1[0], latn.L1[1], 0]) + ns[:, 1:2] * np.array(
+            [latn.L2[0], latn.L2[1], 0]) + ns[:, 2:] * np.array([0, 0, L])
+        for i in range(0, nb, 3):
+            d = dr[i] @ Rm  # the displacement in the cell frame
+            if np.linalg.norm(P.min_image(d, cellt)) < 2.2:
+                continue
+            # only where the reference's image range holds every image within rce (on
+            # skewed KRBC phases and for displacements beyond the box it does not)
+            inside = np.linalg.norm(dr[i] + shifts, axis=1) < rce
+            if np.abs(ns[inside]).max(initial=0) > nmax:
+                continue
+            Me = O.ewald_dense(cell, np.array([[0.0, 0, 0], d]), 0.5)
+            ref = Rm @ Me[0:3, 3:6] @ Rm.T
+            assert np.abs(tab[nt, i] - ref).max() < 1e-11 * max(1.0, np.abs(ref).max())
+            checked += 1
+    assert checked > 100
+
+
 def test_catea_beta_is_the_small_root_and_dilute_limit():
     # beta solves X b^2 - 2b + 1 = 0 (X = 3N(e^2 - e), e = eps/(9N^2 - 3N)) with the
     # smaller root, and tends to 1/2 as the coupling vanishes; C decreases with rowsum
