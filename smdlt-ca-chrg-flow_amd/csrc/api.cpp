@@ -645,8 +645,10 @@ static void rpy_real_host(double r, double alpha, double& t1, double& t2) {
   t2 = C3 * ec + C4 * g;
 }
 
-// Cubic coefficients per interval [2 + i/RTAB_INV, 2 + (i+1)/RTAB_INV) of t1(r) and
-// t2(r)/r^2, interpolating at the 4 Chebyshev nodes of the interval (FP64 solve).
+// Cubic coefficients per interval [R0 + i/RTAB_INV, R0 + (i+1)/RTAB_INV) of t1(r)
+// and t2(r)/r^2, interpolating at the 4 Chebyshev nodes of the interval (FP64
+// solve).  Below r = 2 the exact RPY overlap form replaces the far form, written
+// with erf so the 1/r^3 terms cancel analytically; r = 2 is an interval knot.
 static void build_rtab(double alpha, int ntab, float* out) {
   const double h = 1.0 / RTAB_INV;
   double tn[4], V[4][4];
@@ -674,8 +676,19 @@ static void build_rtab(double alpha, int ntab, float* out) {
   for (int i = 0; i < ntab; ++i) {
     double f1[4], f2[4];
     for (int k = 0; k < 4; ++k) {
-      double r = 2.0 + (i + tn[k]) * h, t1, t2;
-      rpy_real_host(r, alpha, t1, t2);
+      double r = (double)RTAB_R0 + (i + tn[k]) * h, t1, t2;
+      if (r >= 2.0) {
+        rpy_real_host(r, alpha, t1, t2);
+      } else {
+        const double rr = r * r, ir = 1.0 / r, ir3 = ir / rr;
+        const double a2 = alpha * alpha, a3 = a2 * alpha, a5 = a3 * a2, a7 = a5 * a2;
+        const double ef = erf(alpha * r), g = exp(-a2 * rr) / sqrt(M_PI);
+        const double C1 = 0.75 * ir + 0.5 * ir3, C3 = 0.75 * ir - 1.5 * ir3;
+        const double C2 = 4 * a7 * rr * rr + 3 * a3 * rr - 20 * a5 * rr - 4.5 * alpha + 14 * a3 + alpha / rr;
+        const double C4 = -4 * a7 * rr * rr - 3 * a3 * rr + 16 * a5 * rr + 1.5 * alpha - 2 * a3 - 3 * alpha / rr;
+        t1 = (1.0 - 0.28125 * r) - C1 * ef + C2 * g;
+        t2 = 0.09375 * r - C3 * ef + C4 * g;
+      }
       f1[k] = t1;
       f2[k] = t2 / (r * r);
     }
@@ -986,7 +999,7 @@ int smdlt_create(const smdlt_params* params, int n_replicas, int device, smdlt_c
       int tot = 0;
       for (int r = 0; r < n_replicas; ++r) {
         RepDev& R = c->rh[r];
-        R.ntab = std::max(1, (int)ceil((sqrt(R.rc2) - 2.0) * RTAB_INV) + 1);
+        R.ntab = std::max(1, (int)ceil((sqrt(R.rc2) - (double)RTAB_R0) * RTAB_INV) + 1);
         R.tab0 = tot;
         tot += 2 * R.ntab;
       }

@@ -14,12 +14,12 @@
 #define FFT_TILE_Y 1024
 #endif
 #ifndef FFT_TILE_Z
-#define FFT_TILE_Z 1024
+#define FFT_TILE_Z 512
 #endif
 #define FFT_MAXL 64
 // fused x-y pass: one workgroup of XY_THREADS per (replica, component, z plane)
 // when the plane's half spectrum K0*K1/2 fits XY_ELEMS complex values in LDS
-#define XY_THREADS 512
+#define XY_THREADS 1024
 #ifndef XY_ELEMS
 #define XY_ELEMS 8192
 #endif
@@ -309,8 +309,9 @@ struct MobArgs {
   // the scaled input vector (per product), 16-byte aligned for one load per bead
   float4* p32;
   float4* v32;
-  // mixed precision: per replica, for r in [2, rc) on intervals of 1/RTAB_INV,
-  // cubic coefficients (in t = frac) of t1(r) and of t2(r)/r^2: rtab[2i], rtab[2i+1]
+  // mixed precision: per replica, for r in [RTAB_R0, rc) on intervals of 1/RTAB_INV,
+  // cubic coefficients (in t = frac) of t1(r) and of t2(r)/r^2 with the RPY overlap
+  // correction below r = 2 (r = 2 is a knot): rtab[2i], rtab[2i+1]
   const float4* rtab;
   int phase;                // bit 0: bead preparation and input scaling; bit 1: spread .. final
   // mixed precision real space by cluster pairs: cluster c holds beads cl_first[c] ..
@@ -320,6 +321,21 @@ struct MobArgs {
   int capc, cl_begin, ncl;
 };
 #define RTAB_INV 16
+#define RTAB_R0 1.0f  // pairs closer than this take the FP32 closed form
+
+// Overlap-corrected real-space coefficients in FP32 for r < RTAB_R0, written with
+// erf so that the 1/r^3 terms of the Ewald and RPY far forms cancel analytically:
+// t1 = (1 - 9r/32) - C1 erf(alpha r) + C2 g, t2 = 3r/32 - C3 erf(alpha r) + C4 g.
+__device__ __forceinline__ void rpy_real_near_f32(float r, float rr, float alpha, float& t1, float& t2) {
+  const float ir = 1.f / r, ir3 = ir / rr;
+  const float a2 = alpha * alpha, a3 = a2 * alpha, a5 = a3 * a2, a7 = a5 * a2;
+  const float ef = erff(alpha * r), g = expf(-a2 * rr) * 0.56418958354775628695f;
+  const float C1 = 0.75f * ir + 0.5f * ir3, C3 = 0.75f * ir - 1.5f * ir3;
+  const float C2 = 4.f * a7 * rr * rr + 3.f * a3 * rr - 20.f * a5 * rr - 4.5f * alpha + 14.f * a3 + alpha / rr;
+  const float C4 = -4.f * a7 * rr * rr - 3.f * a3 * rr + 16.f * a5 * rr + 1.5f * alpha - 2.f * a3 - 3.f * alpha / rr;
+  t1 = (1.f - 0.28125f * r) - C1 * ef + C2 * g;
+  t2 = 0.09375f * r - C3 * ef + C4 * g;
+}
 template <typename T> void launch_mobility(hipStream_t s, const MobArgs& a);
 
 // One replica's parameters for the TEA sample (ewaldBin): lattice vectors as the

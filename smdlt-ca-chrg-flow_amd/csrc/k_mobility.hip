@@ -714,11 +714,11 @@ __global__ __launch_bounds__(256) void k_mob_final(MobArgs a) {
         dy -= n1 * rb1 + n2 * rb3;
         dz -= Lz * rintf(dz * iLz);
         const float rr = dx * dx + dy * dy + dz * dz;
-        if (rr < 4.f) {
+        if (rr < RTAB_R0 * RTAB_R0) {
           overlap = true;  // evaluated below in FP64
         } else if (rr < rcf) {
           // t1 and t2/r^2 from the replica's cubic table (FP64-fitted, ~1e-7 relative)
-          const float x = (sqrtf(rr) - 2.f) * (float)RTAB_INV;
+          const float x = (sqrtf(rr) - RTAB_R0) * (float)RTAB_INV;
           const int it = min((int)x, R.ntab - 1);
           const float t = x - (float)it;
           const float4 ca = a.rtab[R.tab0 + 2 * it], cb = a.rtab[R.tab0 + 2 * it + 1];
@@ -731,7 +731,7 @@ __global__ __launch_bounds__(256) void k_mob_final(MobArgs a) {
         }
       }
     }
-    // overlapping pairs (r < 2, rare in a physical configuration): the overlap
+    // pairs closer than RTAB_R0 (rare in a physical configuration): the overlap
     // correction cancels 1/r^3 terms, so they are evaluated in FP64 from the FP64
     // positions, in a second pass only by the wavefronts that met one
     if (__ballot(overlap)) {
@@ -743,7 +743,7 @@ __global__ __launch_bounds__(256) void k_mob_final(MobArgs a) {
         fx -= n1 * rb0 + n2 * rb2;
         fy -= n1 * rb1 + n2 * rb3;
         fz -= Lz * rintf(fz * iLz);
-        if (!(fx * fx + fy * fy + fz * fz < 4.f)) continue;  // the same test as the first pass
+        if (!(fx * fx + fy * fy + fz * fz < RTAB_R0 * RTAB_R0)) continue;  // the same test as the first pass
         min_image_reduced(R, ex, ey, ez);
         double r2 = ex * ex + ey * ey + ez * ez, rd = sqrt(r2), t1, t2;
         rpy_real<double>(rd, R.alpha, t1, t2);
@@ -797,7 +797,9 @@ __global__ __launch_bounds__(256) void k_mob_final(MobArgs a) {
 // against bead jj of the listed cluster J, the wavefronts taking J in turn.  The
 // partner data of one J is 8 positions and 8 vectors shared by the wave, so list
 // and gather traffic fall by ~the cluster size against a bead-pair list.
-// Overlapping pairs (r < 2) are evaluated in FP64 in a second pass.  Sums are
+// Coefficients come from the replica's FP64-fitted table, which includes the RPY
+// overlap correction down to RTAB_R0; closer pairs take the FP32 closed form in
+// erf, so no FP64 path (and none of its registers) is needed here.  Sums are
 // compensated per lane and combined in FP64 in a fixed order.
 #define CL_WAVES 8
 template <int P>
@@ -833,8 +835,8 @@ __global__ __launch_bounds__(64 * CL_WAVES) void k_mob_cl(MobArgs a) {
   const float rd0 = (float)R.rd[0], rd1 = (float)R.rd[1], rd2 = (float)R.rd[2], rd3 = (float)R.rd[3];
   const float rb0 = (float)R.rb[0], rb1 = (float)R.rb[1], rb2 = (float)R.rb[2], rb3 = (float)R.rb[3];
   const float Lz = (float)R.L, iLz = (float)R.invL, rcf = (float)R.rc2;
+  const float alpha = (float)R.alpha;
   CompSum<T> acc[3];
-  bool overlap = false;
   const int m = a.nclp[I];
   const int* row = a.lcl + (size_t)I * a.capc;
   for (int k0 = w; k0 < m; k0 += CL_WAVES * PAIR_UNROLL) {
@@ -862,45 +864,24 @@ __global__ __launch_bounds__(64 * CL_WAVES) void k_mob_cl(MobArgs a) {
       dy -= n1 * rb1 + n2 * rb3;
       dz -= Lz * rintf(dz * iLz);
       const float rr = dx * dx + dy * dy + dz * dz;
-      if (!ok[u]) continue;
-      if (rr < 4.f) {
-        overlap = true;
-      } else if (rr < rcf) {
-        const float x = (sqrtf(rr) - 2.f) * (float)RTAB_INV;
+      if (!ok[u] || !(rr < rcf)) continue;
+      float t1, b2;
+      if (rr < RTAB_R0 * RTAB_R0) {  // closer than the table: closed form (rare)
+        float t2;
+        rpy_real_near_f32(sqrtf(rr), rr, alpha, t1, t2);
+        b2 = t2 / rr;
+      } else {
+        const float x = (sqrtf(rr) - RTAB_R0) * (float)RTAB_INV;
         const int it = min((int)x, R.ntab - 1);
         const float t = x - (float)it;
         const float4 ca = a.rtab[R.tab0 + 2 * it], cb = a.rtab[R.tab0 + 2 * it + 1];
-        const float t1 = ca.x + t * (ca.y + t * (ca.z + t * ca.w));
-        const float b2 = cb.x + t * (cb.y + t * (cb.z + t * cb.w));
-        const float dv = (dx * vj[u].x + dy * vj[u].y + dz * vj[u].z) * b2;
-        acc[0].add(t1 * vj[u].x + dv * dx);
-        acc[1].add(t1 * vj[u].y + dv * dy);
-        acc[2].add(t1 * vj[u].z + dv * dz);
+        t1 = ca.x + t * (ca.y + t * (ca.z + t * ca.w));
+        b2 = cb.x + t * (cb.y + t * (cb.z + t * cb.w));
       }
-    }
-  }
-  // overlapping pairs in FP64, only by wavefronts that met one
-  if (__ballot(overlap)) {
-    const double xi = a.rx[si], yi = a.ry[si], zi = a.rz[si];
-    for (int k = w; k < m; k += CL_WAVES) {
-      const int e = row[k], fj = e >> 4, cj = e & 15;
-      if (!(ivalid && jj < cj && !(fj == first && jj == ii))) continue;
-      const int j = fj + jj;
-      float fx = a.p32[j].x - pi.x, fy = a.p32[j].y - pi.y, fz = a.p32[j].z - pi.z;
-      const float n1 = rintf(rd0 * fx + rd1 * fy), n2 = rintf(rd2 * fx + rd3 * fy);
-      fx -= n1 * rb0 + n2 * rb2;
-      fy -= n1 * rb1 + n2 * rb3;
-      fz -= Lz * rintf(fz * iLz);
-      if (!(fx * fx + fy * fy + fz * fz < 4.f)) continue;  // the same test as the first pass
-      double ex = a.rx[j] - xi, ey = a.ry[j] - yi, ez = a.rz[j] - zi;
-      min_image_reduced(R, ex, ey, ez);
-      double r2 = ex * ex + ey * ey + ez * ez, rd = sqrt(r2), t1, t2;
-      rpy_real<double>(rd, R.alpha, t1, t2);
-      double vx = vv[3 * j] * sc, vy = vv[3 * j + 1] * sc, vz = vv[3 * j + 2] * sc;
-      double dv = (ex * vx + ey * vy + ez * vz) * t2 / r2;
-      acc[0].add((T)(t1 * vx + dv * ex));
-      acc[1].add((T)(t1 * vy + dv * ey));
-      acc[2].add((T)(t1 * vz + dv * ez));
+      const float dv = (dx * vj[u].x + dy * vj[u].y + dz * vj[u].z) * b2;
+      acc[0].add(t1 * vj[u].x + dv * dx);
+      acc[1].add(t1 * vj[u].y + dv * dy);
+      acc[2].add(t1 * vj[u].z + dv * dz);
     }
   }
   double u0 = acc[0].value(), u1 = acc[1].value(), u2 = acc[2].value();
