@@ -255,10 +255,12 @@ __global__ __launch_bounds__(512) void k_spread_gather(MobArgs a) {
   __shared__ int sb[GATHER_CHUNK * 3];
   __shared__ T sw[GATHER_CHUNK * 3 * P];
   __shared__ T sv[GATHER_CHUNK * 3];
+  __shared__ unsigned char plist[BRICK][GATHER_CHUNK];  // per z plane: staged beads covering it
   const BrickTask bt = a.bricks[blockIdx.x];
   const RepDev& R = a.reps[bt.rep];
   if (!R.active || (a.done && a.done[bt.rep])) return;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wz = tid >> 6;
+  const unsigned long long below = (1ull << lane) - 1ull;
   const int K0 = R.K[0], K1 = R.K[1], K2 = R.K[2];
   const int g0 = bt.b0 * BRICK + (tid & 7), g1 = bt.b1 * BRICK + ((tid >> 3) & 7), g2 = bt.b2 * BRICK + (tid >> 6);
   const double* vin = a.vout ? a.vout : a.v;
@@ -300,12 +302,32 @@ __global__ __launch_bounds__(512) void k_spread_gather(MobArgs a) {
       for (int j = 0; j < P; ++j) sw[k * P + j] = bw[(size_t)BW_STRIDE * s + d * MAX_ORDER + j];
     }
     __syncthreads();
-    for (int k = 0; k < m; ++k) {
+    // each wavefront is one z plane of the brick: list, in staging order, the beads
+    // whose stencil covers the plane (ballot compaction), then add those only; the
+    // order of the additions per point is the staging order either way
+    int nz = 0;
+    for (int k0c = 0; k0c < m; k0c += 64) {
+      const int k = k0c + lane;
+      bool cov = false;
+      if (k < m) {
+        int d2 = sb[3 * k + 2] - g2;
+        d2 += d2 < 0 ? K2 : 0;
+        cov = d2 < P;
+      }
+      const unsigned long long bal = __ballot(cov);
+      if (cov) plist[wz][nz + __popcll(bal & below)] = (unsigned char)k;
+      nz += __popcll(bal);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int q = 0; q < nz; ++q) {
+      const int k = plist[wz][q];
       int d0 = sb[3 * k] - g0, d1 = sb[3 * k + 1] - g1, d2 = sb[3 * k + 2] - g2;
       d0 += d0 < 0 ? K0 : 0;
       d1 += d1 < 0 ? K1 : 0;
       d2 += d2 < 0 ? K2 : 0;
-      if (d0 < P && d1 < P && d2 < P) {
+      if (d0 < P && d1 < P) {
         T w = sw[(3 * k) * P + d0] * sw[(3 * k + 1) * P + d1] * sw[(3 * k + 2) * P + d2];
         acc0 += w * sv[3 * k];
         acc1 += w * sv[3 * k + 1];
