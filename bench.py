@@ -198,11 +198,35 @@ def batched_leg(nrep, steps, warmup, plain_reps):
     batched = ntot * steps / el
     print(f"[bench] batched {batched:.4g} bead-steps/s ({1e3 * el / steps:.1f} ms/step, Lanczos iters "
           f"{int(iters.min())}..{int(iters.max())})", file=sys.stderr, flush=True)
+    # the roofline on the working set of GBs: per-kernel device times of one eager
+    # profiled step, algorithmic bytes summed over the replicas, PMC traffic beside
+    ctx.set_profiling(True)
+    ctx.run(warmup + steps, 1, use_graph=False)
+    st = ctx.stage_times()
+    ctx.set_profiling(False)
+    cfgs = [ctx.mobility_config(r) for r in range(nrep)]
+    G = sum(int(np.prod(c["grid"])) for c in cfgs)
+    ent = sum(c["cluster_pairs"] or c["list_entries"] for c in cfgs)
+    kb = kernel_bytes(ntot, G, ent, cfgs[0]["order"])
+    kern = {k: v for k, v in st.items() if k in kb and v[1] > 0}
+    dom = max(kern, key=lambda k: kern[k][0])
+    avg_ms = kern[dom][0] / kern[dom][1]
+    ach = kb[dom] / (avg_ms * 1e-3) / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "hbm_traffic_pmc.json")
+    if os.path.exists(tf):
+        with open(tf) as fh:
+            traffic = json.load(fh).get("c4", {}).get(dom)
+    roof = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": ach / HBM_PEAK_GBS, "traffic": traffic, "bytes_per_launch": kb[dom], "avg_launch_ms": avg_ms,
+            "grid_points": G, "list_entries": ent,
+            "kernel_ms_avg": {k: v[0] / v[1] for k, v in kern.items()}}
     ctx.close()
     out = {"workload": f"{nrep} uneven replicas (chains U[50,1000] x U{{1..20}}, volume fraction 0.02, "
                        f"shear or planar extension, Wi U[1,100]), no charges",
            "replicas": nrep, "beads": ntot, "steps": steps, "ms_per_step": 1e3 * el / steps, "value": batched,
-           "lanczos_iters_last_step": [int(iters.min()), int(iters.max())], "lanczos_converged": code == 0}
+           "lanczos_iters_last_step": [int(iters.min()), int(iters.max())], "lanczos_converged": code == 0,
+           "roofline": roof}
     if plain_reps <= 0:
         return out
     # plain path on a sample of the replicas
