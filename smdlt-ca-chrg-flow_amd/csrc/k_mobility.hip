@@ -17,7 +17,7 @@ template <typename T> struct Prec {
 };
 
 // -------------------------------------------------------------------- FFT core
-// One in-place Stockham stage of radix R (2, 3, 4 or 5) over nl lines of length
+// One in-place Stockham stage of radix R (2, 3, 4, 5 or 8) over nl lines of length
 // K in LDS: every thread first loads its butterflies into registers, then
 // (after a barrier) writes them to their autosorted positions.  tw holds
 // exp(-2 pi i t / K), t < K, for this line length.
@@ -93,6 +93,36 @@ __device__ __forceinline__ void fft_stage(C* sh, int K, int pitch, int nl, int N
         dst[0] = cadd(x0, sm);
         dst[Ns] = cadd(m, rt);
         dst[2 * Ns] = csub(m, rt);
+      } else if (R == 8) {
+        // radix-2 decimation in frequency of the 8-point DFT: even outputs from
+        // b_n = x_n + x_{n+4}, odd outputs from d_n = (x_n - x_{n+4}) W^n, W = e^{-+2 pi i/8}
+        const T h = (T)0.70710678118654752440;
+        C x0 = v[q][0], x1 = v[q][1], x2 = v[q][R > 2 ? 2 : 0], x3 = v[q][R > 3 ? 3 : 0];
+        C x4 = v[q][R > 4 ? 4 : 0], x5 = v[q][R > 5 ? 5 : 0], x6 = v[q][R > 6 ? 6 : 0], x7 = v[q][R > 7 ? 7 : 0];
+        C b0 = cadd(x0, x4), b1 = cadd(x1, x5), b2 = cadd(x2, x6), b3 = cadd(x3, x7);
+        C d0 = csub(x0, x4), d1 = csub(x1, x5), d2 = crot(csub(x2, x6), dir), d3 = csub(x3, x7);
+        C t1, t3;  // d1 W, d3 W^3 with W = (1 -+ i)/sqrt2
+        if (dir < 0) {
+          t1.x = h * (d1.x + d1.y);
+          t1.y = h * (d1.y - d1.x);
+          t3.x = h * (d3.y - d3.x);
+          t3.y = -h * (d3.x + d3.y);
+        } else {
+          t1.x = h * (d1.x - d1.y);
+          t1.y = h * (d1.x + d1.y);
+          t3.x = -h * (d3.x + d3.y);
+          t3.y = h * (d3.x - d3.y);
+        }
+        C e0 = cadd(b0, b2), e1 = cadd(b1, b3), f0 = csub(b0, b2), f1 = crot(csub(b1, b3), dir);
+        C g0 = cadd(d0, d2), g1 = cadd(t1, t3), k0 = csub(d0, d2), k1 = crot(csub(t1, t3), dir);
+        dst[0] = cadd(e0, e1);
+        dst[4 * Ns] = csub(e0, e1);
+        dst[2 * Ns] = cadd(f0, f1);
+        dst[6 * Ns] = csub(f0, f1);
+        dst[Ns] = cadd(g0, g1);
+        dst[5 * Ns] = csub(g0, g1);
+        dst[3 * Ns] = cadd(k0, k1);
+        dst[7 * Ns] = csub(k0, k1);
       } else if (R == 5) {
         const T c1 = (T)0.30901699437494742410, c2 = (T)-0.80901699437494742410;
         const T s1 = (T)0.95105651629515357212, s2 = (T)0.58778525229247312917;
@@ -114,12 +144,18 @@ __device__ __forceinline__ void fft_stage(C* sh, int K, int pitch, int nl, int N
   __syncthreads();
 }
 
-// FFT of nl lines of length K = 2^a 3^b 5^c (nl*K <= TILE): radix-4
-// stages, a radix-2 stage for an odd power of two, then radix 3 and radix 5.
+// FFT of nl lines of length K = 2^a 3^b 5^c (nl*K <= TILE): radix-8 stages,
+// one radix-4 or radix-2 stage for the rest of the power of two, then radix 3
+// and radix 5.
 template <typename C, int TILE, int TH = FFT_THREADS>
 __device__ __forceinline__ void fft_lines(C* sh, int K, int pitch, int nl, int dir, const C* __restrict__ tw) {
   int Ns = 1, rest = K;
-  while (rest % 4 == 0) {
+  while (rest % 8 == 0) {
+    fft_stage<C, 8, (TILE + 8 * TH - 1) / (8 * TH)>(sh, K, pitch, nl, Ns, dir, tw);
+    Ns *= 8;
+    rest /= 8;
+  }
+  if (rest % 4 == 0) {
     fft_stage<C, 4, (TILE + 4 * TH - 1) / (4 * TH)>(sh, K, pitch, nl, Ns, dir, tw);
     Ns *= 4;
     rest /= 4;
