@@ -292,6 +292,7 @@ struct smdlt_ctx {
   float4* rtab = nullptr;     // mixed precision: real-space coefficient tables
   // bead clusters and the cluster-pair list (mixed precision)
   int *cl_first = nullptr, *cl_cnt = nullptr, *slot_cl = nullptr, *lcl = nullptr, *nclp = nullptr;
+  double* adot_cl = nullptr;  // per-cluster partial of the Lanczos alpha, written by the cluster pass
   int ncl_total = 0, capc = 0, cl_words = 0;
   float* mlp_w = nullptr;     // BF16-rounded proposal network weights (FP32 storage)
   float* mh_feats = nullptr;  // features of each MH attempt of the last sweep [nq_total][16]
@@ -1035,6 +1036,7 @@ int smdlt_create(const smdlt_params* params, int n_replicas, int device, smdlt_c
       c->capc = std::min(maxncl, c->cap_rpy / 2 + 64);
       if (4LL * c->cl_words > 65536 || alloc_dev(&c->cl_first, c->ncl_total) || alloc_dev(&c->cl_cnt, c->ncl_total) ||
           alloc_dev(&c->slot_cl, c->nst) || alloc_dev(&c->nclp, c->ncl_total) ||
+          alloc_dev(&c->adot_cl, c->ncl_total) ||
           alloc_dev(&c->lcl, (size_t)c->capc * c->ncl_total)) {
         smdlt_destroy(c);
         return SMDLT_E_NOMEM;
@@ -1061,7 +1063,7 @@ void smdlt_destroy(smdlt_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   destroy_graphs(c);
   if (c->h_flags) hipHostFree(c->h_flags);
-  void* ptrs[] = {c->tea_beta, c->tea_C, c->tea_diag, c->cl_first, c->cl_cnt, c->slot_cl, c->lcl, c->nclp, c->rtab, c->p32, c->v32, c->mh_feats, c->mh_log, c->mlp_ws, c->mlp_x, c->mlp_y, c->mlp_part, c->mlp_out, c->saved, c->d_reps, c->d_slot_rep, c->d_seeds, c->rx, c->ry, c->rz, c->dxv, c->dyv, c->dzv, c->dsp, c->f,
+  void* ptrs[] = {c->adot_cl, c->tea_beta, c->tea_C, c->tea_diag, c->cl_first, c->cl_cnt, c->slot_cl, c->lcl, c->nclp, c->rtab, c->p32, c->v32, c->mh_feats, c->mh_log, c->mlp_ws, c->mlp_x, c->mlp_y, c->mlp_part, c->mlp_out, c->saved, c->d_reps, c->d_slot_rep, c->d_seeds, c->rx, c->ry, c->rz, c->dxv, c->dyv, c->dzv, c->dsp, c->f,
                   c->z, c->mf, c->bz, c->tmp, c->tmp2, c->nrpy, c->lrpy, c->nlj, c->llj, c->overflow, c->d_err,
                   c->cell_of, c->cell_cnt, c->cell_start, c->cell_fill, c->cell_beads, c->cell_tmp, c->rep_flags, c->out2, c->aff,
                   c->partial2, c->grid, c->bsp, c->tw, c->tx, c->ty, c->tz, c->txy, c->bricks, c->bbase, c->brick_cnt, c->brick_start,
@@ -1503,7 +1505,7 @@ static MobArgs mob_args(smdlt_ctx* c, int rb, int re, const double* v, double* u
   m.beta = beta;
   m.vout = vout;
   m.u = u;
-  m.partial = partial;
+  m.adot = partial;
   m.done = done;
   m.grid = c->grid;
   m.grid_begin = a.grid0;
@@ -1627,10 +1629,13 @@ static void lz_iteration(smdlt_ctx* c, const double* z, int j, double tol, int m
   const double* vin = (j == 1) ? z : c->W;
   double* vj = c->V + (size_t)(j - 1) * n3;
   const double* vjm1 = (j > 1) ? c->V + (size_t)(j - 2) * n3 : nullptr;
-  mobility_internal(c, vin, c->W, c->bdiv, vj, nullptr, c->done);
+  // the cluster pass (mixed precision) also forms alpha's partials, one per cluster
+  const bool fused_dot = c->lcl != nullptr && c->prec == SMDLT_PREC_MIXED;
+  mobility_internal(c, vin, c->W, c->bdiv, vj, fused_dot ? c->adot_cl : nullptr, c->done);
   StageScope sc(c, ST_LANCZOS);
-  lz_wave_dot(c->stream, c->d_reps, c->d_slot_rep, s0, ns, vj, c->W, c->done, c->partial2);
-  lz_orth(c->stream, c->d_reps, c->d_slot_rep, s0, ns, c->W, vj, vjm1, c->scal, j, c->done, c->partial2, c->partial);
+  if (!fused_dot) lz_wave_dot(c->stream, c->d_reps, c->d_slot_rep, s0, ns, vj, c->W, c->done, c->partial2);
+  lz_orth(c->stream, c->d_reps, c->d_slot_rep, s0, ns, c->W, vj, vjm1, c->scal, j, c->done,
+          fused_dot ? nullptr : c->partial2, fused_dot ? c->adot_cl : nullptr, c->partial);
   lz_tri(c->stream, c->d_reps, rb, re - rb, c->partial, c->scal, j, tol, maxit, c->done, c->bdiv, c->iters);
 }
 

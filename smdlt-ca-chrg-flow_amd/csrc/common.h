@@ -287,7 +287,7 @@ struct MobArgs {
   const double* beta;       // per-replica divisor (Lanczos normalisation) or nullptr
   double* vout;             // where the (scaled) input is stored for the real-space pass
   double* u;                // output M v (AoS 3N)
-  double* partial;          // per-wave partial v.u (nullptr: skip)
+  double* adot;             // cluster path: per-cluster partial v.u of the product, the Lanczos alpha (nullptr: skip)
   const int* done;          // per-replica skip flag (nullptr: none)
   void* grid;               // grid buffer (T)
   long long grid_begin, grid_len;
@@ -364,7 +364,7 @@ void lz_rep_reduce(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, 
                    const int* done, double* out, int stride, int off, int take_sqrt, double* copy_to);
 void lz_orth(hipStream_t st, const RepDev* reps, const int* slot_rep, int slot_begin, int nslots, double* w,
              const double* vj, const double* vjm1, double* scal, int j, const int* done, const double* partial_alpha,
-             double* partial);
+             const double* cluster_alpha, double* partial);
 void lz_tri(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, const double* partial, double* scal, int j,
             double tol, int maxit, int* done, double* bdiv, int* iters);
 void lz_combine(hipStream_t st, const RepDev* reps, const int* slot_rep, int slot_begin, int nslots, const double* V,

@@ -53,13 +53,29 @@ __global__ void k_rep_reduce(const RepDev* __restrict__ reps, int rep_begin, con
   }
 }
 
+// This lane's share (entries lane, lane + 64, ...) of the n partials p, four
+// loads in flight; the grouping depends on n only, so the sum is deterministic.
+__device__ __forceinline__ double lane_sum4(const double* __restrict__ p, int n, int lane) {
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  int q = lane;
+  for (; q + 192 < n; q += 256) {
+    a0 += p[q];
+    a1 += p[q + 64];
+    a2 += p[q + 128];
+    a3 += p[q + 192];
+  }
+  for (; q < n; q += 64) a0 += p[q];
+  return (a0 + a1) + (a2 + a3);
+}
+
 // w <- w - a_j v_j - b_{j-1} v_{j-1}; partial |w|^2 out.  a_j = v_j.w is
 // reduced by every wavefront from its replica's per-wave partials in wave order
 // (a wavefront's 64 slots always belong to one replica), so all wavefronts of a
 // replica use the identical value.
 __global__ void k_lz_orth(const RepDev* __restrict__ reps, const int* __restrict__ slot_rep, int slot_begin,
                           int nslots, double* w, const double* vj, const double* vjm1, double* scal, int j,
-                          const int* done, const double* partial_alpha, double* partial) {
+                          const int* done, const double* partial_alpha, const double* cluster_alpha,
+                          double* partial) {
   const int lane = threadIdx.x & 63;
   int s = slot_begin + blockIdx.x * blockDim.x + threadIdx.x;
   int s_w = s - lane;  // first slot of this wavefront
@@ -68,8 +84,10 @@ __global__ void k_lz_orth(const RepDev* __restrict__ reps, const int* __restrict
   if (r < 0) return;
   const RepDev& R = reps[r];
   if (!R.active || (done && done[r])) return;
-  double a = 0.0;
-  for (int q = lane; q < R.nwaves; q += 64) a += partial_alpha[R.wave0 + q];
+  // alpha from the partials the cluster pass wrote (one per bead cluster) or the
+  // per-wavefront ones of k_wave_dot
+  double a = cluster_alpha ? lane_sum4(cluster_alpha + R.cl0, R.ncl, lane)
+                           : lane_sum4(partial_alpha + R.wave0, R.nwaves, lane);
   a = warp_sum(a);
   double* sc = scal + (size_t)r * SCAL_STRIDE;
   if (lane == 0 && s_w == R.slot0) sc[S_AL + j - 1] = a;
@@ -289,10 +307,10 @@ void lz_rep_reduce(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, 
 }
 void lz_orth(hipStream_t st, const RepDev* reps, const int* slot_rep, int slot_begin, int nslots, double* w,
              const double* vj, const double* vjm1, double* scal, int j, const int* done, const double* partial_alpha,
-             double* partial) {
+             const double* cluster_alpha, double* partial) {
   if (nslots > 0)
     k_lz_orth<<<(nslots + 255) / 256, 256, 0, st>>>(reps, slot_rep, slot_begin, nslots, w, vj, vjm1, scal, j, done,
-                                                     partial_alpha, partial);
+                                                     partial_alpha, cluster_alpha, partial);
 }
 void lz_tri(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, const double* partial, double* scal, int j,
             double tol, int maxit, int* done, double* bdiv, int* iters) {

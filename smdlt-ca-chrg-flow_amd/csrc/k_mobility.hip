@@ -849,8 +849,8 @@ __global__ __launch_bounds__(256) void k_mob_final(MobArgs a) {
   }
 }
 
-// Mixed precision, one workgroup (4 wavefronts) per bead cluster I (<= 8 beads):
-// the interpolation of each bead (one wavefront, a lane per stencil row), then
+// Mixed precision, one workgroup (CL_WAVES wavefronts) per bead cluster I (<= 8 beads):
+// the interpolation of each bead (one wavefront, a lane per stencil column), then
 // the real-space pairs by cluster pairs: lane 8*ii + jj evaluates bead ii of I
 // against bead jj of the listed cluster J, the wavefronts taking J in turn.  The
 // partner data of one J is 8 positions and 8 vectors shared by the wave, so list
@@ -859,7 +859,7 @@ __global__ __launch_bounds__(256) void k_mob_final(MobArgs a) {
 // overlap correction down to RTAB_R0; closer pairs take the FP32 closed form in
 // erf, so no FP64 path (and none of its registers) is needed here.  Sums are
 // compensated per lane and combined in FP64 in a fixed order.
-#define CL_WAVES 8
+#define CL_WAVES 4
 template <int P>
 __global__ __launch_bounds__(64 * CL_WAVES) void k_mob_cl(MobArgs a) {
   typedef float T;
@@ -954,11 +954,19 @@ __global__ __launch_bounds__(64 * CL_WAVES) void k_mob_cl(MobArgs a) {
     part[w][ii][2] = u2;
   }
   __syncthreads();
+  double pr = 0.0;  // this thread's term of v.u
   if (threadIdx.x < 3 * cnt) {
     const int b = threadIdx.x / 3, c = threadIdx.x - 3 * b, s = first + b;
     double t = uin[b][c];
     for (int q = 0; q < CL_WAVES; ++q) t += part[q][b][c];
-    a.u[3 * s + c] = t + R.selfconst * (vv[3 * s + c] * sc);
+    const double vi = vv[3 * s + c] * sc;
+    t += R.selfconst * vi;
+    a.u[3 * s + c] = t;
+    pr = vi * t;
+  }
+  if (a.adot && threadIdx.x < 64) {  // the cluster's share of the Lanczos alpha, (bead, component) order
+    pr = warp_sum(pr);
+    if (threadIdx.x == 0) a.adot[I] = pr;
   }
 }
 
