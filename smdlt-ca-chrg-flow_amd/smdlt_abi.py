@@ -400,6 +400,53 @@ class Context:
         return e
 
 
+class TeaAverages:
+    """CATEA's running averages for one replica (S/main26.c:1317-1381): per strain bin
+    floor(strain/strain_interval), capped at sbins, the sums of beta and of C by bead index
+    within a chain (i mod 3Nb), and the sample count; printTEA's decomp/DC text
+    (S/main26.c:1807-1826) and the per-bin coefficients updateHI uses (S/main26.c:760-764)."""
+
+    def __init__(self, nb, nc, sbins=20):
+        self.nb, self.nc, self.sbins = nb, nc, sbins
+        self.brun = np.zeros(sbins + 1)
+        self.crun = np.zeros((sbins + 1, 3 * nb))
+        self.count = np.zeros(sbins + 1, np.int64)
+
+    def bin_of(self, strain, strain_interval):
+        return min(int(np.floor(strain / strain_interval)), self.sbins)
+
+    def add(self, strain, strain_interval, beta, Cv):
+        """one sample: beta and C[3N] of this replica (smdlt_tea_sample)"""
+        b = self.bin_of(strain, strain_interval)
+        self.brun[b] += beta
+        self.crun[b] += np.asarray(Cv, np.float64).reshape(self.nc, 3 * self.nb).sum(0)
+        self.count[b] += 1
+        return b
+
+    def coefficients(self, b):
+        """(beta, C[3N]) of bin b as updateHI reads them: C of bead i is the bin's average for i mod Nb"""
+        if self.count[b] == 0:
+            raise ValueError(f"strain bin {b} has no samples")
+        beta = self.brun[b] / self.count[b]
+        cb = self.crun[b] / (self.nc * self.count[b])
+        return beta, np.tile(cb, self.nc)
+
+    def write_decomp(self, path, t):
+        """printTEA's decomp/DC layout: t, then per bin "bin count beta" and Nb rows "j Cx Cy Cz" """
+        with open(path, "w") as f:
+            f.write(f"{int(t)}\n")
+            for i in range(self.sbins + 1):
+                n = int(self.count[i])
+                if n > 0:
+                    f.write(f"{i} {n} {self.brun[i] / n:.12e}\n")
+                    c = self.crun[i] / (self.nc * n)
+                else:
+                    f.write(f"{i} 0 {0.0:.12e}\n")
+                    c = np.zeros(3 * self.nb)
+                for j in range(self.nb):
+                    f.write(f"{j} {c[3 * j]:.12e} {c[3 * j + 1]:.12e} {c[3 * j + 2]:.12e}\n")
+
+
 def write_far_table(path, table, dims):
     """smdlt_write_far_table: the reference's mm/G text format."""
     table = np.ascontiguousarray(table, np.float64).reshape(-1)
