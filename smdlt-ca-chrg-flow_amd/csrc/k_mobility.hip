@@ -17,7 +17,7 @@ template <typename T> struct Prec {
 };
 
 // -------------------------------------------------------------------- FFT core
-// One in-place Stockham stage of radix R (2, 3, 4, 5 or 8) over nl lines of length
+// One in-place Stockham stage of radix R (2, 3, 4, 5, 8 or 9) over nl lines of length
 // K in LDS: every thread first loads its butterflies into registers, then
 // (after a barrier) writes them to their autosorted positions.  tw holds
 // exp(-2 pi i t / K), t < K, for this line length.
@@ -38,6 +38,25 @@ template <typename C> __device__ __forceinline__ C crot(C d, int dir) {
     r.y = d.x;
   }
   return r;
+}
+
+template <typename C> __device__ __forceinline__ C make_c(decltype(C().x) x, decltype(C().x) y) {
+  C r;
+  r.x = x;
+  r.y = y;
+  return r;
+}
+// 3-point DFT: y0 = a + (b + c), y1,2 = a - (b + c)/2 -+ i sin(2 pi/3) (b - c) (forward)
+template <typename C>
+__device__ __forceinline__ void dft3(C x0, C x1, C x2, int dir, C& y0, C& y1, C& y2) {
+  typedef decltype(C().x) T;
+  const T s3 = (T)0.86602540378443864676;
+  C sm = cadd(x1, x2), df = csub(x1, x2);
+  C m = csub(x0, cscale(sm, (T)0.5));
+  C rt = cscale(crot(df, dir), s3);
+  y0 = cadd(x0, sm);
+  y1 = cadd(m, rt);
+  y2 = csub(m, rt);
 }
 
 template <typename C, int R, int MAXB>
@@ -123,6 +142,28 @@ __device__ __forceinline__ void fft_stage(C* sh, int K, int pitch, int nl, int N
         dst[5 * Ns] = csub(g0, g1);
         dst[3 * Ns] = cadd(k0, k1);
         dst[7 * Ns] = csub(k0, k1);
+      } else if constexpr (R == 9) {
+        // 3 x 3: DFT3 over n1 of x[3 n1 + n2], twiddles W^(n2 k1), DFT3 over n2;
+        // output k1 + 3 k2, W = e^{-+2 pi i/9}
+        C A[3][3];
+#pragma unroll
+        for (int n2 = 0; n2 < 3; ++n2) dft3<C>(v[q][n2], v[q][3 + n2], v[q][6 + n2], dir, A[n2][0], A[n2][1], A[n2][2]);
+        const T c1 = (T)0.76604444311897803520, s1 = (T)0.64278760968653932632;  // 2 pi/9
+        const T c2 = (T)0.17364817766693034885, s2 = (T)0.98480775301220805936;  // 4 pi/9
+        const T c4 = (T)-0.93969262078590838405, s4 = (T)0.34202014332566873304; // 8 pi/9
+        const T sg = dir < 0 ? (T)-1 : (T)1;
+        A[1][1] = cmul(A[1][1], make_c<C>(c1, sg * s1));
+        A[1][2] = cmul(A[1][2], make_c<C>(c2, sg * s2));
+        A[2][1] = cmul(A[2][1], make_c<C>(c2, sg * s2));
+        A[2][2] = cmul(A[2][2], make_c<C>(c4, sg * s4));
+#pragma unroll
+        for (int k1 = 0; k1 < 3; ++k1) {
+          C y0, y1, y2;
+          dft3<C>(A[0][k1], A[1][k1], A[2][k1], dir, y0, y1, y2);
+          dst[k1 * Ns] = y0;
+          dst[(k1 + 3) * Ns] = y1;
+          dst[(k1 + 6) * Ns] = y2;
+        }
       } else if (R == 5) {
         const T c1 = (T)0.30901699437494742410, c2 = (T)-0.80901699437494742410;
         const T s1 = (T)0.95105651629515357212, s2 = (T)0.58778525229247312917;
@@ -145,8 +186,8 @@ __device__ __forceinline__ void fft_stage(C* sh, int K, int pitch, int nl, int N
 }
 
 // FFT of nl lines of length K = 2^a 3^b 5^c (nl*K <= TILE): radix-8 stages,
-// one radix-4 or radix-2 stage for the rest of the power of two, then radix 3
-// and radix 5.
+// one radix-4 or radix-2 stage for the rest of the power of two, radix-9
+// stages and one radix-3 stage for the power of three, then radix 5.
 template <typename C, int TILE, int TH = FFT_THREADS>
 __device__ __forceinline__ void fft_lines(C* sh, int K, int pitch, int nl, int dir, const C* __restrict__ tw) {
   int Ns = 1, rest = K;
@@ -165,7 +206,12 @@ __device__ __forceinline__ void fft_lines(C* sh, int K, int pitch, int nl, int d
     Ns *= 2;
     rest /= 2;
   }
-  while (rest % 3 == 0) {
+  while (rest % 9 == 0) {
+    fft_stage<C, 9, (TILE + 9 * TH - 1) / (9 * TH)>(sh, K, pitch, nl, Ns, dir, tw);
+    Ns *= 9;
+    rest /= 9;
+  }
+  if (rest % 3 == 0) {
     fft_stage<C, 3, (TILE + 3 * TH - 1) / (3 * TH)>(sh, K, pitch, nl, Ns, dir, tw);
     Ns *= 3;
     rest /= 3;
