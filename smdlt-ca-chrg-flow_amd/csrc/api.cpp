@@ -855,6 +855,9 @@ int smdlt_create(const smdlt_params* params, int n_replicas, int device, smdlt_c
     smdlt_destroy(c);
     return SMDLT_E_NOMEM;
   }
+  // Lanczos scalar blocks start at zero (no previous solve: first square root at iteration 2)
+  HCK(hipMemset(c->scal, 0, sizeof(double) * SCAL_STRIDE * (size_t)n_replicas));
+  HCK(hipMemset(c->iters, 0, sizeof(int) * (size_t)n_replicas));
   // slot -> replica map, seeds
   std::vector<int> sr(c->nst, -1);
   std::vector<uint64_t> seeds(n_replicas);

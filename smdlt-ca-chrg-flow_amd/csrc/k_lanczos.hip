@@ -17,6 +17,7 @@
 #define S_B0 (4 * LZ_MAXIT)
 #define S_IT (4 * LZ_MAXIT + 1)
 #define S_ERR (4 * LZ_MAXIT + 2)
+#define S_JMIN (4 * LZ_MAXIT + 4)  // first iteration whose square root is evaluated
 
 // per-wave partial of x.y (y == nullptr: x.x)
 __global__ void k_wave_dot(const RepDev* __restrict__ reps, const int* __restrict__ slot_rep, int slot_begin,
@@ -164,6 +165,8 @@ __global__ __launch_bounds__(LZ_TRI_TH) void k_lz_tri(const RepDev* __restrict__
     }
     return;
   }
+  __syncthreads();
+  if (j < (int)sc[S_JMIN] && j < maxit && !brk) return;  // too early to stop: beta_j only
   const int n = j;
   for (int i = t; i < n; i += LZ_TRI_TH) {
     d[i] = sc[S_AL + i];
@@ -328,9 +331,18 @@ void lz_tea(hipStream_t st, const int* slot_rep, int slot_begin, int nslots, con
 }
 
 // zero the "previous check" length before a new solve
+// Start of a solve: no previous c, and the square root is first evaluated at half
+// the iteration count the replica's previous solve stopped at (at least 2): a
+// solve converges within a few percent of its predecessor, so the earlier
+// evaluations could not meet the stopping test.  One that converges sooner stops
+// at the first evaluated pair, a few iterations late.
 __global__ void k_lz_reset(double* scal, int rep_begin, int nrep) {
   int r = rep_begin + blockIdx.x * blockDim.x + threadIdx.x;
-  if (r < rep_begin + nrep) scal[(size_t)r * SCAL_STRIDE + S_ERR + 1] = 0.0;
+  if (r < rep_begin + nrep) {
+    double* sc = scal + (size_t)r * SCAL_STRIDE;
+    sc[S_ERR + 1] = 0.0;
+    sc[S_JMIN] = fmax(2.0, floor(0.5 * sc[S_IT]));
+  }
 }
 void lz_reset_checks(hipStream_t st, double* scal, int rep_begin, int nrep) {
   if (nrep > 0) k_lz_reset<<<(nrep + 63) / 64, 64, 0, st>>>(scal, rep_begin, nrep);
