@@ -167,7 +167,9 @@ int smdlt_mobility_apply(smdlt_ctx* ctx, const double* v, double* u);
 int smdlt_mobility_apply_device(smdlt_ctx* ctx, const double* d_v, double* d_u);
 
 /* Brownian displacement y = M^(1/2) z by Lanczos (replaces the TEA product of
- * S/main26.c:787-804).  iters[r], resid[r] per replica (may be NULL). */
+ * S/main26.c:787-804).  iters[r], resid[r] per replica (may be NULL).  The stop
+ * test |c_j - c_(j-1)| < tol |c_j| starts at iteration max(2, iters_prev / 2),
+ * with iters_prev the replica's previous solve (0 on a new context). */
 int smdlt_brownian(smdlt_ctx* ctx, const double* z, double* y, double tol, int maxit, int* iters,
                    double* resid);
 /* The reference's TEA product y_i = C_i (.) (M_ii z_i + beta sum_j M_ij z_j) with
