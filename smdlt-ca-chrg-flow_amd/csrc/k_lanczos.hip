@@ -115,19 +115,24 @@ __global__ void k_lz_orth(const RepDev* __restrict__ reps, const int* __restrict
 // the trapezoid rule with step h has relative error ~4 exp(-pi^2/h) for every
 // eigenvalue alike (poles of sech at +-i pi/2), and truncating at U beyond the
 // spectrum costs ~(2/pi) e^-U.  h = 0.5, U = 20 give ~1e-8, two orders below the
-// stopping tolerance.  The spectrum is bracketed by Gershgorin above and by a
-// 64-point Sturm multisection below.  Each node is one SPD tridiagonal solve
-// (T + sigma I) x = e1 (backward pivots, then the first column of the inverse
-// forwards) by one thread, so the ~90 nodes run side by side; the reciprocals are
-// v_rcp_f64 refined by two Newton steps.  The weighted columns are summed in
-// thread order, so the result does not depend on scheduling.
+// stopping tolerance.  The spectrum is bracketed by Gershgorin above; the
+// lower end of the node range is 1e-12 of that bound (condition numbers of the
+// mobility are far below 1e12), which takes 109 nodes.  Each node is one SPD
+// tridiagonal solve (T + sigma I) x = e1 by one thread: backward pivots, storing
+// the ratios x_(i+1)/x_i = -e_i / pivot_(i+1), then x forwards by those ratios.
+// The reciprocals are v_rcp_f64 refined by one Newton step (relative error
+// ~1e-15 per pivot).  Columns are laid out [i][thread] with a row stride of
+// LZ_TRI_TH + 1 doubles, so that the row sums (one thread per row, four
+// interleaved partial sums over the nodes in fixed order) do not all fall on
+// one LDS bank; the result does not depend on scheduling.
 #define LZ_QH 0.5
 #define LZ_QU 20.0
 #define LZ_TRI_TH 128
 
-__device__ __forceinline__ double rcp_refined(double x) {
-  double y = __builtin_amdgcn_rcp(x);
-  y = fma(fma(-x, y, 1.0), y, y);
+#define LZ_COLS (LZ_TRI_TH + 1)
+
+__device__ __forceinline__ double rcp_nr1(double x) {
+  const double y = __builtin_amdgcn_rcp(x);
   return fma(fma(-x, y, 1.0), y, y);
 }
 
@@ -141,8 +146,8 @@ __global__ __launch_bounds__(LZ_TRI_TH) void k_lz_tri(const RepDev* __restrict__
   const bool w0 = t < 64;
   double* sc = scal + (size_t)r * SCAL_STRIDE;
   __shared__ double d[LZ_MAXIT], e[LZ_MAXIT], e2[LZ_MAXIT], u[LZ_MAXIT];
-  __shared__ double col[LZ_MAXIT * LZ_TRI_TH];  // backward pivots, then weighted first columns: [i][thread]
-  __shared__ double bnd[2];
+  __shared__ double col[LZ_MAXIT * LZ_COLS];  // ratios, then weighted first columns: [i][thread]
+  __shared__ double bnd;
   __shared__ int brk;
   if (w0) {
     double v = 0.0;
@@ -174,61 +179,58 @@ __global__ __launch_bounds__(LZ_TRI_TH) void k_lz_tri(const RepDev* __restrict__
     e2[i] = e[i] * e[i];
   }
   __syncthreads();
-  if (w0) {
-    // Gershgorin upper bound of the spectrum
+  if (w0) {  // Gershgorin upper bound of the spectrum
     double hi = 0.0;
     for (int i = lane; i < n; i += 64) hi = fmax(hi, d[i] + fabs(e[i]) + (i > 0 ? fabs(e[i - 1]) : 0.0));
     hi = warp_max(hi);
-    // lower bound: largest of hi * 10^(-12 (l+1)/64) with no eigenvalue below it
-    const double x = hi * exp(-27.631021115928547 * (lane + 1) / 64.0);
-    int neg = 0;
-    double p = d[0] - x;
-    neg += p < 0.0;
-    for (int i = 1; i < n; ++i) {
-      if (p == 0.0) p = 1e-300;
-      p = d[i] - x - e2[i - 1] / p;
-      neg += p < 0.0;
-    }
-    unsigned long long zero = __ballot(neg == 0);
-    if (lane == 0) {
-      bnd[0] = zero ? hi * exp(-27.631021115928547 * (__ffsll((long long)zero)) / 64.0) : hi * 1e-12;
-      bnd[1] = hi;
-    }
+    if (lane == 0) bnd = hi;
   }
   __syncthreads();
-  const double slo = 0.5 * log(bnd[0]) - LZ_QU, shi = 0.5 * log(bnd[1]) + LZ_QU;
-  double h = LZ_QH;
-  int nodes = (int)ceil((shi - slo) / h) + 1;
-  if (nodes > LZ_TRI_TH) {  // a spectrum wider than e^48: keep one node per thread
-    nodes = LZ_TRI_TH;
-    h = (shi - slo) / (LZ_TRI_TH - 1);
-  }
+  const double hb = fmax(bnd, 1e-290);
+  const double slo = 0.5 * log(hb * 1e-12) - LZ_QU, shi = 0.5 * log(hb) + LZ_QU;
+  const int nodes = min(LZ_TRI_TH, (int)ceil((shi - slo) / LZ_QH) + 1);  // 109
   if (t < nodes) {
-    const double s = slo + t * h;
+    const double s = slo + t * LZ_QH;
     const double sig = exp(2.0 * s);
-    const double wk = h * 0.6366197723675814 * exp(s);
-    double rp = d[n - 1] + sig;
-    double ip = rcp_refined(rp);
-    col[(n - 1) * LZ_TRI_TH + t] = ip;
+    const double wk = LZ_QH * 0.6366197723675814 * exp(s);
+    double ip = rcp_nr1(d[n - 1] + sig);
     for (int i = n - 2; i >= 0; --i) {
-      rp = d[i] + sig - e2[i] * ip;
-      ip = rcp_refined(rp);
-      col[i * LZ_TRI_TH + t] = ip;
+      col[(i + 1) * LZ_COLS + t] = -e[i] * ip;
+      ip = rcp_nr1(fma(-e2[i], ip, d[i] + sig));
     }
     double xi = ip;  // (T + sig)^-1 e1, first entry
     col[t] = wk * xi;
-    for (int i = 0; i < n - 1; ++i) {
-      xi = -e[i] * xi * col[(i + 1) * LZ_TRI_TH + t];
-      col[(i + 1) * LZ_TRI_TH + t] = wk * xi;
+    int i = 1;
+    for (; i + 3 < n; i += 4) {  // ratios loaded ahead of the product chain
+      const double r0 = col[i * LZ_COLS + t], r1 = col[(i + 1) * LZ_COLS + t];
+      const double r2 = col[(i + 2) * LZ_COLS + t], r3 = col[(i + 3) * LZ_COLS + t];
+      xi *= r0;
+      col[i * LZ_COLS + t] = wk * xi;
+      xi *= r1;
+      col[(i + 1) * LZ_COLS + t] = wk * xi;
+      xi *= r2;
+      col[(i + 2) * LZ_COLS + t] = wk * xi;
+      xi *= r3;
+      col[(i + 3) * LZ_COLS + t] = wk * xi;
     }
-  } else {
-    for (int i = 0; i < n; ++i) col[i * LZ_TRI_TH + t] = 0.0;
+    for (; i < n; ++i) {
+      xi *= col[i * LZ_COLS + t];
+      col[i * LZ_COLS + t] = wk * xi;
+    }
   }
   __syncthreads();
   for (int i = t; i < n; i += LZ_TRI_TH) {
-    double a = 0.0;
-    for (int q = 0; q < LZ_TRI_TH; ++q) a += col[i * LZ_TRI_TH + q];
-    u[i] = a;
+    const double* row = col + i * LZ_COLS;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int q = 0;
+    for (; q + 3 < nodes; q += 4) {
+      a0 += row[q];
+      a1 += row[q + 1];
+      a2 += row[q + 2];
+      a3 += row[q + 3];
+    }
+    for (; q < nodes; ++q) a0 += row[q];
+    u[i] = (a0 + a1) + (a2 + a3);
   }
   __syncthreads();
   if (!w0) return;
