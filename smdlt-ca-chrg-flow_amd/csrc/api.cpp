@@ -1544,6 +1544,8 @@ static MobArgs mob_args(smdlt_ctx* c, int rb, int re, const double* v, double* u
   m.capc = c->capc;
   m.cl_begin = a.cl0;
   m.ncl = b.cl0 + b.ncl - a.cl0;
+  m.ntab_max = 0;
+  for (int r = rb; r < re; ++r) m.ntab_max = std::max(m.ntab_max, c->rh[r].ntab);
   return m;
 }
 

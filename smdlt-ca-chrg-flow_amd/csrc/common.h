@@ -319,7 +319,9 @@ struct MobArgs {
   // list, as (first slot << 4) | count
   const int *cl_first, *cl_cnt, *lcl, *nclp;
   int capc, cl_begin, ncl;
+  int ntab_max;             // largest coefficient table of the range (entries of 2 float4)
 };
+#define RTAB_LDS_MAX 1280   // tables up to this many entries (40 KB) are staged in LDS
 #define RTAB_INV 16
 #define RTAB_R0 1.0f  // pairs closer than this take the FP32 closed form
 

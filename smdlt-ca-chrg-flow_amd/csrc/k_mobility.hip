@@ -903,12 +903,16 @@ __global__ __launch_bounds__(256) void k_mob_final(MobArgs a) {
 // and gather traffic fall by ~the cluster size against a bead-pair list.
 // Coefficients come from the replica's FP64-fitted table, which includes the RPY
 // overlap correction down to RTAB_R0; closer pairs take the FP32 closed form in
-// erf, so no FP64 path (and none of its registers) is needed here.  Sums are
-// compensated per lane and combined in FP64 in a fixed order.
+// erf, so no FP64 path (and none of its registers) is needed here.  With TAB_LDS
+// the workgroup first copies the table to LDS (while its interpolation loads are
+// in flight): the per-lane lookups are random, and as LDS reads they no longer
+// queue behind one another in the vector memory path.  Sums are compensated per
+// lane and combined in FP64 in a fixed order.
 #define CL_WAVES 4
-template <int P>
+template <int P, bool TAB_LDS>
 __global__ __launch_bounds__(64 * CL_WAVES) void k_mob_cl(MobArgs a) {
   typedef float T;
+  extern __shared__ float4 stab[];  // 2 * ntab_max (TAB_LDS)
   const int I = a.cl_begin + xcd_block(blockIdx.x, gridDim.x);
   const int first = a.cl_first[I], cnt = a.cl_cnt[I];
   const int r = a.slot_rep[first];
@@ -919,6 +923,10 @@ __global__ __launch_bounds__(64 * CL_WAVES) void k_mob_cl(MobArgs a) {
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const double* vv = a.vout ? a.vout : a.v;
   const double sc = (a.beta && !a.vout) ? (a.beta[r] > 0.0 ? 1.0 / a.beta[r] : 0.0) : 1.0;
+  if constexpr (TAB_LDS) {
+    const float4* g = a.rtab + R.tab0;
+    for (int q = threadIdx.x; q < 2 * R.ntab; q += 64 * CL_WAVES) stab[q] = g[q];
+  }
   // reciprocal part, bead ii by wavefront ii % CL_WAVES
   for (int ii = w; ii < cnt; ii += CL_WAVES) {
     const int s = first + ii;
@@ -931,6 +939,7 @@ __global__ __launch_bounds__(64 * CL_WAVES) void k_mob_cl(MobArgs a) {
       uin[ii][2] = x2;
     }
   }
+  if constexpr (TAB_LDS) __syncthreads();
   // real-space part by cluster pairs
   const int ii = lane >> 3, jj = lane & 7;
   const bool ivalid = ii < cnt;
@@ -978,7 +987,14 @@ __global__ __launch_bounds__(64 * CL_WAVES) void k_mob_cl(MobArgs a) {
         const float x = (sqrtf(rr) - RTAB_R0) * (float)RTAB_INV;
         const int it = min((int)x, R.ntab - 1);
         const float t = x - (float)it;
-        const float4 ca = a.rtab[R.tab0 + 2 * it], cb = a.rtab[R.tab0 + 2 * it + 1];
+        float4 ca, cb;
+        if constexpr (TAB_LDS) {
+          ca = stab[2 * it];
+          cb = stab[2 * it + 1];
+        } else {
+          ca = a.rtab[R.tab0 + 2 * it];
+          cb = a.rtab[R.tab0 + 2 * it + 1];
+        }
         t1 = ca.x + t * (ca.y + t * (ca.z + t * ca.w));
         b2 = cb.x + t * (cb.y + t * (cb.z + t * cb.w));
       }
@@ -1050,7 +1066,10 @@ template <typename T, int P> static void launch_mob_p(hipStream_t st, const MobA
   if (a.ntx > 0) k_fft_x<T><<<a.ntx, FFT_THREADS, 0, st>>>(a, 1);
   mark(7);
   if (sizeof(T) == 4 && a.lcl) {
-    if (a.ncl > 0) k_mob_cl<P><<<a.ncl, 64 * CL_WAVES, 0, st>>>(a);
+    if (a.ncl > 0 && a.ntab_max <= RTAB_LDS_MAX)
+      k_mob_cl<P, true><<<a.ncl, 64 * CL_WAVES, 32 * a.ntab_max, st>>>(a);
+    else if (a.ncl > 0)
+      k_mob_cl<P, false><<<a.ncl, 64 * CL_WAVES, 0, st>>>(a);
   } else if (a.nslots > 0) {
     k_mob_final<T, P><<<(a.nslots + 3) / 4, 256, 0, st>>>(a);
   }
