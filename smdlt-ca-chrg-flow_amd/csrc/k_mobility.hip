@@ -328,16 +328,22 @@ __global__ void k_scale_v(MobArgs a) {
 // Spreading as a gather: one workgroup per 8x8x8 brick of one replica's grid,
 // one thread per grid point.  A bead with stencil base b touches points b-P+1..b,
 // so the beads reaching this brick live in it or in the next brick along each
-// axis (P <= 8); they are staged through LDS in chunks and every thread adds
-// the ones whose stencil covers its point.  No atomics: the grid needs no
-// zeroing and the result does not depend on scheduling.
+// axis (P <= 8); they are staged through LDS in chunks, with stencil bases
+// relative to the brick origin (in [0, K); grids are at least 16 long, so a
+// base below the point never reaches it through the periodic wrap), and every
+// thread adds the ones whose stencil covers its point.  Each wavefront is one z
+// plane (8 x 8 points) and lists, per 4 x 4 quadrant of its lanes, the staged
+// beads that reach the quadrant; a lane walks its quadrant's list, which holds
+// about half the beads that reach the plane.  A point adds exactly the beads
+// covering it, in staging order.  No atomics: the grid needs no zeroing and
+// the result does not depend on scheduling.
 #define GATHER_CHUNK 128
 template <typename T, int P>
 __global__ __launch_bounds__(512) void k_spread_gather(MobArgs a) {
   __shared__ int sb[GATHER_CHUNK * 3];
   __shared__ T sw[GATHER_CHUNK * 3 * P];
   __shared__ T sv[GATHER_CHUNK * 3];
-  __shared__ unsigned char plist[BRICK][GATHER_CHUNK];  // per z plane: staged beads covering it
+  __shared__ unsigned char plist[BRICK][4][GATHER_CHUNK];  // per z plane and quadrant: staged beads reaching it
   const BrickTask bt = a.bricks[blockIdx.x];
   const RepDev& R = a.reps[bt.rep];
   if (!R.active || (a.done && a.done[bt.rep])) return;
@@ -345,6 +351,8 @@ __global__ __launch_bounds__(512) void k_spread_gather(MobArgs a) {
   const unsigned long long below = (1ull << lane) - 1ull;
   const int K0 = R.K[0], K1 = R.K[1], K2 = R.K[2];
   const int g0 = bt.b0 * BRICK + (tid & 7), g1 = bt.b1 * BRICK + ((tid >> 3) & 7), g2 = bt.b2 * BRICK + (tid >> 6);
+  const int lo0 = bt.b0 * BRICK, lo1 = bt.b1 * BRICK, lo2 = bt.b2 * BRICK;
+  const int qd = ((lane >> 2) & 1) | (((lane >> 5) & 1) << 1);  // this lane's 4 x 4 quadrant
   const double* vin = a.vout ? a.vout : a.v;
   const T* bw = (const T*)a.bw;
   // the 8 source bricks (this one and its +1 neighbours): starts and counts in one step
@@ -373,7 +381,9 @@ __global__ __launch_bounds__(512) void k_spread_gather(MobArgs a) {
       int gi = k0 + kk, nbk = 0;
       while (gi >= noff[nbk + 1]) ++nbk;
       int s = a.brick_beads[nbeg[nbk] + gi - noff[nbk]];
-      sb[k] = a.bbase[3 * s + d];
+      int rel = a.bbase[3 * s + d] - (d == 0 ? lo0 : d == 1 ? lo1 : lo2);
+      rel += rel < 0 ? (d == 0 ? K0 : d == 1 ? K1 : K2) : 0;
+      sb[k] = rel;
       if (a.v32) {
         const float4 v4 = a.v32[s];
         sv[k] = (T)(d == 0 ? v4.x : d == 1 ? v4.y : v4.z);
@@ -384,32 +394,43 @@ __global__ __launch_bounds__(512) void k_spread_gather(MobArgs a) {
       for (int j = 0; j < P; ++j) sw[k * P + j] = bw[(size_t)BW_STRIDE * s + d * MAX_ORDER + j];
     }
     __syncthreads();
-    // each wavefront is one z plane of the brick: list, in staging order, the beads
-    // whose stencil covers the plane (ballot compaction), then add those only; the
-    // order of the additions per point is the staging order either way
-    int nz = 0;
+    // list, in staging order, the beads that reach each quadrant of this plane
+    // (ballot compaction): relative base b reaches points b-P+1..b, so a quadrant
+    // starting at 4q is reached when 4q <= b <= 4q + 3 + P - 1
+    int n0 = 0, n1 = 0, n2 = 0, n3 = 0;
     for (int k0c = 0; k0c < m; k0c += 64) {
       const int k = k0c + lane;
-      bool cov = false;
+      bool c0 = false, c1 = false, c2 = false, c3 = false;
       if (k < m) {
-        int d2 = sb[3 * k + 2] - g2;
-        d2 += d2 < 0 ? K2 : 0;
-        cov = d2 < P;
+        const int e0 = sb[3 * k], e1 = sb[3 * k + 1];
+        const bool cz = (unsigned)(sb[3 * k + 2] - wz) < (unsigned)P;
+        const bool x0 = e0 <= 2 + P, x1 = e0 >= 4 && e0 <= 6 + P;
+        const bool y0 = e1 <= 2 + P, y1 = e1 >= 4 && e1 <= 6 + P;
+        c0 = cz && x0 && y0;
+        c1 = cz && x1 && y0;
+        c2 = cz && x0 && y1;
+        c3 = cz && x1 && y1;
       }
-      const unsigned long long bal = __ballot(cov);
-      if (cov) plist[wz][nz + __popcll(bal & below)] = (unsigned char)k;
-      nz += __popcll(bal);
+      const unsigned long long b0 = __ballot(c0), b1 = __ballot(c1), b2 = __ballot(c2), b3 = __ballot(c3);
+      if (c0) plist[wz][0][n0 + __popcll(b0 & below)] = (unsigned char)k;
+      if (c1) plist[wz][1][n1 + __popcll(b1 & below)] = (unsigned char)k;
+      if (c2) plist[wz][2][n2 + __popcll(b2 & below)] = (unsigned char)k;
+      if (c3) plist[wz][3][n3 + __popcll(b3 & below)] = (unsigned char)k;
+      n0 += __popcll(b0);
+      n1 += __popcll(b1);
+      n2 += __popcll(b2);
+      n3 += __popcll(b3);
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    for (int q = 0; q < nz; ++q) {
-      const int k = plist[wz][q];
-      int d0 = sb[3 * k] - g0, d1 = sb[3 * k + 1] - g1, d2 = sb[3 * k + 2] - g2;
-      d0 += d0 < 0 ? K0 : 0;
-      d1 += d1 < 0 ? K1 : 0;
-      d2 += d2 < 0 ? K2 : 0;
-      if (d0 < P && d1 < P) {
+    const int mine = qd == 0 ? n0 : qd == 1 ? n1 : qd == 2 ? n2 : n3;
+    const int gr0 = tid & 7, gr1 = (tid >> 3) & 7;
+    const unsigned char* pl = plist[wz][qd];
+    for (int q = 0; q < mine; ++q) {
+      const int k = pl[q];
+      const int d0 = sb[3 * k] - gr0, d1 = sb[3 * k + 1] - gr1, d2 = sb[3 * k + 2] - wz;
+      if ((unsigned)d0 < (unsigned)P && (unsigned)d1 < (unsigned)P) {
         T w = sw[(3 * k) * P + d0] * sw[(3 * k + 1) * P + d1] * sw[(3 * k + 2) * P + d2];
         acc0 += w * sv[3 * k];
         acc1 += w * sv[3 * k + 1];

@@ -210,6 +210,18 @@ def test_mobility_split_invariance():
     assert rel(vals[0], vals[1]) < 2e-5
 
 
+@pytest.mark.parametrize("rc", [80.0, 85.0])
+def test_mobility_long_cutoff_tables_match_fp64(rc):
+    # the cluster pass stages coefficient tables of up to 1280 entries (rc <= 81) in LDS
+    # (rc = 80: the largest such table) and reads longer ones from global memory (rc = 85);
+    # the FP64 path at the same split evaluates the closed forms
+    mixed, lat, pos = mob_case(0, L=180.0, rc_real=rc)
+    fp64, _, _ = mob_case(0, L=180.0, rc_real=rc, prec=S.PREC_FP64)
+    assert mixed.mobility_config()["rc"] == rc
+    v = np.random.default_rng(13).normal(size=pos.size)
+    assert rel(mixed.mobility_apply(v), fp64.mobility_apply(v)) < 1e-5
+
+
 def test_mobility_symmetric():
     ctx, lat, pos = mob_case(1)
     rng = np.random.default_rng(6)
