@@ -80,6 +80,17 @@ template <> __device__ __forceinline__ float2 mk2<float2>(double x, double y) { 
 template <> __device__ __forceinline__ double2 mk2<double2>(double x, double y) { return make_double2(x, y); }
 
 // Compensated (TwoSum) accumulator for per-bead velocity sums.
+// n / d for n >= 0 with n * d < 2^32 by a multiply-high: with m = floor((2^32 - 1)
+// / d) + 1 = ceil(2^32 / d) (2^32 / d when d is a power of two),
+// floor(n m / 2^32) = floor(n / d) because n (m d - 2^32) < n d < 2^32.  Built
+// once from a wave-uniform divisor, it replaces the per-lane division sequence.
+struct FDiv {
+  unsigned m;
+  int d;
+  __device__ __forceinline__ explicit FDiv(int d_) : m(0xFFFFFFFFu / (unsigned)d_ + 1u), d(d_) {}
+  __device__ __forceinline__ int div(int n) const { return d == 1 ? n : (int)__umulhi((unsigned)n, m); }
+};
+
 template <typename T> struct CompSum {
   T s, c;
   __device__ __forceinline__ CompSum() : s(0), c(0) {}

@@ -65,12 +65,13 @@ __device__ __forceinline__ void fft_stage(C* sh, int K, int pitch, int nl, int N
   const int TH = blockDim.x, t = threadIdx.x;
   const int KR = K / R;
   const int nb = nl * KR;
+  const FDiv dKR(KR), dNs(Ns);
   C v[MAXB][R];
 #pragma unroll
   for (int q = 0; q < MAXB; ++q) {
     int b = t + q * TH;
     if (b < nb) {
-      int l = b / KR, j = b - l * KR;
+      int l = dKR.div(b), j = b - l * KR;
       const C* src = sh + l * pitch + j;
 #pragma unroll
       for (int r = 0; r < R; ++r) v[q][r] = src[r * KR];
@@ -82,8 +83,8 @@ __device__ __forceinline__ void fft_stage(C* sh, int K, int pitch, int nl, int N
   for (int q = 0; q < MAXB; ++q) {
     int b = t + q * TH;
     if (b < nb) {
-      int l = b / KR, j = b - l * KR;
-      int jm = j % Ns;
+      int l = dKR.div(b), j = b - l * KR;
+      int jm = j - dNs.div(j) * Ns;
 #pragma unroll
       for (int r = 1; r < R; ++r) {
         C w = tw[jm * r * tstep];
@@ -465,13 +466,14 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_x(MobArgs a, int dir) {
   const RepDev& R = a.reps[tk.rep];
   if (!R.active || (a.done && a.done[tk.rep])) return;
   const int K = R.K[0], Kh = K / 2, pitch = K + 1, nl = tk.nl;
+  const FDiv dK(K), dKh(Kh);
   const long long ncomp = (long long)R.K[0] * R.K[1] * R.K[2];
   T* g = (T*)a.grid + R.grid0 + tk.comp * ncomp;
   C* gc = (C*)g;
   stage_twiddles(tw, (const C*)a.tw + R.twoff[0], K);
   if (dir < 0) {
     for (int e = threadIdx.x; e < nl * K; e += blockDim.x) {
-      int l = e / K, x = e - l * K;
+      int l = dK.div(e), x = e - l * K;
       long long row = 2LL * (tk.l0 + l);
       C z;
       z.x = g[row * K + x];
@@ -481,7 +483,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_x(MobArgs a, int dir) {
     __syncthreads();
     fft_lines<C, FFT_TILE_X>(sh, K, pitch, nl, -1, tw);
     for (int e = threadIdx.x; e < nl * Kh; e += blockDim.x) {
-      int l = e / Kh, k = e - l * Kh;
+      int l = dKh.div(e), k = e - l * Kh;
       C zk = sh[l * pitch + k], zm = sh[l * pitch + (k == 0 ? 0 : K - k)];
       C A, B;
       A.x = (T)0.5 * (zk.x + zm.x);
@@ -494,7 +496,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_x(MobArgs a, int dir) {
     }
   } else {
     for (int e = threadIdx.x; e < nl * Kh; e += blockDim.x) {
-      int l = e / Kh, k = e - l * Kh;
+      int l = dKh.div(e), k = e - l * Kh;
       long long row = 2LL * (tk.l0 + l);
       C A = gc[row * Kh + k], B = gc[(row + 1) * Kh + k];
       C z;
@@ -516,7 +518,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_x(MobArgs a, int dir) {
     __syncthreads();
     fft_lines<C, FFT_TILE_X>(sh, K, pitch, nl, 1, tw);
     for (int e = threadIdx.x; e < nl * K; e += blockDim.x) {
-      int l = e / K, x = e - l * K;
+      int l = dK.div(e), x = e - l * K;
       long long row = 2LL * (tk.l0 + l);
       C z = sh[l * pitch + x];
       g[row * K + x] = z.x;
@@ -543,6 +545,7 @@ __global__ __launch_bounds__(XY_THREADS) void k_fft_xy(MobArgs a, int dir) {
   const RepDev& R = a.reps[tk.rep];
   if (!R.active || (a.done && a.done[tk.rep])) return;
   const int K0 = R.K[0], K1 = R.K[1], Kh = K0 / 2, P2 = K1 / 2;
+  const FDiv dK0(K0), dKh(Kh);
   const int pa = K0 + 1, pb = K1 + 1;  // x-line and y-line pitches
   const long long ncomp = (long long)K0 * K1 * R.K[2];
   const int z = tk.l0;
@@ -554,7 +557,7 @@ __global__ __launch_bounds__(XY_THREADS) void k_fft_xy(MobArgs a, int dir) {
   C ra[MAXP], rb[MAXP];
   if (dir < 0) {
     for (int e = t; e < P2 * K0; e += TH) {
-      int p = e / K0, x = e - p * K0;
+      int p = dK0.div(e), x = e - p * K0;
       C v;
       v.x = g[(2 * p) * K0 + x];
       v.y = g[(2 * p + 1) * K0 + x];
@@ -566,7 +569,7 @@ __global__ __launch_bounds__(XY_THREADS) void k_fft_xy(MobArgs a, int dir) {
     for (int q = 0; q < MAXP; ++q) {
       int e = t + q * TH;
       if (e < np) {
-        int p = e / Kh, k = e - p * Kh;
+        int p = dKh.div(e), k = e - p * Kh;
         C zk = sh[p * pa + k], zm = sh[p * pa + (k == 0 ? 0 : K0 - k)];
         ra[q].x = (T)0.5 * (zk.x + zm.x);
         ra[q].y = (T)0.5 * (zk.y - zm.y);
@@ -579,7 +582,7 @@ __global__ __launch_bounds__(XY_THREADS) void k_fft_xy(MobArgs a, int dir) {
     for (int q = 0; q < MAXP; ++q) {
       int e = t + q * TH;
       if (e < np) {
-        int p = e / Kh, k = e - p * Kh;
+        int p = dKh.div(e), k = e - p * Kh;
         sh[k * pb + 2 * p] = ra[q];
         sh[k * pb + 2 * p + 1] = rb[q];
       }
@@ -587,12 +590,12 @@ __global__ __launch_bounds__(XY_THREADS) void k_fft_xy(MobArgs a, int dir) {
     __syncthreads();
     fft_lines<C, XY_ELEMS, XY_THREADS>(sh, K1, pb, Kh, -1, twy);
     for (int e = t; e < K1 * Kh; e += TH) {
-      int y = e / Kh, k = e - y * Kh;
+      int y = dKh.div(e), k = e - y * Kh;
       gc[(long long)y * Kh + k] = sh[k * pb + y];
     }
   } else {
     for (int e = t; e < K1 * Kh; e += TH) {
-      int y = e / Kh, k = e - y * Kh;
+      int y = dKh.div(e), k = e - y * Kh;
       sh[k * pb + y] = gc[(long long)y * Kh + k];
     }
     __syncthreads();
@@ -601,7 +604,7 @@ __global__ __launch_bounds__(XY_THREADS) void k_fft_xy(MobArgs a, int dir) {
     for (int q = 0; q < MAXP; ++q) {
       int e = t + q * TH;
       if (e < np) {
-        int p = e / Kh, k = e - p * Kh;
+        int p = dKh.div(e), k = e - p * Kh;
         ra[q] = sh[k * pb + 2 * p];
         rb[q] = sh[k * pb + 2 * p + 1];
       }
@@ -611,7 +614,7 @@ __global__ __launch_bounds__(XY_THREADS) void k_fft_xy(MobArgs a, int dir) {
     for (int q = 0; q < MAXP; ++q) {
       int e = t + q * TH;
       if (e < np) {
-        int p = e / Kh, k = e - p * Kh;
+        int p = dKh.div(e), k = e - p * Kh;
         C A = ra[q], B = rb[q], v;
         v.x = A.x - B.y;
         v.y = A.y + B.x;
@@ -632,7 +635,7 @@ __global__ __launch_bounds__(XY_THREADS) void k_fft_xy(MobArgs a, int dir) {
     __syncthreads();
     fft_lines<C, XY_ELEMS, XY_THREADS>(sh, K0, pa, P2, 1, twx);
     for (int e = t; e < P2 * K0; e += TH) {
-      int p = e / K0, x = e - p * K0;
+      int p = dK0.div(e), x = e - p * K0;
       C v = sh[p * pa + x];
       g[(2 * p) * K0 + x] = v.x;
       g[(2 * p + 1) * K0 + x] = v.y;
@@ -650,19 +653,20 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_y(MobArgs a, int dir) {
   const RepDev& R = a.reps[tk.rep];
   if (!R.active || (a.done && a.done[tk.rep])) return;
   const int Kh = R.K[0] / 2, K = R.K[1], pitch = K + 1, nl = tk.nl;
+  const FDiv dnl(nl), dKh(Kh);
   const long long ncomp = (long long)R.K[0] * R.K[1] * R.K[2];
   C* gc = (C*)((T*)a.grid + R.grid0 + tk.comp * ncomp);
   stage_twiddles(tw, (const C*)a.tw + R.twoff[1], K);
   for (int e = threadIdx.x; e < nl * K; e += blockDim.x) {
-    int k = e / nl, l = e - k * nl;
-    int L = tk.l0 + l, z = L / Kh, x = L - z * Kh;
+    int k = dnl.div(e), l = e - k * nl;
+    int L = tk.l0 + l, z = dKh.div(L), x = L - z * Kh;
     sh[l * pitch + k] = gc[((long long)z * K + k) * Kh + x];
   }
   __syncthreads();
   fft_lines<C, FFT_TILE_Y>(sh, K, pitch, nl, dir, tw);
   for (int e = threadIdx.x; e < nl * K; e += blockDim.x) {
-    int k = e / nl, l = e - k * nl;
-    int L = tk.l0 + l, z = L / Kh, x = L - z * Kh;
+    int k = dnl.div(e), l = e - k * nl;
+    int L = tk.l0 + l, z = dKh.div(L), x = L - z * Kh;
     gc[((long long)z * K + k) * Kh + x] = sh[l * pitch + k];
   }
 }
@@ -679,13 +683,14 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_z_green(MobArgs a) {
   if (!R.active || (a.done && a.done[tk.rep])) return;
   stage_twiddles(tw, (const C*)a.tw + R.twoff[2], R.K[2]);
   const int K0 = R.K[0], Kh = K0 / 2, K1 = R.K[1], K = R.K[2], pitch = K + 1, nl = tk.nl;
+  const FDiv dnl(nl), dKh(Kh);
   const long long ncomp = (long long)R.K[0] * R.K[1] * R.K[2];
   const long long stride = (long long)K1 * Kh;
   C* gc[3];
   for (int c = 0; c < 3; ++c) gc[c] = (C*)((T*)a.grid + R.grid0 + c * ncomp);
   const int cpitch = nl * pitch;
   for (int e = threadIdx.x; e < nl * K; e += blockDim.x) {
-    int k = e / nl, l = e - k * nl;
+    int k = dnl.div(e), l = e - k * nl;
     long long idx = k * stride + tk.l0 + l;
     for (int c = 0; c < 3; ++c) sh[c * cpitch + l * pitch + k] = gc[c][idx];
   }
@@ -695,8 +700,8 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_z_green(MobArgs a) {
   const T ia2 = (T)(1.0 / (4.0 * R.alpha * R.alpha));
   const T invV = (T)(1.0 / R.vol);
   for (int e = threadIdx.x; e < nl * K; e += blockDim.x) {
-    int k = e / nl, l = e - k * nl;
-    int L = tk.l0 + l, y = L / Kh, x = L - y * Kh;
+    int k = dnl.div(e), l = e - k * nl;
+    int L = tk.l0 + l, y = dKh.div(L), x = L - y * Kh;
     C* p0 = sh + l * pitch + k;
     C* p1 = p0 + cpitch;
     C* p2 = p1 + cpitch;
@@ -736,7 +741,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_z_green(MobArgs a) {
   __syncthreads();
   fft_lines<C, 3 * FFT_TILE_Z>(sh, K, pitch, 3 * nl, 1, tw);
   for (int e = threadIdx.x; e < nl * K; e += blockDim.x) {
-    int k = e / nl, l = e - k * nl;
+    int k = dnl.div(e), l = e - k * nl;
     long long idx = k * stride + tk.l0 + l;
     for (int c = 0; c < 3; ++c) gc[c][idx] = sh[c * cpitch + l * pitch + k];
   }
@@ -990,6 +995,7 @@ __global__ __launch_bounds__(64 * CL_WAVES) void k_mob_cl(MobArgs a) {
       pj[u] = a.p32[sj[u]];
       vj[u] = a.v32[sj[u]];
     }
+    float g0 = 0.f, g1 = 0.f, g2 = 0.f;  // this step's terms, summed plainly, then compensated once
 #pragma unroll
     for (int u = 0; u < PAIR_UNROLL; ++u) {
       float dx = pj[u].x - pi.x, dy = pj[u].y - pi.y, dz = pj[u].z - pi.z;
@@ -1020,10 +1026,13 @@ __global__ __launch_bounds__(64 * CL_WAVES) void k_mob_cl(MobArgs a) {
         b2 = cb.x + t * (cb.y + t * (cb.z + t * cb.w));
       }
       const float dv = (dx * vj[u].x + dy * vj[u].y + dz * vj[u].z) * b2;
-      acc[0].add(t1 * vj[u].x + dv * dx);
-      acc[1].add(t1 * vj[u].y + dv * dy);
-      acc[2].add(t1 * vj[u].z + dv * dz);
+      g0 += t1 * vj[u].x + dv * dx;
+      g1 += t1 * vj[u].y + dv * dy;
+      g2 += t1 * vj[u].z + dv * dz;
     }
+    acc[0].add(g0);
+    acc[1].add(g1);
+    acc[2].add(g2);
   }
   double u0 = acc[0].value(), u1 = acc[1].value(), u2 = acc[2].value();
   for (int o = 1; o < 8; o <<= 1) {
