@@ -978,6 +978,7 @@ __global__ __launch_bounds__(64 * CL_WAVES) void k_mob_cl(MobArgs a) {
   CompSum<T> acc[3];
   const int m = a.nclp[I];
   const int* row = a.lcl + (size_t)I * a.capc;
+  const int ntab1 = R.ntab - 1;
   for (int k0 = w; k0 < m; k0 += CL_WAVES * PAIR_UNROLL) {
     int sj[PAIR_UNROLL];
     bool ok[PAIR_UNROLL];
@@ -1004,31 +1005,34 @@ __global__ __launch_bounds__(64 * CL_WAVES) void k_mob_cl(MobArgs a) {
       dy -= n1 * rb1 + n2 * rb3;
       dz -= Lz * rintf(dz * iLz);
       const float rr = dx * dx + dy * dy + dz * dz;
-      if (!ok[u] || !(rr < rcf)) continue;
-      float t1, b2;
-      if (rr < RTAB_R0 * RTAB_R0) {  // closer than the table: closed form (rare)
+      const bool use = ok[u] && rr < rcf;
+      // Table coefficients on every lane (index clamped into the table; the
+      // hardware square root, 1 ulp, is ample for the lookup), so the common
+      // path has no branch and the partner loads stay ahead of it; pairs
+      // closer than the table take the closed form (rare); unused lanes are
+      // masked out of the sums.
+      const float x = fmaxf(__builtin_amdgcn_sqrtf(rr) - RTAB_R0, 0.f) * (float)RTAB_INV;
+      const int it = min((int)x, ntab1);
+      const float t = x - (float)it;
+      float4 ca, cb;
+      if constexpr (TAB_LDS) {
+        ca = stab[2 * it];
+        cb = stab[2 * it + 1];
+      } else {
+        ca = a.rtab[R.tab0 + 2 * it];
+        cb = a.rtab[R.tab0 + 2 * it + 1];
+      }
+      float t1 = ca.x + t * (ca.y + t * (ca.z + t * ca.w));
+      float b2 = cb.x + t * (cb.y + t * (cb.z + t * cb.w));
+      if (use && rr < RTAB_R0 * RTAB_R0) {
         float t2;
         rpy_real_near_f32(sqrtf(rr), rr, alpha, t1, t2);
         b2 = t2 / rr;
-      } else {
-        const float x = (sqrtf(rr) - RTAB_R0) * (float)RTAB_INV;
-        const int it = min((int)x, R.ntab - 1);
-        const float t = x - (float)it;
-        float4 ca, cb;
-        if constexpr (TAB_LDS) {
-          ca = stab[2 * it];
-          cb = stab[2 * it + 1];
-        } else {
-          ca = a.rtab[R.tab0 + 2 * it];
-          cb = a.rtab[R.tab0 + 2 * it + 1];
-        }
-        t1 = ca.x + t * (ca.y + t * (ca.z + t * ca.w));
-        b2 = cb.x + t * (cb.y + t * (cb.z + t * cb.w));
       }
       const float dv = (dx * vj[u].x + dy * vj[u].y + dz * vj[u].z) * b2;
-      g0 += t1 * vj[u].x + dv * dx;
-      g1 += t1 * vj[u].y + dv * dy;
-      g2 += t1 * vj[u].z + dv * dz;
+      g0 += use ? t1 * vj[u].x + dv * dx : 0.f;
+      g1 += use ? t1 * vj[u].y + dv * dy : 0.f;
+      g2 += use ? t1 * vj[u].z + dv * dz : 0.f;
     }
     acc[0].add(g0);
     acc[1].add(g1);
