@@ -325,6 +325,7 @@ struct MobArgs {
   // correction below r = 2 (r = 2 is a knot): rtab[2i], rtab[2i+1]
   const float4* rtab;
   int phase;                // bit 0: bead preparation and input scaling; bit 1: spread .. final
+  int scale_in_spread;      // the spread stages v / beta itself and stores vout, v32 (set by the launcher)
   // mixed precision real space by cluster pairs: cluster c holds beads cl_first[c] ..
   // + cl_cnt[c] - 1; lcl[c * capc + k], k < nclp[c]: clusters with a bead pair in the
   // list, as (first slot << 4) | count
@@ -333,6 +334,7 @@ struct MobArgs {
   int ntab_max;             // largest coefficient table of the range (entries of 2 float4)
 };
 #define RTAB_LDS_MAX 1280   // tables up to this many entries (40 KB) are staged in LDS
+#define SCALE_FUSE_MAX_SLOTS 32768  // ranges up to this many slots normalise the input in the spread
 #define RTAB_INV 16
 #define RTAB_R0 1.0f  // pairs closer than this take the FP32 closed form
 

@@ -354,7 +354,12 @@ __global__ __launch_bounds__(512) void k_spread_gather(MobArgs a) {
   const int g0 = bt.b0 * BRICK + (tid & 7), g1 = bt.b1 * BRICK + ((tid >> 3) & 7), g2 = bt.b2 * BRICK + (tid >> 6);
   const int lo0 = bt.b0 * BRICK, lo1 = bt.b1 * BRICK, lo2 = bt.b2 * BRICK;
   const int qd = ((lane >> 2) & 1) | (((lane >> 5) & 1) << 1);  // this lane's 4 x 4 quadrant
-  const double* vin = a.vout ? a.vout : a.v;
+  // With scale_in_spread the input is normalised here (v / beta, as k_scale_v
+  // would) and the bricks store the scaled input of their own beads (source
+  // brick 0 is the brick itself, and every bead has one home brick).
+  const bool fs = a.scale_in_spread != 0;
+  const double* vin = (a.vout && !fs) ? a.vout : a.v;
+  const double vsc = (fs && a.beta) ? (a.beta[bt.rep] > 0.0 ? 1.0 / a.beta[bt.rep] : 0.0) : 1.0;
   const T* bw = (const T*)a.bw;
   // the 8 source bricks (this one and its +1 neighbours): starts and counts in one step
   __shared__ int nbeg[8], noff[9];
@@ -385,7 +390,18 @@ __global__ __launch_bounds__(512) void k_spread_gather(MobArgs a) {
       int rel = a.bbase[3 * s + d] - (d == 0 ? lo0 : d == 1 ? lo1 : lo2);
       rel += rel < 0 ? (d == 0 ? K0 : d == 1 ? K1 : K2) : 0;
       sb[k] = rel;
-      if (a.v32) {
+      if (fs) {
+        const double x = vin[3 * s + d] * vsc;
+        sv[k] = (T)x;
+        if (nbk == 0) {
+          if (a.vout) a.vout[3 * s + d] = x;
+          if (a.v32) {
+            float* v = (float*)(a.v32 + s);
+            v[d] = (float)x;
+            if (d == 0) v[3] = 0.f;
+          }
+        }
+      } else if (a.v32) {
         const float4 v4 = a.v32[s];
         sv[k] = (T)(d == 0 ? v4.x : d == 1 ? v4.y : v4.z);
       } else {
@@ -1072,6 +1088,11 @@ template <typename T, int P> static void launch_mob_p(hipStream_t st, const MobA
   };
   mark(0);
   int nt = (a.nslots + 255) / 256;
+  // The spread normalises the input itself when it runs in this launch and the
+  // range is small: that saves a launch per product where launches dominate
+  // (-2.5% on an 8000-bead step), while over 32k slots the staged FP64 reads
+  // and stores cost more than the launch (+2.6% on a 380k-bead batch).
+  const bool fuse = (a.phase & 3) == 3 && a.nbricks > 0 && (a.vout || a.v32) && a.nslots <= SCALE_FUSE_MAX_SLOTS;
   if (a.phase & 1) {  // per-bead preparation over the whole replica range
     if (a.prep && a.nslots > 0) {
       launch_fill_i32(st, a.brick_cnt + a.brick_begin, 0, a.brick_len);
@@ -1081,11 +1102,15 @@ template <typename T, int P> static void launch_mob_p(hipStream_t st, const MobA
       k_brick_fill<<<nt, 256, 0, st>>>(a);
       if (a.brick_len > 0) k_brick_sort<<<(a.brick_len + 255) / 256, 256, 0, st>>>(a);
     }
-    if ((a.vout || a.v32) && a.nslots > 0) k_scale_v<<<nt, 256, 0, st>>>(a);
+    if ((a.vout || a.v32) && a.nslots > 0 && !fuse) k_scale_v<<<nt, 256, 0, st>>>(a);
   }
   mark(1);
   if (!(a.phase & 2)) return;
-  if (a.nbricks > 0) k_spread_gather<T, P><<<a.nbricks, 512, 0, st>>>(a);
+  if (a.nbricks > 0) {
+    MobArgs as = a;
+    as.scale_in_spread = fuse ? 1 : 0;
+    k_spread_gather<T, P><<<a.nbricks, 512, 0, st>>>(as);
+  }
   mark(2);
   if (a.ntxy > 0) k_fft_xy<T><<<a.ntxy, XY_THREADS, 0, st>>>(a, -1);
   if (a.ntx > 0) k_fft_x<T><<<a.ntx, FFT_THREADS, 0, st>>>(a, -1);
