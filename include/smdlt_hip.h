@@ -201,6 +201,22 @@ int smdlt_far_table(smdlt_ctx* ctx, int replica, int nt_avg, double bin_size, do
  * of three %.6e values. */
 int smdlt_write_far_table(const char* path, const double* table, int nt_avg, const int dims[3]);
 
+/* Virial stress summed over the steps since the last reset, per replica:
+ * tau[3r..3r+2] = (xx, yy, xy) of -2 x_a F x_b over springs and LJ pairs plus the
+ * reference's bending terms (the bondforce/LJforce accumulators tauxx, tauyy,
+ * tauxy, S/main26.c:576-578,606-613,662-664, X/main14.c:583-619), lab frame.
+ * Every step adds its forces' virial; smdlt_forces does not.  reset != 0 zeroes
+ * the sums after the read, as printVisc does (S/main26.c:1866-1880, which writes
+ * tauxy / printperiod; X/main14.c:1886-1897 writes tauxx and tauyy). */
+int smdlt_stress(smdlt_ctx* ctx, double* tau, int reset);
+/* Per-chain observables of the output step (calcExt, calcCOM, calcRg;
+ * S/main26.c:74-93,925-962,2391-2415), chains of all replicas in order:
+ * unwrapped chain p_{j+1} = p_j + getNID(j, j+1) from p_0 = 0, ext[k] = max - min
+ * of p_x, com[3k..] = mean p, rg[6k..] = mean (p - com)(p - com) as xx, xy, xz,
+ * yy, yz, zz.  Positions as the device holds them, in the frame of the current
+ * lattice (call smdlt_remap(t) first for KRBC).  Any output may be NULL. */
+int smdlt_chain_observables(smdlt_ctx* ctx, double* ext, double* com, double* rg);
+
 /* TEA parameters for SMDLT_NOISE_TEA steps (the output of ewaldBin/CATEA,
  * S/main26.c:975-1381): beta per replica, C per bead component (3N AoS). */
 int smdlt_set_tea(smdlt_ctx* ctx, const double* beta, const double* C);

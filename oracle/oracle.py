@@ -101,6 +101,8 @@ def lib():
         L.orc_bond_force.argtypes = [C.POINTER(Lattice), C.c_int, C.c_int, _dp, _dp, _dp, C.c_int,
                                      C.c_double, C.c_double, C.c_double, _dp, _dp, _dp, _dp]
         L.orc_bond_force.restype = C.c_int
+        L.orc_chain_obs.argtypes = [C.POINTER(Lattice), C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_chain_obs.restype = None
         L.orc_lj_force.argtypes = [C.POINTER(Lattice), C.c_int, _dp, _dp, _dp, _ip, _ip, C.c_int,
                                    C.c_double, C.c_double, C.c_int, _dp, _dp, _dp, _dp]
         L.orc_coulomb_force.argtypes = [C.POINTER(Lattice), C.c_int, _dp, _dp, _dp, _ip, C.c_double,
@@ -234,6 +236,15 @@ def lj_force(lat, rx, ry, rz, nlist, lst, r2cut, epsilon, ext_form):
     lib().orc_lj_force(C.byref(lat), n, rx, ry, rz, np.ascontiguousarray(nlist, np.int32),
                        lst.reshape(-1), stride, r2cut, epsilon, int(ext_form), f[0], f[1], f[2], tau)
     return np.stack(f, 1), tau
+
+
+def chain_obs(lat, nb, nc, rx, ry, rz):
+    """Per-chain extension, centre of mass and gyration tensor of the unwrapped
+    chains (calcExt/calcCOM/calcRg, S/main26.c:74-93,925-962,2391-2415)."""
+    ext, com, rg = np.zeros(nc), np.zeros(3 * nc), np.zeros(6 * nc)
+    lib().orc_chain_obs(C.byref(lat), nb, nc, *(np.ascontiguousarray(a, np.float64) for a in (rx, ry, rz)),
+                        ext, com, rg)
+    return ext, com.reshape(nc, 3), rg.reshape(nc, 6)
 
 
 def coulomb_force(lat, rx, ry, rz, q, lambda_b, lambda_d):

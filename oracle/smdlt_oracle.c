@@ -416,10 +416,61 @@ int orc_bond_force(const orc_lattice_t* lat, int nb, int nc, const double* rx, c
           f[c][c0 - 1] += coeff * (var * n1[c] / (a1 * a1) - n2[c] / (a1 * a2));
           f[c][c0 + 1] += coeff * (var * n2[c] / (a2 * a2) - n1[c] / (a1 * a2));
         }
+        if (tau) {  /* S/main26.c:606-613: xx and yy only; two yy terms read the z components */
+          tau[0] += n1[0] * coeff * (var * n1[0] / (a1 * a1) - n2[0] / (a1 * a2));
+          tau[0] -= n1[0] * coeff * ((n1[0]) / (a1 * a2) - var * (n1[0] / (a1 * a1)));
+          tau[0] += n2[0] * coeff * ((n2[0]) / (a1 * a2) - var * (n2[0] / (a2 * a2)));
+          tau[0] -= n2[0] * coeff * (var * n2[0] / (a2 * a2) - n1[0] / (a1 * a2));
+          tau[1] += n1[1] * coeff * (var * n1[1] / (a1 * a1) - n2[1] / (a1 * a2));
+          tau[1] -= n1[1] * coeff * ((n1[2]) / (a1 * a2) - var * (n1[2] / (a1 * a1)));
+          tau[1] += n2[1] * coeff * ((n2[2]) / (a1 * a2) - var * (n2[2] / (a2 * a2)));
+          tau[1] -= n2[1] * coeff * (var * n2[1] / (a2 * a2) - n1[1] / (a1 * a2));
+        }
       }
     }
   }
   return 0;
+}
+
+/* Chain observables of the output step (S/main26.c:74-93 unwrapping, 925-962
+ * calcExt/calcRg, 2391-2415 calcCOM): p_{j+1} = p_j + getNID(j, j+1) from
+ * p_0 = 0, so each chain is unwrapped (mirrored: getNID is r_j - r_{j+1});
+ * ext = max - min of p_x; com = mean of p; rg = mean of (p - com)(p - com) as
+ * xx, xy, xz, yy, yz, zz.  The reference's calcCOM and calcRg accumulators are
+ * neither initialised nor reset between chains (undefined behaviour); this is
+ * the per-chain value they are meant to hold. */
+void orc_chain_obs(const orc_lattice_t* lat, int nb, int nc, const double* rx, const double* ry,
+                   const double* rz, double* ext, double* com, double* rg) {
+  double* p = (double*)malloc(sizeof(double) * 3 * (size_t)nb);
+  for (int i = 0; i < nc; ++i) {
+    p[0] = p[1] = p[2] = 0.0;
+    for (int j = 0; j < nb - 1; ++j) {
+      double d[3];
+      orc_nid(lat, rx, ry, rz, i * nb + j, i * nb + j + 1, d);
+      for (int k = 0; k < 3; ++k) p[3 * (j + 1) + k] = p[3 * j + k] + d[k];
+    }
+    double mx = p[0], mn = p[0], c[3] = {0.0, 0.0, 0.0};
+    for (int j = 0; j < nb; ++j) {
+      mx = fmax(mx, p[3 * j]);
+      mn = fmin(mn, p[3 * j]);
+      for (int k = 0; k < 3; ++k) c[k] += p[3 * j + k];
+    }
+    for (int k = 0; k < 3; ++k) c[k] /= nb;
+    double g[6] = {0, 0, 0, 0, 0, 0};
+    for (int j = 0; j < nb; ++j) {
+      double a = p[3 * j] - c[0], b = p[3 * j + 1] - c[1], e = p[3 * j + 2] - c[2];
+      g[0] += a * a;
+      g[1] += a * b;
+      g[2] += a * e;
+      g[3] += b * b;
+      g[4] += b * e;
+      g[5] += e * e;
+    }
+    ext[i] = mx - mn;
+    for (int k = 0; k < 3; ++k) com[3 * i + k] = c[k];
+    for (int k = 0; k < 6; ++k) rg[6 * i + k] = g[k] / nb;
+  }
+  free(p);
 }
 
 /* LJforce over the Verlet list: shear 12eps(r6^2-r6)/r^2 (S/main26.c:619-681),

@@ -74,6 +74,8 @@ int  orc_check_verlet(int n, const double* dxv, const double* dyv, const double*
 int  orc_bond_force(const orc_lattice_t* lat, int nb, int nc, const double* rx, const double* ry,
                     const double* rz, int spring_fene, double kappas, double qmax, double kappab,
                     double* fx, double* fy, double* fz, double tau[3]);
+void orc_chain_obs(const orc_lattice_t* lat, int nb, int nc, const double* rx, const double* ry,
+                   const double* rz, double* ext, double* com, double* rg);
 void orc_lj_force(const orc_lattice_t* lat, int n, const double* rx, const double* ry, const double* rz,
                   const int* nlist, const int* list, int stride, double r2cut, double epsilon,
                   int ext_form, double* fx, double* fy, double* fz, double tau[3]);

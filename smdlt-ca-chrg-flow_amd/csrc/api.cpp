@@ -321,7 +321,13 @@ struct smdlt_ctx {
   std::vector<hipGraphExec_t> gchunks;
   int* h_flags = nullptr;  // pinned: done[nrep], iters[nrep], overflow
   double *tea_beta = nullptr, *tea_C = nullptr, *tea_diag = nullptr;  // SMDLT_NOISE_TEA steps
-  double* saved = nullptr; // positions and displacement trackers at the start of a graph step
+  double* saved = nullptr; // positions, displacement trackers and stress sums at the start of a graph step
+  // observables: per-bead virial terms, per-replica stress sums (xx, yy, xy),
+  // first chain of each replica, per-chain extension/com/gyration output
+  double *vir = nullptr, *tau = nullptr, *obs = nullptr;
+  int* d_chain0 = nullptr;
+  std::vector<int> chain0;
+  int nchains = 0, max_chains = 0;
   bool force_rebuild = false;
   int expect_iters = 0;    // largest iteration count of the previous graph step
   bool graph_ready = false;
@@ -846,7 +852,18 @@ int smdlt_create(const smdlt_params* params, int n_replicas, int device, smdlt_c
   st |= alloc_dev(&c->done, n_replicas);
   st |= alloc_dev(&c->iters, n_replicas);
   st |= alloc_dev(&c->mc_err, 4);
-  st |= alloc_dev(&c->saved, 9 * (size_t)c->nst);
+  st |= alloc_dev(&c->saved, 9 * (size_t)c->nst + 3 * (size_t)n_replicas);
+  c->chain0.assign(n_replicas + 1, 0);
+  for (int r = 0; r < n_replicas; ++r) {
+    const int ncr = c->rh[r].n / c->rh[r].nb;
+    c->chain0[r + 1] = c->chain0[r] + ncr;
+    c->max_chains = std::max(c->max_chains, ncr);
+  }
+  c->nchains = c->chain0[n_replicas];
+  st |= alloc_dev(&c->vir, 3 * (size_t)c->nst);
+  st |= alloc_dev(&c->tau, 3 * (size_t)n_replicas);
+  st |= alloc_dev(&c->d_chain0, n_replicas + 1);
+  st |= alloc_dev(&c->obs, 10 * (size_t)std::max(1, c->nchains));
   if (hipHostMalloc((void**)&c->h_flags, 8 * (size_t)n_replicas + 4, hipHostMallocDefault) != hipSuccess) {
     c->h_flags = nullptr;
     st |= SMDLT_E_NOMEM;
@@ -858,6 +875,8 @@ int smdlt_create(const smdlt_params* params, int n_replicas, int device, smdlt_c
   // Lanczos scalar blocks start at zero (no previous solve: first square root at iteration 2)
   HCK(hipMemset(c->scal, 0, sizeof(double) * SCAL_STRIDE * (size_t)n_replicas));
   HCK(hipMemset(c->iters, 0, sizeof(int) * (size_t)n_replicas));
+  HCK(hipMemset(c->tau, 0, sizeof(double) * 3 * (size_t)n_replicas));
+  HCK(hipMemcpy(c->d_chain0, c->chain0.data(), sizeof(int) * (n_replicas + 1), hipMemcpyHostToDevice));
   // slot -> replica map, seeds
   std::vector<int> sr(c->nst, -1);
   std::vector<uint64_t> seeds(n_replicas);
@@ -1066,7 +1085,7 @@ void smdlt_destroy(smdlt_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   destroy_graphs(c);
   if (c->h_flags) hipHostFree(c->h_flags);
-  void* ptrs[] = {c->adot_cl, c->tea_beta, c->tea_C, c->tea_diag, c->cl_first, c->cl_cnt, c->slot_cl, c->lcl, c->nclp, c->rtab, c->p32, c->v32, c->mh_feats, c->mh_log, c->mlp_ws, c->mlp_x, c->mlp_y, c->mlp_part, c->mlp_out, c->saved, c->d_reps, c->d_slot_rep, c->d_seeds, c->rx, c->ry, c->rz, c->dxv, c->dyv, c->dzv, c->dsp, c->f,
+  void* ptrs[] = {c->adot_cl, c->tea_beta, c->tea_C, c->tea_diag, c->cl_first, c->cl_cnt, c->slot_cl, c->lcl, c->nclp, c->rtab, c->p32, c->v32, c->mh_feats, c->mh_log, c->mlp_ws, c->mlp_x, c->mlp_y, c->mlp_part, c->mlp_out, c->saved, c->vir, c->tau, c->obs, c->d_chain0, c->d_reps, c->d_slot_rep, c->d_seeds, c->rx, c->ry, c->rz, c->dxv, c->dyv, c->dzv, c->dsp, c->f,
                   c->z, c->mf, c->bz, c->tmp, c->tmp2, c->nrpy, c->lrpy, c->nlj, c->llj, c->overflow, c->d_err,
                   c->cell_of, c->cell_cnt, c->cell_start, c->cell_fill, c->cell_beads, c->cell_tmp, c->rep_flags, c->out2, c->aff,
                   c->partial2, c->grid, c->bsp, c->tw, c->tx, c->ty, c->tz, c->txy, c->bricks, c->bbase, c->brick_cnt, c->brick_start,
@@ -1431,14 +1450,17 @@ int smdlt_get_verlet(smdlt_ctx* c, int replica, int* nlist, int* list, int strid
   return 0;
 }
 
-static int forces_internal(smdlt_ctx* c) {
+// accumulate: a step's forces also add their virial to the stress sums (as
+// bondforce/LJforce do every step); a stand-alone force evaluation does not.
+static int forces_internal(smdlt_ctx* c, bool accumulate) {
   StageScope sc(c, ST_FORCES);
   int rb = c->cur_rep_begin, re = c->cur_rep_end;
   const RepDev& a = c->rh[rb];
   const RepDev& b = c->rh[re - 1];
   int s0 = a.slot0, ns = b.slot0 + pad_slots(b.n) - s0;
   launch_forces(c->stream, c->d_reps, c->d_slot_rep, s0, ns, c->rx, c->ry, c->rz, c->nst, c->nlj, c->llj, c->f,
-                c->d_err);
+                accumulate ? c->vir : nullptr, c->d_err);
+  if (accumulate) launch_virial_acc(c->stream, c->d_reps, rb, re - rb, c->vir, c->tau);
   if (c->max_nq > 0)
     launch_coulomb(c->stream, c->d_reps, rb, re - rb, c->rx, c->ry, c->rz, c->qtrack, c->qsign, c->f, c->max_nq);
   HCK(hipGetLastError());
@@ -1465,7 +1487,7 @@ int smdlt_forces(smdlt_ctx* c, double* fx, double* fy, double* fz) {
   }
   int st = upload_reps(c);
   if (st) return st;
-  st = forces_internal(c);
+  st = forces_internal(c, false);
   if (st) return st;
   st = check_err(c);
   if (st) return st;
@@ -2105,7 +2127,7 @@ static int step_internal(smdlt_ctx* c, uint64_t t, double* noise_dev, bool host_
       HCK(hipStreamSynchronize(c->stream));
     }
   }
-  st = forces_internal(c);
+  st = forces_internal(c, true);
   if (st) return st;
   const double* zz = noise_dev;
   {
@@ -2193,6 +2215,10 @@ static void save_state(smdlt_ctx* c, bool restore) {
       launch_copy_f64(c->stream, c->saved + off, cur[k], len[k]);
     off += len[k];
   }
+  if (restore)
+    launch_copy_f64(c->stream, c->tau, c->saved + off, 3LL * c->nrep);
+  else
+    launch_copy_f64(c->stream, c->saved + off, c->tau, 3LL * c->nrep);
 }
 
 // The graphs of one step for the current replica range (see smdlt_ctx).
@@ -2206,7 +2232,7 @@ static int build_step_graphs(smdlt_ctx* c, uint64_t t) {
     save_state(c, false);
     launch_remap(c->stream, c->d_reps, c->d_slot_rep, s0, ns, c->rx, c->ry, c->rz, c->dxv, c->dyv, c->dzv, c->dsp, 1);
     int s = neighbors_internal(c, 0);
-    if (!s) s = forces_internal(c);
+    if (!s) s = forces_internal(c, true);
     if (s) return s;
     launch_noise(c->stream, c->d_slot_rep, c->d_reps, s0, ns, c->d_seeds, t, c->z);
     mobility_internal(c, c->f, c->mf, nullptr, nullptr, nullptr, nullptr);
@@ -2558,3 +2584,35 @@ int smdlt_select_replicas(smdlt_ctx* c, int begin, int end) {
 }
 
 }  // extern "C"
+
+int smdlt_stress(smdlt_ctx* c, double* tau, int reset) {
+  if (!c || !tau) return SMDLT_E_ARG;
+  HCK(hipSetDevice(c->device));
+  HCK(hipMemcpyAsync(tau, c->tau, sizeof(double) * 3 * (size_t)c->nrep, hipMemcpyDeviceToHost, c->stream));
+  HCK(hipStreamSynchronize(c->stream));
+  if (reset) HCK(hipMemset(c->tau, 0, sizeof(double) * 3 * (size_t)c->nrep));
+  return 0;
+}
+
+int smdlt_chain_observables(smdlt_ctx* c, double* ext, double* com, double* rg) {
+  if (!c) return SMDLT_E_ARG;
+  if (!c->have_pos) return SMDLT_E_STATE;
+  HCK(hipSetDevice(c->device));
+  const int rb = c->cur_rep_begin, re = c->cur_rep_end, nk = c->nchains;
+  double* d = c->obs;
+  launch_chain_obs(c->stream, c->d_reps, rb, re - rb, c->max_chains, c->d_chain0, c->rx, c->ry, c->rz, d, d + nk,
+                   d + 4 * (size_t)nk);
+  HCK(hipGetLastError());
+  std::vector<double> h(10 * (size_t)nk);
+  HCK(hipMemcpyAsync(h.data(), d, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
+  HCK(hipStreamSynchronize(c->stream));
+  for (int r = rb; r < re; ++r)
+    for (int k = c->chain0[r]; k < c->chain0[r + 1]; ++k) {
+      if (ext) ext[k] = h[k];
+      for (int q = 0; q < 3; ++q)
+        if (com) com[3 * k + q] = h[nk + 3 * k + q];
+      for (int q = 0; q < 6; ++q)
+        if (rg) rg[6 * k + q] = h[4 * (size_t)nk + 6 * k + q];
+    }
+  return 0;
+}

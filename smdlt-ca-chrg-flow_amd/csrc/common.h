@@ -274,7 +274,10 @@ void launch_fill_f32(hipStream_t s, float* p, float v, long long n);
 // k_forces.hip
 void launch_forces(hipStream_t s, const RepDev* reps, const int* slot_rep, int slot_begin, int nslots,
                    const double* rx, const double* ry, const double* rz, int nslots_total, const int* nlj,
-                   const int* llj, double* f, int* err);
+                   const int* llj, double* f, double* vir, int* err);
+void launch_virial_acc(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, const double* vir, double* tau);
+void launch_chain_obs(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, int max_chains, const int* chain0,
+                      const double* rx, const double* ry, const double* rz, double* ext, double* com, double* rg);
 void launch_coulomb(hipStream_t s, const RepDev* reps, int rep_begin, int nrep, const double* rx,
                     const double* ry, const double* rz, const int* qtrack, const int* qsign, double* f, int max_nq);
 

@@ -46,10 +46,58 @@ __device__ __forceinline__ void bend_term(const RepDev& R, const double* rx, con
   }
 }
 
+// Virial stress (the bondforce/LJforce tau accumulators).  The reference adds
+// -2 x_a Fs x_b once per spring and per LJ pair (S/main26.c:576-578,662-664)
+// with getNID's vector, which for KRBC is in the lab frame; gathered per bead,
+// every pair is met from both ends, so each end adds -x_a Fs x_b.  Components
+// xx, yy, xy.
+__device__ __forceinline__ void lab_vec(const RepDev& R, const double d[3], double o[3]) {
+#pragma clang fp contract(off)
+  if (R.flow == 2) {
+    o[0] = d[0] * R.ct - d[1] * R.st;
+    o[1] = d[0] * R.st + d[1] * R.ct;
+  } else {
+    o[0] = d[0];
+    o[1] = d[1];
+  }
+  o[2] = d[2];
+}
+__device__ __forceinline__ void vir_pair(const RepDev& R, double v[3], double fs, const double d[3]) {
+  double l[3];
+  lab_vec(R, d, l);
+  v[0] -= l[0] * fs * l[0];
+  v[1] -= l[1] * fs * l[1];
+  v[2] -= l[0] * fs * l[1];
+}
+// The bending part of the virial, once per angle (centre c): the reference's
+// eight terms (S/main26.c:606-613, X/main14.c:612-619), xx and yy only, two of
+// the yy terms reading the z components.
+__device__ __forceinline__ void bend_virial(const RepDev& R, const double* rx, const double* ry, const double* rz,
+                                            int c, double v[3]) {
+  double n1[3], n2[3], p[3], q[3];
+  nid_slots(R, rx, ry, rz, c, c - 1, n1);
+  nid_slots(R, rx, ry, rz, c, c + 1, n2);
+  double a1 = sqrt(n1[0] * n1[0] + n1[1] * n1[1] + n1[2] * n1[2]);
+  double a2 = sqrt(n2[0] * n2[0] + n2[1] * n2[1] + n2[2] * n2[2]);
+  double var = (n1[0] * n2[0] + n1[1] * n2[1] + n1[2] * n2[2]) / (a1 * a2);
+  double thetab = M_PI - acos(var);
+  double coeff = -R.kappab * thetab / sqrt(1 - var * var);
+  lab_vec(R, n1, p);
+  lab_vec(R, n2, q);
+  v[0] += p[0] * coeff * (var * p[0] / (a1 * a1) - q[0] / (a1 * a2));
+  v[0] -= p[0] * coeff * ((p[0]) / (a1 * a2) - var * (p[0] / (a1 * a1)));
+  v[0] += q[0] * coeff * ((q[0]) / (a1 * a2) - var * (q[0] / (a2 * a2)));
+  v[0] -= q[0] * coeff * (var * q[0] / (a2 * a2) - p[0] / (a1 * a2));
+  v[1] += p[1] * coeff * (var * p[1] / (a1 * a1) - q[1] / (a1 * a2));
+  v[1] -= p[1] * coeff * ((p[2]) / (a1 * a2) - var * (p[2] / (a1 * a1)));
+  v[1] += q[1] * coeff * ((q[2]) / (a1 * a2) - var * (q[2] / (a2 * a2)));
+  v[1] -= q[1] * coeff * (var * q[1] / (a2 * a2) - p[1] / (a1 * a2));
+}
+
 __global__ void k_forces(const RepDev* __restrict__ reps, const int* __restrict__ slot_rep, int slot_begin,
                          int nslots, const double* __restrict__ rx, const double* __restrict__ ry,
                          const double* __restrict__ rz, int nst, const int* __restrict__ nlj,
-                         const int* __restrict__ llj, double* f, int* err) {
+                         const int* __restrict__ llj, double* f, double* vir, int* err) {
   int i = slot_begin + blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= slot_begin + nslots) return;
   int r = slot_rep[i];
@@ -59,25 +107,29 @@ __global__ void k_forces(const RepDev* __restrict__ reps, const int* __restrict_
   int li = i - R.slot0;
   if (li >= R.n) {
     f[3 * i] = f[3 * i + 1] = f[3 * i + 2] = 0.0;
+    if (vir) vir[3 * i] = vir[3 * i + 1] = vir[3 * i + 2] = 0.0;
     return;
   }
   int pos = li % R.nb;
-  double F[3] = {0.0, 0.0, 0.0}, d[3];
+  double F[3] = {0.0, 0.0, 0.0}, d[3], V[3] = {0.0, 0.0, 0.0};
   if (pos > 0) {  // spring (i, i-1): f_i += Fs * (r_i - r_{i-1})
     nid_slots(R, rx, ry, rz, i, i - 1, d);
     double fs = spring_fs(R, d[0] * d[0] + d[1] * d[1] + d[2] * d[2], err);
     for (int k = 0; k < 3; ++k) F[k] += fs * d[k];
+    if (vir) vir_pair(R, V, fs, d);
   }
   if (pos < R.nb - 1) {  // spring (i+1, i): f_i -= Fs * (r_{i+1} - r_i)
     nid_slots(R, rx, ry, rz, i + 1, i, d);
     double fs = spring_fs(R, d[0] * d[0] + d[1] * d[1] + d[2] * d[2], err);
     for (int k = 0; k < 3; ++k) F[k] -= fs * d[k];
+    if (vir) vir_pair(R, V, fs, d);
   }
   if (R.kappab > 0.0) {
     for (int c = i - 1; c <= i + 1; ++c) {
       int pc = pos + (c - i);
       if (pc >= 1 && pc <= R.nb - 2) bend_term(R, rx, ry, rz, c, i, F);
     }
+    if (vir && pos >= 1 && pos <= R.nb - 2) bend_virial(R, rx, ry, rz, i, V);
   }
   int m = nlj[i];
   double xi = rx[i], yi = ry[i], zi = rz[i];
@@ -90,19 +142,123 @@ __global__ void k_forces(const RepDev* __restrict__ reps, const int* __restrict_
       double r6 = ratio * ratio * ratio;
       double coeff = R.lj_ext ? (48.0 * R.epsilon / rr) * (r6 * r6 - 0.5 * r6) : (12 * R.epsilon / rr) * (r6 * r6 - r6);
       for (int q = 0; q < 3; ++q) F[q] += coeff * d[q];
+      if (vir) vir_pair(R, V, coeff, d);
     }
   }
   f[3 * i] = F[0];
   f[3 * i + 1] = F[1];
   f[3 * i + 2] = F[2];
+  if (vir) {
+    vir[3 * i] = V[0];
+    vir[3 * i + 1] = V[1];
+    vir[3 * i + 2] = V[2];
+  }
+}
+
+// tau[r] += the replica's per-bead virial terms, summed in a fixed order (one
+// workgroup per replica: strided per-thread sums, then a tree in LDS).
+__global__ __launch_bounds__(256) void k_virial_acc(const RepDev* __restrict__ reps, int rep_begin,
+                                                    const double* __restrict__ vir, double* tau) {
+  const int r = rep_begin + blockIdx.x, t = threadIdx.x;
+  const RepDev& R = reps[r];
+  if (!R.active) return;
+  __shared__ double red[3][256];
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  for (int s = R.slot0 + t; s < R.slot0 + R.n; s += 256) {
+    a0 += vir[3 * s];
+    a1 += vir[3 * s + 1];
+    a2 += vir[3 * s + 2];
+  }
+  red[0][t] = a0;
+  red[1][t] = a1;
+  red[2][t] = a2;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w)
+      for (int k = 0; k < 3; ++k) red[k][t] += red[k][t + w];
+    __syncthreads();
+  }
+  if (t == 0)
+    for (int k = 0; k < 3; ++k) tau[3 * r + k] += red[k][0];
+}
+
+void launch_virial_acc(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, const double* vir, double* tau) {
+  if (nrep > 0) k_virial_acc<<<nrep, 256, 0, st>>>(reps, rep_begin, vir, tau);
+}
+
+// Per-chain observables (calcExt, calcCOM, calcRg; S/main26.c:74-93,925-962,
+// 2391-2415), one thread per chain in the reference's sequential order:
+// p_{j+1} = p_j + getNID(j, j+1) from p_0 = 0 (lab frame), ext = max - min of
+// p_x, com = mean p, rg = mean (p - com)(p - com) as xx, xy, xz, yy, yz, zz.
+// The unwrapped chain is walked twice (sums, then the tensor about the mean);
+// no contraction, so the values equal a sequential FP64 evaluation.
+__global__ void k_chain_obs(const RepDev* __restrict__ reps, int rep_begin, const int* __restrict__ chain0,
+                            const double* __restrict__ rx, const double* __restrict__ ry,
+                            const double* __restrict__ rz, double* ext, double* com, double* rg) {
+#pragma clang fp contract(off)
+  const int r = rep_begin + blockIdx.y;
+  const RepDev& R = reps[r];
+  const int ch = blockIdx.x * blockDim.x + threadIdx.x;
+  if (!R.active || ch >= R.n / R.nb) return;
+  const int b0 = R.slot0 + ch * R.nb, k = chain0[r] + ch;
+  double px = 0.0, py = 0.0, pz = 0.0, mx = 0.0, mn = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
+  for (int j = 0; j < R.nb; ++j) {
+    if (j > 0) {
+      double d[3];
+      ref_nid(R, rx[b0 + j - 1], ry[b0 + j - 1], rz[b0 + j - 1], rx[b0 + j], ry[b0 + j], rz[b0 + j], d[0], d[1],
+              d[2], true);
+      px += d[0];
+      py += d[1];
+      pz += d[2];
+    }
+    mx = j ? fmax(mx, px) : px;
+    mn = j ? fmin(mn, px) : px;
+    cx += px;
+    cy += py;
+    cz += pz;
+  }
+  cx /= R.nb;
+  cy /= R.nb;
+  cz /= R.nb;
+  double g[6] = {0, 0, 0, 0, 0, 0};
+  px = py = pz = 0.0;
+  for (int j = 0; j < R.nb; ++j) {
+    if (j > 0) {
+      double d[3];
+      ref_nid(R, rx[b0 + j - 1], ry[b0 + j - 1], rz[b0 + j - 1], rx[b0 + j], ry[b0 + j], rz[b0 + j], d[0], d[1],
+              d[2], true);
+      px += d[0];
+      py += d[1];
+      pz += d[2];
+    }
+    const double a = px - cx, b = py - cy, e = pz - cz;
+    g[0] += a * a;
+    g[1] += a * b;
+    g[2] += a * e;
+    g[3] += b * b;
+    g[4] += b * e;
+    g[5] += e * e;
+  }
+  ext[k] = mx - mn;
+  com[3 * k] = cx;
+  com[3 * k + 1] = cy;
+  com[3 * k + 2] = cz;
+  for (int q = 0; q < 6; ++q) rg[6 * k + q] = g[q] / R.nb;
+}
+
+void launch_chain_obs(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, int max_chains, const int* chain0,
+                      const double* rx, const double* ry, const double* rz, double* ext, double* com, double* rg) {
+  if (nrep <= 0 || max_chains <= 0) return;
+  dim3 grid((max_chains + 63) / 64, nrep);
+  k_chain_obs<<<grid, 64, 0, st>>>(reps, rep_begin, chain0, rx, ry, rz, ext, com, rg);
 }
 
 void launch_forces(hipStream_t st, const RepDev* reps, const int* slot_rep, int slot_begin, int nslots,
                    const double* rx, const double* ry, const double* rz, int nslots_total, const int* nlj,
-                   const int* llj, double* f, int* err) {
+                   const int* llj, double* f, double* vir, int* err) {
   if (nslots <= 0) return;
   k_forces<<<(nslots + 127) / 128, 128, 0, st>>>(reps, slot_rep, slot_begin, nslots, rx, ry, rz, nslots_total, nlj,
-                                                 llj, f, err);
+                                                 llj, f, vir, err);
 }
 
 // Screened Coulomb between all charged beads of a replica: one wavefront per

@@ -88,6 +88,8 @@ def lib():
         L.smdlt_far_table_dims.argtypes = [C.c_void_p, C.c_int, C.c_double, _ip, _dp]
         L.smdlt_far_table.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, _dp]
         L.smdlt_write_far_table.argtypes = [C.c_char_p, _dp, C.c_int, _ip]
+        L.smdlt_stress.argtypes = [C.c_void_p, _dp, C.c_int]
+        L.smdlt_chain_observables.argtypes = [C.c_void_p, _dp, _dp, _dp]
         L.smdlt_set_profiling.argtypes = [C.c_void_p, C.c_int]
         L.smdlt_stage_times.argtypes = [C.c_void_p, _dp, _ip, C.c_int]
         L.smdlt_stage_name.restype = C.c_char_p
@@ -288,6 +290,20 @@ class Context:
         _chk(lib().smdlt_far_table(self.h, replica, nt_avg, bin_size, _ptr(table, C.c_double)), "far_table")
         return table.reshape(nt_avg, nb, 3, 3), tuple(int(d) for d in dims), sizes
 
+    def stress(self, reset=False):
+        """smdlt_stress: per replica (xx, yy, xy) virial sums of the steps since the last reset."""
+        tau = np.zeros(3 * self.nrep)
+        _chk(lib().smdlt_stress(self.h, _ptr(tau, C.c_double), int(bool(reset))), "stress")
+        return tau.reshape(self.nrep, 3)
+
+    def chain_observables(self):
+        """smdlt_chain_observables: (ext[nchains], com[nchains, 3], rg[nchains, 6]) of all chains."""
+        nk = sum(p.nc for p in self.params)
+        ext, com, rg = np.zeros(nk), np.zeros(3 * nk), np.zeros(6 * nk)
+        _chk(lib().smdlt_chain_observables(self.h, _ptr(ext, C.c_double), _ptr(com, C.c_double),
+                                           _ptr(rg, C.c_double)), "chain_observables")
+        return ext, com.reshape(nk, 3), rg.reshape(nk, 6)
+
     def set_tea(self, beta, Cv):
         beta = np.ascontiguousarray(np.broadcast_to(beta, (self.nrep,)), np.float64)
         Cv = np.ascontiguousarray(Cv, np.float64).reshape(-1)
@@ -445,6 +461,62 @@ class TeaAverages:
                     c = np.zeros(3 * self.nb)
                 for j in range(self.nb):
                     f.write(f"{j} {c[3 * j]:.12e} {c[3 * j + 1]:.12e} {c[3 * j + 2]:.12e}\n")
+
+
+# ------------------------------------------------------------- output files
+# The reference's text outputs, appended per output step as its writers do
+# (C %lf is six decimals).  Callers choose the cadence (printperiod, printprops).
+
+def append_visc(path, t, tau, printperiod, flow_type):
+    """printVisc: shear "t tauxy/printperiod" (S/main26.c:1866-1880), planar
+    extension "t tauxx/printperiod tauyy/printperiod" (X/main14.c:1886-1897);
+    tau is one replica's (xx, yy, xy) row of Context.stress(reset=True)."""
+    with open(path, "a") as f:
+        if flow_type == FLOW_EXTENSION:
+            f.write(f"{t} {tau[0] / printperiod:.6f} {tau[1] / printperiod:.6f}\n")
+        else:
+            f.write(f"{t} {tau[2] / printperiod:.6f}\n")
+
+
+def append_ext(path, t, strain, ext, nb, qmax):
+    """printExt (S/main26.c:963-974): per chain "t strain x/L (x/L)^2" with
+    x/L = ext / (nb qmax), ext from Context.chain_observables."""
+    with open(path, "a") as f:
+        for e in ext:
+            x = e / (nb * qmax)
+            f.write(f"{t} {strain:.6f} {x:.6f} {x ** 2:.6f}\n")
+
+
+def append_rg(path, t, rg):
+    """calcRg's file (S/main26.c:942-962): per chain "t i xx xy xz yy yz zz"."""
+    with open(path, "a") as f:
+        for i, g in enumerate(rg):
+            f.write(f"{t} {i} " + " ".join(f"{v:.6f}" for v in g) + "\n")
+
+
+def append_xyz_frame(path, t, lattice_row, L, rx, ry, rz, charge):
+    """printTrajectory (S/main26.c:870-901): extended-xyz frame with the lattice
+    L1 = (L1x, L1y), L2 = (L2x, L2y) (a row of Context.update_lattice) and the box
+    origin; charge 0 beads as A, charge 1 beads as B (other values are not
+    written, as in the reference)."""
+    L1x, L1y, L2x, L2y = lattice_row[:4]
+    with open(path, "a") as f:
+        f.write(f"{len(rx)}\n")
+        f.write(f'Lattice="{L1x:.6f} {L1y:.6f} {0.0:.6f} {L2x:.6f} {L2y:.6f} {0.0:.6f} {0.0:.6f} {0.0:.6f} {L:.6f}" '
+                f'Origin="{-(L1x + L2x) / 2:.6f} {-(L1y + L2y) / 2:.6f} {-L / 2:.6f}" Properties="" Time={t}\n')
+        for x, y, z, q in zip(rx, ry, rz, charge):
+            if q == 0:
+                f.write(f"A {x:.6f} {y:.6f} {z:.6f}\n")
+            elif q == 1:
+                f.write(f"B {x:.6f} {y:.6f} {z:.6f}\n")
+
+
+def write_restart(path, t, rx, ry, rz):
+    """printRestart (S/main26.c:902-924): "N", "t", then "A x y z" per bead."""
+    with open(path, "w") as f:
+        f.write(f"{len(rx)}\n{t}\n")
+        for x, y, z in zip(rx, ry, rz):
+            f.write(f"A {x:.6f} {y:.6f} {z:.6f}\n")
 
 
 def write_far_table(path, table, dims):

@@ -52,3 +52,32 @@ def test_tea_averages_bins_and_decomp_layout(tmp_path):
     row = [float(x) for x in lines[2 + (1 + nb)].split()]
     assert row[0] == 0 and np.allclose(row[1:], want[:3])
     assert lines[1 + 2 * (1 + nb)].split()[1] == "0"  # an empty bin is written as zeros
+
+
+def test_output_writers_follow_the_reference_formats(tmp_path):
+    # the fprintf layouts of printVisc (S/main26.c:1874, X/main14.c:1894), printExt
+    # (S/main26.c:971), calcRg (S/main26.c:958), printTrajectory (S/main26.c:874-884)
+    # and printRestart (S/main26.c:917-921); %lf prints six decimals
+    v = tmp_path / "visc.txt"
+    S.append_visc(v, 1000, [1.5, -2.0, 3.5], 1000, S.FLOW_SHEAR)
+    S.append_visc(v, 2000, [1.5, -2.0, 3.5], 1000, S.FLOW_EXTENSION)
+    assert v.read_text() == "1000 0.003500\n2000 0.001500 -0.002000\n"
+    e = tmp_path / "ext.txt"
+    S.append_ext(e, 5, 0.25, [18.0, 9.0], 10, 2.0)
+    assert e.read_text() == "5 0.250000 0.900000 0.810000\n5 0.250000 0.450000 0.202500\n"
+    g = tmp_path / "rg.txt"
+    S.append_rg(g, 7, np.array([[1.0, 0.5, -0.25, 2.0, 0.0, 3.0]]))
+    assert g.read_text() == "7 0 1.000000 0.500000 -0.250000 2.000000 0.000000 3.000000\n"
+    x = tmp_path / "traj.xyz"
+    S.append_xyz_frame(x, 12, [10.0, 0.0, 2.5, 10.0], 10.0, [1.0, 2.0, 3.0], [0.0, 0.5, 1.0], [-1.0, 0.0, 1.0],
+                       [0, 1, -1])
+    assert x.read_text().splitlines() == [
+        "3",
+        'Lattice="10.000000 0.000000 0.000000 2.500000 10.000000 0.000000 0.000000 0.000000 10.000000" '
+        'Origin="-6.250000 -5.000000 -5.000000" Properties="" Time=12',
+        "A 1.000000 0.000000 -1.000000",
+        "B 2.000000 0.500000 0.000000",
+    ]
+    r = tmp_path / "R.xyz"
+    S.write_restart(r, 99, [1.0, 2.0], [3.0, 4.0], [5.0, 6.0])
+    assert r.read_text() == "2\n99\nA 1.000000 3.000000 5.000000\nA 2.000000 4.000000 6.000000\n"

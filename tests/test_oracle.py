@@ -183,6 +183,60 @@ def test_two_bead_forces_closed_form():
     assert O.coulomb_energy(lat, rx, ry, rz, q, 3.59, 5.0) == pytest.approx(3.59 * np.exp(-r / 5) / r)
 
 
+def test_virial_equals_clausius_sum():
+    # pair forces f_i = c d, f_j = -c d with d = r_i - r_j give sum_pairs c d d =
+    # sum_i r_i F_i, so the accumulators -2 sum_pairs d c d equal -2 sum_i r_i F_i
+    # for an unwrapped chain that meets no image (springs and LJ; bending adds
+    # its own quirky terms, S/main26.c:606-613)
+    L = 400.0
+    pos = random_walk(30, 2, L, 5)
+    p = pos - pos.mean(0)
+    p = np.concatenate([np.cumsum(np.vstack([p[:1] * 0, np.diff(p[:30], axis=0)]), 0),
+                        np.cumsum(np.vstack([p[30:31] * 0 + 3.0, np.diff(p[30:], axis=0)]), 0)])
+    lat = O.lattice(0, L, 0.0, 1e-3)
+    rx, ry, rz = xyz(p)
+    for fene, kappas, qmax in ((0, 200.0, 2.0), (1, 30.0, 3.0)):
+        st, fb, tb = O.bond_force(lat, 30, 2, rx, ry, rz, fene, kappas, qmax)
+        assert st == 0
+        nl, lst = O.verlet(lat, rx, ry, rz, 6.0)
+        fl, tl = O.lj_force(lat, rx, ry, rz, nl, lst, 25.0, 0.1, 0)
+        for f, tau in ((fb, tb), (fl, tl)):
+            ref = -2.0 * np.array([(p[:, 0] * f[:, 0]).sum(), (p[:, 1] * f[:, 1]).sum(), (p[:, 0] * f[:, 1]).sum()])
+            assert np.abs(tau - ref).max() <= 1e-9 * np.abs(ref).max()
+        assert np.abs(tl).max() > 0 and np.abs(tb).max() > 0
+
+
+def test_chain_observables_closed_forms_and_wrap_invariance():
+    # a straight rod of nb beads spaced b along x: extension (nb-1) b, Rg_xx =
+    # b^2 (nb^2 - 1)/12, centre of mass at -(nb-1) b/2 (the unwrapping adds
+    # getNID(j, j+1) = r_j - r_{j+1}, so chains come out mirrored)
+    lat = O.lattice(1, 40.0, 0.3, 1e-3, 1, 500)
+    nb, b = 12, 1.9
+    x = np.arange(nb) * b - 10.0
+    ext, com, rg = O.chain_obs(lat, nb, 1, x, np.full(nb, 0.4), np.full(nb, -3.0))
+    assert ext[0] == pytest.approx((nb - 1) * b)
+    assert com[0] == pytest.approx([-(nb - 1) * b / 2, 0.0, 0.0])
+    assert rg[0] == pytest.approx([b * b * (nb * nb - 1) / 12, 0, 0, 0, 0, 0], abs=1e-12)
+    # random chains: equal to the moments of the unwrapped coordinates, and unchanged
+    # when beads are moved by lattice vectors (the minimum image undoes the wrap)
+    L, nb, nc = 30.0, 25, 3
+    pos = random_walk(nb, nc, L, 8)
+    unw = np.concatenate([np.cumsum(np.vstack([np.zeros((1, 3)), np.diff(pos[k * nb:(k + 1) * nb], axis=0)
+                                               - L * np.round(np.diff(pos[k * nb:(k + 1) * nb], axis=0) / L)]), 0)
+                          for k in range(nc)])
+    lat = O.lattice(0, L, 0.0, 1e-3)
+    ext, com, rg = O.chain_obs(lat, nb, nc, *xyz(pos))
+    for k in range(nc):
+        u = -unw[k * nb:(k + 1) * nb]
+        assert ext[k] == pytest.approx(u[:, 0].max() - u[:, 0].min(), rel=1e-12)
+        assert com[k] == pytest.approx(u.mean(0), abs=1e-10)
+        c = (u - u.mean(0)).T @ (u - u.mean(0)) / nb
+        assert rg[k] == pytest.approx([c[0, 0], c[0, 1], c[0, 2], c[1, 1], c[1, 2], c[2, 2]], abs=1e-10)
+    shifted = pos + L * np.random.default_rng(3).integers(-2, 3, size=pos.shape)
+    e2, c2, g2 = O.chain_obs(lat, nb, nc, *xyz(shifted))
+    assert np.allclose(e2, ext, rtol=1e-12) and np.allclose(c2, com, atol=1e-10) and np.allclose(g2, rg, atol=1e-10)
+
+
 def test_forces_sum_to_zero():
     L = 30.0
     pos = random_walk(25, 3, L, 2)
