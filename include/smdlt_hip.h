@@ -217,6 +217,15 @@ int smdlt_stress(smdlt_ctx* ctx, double* tau, int reset);
  * lattice (call smdlt_remap(t) first for KRBC).  Any output may be NULL. */
 int smdlt_chain_observables(smdlt_ctx* ctx, double* ext, double* com, double* rg);
 
+/* The charge displacement log (dxt/dyt/dzt), summed since the last reset, per
+ * charge in upload order: disp[3k..3k+2].  Every step adds the tracked bead's
+ * displacement without the flow plus the flow relative to its chain's unwrapped
+ * centre of mass (updateHI/updateFD, S/main26.c:815-838,2436-2446; X/main14.c:
+ * 853-876); every accepted hop adds new bead minus old bead reduced by the cell
+ * (chargeHop, S/main26.c:2221-2233).  calcCH prints these window sums
+ * (S/main26.c:2452-2490); reset != 0 zeroes them after the read. */
+int smdlt_charge_displacements(smdlt_ctx* ctx, double* disp, int reset);
+
 /* TEA parameters for SMDLT_NOISE_TEA steps (the output of ewaldBin/CATEA,
  * S/main26.c:975-1381): beta per replica, C per bead component (3N AoS). */
 int smdlt_set_tea(smdlt_ctx* ctx, const double* beta, const double* C);

@@ -325,6 +325,9 @@ struct smdlt_ctx {
   // observables: per-bead virial terms, per-replica stress sums (xx, yy, xy),
   // first chain of each replica, per-chain extension/com/gyration output
   double *vir = nullptr, *tau = nullptr, *obs = nullptr;
+  // charge displacement log: per-charge sums, their copy at the start of a graph
+  // step, and each bead's offset from its chain's centre of mass (step start)
+  double *qdisp = nullptr, *qdisp_saved = nullptr, *qrel = nullptr;
   int* d_chain0 = nullptr;
   std::vector<int> chain0;
   int nchains = 0, max_chains = 0;
@@ -861,6 +864,7 @@ int smdlt_create(const smdlt_params* params, int n_replicas, int device, smdlt_c
   }
   c->nchains = c->chain0[n_replicas];
   st |= alloc_dev(&c->vir, 3 * (size_t)c->nst);
+  st |= alloc_dev(&c->qrel, 3 * (size_t)c->nst);
   st |= alloc_dev(&c->tau, 3 * (size_t)n_replicas);
   st |= alloc_dev(&c->d_chain0, n_replicas + 1);
   st |= alloc_dev(&c->obs, 10 * (size_t)std::max(1, c->nchains));
@@ -1085,7 +1089,7 @@ void smdlt_destroy(smdlt_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   destroy_graphs(c);
   if (c->h_flags) hipHostFree(c->h_flags);
-  void* ptrs[] = {c->adot_cl, c->tea_beta, c->tea_C, c->tea_diag, c->cl_first, c->cl_cnt, c->slot_cl, c->lcl, c->nclp, c->rtab, c->p32, c->v32, c->mh_feats, c->mh_log, c->mlp_ws, c->mlp_x, c->mlp_y, c->mlp_part, c->mlp_out, c->saved, c->vir, c->tau, c->obs, c->d_chain0, c->d_reps, c->d_slot_rep, c->d_seeds, c->rx, c->ry, c->rz, c->dxv, c->dyv, c->dzv, c->dsp, c->f,
+  void* ptrs[] = {c->adot_cl, c->tea_beta, c->tea_C, c->tea_diag, c->cl_first, c->cl_cnt, c->slot_cl, c->lcl, c->nclp, c->rtab, c->p32, c->v32, c->mh_feats, c->mh_log, c->mlp_ws, c->mlp_x, c->mlp_y, c->mlp_part, c->mlp_out, c->saved, c->vir, c->tau, c->obs, c->d_chain0, c->qdisp, c->qdisp_saved, c->qrel, c->d_reps, c->d_slot_rep, c->d_seeds, c->rx, c->ry, c->rz, c->dxv, c->dyv, c->dzv, c->dsp, c->f,
                   c->z, c->mf, c->bz, c->tmp, c->tmp2, c->nrpy, c->lrpy, c->nlj, c->llj, c->overflow, c->d_err,
                   c->cell_of, c->cell_cnt, c->cell_start, c->cell_fill, c->cell_beads, c->cell_tmp, c->rep_flags, c->out2, c->aff,
                   c->partial2, c->grid, c->bsp, c->tw, c->tx, c->ty, c->tz, c->txy, c->bricks, c->bbase, c->brick_cnt, c->brick_start,
@@ -1448,6 +1452,19 @@ int smdlt_get_verlet(smdlt_ctx* c, int replica, int* nlist, int* list, int strid
     nlist[i] = k;
   }
   return 0;
+}
+
+// The charge displacement log of a step: offsets from the chain centres of mass
+// at the step start, then each charge's displacement after the update.
+static void charge_rel_internal(smdlt_ctx* c) {
+  if (c->nq_total == 0) return;
+  launch_charge_rel(c->stream, c->d_reps, c->cur_rep_begin, c->cur_rep_end - c->cur_rep_begin, c->max_chains, c->rx,
+                    c->ry, c->rz, c->qrel);
+}
+static void charge_log_internal(smdlt_ctx* c) {
+  if (c->nq_total == 0) return;
+  launch_charge_log(c->stream, c->d_reps, c->cur_rep_begin, c->cur_rep_end - c->cur_rep_begin, c->max_nq, c->qtrack,
+                    c->mf, c->bz, c->qrel, c->qdisp);
 }
 
 // accumulate: a step's forces also add their virial to the stress sums (as
@@ -2129,6 +2146,7 @@ static int step_internal(smdlt_ctx* c, uint64_t t, double* noise_dev, bool host_
   }
   st = forces_internal(c, true);
   if (st) return st;
+  charge_rel_internal(c);
   const double* zz = noise_dev;
   {
     StageScope sc(c, ST_NOISE);
@@ -2153,6 +2171,7 @@ static int step_internal(smdlt_ctx* c, uint64_t t, double* noise_dev, bool host_
     StageScope sc(c, ST_UPDATE);
     launch_update_positions(c->stream, c->d_reps, c->d_slot_rep, s0, ns, c->rx, c->ry, c->rz, c->mf, c->bz, c->dxv,
                             c->dyv, c->dzv, c->dsp, 1);
+    charge_log_internal(c);
     c->pos_version++;
   }
   HCK(hipGetLastError());
@@ -2219,6 +2238,12 @@ static void save_state(smdlt_ctx* c, bool restore) {
     launch_copy_f64(c->stream, c->tau, c->saved + off, 3LL * c->nrep);
   else
     launch_copy_f64(c->stream, c->saved + off, c->tau, 3LL * c->nrep);
+  if (c->nq_total > 0) {
+    if (restore)
+      launch_copy_f64(c->stream, c->qdisp, c->qdisp_saved, 3LL * c->nq_total);
+    else
+      launch_copy_f64(c->stream, c->qdisp_saved, c->qdisp, 3LL * c->nq_total);
+  }
 }
 
 // The graphs of one step for the current replica range (see smdlt_ctx).
@@ -2234,6 +2259,7 @@ static int build_step_graphs(smdlt_ctx* c, uint64_t t) {
     int s = neighbors_internal(c, 0);
     if (!s) s = forces_internal(c, true);
     if (s) return s;
+    charge_rel_internal(c);
     launch_noise(c->stream, c->d_slot_rep, c->d_reps, s0, ns, c->d_seeds, t, c->z);
     mobility_internal(c, c->f, c->mf, nullptr, nullptr, nullptr, nullptr);
     lz_start(c, c->z);
@@ -2254,6 +2280,7 @@ static int build_step_graphs(smdlt_ctx* c, uint64_t t) {
     lz_finish(c, c->bz);
     launch_update_positions(c->stream, c->d_reps, c->d_slot_rep, s0, ns, c->rx, c->ry, c->rz, c->mf, c->bz, c->dxv,
                             c->dyv, c->dzv, c->dsp, 1);
+    charge_log_internal(c);
     c->pos_version++;
     return 0;
   });
@@ -2380,10 +2407,11 @@ int smdlt_upload_charges(smdlt_ctx* c, const int* nq, const int* track, const in
       if (sign && sign[k] != 1 && sign[k] != -1) return SMDLT_E_ARG;
     }
   }
-  void* ptrs[] = {c->qtrack, c->qsign, c->qold, c->occ, c->log_off, c->mh_feats, c->mh_log};
+  void* ptrs[] = {c->qtrack, c->qsign, c->qold, c->occ, c->log_off, c->mh_feats, c->mh_log, c->qdisp, c->qdisp_saved};
   for (void* p : ptrs)
     if (p) hipFree(p);
   c->qtrack = c->qsign = c->qold = c->log_off = c->mh_log = nullptr;
+  c->qdisp = c->qdisp_saved = nullptr;
   c->occ = nullptr;
   c->mh_feats = nullptr;
   int st = 0;
@@ -2391,6 +2419,8 @@ int smdlt_upload_charges(smdlt_ctx* c, const int* nq, const int* track, const in
   st |= alloc_dev(&c->qsign, tot);
   st |= alloc_dev(&c->qold, tot);
   st |= alloc_dev(&c->log_off, c->nrep);
+  st |= alloc_dev(&c->qdisp, 3 * (size_t)std::max(1, tot));
+  st |= alloc_dev(&c->qdisp_saved, 3 * (size_t)std::max(1, tot));
   c->occ_stride = ((long long)maxn + 255) / 256 * 256;
   st |= alloc_dev(&c->occ, (size_t)c->occ_stride * c->nrep);
   if (st) return SMDLT_E_NOMEM;
@@ -2402,6 +2432,7 @@ int smdlt_upload_charges(smdlt_ctx* c, const int* nq, const int* track, const in
     HCK(hipMemcpy(c->qsign, sg.data(), 4 * tot, hipMemcpyHostToDevice));
   }
   HCK(hipMemcpy(c->log_off, c->q0.data(), 4 * c->nrep, hipMemcpyHostToDevice));
+  HCK(hipMemset(c->qdisp, 0, sizeof(double) * 3 * (size_t)std::max(1, tot)));
   for (int r = 0; r < c->nrep; ++r) {
     c->rh[r].q0 = c->q0[r];
     c->rh[r].nq = c->nq[r];
@@ -2455,11 +2486,11 @@ int smdlt_mc_sweep(smdlt_ctx* c, int mode, const float* uniforms, int n_uniforms
     launch_mh_sweep(c->stream, c->d_reps, rb, re - rb, c->rx, c->ry, c->rz, c->qtrack, c->qold, c->qsign,
                     (signed char*)c->occ, c->occ_stride, du, uniforms ? n_uniforms : 0, c->d_seeds, c->mc_sweeps++,
                     mode == SMDLT_MC_MH_LEARNED ? c->mlp_w : nullptr, dcons, dlog, c->mh_feats, c->log_off,
-                    c->mc_err);
+                    c->mc_err, c->qdisp);
   else
     launch_mc_sweep(c->stream, c->d_reps, rb, re - rb, c->rx, c->ry, c->rz, c->qtrack, c->qold, c->qsign, c->occ,
                     c->occ_stride, du, uniforms ? n_uniforms : 0, c->d_seeds, c->mc_sweeps++, dcons, dlog, c->log_off,
-                    c->mc_err);
+                    c->mc_err, c->qdisp);
   HCK(hipGetLastError());
   HCK(hipStreamSynchronize(c->stream));
   int e = 0;
@@ -2614,5 +2645,15 @@ int smdlt_chain_observables(smdlt_ctx* c, double* ext, double* com, double* rg) 
       for (int q = 0; q < 6; ++q)
         if (rg) rg[6 * k + q] = h[4 * (size_t)nk + 6 * k + q];
     }
+  return 0;
+}
+
+int smdlt_charge_displacements(smdlt_ctx* c, double* disp, int reset) {
+  if (!c || !disp) return SMDLT_E_ARG;
+  HCK(hipSetDevice(c->device));
+  if (c->nq_total == 0) return 0;
+  HCK(hipMemcpyAsync(disp, c->qdisp, sizeof(double) * 3 * (size_t)c->nq_total, hipMemcpyDeviceToHost, c->stream));
+  HCK(hipStreamSynchronize(c->stream));
+  if (reset) HCK(hipMemset(c->qdisp, 0, sizeof(double) * 3 * (size_t)c->nq_total));
   return 0;
 }

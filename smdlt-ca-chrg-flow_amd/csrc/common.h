@@ -91,6 +91,21 @@ struct FDiv {
   __device__ __forceinline__ int div(int n) const { return d == 1 ? n : (int)__umulhi((unsigned)n, m); }
 };
 
+// chargeHop's displacement of an accepted hop (S/main26.c:2221-2233): new bead
+// minus old bead, reduced by the cell without the shear shift of getNID.
+__device__ __forceinline__ void hop_vector(const RepDev& R, const double* rx, const double* ry, const double* rz,
+                                           int snew, int sold, double d[3]) {
+#pragma clang fp contract(off)
+  double dx = rx[snew] - rx[sold], dy = ry[snew] - ry[sold], dz = rz[snew] - rz[sold];
+  dx -= R.L2xp * round(dy / R.L2yp);
+  dy -= R.L2yp * round(dy / R.L2yp);
+  dx -= R.L1xp * round(dx / R.L1xp);
+  dz -= R.L * round(dz / R.L);
+  d[0] = dx;
+  d[1] = dy;
+  d[2] = dz;
+}
+
 template <typename T> struct CompSum {
   T s, c;
   __device__ __forceinline__ CompSum() : s(0), c(0) {}
@@ -275,6 +290,10 @@ void launch_fill_f32(hipStream_t s, float* p, float v, long long n);
 void launch_forces(hipStream_t s, const RepDev* reps, const int* slot_rep, int slot_begin, int nslots,
                    const double* rx, const double* ry, const double* rz, int nslots_total, const int* nlj,
                    const int* llj, double* f, double* vir, int* err);
+void launch_charge_rel(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, int max_chains, const double* rx,
+                       const double* ry, const double* rz, double* rel);
+void launch_charge_log(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, int max_nq, const int* qtrack,
+                       const double* mf, const double* bz, const double* rel, double* qdisp);
 void launch_virial_acc(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, const double* vir, double* tau);
 void launch_chain_obs(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, int max_chains, const int* chain0,
                       const double* rx, const double* ry, const double* rz, double* ext, double* com, double* rg);
@@ -399,10 +418,10 @@ void launch_mlp_loss_grad(hipStream_t st, const float* w, const float* X, const 
 void launch_mh_sweep(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, const double* rx, const double* ry,
                      const double* rz, int* track, int* old, const int* sign, signed char* occ, long long occ_stride,
                      const float* uniforms, int nu, const uint64_t* seeds, uint64_t sweep_id, const float* weights,
-                     int* consumed, int* log, float* feats, const int* log_off, int* err);
+                     int* consumed, int* log, float* feats, const int* log_off, int* err, double* qdisp);
 void launch_mc_sweep(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, const double* rx, const double* ry,
                      const double* rz, int* track, int* old, const int* sign, unsigned char* occ,
                      long long occ_stride, const float* uniforms, int nu, const uint64_t* seeds, uint64_t sweep_id,
-                     int* consumed, int* log, const int* log_off, int* err);
+                     int* consumed, int* log, const int* log_off, int* err, double* qdisp);
 void launch_mc_energy(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, const double* rx, const double* ry,
                       const double* rz, const int* track, const int* sign, double* e);

@@ -246,6 +246,88 @@ __global__ void k_chain_obs(const RepDev* __restrict__ reps, int rep_begin, cons
   for (int q = 0; q < 6; ++q) rg[6 * k + q] = g[q] / R.nb;
 }
 
+// The charge displacement log of updateHI/updateFD (S/main26.c:815-838,
+// 2436-2446; X/main14.c:853-876).  At the start of a step each bead's offset
+// from its chain's unwrapped centre of mass, rel_j = com - p_j (lab frame, the
+// sequential unwrapping of calcCOM), is stored; after the position update every
+// charge adds its bead's step displacement without the flow (dt M F + noise
+// term, lab frame) plus the flow taken relative to the centre of mass: shear
+// dx += dt edot rel_y; planar extension dx += dt edot rel_x, dy -= dt edot rel_y.
+// (The unwrapping here is a prefix sum, so rel may differ from the sequential
+// walk in the last bits.)
+__global__ __launch_bounds__(256) void k_charge_rel(const RepDev* __restrict__ reps, int rep_begin,
+                                                    const double* __restrict__ rx, const double* __restrict__ ry,
+                                                    const double* __restrict__ rz, double* rel) {
+  // one wavefront per chain: lane j takes getNID(j-1, j) of its 64-bead block and
+  // a shuffle prefix sum (plus the running carry) gives p_j
+  const RepDev& R = reps[rep_begin + blockIdx.y];
+  const int lane = threadIdx.x & 63, ch = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (!R.active || R.nq == 0 || ch >= R.n / R.nb) return;
+  const int b0 = R.slot0 + ch * R.nb;
+  double carry[3] = {0.0, 0.0, 0.0}, sum[3] = {0.0, 0.0, 0.0};
+  for (int base = 0; base < R.nb; base += 64) {
+    const int j = base + lane;
+    double d[3] = {0.0, 0.0, 0.0};
+    if (j > 0 && j < R.nb)
+      ref_nid(R, rx[b0 + j - 1], ry[b0 + j - 1], rz[b0 + j - 1], rx[b0 + j], ry[b0 + j], rz[b0 + j], d[0], d[1],
+              d[2], true);
+    for (int o = 1; o < 64; o <<= 1)
+      for (int k = 0; k < 3; ++k) {
+        const double u = __shfl_up(d[k], o, 64);
+        if (lane >= o) d[k] += u;
+      }
+    for (int k = 0; k < 3; ++k) {
+      const double pj = carry[k] + d[k];
+      if (j < R.nb) {
+        rel[3 * (b0 + j) + k] = pj;
+        sum[k] += pj;
+      }
+      carry[k] += __shfl(d[k], 63, 64);
+    }
+  }
+  double com[3];
+  for (int k = 0; k < 3; ++k) com[k] = warp_sum(sum[k]) / R.nb;
+  for (int j = lane; j < R.nb; j += 64)
+    for (int k = 0; k < 3; ++k) rel[3 * (b0 + j) + k] = com[k] - rel[3 * (b0 + j) + k];
+}
+
+__global__ void k_charge_log(const RepDev* __restrict__ reps, int rep_begin, const int* __restrict__ qtrack,
+                             const double* __restrict__ mf, const double* __restrict__ bz,
+                             const double* __restrict__ rel, double* qdisp) {
+#pragma clang fp contract(off)
+  const RepDev& R = reps[rep_begin + blockIdx.y];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (!R.active || i >= R.nq) return;
+  const int s = R.slot0 + qtrack[R.q0 + i];
+  double d[3];
+  for (int c = 0; c < 3; ++c) d[c] = R.dt * mf[3 * s + c] + R.sqrt2dt * bz[3 * s + c];
+  double dx = d[0], dy = d[1];
+  if (R.flow == 2) {  // the update's rotation to the lab frame
+    dx = d[0] * R.ct - d[1] * R.st;
+    dy = d[0] * R.st + d[1] * R.ct;
+    dx += R.dt * R.edot * rel[3 * s];
+    dy -= R.dt * R.edot * rel[3 * s + 1];
+  } else {
+    dx += R.dt * R.edot * rel[3 * s + 1];
+  }
+  double* o = qdisp + 3 * (R.q0 + i);
+  o[0] += dx;
+  o[1] += dy;
+  o[2] += d[2];
+}
+
+void launch_charge_rel(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, int max_chains, const double* rx,
+                       const double* ry, const double* rz, double* rel) {
+  if (nrep <= 0 || max_chains <= 0) return;
+  k_charge_rel<<<dim3((max_chains + 3) / 4, nrep), 256, 0, st>>>(reps, rep_begin, rx, ry, rz, rel);
+}
+
+void launch_charge_log(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, int max_nq, const int* qtrack,
+                       const double* mf, const double* bz, const double* rel, double* qdisp) {
+  if (nrep <= 0 || max_nq <= 0) return;
+  k_charge_log<<<dim3((max_nq + 255) / 256, nrep), 256, 0, st>>>(reps, rep_begin, qtrack, mf, bz, rel, qdisp);
+}
+
 void launch_chain_obs(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, int max_chains, const int* chain0,
                       const double* rx, const double* ry, const double* rz, double* ext, double* com, double* rg) {
   if (nrep <= 0 || max_chains <= 0) return;

@@ -85,7 +85,7 @@ __global__ __launch_bounds__(MC_THREADS) void k_mc_sweep(const RepDev* __restric
                                                          const int* sign, unsigned char* occ, long long occ_stride,
                                                          const float* uniforms, int nu, const uint64_t* seeds,
                                                          uint64_t sweep_id, int* consumed, int* log,
-                                                         const int* log_off, int* err) {
+                                                         const int* log_off, int* err, double* qdisp) {
   int r = rep_begin + blockIdx.x;
   const RepDev& R = reps[r];
   if (!R.active) return;
@@ -225,6 +225,11 @@ __global__ __launch_bounds__(MC_THREADS) void k_mc_sweep(const RepDev* __restric
         old[R.q0 + k] = c;
         oc[c] = 0;
         oc[target] = 1;
+        if (qdisp) {  // the hop joins the charge's displacement log (dxt of chargeHop)
+          double d[3];
+          hop_vector(R, rx, ry, rz, R.slot0 + target, R.slot0 + c, d);
+          for (int q = 0; q < 3; ++q) qdisp[3 * (R.q0 + k) + q] += d[q];
+        }
       } else {
         old[R.q0 + k] = c;
       }
@@ -258,10 +263,10 @@ __global__ __launch_bounds__(MC_THREADS) void k_mc_sweep(const RepDev* __restric
 void launch_mc_sweep(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, const double* rx, const double* ry,
                      const double* rz, int* track, int* old, const int* sign, unsigned char* occ,
                      long long occ_stride, const float* uniforms, int nu, const uint64_t* seeds, uint64_t sweep_id,
-                     int* consumed, int* log, const int* log_off, int* err) {
+                     int* consumed, int* log, const int* log_off, int* err, double* qdisp) {
   if (nrep <= 0) return;
   k_mc_sweep<<<nrep, MC_THREADS, 0, st>>>(reps, rep_begin, rx, ry, rz, track, old, sign, occ, occ_stride, uniforms, nu,
-                                          seeds, sweep_id, consumed, log, log_off, err);
+                                          seeds, sweep_id, consumed, log, log_off, err, qdisp);
 }
 
 __global__ __launch_bounds__(MC_THREADS) void k_mc_energy(const RepDev* __restrict__ reps, int rep_begin,
@@ -454,7 +459,8 @@ __global__ __launch_bounds__(MC_THREADS) void k_mh_sweep(const RepDev* __restric
                                                          const int* sign, signed char* occ, long long occ_stride,
                                                          const float* uniforms, int nu, const uint64_t* seeds,
                                                          uint64_t sweep_id, const float* weights, int* consumed,
-                                                         int* mlog, float* feats, const int* log_off, int* err) {
+                                                         int* mlog, float* feats, const int* log_off, int* err,
+                                                         double* qdisp) {
   const int r = rep_begin + blockIdx.x;
   const RepDev& R = reps[r];
   if (!R.active) return;
@@ -639,6 +645,11 @@ __global__ __launch_bounds__(MC_THREADS) void k_mh_sweep(const RepDev* __restric
         track[R.q0 + k] = y;
         qb[x] = 0;
         qb[y] = (signed char)qk;
+        if (qdisp) {
+          double d[3];
+          hop_vector(R, rx, ry, rz, R.slot0 + y, R.slot0 + x, d);
+          for (int q = 0; q < 3; ++q) qdisp[3 * (R.q0 + k) + q] += d[q];
+        }
       }
       if (lg) {
         lg[0] = k;
@@ -659,8 +670,8 @@ __global__ __launch_bounds__(MC_THREADS) void k_mh_sweep(const RepDev* __restric
 void launch_mh_sweep(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, const double* rx, const double* ry,
                      const double* rz, int* track, int* old, const int* sign, signed char* occ, long long occ_stride,
                      const float* uniforms, int nu, const uint64_t* seeds, uint64_t sweep_id, const float* weights,
-                     int* consumed, int* log, float* feats, const int* log_off, int* err) {
+                     int* consumed, int* log, float* feats, const int* log_off, int* err, double* qdisp) {
   if (nrep <= 0) return;
   k_mh_sweep<<<nrep, MC_THREADS, 0, st>>>(reps, rep_begin, rx, ry, rz, track, old, sign, occ, occ_stride, uniforms, nu,
-                                          seeds, sweep_id, weights, consumed, log, feats, log_off, err);
+                                          seeds, sweep_id, weights, consumed, log, feats, log_off, err, qdisp);
 }

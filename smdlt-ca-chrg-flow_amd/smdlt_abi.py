@@ -90,6 +90,7 @@ def lib():
         L.smdlt_write_far_table.argtypes = [C.c_char_p, _dp, C.c_int, _ip]
         L.smdlt_stress.argtypes = [C.c_void_p, _dp, C.c_int]
         L.smdlt_chain_observables.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.smdlt_charge_displacements.argtypes = [C.c_void_p, _dp, C.c_int]
         L.smdlt_set_profiling.argtypes = [C.c_void_p, C.c_int]
         L.smdlt_stage_times.argtypes = [C.c_void_p, _dp, _ip, C.c_int]
         L.smdlt_stage_name.restype = C.c_char_p
@@ -304,6 +305,13 @@ class Context:
                                            _ptr(rg, C.c_double)), "chain_observables")
         return ext, com.reshape(nk, 3), rg.reshape(nk, 6)
 
+    def charge_displacements(self, reset=False):
+        """smdlt_charge_displacements: per charge (dx, dy, dz) of the log since the last reset."""
+        tot = int(self.nq.sum()) if hasattr(self, "nq") else 0
+        d = np.zeros(3 * max(tot, 1))
+        _chk(lib().smdlt_charge_displacements(self.h, _ptr(d, C.c_double), int(bool(reset))), "charge_displacements")
+        return d[:3 * tot].reshape(-1, 3)
+
     def set_tea(self, beta, Cv):
         beta = np.ascontiguousarray(np.broadcast_to(beta, (self.nrep,)), np.float64)
         Cv = np.ascontiguousarray(Cv, np.float64).reshape(-1)
@@ -476,6 +484,14 @@ def append_visc(path, t, tau, printperiod, flow_type):
             f.write(f"{t} {tau[0] / printperiod:.6f} {tau[1] / printperiod:.6f}\n")
         else:
             f.write(f"{t} {tau[2] / printperiod:.6f}\n")
+
+
+def append_ch(path, t, old, track, disp):
+    """calcCH (S/main26.c:2452-2490): per charge "t j Charge_old Charge_track dx dy dz"
+    with the window sums of Context.charge_displacements (zeros at t = 0)."""
+    with open(path, "a") as f:
+        for j, (o, k, d) in enumerate(zip(old, track, disp)):
+            f.write(f"{t} {j} {o} {k} {d[0]:.6f} {d[1]:.6f} {d[2]:.6f}\n")
 
 
 def append_ext(path, t, strain, ext, nb, qmax):

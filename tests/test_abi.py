@@ -81,3 +81,10 @@ def test_output_writers_follow_the_reference_formats(tmp_path):
     r = tmp_path / "R.xyz"
     S.write_restart(r, 99, [1.0, 2.0], [3.0, 4.0], [5.0, 6.0])
     assert r.read_text() == "2\n99\nA 1.000000 3.000000 5.000000\nA 2.000000 4.000000 6.000000\n"
+
+
+def test_charge_log_writer_follows_calcch(tmp_path):
+    # calcCH's "%lu %d %d %d %Lf %Lf %Lf" lines (S/main26.c:2463,2480)
+    c = tmp_path / "ch.txt"
+    S.append_ch(c, 2000, [4, 9], [5, 7], np.array([[0.5, -1.25, 0.0], [1e-7, 2.0, -3.5]]))
+    assert c.read_text() == "2000 0 4 5 0.500000 -1.250000 0.000000\n2000 1 9 7 0.000000 2.000000 -3.500000\n"
