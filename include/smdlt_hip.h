@@ -286,6 +286,39 @@ const char* smdlt_stage_name(int stage);
 /* Synchronise the context stream. */
 int smdlt_sync(smdlt_ctx* ctx);
 
+/* ------------------------------------------------------ run set-up (host)
+ * The reference's random stream and initial state, on the host as in the
+ * reference, so that a driver reproduces them from the same seed.  One stream
+ * serves all draws of a run in the reference's order (initChains, then getNoise
+ * and chargeHop every step). */
+typedef struct smdlt_ran1 {
+  int64_t idum, idum2, iy; /* ran1's seed word and its static state */
+  int64_t iv[32];          /* shuffle table */
+  int iset;                /* gasdev holds a second value */
+  float gset;
+} smdlt_ran1;
+/* ran1 with the static initial state of a fresh process and *idum = seed
+ * (830062920 shear, 981272199 planar extension; S/Initialization.c:438,
+ * X/Initialization.c:426). */
+void smdlt_ran1_seed(smdlt_ran1* s, int64_t seed);
+/* ran1 (S/Initialization.c:601-634): the next uniform in (0, 1 - 1.2e-7]. */
+float smdlt_ran1_next(smdlt_ran1* s);
+void smdlt_ran1_fill(smdlt_ran1* s, int n, float* out);
+/* getNoise (S/main26.c:515-540,708-713): n gasdev values (floats) as doubles;
+ * a step's host noise is 3N of them. */
+void smdlt_gasdev_fill(smdlt_ran1* s, int n, double* out);
+/* initChains for a new run (S/main26.c:346-470): nc self-avoiding chains of nb
+ * beads with bonds 2.05 in the cube [-L/2, L/2)^3, positions chain by chain.
+ * SMDLT_E_GEOMETRY if the box is too full. */
+int smdlt_init_chains(int nb, int nc, double L, smdlt_ran1* s, double* rx, double* ry, double* rz);
+/* The charge placement of initChains (S/main26.c:485-500): ncharges distinct
+ * beads of n, in draw order (Charge_track). */
+int smdlt_init_charges(int n, int ncharges, smdlt_ran1* s, int* track);
+/* The restart reader of initChains (S/main26.c:264-275): the last complete
+ * frame of "N", "t", N lines "A x y z" (printRestart's format).  Returns n, or
+ * SMDLT_E_ARG (no file, no frame, or a frame of another size); *t may be NULL. */
+int smdlt_read_restart(const char* path, int n, double* rx, double* ry, double* rz, uint64_t* t);
+
 #ifdef __cplusplus
 }
 #endif

@@ -86,6 +86,10 @@ def lib():
         L.orc_gasdev.restype = C.c_float
         L.orc_ran1_fill.argtypes = [C.POINTER(RNG), C.c_int, _fp]
         L.orc_gasdev_fill.argtypes = [C.POINTER(RNG), C.c_int, _dp]
+        L.orc_init_chains.argtypes = [C.POINTER(RNG), C.c_int, C.c_int, C.c_double, _dp, _dp, _dp]
+        L.orc_init_chains.restype = C.c_int
+        L.orc_init_charges.argtypes = [C.POINTER(RNG), C.c_int, C.c_int, _ip]
+        L.orc_init_charges.restype = None
         L.orc_lattice_init.argtypes = [C.POINTER(Lattice), C.c_int, C.c_double, C.c_double, C.c_double, C.c_int]
         L.orc_lattice_update.argtypes = [C.POINTER(Lattice), C.c_uint64]
         L.orc_apply_pbc.argtypes = [C.POINTER(Lattice), C.c_int, _dp, _dp, _dp]
@@ -157,6 +161,18 @@ class Ran1:
         out = np.empty(n, np.float64)
         lib().orc_gasdev_fill(C.byref(self.s), n, out)
         return out
+
+    def init_chains(self, nb, nc, L):
+        """initChains for a new run (S/main26.c:346-470)."""
+        r = [np.zeros(nb * nc) for _ in range(3)]
+        assert lib().orc_init_chains(C.byref(self.s), nb, nc, L, *r) == 0
+        return r
+
+    def init_charges(self, n, ncharges):
+        """the charge placement of initChains (S/main26.c:485-500)."""
+        track = np.zeros(max(ncharges, 1), np.int32)
+        lib().orc_init_charges(C.byref(self.s), n, ncharges, track)
+        return track[:ncharges]
 
 
 # ---------------------------------------------------------------- lattice

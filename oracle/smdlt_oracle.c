@@ -110,6 +110,97 @@ void orc_gasdev_fill(orc_rng_t* s, int n, double* out) {
   for (int i = 0; i < n; ++i) out[i] = (double)orc_gasdev(s);
 }
 
+/* initChains for a new run (S/main26.c:346-470), in its own control flow: flags
+ * test/inittest and the per-chain attempt count initcount (checked after each
+ * attempt, so the 1001st attempt restarts even when it fits). */
+int orc_init_chains(orc_rng_t* s, int Nb, int Nc, double L, double* rx, double* ry, double* rz) {
+  double box_side = L / 2, dx, dy, dz, theta, phi;
+  int i, j, k, index, monindex, test, initcount, inittest = 0, rounds = 0;
+  while (inittest == 0) {
+    if (++rounds > 100000) return -1;
+    inittest = 1;
+    for (i = 0; i < Nc; ++i) {
+      initcount = 0;
+      test = 0;
+      index = Nb * i;
+      while (test == 0) {
+        test = 1;
+        if (Nc == 1) {
+          rx[index] = 0;
+          ry[index] = 0;
+          rz[index] = 0;
+        } else {
+          rx[index] = orc_ran1(s) * L - box_side;
+          ry[index] = orc_ran1(s) * L - box_side;
+          rz[index] = orc_ran1(s) * L - box_side;
+        }
+        for (k = 0; k < index; k++) {
+          dx = rx[index] - rx[k];
+          dy = ry[index] - ry[k];
+          dz = rz[index] - rz[k];
+          dx -= L * round(dx / L);
+          dy -= L * round(dy / L);
+          dz -= L * round(dz / L);
+          if (dx * dx + dy * dy + dz * dz < 10.0) test = 0;
+        }
+        initcount++;
+        if (initcount > 1e3) {
+          inittest = 0;
+          break;
+        }
+      }
+      if (inittest == 0) break;
+      for (j = 1; j < Nb; j++) {
+        monindex = index + j;
+        test = 0;
+        while (test == 0) {
+          test = 1;
+          theta = orc_ran1(s) * 2.0 * M_PI;
+          phi = acos(2.0 * orc_ran1(s) - 1.0);
+          rx[monindex] = rx[monindex - 1] + 2.05 * sin(phi) * cos(theta);
+          ry[monindex] = ry[monindex - 1] + 2.05 * sin(phi) * sin(theta);
+          rz[monindex] = rz[monindex - 1] + 2.05 * cos(phi);
+          rx[monindex] -= L * round(rx[monindex] / L);
+          ry[monindex] -= L * round(ry[monindex] / L);
+          rz[monindex] -= L * round(rz[monindex] / L);
+          for (k = 0; k < monindex; k++) {
+            dx = rx[monindex] - rx[k];
+            dy = ry[monindex] - ry[k];
+            dz = rz[monindex] - rz[k];
+            dx -= L * round(dx / L);
+            dy -= L * round(dy / L);
+            dz -= L * round(dz / L);
+            if (dx * dx + dy * dy + dz * dz < 4.0) test = 0;
+          }
+          initcount++;
+          if (initcount > 1e3) {
+            inittest = 0;
+            break;
+          }
+        }
+        if (inittest == 0) break;
+      }
+      if (inittest == 0) break;
+    }
+  }
+  return 0;
+}
+
+/* The charge placement of initChains (S/main26.c:485-500). */
+void orc_init_charges(orc_rng_t* s, int N, int Ncharges, int* track) {
+  int* charge = (int*)calloc((size_t)N, sizeof(int));
+  for (int i = 0; i < Ncharges; i++) {
+    int cnumber = orc_ran1(s) * N;
+    if (charge[cnumber] == 0) {
+      charge[cnumber] = 1;
+      track[i] = cnumber;
+    } else {
+      i = i - 1;
+    }
+  }
+  free(charge);
+}
+
 /* ==================================================================== Lattice */
 /* initLattice (S/Initialization.c:217-298, X/Initialization.c:236-287):
  * corner points of a square of side L rotated by 90-(45+THETA0) degrees,

@@ -32,6 +32,8 @@ float orc_gasdev(orc_rng_t* s);
 /* fill out[n] with successive ran1 values */
 void  orc_ran1_fill(orc_rng_t* s, int n, float* out);
 void  orc_gasdev_fill(orc_rng_t* s, int n, double* out);
+int   orc_init_chains(orc_rng_t* s, int Nb, int Nc, double L, double* rx, double* ry, double* rz);
+void  orc_init_charges(orc_rng_t* s, int N, int Ncharges, int* track);
 
 /* ---------------- Lattice (S/main26.c:1382-1457, X/main14.c:1426-1535) */
 typedef struct {

@@ -91,6 +91,15 @@ def lib():
         L.smdlt_stress.argtypes = [C.c_void_p, _dp, C.c_int]
         L.smdlt_chain_observables.argtypes = [C.c_void_p, _dp, _dp, _dp]
         L.smdlt_charge_displacements.argtypes = [C.c_void_p, _dp, C.c_int]
+        L.smdlt_ran1_seed.argtypes = [C.c_void_p, C.c_int64]
+        L.smdlt_ran1_seed.restype = None
+        L.smdlt_ran1_fill.argtypes = [C.c_void_p, C.c_int, _fp]
+        L.smdlt_ran1_fill.restype = None
+        L.smdlt_gasdev_fill.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.smdlt_gasdev_fill.restype = None
+        L.smdlt_init_chains.argtypes = [C.c_int, C.c_int, C.c_double, C.c_void_p, _dp, _dp, _dp]
+        L.smdlt_init_charges.argtypes = [C.c_int, C.c_int, C.c_void_p, _ip]
+        L.smdlt_read_restart.argtypes = [C.c_char_p, C.c_int, _dp, _dp, _dp, C.POINTER(C.c_uint64)]
         L.smdlt_set_profiling.argtypes = [C.c_void_p, C.c_int]
         L.smdlt_stage_times.argtypes = [C.c_void_p, _dp, _ip, C.c_int]
         L.smdlt_stage_name.restype = C.c_char_p
@@ -533,6 +542,50 @@ def write_restart(path, t, rx, ry, rz):
         f.write(f"{len(rx)}\n{t}\n")
         for x, y, z in zip(rx, ry, rz):
             f.write(f"A {x:.6f} {y:.6f} {z:.6f}\n")
+
+
+class Ran1State(C.Structure):
+    _fields_ = [("idum", C.c_int64), ("idum2", C.c_int64), ("iy", C.c_int64), ("iv", C.c_int64 * 32),
+                ("iset", C.c_int), ("gset", C.c_float)]
+
+
+class RefStream:
+    """The reference's ran1/gasdev stream and the initial state it draws
+    (smdlt_ran1_seed/_fill, smdlt_gasdev_fill, smdlt_init_chains, smdlt_init_charges)."""
+
+    def __init__(self, seed=830062920):
+        self.state = Ran1State()
+        lib().smdlt_ran1_seed(C.byref(self.state), seed)
+
+    def uniforms(self, n):
+        out = np.zeros(n, np.float32)
+        lib().smdlt_ran1_fill(C.byref(self.state), n, _ptr(out, C.c_float))
+        return out
+
+    def gauss(self, n):
+        out = np.zeros(n)
+        lib().smdlt_gasdev_fill(C.byref(self.state), n, _ptr(out, C.c_double))
+        return out
+
+    def init_chains(self, nb, nc, L):
+        r = [np.zeros(nb * nc) for _ in range(3)]
+        _chk(lib().smdlt_init_chains(nb, nc, L, C.byref(self.state), *(_ptr(a, C.c_double) for a in r)),
+             "init_chains")
+        return r
+
+    def init_charges(self, n, ncharges):
+        track = np.zeros(max(ncharges, 1), np.int32)
+        _chk(lib().smdlt_init_charges(n, ncharges, C.byref(self.state), _ptr(track, C.c_int)), "init_charges")
+        return track[:ncharges]
+
+
+def read_restart(path, n):
+    """smdlt_read_restart: (rx, ry, rz, t) of the last frame of a printRestart file."""
+    r = [np.zeros(n) for _ in range(3)]
+    t = C.c_uint64(0)
+    _chk(lib().smdlt_read_restart(str(path).encode(), n, *(_ptr(a, C.c_double) for a in r), C.byref(t)),
+         "read_restart")
+    return r[0], r[1], r[2], int(t.value)
 
 
 def write_far_table(path, table, dims):
