@@ -81,7 +81,7 @@ typedef struct smdlt_params {
   double ewald_alpha;      /* splitting parameter xi [1/a]; 0 = auto */
   double rc_real;          /* real-space cutoff [a]; 0 = auto */
   double skin;             /* Verlet skin of the real-space list [a]; 0 = 1.0 */
-  int grid[3];             /* PME grid along a1,a2,a3 (powers of two); 0 = auto */
+  int grid[3];             /* PME grid along a1,a2,a3 (multiples of 8 of the form 2^a 3^b 5^c, 16..512); 0 = auto */
   int spline_order;        /* B-spline order 4..8; 0 = auto */
   double mobility_tol;     /* target relative M.F error for the auto choices; 0 = 1e-5 */
   double lanczos_tol;      /* 0 = 1e-6 */
