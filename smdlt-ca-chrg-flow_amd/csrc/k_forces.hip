@@ -157,14 +157,15 @@ __global__ void k_forces(const RepDev* __restrict__ reps, const int* __restrict_
 
 // tau[r] += the replica's per-bead virial terms, summed in a fixed order (one
 // workgroup per replica: strided per-thread sums, then a tree in LDS).
-__global__ __launch_bounds__(256) void k_virial_acc(const RepDev* __restrict__ reps, int rep_begin,
-                                                    const double* __restrict__ vir, double* tau) {
+#define VIR_TH 1024
+__global__ __launch_bounds__(VIR_TH) void k_virial_acc(const RepDev* __restrict__ reps, int rep_begin,
+                                                       const double* __restrict__ vir, double* tau) {
   const int r = rep_begin + blockIdx.x, t = threadIdx.x;
   const RepDev& R = reps[r];
   if (!R.active) return;
-  __shared__ double red[3][256];
+  __shared__ double red[3][VIR_TH];
   double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-  for (int s = R.slot0 + t; s < R.slot0 + R.n; s += 256) {
+  for (int s = R.slot0 + t; s < R.slot0 + R.n; s += VIR_TH) {
     a0 += vir[3 * s];
     a1 += vir[3 * s + 1];
     a2 += vir[3 * s + 2];
@@ -173,7 +174,7 @@ __global__ __launch_bounds__(256) void k_virial_acc(const RepDev* __restrict__ r
   red[1][t] = a1;
   red[2][t] = a2;
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
+  for (int w = VIR_TH / 2; w > 0; w >>= 1) {
     if (t < w)
       for (int k = 0; k < 3; ++k) red[k][t] += red[k][t + w];
     __syncthreads();
@@ -183,7 +184,7 @@ __global__ __launch_bounds__(256) void k_virial_acc(const RepDev* __restrict__ r
 }
 
 void launch_virial_acc(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, const double* vir, double* tau) {
-  if (nrep > 0) k_virial_acc<<<nrep, 256, 0, st>>>(reps, rep_begin, vir, tau);
+  if (nrep > 0) k_virial_acc<<<nrep, VIR_TH, 0, st>>>(reps, rep_begin, vir, tau);
 }
 
 // Per-chain observables (calcExt, calcCOM, calcRg; S/main26.c:74-93,925-962,
