@@ -1366,9 +1366,11 @@ static int neighbors_internal(smdlt_ctx* c, int force) {
   launch_nl_check(c->stream, c->d_reps, rb, re - rb, c->out2, c->aff, force | (c->lists_built ? 0 : 3), c->rep_flags);
   launch_fill_i32(c->stream, c->cell_cnt + cb, 0, ce - cb);
   launch_fill_i32(c->stream, c->cell_fill + cb, 0, ce - cb);
-  launch_cell_count(c->stream, c->d_reps, c->d_slot_rep, s0, ns, c->rx, c->ry, c->rz, c->cell_of, c->cell_cnt);
+  launch_cell_count(c->stream, c->d_reps, c->d_slot_rep, s0, ns, c->rx, c->ry, c->rz, c->cell_of, c->cell_cnt,
+                    c->rep_flags);
   launch_scan(c->stream, c->cell_cnt + cb, c->cell_start + cb, ce - cb);
-  launch_cell_fill(c->stream, c->d_reps, c->d_slot_rep, s0, ns, c->cell_of, c->cell_start, c->cell_fill, c->cell_tmp);
+  launch_cell_fill(c->stream, c->d_reps, c->d_slot_rep, s0, ns, c->cell_of, c->cell_start, c->cell_fill, c->cell_tmp,
+                   c->rep_flags);
   launch_cell_sort(c->stream, c->cell_start, c->cell_cnt, cb, ce - cb, c->cell_tmp, c->cell_beads);
   launch_build_lists(c->stream, c->d_reps, c->d_slot_rep, s0, ns, c->rx, c->ry, c->rz, c->cell_start, c->cell_cnt,
                      c->cell_beads, c->rep_flags, c->nst, c->cap_rpy, c->nrpy, c->lrpy, c->cap_lj, c->nlj, c->llj,

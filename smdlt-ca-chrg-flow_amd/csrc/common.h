@@ -266,14 +266,16 @@ void launch_rotate_vec(hipStream_t s, const RepDev* reps, const int* slot_rep, i
 
 // k_nlist.hip
 void launch_cell_count(hipStream_t s, const RepDev* reps, const int* slot_rep, int slot_begin, int nslots,
-                       const double* rx, const double* ry, const double* rz, int* cell_of, int* cell_cnt);
+                       const double* rx, const double* ry, const double* rz, int* cell_of, int* cell_cnt,
+                       const int* rep_flags);
 void launch_scan(hipStream_t s, const int* in, int* out, int n);
 void launch_reset_trackers(hipStream_t s, const RepDev* reps, const int* slot_rep, int slot_begin, int nslots,
                            const int* rep_flags, double* dxv, double* dyv, double* dzv, double* dsp);
 void launch_nl_check(hipStream_t s, const RepDev* reps, int rep_begin, int nrep, const double* out2, double* aff,
                      int force, int* rep_flags);
 void launch_cell_fill(hipStream_t s, const RepDev* reps, const int* slot_rep, int slot_begin, int nslots,
-                      const int* cell_of, const int* cell_start, int* cell_fill, int* cell_beads);
+                      const int* cell_of, const int* cell_start, int* cell_fill, int* cell_beads,
+                      const int* rep_flags);
 void launch_cell_sort(hipStream_t s, const int* cell_start, const int* cell_cnt, int cell_begin, int ncells,
                       const int* unsorted, int* cell_beads);
 void launch_build_lists(hipStream_t s, const RepDev* reps, const int* slot_rep, int slot_begin, int nslots,

@@ -22,15 +22,17 @@ __device__ __forceinline__ void cell_coords(const RepDev& R, double x, double y,
   }
 }
 
+// Cells are only read by the list builders, so a replica whose lists are kept
+// this step (rep_flags 0) is not binned: its cells stay empty.
 __global__ void k_cell_count(const RepDev* __restrict__ reps, const int* __restrict__ slot_rep, int slot_begin,
                              int nslots, const double* rx, const double* ry, const double* rz, int* cell_of,
-                             int* cell_cnt) {
+                             int* cell_cnt, const int* __restrict__ rep_flags) {
   int s = slot_begin + blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= slot_begin + nslots) return;
   int r = slot_rep[s];
   if (r < 0) return;
   const RepDev& R = reps[r];
-  if (!R.active) return;
+  if (!R.active || !(rep_flags[r] & 3)) return;
   if (s - R.slot0 >= R.n) {  // padding slot: never binned
     cell_of[s] = -1;
     return;
@@ -43,10 +45,11 @@ __global__ void k_cell_count(const RepDev* __restrict__ reps, const int* __restr
 }
 
 void launch_cell_count(hipStream_t st, const RepDev* reps, const int* slot_rep, int slot_begin, int nslots,
-                       const double* rx, const double* ry, const double* rz, int* cell_of, int* cell_cnt) {
+                       const double* rx, const double* ry, const double* rz, int* cell_of, int* cell_cnt,
+                       const int* rep_flags) {
   if (nslots <= 0) return;
   k_cell_count<<<(nslots + 255) / 256, 256, 0, st>>>(reps, slot_rep, slot_begin, nslots, rx, ry, rz, cell_of,
-                                                     cell_cnt);
+                                                     cell_cnt, rep_flags);
 }
 
 // Exclusive scan of n ints by one workgroup (chunked per thread).
@@ -79,11 +82,12 @@ void launch_scan(hipStream_t st, const int* in, int* out, int n) {
 }
 
 __global__ void k_cell_fill(const RepDev* __restrict__ reps, const int* __restrict__ slot_rep, int slot_begin,
-                            int nslots, const int* cell_of, const int* cell_start, int* cell_fill, int* cell_beads) {
+                            int nslots, const int* cell_of, const int* cell_start, int* cell_fill, int* cell_beads,
+                            const int* __restrict__ rep_flags) {
   int s = slot_begin + blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= slot_begin + nslots) return;
   int r = slot_rep[s];
-  if (r < 0 || !reps[r].active || s - reps[r].slot0 >= reps[r].n) return;
+  if (r < 0 || !reps[r].active || s - reps[r].slot0 >= reps[r].n || !(rep_flags[r] & 3)) return;
   int id = cell_of[s];
   if (id < 0) return;
   int k = atomicAdd(&cell_fill[id], 1);
@@ -91,10 +95,11 @@ __global__ void k_cell_fill(const RepDev* __restrict__ reps, const int* __restri
 }
 
 void launch_cell_fill(hipStream_t st, const RepDev* reps, const int* slot_rep, int slot_begin, int nslots,
-                      const int* cell_of, const int* cell_start, int* cell_fill, int* cell_beads) {
+                      const int* cell_of, const int* cell_start, int* cell_fill, int* cell_beads,
+                      const int* rep_flags) {
   if (nslots <= 0) return;
   k_cell_fill<<<(nslots + 255) / 256, 256, 0, st>>>(reps, slot_rep, slot_begin, nslots, cell_of, cell_start,
-                                                    cell_fill, cell_beads);
+                                                    cell_fill, cell_beads, rep_flags);
 }
 
 // Writes each cell's members in ascending slot order: a member's place is the number of
