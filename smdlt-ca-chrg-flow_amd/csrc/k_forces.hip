@@ -367,9 +367,11 @@ __global__ __launch_bounds__(256) void k_coulomb(const RepDev* __restrict__ reps
     double d[3];
     ref_nid(R, xa, ya, za, rx[ib], ry[ib], rz[ib], d[0], d[1], d[2], false);
     double rr = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-    double rd = sqrt(rr);
-    double Fc = R.lambda_b * (double)(qa * qsign[R.q0 + b]) * exp(-R.kappa_d * rd) * (1 + rd * R.kappa_d) / rr;
-    for (int q = 0; q < 3; ++q) F[q] += Fc * d[q] / rd;
+    // lambda_B qa qb e^{-kappa r} (1 + kappa r) / r^2 along d / r, with one division
+    const double rd = sqrt(rr), ir = 1.0 / rd;
+    const double Fc = R.lambda_b * (double)(qa * qsign[R.q0 + b]) * exp(-R.kappa_d * rd) * (1 + rd * R.kappa_d) *
+                      (ir * ir * ir);
+    for (int q = 0; q < 3; ++q) F[q] += Fc * d[q];
   }
   double fx = warp_sum(F[0]), fy = warp_sum(F[1]), fz = warp_sum(F[2]);
   if (lane == 0) {
