@@ -11,11 +11,13 @@
 #define FFT_TILE_X 1024
 #endif
 #ifndef FFT_TILE_Y
-#define FFT_TILE_Y 1024
+#define FFT_TILE_Y 2048
 #endif
-#ifndef FFT_TILE_Z
+// the z pass has two tiles: the larger one where the grids are large enough to
+// keep at least FFT_Z_LARGE_MIN_TASKS workgroups with it, the smaller one otherwise
 #define FFT_TILE_Z 512
-#endif
+#define FFT_TILE_Z_LARGE 1024
+#define FFT_Z_LARGE_MIN_TASKS 1024
 #define FFT_MAXL 64
 // fused x-y pass: one workgroup of XY_THREADS per (replica, component, z plane)
 // when the plane's half spectrum K0*K1/2 fits XY_ELEMS complex values in LDS
@@ -330,6 +332,7 @@ struct MobArgs {
   const void* tw;           // twiddles (complex T, KMAX)
   const FftTask *tx, *ty, *tz, *txy;
   int ntx, nty, ntz, ntxy;
+  int ztile_large;          // the z tasks hold FFT_TILE_Z_LARGE elements per component (else FFT_TILE_Z)
   int order;                // B-spline order (4, 6 or 8), one per context
   hipEvent_t* ev;           // 9 events bracketing the 8 launches (profiling) or nullptr
   // spreading gather: per-bead stencil base and weights, beads binned by brick

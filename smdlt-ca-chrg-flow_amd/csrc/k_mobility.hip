@@ -689,10 +689,10 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_y(MobArgs a, int dir) {
 
 // z pass: forward FFT of the three components, Green multiply, inverse FFT.
 // Line L = y*Kh + x, element k at k*(K1*Kh) + L.
-template <typename T>
+template <typename T, int TILE>
 __global__ __launch_bounds__(FFT_THREADS) void k_fft_z_green(MobArgs a) {
   typedef typename Prec<T>::C C;
-  __shared__ C sh[3 * (FFT_TILE_Z + FFT_MAXL)];
+  __shared__ C sh[3 * (TILE + FFT_MAXL)];
   __shared__ C tw[KMAX];
   FftTask tk = a.tz[blockIdx.x];
   const RepDev& R = a.reps[tk.rep];
@@ -711,7 +711,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_z_green(MobArgs a) {
     for (int c = 0; c < 3; ++c) sh[c * cpitch + l * pitch + k] = gc[c][idx];
   }
   __syncthreads();
-  fft_lines<C, 3 * FFT_TILE_Z>(sh, K, pitch, 3 * nl, -1, tw);
+  fft_lines<C, 3 * TILE>(sh, K, pitch, 3 * nl, -1, tw);
   const T* bsp = (const T*)a.bsp + R.bsp0;
   const T ia2 = (T)(1.0 / (4.0 * R.alpha * R.alpha));
   const T invV = (T)(1.0 / R.vol);
@@ -755,7 +755,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_z_green(MobArgs a) {
     *p2 = o2;
   }
   __syncthreads();
-  fft_lines<C, 3 * FFT_TILE_Z>(sh, K, pitch, 3 * nl, 1, tw);
+  fft_lines<C, 3 * TILE>(sh, K, pitch, 3 * nl, 1, tw);
   for (int e = threadIdx.x; e < nl * K; e += blockDim.x) {
     int k = dnl.div(e), l = e - k * nl;
     long long idx = k * stride + tk.l0 + l;
@@ -1117,7 +1117,10 @@ template <typename T, int P> static void launch_mob_p(hipStream_t st, const MobA
   mark(3);
   if (a.nty > 0) k_fft_y<T><<<a.nty, FFT_THREADS, 0, st>>>(a, -1);
   mark(4);
-  if (a.ntz > 0) k_fft_z_green<T><<<a.ntz, FFT_THREADS, 0, st>>>(a);
+  if (a.ntz > 0 && a.ztile_large)
+    k_fft_z_green<T, FFT_TILE_Z_LARGE><<<a.ntz, FFT_THREADS, 0, st>>>(a);
+  else if (a.ntz > 0)
+    k_fft_z_green<T, FFT_TILE_Z><<<a.ntz, FFT_THREADS, 0, st>>>(a);
   mark(5);
   if (a.nty > 0) k_fft_y<T><<<a.nty, FFT_THREADS, 0, st>>>(a, 1);
   mark(6);
