@@ -267,7 +267,7 @@ struct smdlt_ctx {
   void* tw = nullptr;
   FftTask *tx = nullptr, *ty = nullptr, *tz = nullptr, *txy = nullptr;
   int ntx = 0, nty = 0, ntz = 0, ntxy = 0;
-  int ztile_large = 0;  // z tasks built with FFT_TILE_Z_LARGE
+  int ztile = FFT_TILE_Z;  // complex elements per component of the z tasks
   BrickTask* bricks = nullptr;
   int nbricks = 0;
   std::vector<int> bk0;  // per-replica brick task ranges (size nrep+1)
@@ -950,16 +950,21 @@ int smdlt_create(const smdlt_params* params, int n_replicas, int device, smdlt_c
     c->tz0.assign(n_replicas + 1, 0);
     c->txy0.assign(n_replicas + 1, 0);
     // The z pass reads runs of nlz consecutive lines, so longer runs coalesce
-    // better; on small grids the larger tile leaves too few workgroups (the
-    // 40x48x40 grid: 39 instead of 80, 7% slower per step), so it is taken only
-    // where the context still has FFT_Z_LARGE_MIN_TASKS of them.
-    long long nz_large = 0;
-    for (int r = 0; r < n_replicas; ++r) {
-      const int nl = std::max(1, std::min(FFT_MAXL, FFT_TILE_Z_LARGE / c->rh[r].K[2]));
-      nz_large += ((long long)c->rh[r].K[1] * (c->rh[r].K[0] / 2) + nl - 1) / nl;
-    }
-    c->ztile_large = nz_large >= FFT_Z_LARGE_MIN_TASKS ? 1 : 0;
-    const int ztile = c->ztile_large ? FFT_TILE_Z_LARGE : FFT_TILE_Z;
+    // better, but on small grids a large tile leaves too few workgroups: the
+    // 40x48x40 grid has 39 with 1024 elements (7% slower per step than 512)
+    // and 160 with 256 (3% faster than 512).
+    auto z_tasks = [&](int tile) {
+      long long n = 0;
+      for (int r = 0; r < n_replicas; ++r) {
+        const int nl = std::max(1, std::min(FFT_MAXL, tile / c->rh[r].K[2]));
+        n += ((long long)c->rh[r].K[1] * (c->rh[r].K[0] / 2) + nl - 1) / nl;
+      }
+      return n;
+    };
+    c->ztile = z_tasks(FFT_TILE_Z_LARGE) >= FFT_Z_LARGE_MIN_TASKS ? FFT_TILE_Z_LARGE
+               : z_tasks(FFT_TILE_Z) >= FFT_Z_MIN_TASKS          ? FFT_TILE_Z
+                                                                 : FFT_TILE_Z_SMALL;
+    const int ztile = c->ztile;
     for (int r = 0; r < n_replicas; ++r) {
       const RepDev& R = c->rh[r];
       int K0 = R.K[0], K1 = R.K[1], K2 = R.K[2], Kh = K0 / 2;
@@ -1574,7 +1579,7 @@ static MobArgs mob_args(smdlt_ctx* c, int rb, int re, const double* v, double* u
   m.ntx = c->tx0[re] - c->tx0[rb];
   m.nty = c->ty0[re] - c->ty0[rb];
   m.ntz = c->tz0[re] - c->tz0[rb];
-  m.ztile_large = c->ztile_large;
+  m.ztile = c->ztile;
   m.txy = c->txy + c->txy0[rb];
   m.ntxy = c->txy0[re] - c->txy0[rb];
   m.order = c->rh[rb].order;

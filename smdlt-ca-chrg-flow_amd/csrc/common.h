@@ -13,11 +13,14 @@
 #ifndef FFT_TILE_Y
 #define FFT_TILE_Y 2048
 #endif
-// the z pass has two tiles: the larger one where the grids are large enough to
-// keep at least FFT_Z_LARGE_MIN_TASKS workgroups with it, the smaller one otherwise
+// the z pass has three tiles, the largest that keeps enough workgroups on the
+// context's grids: FFT_TILE_Z_LARGE with at least FFT_Z_LARGE_MIN_TASKS of them,
+// FFT_TILE_Z with at least FFT_Z_MIN_TASKS, else FFT_TILE_Z_SMALL
+#define FFT_TILE_Z_SMALL 256
 #define FFT_TILE_Z 512
 #define FFT_TILE_Z_LARGE 1024
 #define FFT_Z_LARGE_MIN_TASKS 1024
+#define FFT_Z_MIN_TASKS 256
 #define FFT_MAXL 64
 // fused x-y pass: one workgroup of XY_THREADS per (replica, component, z plane)
 // when the plane's half spectrum K0*K1/2 fits XY_ELEMS complex values in LDS
@@ -332,7 +335,7 @@ struct MobArgs {
   const void* tw;           // twiddles (complex T, KMAX)
   const FftTask *tx, *ty, *tz, *txy;
   int ntx, nty, ntz, ntxy;
-  int ztile_large;          // the z tasks hold FFT_TILE_Z_LARGE elements per component (else FFT_TILE_Z)
+  int ztile;                // complex elements per component of a z task: FFT_TILE_Z_SMALL, FFT_TILE_Z or FFT_TILE_Z_LARGE
   int order;                // B-spline order (4, 6 or 8), one per context
   hipEvent_t* ev;           // 9 events bracketing the 8 launches (profiling) or nullptr
   // spreading gather: per-bead stencil base and weights, beads binned by brick

@@ -1117,10 +1117,12 @@ template <typename T, int P> static void launch_mob_p(hipStream_t st, const MobA
   mark(3);
   if (a.nty > 0) k_fft_y<T><<<a.nty, FFT_THREADS, 0, st>>>(a, -1);
   mark(4);
-  if (a.ntz > 0 && a.ztile_large)
+  if (a.ntz > 0 && a.ztile == FFT_TILE_Z_LARGE)
     k_fft_z_green<T, FFT_TILE_Z_LARGE><<<a.ntz, FFT_THREADS, 0, st>>>(a);
-  else if (a.ntz > 0)
+  else if (a.ntz > 0 && a.ztile == FFT_TILE_Z)
     k_fft_z_green<T, FFT_TILE_Z><<<a.ntz, FFT_THREADS, 0, st>>>(a);
+  else if (a.ntz > 0)
+    k_fft_z_green<T, FFT_TILE_Z_SMALL><<<a.ntz, FFT_THREADS, 0, st>>>(a);
   mark(5);
   if (a.nty > 0) k_fft_y<T><<<a.nty, FFT_THREADS, 0, st>>>(a, 1);
   mark(6);
