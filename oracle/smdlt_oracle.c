@@ -186,12 +186,13 @@ int orc_init_chains(orc_rng_t* s, int Nb, int Nc, double L, double* rx, double* 
   return 0;
 }
 
-/* The charge placement of initChains (S/main26.c:485-500). */
+/* The charge placement of initChains (S/main26.c:485-500).  A draw that the
+   float product rounds up to N (possible for N above 2^22) is redrawn. */
 void orc_init_charges(orc_rng_t* s, int N, int Ncharges, int* track) {
   int* charge = (int*)calloc((size_t)N, sizeof(int));
   for (int i = 0; i < Ncharges; i++) {
     int cnumber = orc_ran1(s) * N;
-    if (charge[cnumber] == 0) {
+    if (cnumber < N && charge[cnumber] == 0) {
       charge[cnumber] = 1;
       track[i] = cnumber;
     } else {

@@ -156,7 +156,9 @@ int smdlt_init_chains(int nb, int nc, double L, smdlt_ran1* s, double* rx, doubl
 }
 
 // The charges of initChains (S/main26.c:485-500): ncharges distinct beads, each
-// drawn as (int)(ran1 * N) in float and redrawn when already charged.
+// drawn as (int)(ran1 * N) in float and redrawn when already charged.  For N
+// above 2^22 the float product of ran1's largest value and N can round to N;
+// such a draw is also redrawn (the reference would index past its arrays).
 int smdlt_init_charges(int n, int ncharges, smdlt_ran1* s, int* track) {
   if (n < 1 || ncharges < 0 || ncharges > n || !s || (!track && ncharges)) return SMDLT_E_ARG;
   std::vector<char> charged(n, 0);
@@ -164,7 +166,7 @@ int smdlt_init_charges(int n, int ncharges, smdlt_ran1* s, int* track) {
     int c;
     do {
       c = (int)(smdlt_ran1_next(s) * (float)n);
-    } while (charged[c]);
+    } while (c >= n || charged[c]);
     charged[c] = 1;
     track[i] = c;
   }
