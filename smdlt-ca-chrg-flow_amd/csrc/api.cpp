@@ -2641,7 +2641,11 @@ int smdlt_stress(smdlt_ctx* c, double* tau, int reset) {
   HCK(hipSetDevice(c->device));
   HCK(hipMemcpyAsync(tau, c->tau, sizeof(double) * 3 * (size_t)c->nrep, hipMemcpyDeviceToHost, c->stream));
   HCK(hipStreamSynchronize(c->stream));
-  if (reset) HCK(hipMemset(c->tau, 0, sizeof(double) * 3 * (size_t)c->nrep));
+  if (reset) {  // on the context stream, so the next step's sums start after it
+    launch_fill_f64(c->stream, c->tau, 0.0, 3LL * c->nrep);
+    HCK(hipGetLastError());
+    HCK(hipStreamSynchronize(c->stream));
+  }
   return 0;
 }
 
@@ -2674,6 +2678,10 @@ int smdlt_charge_displacements(smdlt_ctx* c, double* disp, int reset) {
   if (c->nq_total == 0) return 0;
   HCK(hipMemcpyAsync(disp, c->qdisp, sizeof(double) * 3 * (size_t)c->nq_total, hipMemcpyDeviceToHost, c->stream));
   HCK(hipStreamSynchronize(c->stream));
-  if (reset) HCK(hipMemset(c->qdisp, 0, sizeof(double) * 3 * (size_t)c->nq_total));
+  if (reset) {
+    launch_fill_f64(c->stream, c->qdisp, 0.0, 3LL * c->nq_total);
+    HCK(hipGetLastError());
+    HCK(hipStreamSynchronize(c->stream));
+  }
   return 0;
 }

@@ -344,46 +344,90 @@ void launch_forces(hipStream_t st, const RepDev* reps, const int* slot_rep, int 
                                                  llj, f, vir, err);
 }
 
-// Screened Coulomb between all charged beads of a replica: one wavefront per
-// charge, lanes stride over the partners, FP64 sums combined across the
-// wavefront.  qtrack holds bead indices local to the replica (Charge_track).
-__global__ __launch_bounds__(256) void k_coulomb(const RepDev* __restrict__ reps, int rep_begin,
-                                                 const double* __restrict__ rx, const double* __restrict__ ry,
-                                                 const double* __restrict__ rz, const int* __restrict__ qtrack,
-                                                 const int* __restrict__ qsign, double* f) {
+// Screened Coulomb between all charged beads of a replica.  Each wavefront
+// holds COUL_PER_WAVE charges and its lanes stride over the partners; the
+// workgroup stages the partners' positions and signs in LDS tile by tile
+// (gathered through qtrack once per workgroup instead of once per charge), so
+// the pair loop reads no global memory.  Lane l visits partners l, l + 64, ...
+// in increasing order whatever the tiling (COUL_TILE is a multiple of 64), and
+// FP64 sums are combined across the wavefront, so the forces do not depend on
+// the launch shape.  qtrack holds bead indices local to the replica
+// (Charge_track).
+#define COUL_TILE 1024
+#define COUL_PER_WAVE 2
+#define COUL_WAVES 4
+__global__ __launch_bounds__(64 * COUL_WAVES) void k_coulomb(const RepDev* __restrict__ reps, int rep_begin,
+                                                            const double* __restrict__ rx,
+                                                            const double* __restrict__ ry,
+                                                            const double* __restrict__ rz,
+                                                            const int* __restrict__ qtrack,
+                                                            const int* __restrict__ qsign, double* f) {
+  __shared__ double sx[COUL_TILE], sy[COUL_TILE], sz[COUL_TILE];
+  __shared__ int ss[COUL_TILE];
   const RepDev& R = reps[rep_begin + blockIdx.y];
-  if (!R.active) return;
-  const int lane = threadIdx.x & 63;
-  const int a = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int nq = R.nq;
-  if (a >= nq) return;
-  const int ia = R.slot0 + qtrack[R.q0 + a];
-  const double xa = rx[ia], ya = ry[ia], za = rz[ia];
-  const int qa = qsign[R.q0 + a];
-  double F[3] = {0, 0, 0};
-  for (int b = lane; b < nq; b += 64) {
-    if (b == a) continue;
-    int ib = R.slot0 + qtrack[R.q0 + b];
-    double d[3];
-    ref_nid(R, xa, ya, za, rx[ib], ry[ib], rz[ib], d[0], d[1], d[2], false);
-    double rr = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-    // lambda_B qa qb e^{-kappa r} (1 + kappa r) / r^2 along d / r, with one division
-    const double rd = sqrt(rr), ir = 1.0 / rd;
-    const double Fc = R.lambda_b * (double)(qa * qsign[R.q0 + b]) * exp(-R.kappa_d * rd) * (1 + rd * R.kappa_d) *
-                      (ir * ir * ir);
-    for (int q = 0; q < 3; ++q) F[q] += Fc * d[q];
+  const int a0 = blockIdx.x * (COUL_WAVES * COUL_PER_WAVE);
+  if (!R.active || a0 >= nq) return;  // uniform across the workgroup
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int ia[COUL_PER_WAVE], qa[COUL_PER_WAVE], an[COUL_PER_WAVE];
+  double xa[COUL_PER_WAVE], ya[COUL_PER_WAVE], za[COUL_PER_WAVE];
+  double F[COUL_PER_WAVE][3];
+#pragma unroll
+  for (int u = 0; u < COUL_PER_WAVE; ++u) {
+    const int a = a0 + w * COUL_PER_WAVE + u;
+    an[u] = a < nq ? a : -1;  // -1: no charge (never equal to a partner index)
+    const int ac = a < nq ? a : 0;
+    ia[u] = R.slot0 + qtrack[R.q0 + ac];
+    qa[u] = qsign[R.q0 + ac];
+    xa[u] = rx[ia[u]];
+    ya[u] = ry[ia[u]];
+    za[u] = rz[ia[u]];
+    F[u][0] = F[u][1] = F[u][2] = 0.0;
   }
-  double fx = warp_sum(F[0]), fy = warp_sum(F[1]), fz = warp_sum(F[2]);
-  if (lane == 0) {
-    f[3 * ia] += fx;
-    f[3 * ia + 1] += fy;
-    f[3 * ia + 2] += fz;
+  for (int t0 = 0; t0 < nq; t0 += COUL_TILE) {
+    const int m = min(COUL_TILE, nq - t0);
+    if (t0 > 0) __syncthreads();  // the previous tile is consumed
+    for (int k = threadIdx.x; k < m; k += 64 * COUL_WAVES) {
+      const int ib = R.slot0 + qtrack[R.q0 + t0 + k];
+      sx[k] = rx[ib];
+      sy[k] = ry[ib];
+      sz[k] = rz[ib];
+      ss[k] = qsign[R.q0 + t0 + k];
+    }
+    __syncthreads();
+    for (int k = lane; k < m; k += 64) {
+      const int b = t0 + k;
+      const double xb = sx[k], yb = sy[k], zb = sz[k];
+      const int qb = ss[k];
+#pragma unroll
+      for (int u = 0; u < COUL_PER_WAVE; ++u) {
+        if (b == an[u] || an[u] < 0) continue;
+        double d[3];
+        ref_nid(R, xa[u], ya[u], za[u], xb, yb, zb, d[0], d[1], d[2], false);
+        double rr = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        // lambda_B qa qb e^{-kappa r} (1 + kappa r) / r^2 along d / r, with one division
+        const double rd = sqrt(rr), ir = 1.0 / rd;
+        const double Fc = R.lambda_b * (double)(qa[u] * qb) * exp(-R.kappa_d * rd) * (1 + rd * R.kappa_d) *
+                          (ir * ir * ir);
+        for (int q = 0; q < 3; ++q) F[u][q] += Fc * d[q];
+      }
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < COUL_PER_WAVE; ++u) {
+    const double fx = warp_sum(F[u][0]), fy = warp_sum(F[u][1]), fz = warp_sum(F[u][2]);
+    if (lane == 0 && an[u] >= 0) {
+      f[3 * ia[u]] += fx;
+      f[3 * ia[u] + 1] += fy;
+      f[3 * ia[u] + 2] += fz;
+    }
   }
 }
 
 void launch_coulomb(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, const double* rx, const double* ry,
                     const double* rz, const int* qtrack, const int* qsign, double* f, int max_nq) {
   if (nrep <= 0 || max_nq <= 0) return;
-  dim3 grid((max_nq + 3) / 4, nrep);
-  k_coulomb<<<grid, 256, 0, st>>>(reps, rep_begin, rx, ry, rz, qtrack, qsign, f);
+  const int per = COUL_WAVES * COUL_PER_WAVE;
+  dim3 grid((max_nq + per - 1) / per, nrep);
+  k_coulomb<<<grid, 64 * COUL_WAVES, 0, st>>>(reps, rep_begin, rx, ry, rz, qtrack, qsign, f);
 }
