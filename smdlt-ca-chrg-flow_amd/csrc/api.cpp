@@ -961,10 +961,11 @@ int smdlt_create(const smdlt_params* params, int n_replicas, int device, smdlt_c
       }
       return n;
     };
-    c->ztile = z_tasks(FFT_TILE_Z_LARGE) >= FFT_Z_LARGE_MIN_TASKS ? FFT_TILE_Z_LARGE
-               : z_tasks(FFT_TILE_Z) >= FFT_Z_MIN_TASKS          ? FFT_TILE_Z
-                                                                 : FFT_TILE_Z_SMALL;
-    const int ztile = c->ztile;
+    const bool fp64 = c->prec == SMDLT_PREC_FP64;
+    c->ztile = (!fp64 && z_tasks(FFT_TILE_Z_LARGE) >= FFT_Z_LARGE_MIN_TASKS) ? FFT_TILE_Z_LARGE
+               : z_tasks(FFT_TILE_Z) >= FFT_Z_MIN_TASKS                     ? FFT_TILE_Z
+                                                                            : FFT_TILE_Z_SMALL;
+    const int ztile = c->ztile, ytile = fp64 ? FFT_TILE_Y64 : FFT_TILE_Y;
     for (int r = 0; r < n_replicas; ++r) {
       const RepDev& R = c->rh[r];
       int K0 = R.K[0], K1 = R.K[1], K2 = R.K[2], Kh = K0 / 2;
@@ -973,7 +974,7 @@ int smdlt_create(const smdlt_params* params, int n_replicas, int device, smdlt_c
       c->tz0[r] = (int)vz.size();
       c->txy0[r] = (int)vxy.size();
       int nlx = std::max(1, std::min(FFT_MAXL, FFT_TILE_X / K0));
-      int nly = std::max(1, std::min(FFT_MAXL, FFT_TILE_Y / K1));
+      int nly = std::max(1, std::min(FFT_MAXL, ytile / K1));
       int nlz = std::max(1, std::min(FFT_MAXL, ztile / K2));
       const bool fused = K0 * K1 / 2 <= XY_ELEMS && gtot <= XY_MAX_POINTS;  // the plane fits one workgroup's LDS
       for (int comp = 0; comp < 3; ++comp) {

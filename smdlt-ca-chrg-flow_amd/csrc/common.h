@@ -13,9 +13,11 @@
 #ifndef FFT_TILE_Y
 #define FFT_TILE_Y 2048
 #endif
+#define FFT_TILE_Y64 1024  // FP64 grids: twice the bytes per value, so the 2048 tile would cut occupancy to 3 waves/SIMD
 // the z pass has three tiles, the largest that keeps enough workgroups on the
-// context's grids: FFT_TILE_Z_LARGE with at least FFT_Z_LARGE_MIN_TASKS of them,
-// FFT_TILE_Z with at least FFT_Z_MIN_TASKS, else FFT_TILE_Z_SMALL
+// context's grids: FFT_TILE_Z_LARGE with at least FFT_Z_LARGE_MIN_TASKS of them
+// (FP32 grids only: in FP64 it runs at 2 waves/SIMD), FFT_TILE_Z with at least
+// FFT_Z_MIN_TASKS, else FFT_TILE_Z_SMALL
 #define FFT_TILE_Z_SMALL 256
 #define FFT_TILE_Z 512
 #define FFT_TILE_Z_LARGE 1024

@@ -663,7 +663,8 @@ __global__ __launch_bounds__(XY_THREADS) void k_fft_xy(MobArgs a, int dir) {
 template <typename T>
 __global__ __launch_bounds__(FFT_THREADS) void k_fft_y(MobArgs a, int dir) {
   typedef typename Prec<T>::C C;
-  __shared__ C sh[FFT_TILE_Y + FFT_MAXL];
+  constexpr int TILE = sizeof(T) == 4 ? FFT_TILE_Y : FFT_TILE_Y64;
+  __shared__ C sh[TILE + FFT_MAXL];
   __shared__ C tw[KMAX];
   FftTask tk = a.ty[blockIdx.x];
   const RepDev& R = a.reps[tk.rep];
@@ -679,7 +680,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_y(MobArgs a, int dir) {
     sh[l * pitch + k] = gc[((long long)z * K + k) * Kh + x];
   }
   __syncthreads();
-  fft_lines<C, FFT_TILE_Y>(sh, K, pitch, nl, dir, tw);
+  fft_lines<C, TILE>(sh, K, pitch, nl, dir, tw);
   for (int e = threadIdx.x; e < nl * K; e += blockDim.x) {
     int k = dnl.div(e), l = e - k * nl;
     int L = tk.l0 + l, z = dKh.div(L), x = L - z * Kh;
