@@ -961,10 +961,17 @@ int smdlt_create(const smdlt_params* params, int n_replicas, int device, smdlt_c
       }
       return n;
     };
+    // A z task holds nl >= 1 whole lines of every replica, so a tile is only
+    // eligible when the longest z line fits it (FFT_TILE_Z >= KMAX always does):
+    // a shorter tile would overrun the LDS lines and the butterfly slots.
     const bool fp64 = c->prec == SMDLT_PREC_FP64;
+    int kzmax = 0;
+    for (int r = 0; r < n_replicas; ++r) kzmax = std::max(kzmax, c->rh[r].K[2]);
     c->ztile = (!fp64 && z_tasks(FFT_TILE_Z_LARGE) >= FFT_Z_LARGE_MIN_TASKS) ? FFT_TILE_Z_LARGE
-               : z_tasks(FFT_TILE_Z) >= FFT_Z_MIN_TASKS                     ? FFT_TILE_Z
-                                                                            : FFT_TILE_Z_SMALL;
+               : (z_tasks(FFT_TILE_Z) >= FFT_Z_MIN_TASKS || kzmax > FFT_TILE_Z_SMALL) ? FFT_TILE_Z
+                                                                                      : FFT_TILE_Z_SMALL;
+    static_assert(FFT_TILE_Z >= KMAX && FFT_TILE_X >= KMAX && FFT_TILE_Y64 >= KMAX && FFT_TILE_Y >= KMAX,
+                  "every FFT tile except FFT_TILE_Z_SMALL must hold a whole line of the longest length");
     const int ztile = c->ztile, ytile = fp64 ? FFT_TILE_Y64 : FFT_TILE_Y;
     for (int r = 0; r < n_replicas; ++r) {
       const RepDev& R = c->rh[r];

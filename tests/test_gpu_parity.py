@@ -176,7 +176,8 @@ def test_mobility_fp64_equals_pme_restatement(flow):
     assert rel(u, un) < 1e-12
 
 
-@pytest.mark.parametrize("grid", [(40, 48, 72), (24, 80, 120), (96, 32, 40), (160, 144, 16), (16, 40, 160)])
+@pytest.mark.parametrize("grid", [(40, 48, 72), (24, 80, 120), (96, 32, 40), (160, 144, 16), (16, 40, 160),
+                                  (16, 16, 384), (16, 24, 320)])
 def test_mobility_mixed_radix_grids_equal_pme_restatement(grid):
     # line lengths 2^a 3^b 5^c exercise the radix-3 and radix-5 Stockham stages; planes
     # with K0*K1/2 <= 8192 take the fused x-y pass, (160, 144, 16) the separate x and y passes
@@ -199,6 +200,19 @@ def test_mobility_matches_dense_ewald(flow, prec):
     for seed in (2, 3):
         v = np.random.default_rng(seed).normal(size=pos.size)
         assert rel(ctx.mobility_apply(v), Md @ v) < 1e-5
+
+
+@pytest.mark.parametrize("grid", [(16, 16, 384), (16, 24, 320)])
+def test_mobility_long_z_lines_few_tasks_mixed(grid):
+    # few z lines of a length above the smallest z tile (256 values): the z pass
+    # must keep a tile that holds a whole line
+    ctx, lat, pos = mob_case(1, grid=grid, spline_order=6)
+    cfg = ctx.mobility_config()
+    assert cfg["grid"] == grid
+    v = np.random.default_rng(14).normal(size=pos.size)
+    un = P.mobility_apply(pos, v, (lat.L1xp, lat.L2xp, lat.L2yp, lat.L), cfg["alpha"], cfg["rc"], cfg["grid"],
+                          cfg["order"])
+    assert rel(ctx.mobility_apply(v), un) < 1e-5
 
 
 def test_mobility_split_invariance():
