@@ -252,6 +252,20 @@ int smdlt_lanczos_status(smdlt_ctx* ctx, int* iters, int* status, double* change
  * accepted, draws}, concatenated. */
 int smdlt_mc_sweep(smdlt_ctx* ctx, int mode, const float* uniforms, int n_uniforms, int* consumed,
                    int* log);
+/* smdlt_mc_sweep without allocating: no counterpart in the reference.  For
+ * callers that cannot allocate device memory per call (tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, results and errors as
+ * smdlt_mc_sweep; the uniforms, consumed counts and reference-mode log are
+ * staged in one library-owned device array per GPU (held until the process
+ * ends) instead of per-call allocations.  Needs
+ *   align256(4 nrep) + align256(24 nq_total) [reference mode only]
+ *   + 4 n_uniforms nrep [host-fed uniforms only] <= SMDLT_MC_NOALLOC_MAX_BYTES
+ * (align256 rounds up to a multiple of 256), else SMDLT_E_ARG.  All contexts
+ * on one GPU share the array: calls that overlap in time on one GPU (from
+ * several host threads) race; calls in sequence do not. */
+#define SMDLT_MC_NOALLOC_MAX_BYTES (8 << 20)
+int smdlt_mc_sweep_noalloc(smdlt_ctx* ctx, int mode, const float* uniforms, int n_uniforms, int* consumed,
+                           int* log);
 /* Screened-Coulomb energy per replica (the double loop of chargeHop, S/main26.c:1955-1970). */
 int smdlt_mc_energy(smdlt_ctx* ctx, double* energy);
 

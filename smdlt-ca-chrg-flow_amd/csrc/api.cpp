@@ -12,6 +12,7 @@
 #include <cstring>
 #include <string>
 #include <functional>
+#include <mutex>
 #include <vector>
 
 #include "common.h"
@@ -30,6 +31,8 @@ static std::string& last_hip_error() {
   static thread_local std::string s;
   return s;
 }
+
+static int resolve_scratch(int device);  // device address of smdlt_mc_sweep_noalloc's buffer
 
 static const double kEP = 0.9624;  // lattice reset strain (S/Parameters.h:37)
 
@@ -298,6 +301,7 @@ struct smdlt_ctx {
   float* mlp_w = nullptr;     // BF16-rounded proposal network weights (FP32 storage)
   float* mh_feats = nullptr;  // features of each MH attempt of the last sweep [nq_total][16]
   int* mh_log = nullptr;      // its log [nq_total][6]
+  bool mh_swept = false;      // an MH sweep has filled mh_feats / mh_log
   float* mlp_ws = nullptr;    // training: weights, batch, per-block partial gradients
   float* mlp_x = nullptr;
   float* mlp_y = nullptr;
@@ -719,6 +723,10 @@ int smdlt_create(const smdlt_params* params, int n_replicas, int device, smdlt_c
   if (!params || n_replicas <= 0 || !out) return SMDLT_E_ARG;
   *out = nullptr;
   HCK(hipSetDevice(device));
+  {
+    int st0 = resolve_scratch(device);  // smdlt_mc_sweep_noalloc's buffer on this device
+    if (st0) return st0;
+  }
   smdlt_ctx* c = new smdlt_ctx();
   c->device = device;
   c->nrep = n_replicas;
@@ -2442,8 +2450,11 @@ int smdlt_upload_charges(smdlt_ctx* c, const int* nq, const int* track, const in
   c->qdisp = c->qdisp_saved = nullptr;
   c->occ = nullptr;
   c->mh_feats = nullptr;
+  c->mh_swept = false;
   int st = 0;
   st |= alloc_dev(&c->qtrack, tot);
+  st |= alloc_dev(&c->mh_feats, 16 * (size_t)std::max(1, tot));  // MH sweeps' attempt records
+  st |= alloc_dev(&c->mh_log, 6 * (size_t)std::max(1, tot));
   st |= alloc_dev(&c->qsign, tot);
   st |= alloc_dev(&c->qold, tot);
   st |= alloc_dev(&c->log_off, c->nrep);
@@ -2480,56 +2491,117 @@ int smdlt_download_charges(smdlt_ctx* c, int* track, int* old) {
   return 0;
 }
 
-int smdlt_mc_sweep(smdlt_ctx* c, int mode, const float* uniforms, int n_uniforms, int* consumed, int* log) {
-  if (!c) return SMDLT_E_ARG;
-  if (!c->have_pos) return SMDLT_E_STATE;
-  if (mode != SMDLT_MC_REFERENCE && mode != SMDLT_MC_MH_UNIFORM && mode != SMDLT_MC_MH_LEARNED) return SMDLT_E_ARG;
-  if (mode == SMDLT_MC_MH_LEARNED && !c->mlp_w) return SMDLT_E_STATE;
-  if (uniforms && n_uniforms <= 0) return SMDLT_E_ARG;
-  HCK(hipSetDevice(c->device));
-  int st = upload_reps(c);
-  if (st) return st;
-  StageScope sc(c, ST_MC);
-  float* du = nullptr;
-  int* dcons = nullptr;
-  int* dlog = nullptr;
+// The sweep of smdlt_mc_sweep and smdlt_mc_sweep_noalloc on device buffers:
+// du (the replicas' host-fed uniforms, or null), dcons (nrep ints) and dlog
+// (reference mode: 6 ints per charge, or null).  The MH modes log into the
+// context's own buffers.
+static int mc_sweep_internal(smdlt_ctx* c, int mode, const float* du, int n_uniforms, int* dcons, int* dlog,
+                             int* consumed, int* log) {
   const bool mh = mode != SMDLT_MC_REFERENCE;
-  if (uniforms) {
-    HCK(hipMalloc(&du, sizeof(float) * (size_t)n_uniforms * c->nrep));
-    HCK(hipMemcpy(du, uniforms, sizeof(float) * (size_t)n_uniforms * c->nrep, hipMemcpyHostToDevice));
-  }
-  HCK(hipMalloc(&dcons, 4 * c->nrep));
-  if (mh) {
-    if (!c->mh_feats && c->nq_total) {
-      if (alloc_dev(&c->mh_feats, 16 * (size_t)c->nq_total) || alloc_dev(&c->mh_log, 6 * (size_t)c->nq_total))
-        return SMDLT_E_NOMEM;
-    }
-    dlog = c->mh_log;
-  } else if (log && c->nq_total) {
-    HCK(hipMalloc(&dlog, 24 * (size_t)c->nq_total));
-  }
-  HCK(hipMemset(c->mc_err, 0, 4));
+  if (mh) dlog = c->mh_log;
+  HCK(hipMemsetAsync(c->mc_err, 0, 4, c->stream));
   int rb = c->cur_rep_begin, re = c->cur_rep_end;
   if (mh)
     launch_mh_sweep(c->stream, c->d_reps, rb, re - rb, c->rx, c->ry, c->rz, c->qtrack, c->qold, c->qsign,
-                    (signed char*)c->occ, c->occ_stride, du, uniforms ? n_uniforms : 0, c->d_seeds, c->mc_sweeps++,
+                    (signed char*)c->occ, c->occ_stride, du, du ? n_uniforms : 0, c->d_seeds, c->mc_sweeps++,
                     mode == SMDLT_MC_MH_LEARNED ? c->mlp_w : nullptr, dcons, dlog, c->mh_feats, c->log_off,
                     c->mc_err, c->qdisp);
   else
     launch_mc_sweep(c->stream, c->d_reps, rb, re - rb, c->rx, c->ry, c->rz, c->qtrack, c->qold, c->qsign, c->occ,
-                    c->occ_stride, du, uniforms ? n_uniforms : 0, c->d_seeds, c->mc_sweeps++, dcons, dlog, c->log_off,
+                    c->occ_stride, du, du ? n_uniforms : 0, c->d_seeds, c->mc_sweeps++, dcons, dlog, c->log_off,
                     c->mc_err, c->qdisp);
   HCK(hipGetLastError());
+  if (mh) c->mh_swept = true;
   HCK(hipStreamSynchronize(c->stream));
   int e = 0;
   HCK(hipMemcpy(&e, c->mc_err, 4, hipMemcpyDeviceToHost));
   if (consumed) HCK(hipMemcpy(consumed, dcons, 4 * c->nrep, hipMemcpyDeviceToHost));
   if (log && dlog) HCK(hipMemcpy(log, dlog, 24 * (size_t)c->nq_total, hipMemcpyDeviceToHost));
-  if (du) hipFree(du);
-  hipFree(dcons);
-  if (dlog && !mh) hipFree(dlog);
   if (e & 8) return SMDLT_E_GEOMETRY;
   return (e & 2) ? SMDLT_E_MC_STREAM : 0;
+}
+
+static int mc_sweep_check(smdlt_ctx* c, int mode, const float* uniforms, int n_uniforms) {
+  if (!c) return SMDLT_E_ARG;
+  if (!c->have_pos) return SMDLT_E_STATE;
+  if (mode != SMDLT_MC_REFERENCE && mode != SMDLT_MC_MH_UNIFORM && mode != SMDLT_MC_MH_LEARNED) return SMDLT_E_ARG;
+  if (mode == SMDLT_MC_MH_LEARNED && !c->mlp_w) return SMDLT_E_STATE;
+  if (uniforms && n_uniforms <= 0) return SMDLT_E_ARG;
+  if (mode != SMDLT_MC_REFERENCE && c->nq_total && !c->mh_feats) return SMDLT_E_STATE;
+  return 0;
+}
+
+int smdlt_mc_sweep(smdlt_ctx* c, int mode, const float* uniforms, int n_uniforms, int* consumed, int* log) {
+  int st = mc_sweep_check(c, mode, uniforms, n_uniforms);
+  if (st) return st;
+  HCK(hipSetDevice(c->device));
+  st = upload_reps(c);
+  if (st) return st;
+  StageScope sc(c, ST_MC);
+  float* du = nullptr;
+  int* dcons = nullptr;
+  int* dlog = nullptr;
+  if (uniforms) {
+    HCK(hipMalloc(&du, sizeof(float) * (size_t)n_uniforms * c->nrep));
+    HCK(hipMemcpy(du, uniforms, sizeof(float) * (size_t)n_uniforms * c->nrep, hipMemcpyHostToDevice));
+  }
+  HCK(hipMalloc(&dcons, 4 * c->nrep));
+  if (mode == SMDLT_MC_REFERENCE && log && c->nq_total) HCK(hipMalloc(&dlog, 24 * (size_t)c->nq_total));
+  st = mc_sweep_internal(c, mode, du, n_uniforms, dcons, dlog, consumed, log);
+  if (du) hipFree(du);
+  hipFree(dcons);
+  if (dlog) hipFree(dlog);
+  return st;
+}
+
+// ------------------------------------------------ allocation-free MC sweep
+// smdlt_mc_sweep_noalloc stages the host-fed uniforms, the consumed counts and
+// the reference-mode log in `scratch`, one file-scope array per GPU, instead of
+// allocating them per call.  Its device address is looked up once per device
+// by smdlt_create (with that device current) and kept in g_scratch_ptr; the
+// array's name is a host-side stand-in and is never passed to a kernel or a
+// copy.  Layout: consumed [nrep] ints, the log [6 nq_total] ints, the uniforms
+// [nrep][n_uniforms] floats, each part starting on a 256-byte boundary.
+static __device__ __align__(256) unsigned char scratch[SMDLT_MC_NOALLOC_MAX_BYTES];
+#define SCRATCH_MAX_DEVICES 64
+static std::mutex g_scratch_mu;
+static unsigned char* g_scratch_ptr[SCRATCH_MAX_DEVICES];
+
+// Called with `device` current.
+static int resolve_scratch(int device) {
+  if (device < 0 || device >= SCRATCH_MAX_DEVICES) return SMDLT_E_ARG;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return 0;
+  void* p = nullptr;
+  HCK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) return SMDLT_E_HIP;
+  g_scratch_ptr[device] = (unsigned char*)p;
+  return 0;
+}
+
+static size_t align256(size_t n) { return (n + 255) / 256 * 256; }
+
+int smdlt_mc_sweep_noalloc(smdlt_ctx* c, int mode, const float* uniforms, int n_uniforms, int* consumed, int* log) {
+  int st = mc_sweep_check(c, mode, uniforms, n_uniforms);
+  if (st) return st;
+  const size_t b_cons = align256(4 * (size_t)c->nrep);
+  const size_t b_log = (mode == SMDLT_MC_REFERENCE) ? align256(24 * (size_t)c->nq_total) : 0;
+  const size_t b_u = uniforms ? 4 * (size_t)n_uniforms * c->nrep : 0;
+  if (b_cons + b_log + b_u > (size_t)SMDLT_MC_NOALLOC_MAX_BYTES) return SMDLT_E_ARG;
+  HCK(hipSetDevice(c->device));
+  unsigned char* base = (c->device >= 0 && c->device < SCRATCH_MAX_DEVICES) ? g_scratch_ptr[c->device] : nullptr;
+  if (!base) return SMDLT_E_STATE;
+  st = upload_reps(c);
+  if (st) return st;
+  int* dcons = (int*)base;
+  int* dlog = b_log ? (int*)(base + b_cons) : nullptr;
+  float* du = uniforms ? (float*)(base + b_cons + b_log) : nullptr;
+  // the array keeps the previous call's contents: replicas outside the active
+  // range report 0 consumed, and skipped log records read as zeros
+  HCK(hipMemsetAsync(dcons, 0, b_cons, c->stream));
+  if (dlog) HCK(hipMemsetAsync(dlog, 0, b_log, c->stream));
+  if (du) HCK(hipMemcpyAsync(du, uniforms, b_u, hipMemcpyHostToDevice, c->stream));
+  return mc_sweep_internal(c, mode, du, n_uniforms, dcons, dlog, consumed, log);
 }
 
 int smdlt_mc_energy(smdlt_ctx* c, double* energy) {
@@ -2615,7 +2687,7 @@ int smdlt_mlp_loss_grad(smdlt_ctx* c, const float* w, const float* x, const floa
 int smdlt_mc_features(smdlt_ctx* c, int replica, float* features, int* from, int* to, int* accepted, int cap,
                       int* n) {
   if (!c || replica < 0 || replica >= c->nrep || !n) return SMDLT_E_ARG;
-  if (!c->mh_feats) return SMDLT_E_STATE;
+  if (!c->mh_feats || !c->mh_swept) return SMDLT_E_STATE;
   HCK(hipSetDevice(c->device));
   const int nq = c->nq[replica], off = c->q0[replica];
   *n = nq;

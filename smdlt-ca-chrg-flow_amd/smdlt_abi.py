@@ -107,6 +107,7 @@ def lib():
         L.smdlt_sync.argtypes = [C.c_void_p]
         L.smdlt_select_replicas.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.smdlt_mc_sweep.argtypes = [C.c_void_p, C.c_int, _fp, C.c_int, _ip, _ip]
+        L.smdlt_mc_sweep_noalloc.argtypes = [C.c_void_p, C.c_int, _fp, C.c_int, _ip, _ip]
         L.smdlt_mc_energy.argtypes = [C.c_void_p, _dp]
         L.smdlt_mlp_set_weights.argtypes = [C.c_void_p, _fp]
         L.smdlt_mlp_loss_grad.argtypes = [C.c_void_p, _fp, C.c_void_p, C.c_void_p, C.c_int, _fp, _fp]
@@ -365,18 +366,19 @@ class Context:
         _chk(lib().smdlt_download_charges(self.h, _ptr(tr, C.c_int), _ptr(old, C.c_int)), "download_charges")
         return tr[:tot], old[:tot]
 
-    def mc_sweep(self, mode=MC_REFERENCE, uniforms=None):
+    def mc_sweep(self, mode=MC_REFERENCE, uniforms=None, noalloc=False):
+        """chargeHop for every replica; noalloc=True calls smdlt_mc_sweep_noalloc."""
         tot = int(self.nq.sum())
         consumed = np.zeros(self.nrep, np.int32)
         log = np.zeros(6 * max(tot, 1), np.int32)
+        fn = lib().smdlt_mc_sweep_noalloc if noalloc else lib().smdlt_mc_sweep
         if uniforms is None:
-            code = lib().smdlt_mc_sweep(self.h, mode, None, 0, _ptr(consumed, C.c_int), _ptr(log, C.c_int))
+            code = fn(self.h, mode, None, 0, _ptr(consumed, C.c_int), _ptr(log, C.c_int))
         else:
             u = np.ascontiguousarray(uniforms, np.float32)
             nu = u.shape[-1]
-            code = lib().smdlt_mc_sweep(self.h, mode, _ptr(u, C.c_float), nu, _ptr(consumed, C.c_int),
-                                        _ptr(log, C.c_int))
-        _chk(code, "mc_sweep")
+            code = fn(self.h, mode, _ptr(u, C.c_float), nu, _ptr(consumed, C.c_int), _ptr(log, C.c_int))
+        _chk(code, "mc_sweep_noalloc" if noalloc else "mc_sweep")
         return consumed, log[:6 * tot].reshape(-1, 6)
 
     def mlp_set_weights(self, w):
