@@ -120,6 +120,8 @@ def lib():
         L.orc_catea.restype = C.c_double
         L.orc_far_table.argtypes = [C.POINTER(Lattice), C.c_int, C.POINTER(Bins), C.c_double, _dp]
         L.orc_rpy_ewald_dense.argtypes = [C.POINTER(Cell), C.c_int, _dp, C.c_double, C.c_double, C.c_double, C.c_int, _dp]
+        L.orc_rpy_ewald_matvec.argtypes = [C.POINTER(Cell), C.c_int, _dp, C.c_double, C.c_double, C.c_double, C.c_int,
+                                           _dp, _dp]
         L.orc_rpy_free_dense.argtypes = [C.c_int, _dp, _dp]
         L.orc_ewald_selfconst.argtypes = [C.c_double]
         L.orc_ewald_selfconst.restype = C.c_double
@@ -298,6 +300,57 @@ def ewald_dense(cell, pos, alpha, rcut=None, kcut=None, overlap_mode=0):
     M = np.zeros((3 * n) * (3 * n))
     lib().orc_rpy_ewald_dense(C.byref(cell), n, pos, alpha, rcut, kcut, overlap_mode, M)
     return M.reshape(3 * n, 3 * n)
+
+
+def ewald_dense_alpha(cell):
+    """Splitting parameter that minimises the per-pair work of ewald_dense
+    (real-space images within 6.2/alpha plus wave vectors within 12.4 alpha)."""
+    a1 = np.hypot(cell.a1x, cell.a1y)
+    a2 = np.hypot(cell.a2x, cell.a2y)
+    area = abs(cell.a1x * cell.a2y - cell.a1y * cell.a2x)
+    V = area * cell.Lz
+    w = np.array([area / a2, area / a1, cell.Lz])
+    best = None
+    for alpha in np.geomspace(0.03, 3.0, 80):
+        rcut, kcut = 6.2 / alpha, 12.4 * alpha
+        images = np.prod(2 * (np.ceil(rcut / w) + 1) + 1)
+        nk = 4.19 * kcut ** 3 * V / (2 * np.pi) ** 3
+        if best is None or images + nk < best[0]:
+            best = (images + nk, alpha)
+    return best[1]
+
+
+def ewald_matvec_alpha(cell, n):
+    """Splitting parameter that minimises a rough cost model of ewald_matvec
+    (pair distance tests and real-space terms against structure-factor terms)."""
+    a1 = np.hypot(cell.a1x, cell.a1y)
+    a2 = np.hypot(cell.a2x, cell.a2y)
+    area = abs(cell.a1x * cell.a2y - cell.a1y * cell.a2x)
+    V = area * cell.Lz
+    w = np.array([area / max(a1, a2), area / min(a1, a2), cell.Lz])
+    best = None
+    for alpha in np.geomspace(0.03, 3.0, 80):
+        rcut, kcut = 6.2 / alpha, 12.4 * alpha
+        nimg = np.prod(2 * np.floor(rcut / w + 0.5) + 1)
+        real = n * n * nimg * 2e-9 + n * min(n * nimg, 4.19 * rcut ** 3 * n / V) * 6e-8
+        nk = 4.19 * kcut ** 3 * V / (2 * np.pi) ** 3 / 2
+        cost = real + 2 * n * nk * 4e-9
+        if best is None or cost < best[0]:
+            best = (cost, alpha)
+    return best[1]
+
+
+def ewald_matvec(cell, pos, v, alpha=None, rcut=None, kcut=None, overlap_mode=0):
+    """u = M v for the matrix of ewald_dense, computed without forming M."""
+    pos = np.ascontiguousarray(pos, np.float64).reshape(-1)
+    n = len(pos) // 3
+    alpha = alpha if alpha is not None else ewald_matvec_alpha(cell, n)
+    rcut = rcut if rcut is not None else 6.2 / alpha
+    kcut = kcut if kcut is not None else 12.4 * alpha
+    u = np.zeros(3 * n)
+    lib().orc_rpy_ewald_matvec(C.byref(cell), n, pos, alpha, rcut, kcut, overlap_mode,
+                               np.ascontiguousarray(v, np.float64), u)
+    return u
 
 
 def rpy_free_dense(pos):

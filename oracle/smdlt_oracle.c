@@ -792,6 +792,170 @@ void orc_rpy_ewald_dense(const orc_cell_t* c, int n, const double* pos, double a
   free(kv);
 }
 
+/* u = M v for the Ewald-summed RPY matrix of orc_rpy_ewald_dense without forming
+ * M, so that systems of 10^4-10^5 beads can be checked.  Same split and terms,
+ * arranged differently:
+ *  - real space: the pair vector is wrapped to fractional coordinates in
+ *    [-1/2, 1/2] of a Gauss-reduced basis; images p with |s_i + p_i| below
+ *    rcut/w_i + 1/2 (w_i the width across basis vector i) are the only ones that
+ *    can lie within rcut, so p_i runs over [-m_i, m_i], m_i = floor(rcut/w_i + 1/2);
+ *  - reciprocal space: cos(k.(r_j - r_i)) = Re(e^{i k.r_j} e^{-i k.r_i}), so
+ *    sum_j M2(k) v_j cos(..) = Re(conj(e^{i k.r_i}) S(k)) with the structure
+ *    factor S(k) = sum_j v_j e^{i k.r_j}; k and -k give equal terms, so the half
+ *    space is summed twice.  e^{i k.r} is the product of per-bead phase tables
+ *    e^{i m_d b_d.r} (one sincos per bead and index);
+ *  - the self term selfconst * v_i.
+ * Deterministic: each structure factor and each u_i is summed by one thread in
+ * a fixed order. */
+void orc_rpy_ewald_matvec(const orc_cell_t* c0, int n, const double* pos, double alpha, double rcut, double kcut,
+                          int overlap_mode, const double* v, double* u) {
+  /* Gauss-reduce the in-plane basis */
+  double a1[2] = {c0->a1x, c0->a1y}, a2[2] = {c0->a2x, c0->a2y};
+  for (int it = 0; it < 100; ++it) {
+    double n1 = a1[0] * a1[0] + a1[1] * a1[1], n2 = a2[0] * a2[0] + a2[1] * a2[1];
+    if (n2 < n1) {
+      double t0 = a1[0], t1 = a1[1];
+      a1[0] = a2[0];
+      a1[1] = a2[1];
+      a2[0] = t0;
+      a2[1] = t1;
+      n1 = n2;
+    }
+    double mu = round((a1[0] * a2[0] + a1[1] * a2[1]) / n1);
+    if (mu == 0.0) break;
+    a2[0] -= mu * a1[0];
+    a2[1] -= mu * a1[1];
+  }
+  const double Lz = c0->Lz;
+  const double area = a1[0] * a2[1] - a1[1] * a2[0], aarea = fabs(area), V = aarea * Lz;
+  const double la1 = sqrt(a1[0] * a1[0] + a1[1] * a1[1]), la2 = sqrt(a2[0] * a2[0] + a2[1] * a2[1]);
+  const double w[3] = {aarea / la2, aarea / la1, Lz};
+  int m[3];
+  for (int d = 0; d < 3; ++d) m[d] = (int)floor(rcut / w[d] + 0.5);
+  const double self = orc_ewald_selfconst(alpha);
+  /* real space and self term */
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int i = 0; i < n; ++i) {
+    double acc[3] = {self * v[3 * i], self * v[3 * i + 1], self * v[3 * i + 2]};
+    for (int j = 0; j < n; ++j) {
+      double dx = pos[3 * j] - pos[3 * i], dy = pos[3 * j + 1] - pos[3 * i + 1], dz = pos[3 * j + 2] - pos[3 * i + 2];
+      double s1 = (dx * a2[1] - dy * a2[0]) / area, s2 = (a1[0] * dy - a1[1] * dx) / area, s3 = dz / Lz;
+      s1 -= round(s1);
+      s2 -= round(s2);
+      s3 -= round(s3);
+      for (int p1 = -m[0]; p1 <= m[0]; ++p1)
+        for (int p2 = -m[1]; p2 <= m[1]; ++p2)
+          for (int p3 = -m[2]; p3 <= m[2]; ++p3) {
+            double f1 = s1 + p1, f2 = s2 + p2, f3 = s3 + p3;
+            if (i == j && f1 == 0.0 && f2 == 0.0 && f3 == 0.0) continue;
+            double d[3] = {f1 * a1[0] + f2 * a2[0], f1 * a1[1] + f2 * a2[1], f3 * Lz};
+            double rr = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            if (rr >= rcut * rcut) continue;
+            double r = sqrt(rr), t1, t2;
+            orc_rpy_real_coeffs(r, alpha, overlap_mode, &t1, &t2);
+            double dv = (d[0] * v[3 * j] + d[1] * v[3 * j + 1] + d[2] * v[3 * j + 2]) * t2 / rr;
+            for (int a = 0; a < 3; ++a) acc[a] += t1 * v[3 * j + a] + dv * d[a];
+          }
+    }
+    for (int a = 0; a < 3; ++a) u[3 * i + a] = acc[a];
+  }
+  /* reciprocal basis b_i . a_j = 2 pi delta_ij */
+  const double b1[2] = {2 * M_PI * a2[1] / area, -2 * M_PI * a2[0] / area};
+  const double b2[2] = {-2 * M_PI * a1[1] / area, 2 * M_PI * a1[0] / area};
+  const double b3 = 2 * M_PI / Lz;
+  const int M1 = (int)ceil(kcut * la1 / (2 * M_PI)) + 1, M2 = (int)ceil(kcut * la2 / (2 * M_PI)) + 1,
+            M3 = (int)ceil(kcut * Lz / (2 * M_PI)) + 1;
+  /* half space: m1 > 0, or m1 = 0 and m2 > 0, or m1 = m2 = 0 and m3 > 0 */
+  int cap = (M1 + 1) * (2 * M2 + 1) * (2 * M3 + 1), nk = 0;
+  int* km = (int*)malloc(sizeof(int) * 3 * (size_t)cap);
+  double* kw = (double*)malloc(sizeof(double) * 4 * (size_t)cap);
+  const double a2s = alpha * alpha, a4s = a2s * a2s;
+  for (int q1 = 0; q1 <= M1; ++q1)
+    for (int q2 = -M2; q2 <= M2; ++q2)
+      for (int q3 = -M3; q3 <= M3; ++q3) {
+        if (q1 == 0 && (q2 < 0 || (q2 == 0 && q3 <= 0))) continue;
+        double kx = q1 * b1[0] + q2 * b2[0], ky = q1 * b1[1] + q2 * b2[1], kz = q3 * b3;
+        double kk = kx * kx + ky * ky + kz * kz;
+        if (kk >= kcut * kcut) continue;
+        km[3 * nk] = q1;
+        km[3 * nk + 1] = q2;
+        km[3 * nk + 2] = q3;
+        kw[4 * nk] = kx;
+        kw[4 * nk + 1] = ky;
+        kw[4 * nk + 2] = kz;
+        kw[4 * nk + 3] = 2.0 * (1.0 - kk / 3.0) * (1.0 + kk / (4.0 * a2s) + kk * kk / (8.0 * a4s)) * 6.0 * M_PI / kk *
+                         exp(-kk / (4.0 * a2s)) / V;
+        nk++;
+      }
+  /* phase tables [index][bead] of e^{i m b_d . r} */
+  const int n1t = M1 + 1, n2t = 2 * M2 + 1, n3t = 2 * M3 + 1;
+  double* T1 = (double*)malloc(sizeof(double) * 2 * (size_t)n1t * n);
+  double* T2 = (double*)malloc(sizeof(double) * 2 * (size_t)n2t * n);
+  double* T3 = (double*)malloc(sizeof(double) * 2 * (size_t)n3t * n);
+#pragma omp parallel for
+  for (int j = 0; j < n; ++j) {
+    double p1 = b1[0] * pos[3 * j] + b1[1] * pos[3 * j + 1], p2 = b2[0] * pos[3 * j] + b2[1] * pos[3 * j + 1],
+           p3 = b3 * pos[3 * j + 2];
+    for (int q = 0; q < n1t; ++q) {
+      T1[2 * ((size_t)q * n + j)] = cos(q * p1);
+      T1[2 * ((size_t)q * n + j) + 1] = sin(q * p1);
+    }
+    for (int q = -M2; q <= M2; ++q) {
+      T2[2 * ((size_t)(q + M2) * n + j)] = cos(q * p2);
+      T2[2 * ((size_t)(q + M2) * n + j) + 1] = sin(q * p2);
+    }
+    for (int q = -M3; q <= M3; ++q) {
+      T3[2 * ((size_t)(q + M3) * n + j)] = cos(q * p3);
+      T3[2 * ((size_t)(q + M3) * n + j) + 1] = sin(q * p3);
+    }
+  }
+  /* projected, weighted structure factors S'(k) = w(k) (I - k k^T/k^2) S(k) */
+  double* Sk = (double*)malloc(sizeof(double) * 6 * (size_t)(nk ? nk : 1));
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int q = 0; q < nk; ++q) {
+    const double* t1 = T1 + 2 * (size_t)km[3 * q] * n;
+    const double* t2 = T2 + 2 * (size_t)(km[3 * q + 1] + M2) * n;
+    const double* t3 = T3 + 2 * (size_t)(km[3 * q + 2] + M3) * n;
+    double sr[3] = {0, 0, 0}, si[3] = {0, 0, 0};
+    for (int j = 0; j < n; ++j) {
+      double er = t1[2 * j] * t2[2 * j] - t1[2 * j + 1] * t2[2 * j + 1];
+      double ei = t1[2 * j] * t2[2 * j + 1] + t1[2 * j + 1] * t2[2 * j];
+      double fr = er * t3[2 * j] - ei * t3[2 * j + 1], fi = er * t3[2 * j + 1] + ei * t3[2 * j];
+      for (int a = 0; a < 3; ++a) {
+        sr[a] += v[3 * j + a] * fr;
+        si[a] += v[3 * j + a] * fi;
+      }
+    }
+    const double* k = kw + 4 * q;
+    double kk = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
+    double pr = (k[0] * sr[0] + k[1] * sr[1] + k[2] * sr[2]) / kk, pi = (k[0] * si[0] + k[1] * si[1] + k[2] * si[2]) / kk;
+    for (int a = 0; a < 3; ++a) {
+      Sk[6 * q + a] = k[3] * (sr[a] - pr * k[a]);
+      Sk[6 * q + 3 + a] = k[3] * (si[a] - pi * k[a]);
+    }
+  }
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int i = 0; i < n; ++i) {
+    double acc[3] = {0, 0, 0};
+    for (int q = 0; q < nk; ++q) {
+      const double* t1 = T1 + 2 * ((size_t)km[3 * q] * n + i);
+      const double* t2 = T2 + 2 * ((size_t)(km[3 * q + 1] + M2) * n + i);
+      const double* t3 = T3 + 2 * ((size_t)(km[3 * q + 2] + M3) * n + i);
+      double er = t1[0] * t2[0] - t1[1] * t2[1], ei = t1[0] * t2[1] + t1[1] * t2[0];
+      double fr = er * t3[0] - ei * t3[1], fi = er * t3[1] + ei * t3[0];
+      /* Re(S' conj(e)) */
+      for (int a = 0; a < 3; ++a) acc[a] += Sk[6 * q + a] * fr + Sk[6 * q + 3 + a] * fi;
+    }
+    for (int a = 0; a < 3; ++a) u[3 * i + a] += acc[a];
+  }
+  free(Sk);
+  free(T1);
+  free(T2);
+  free(T3);
+  free(km);
+  free(kw);
+}
+
 /* The reciprocal-space sum of calcHI and ewaldBin, restated with its range logic
  * (S/main26.c:1593-1652 and 1000-1097, X/main14.c:1034-1131): the wave vectors
  * (kx >= 0 half space in the reference's index names, shear rkx from ky and rky
@@ -1329,26 +1493,103 @@ int orc_lanczos_sqrt(int n3, const double* A, const double* z, double tol, int m
 /* ========================================================== Charge-hopping MC */
 static void rebuild_charge(orc_mc_state_t* s, const int* from) {
   for (int i = 0; i < s->n; ++i) s->charge[i] = 0;
-  for (int i = 0; i < s->n; ++i)
-    for (int j = 0; j < s->nq; ++j)
-      if (i == from[j]) s->charge[i] = 1;
+  for (int j = 0; j < s->nq; ++j) s->charge[from[j]] = 1;
 }
 
+/* Pair terms of the double loop of chargeHop for unit charges,
+ * (lambda_b*1*1/r)*exp(-kappa_d*r), cached per bead pair for one sweep (the
+ * positions do not change during a sweep).  A term for charges qi, qj in
+ * {-1, 1} is qi*qj times the cached value exactly (round to nearest is
+ * symmetric under negation).  Slots are given to beads as they first carry a
+ * charge; g holds NaN until a pair is evaluated. */
+typedef struct {
+  int cap, used;
+  int* slot;  /* bead -> slot or -1 */
+  double* g;  /* cap x cap */
+} pair_cache_t;
+
+static double pair_term(const orc_lattice_t* lat, const double* rx, const double* ry, const double* rz, int i, int j,
+                        double lambda_b, double kappa_d) {
+  double d[3];
+  orc_nid(lat, rx, ry, rz, i, j, d);
+  double rr = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  double r = sqrt(rr);
+  return (lambda_b * 1 * 1 / r) * exp(-kappa_d * r);
+}
+
+static int cache_slot(pair_cache_t* c, int bead) {
+  if (c->slot[bead] < 0) {
+    if (c->used >= c->cap) return -1;
+    c->slot[bead] = c->used++;
+  }
+  return c->slot[bead];
+}
+
+/* E = sum over bead pairs i < j, both charged, in the reference's loop order
+ * (S/main26.c:1955-1970, 2181-2197).  With a cache the terms of new pairs are
+ * evaluated (in parallel) before the sequential sum; without one, in the loop. */
 static double mc_energy(orc_mc_state_t* s, const orc_lattice_t* lat, const double* rx, const double* ry,
-                        const double* rz, double lambda_b, double kappa_d) {
+                        const double* rz, double lambda_b, double kappa_d, pair_cache_t* pc, int* list) {
   double E = 0;
-  if (s->nq > 1) {
-    for (int i = 0; i < s->n; ++i)
-      for (int j = i + 1; j < s->n; ++j)
-        if (s->charge[i] != 0 && s->charge[j] != 0) {
-          double d[3];
-          orc_nid(lat, rx, ry, rz, i, j, d);
-          double rr = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-          double r = sqrt(rr);
-          E += (lambda_b * s->charge[i] * s->charge[j] / r) * exp(-kappa_d * r);
-        }
+  if (s->nq <= 1) return E;
+  int q = 0;
+  for (int i = 0; i < s->n; ++i)
+    if (s->charge[i] != 0) list[q++] = i;
+  int cached = pc != NULL;
+  if (cached)
+    for (int a = 0; a < q && cached; ++a) cached = cache_slot(pc, list[a]) >= 0;
+  if (!cached) {
+    for (int a = 0; a < q; ++a)
+      for (int b = a + 1; b < q; ++b) {
+        int i = list[a], j = list[b];
+        E += s->charge[i] * s->charge[j] * pair_term(lat, rx, ry, rz, i, j, lambda_b, kappa_d);
+      }
+    return E;
+  }
+  const size_t cap = (size_t)pc->cap;
+#pragma omp parallel for schedule(dynamic, 8)
+  for (int a = 0; a < q; ++a) {
+    double* row = pc->g + (size_t)pc->slot[list[a]] * cap;
+    for (int b = a + 1; b < q; ++b) {
+      double* gv = row + pc->slot[list[b]];
+      if (isnan(*gv)) {
+        double t = pair_term(lat, rx, ry, rz, list[a], list[b], lambda_b, kappa_d);
+        *gv = t;
+        pc->g[(size_t)pc->slot[list[b]] * cap + pc->slot[list[a]]] = t;
+      }
+    }
+  }
+  /* the bead list is rewritten in place as slots; charges of unit magnitude */
+  int all_pos = 1;
+  for (int a = 0; a < q; ++a) {
+    all_pos &= s->charge[list[a]] == 1;
+    list[a] = pc->slot[list[a]];
+  }
+  if (all_pos) {
+    for (int a = 0; a < q; ++a) {
+      const double* row = pc->g + (size_t)list[a] * cap;
+      for (int b = a + 1; b < q; ++b) E += row[list[b]];
+    }
+    return E;
+  }
+  for (int a = 0; a < q; ++a) list[a] = -1;  /* recover the beads: rescan */
+  q = 0;
+  for (int i = 0; i < s->n; ++i)
+    if (s->charge[i] != 0) list[q++] = i;
+  for (int a = 0; a < q; ++a) {
+    const double* row = pc->g + (size_t)pc->slot[list[a]] * cap;
+    const int ci = s->charge[list[a]];
+    for (int b = a + 1; b < q; ++b) E += ci * s->charge[list[b]] * row[pc->slot[list[b]]];
   }
   return E;
+}
+
+static void free_cache(pair_cache_t* pc, int* qlist) {
+  if (pc) {
+    free(pc->slot);
+    free(pc->g);
+  }
+  free(qlist);
 }
 
 /* chargeHop (S/main26.c:1926-2390): one MC sweep over the charges, written as
@@ -1368,7 +1609,24 @@ int orc_charge_hop(orc_mc_state_t* s, const orc_lattice_t* lat, const double* rx
   for (i = 0; i < NCharges; ++i) s->indices[i] = s->track[i];
   /* the reference's Charge[] at this point reflects the previous state (track) */
   rebuild_charge(s, s->track);
-  E_initial = mc_energy(s, lat, rx, ry, rz, lambda_b, kappa_d);
+  /* the pair-term cache: at most nq + NCharges beads carry a charge in a sweep */
+  pair_cache_t pcs, *pc = NULL;
+  int* qlist = (int*)malloc(sizeof(int) * (size_t)(NCharges + 2));
+  if (NCharges <= 5000) {
+    pcs.cap = 2 * NCharges + 2;
+    pcs.used = 0;
+    pcs.slot = (int*)malloc(sizeof(int) * (size_t)N);
+    pcs.g = (double*)malloc(sizeof(double) * (size_t)pcs.cap * pcs.cap);
+    if (pcs.slot && pcs.g) {
+      for (i = 0; i < N; ++i) pcs.slot[i] = -1;
+      for (size_t e = 0; e < (size_t)pcs.cap * pcs.cap; ++e) pcs.g[e] = NAN;
+      pc = &pcs;
+    } else {
+      free(pcs.slot);
+      free(pcs.g);
+    }
+  }
+  E_initial = mc_energy(s, lat, rx, ry, rz, lambda_b, kappa_d, pc, qlist);
   if (e_initial_out) *e_initial_out = E_initial;
   for (k = 0; k < NCharges; ++k) {
     int draws = 0;
@@ -1382,7 +1640,10 @@ int orc_charge_hop(orc_mc_state_t* s, const orc_lattice_t* lat, const double* rx
       lg[5] = 0;
     }
     rebuild_charge(s, s->track);
-    if (*cursor >= nu) return -2;
+    if (*cursor >= nu) {
+        free_cache(pc, qlist);
+        return -2;
+      }
     mm = DRAW();
     draws++;
     cnumber = s->indices[k];
@@ -1479,13 +1740,16 @@ int orc_charge_hop(orc_mc_state_t* s, const orc_lattice_t* lat, const double* rx
       lg[3] = test;
     }
     rebuild_charge(s, s->indices);
-    E_final = mc_energy(s, lat, rx, ry, rz, lambda_b, kappa_d);
+    E_final = mc_energy(s, lat, rx, ry, rz, lambda_b, kappa_d, pc, qlist);
     double b = (test == 2) ? barrier2 : barrier;
     int accepted = 0;
     if (1 <= exp((-b - 0.5 * (E_final - E_initial)))) {
       accepted = 1;
     } else {
-      if (*cursor >= nu) return -2;
+      if (*cursor >= nu) {
+        free_cache(pc, qlist);
+        return -2;
+      }
       dice_roll = DRAW();
       draws++;
       if (dice_roll <= exp((-b - 0.5 * (E_final - E_initial)))) accepted = 1;
@@ -1514,6 +1778,7 @@ int orc_charge_hop(orc_mc_state_t* s, const orc_lattice_t* lat, const double* rx
     }
   }
   rebuild_charge(s, s->track);
+  free_cache(pc, qlist);
 #undef DRAW
   return 0;
 }

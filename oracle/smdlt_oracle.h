@@ -96,6 +96,10 @@ typedef struct { double a1x, a1y, a2x, a2y, Lz; } orc_cell_t;
 void orc_rpy_real_coeffs(double r, double alpha, int overlap_mode, double* t1, double* t2);
 void orc_rpy_ewald_dense(const orc_cell_t* c, int n, const double* pos, double alpha, double rcut,
                          double kcut, int overlap_mode, double* M);
+/* u = M v with M of orc_rpy_ewald_dense, without forming M (structure factors
+ * for the reciprocal part): for checks at 10^4 - 10^5 beads. */
+void orc_rpy_ewald_matvec(const orc_cell_t* c, int n, const double* pos, double alpha, double rcut, double kcut,
+                          int overlap_mode, const double* v, double* u);
 /* free-space RPY (no periodic images), for the L -> infinity closed form */
 void orc_rpy_free_dense(int n, const double* pos, double* M);
 double orc_ewald_selfconst(double alpha);

@@ -579,3 +579,145 @@ def test_mc_energy_bookkeeping_and_uniqueness():
     assert len(set(mc.track.tolist())) == len(track)
     assert cur == sum(int(l[5]) for l in log)
     assert np.all(np.isin(log[:, 5], [1, 2]))
+
+
+# ------------------------------------------------------------------ matrix-free Ewald
+@pytest.mark.parametrize("flow", [0, 1, 2])
+def test_ewald_matvec_equals_dense_product(flow):
+    # the structure-factor arrangement equals the dense matrix for any split,
+    # on deformed lattices and with overlapping beads
+    L = 22.0
+    lat = O.lattice(flow, L, 0.5 if flow else 0.0, 1e-3, 1, 300)
+    pos = random_walk(30, 4, L, 3, min_sep=0.4)
+    if flow == 2:
+        rx, ry = O.rotate(lat, pos[:, 0].copy(), pos[:, 1].copy(), 0)
+        pos = np.stack([rx, ry, pos[:, 2]], 1)
+    cell = O.cell_of(lat)
+    v = np.random.default_rng(1).normal(size=pos.size)
+    ref = O.ewald_dense(cell, pos, 0.5) @ v
+    for a in (None, 0.2, 0.9):
+        u = O.ewald_matvec(cell, pos, v, alpha=a)
+        assert np.linalg.norm(u - ref) / np.linalg.norm(ref) < 1e-12
+
+
+# ------------------------------------------------------------------ shear wrap shift
+def test_shear_delx_shift_is_minus_gamma_above_and_plus_below():
+    # applyPBC/getNID (S/main26.c:1409-1457): with the reference's compat shift, a
+    # bead or pair vector whose y lies beyond +L2yp/2 moves by -delx = -edot*treal*dt
+    # in x (modulo the cell), one beyond -L2yp/2 by +delx, others not at all
+    L, edot, dt, t = 20.0, 0.4, 1e-3, 700
+    ex = O.lattice(1, L, edot, dt, 0, t)
+    cp = O.lattice(1, L, edot, dt, 1, t)
+    delx = edot * (t % cp.tp) * dt
+    assert delx > 0.1
+    rng = np.random.default_rng(3)
+    rx, ry, rz = (rng.uniform(-1.4 * L, 1.4 * L, 400) for _ in range(3))
+    a = O.apply_pbc(cp, rx.copy(), ry.copy(), rz.copy())
+    b = O.apply_pbc(ex, rx.copy(), ry.copy(), rz.copy())
+    assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+    d = a[0] - b[0]
+    up, dn = ry > cp.L2yp / 2, ry < -cp.L2yp / 2
+    k_up = (d[up] + delx) / cp.L1xp
+    k_dn = (d[dn] - delx) / cp.L1xp
+    assert np.allclose(k_up, np.round(k_up), atol=1e-9) and np.allclose(k_dn, np.round(k_dn), atol=1e-9)
+    mid = ~(up | dn)
+    assert np.array_equal(d[mid], np.zeros(mid.sum()))
+    # pair vectors
+    n = 60
+    px, py, pz = (rng.uniform(-L / 2, L / 2, n) for _ in range(3))
+    for i in range(0, n, 3):
+        for j in range(1, n, 7):
+            gc = O.nid(cp, px, py, pz, i, j)
+            ge = O.nid(ex, px, py, pz, i, j)
+            dy = py[i] - py[j]
+            s = -1.0 if dy > cp.L2yp / 2 else (1.0 if dy < -cp.L2yp / 2 else 0.0)
+            kk = (gc[0] - ge[0] - s * delx) / cp.L1xp
+            assert abs(kk - round(kk)) < 1e-9 and gc[1] == ge[1] and gc[2] == ge[2]
+
+
+# ------------------------------------------------------------------ bending force
+def bend_energy(pos, nb, nc, kappab):
+    """U = kappab theta^2 / 2 per interior bead, theta = pi - acos(cos) of the angle
+    between r_j - r_{j-1} and r_j - r_{j+1} (S/main26.c:581-602)."""
+    U = 0.0
+    for c in range(nc):
+        for j in range(1, nb - 1):
+            b = c * nb + j
+            a1, a2 = pos[b] - pos[b - 1], pos[b] - pos[b + 1]
+            cs = a1 @ a2 / np.linalg.norm(a1) / np.linalg.norm(a2)
+            U += 0.5 * kappab * (np.pi - np.arccos(cs)) ** 2
+    return U
+
+
+def test_bending_force_is_minus_gradient_of_bending_energy():
+    L, nb, nc, kb = 200.0, 8, 2, 2.5
+    lat = O.lattice(0, L, 0.0, 1e-3)
+    rng = np.random.default_rng(5)
+    steps = rng.normal(size=(nc, nb, 3))
+    steps *= 2.0 / np.linalg.norm(steps, axis=2, keepdims=True)
+    pos = (np.cumsum(steps, axis=1) + rng.uniform(-20, 20, (nc, 1, 3))).reshape(-1, 3)  # no images
+    rx, ry, rz = (np.ascontiguousarray(pos[:, k]) for k in range(3))
+    st, f, _ = O.bond_force(lat, nb, nc, rx, ry, rz, 0, 0.0, 2.0, kb)  # Hookean with kappas = 0: bending only
+    h = 1e-6
+    g = np.zeros_like(pos)
+    for b in range(nb * nc):
+        for k in range(3):
+            p1, p2 = pos.copy(), pos.copy()
+            p1[b, k] += h
+            p2[b, k] -= h
+            g[b, k] = (bend_energy(p1, nb, nc, kb) - bend_energy(p2, nb, nc, kb)) / (2 * h)
+    assert np.abs(f.reshape(-1, 3) + g).max() < 1e-6 * max(1.0, np.abs(g).max())
+    assert np.abs(g).max() > 0.1
+
+
+# ------------------------------------------------------------------ chargeHop branches
+def hop_geometry():
+    """One chain of 6 in a large box; bead 2 has two close non-adjacent beads (r < 2.1):
+    bead 4 at r = 2.06 and bead 5 at r = 1.68; no other pair of the chain is that close
+    except along bonds."""
+    rx = np.array([0.0, 2.0, 4.0, 6.0, 5.0, 3.5])
+    ry = np.array([0.0, 0.0, 0.0, 0.0, 1.8, 1.6])
+    rz = np.zeros(6)
+    return O.lattice(0, 40.0, 0.0, 1e-3), rx, ry, rz
+
+
+HOP_CASES = [
+    # (charges, uniforms, barrier, barrier2, expected log[0] = {k, from, to, kind, accepted, draws})
+    # mm < .33 with a close bead: right neighbour, "adjacent near" kind, exp(0) = 1 accepts on one draw
+    ([2], [0.2], 0.0, 5.0, [0, 2, 3, 1, 1, 1]),
+    # mm > .66 with a close bead: left neighbour; exp(-0.5) < 1 so a dice is drawn: 0.9 rejects, 0.3 accepts
+    ([2], [0.8, 0.9], 0.5, 5.0, [0, 2, 1, 1, 0, 2]),
+    ([2], [0.8, 0.3], 0.5, 5.0, [0, 2, 1, 1, 1, 2]),
+    # .33 <= mm <= .66: pop to the lowest-index close bead (4), with barrier2 (0) not barrier (5)
+    ([2], [0.5], 5.0, 0.0, [0, 2, 4, 2, 1, 1]),
+    # pop blocked (bead 4 charged): the flag stays set, so bead 5 and the neighbour fallback are
+    # blocked too and the charge is skipped after one draw
+    ([2, 4], [0.5, 0.9, 0.9, 0.9], 0.0, 0.0, [0, 2, -1, -1, 0, 1]),
+    # mm < .33 with the right neighbour charged: blocked, and the fallback to the left is blocked too
+    ([2, 3], [0.1, 0.9, 0.9, 0.9], 0.0, 0.0, [0, 2, -1, -1, 0, 1]),
+    # no close bead for bead 0 (chain start): mm > .5 hops right without the near kind
+    ([0], [0.7], 0.0, 0.0, [0, 0, 1, 0, 1, 1]),
+    # chain end (bead 5) cannot hop right: skipped
+    ([5], [0.2], 0.0, 0.0, None),
+]
+
+
+@pytest.mark.parametrize("case", range(len(HOP_CASES)))
+def test_chargehop_branches_hand_worked(case):
+    track, u, b1, b2, expect = HOP_CASES[case]
+    lat, rx, ry, rz = hop_geometry()
+    mc = O.ChargeHop(6, 1, track)
+    st, cur, log, e0 = mc.sweep(lat, rx, ry, rz, 3.59, 5.0, b1, b2, np.array(u, np.float32))
+    if expect is None:
+        # bead 5: close to bead 2 (r = 1.68), mm = 0.2 < .33 asks for the right neighbour,
+        # which does not exist on a chain end: skipped, fallback left also blocked by the flag
+        assert log[0][3] == -1 and log[0][4] == 0 and mc.track[0] == 5
+        return
+    got = list(log[0])
+    if expect[2] == -1:  # skipped: the target column is not meaningful
+        got[2] = -1
+    assert got == expect
+    moved = expect[4] == 1
+    assert mc.track[0] == (expect[2] if moved else track[0])
+    if expect[2] >= 0 and not moved:
+        assert mc.old[0] == track[0]
