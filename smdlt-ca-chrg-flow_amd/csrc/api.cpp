@@ -337,6 +337,7 @@ struct smdlt_ctx {
   std::vector<int> chain0;
   int nchains = 0, max_chains = 0;
   bool force_rebuild = false;
+  uint64_t lat_t = UINT64_MAX;  // step of the last lattice set on the device
   int expect_iters = 0;    // largest iteration count of the previous graph step
   bool graph_ready = false;
   int cur_rep_begin = 0, cur_rep_end = 0;
@@ -1238,6 +1239,13 @@ int smdlt_download_positions(smdlt_ctx* c, double* rx, double* ry, double* rz) {
 
 static int lattice_to_device(smdlt_ctx* c, uint64_t t) {
   c->pos_version++;
+  // The lists follow the lattice by one step's strain per neighbour check; a
+  // lattice set to a step other than the last one or its successor (a jump in
+  // strain) makes the next list use rebuild them.
+  if (c->lat_t != UINT64_MAX && t != c->lat_t && t != c->lat_t + 1)
+    for (int r = 0; r < c->nrep; ++r)
+      if (c->rh[r].edot != 0.0) c->force_rebuild = true;
+  c->lat_t = t;
   for (int r = 0; r < c->nrep; ++r) {
     c->ct_prev[r] = c->lat[r].ct;
     c->st_prev[r] = c->lat[r].st;
@@ -1656,7 +1664,7 @@ static void mobility_internal(smdlt_ctx* c, const double* v, double* u, const do
 }
 
 static int ensure_lists(smdlt_ctx* c) {
-  if (c->lists_built) return 0;
+  if (c->lists_built && !c->force_rebuild) return 0;
   int st = smdlt_neighbors(c, 1, nullptr);
   return st;
 }
@@ -2329,9 +2337,12 @@ int smdlt_run(smdlt_ctx* c, uint64_t t0, int nsteps, int use_graph) {
   int st = 0;
   for (int k = 0; k < nsteps; ++k) {
     uint64_t t = t0 + k;
-    if (!use_graph || !c->lists_built || c->P[c->cur_rep_begin].noise_mode != SMDLT_NOISE_LANCZOS) {
+    const bool jump = c->lat_t != UINT64_MAX && t != c->lat_t && t != c->lat_t + 1;
+    if (!use_graph || !c->lists_built || c->force_rebuild || jump ||
+        c->P[c->cur_rep_begin].noise_mode != SMDLT_NOISE_LANCZOS) {
       // the first step builds the lists after its remap; TEA and free-draining steps run eagerly.
       // An eager step checks for list overflow on the host and grows the lists before its forces.
+      // A step after a jump in strain rebuilds its lists eagerly (a captured step never forces it).
       st = step_internal(c, t, nullptr, true);
       if (st) return st;
       continue;
