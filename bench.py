@@ -1,19 +1,22 @@
 #!/usr/bin/env python3
 """Throughput of the Brownian-dynamics timestep on one MI355X, in bead-steps/s.
 
-Workload (the metric's configuration): semidilute polyelectrolyte under shear,
-40 chains x 200 beads (8000 beads), box L = 80a, Wi = 50 (edot = 50/175, the
-notebooks' Wi = 175 edot), dt = 5e-4, 50 % of the beads charged +-1 with zero
-net charge, Debye length 5a, Bjerrum length 1a, Hookean springs of the reference
-defaults.  One step = remap + neighbour check + forces + device noise +
-M.F + Lanczos M^(1/2) z (tol 1e-6, <= 128 iterations) + position update,
-mixed precision (FP32 pair/grid/FFT sums, FP64 Lanczos), replayed as a hipGraph.
+Headline workload (`value`): the uneven replica ensemble, 512 independent
+replicas (chain length U[50, 1000], chain count U{1..20}, box at volume
+fraction 0.02, shear or planar extension with Wi U[1, 100]; 2.88 M beads),
+advanced together: one launch per stage over all replicas, each step replayed
+as hipGraphs, mixed precision (FP32 pair/grid/FFT sums with compensated
+per-bead sums, FP64 Lanczos at tol 1e-6).  One step = remap + neighbour check
++ forces + device noise + M.F + Lanczos M^(1/2) z + position update.  Beside
+it: the plain per-replica FP64 path (the same kernels in FP64, one replica per
+launch sequence, eager, host-checked Lanczos exit) over all 512 replicas, the
+roofline of the dominant kernel, and the CPU restatement of the reference
+algorithm timed on the host over a seeded sample of the replicas.
 
-Also reported: the same workload on the plain per-replica FP64 path (no graph,
-host-checked Lanczos exit), the M.F error of the mixed path against a tight
-FP64 evaluation, per-kernel device times (HIP events on the context stream),
-the roofline of the dominant kernel and the CPU restatement of the reference
-algorithm (dense Ewald M every 100 steps + TEA matvec) timed on the host.
+Also reported (`semidilute`): the 8000-bead semidilute polyelectrolyte under
+shear (40 x 200 beads, L = 80, Wi = 50, 4000 charges +-1), with its plain
+FP64 path and the M.F error of the mixed path against a tight FP64 evaluation.
+M.F and M^(1/2) z errors against the CPU oracle are measured by the tests.
 """
 import argparse
 import json
@@ -27,23 +30,32 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "smdlt-ca-chrg-flow_amd"))
 import smdlt_abi as S  # noqa: E402
 from smdlt_workloads import (c2_charges, c2_params, c4_params, c4_specs, chains, grow_chains,  # noqa: E402
-                             to_lab_frame)
+                             grow_chains_grouped, to_lab_frame)
 
 HBM_PEAK_GBS = 8000.0
 
 
-def batched_leg(nrep, steps, warmup, plain_reps):
-    """Batched replicas (one launch per stage, graph-replayed) against the plain
-    per-replica FP64 path (same kernels in FP64, one replica at a time, eager,
-    host-checked Lanczos exit) timed on the first plain_reps replicas."""
+def tuner_summary(cfgs):
+    """The autotuner's per-replica mobility settings (chosen for M.F error <= 1e-5;
+    the configurations' nominal grid and cutoff are not used)."""
+    g = [int(np.prod(c["grid"])) for c in cfgs]
+    rc = [c["rc"] for c in cfgs]
+    return {"spline_order": sorted({c["order"] for c in cfgs}), "rc": [min(rc), max(rc)],
+            "grid_points": [min(g), max(g)], "ewald_alpha": [round(min(c["alpha"] for c in cfgs), 4),
+                                                             round(max(c["alpha"] for c in cfgs), 4)]}
+
+
+def ensemble_leg(nrep, steps, warmup, plain_steps):
+    """The uneven replica ensemble, batched (graph-replayed, one launch per stage)
+    against the plain per-replica FP64 path over the same replicas."""
     specs = c4_specs(nrep)
     t0 = time.perf_counter()
-    pos = grow_chains([(sp["nb"], sp["nc"], sp["L"]) for sp in specs], 2)
+    pos = grow_chains_grouped([(sp["nb"], sp["nc"], sp["L"]) for sp in specs], 2)
     ctx = S.Context(c4_params(specs, S.PREC_MIXED))
     pos = to_lab_frame(ctx, specs, pos)
     allp = np.concatenate(pos)
     ntot = len(allp)
-    print(f"[bench] batched: {nrep} replicas, {ntot} beads, chains grown in {time.perf_counter() - t0:.1f}s",
+    print(f"[bench] ensemble: {nrep} replicas, {ntot} beads, chains grown in {time.perf_counter() - t0:.1f}s",
           file=sys.stderr, flush=True)
     ctx.upload_positions(allp[:, 0], allp[:, 1], allp[:, 2])
     ctx.run(0, warmup, use_graph=True)
@@ -52,12 +64,12 @@ def batched_leg(nrep, steps, warmup, plain_reps):
     code = ctx.run(warmup, steps, use_graph=True)
     ctx.sync()
     el = time.perf_counter() - t1
-    iters = ctx.last_iters()
-    batched = ntot * steps / el
-    print(f"[bench] batched {batched:.4g} bead-steps/s ({1e3 * el / steps:.1f} ms/step, Lanczos iters "
+    iters, status, _ = ctx.lanczos_status()
+    value = ntot * steps / el
+    print(f"[bench] ensemble {value:.4g} bead-steps/s ({1e3 * el / steps:.1f} ms/step, Lanczos iters "
           f"{int(iters.min())}..{int(iters.max())})", file=sys.stderr, flush=True)
-    # the roofline on the working set of GBs: per-kernel device times of one eager
-    # profiled step, algorithmic bytes summed over the replicas, PMC traffic beside
+    # per-kernel device times of one eager profiled step; algorithmic bytes summed
+    # over the replicas; PMC traffic of the same kernel beside it
     ctx.set_profiling(True)
     ctx.run(warmup + steps, 1, use_graph=False)
     st = ctx.stage_times()
@@ -74,36 +86,144 @@ def batched_leg(nrep, steps, warmup, plain_reps):
     roof = {"bound": "hbm", "kernel": sym or dom, "stage": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": ach / HBM_PEAK_GBS, "traffic": traffic, "bytes_per_launch": kb[dom], "avg_launch_ms": avg_ms,
             "grid_points": G, "list_entries": ent,
-            "kernel_ms_avg": {k: v[0] / v[1] for k, v in kern.items()}}
+            "note": "dominant kernel of one eager step (HIP events on the context stream); achieved = algorithmic "
+                    "bytes per launch / average launch time; traffic = PMC FETCH+WRITE bytes per dispatch"}
+    kernel_ms = {k: v[0] / v[1] for k, v in kern.items()}
     ctx.close()
-    out = {"workload": f"{nrep} uneven replicas (chains U[50,1000] x U{{1..20}}, volume fraction 0.02, "
-                       f"shear or planar extension, Wi U[1,100]), no charges",
-           "replicas": nrep, "beads": ntot, "steps": steps, "ms_per_step": 1e3 * el / steps, "value": batched,
-           "lanczos_iters_last_step": [int(iters.min()), int(iters.max())], "lanczos_converged": code == 0,
-           "roofline": roof}
-    if plain_reps <= 0:
+    out = {"value": value, "ms_per_step": 1e3 * el / steps, "beads": ntot, "replicas": nrep,
+           "lanczos_iters_last_step": [int(iters.min()), int(iters.max())],
+           "lanczos_converged": bool(code == 0 and np.all(status == 1)), "roofline": roof,
+           "kernel_ms_avg": kernel_ms, "tuner": tuner_summary(cfgs), "specs": specs, "pos": pos}
+    if plain_steps <= 0:
         return out
-    # plain path on a sample of the replicas
-    sub = list(range(min(plain_reps, nrep)))
+    # plain path: every replica on its own launch sequence, FP64, eager
     c64 = S.Context(c4_params(specs, S.PREC_FP64))
     c64.upload_positions(allp[:, 0], allp[:, 1], allp[:, 2])
-    nsub = sum(len(pos[r]) for r in sub)
-    for r in sub:  # first step builds the lists
+    for r in range(nrep):  # a first step per replica builds its lists
         c64.select_replicas(r, r + 1)
         c64.step(0)
     c64.sync()
     t2 = time.perf_counter()
-    for r in sub:
+    for r in range(nrep):
         c64.select_replicas(r, r + 1)
-        c64.step(1)
+        for k in range(plain_steps):
+            c64.step(1 + k)
     c64.sync()
-    plain = nsub / (time.perf_counter() - t2)
+    plain = ntot * plain_steps / (time.perf_counter() - t2)
     c64.close()
-    print(f"[bench] plain per-replica fp64 {plain:.4g} bead-steps/s over {len(sub)} replicas ({nsub} beads)",
+    print(f"[bench] plain per-replica fp64 {plain:.4g} bead-steps/s over {nrep} replicas x {plain_steps} steps",
           file=sys.stderr, flush=True)
-    out.update({"plain_fp64_value": plain, "plain_sample": f"{len(sub)} replicas, {nsub} beads, 1 step each",
-                "speedup_vs_plain": batched / plain})
+    out.update({"plain_fp64_value": plain, "plain_sample": f"all {nrep} replicas ({ntot} beads), {plain_steps} "
+                                                           f"timed steps each after one list-building step",
+                "speedup_vs_plain": value / plain})
     return out
+
+
+def semidilute_leg(steps, warmup, plain_steps):
+    """40 x 200 beads under shear with 4000 charges: graph-replayed mixed path, its
+    roofline, the plain FP64 path and the mixed M.F error against tight FP64."""
+    pos = chains(200, 40, 80.0, 1)
+    n = len(pos)
+    track, sign = c2_charges(n)
+    ctx = S.Context(c2_params(S.PREC_MIXED))
+    cfg = ctx.mobility_config()
+    ctx.upload_positions(pos[:, 0], pos[:, 1], pos[:, 2])
+    ctx.upload_charges([len(track)], track, sign)
+    ctx.run(0, warmup, use_graph=True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    code = ctx.run(warmup, steps, use_graph=True)
+    ctx.sync()
+    el = time.perf_counter() - t0
+    value = n * steps / el
+    iters = int(ctx.last_iters()[0])
+    ctx.set_profiling(True)
+    nprof = 3
+    ctx.run(warmup + steps, nprof, use_graph=False)
+    st = ctx.stage_times()
+    ctx.set_profiling(False)
+    G = cfg["grid"][0] * cfg["grid"][1] * cfg["grid"][2]
+    mc = ctx.mobility_config()
+    nbr = mc["cluster_pairs"] or mc["list_entries"]
+    kb = kernel_bytes(n, G, nbr, cfg["order"])
+    kern = {k: v for k, v in st.items() if k.startswith("k_") and v[1] > 0}
+    dom = max(kern, key=lambda k: kern[k][0])
+    avg_ms = kern[dom][0] / kern[dom][1]
+    achieved = kb[dom] / (avg_ms * 1e-3) / 1e9
+    sym, traffic = pmc_traffic("c2", stage_kernels(dom, cfg["order"], bool(mc["cluster_pairs"])))
+    rng = np.random.default_rng(3)
+    v = rng.normal(size=3 * n)
+    p64 = c2_params(S.PREC_FP64)
+    p64.mobility_tol = 1e-9
+    ctx_t = S.Context(p64)
+    ctx_t.upload_positions(*ctx.download_positions())
+    ctx.remap(warmup + steps + nprof)  # M is taken in the lattice frame
+    ctx_t.remap(warmup + steps + nprof)
+    u_mix = ctx.mobility_apply(v)
+    u_ref = ctx_t.mobility_apply(v)
+    mf_err = float(np.linalg.norm(u_mix - u_ref) / np.linalg.norm(u_ref))
+    ctx_t.close()
+    ctx.close()
+    out = {"workload": "semidilute polyelectrolyte under shear: 40x200 beads, L=80, Wi=50, dt=5e-4, 50% charged "
+                       "(+-1, zero net), lambda_D=5, lambda_B=1",
+           "value": value, "unit": "bead-steps/s", "steps": steps, "ms_per_step": 1e3 * el / steps,
+           "lanczos_iters_last_step": iters, "lanczos_converged": code == 0,
+           "tuner": {"grid": list(cfg["grid"]), "spline_order": cfg["order"], "rc": cfg["rc"],
+                     "ewald_alpha": round(cfg["alpha"], 5),
+                     "note": "the configuration names a 64^3 grid and rc = 5; the autotuner's choice holds "
+                             "M.F error <= 1e-5 instead"},
+           "mf_rel_err_vs_fp64": mf_err,
+           "kernel_ms_avg": {k: v[0] / v[1] for k, v in kern.items()},
+           "roofline": {"bound": "hbm", "kernel": sym or dom, "stage": dom, "achieved": achieved,
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                        "bytes_per_launch": kb[dom], "avg_launch_ms": avg_ms, "list_entries": nbr}}
+    if plain_steps > 0:
+        c64 = S.Context(c2_params(S.PREC_FP64))
+        c64.upload_positions(pos[:, 0], pos[:, 1], pos[:, 2])
+        c64.upload_charges([len(track)], track, sign)
+        c64.step(0)
+        c64.sync()
+        t1 = time.perf_counter()
+        for k in range(plain_steps):
+            c64.step(1 + k)
+        c64.sync()
+        plain = n * plain_steps / (time.perf_counter() - t1)
+        c64.close()
+        out.update({"plain_fp64_value": plain, "speedup_vs_plain": value / plain})
+    print(f"[bench] semidilute {value:.4g} bead-steps/s ({out['ms_per_step']:.2f} ms/step)", file=sys.stderr,
+          flush=True)
+    return out
+
+
+def cpu_baseline(specs, pos, budget_s=20.0):
+    """The CPU restatement of the reference algorithm (dense Ewald M every srpy =
+    100 steps with far-pair table lookups, TEA matvec per step; no charges, as in
+    the ensemble) timed on the host over a seeded sample of the ensemble's
+    replicas, until about budget_s of CPU work is done."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O  # the checker's CPU restatement, timed as the baseline
+    order = np.random.default_rng(17).permutation(len(specs))
+    beads, secs, wall, used = 0, 0.0, 0.0, []
+    for r in order:
+        sp = specs[int(r)]
+        if sp["nb"] * sp["nc"] > 6000:
+            continue
+        p = dict(nb=sp["nb"], nc=sp["nc"], flow=sp["flow"], L=sp["L"], edot=sp["edot"], dt=5e-4, kappas=200.0,
+                 qmax=2.0, epsilon=0.1, lambda_b=1.0, lambda_d=5.0, barrier=3.0, barrier2=3.0, spring_fene=0, nq=0,
+                 mstep=0, srpy=100, nt_avg=8, beta=0.5, seed=830062920)
+        x = pos[int(r)]
+        nsteps = 3
+        elc, stc, _, thi = O.ref_run(p, x[:, 0], x[:, 1], x[:, 2], nsteps, 0)
+        beads += len(x)
+        secs += (elc - thi) / nsteps + thi / 100.0  # one step with the M build amortised
+        wall += elc
+        used.append(int(r))
+        if (len(used) >= 3 and wall > budget_s) or len(used) >= 16:
+            break
+    return {"value": beads / secs, "unit": "bead-steps/s", "cores": O.num_threads(), "kind": "port",
+            "sample": f"{len(used)} replicas of the ensemble (seeded, <= 6000 beads each, {beads} beads): "
+                      f"dense Ewald M build (calcHI restated, far-pair table lookups) amortised over srpy=100 "
+                      f"steps + TEA matvec per step, no charges; 3 steps each"}
 
 
 def tau_int_multi(E, c=5.0, per_column_mean=False, groups=1):
@@ -369,18 +489,17 @@ def pmc_traffic(section, names):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=10, help="timed steps of the replica ensemble")
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--replicas", type=int, default=512)
     ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--c2-steps", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--workload", choices=["all", "c2", "batched", "learned", "c3", "c1"], default="all",
-                    help="c2: only the 8000-bead shear line; batched: only the replica ensemble")
+    ap.add_argument("--workload", choices=["all", "ensemble", "c2", "learned", "c3", "c1"], default="all",
+                    help="ensemble: only the replica ensemble; c2: only the 8000-bead shear line")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 256 MiB device-to-device three times (HBM counter calibration)")
-    ap.add_argument("--batched-replicas", type=int, default=64)
-    ap.add_argument("--batched-steps", type=int, default=3)
-    ap.add_argument("--plain-replicas", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -405,114 +524,37 @@ def main():
     if args.workload == "learned":
         print(json.dumps({"learned": learned_leg()}, default=float))
         return
-    if args.workload == "batched":
-        print(json.dumps({"batched": batched_leg(args.batched_replicas, args.batched_steps, 1, args.plain_replicas)}))
+    if args.workload == "c2":
+        print(json.dumps({"semidilute": semidilute_leg(args.c2_steps, 3, 0 if args.no_plain else 3)}))
         return
 
-    pos = chains(200, 40, 80.0, 1)
-    n = len(pos)
-    track, sign = c2_charges(n)
-    ctx = S.Context(c2_params(S.PREC_MIXED))
-    cfg = ctx.mobility_config()
-    ctx.upload_positions(pos[:, 0], pos[:, 1], pos[:, 2])
-    ctx.upload_charges([len(track)], track, sign)
-    print(f"[bench] mixed context ready: {cfg}", file=sys.stderr, flush=True)
-    ctx.run(0, args.warmup, use_graph=True)
-    ctx.sync()
-    t0 = time.perf_counter()
-    code = ctx.run(args.warmup, args.steps, use_graph=True)
-    ctx.sync()
-    el = time.perf_counter() - t0
-    value = n * args.steps / el
-    iters = int(ctx.last_iters()[0])
-    print(f"[bench] timed {args.steps} steps in {el:.3f}s", file=sys.stderr, flush=True)
-
-    # per-kernel device times on an eager (profiled) pass
-    ctx.set_profiling(True)
-    nprof = 3
-    ctx.run(args.warmup + args.steps, nprof, use_graph=False)
-    st = ctx.stage_times()
-    ctx.set_profiling(False)
-    G = cfg["grid"][0] * cfg["grid"][1] * cfg["grid"][2]
-    mc = ctx.mobility_config()
-    nbr = mc["cluster_pairs"] or mc["list_entries"]  # the list the real-space pass reads
-    kb = kernel_bytes(n, G, nbr, cfg["order"])
-    kern = {k: v for k, v in st.items() if k.startswith("k_") and v[1] > 0}
-    dom = max(kern, key=lambda k: kern[k][0])
-    avg_ms = kern[dom][0] / kern[dom][1]
-    achieved = kb[dom] / (avg_ms * 1e-3) / 1e9
-    sym, traffic = pmc_traffic("c2", stage_kernels(dom, cfg["order"], bool(mc["cluster_pairs"])))
-
-    # M.F accuracy of the mixed path against a tight FP64 evaluation
-    rng = np.random.default_rng(3)
-    v = rng.normal(size=3 * n)
-    p64 = c2_params(S.PREC_FP64)
-    p64.mobility_tol = 1e-9
-    ctx_t = S.Context(p64)
-    ctx_t.upload_positions(*ctx.download_positions())
-    ctx.remap(args.warmup + args.steps + nprof)  # M is taken in the lattice frame
-    ctx_t.remap(args.warmup + args.steps + nprof)
-    u_mix = ctx.mobility_apply(v)
-    u_ref = ctx_t.mobility_apply(v)
-    mf_err = float(np.linalg.norm(u_mix - u_ref) / np.linalg.norm(u_ref))
-    ctx_t.close()
-
-    plain = None
-    if not args.no_plain:
-        c64 = S.Context(c2_params(S.PREC_FP64))
-        c64.upload_positions(pos[:, 0], pos[:, 1], pos[:, 2])
-        c64.upload_charges([len(track)], track, sign)
-        c64.step(0)
-        c64.sync()
-        t1 = time.perf_counter()
-        for k in range(args.plain_steps):
-            c64.step(1 + k)
-        c64.sync()
-        plain = n * args.plain_steps / (time.perf_counter() - t1)
-        c64.close()
-        print(f"[bench] plain fp64 path {plain:.4g} bead-steps/s", file=sys.stderr, flush=True)
-
-    batched = None
-    if args.batched_replicas > 0 and args.workload == "all":
-        batched = batched_leg(args.batched_replicas, args.batched_steps, 1, args.plain_replicas)
-
+    ens = ensemble_leg(args.replicas, args.steps, args.warmup, 0 if args.no_plain else args.plain_steps)
+    specs, pos = ens.pop("specs"), ens.pop("pos")
+    semi = None
+    if args.workload == "all":
+        semi = semidilute_leg(args.c2_steps, 3, 0 if args.no_plain else 3)
     cpu = None
     if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle as O  # the checker's CPU restatement, timed as the baseline
-        p = dict(nb=200, nc=40, flow=1, L=80.0, edot=50.0 / 175.0, dt=5e-4, kappas=200.0, qmax=2.0, epsilon=0.1,
-                 lambda_b=1.0, lambda_d=5.0, barrier=3.0, barrier2=3.0, spring_fene=0, nq=0, mstep=0, srpy=100,
-                 nt_avg=8, beta=0.5, seed=830062920)
-        nsteps = 3
-        elc, stc, _, thi = O.ref_run(p, pos[:, 0], pos[:, 1], pos[:, 2], nsteps, 0)
-        per_step = (elc - thi) / nsteps + thi / 100.0
-        cpu = {"value": n / per_step, "unit": "bead-steps/s", "cores": O.num_threads(), "kind": "port",
-               "sample": f"8000 beads: one dense Ewald M build (calcHI restated, far-pair table lookups) "
-                         f"+ {nsteps} TEA steps; build time amortised over srpy=100 steps "
-                         f"(build {thi:.2f}s, step {(elc - thi) / nsteps:.3f}s)"}
+        cpu = cpu_baseline(specs, pos)
         print(f"[bench] cpu baseline {cpu['value']:.4g} bead-steps/s", file=sys.stderr, flush=True)
-
     out = {
         "metric": "bead-steps/s at Lanczos tol 1e-6, batched/learned vs plain GPU; % HBM roofline",
-        "value": value, "unit": "bead-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": 1e3 * el / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "value": ens["value"], "unit": "bead-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ens["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32 pair/grid/FFT + fp64 Lanczos", "data": "synthetic",
-        "config": {"workload": "semidilute polyelectrolyte under shear: 40x200 beads, L=80, Wi=50, dt=5e-4, "
-                               "50% charged, lambda_D=5, lambda_B=1",
-                   "grid": list(cfg["grid"]), "spline_order": cfg["order"], "rc": cfg["rc"],
-                   "ewald_alpha": round(cfg["alpha"], 5), "lanczos_tol": 1e-6, "lanczos_iters_last_step": iters,
-                   "lanczos_converged": code == 0, "parallelism": "1 GPU"},
-        "mf_rel_err_vs_fp64": mf_err,
-        "plain_fp64_value": plain,
-        "speedup_vs_plain": (value / plain) if plain else None,
-        "kernel_ms_avg": {k: v[0] / v[1] for k, v in kern.items()},
-        "roofline": {"bound": "hbm", "kernel": sym or dom, "stage": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "bytes_per_launch": kb[dom],
-                     "avg_launch_ms": avg_ms, "list_entries": nbr, "bead_pair_entries": mc["list_entries"],
-                     "note": "the last pass of M.v (cluster pairs + interpolation) is latency/L2-bound "
-                             "(gathers), not HBM-bound; traffic is the PMC FETCH+WRITE bytes per dispatch"},
+        "config": {"workload": f"batched ensemble: {args.replicas} uneven replicas (chain length U[50,1000], chain "
+                               f"count U{{1..20}}, volume fraction 0.02, shear or planar extension, Wi U[1,100], "
+                               f"dt=5e-4), no charges, one launch per stage, steps replayed as hipGraphs",
+                   "replicas": args.replicas, "beads": ens["beads"], "lanczos_tol": 1e-6,
+                   "lanczos_iters_last_step": ens["lanczos_iters_last_step"],
+                   "lanczos_converged": ens["lanczos_converged"], "tuner": ens["tuner"],
+                   "parallelism": "1 GPU"},
+        "plain_fp64_value": ens.get("plain_fp64_value"), "plain_sample": ens.get("plain_sample"),
+        "speedup_vs_plain": ens.get("speedup_vs_plain"),
+        "kernel_ms_avg": ens["kernel_ms_avg"],
+        "roofline": ens["roofline"],
         "cpu_baseline": cpu,
-        "batched": batched,
+        "semidilute": semi,
     }
     print(json.dumps(out))
 
