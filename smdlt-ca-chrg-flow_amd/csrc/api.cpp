@@ -413,6 +413,7 @@ static void fill_lattice(smdlt_ctx* c, int r) {
   R.st0 = c->st_prev[r];
   double wm;
   reduce_basis(h.L1xp, 0.0, h.L2xp, h.L2yp, R.rb, R.rd, &wm);
+  R.halfw = 0.5 * std::min(wm, h.L);
   R.hinv[0] = 1.0 / h.L1xp;
   R.hinv[1] = 1.0 / h.L2yp;
   R.hinv[2] = 1.0 / h.L;
@@ -549,7 +550,7 @@ struct MobChoice {
 };
 
 static MobChoice mob_candidate(double rc, int order, double tol, double skin, const double len[3], int n, double vol,
-                               int nb) {
+                               int nb, bool fp64) {
   MobChoice m;
   m.rc = rc;
   m.order = order;
@@ -560,12 +561,22 @@ static MobChoice mob_candidate(double rc, int order, double tol, double skin, co
   double chain = std::min((double)nb, 2.0 * pow(rl / 2.0, 1.7));
   m.nbr = (double)n / vol * (4.0 / 3.0) * M_PI * rl * rl * rl + chain;
   double G = (double)m.K[0] * m.K[1] * m.K[2];
-  // measured costs (ns) on MI355X: per real-space pair, per grid point (gather +
-  // five FFT passes) and per interpolated stencil point
-  m.cost = 0.02 * n * m.nbr + 0.15 * G + 0.01 * n * order * order * order;
+  // Costs (ns per M.v) fitted to MI355X kernel times of 128 replicas of the
+  // uneven ensemble (0.68 M beads) over cutoffs 12-32 and orders 6 and 8: per
+  // real-space list pair (within rc + skin) 0.0014, per grid point 0.047 (the
+  // five FFT passes 0.039 and the gather's per-point part 0.0078), per stencil
+  // point (bead x p^3) 0.0051 (the gather's per-bead part 0.0024 and the
+  // interpolation 0.0027).  The list of a replica is capped at 2500 pairs per
+  // bead, which bounds the bead-pair list memory.
+  // FP64 contexts evaluate every pair's coefficients in closed form (erfc, exp)
+  // on a bead-pair list; their model prices pairs as in the FP64 kernels.
+  if (fp64)
+    m.cost = 0.02 * n * m.nbr + 0.15 * G + 0.01 * n * order * order * order;
+  else
+    m.cost = 0.0014 * n * m.nbr + 0.047 * G + 0.0051 * n * order * order * order;
   bool bad = false;
   for (int d = 0; d < 3; ++d) bad = bad || m.K[d] > KMAX;
-  if (bad || m.nbr > 4000) m.cost = 1e300;
+  if (bad || m.nbr > 2500) m.cost = 1e300;
   return m;
 }
 
@@ -593,14 +604,15 @@ static int choose_mobility(smdlt_ctx* c, int r) {
   double vol = p.L * p.L * p.L;
   MobChoice best;
   best.cost = 1e301;
-  const double rcs[] = {5, 6, 7, 8, 10, 12, 14, 16, 20, 24, 28, 32};
+  const double rcs[] = {5, 6, 7, 8, 10, 12, 14, 16, 20, 24, 28, 32, 36, 40, 44, 48};
   const int orders[] = {4, 6, 8};
   for (double rc : rcs) {
     if (p.rc_real > 0) rc = p.rc_real;
     if (rc > rcmax) continue;
+    if (c->prec == SMDLT_PREC_FP64 && rc > 32 && p.rc_real <= 0) continue;  // the FP64 model's fitted range
     for (int o : orders) {
       if (p.spline_order > 0 && o != p.spline_order) continue;
-      MobChoice m = mob_candidate(rc, o, tol, skin, len, n, vol, p.nb);
+      MobChoice m = mob_candidate(rc, o, tol, skin, len, n, vol, p.nb, c->prec == SMDLT_PREC_FP64);
       if (m.cost < best.cost) best = m;
     }
     if (p.rc_real > 0) break;
@@ -608,7 +620,8 @@ static int choose_mobility(smdlt_ctx* c, int r) {
   if (best.cost >= 1e300) {
     if (p.rc_real > 0 && p.rc_real > rcmax) return SMDLT_E_GEOMETRY;
     if (rcmax < 2.0) return SMDLT_E_GEOMETRY;
-    best = mob_candidate(std::min(rcmax, 8.0), p.spline_order > 0 ? p.spline_order : 8, tol, skin, len, n, vol, p.nb);
+    best = mob_candidate(std::min(rcmax, 8.0), p.spline_order > 0 ? p.spline_order : 8, tol, skin, len, n, vol, p.nb,
+                         c->prec == SMDLT_PREC_FP64);
   }
   if (p.ewald_alpha > 0) {
     best.alpha = p.ewald_alpha;
@@ -1097,7 +1110,7 @@ int smdlt_create(const smdlt_params* params, int n_replicas, int device, smdlt_c
       if (4LL * c->cl_words > 65536 || alloc_dev(&c->cl_first, c->ncl_total) || alloc_dev(&c->cl_cnt, c->ncl_total) ||
           alloc_dev(&c->slot_cl, c->nst) || alloc_dev(&c->nclp, c->ncl_total) ||
           alloc_dev(&c->adot_cl, c->ncl_total) ||
-          alloc_dev(&c->lcl, (size_t)c->capc * c->ncl_total)) {
+          alloc_dev(&c->lcl, (size_t)c->capc * c->ncl_total + CL_LIST_PAD)) {
         smdlt_destroy(c);
         return SMDLT_E_NOMEM;
       }
@@ -1445,7 +1458,7 @@ static int grow_lists(smdlt_ctx* c) {
     hipFree(c->lcl);
     c->lcl = nullptr;
     c->capc *= 2;
-    if (alloc_dev(&c->lcl, (size_t)c->capc * c->ncl_total)) return SMDLT_E_NOMEM;
+    if (alloc_dev(&c->lcl, (size_t)c->capc * c->ncl_total + CL_LIST_PAD)) return SMDLT_E_NOMEM;
   }
   HCK(hipMemset(c->overflow, 0, 4));
   destroy_graphs(c);

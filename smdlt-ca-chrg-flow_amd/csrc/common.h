@@ -66,6 +66,7 @@ struct RepDev {
   double barrier, barrier2;
   double hop2, nbhd2, feat_r, pfloor;  // MH hops: hop radius^2, neighbourhood radius^2, feature radius, floor
   double rnew, skin;           // Verlet trigger (rv-rc)/2 and mobility-list skin
+  double halfw;                // half the shortest nonzero lattice vector bound: min(reduced widths, L) / 2
   int treal, reset;            // t % tp and whether this step starts a lattice period
   unsigned long long step;     // step counter for the device noise stream
 };
@@ -366,6 +367,7 @@ struct MobArgs {
   int ntab_max;             // largest coefficient table of the range (entries of 2 float4)
 };
 #define RTAB_LDS_MAX 1280   // tables up to this many entries (40 KB) are staged in LDS
+#define CL_LIST_PAD 8       // entries past the last cluster-pair row, so a row's 4-entry groups read in bounds
 #define SCALE_FUSE_MAX_SLOTS 32768  // ranges up to this many slots normalise the input in the spread
 #define RTAB_INV 16
 #define RTAB_R0 1.0f  // pairs closer than this take the FP32 closed form

@@ -274,8 +274,34 @@ __global__ void k_bead_prep(MobArgs a) {
 #pragma unroll
     for (int j = 0; j < P; ++j) bw[d * MAX_ORDER + j] = W[d][j];
   }
-  if (a.p32) a.p32[s] = make_float4((float)a.rx[s], (float)a.ry[s], (float)a.rz[s], 0.f);
+  if (a.p32) {
+    // cluster-consistent FP32 position: the first bead of the cluster (a run of
+    // <= 8 consecutive beads of one chain) plus the minimum image of this bead
+    // relative to it, so a cluster is never split by the periodic wrap
+    const int j = (s - R.slot0) % R.nb, f = s - (j & 7);
+    double dx = a.rx[s] - a.rx[f], dy = a.ry[s] - a.ry[f], dz = a.rz[s] - a.rz[f];
+    min_image_reduced(R, dx, dy, dz);
+    a.p32[s] = make_float4((float)(a.rx[f] + dx), (float)(a.ry[f] + dy), (float)(a.rz[f] + dz), 0.f);
+  }
   atomicAdd(&a.brick_cnt[brick_of(R, base)], 1);
+}
+
+// Per cluster: the largest distance of its beads from its first bead, stored in
+// the first bead's FP32 position (.w), for the shared-image test of the cluster pass.
+__global__ void k_cluster_prep(MobArgs a) {
+  const int c = a.cl_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.cl_begin + a.ncl) return;
+  const int first = a.cl_first[c], cnt = a.cl_cnt[c];
+  const RepDev& R = a.reps[a.slot_rep[first]];
+  if (!R.active) return;
+  const float4 p0 = a.p32[first];
+  float rad2 = 0.f;
+  for (int k = 1; k < cnt; ++k) {
+    const float4 q = a.p32[first + k];
+    const float dx = q.x - p0.x, dy = q.y - p0.y, dz = q.z - p0.z;
+    rad2 = fmaxf(rad2, dx * dx + dy * dy + dz * dz);
+  }
+  a.p32[first].w = sqrtf(rad2);
 }
 
 __global__ void k_brick_fill(MobArgs a) {
@@ -771,6 +797,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_z_green(MobArgs a) {
 // (P rows of one plane) rather than P^2 scattered rows.
 template <typename T, int P>
 __device__ __forceinline__ void interp_bead(const MobArgs& a, const RepDev& R, int s, int lane, CompSum<T> acc[3]) {
+  s = __builtin_amdgcn_readfirstlane(s);  // wave-uniform: the stencil base and weights are scalar loads
   if (lane >= P * P) return;
   int base[3];
   T W[3][P];
@@ -788,17 +815,36 @@ __device__ __forceinline__ void interp_bead(const MobArgs& a, const RepDev& R, i
   g1 += g1 < 0 ? K0 : 0;
   g2 += g2 < 0 ? K1 : 0;
   const T w12 = pick<T, P>(W[0], j1) * pick<T, P>(W[1], j2);
-  const T* col = (const T*)a.grid + R.grid0 + (long long)g2 * K0 + g1;
   T s0 = 0, s1 = 0, s2 = 0;
+  if constexpr (sizeof(T) == 4) {
+    // buffer loads: the lane's column as a 32-bit byte offset, the plane and
+    // component as a scalar one (a replica's grid is below 2 GB)
+    const __amdgpu_buffer_rsrc_t rg = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)((const T*)a.grid + R.grid0), (short)0, (int)(3 * ncomp * (long long)sizeof(T)), 0x00020000);
+    const int vo = (g2 * K0 + g1) * (int)sizeof(T);
 #pragma unroll
-  for (int j3 = 0; j3 < P; ++j3) {
-    int g3 = base[2] - j3;
-    g3 += g3 < 0 ? K2 : 0;
-    const T* q = col + g3 * plane;
-    const T wt = W[2][j3];
-    s0 += wt * q[0];
-    s1 += wt * q[ncomp];
-    s2 += wt * q[2 * ncomp];
+    for (int j3 = 0; j3 < P; ++j3) {
+      int g3 = base[2] - j3;
+      g3 += g3 < 0 ? K2 : 0;
+      const int so = (int)(g3 * plane * (long long)sizeof(T));
+      const int sc = (int)(ncomp * (long long)sizeof(T));
+      const T wt = W[2][j3];
+      s0 += wt * __int_as_float(__builtin_amdgcn_raw_buffer_load_b32(rg, vo, so, 0));
+      s1 += wt * __int_as_float(__builtin_amdgcn_raw_buffer_load_b32(rg, vo, so + sc, 0));
+      s2 += wt * __int_as_float(__builtin_amdgcn_raw_buffer_load_b32(rg, vo, so + 2 * sc, 0));
+    }
+  } else {
+    const T* col = (const T*)a.grid + R.grid0 + (long long)g2 * K0 + g1;
+#pragma unroll
+    for (int j3 = 0; j3 < P; ++j3) {
+      int g3 = base[2] - j3;
+      g3 += g3 < 0 ? K2 : 0;
+      const T* q = col + g3 * plane;
+      const T wt = W[2][j3];
+      s0 += wt * q[0];
+      s1 += wt * q[ncomp];
+      s2 += wt * q[2 * ncomp];
+    }
   }
   acc[0].add(w12 * s0);
   acc[1].add(w12 * s1);
@@ -952,8 +998,18 @@ __global__ __launch_bounds__(256) void k_mob_final(MobArgs a) {
 // queue behind one another in the vector memory path.  Sums are compensated per
 // lane and combined in FP64 in a fixed order.
 #define CL_WAVES 4
+// Cluster pairs in flight per wave (CL_UNROLL, a divisor of 64) and the
+// occupancy the register budget is held to: two pairs at 6 waves per SIMD ran
+// 10% faster than four pairs at the 4 waves their registers allowed.
+#ifndef CL_UNROLL
+#define CL_UNROLL 2
+#endif
+#ifndef CL_WAVES_PER_EU
+#define CL_WAVES_PER_EU 6
+#endif
 template <int P, bool TAB_LDS>
-__global__ __launch_bounds__(64 * CL_WAVES) void k_mob_cl(MobArgs a) {
+__global__ __launch_bounds__(64 * CL_WAVES) __attribute__((amdgpu_waves_per_eu(CL_WAVES_PER_EU))) void k_mob_cl(
+    MobArgs a) {
   typedef float T;
   extern __shared__ float4 stab[];  // 2 * ntab_max (TAB_LDS)
   const int I = a.cl_begin + xcd_block(blockIdx.x, gridDim.x);
@@ -983,77 +1039,135 @@ __global__ __launch_bounds__(64 * CL_WAVES) void k_mob_cl(MobArgs a) {
     }
   }
   if constexpr (TAB_LDS) __syncthreads();
-  // real-space part by cluster pairs
+  // real-space part by cluster pairs.  The wave takes its list entries in groups
+  // of CL_UNROLL consecutive ones (read as scalars) and, every 64 entries, its
+  // lanes compute one entry's image shift each: the minimum image of the vector
+  // between the two clusters' first beads.  Positions are cluster-consistent, so
+  // when that vector's length plus both cluster radii is below half the shortest
+  // lattice vector every bead pair of the two clusters has that shift as its
+  // minimum image (any shorter image would need a lattice vector shorter than
+  // twice the bound), and the lanes skip their own minimum-image reduction.
   const int ii = lane >> 3, jj = lane & 7;
   const bool ivalid = ii < cnt;
   const int si = first + (ivalid ? ii : 0);
   const float4 pi = a.p32[si];
+  const float4 pc = a.p32[first];  // first bead; .w = cluster radius
   const float rd0 = (float)R.rd[0], rd1 = (float)R.rd[1], rd2 = (float)R.rd[2], rd3 = (float)R.rd[3];
   const float rb0 = (float)R.rb[0], rb1 = (float)R.rb[1], rb2 = (float)R.rb[2], rb3 = (float)R.rb[3];
   const float Lz = (float)R.L, iLz = (float)R.invL, rcf = (float)R.rc2;
   const float alpha = (float)R.alpha;
+  const float halfw = (float)R.halfw * 0.999f - 1e-3f;  // margin for the FP32 arithmetic of the test
   CompSum<T> acc[3];
   const int m = a.nclp[I];
   const int* row = a.lcl + (size_t)I * a.capc;
   const int ntab1 = R.ntab - 1;
-  for (int k0 = w; k0 < m; k0 += CL_WAVES * PAIR_UNROLL) {
-    int sj[PAIR_UNROLL];
-    bool ok[PAIR_UNROLL];
-    float4 pj[PAIR_UNROLL], vj[PAIR_UNROLL];
-#pragma unroll
-    for (int u = 0; u < PAIR_UNROLL; ++u) {
-      const int k = k0 + CL_WAVES * u;
-      const int e = k < m ? row[k] : (first << 4);
-      const int fj = e >> 4, cj = e & 15;
-      ok[u] = ivalid && jj < cj && !(fj == first && jj == ii);
-      sj[u] = fj + (jj < cj ? jj : 0);
-    }
-#pragma unroll
-    for (int u = 0; u < PAIR_UNROLL; ++u) {
-      pj[u] = a.p32[sj[u]];
-      vj[u] = a.v32[sj[u]];
-    }
-    float g0 = 0.f, g1 = 0.f, g2 = 0.f;  // this step's terms, summed plainly, then compensated once
-#pragma unroll
-    for (int u = 0; u < PAIR_UNROLL; ++u) {
-      float dx = pj[u].x - pi.x, dy = pj[u].y - pi.y, dz = pj[u].z - pi.z;
-      const float n1 = rintf(rd0 * dx + rd1 * dy), n2 = rintf(rd2 * dx + rd3 * dy);
-      dx -= n1 * rb0 + n2 * rb2;
-      dy -= n1 * rb1 + n2 * rb3;
-      dz -= Lz * rintf(dz * iLz);
-      const float rr = dx * dx + dy * dy + dz * dz;
-      const bool use = ok[u] && rr < rcf;
-      // Table coefficients on every lane (index clamped into the table; the
-      // hardware square root, 1 ulp, is ample for the lookup), so the common
-      // path has no branch and the partner loads stay ahead of it; pairs
-      // closer than the table take the closed form (rare); unused lanes are
-      // masked out of the sums.
-      const float x = fmaxf(__builtin_amdgcn_sqrtf(rr) - RTAB_R0, 0.f) * (float)RTAB_INV;
-      const int it = min((int)x, ntab1);
-      const float t = x - (float)it;
-      float4 ca, cb;
-      if constexpr (TAB_LDS) {
-        ca = stab[2 * it];
-        cb = stab[2 * it + 1];
-      } else {
-        ca = a.rtab[R.tab0 + 2 * it];
-        cb = a.rtab[R.tab0 + 2 * it + 1];
+  __shared__ float4 sshift[CL_WAVES][64];
+  const __amdgpu_buffer_rsrc_t rp32 = __builtin_amdgcn_make_buffer_rsrc((void*)a.p32, (short)0, a.nst * 16, 0x00020000);
+  const __amdgpu_buffer_rsrc_t rv32 = __builtin_amdgcn_make_buffer_rsrc((void*)a.v32, (short)0, a.nst * 16, 0x00020000);
+  const int wu = __builtin_amdgcn_readfirstlane(w);
+  constexpr int G = CL_UNROLL;
+  static_assert(64 % G == 0, "a wave's 64 precomputed shifts cover whole groups of CL_UNROLL entries");
+  for (int c0 = wu * G; c0 < m; c0 += 64 * CL_WAVES) {
+    // lane l: entry c0 + CL_WAVES * G * (l / G) + l % G of this wave's next 64
+    {
+      const int e_idx = c0 + CL_WAVES * G * (lane / G) + lane % G;
+      float4 sh = make_float4(0.f, 0.f, 0.f, 1.f);  // past the row: no pairs, no veto
+      if (e_idx < m) {
+        const float4 pr = a.p32[row[e_idx] >> 4];
+        float cx = pr.x - pc.x, cy = pr.y - pc.y, cz = pr.z - pc.z;
+        const float n1 = rintf(rd0 * cx + rd1 * cy), n2 = rintf(rd2 * cx + rd3 * cy), n3 = rintf(cz * iLz);
+        const float sx = -(n1 * rb0 + n2 * rb2), sy = -(n1 * rb1 + n2 * rb3), sz = -Lz * n3;
+        cx += sx;
+        cy += sy;
+        cz += sz;
+        const float reach = __builtin_amdgcn_sqrtf(cx * cx + cy * cy + cz * cz) + pr.w + pc.w;
+        sh = make_float4(sx, sy, sz, reach < halfw ? 1.f : 0.f);
       }
-      float t1 = ca.x + t * (ca.y + t * (ca.z + t * ca.w));
-      float b2 = cb.x + t * (cb.y + t * (cb.z + t * cb.w));
-      if (use && rr < RTAB_R0 * RTAB_R0) {
-        float t2;
-        rpy_real_near_f32(sqrtf(rr), rr, alpha, t1, t2);
-        b2 = t2 / rr;
-      }
-      const float dv = (dx * vj[u].x + dy * vj[u].y + dz * vj[u].z) * b2;
-      g0 += use ? t1 * vj[u].x + dv * dx : 0.f;
-      g1 += use ? t1 * vj[u].y + dv * dy : 0.f;
-      g2 += use ? t1 * vj[u].z + dv * dz : 0.f;
+      sshift[w][lane] = sh;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
-    acc[0].add(g0);
-    acc[1].add(g1);
-    acc[2].add(g2);
+    const int c1 = min(m, c0 + 64 * CL_WAVES);
+    // the chunk's (<= 64) terms of each lane are summed plainly in FP32, then
+    // added to the lane's compensated sum
+    float px = 0.f, py = 0.f, pz = 0.f;
+    for (int k0 = c0, q = 0; k0 < c1; k0 += CL_WAVES * G, q += G) {
+      int sj[G];
+      bool ok[G];
+      float3 pj[G], vj[G];
+      float4 S[G];
+      int cflag = 1;
+#pragma unroll
+      for (int u = 0; u < G; ++u) {
+        const int k = k0 + u;
+        const int e = row[k];  // the list is padded, so a group past the row's end still reads in bounds
+        const int fj = e >> 4, cj = k < m ? (e & 15) : 0;
+        ok[u] = ivalid && jj < cj && !(fj == first && jj == ii);
+        sj[u] = fj + (jj < cj ? jj : 0);
+        S[u] = sshift[w][q + u];
+        cflag &= S[u].w != 0.f ? 1 : 0;
+      }
+      const bool compact = __builtin_amdgcn_readfirstlane(cflag) != 0;
+#pragma unroll
+      for (int u = 0; u < G; ++u) {  // 32-bit offsets into the position and vector copies
+        const auto lp = __builtin_amdgcn_raw_buffer_load_b96(rp32, sj[u] * 16, 0, 0);
+        const auto lv = __builtin_amdgcn_raw_buffer_load_b96(rv32, sj[u] * 16, 0, 0);
+        pj[u] = make_float3(__int_as_float(lp[0]), __int_as_float(lp[1]), __int_as_float(lp[2]));
+        vj[u] = make_float3(__int_as_float(lv[0]), __int_as_float(lv[1]), __int_as_float(lv[2]));
+      }
+#pragma unroll
+      for (int u = 0; u < G; ++u) {
+        float dx, dy, dz;
+        if (compact) {
+          dx = (pj[u].x + S[u].x) - pi.x;
+          dy = (pj[u].y + S[u].y) - pi.y;
+          dz = (pj[u].z + S[u].z) - pi.z;
+        } else {
+          dx = pj[u].x - pi.x;
+          dy = pj[u].y - pi.y;
+          dz = pj[u].z - pi.z;
+          const float n1 = rintf(rd0 * dx + rd1 * dy), n2 = rintf(rd2 * dx + rd3 * dy);
+          dx -= n1 * rb0 + n2 * rb2;
+          dy -= n1 * rb1 + n2 * rb3;
+          dz -= Lz * rintf(dz * iLz);
+        }
+        const float rr = dx * dx + dy * dy + dz * dz;
+        const bool use = ok[u] && rr < rcf;
+        // Table coefficients on every lane (index clamped into the table; the
+        // hardware square root, 1 ulp, is ample for the lookup), so the common
+        // path has no branch and the partner loads stay ahead of it; pairs
+        // closer than the table take the closed form (rare); unused lanes get
+        // zero coefficients.
+        const float x = fmaxf(__builtin_amdgcn_sqrtf(rr) - RTAB_R0, 0.f) * (float)RTAB_INV;
+        const int it = min((int)x, ntab1);
+        const float t = x - (float)it;
+        float4 ca, cb;
+        if constexpr (TAB_LDS) {
+          ca = stab[2 * it];
+          cb = stab[2 * it + 1];
+        } else {
+          ca = a.rtab[R.tab0 + 2 * it];
+          cb = a.rtab[R.tab0 + 2 * it + 1];
+        }
+        float t1 = ca.x + t * (ca.y + t * (ca.z + t * ca.w));
+        float b2 = cb.x + t * (cb.y + t * (cb.z + t * cb.w));
+        if (use && rr < RTAB_R0 * RTAB_R0) {
+          float t2;
+          rpy_real_near_f32(sqrtf(rr), rr, alpha, t1, t2);
+          b2 = t2 / rr;
+        }
+        t1 = use ? t1 : 0.f;
+        b2 = use ? b2 : 0.f;
+        const float dv = (dx * vj[u].x + dy * vj[u].y + dz * vj[u].z) * b2;
+        px = fmaf(dv, dx, fmaf(t1, vj[u].x, px));
+        py = fmaf(dv, dy, fmaf(t1, vj[u].y, py));
+        pz = fmaf(dv, dz, fmaf(t1, vj[u].z, pz));
+      }
+    }
+    acc[0].add(px);
+    acc[1].add(py);
+    acc[2].add(pz);
   }
   double u0 = acc[0].value(), u1 = acc[1].value(), u2 = acc[2].value();
   for (int o = 1; o < 8; o <<= 1) {
@@ -1099,6 +1213,7 @@ template <typename T, int P> static void launch_mob_p(hipStream_t st, const MobA
       launch_fill_i32(st, a.brick_cnt + a.brick_begin, 0, a.brick_len);
       launch_fill_i32(st, a.brick_fill + a.brick_begin, 0, a.brick_len);
       k_bead_prep<T, P><<<nt, 256, 0, st>>>(a);
+      if (a.p32 && a.lcl && a.ncl > 0) k_cluster_prep<<<(a.ncl + 255) / 256, 256, 0, st>>>(a);
       launch_scan(st, a.brick_cnt + a.brick_begin, a.brick_start + a.brick_begin, a.brick_len);
       k_brick_fill<<<nt, 256, 0, st>>>(a);
       if (a.brick_len > 0) k_brick_sort<<<(a.brick_len + 255) / 256, 256, 0, st>>>(a);
