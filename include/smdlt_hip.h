@@ -36,6 +36,8 @@ extern "C" {
 #define SMDLT_E_STATE (-7)           /* call order violated (e.g. no positions uploaded) */
 #define SMDLT_E_NOMEM (-8)
 #define SMDLT_E_MC_STREAM (-9)       /* MC ran out of host-fed uniforms */
+#define SMDLT_E_NONFINITE (-10)      /* a step produced a non-finite displacement; it was not applied (positions
+                                        stay finite) and the step's other beads moved as usual */
 
 /* Flow types: the reference's -o flag and its two programs. */
 #define SMDLT_FLOW_EQUILIBRIUM 0
@@ -74,7 +76,8 @@ typedef struct smdlt_params {
   double lambda_d;         /* -l Debye length */
   double barrier, barrier2;/* -x, -y */
   int lj_ext_form;         /* 0: shear LJ 12eps(r6^2-r6), rc=5; 1: extensional WCA 48eps form */
-  int pbc_compat;          /* 1: reproduce the shear +-delx wrap shift (S/main26.c:1412-1423) */
+  int pbc_compat;          /* 1: reproduce the shear +-delx wrap shift (S/main26.c:1412-1423) for parity
+                              runs; 0 (default): the exact Lees-Edwards wrap */
   /* mobility (this build) */
   int precision;           /* SMDLT_PREC_* */
   int noise_mode;          /* SMDLT_NOISE_* */

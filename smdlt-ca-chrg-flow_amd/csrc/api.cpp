@@ -480,6 +480,7 @@ const char* smdlt_strerror(int code) {
     case SMDLT_E_STATE: return "call order violated";
     case SMDLT_E_NOMEM: return "device allocation failed";
     case SMDLT_E_MC_STREAM: return "MC uniform stream exhausted";
+    case SMDLT_E_NONFINITE: return "non-finite displacement (not applied)";
     default: return "unknown error";
   }
 }
@@ -502,7 +503,7 @@ void smdlt_default_params(smdlt_params* p) {
   p->barrier = 3.0;
   p->barrier2 = 3.0;
   p->lj_ext_form = 0;
-  p->pbc_compat = 1;
+  p->pbc_compat = 0;  // exact Lees-Edwards wrap; 1 reproduces the reference's +-delx shift (parity runs)
   p->precision = SMDLT_PREC_MIXED;
   p->noise_mode = SMDLT_NOISE_LANCZOS;
   p->mobility_tol = 1e-5;
@@ -1225,6 +1226,12 @@ static int scatter_aos(smdlt_ctx* c, const double* d, double* h) {
 
 int smdlt_upload_positions(smdlt_ctx* c, const double* rx, const double* ry, const double* rz) {
   if (!c || !rx || !ry || !rz) return SMDLT_E_ARG;
+  {  // the remap, cell and brick indices need finite positions
+    int n = 0;
+    for (int r = 0; r < c->nrep; ++r) n += c->rh[r].n;
+    for (int k = 0; k < n; ++k)
+      if (!std::isfinite(rx[k]) || !std::isfinite(ry[k]) || !std::isfinite(rz[k])) return SMDLT_E_ARG;
+  }
   HCK(hipSetDevice(c->device));
   int st = gather_soa(c, rx, c->rx);
   st = st ? st : gather_soa(c, ry, c->ry);
@@ -1546,7 +1553,7 @@ static int check_err(smdlt_ctx* c) {
   HCK(hipStreamSynchronize(c->stream));
   if (e) {
     HCK(hipMemset(c->d_err, 0, 4));
-    return SMDLT_E_FENE_OVERSTRETCH;
+    return (e & 1) ? SMDLT_E_FENE_OVERSTRETCH : SMDLT_E_NONFINITE;
   }
   return 0;
 }
@@ -2227,7 +2234,7 @@ static int step_internal(smdlt_ctx* c, uint64_t t, double* noise_dev, bool host_
   {
     StageScope sc(c, ST_UPDATE);
     launch_update_positions(c->stream, c->d_reps, c->d_slot_rep, s0, ns, c->rx, c->ry, c->rz, c->mf, c->bz, c->dxv,
-                            c->dyv, c->dzv, c->dsp, 1);
+                            c->dyv, c->dzv, c->dsp, c->d_err);
     charge_log_internal(c);
     c->pos_version++;
   }
@@ -2336,7 +2343,7 @@ static int build_step_graphs(smdlt_ctx* c, uint64_t t) {
   return capture_graph(c, &c->gtail, [&]() -> int {
     lz_finish(c, c->bz);
     launch_update_positions(c->stream, c->d_reps, c->d_slot_rep, s0, ns, c->rx, c->ry, c->rz, c->mf, c->bz, c->dxv,
-                            c->dyv, c->dzv, c->dsp, 1);
+                            c->dyv, c->dzv, c->dsp, c->d_err);
     charge_log_internal(c);
     c->pos_version++;
     return 0;

@@ -264,7 +264,7 @@ void launch_bin_index(hipStream_t s, const RepDev* rep, const double* rx, const 
                       int npairs, const int* pi, const int* pj, const double* bins, const int* nbins, int* out);
 void launch_update_positions(hipStream_t s, const RepDev* reps, const int* slot_rep, int slot_begin, int nslots,
                              double* rx, double* ry, double* rz, const double* mf, const double* bz,
-                             double* dxv, double* dyv, double* dzv, double* dsp, int noise_is_scaled);
+                             double* dxv, double* dyv, double* dzv, double* dsp, int* err);
 void launch_disp_max(hipStream_t s, const RepDev* reps, int rep_begin, int nrep, const double* dxv,
                      const double* dyv, const double* dzv, const double* dsp, double* out2);
 void launch_noise(hipStream_t s, const int* slot_rep, const RepDev* reps, int slot_begin, int nslots,

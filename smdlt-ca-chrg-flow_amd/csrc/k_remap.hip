@@ -150,7 +150,8 @@ void launch_bin_index(hipStream_t st, const RepDev* rep, const double* rx, const
 // tracks the displacement relative to the affine flow for the mobility list.
 __global__ void k_update(const RepDev* __restrict__ reps, const int* __restrict__ slot_rep, int slot_begin,
                          int nslots, double* rx, double* ry, double* rz, const double* __restrict__ mf,
-                         const double* __restrict__ bz, double* dxv, double* dyv, double* dzv, double* dsp) {
+                         const double* __restrict__ bz, double* dxv, double* dyv, double* dzv, double* dsp,
+                         int* err) {
   int s = slot_begin + blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= slot_begin + nslots) return;
   int r = slot_rep[s];
@@ -159,9 +160,11 @@ __global__ void k_update(const RepDev* __restrict__ reps, const int* __restrict_
   if (!R.active || s - R.slot0 >= R.n) return;
   double d[3];
   for (int c = 0; c < 3; ++c) d[c] = R.dt * mf[3 * s + c] + R.sqrt2dt * bz[3 * s + c];
-  dsp[3 * s] += d[0];
-  dsp[3 * s + 1] += d[1];
-  dsp[3 * s + 2] += d[2];
+  if (isfinite(d[0]) && isfinite(d[1]) && isfinite(d[2])) {
+    dsp[3 * s] += d[0];
+    dsp[3 * s + 1] += d[1];
+    dsp[3 * s + 2] += d[2];
+  }
   double x = rx[s], y = ry[s], z = rz[s];
   double ux, uy;
   if (R.flow == 2) {
@@ -176,6 +179,13 @@ __global__ void k_update(const RepDev* __restrict__ reps, const int* __restrict_
     uy = d[1];
   }
   double uz = d[2];
+  if (!isfinite(ux) || !isfinite(uy) || !isfinite(uz)) {
+    // a non-finite displacement (overflowing forces, coincident beads) is not
+    // applied, so every later kernel still sees finite positions; the step
+    // reports SMDLT_E_NONFINITE
+    atomicOr(err, 2);
+    ux = uy = uz = 0.0;
+  }
   dxv[s] += ux;
   dyv[s] += uy;
   dzv[s] += uz;
@@ -186,11 +196,10 @@ __global__ void k_update(const RepDev* __restrict__ reps, const int* __restrict_
 
 void launch_update_positions(hipStream_t st, const RepDev* reps, const int* slot_rep, int slot_begin, int nslots,
                              double* rx, double* ry, double* rz, const double* mf, const double* bz, double* dxv,
-                             double* dyv, double* dzv, double* dsp, int noise_is_scaled) {
-  (void)noise_is_scaled;
+                             double* dyv, double* dzv, double* dsp, int* err) {
   if (nslots <= 0) return;
   k_update<<<(nslots + 255) / 256, 256, 0, st>>>(reps, slot_rep, slot_begin, nslots, rx, ry, rz, mf, bz, dxv, dyv,
-                                                 dzv, dsp);
+                                                 dzv, dsp, err);
 }
 
 // Per replica: out2[2r] = max |dxv|^2, out2[2r+1] = max |dsp|^2.

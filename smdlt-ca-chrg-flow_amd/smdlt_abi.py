@@ -20,7 +20,7 @@ MC_REFERENCE, MC_MH_UNIFORM, MC_MH_LEARNED = 0, 1, 2
 
 ERRORS = {0: "ok", -1: "SMDLT_E_ARG", -2: "SMDLT_E_HIP", -3: "SMDLT_E_FENE_OVERSTRETCH",
           -4: "SMDLT_E_LANCZOS_NOCONV", -5: "SMDLT_E_NLIST_OVERFLOW", -6: "SMDLT_E_GEOMETRY",
-          -7: "SMDLT_E_STATE", -8: "SMDLT_E_NOMEM", -9: "SMDLT_E_MC_STREAM"}
+          -7: "SMDLT_E_STATE", -8: "SMDLT_E_NOMEM", -9: "SMDLT_E_MC_STREAM", -10: "SMDLT_E_NONFINITE"}
 
 
 class Params(C.Structure):

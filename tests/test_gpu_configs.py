@@ -51,7 +51,7 @@ def dense_sqrt(M, z):
 def c1_ctx(**kw):
     p = S.default_params(nb=100, nc=1, L=60.0, flow_type=S.FLOW_SHEAR, edot=10.0 / 175.0, dt=1e-3, spring_fene=1,
                          kappas=30.0, qmax=10.0, precision=S.PREC_FP64, lanczos_tol=1e-6, lanczos_maxit=30,
-                         **kw)
+                         pbc_compat=1, **kw)
     ctx = S.Context(p)
     pos = W.chains(100, 1, 60.0, 8)
     ctx.upload_positions(*xyz(pos))
@@ -242,7 +242,7 @@ def hop_system():
 def hop_ctx(hs, **kw):
     nb, nc, L, t, pos, track = hs
     p = S.default_params(nb=nb, nc=nc, L=L, flow_type=S.FLOW_SHEAR, edot=50.0 / 175.0, dt=5e-4, lambda_b=3.59,
-                         lambda_d=5.0, hop_radius=3.0, **kw)
+                         lambda_d=5.0, hop_radius=3.0, pbc_compat=1, **kw)
     ctx = S.Context(p)
     ctx.upload_positions(*xyz(pos))
     ctx.remap(t)

@@ -62,7 +62,7 @@ def test_stress_of_batched_replicas():
     # two replicas with different chains: each replica's sums are its own
     specs = [(30, 4, 26.0, 1), (20, 6, 30.0, 2)]
     ps = [S.default_params(nb=nb, nc=nc, L=L, flow_type=fl, edot=0.2, dt=1e-3, kappas=200.0, qmax=2.0, kappab=0.5,
-                           lj_ext_form=1 if fl == 2 else 0, seed=0x5EED0000 + r)
+                           lj_ext_form=1 if fl == 2 else 0, seed=0x5EED0000 + r, pbc_compat=1)
           for r, (nb, nc, L, fl) in enumerate(specs)]
     ctx = S.Context(ps)
     pos = [random_walk(nb, nc, L, 20 + r) for r, (nb, nc, L, _) in enumerate(specs)]
@@ -91,7 +91,8 @@ def test_chain_observables_equal_oracle(flow):
 
 def test_chain_observables_of_batched_replicas():
     specs = [(30, 4, 26.0, 1), (20, 6, 30.0, 2), (50, 1, 40.0, 1)]
-    ps = [S.default_params(nb=nb, nc=nc, L=L, flow_type=fl, edot=0.2, dt=1e-3, lj_ext_form=1 if fl == 2 else 0)
+    ps = [S.default_params(nb=nb, nc=nc, L=L, flow_type=fl, edot=0.2, dt=1e-3, lj_ext_form=1 if fl == 2 else 0,
+                           pbc_compat=1)
           for nb, nc, L, fl in specs]
     ctx = S.Context(ps)
     pos = [random_walk(nb, nc, L, 40 + r) for r, (nb, nc, L, _) in enumerate(specs)]

@@ -438,18 +438,81 @@ def test_step_matches_dense_update(flow, mode):
     assert e < 1e-5
 
 
-def test_graph_replay_matches_eager():
+@pytest.mark.parametrize("prec", [S.PREC_FP64, S.PREC_MIXED])
+def test_graph_replay_matches_eager(prec):
+    # replayed steps run the same kernels on the same data in the same order as
+    # eager ones (a converged replica's further Lanczos iterations are skipped
+    # by every kernel), so the trajectories agree bit for bit
     L, nb, nc = 30.0, 50, 4
     out = []
     for graph in (False, True):
-        ctx, p = ctx_for(1, L, nb, nc, prec=S.PREC_FP64, edot=0.3, dt=5e-4)
+        ctx, p = ctx_for(1, L, nb, nc, prec=prec, edot=0.3, dt=5e-4)
         rx, ry, rz = xyz(random_walk(nb, nc, L, 10))
         ctx.upload_positions(rx, ry, rz)
         ctx.run(0, 6, use_graph=graph)
         out.append(np.stack(ctx.download_positions(), 1))
         ctx.close()
-    print("graph vs eager max diff", np.abs(out[0] - out[1]).max())
-    assert np.abs(out[0] - out[1]).max() < 1e-9
+    assert np.array_equal(out[0], out[1])
+
+
+def test_nonfinite_displacement_is_reported_and_not_applied():
+    # Hookean springs so stiff that the forces overflow: the free-draining step
+    # reports SMDLT_E_NONFINITE and leaves the affected beads where they were
+    L, nb, nc = 30.0, 10, 2
+    ctx, p = ctx_for(0, L, nb, nc, prec=S.PREC_FP64, edot=0.0, dt=1e-3, kappas=1.7e308, qmax=0.0,
+                     noise_mode=S.NOISE_FREE_DRAINING)  # |F| = kappas |d|: bond components above 1.06 overflow
+    pos = random_walk(nb, nc, L, 3)
+    ctx.upload_positions(*xyz(pos))
+    with pytest.raises(S.SmdltError) as e:
+        ctx.step(0)
+    assert e.value.code == -10
+    x, y, z = ctx.download_positions()
+    assert np.all(np.isfinite(x)) and np.all(np.isfinite(y)) and np.all(np.isfinite(z))
+    # a non-finite upload is refused on the host
+    bad = pos.copy()
+    bad[3, 1] = np.nan
+    with pytest.raises(S.SmdltError) as e:
+        ctx.upload_positions(*xyz(bad))
+    assert e.value.code == -1
+    ctx.close()
+
+
+def test_device_noise_moments_and_independence():
+    # free draining with no forces and no flow: each step moves a bead by
+    # sqrt(2 dt) z with z the device noise (Philox4x32-10 + Box-Muller)
+    L, nb, nc, dt = 400.0, 100, 100, 1e-2
+    ctx, p = ctx_for(0, L, nb, nc, prec=S.PREC_FP64, edot=0.0, dt=dt, kappas=0.0, epsilon=0.0,
+                     noise_mode=S.NOISE_FREE_DRAINING)
+    g = np.linspace(-150, 150, 22)
+    pos = np.stack(np.meshgrid(g, g, g[:21], indexing="ij"), -1).reshape(-1, 3)[:nb * nc]  # 14 apart: no LJ
+    ctx.upload_positions(*xyz(pos))
+    zs = []
+    for t in range(3):
+        a = np.stack(ctx.download_positions(), 1)
+        ctx.step(t)
+        b = np.stack(ctx.download_positions(), 1)
+        zs.append(((b - a) / np.sqrt(2 * dt)).reshape(-1))
+    z = np.concatenate(zs)
+    n = len(z)
+    assert abs(z.mean()) < 5 / np.sqrt(n)
+    assert abs(z.var() - 1) < 5 * np.sqrt(2 / n)
+    assert abs((z ** 3).mean()) < 5 * np.sqrt(15 / n)
+    assert abs((z ** 4).mean() - 3) < 5 * np.sqrt(96 / n)
+    # components, neighbouring beads and consecutive steps are uncorrelated
+    Z = zs[0].reshape(-1, 3)
+    for c1, c2 in ((0, 1), (1, 2), (0, 2)):
+        assert abs(np.corrcoef(Z[:, c1], Z[:, c2])[0, 1]) < 5 / np.sqrt(len(Z))
+    assert abs(np.corrcoef(Z[:-1, 0], Z[1:, 0])[0, 1]) < 5 / np.sqrt(len(Z))
+    assert abs(np.corrcoef(zs[0], zs[1])[0, 1]) < 5 / np.sqrt(len(zs[0]))
+    # the histogram follows the normal density (chi^2 over 20 bins)
+    from math import erf
+    edges = np.linspace(-3, 3, 21)
+    cdf = np.array([0.5 * (1 + erf(e / np.sqrt(2))) for e in edges])
+    expct = n * np.diff(cdf)
+    obs = np.histogram(z, edges)[0]
+    chi2 = ((obs - expct) ** 2 / expct).sum()
+    assert chi2 < 60  # 19 degrees of freedom; p ~ 3e-6
+    ctx.close()
 
 
 @pytest.mark.parametrize("graph", [False, True])
