@@ -336,6 +336,55 @@ int smdlt_init_charges(int n, int ncharges, smdlt_ran1* s, int* track);
  * SMDLT_E_ARG (no file, no frame, or a frame of another size); *t may be NULL. */
 int smdlt_read_restart(const char* path, int n, double* rx, double* ry, double* rz, uint64_t* t);
 
+/* ------------------------------------------- TEA averages and hand-off (host)
+ * Iteration 0 of the reference samples the TEA parameters (smdlt_tea_sample)
+ * and averages them per strain bin; iteration 1 steps with the bin averages
+ * (SMDLT_NOISE_TEA).  One averaging object per replica. */
+typedef struct smdlt_tea_avg smdlt_tea_avg;
+/* The running sums of CATEA (S/main26.c:1317-1381, S/Initialization.c:401-428):
+ * sbins + 1 strain bins of width strain_interval (the last one collects every
+ * larger strain), per bin the sum of beta, of C by index within a chain
+ * (3 nb values) and the sample count. */
+int smdlt_tea_avg_create(int nb, int nc, int sbins, double strain_interval, smdlt_tea_avg** out);
+void smdlt_tea_avg_destroy(smdlt_tea_avg* avg);
+/* CATEA's bookkeeping for one sample at step t and strain (S/main26.c:1346-1366):
+ * bin = floor(strain / strain_interval) capped at sbins; beta and C[3 nb nc]
+ * (one replica's smdlt_tea_sample output) added to the bin.  beta_path (may be
+ * NULL) gets CATEA's decomp/B record, "t beta" then "i Cx Cy Cz" per bead
+ * (S/main26.c:1371-1376).  Returns the bin. */
+int smdlt_tea_avg_add(smdlt_tea_avg* avg, uint64_t t, double strain, double beta, const double* C,
+                      const char* beta_path);
+/* printTEA (S/main26.c:1807-1826): writes the decomp/DC file: "t", then per bin
+ * "bin count mean-beta" and nb rows "j Cx Cy Cz" of mean C (zeros for an empty bin). */
+int smdlt_tea_avg_write(const smdlt_tea_avg* avg, uint64_t t, const char* path);
+/* resetAverage (S/main26.c:1827-1839): the per-bin means for iteration 1,
+ * beta[sbins + 1] and C[(sbins + 1) * 3 nb] (an empty bin gives 0/0, as in the
+ * reference), counts[sbins + 1]; the sums restart from zero.  Any output may be NULL. */
+int smdlt_tea_avg_finish(smdlt_tea_avg* avg, double* beta, double* C, int* counts);
+/* The decomp/DC reader of a restarted iteration (S/Initialization.c:522-540). */
+int smdlt_tea_read(const char* path, int nb, int sbins, double* beta, double* C, uint64_t* t);
+/* updateHI's choice of coefficients at a strain (S/main26.c:757-764): the bin's
+ * beta and C[3 nb nc] (C of bead i is the bin's entry i mod 3 nb), ready for
+ * smdlt_set_tea.  Returns the bin. */
+int smdlt_tea_coefficients(int nb, int nc, int sbins, double strain_interval, double strain, const double* beta_bins,
+                           const double* C_bins, double* beta, double* C);
+
+/* ------------------------------------------------ MSD post-processing (host)
+ * calcMSD (S/main26.c:2494-2637): reads the calcCH file (ncharges lines "t j old
+ * track dx dy dz" for each of the tsteps + 1 output steps, tsteps = tmax /
+ * printprops), and writes prop/MSD rows "tau,MSD,MSDx,MSDy,MSDz" for shifts
+ * 1 .. tsteps - msd_start, tau = shift dt printprops, averaged over the first
+ * min(25, ncharges) charges with the reference's index layout (series q > 0
+ * read from q tsteps, one entry before the slot its sort put them in).
+ * out_path and out (rows x 5 doubles, at most cap rows) may be NULL.  Returns
+ * the row count. */
+int smdlt_msd(const char* ch_path, int ncharges, int tsteps, int msd_start, double dt, int printprops,
+              const char* out_path, double* out, int cap);
+/* calcMSDcom (X/main14.c:3096-3214): one series of lines "t dx dy dz", shifts
+ * 1 .. tsteps - 1, windows from the second entry. */
+int smdlt_msd_com(const char* com_path, int tsteps, double dt, int printprops, const char* out_path, double* out,
+                  int cap);
+
 #ifdef __cplusplus
 }
 #endif

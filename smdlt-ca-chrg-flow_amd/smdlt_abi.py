@@ -112,6 +112,16 @@ def lib():
         L.smdlt_mlp_set_weights.argtypes = [C.c_void_p, _fp]
         L.smdlt_mlp_loss_grad.argtypes = [C.c_void_p, _fp, C.c_void_p, C.c_void_p, C.c_int, _fp, _fp]
         L.smdlt_mc_features.argtypes = [C.c_void_p, C.c_int, _fp, _ip, _ip, _ip, C.c_int, _ip]
+        L.smdlt_tea_avg_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.POINTER(C.c_void_p)]
+        L.smdlt_tea_avg_destroy.argtypes = [C.c_void_p]
+        L.smdlt_tea_avg_destroy.restype = None
+        L.smdlt_tea_avg_add.argtypes = [C.c_void_p, C.c_uint64, C.c_double, C.c_double, _dp, C.c_char_p]
+        L.smdlt_tea_avg_write.argtypes = [C.c_void_p, C.c_uint64, C.c_char_p]
+        L.smdlt_tea_avg_finish.argtypes = [C.c_void_p, _dp, _dp, _ip]
+        L.smdlt_tea_read.argtypes = [C.c_char_p, C.c_int, C.c_int, _dp, _dp, C.POINTER(C.c_uint64)]
+        L.smdlt_tea_coefficients.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, _dp, _dp, _dp, _dp]
+        L.smdlt_msd.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_char_p, _dp, C.c_int]
+        L.smdlt_msd_com.argtypes = [C.c_char_p, C.c_int, C.c_double, C.c_int, C.c_char_p, _dp, C.c_int]
         _lib = L
     return _lib
 
@@ -436,50 +446,84 @@ class Context:
 
 
 class TeaAverages:
-    """CATEA's running averages for one replica (S/main26.c:1317-1381): per strain bin
-    floor(strain/strain_interval), capped at sbins, the sums of beta and of C by bead index
-    within a chain (i mod 3Nb), and the sample count; printTEA's decomp/DC text
-    (S/main26.c:1807-1826) and the per-bin coefficients updateHI uses (S/main26.c:760-764)."""
+    """CATEA's running averages for one replica (smdlt_tea_avg_*; S/main26.c:1317-1381):
+    per strain bin floor(strain/strain_interval), capped at sbins, the sums of beta and of C
+    by index within a chain (i mod 3Nb) and the sample count; printTEA's decomp/DC text
+    (S/main26.c:1807-1826), resetAverage's means (S/main26.c:1827-1839) and the
+    coefficients updateHI reads at a strain (S/main26.c:757-764)."""
 
-    def __init__(self, nb, nc, sbins=20):
-        self.nb, self.nc, self.sbins = nb, nc, sbins
-        self.brun = np.zeros(sbins + 1)
-        self.crun = np.zeros((sbins + 1, 3 * nb))
-        self.count = np.zeros(sbins + 1, np.int64)
+    def __init__(self, nb, nc, sbins=20, strain_interval=1.0):
+        self.nb, self.nc, self.sbins, self.strain_interval = nb, nc, sbins, strain_interval
+        h = C.c_void_p()
+        _chk(lib().smdlt_tea_avg_create(nb, nc, sbins, strain_interval, C.byref(h)), "tea_avg_create")
+        self.h = h
 
-    def bin_of(self, strain, strain_interval):
-        return min(int(np.floor(strain / strain_interval)), self.sbins)
+    def __del__(self):
+        try:
+            if self.h:
+                lib().smdlt_tea_avg_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
 
-    def add(self, strain, strain_interval, beta, Cv):
-        """one sample: beta and C[3N] of this replica (smdlt_tea_sample)"""
-        b = self.bin_of(strain, strain_interval)
-        self.brun[b] += beta
-        self.crun[b] += np.asarray(Cv, np.float64).reshape(self.nc, 3 * self.nb).sum(0)
-        self.count[b] += 1
-        return b
-
-    def coefficients(self, b):
-        """(beta, C[3N]) of bin b as updateHI reads them: C of bead i is the bin's average for i mod Nb"""
-        if self.count[b] == 0:
-            raise ValueError(f"strain bin {b} has no samples")
-        beta = self.brun[b] / self.count[b]
-        cb = self.crun[b] / (self.nc * self.count[b])
-        return beta, np.tile(cb, self.nc)
+    def add(self, strain, beta, Cv, t=0, beta_path=None):
+        """one sample (smdlt_tea_sample's beta and C[3N] of this replica); returns its bin;
+        beta_path gets CATEA's decomp/B record"""
+        Cv = np.ascontiguousarray(Cv, np.float64).reshape(-1)
+        return _chk(lib().smdlt_tea_avg_add(self.h, int(t), float(strain), float(beta), _ptr(Cv, C.c_double),
+                                            beta_path.encode() if beta_path else None), "tea_avg_add")
 
     def write_decomp(self, path, t):
-        """printTEA's decomp/DC layout: t, then per bin "bin count beta" and Nb rows "j Cx Cy Cz" """
-        with open(path, "w") as f:
-            f.write(f"{int(t)}\n")
-            for i in range(self.sbins + 1):
-                n = int(self.count[i])
-                if n > 0:
-                    f.write(f"{i} {n} {self.brun[i] / n:.12e}\n")
-                    c = self.crun[i] / (self.nc * n)
-                else:
-                    f.write(f"{i} 0 {0.0:.12e}\n")
-                    c = np.zeros(3 * self.nb)
-                for j in range(self.nb):
-                    f.write(f"{j} {c[3 * j]:.12e} {c[3 * j + 1]:.12e} {c[3 * j + 2]:.12e}\n")
+        _chk(lib().smdlt_tea_avg_write(self.h, int(t), path.encode()), "tea_avg_write")
+
+    def finish(self):
+        """resetAverage: per-bin (beta[sbins+1], C[sbins+1, 3Nb], counts); the sums restart"""
+        beta = np.zeros(self.sbins + 1)
+        Cb = np.zeros((self.sbins + 1) * 3 * self.nb)
+        cnt = np.zeros(self.sbins + 1, np.int32)
+        _chk(lib().smdlt_tea_avg_finish(self.h, _ptr(beta, C.c_double), _ptr(Cb, C.c_double), _ptr(cnt, C.c_int)),
+             "tea_avg_finish")
+        return beta, Cb.reshape(self.sbins + 1, 3 * self.nb), cnt
+
+
+def tea_read(path, nb, sbins=20):
+    """The decomp/DC reader (smdlt_tea_read): (beta[sbins+1], C[sbins+1, 3Nb], t)."""
+    beta = np.zeros(sbins + 1)
+    Cb = np.zeros((sbins + 1) * 3 * nb)
+    t = C.c_uint64()
+    _chk(lib().smdlt_tea_read(path.encode(), nb, sbins, _ptr(beta, C.c_double), _ptr(Cb, C.c_double), C.byref(t)),
+         "tea_read")
+    return beta, Cb.reshape(sbins + 1, 3 * nb), t.value
+
+
+def tea_coefficients(nb, nc, sbins, strain_interval, strain, beta_bins, C_bins):
+    """updateHI's (beta, C[3N]) at a strain (smdlt_tea_coefficients) and its bin."""
+    bb = np.ascontiguousarray(beta_bins, np.float64).reshape(-1)
+    cb = np.ascontiguousarray(C_bins, np.float64).reshape(-1)
+    beta = np.zeros(1)
+    Cv = np.zeros(3 * nb * nc)
+    b = _chk(lib().smdlt_tea_coefficients(nb, nc, sbins, strain_interval, strain, _ptr(bb, C.c_double),
+                                          _ptr(cb, C.c_double), _ptr(beta, C.c_double), _ptr(Cv, C.c_double)),
+             "tea_coefficients")
+    return float(beta[0]), Cv, b
+
+
+def msd(ch_path, ncharges, tsteps, msd_start, dt, printprops, out_path=None):
+    """calcMSD (smdlt_msd): rows (tau, MSD, MSDx, MSDy, MSDz) from a calcCH file."""
+    cap = max(1, tsteps)
+    out = np.zeros(5 * cap)
+    n = _chk(lib().smdlt_msd(ch_path.encode(), ncharges, tsteps, msd_start, dt, printprops,
+                             out_path.encode() if out_path else None, _ptr(out, C.c_double), cap), "msd")
+    return out[:5 * n].reshape(-1, 5)
+
+
+def msd_com(com_path, tsteps, dt, printprops, out_path=None):
+    """calcMSDcom (smdlt_msd_com): rows (tau, MSD, MSDx, MSDy, MSDz) from "t dx dy dz" lines."""
+    cap = max(1, tsteps)
+    out = np.zeros(5 * cap)
+    n = _chk(lib().smdlt_msd_com(com_path.encode(), tsteps, dt, printprops, out_path.encode() if out_path else None,
+                                 _ptr(out, C.c_double), cap), "msd_com")
+    return out[:5 * n].reshape(-1, 5)
 
 
 # ------------------------------------------------------------- output files
