@@ -154,6 +154,14 @@ class Ran1:
         self.s = RNG()
         lib().orc_rng_seed(C.byref(self.s), int(seed))
 
+    def copy(self):
+        """An independent stream in the same state (ran1's statics and gasdev's
+        cached deviate included)."""
+        other = Ran1.__new__(Ran1)
+        other.s = RNG()
+        C.memmove(C.byref(other.s), C.byref(self.s), C.sizeof(RNG))
+        return other
+
     def uniforms(self, n):
         out = np.empty(n, np.float32)
         lib().orc_ran1_fill(C.byref(self.s), n, out)
