@@ -103,13 +103,35 @@ def test_verlet_list_bit_exact(flow, compat, nb, nc, L):
 
 
 def test_verlet_trigger_follows_reference():
-    # rebuild only when some accumulated displacement exceeds (rv - rc)/2 = 0.5
-    L = 30.0
-    ctx, p = ctx_for(1, L, 30, 3, edot=0.0)
-    rx, ry, rz = xyz(random_walk(30, 3, L, 6))
-    ctx.upload_positions(rx, ry, rz)
+    """checkVerlet (S/main26.c:858-869): the lists are rebuilt when some bead's
+    displacement summed as a vector since the last build (dxv, S/main26.c:726,742,849)
+    exceeds rnew = (rv - rc)/2 = 0.5 (S/Initialization.c:303-307).  Force-free
+    free-draining steps move bead 0 by exactly sqrt(2 dt) z."""
+    L, dt = 30.0, 1e-3
+    ctx, p = ctx_for(1, L, 2, 8, edot=0.0, dt=dt, kappas=0.0, epsilon=0.0, noise_mode=S.NOISE_FREE_DRAINING,
+                     prec=S.PREC_FP64)
+    pos = np.array([[x, y, z] for x in (-10.5, -3.5, 3.5, 10.5) for y in (-7.5, 7.5) for z in (-7.5, 7.5)])
+    ctx.upload_positions(*xyz(pos))
     assert ctx.neighbors(False) == 1  # first call builds
     assert ctx.neighbors(False) == 0
+    n = len(pos)
+
+    def move(dx):  # one step moving bead 0 by dx in x, every other bead by 0
+        z = np.zeros(3 * n)
+        z[0] = dx / np.sqrt(2 * dt)
+        ctx.step(1, z)
+        return ctx.neighbors(False)
+
+    # vector sum: +0.3, -0.3, +0.45 travels 1.05 but sits 0.45 from the build
+    assert [move(0.3), move(-0.3), move(0.45)] == [0, 0, 0]
+    assert move(0.04) == 0  # 0.49
+    assert move(0.02) == 1  # 0.51 > 0.5: rebuilt, sums reset
+    assert move(0.49) == 0
+    assert move(-0.98) == 0  # |-0.49|
+    assert move(-0.02) == 1
+    gx, _, _ = ctx.download_positions()
+    assert abs(gx[0] - (pos[0, 0] + 0.3 - 0.3 + 0.45 + 0.04 + 0.02 + 0.49 - 0.98 - 0.02)) < 1e-12
+    ctx.close()
 
 
 # ----------------------------------------------------------------- forces
