@@ -289,6 +289,9 @@ struct smdlt_ctx {
   int nq_total = 0, max_nq = 0;
   int *qtrack = nullptr, *qsign = nullptr, *qold = nullptr, *qidx = nullptr, *mc_err = nullptr;
   unsigned char* occ = nullptr;
+  int *mc_start = nullptr, *mc_jf = nullptr;  // reference sweep: start beads, j* per charge
+  double* mc_de = nullptr;                     // reference sweep: candidate energy changes [nq_total][3]
+  void* mc_state = nullptr;                    // reference sweep: per-replica cursor, draws, E - E_initial
   long long occ_stride = 0;
   int* log_off = nullptr;
   float4* p32 = nullptr;      // mixed precision: FP32 positions and scaled input vector
@@ -1142,7 +1145,8 @@ void smdlt_destroy(smdlt_ctx* c) {
                   c->cell_of, c->cell_cnt, c->cell_start, c->cell_fill, c->cell_beads, c->cell_tmp, c->rep_flags, c->out2, c->aff,
                   c->partial2, c->grid, c->bsp, c->tw, c->tx, c->ty, c->tz, c->txy, c->bricks, c->bbase, c->brick_cnt, c->brick_start,
                   c->brick_fill, c->brick_beads, c->bw, c->V, c->W, c->partial, c->scal, c->bdiv, c->done,
-                  c->iters, c->qtrack, c->qsign, c->qold, c->qidx, c->mc_err, c->mlp_w, c->occ, c->log_off};
+                  c->iters, c->qtrack, c->qsign, c->qold, c->qidx, c->mc_err, c->mlp_w, c->occ, c->log_off,
+                  c->mc_start, c->mc_jf, c->mc_de, c->mc_state};
   for (void* p : ptrs)
     if (p) hipFree(p);
   for (auto& t : c->timers) {
@@ -2474,10 +2478,13 @@ int smdlt_upload_charges(smdlt_ctx* c, const int* nq, const int* track, const in
       if (sign && sign[k] != 1 && sign[k] != -1) return SMDLT_E_ARG;
     }
   }
-  void* ptrs[] = {c->qtrack, c->qsign, c->qold, c->occ, c->log_off, c->mh_feats, c->mh_log, c->qdisp, c->qdisp_saved};
+  void* ptrs[] = {c->qtrack, c->qsign, c->qold, c->occ, c->log_off, c->mh_feats, c->mh_log, c->qdisp, c->qdisp_saved,
+                  c->mc_start, c->mc_jf, c->mc_de, c->mc_state};
   for (void* p : ptrs)
     if (p) hipFree(p);
-  c->qtrack = c->qsign = c->qold = c->log_off = c->mh_log = nullptr;
+  c->qtrack = c->qsign = c->qold = c->log_off = c->mh_log = c->mc_start = c->mc_jf = nullptr;
+  c->mc_de = nullptr;
+  c->mc_state = nullptr;
   c->qdisp = c->qdisp_saved = nullptr;
   c->occ = nullptr;
   c->mh_feats = nullptr;
@@ -2493,6 +2500,10 @@ int smdlt_upload_charges(smdlt_ctx* c, const int* nq, const int* track, const in
   st |= alloc_dev(&c->qdisp_saved, 3 * (size_t)std::max(1, tot));
   c->occ_stride = ((long long)maxn + 255) / 256 * 256;
   st |= alloc_dev(&c->occ, (size_t)c->occ_stride * c->nrep);
+  st |= alloc_dev(&c->mc_start, std::max(1, tot));
+  st |= alloc_dev(&c->mc_jf, std::max(1, tot));
+  st |= alloc_dev(&c->mc_de, 3 * (size_t)std::max(1, tot));
+  if (!st && hipMalloc(&c->mc_state, mc_state_bytes() * c->nrep) != hipSuccess) st = 1;
   if (st) return SMDLT_E_NOMEM;
   std::vector<int> sg(tot, 1);
   if (sign) sg.assign(sign, sign + tot);
@@ -2538,9 +2549,9 @@ static int mc_sweep_internal(smdlt_ctx* c, int mode, const float* du, int n_unif
                     mode == SMDLT_MC_MH_LEARNED ? c->mlp_w : nullptr, dcons, dlog, c->mh_feats, c->log_off,
                     c->mc_err, c->qdisp);
   else
-    launch_mc_sweep(c->stream, c->d_reps, rb, re - rb, c->rx, c->ry, c->rz, c->qtrack, c->qold, c->qsign, c->occ,
-                    c->occ_stride, du, du ? n_uniforms : 0, c->d_seeds, c->mc_sweeps++, dcons, dlog, c->log_off,
-                    c->mc_err, c->qdisp);
+    launch_mc_sweep(c->stream, c->d_reps, rb, re - rb, c->max_nq, c->rx, c->ry, c->rz, c->qtrack, c->qold, c->qsign,
+                    c->occ, c->occ_stride, c->mc_start, c->mc_jf, c->mc_de, c->mc_state, du, du ? n_uniforms : 0,
+                    c->d_seeds, c->mc_sweeps++, dcons, dlog, c->log_off, c->mc_err, c->qdisp);
   HCK(hipGetLastError());
   if (mh) c->mh_swept = true;
   HCK(hipStreamSynchronize(c->stream));

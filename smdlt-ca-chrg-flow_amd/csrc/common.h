@@ -431,9 +431,13 @@ void launch_mh_sweep(hipStream_t st, const RepDev* reps, int rep_begin, int nrep
                      const double* rz, int* track, int* old, const int* sign, signed char* occ, long long occ_stride,
                      const float* uniforms, int nu, const uint64_t* seeds, uint64_t sweep_id, const float* weights,
                      int* consumed, int* log, float* feats, const int* log_off, int* err, double* qdisp);
-void launch_mc_sweep(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, const double* rx, const double* ry,
-                     const double* rz, int* track, int* old, const int* sign, unsigned char* occ,
-                     long long occ_stride, const float* uniforms, int nu, const uint64_t* seeds, uint64_t sweep_id,
-                     int* consumed, int* log, const int* log_off, int* err, double* qdisp);
+// reference-mode sweep; start/jfirst (nq_total ints), de0 (3 nq_total doubles)
+// and state (nrep x mc_state_bytes()) are the sweep's working arrays
+size_t mc_state_bytes();
+void launch_mc_sweep(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, int max_nq, const double* rx,
+                     const double* ry, const double* rz, int* track, int* old, const int* sign, unsigned char* occ,
+                     long long occ_stride, int* start, int* jfirst, double* de0, void* state, const float* uniforms,
+                     int nu, const uint64_t* seeds, uint64_t sweep_id, int* consumed, int* log, const int* log_off,
+                     int* err, double* qdisp);
 void launch_mc_energy(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, const double* rx, const double* ry,
                       const double* rz, const int* track, const int* sign, double* e);

@@ -1,8 +1,7 @@
 // Charge-hopping Monte Carlo (chargeHop, S/main26.c:1926-2390; X/main14.c:1945-2409)
-// on the device: one workgroup per replica walks the charges in order (the
-// moves are sequential by definition) while the lanes share the O(N) scan for
-// the first nearby non-adjacent bead and the O(Nq) screened-Coulomb energy
-// change of each proposal.
+// on the device.  The moves are sequential by definition; the sweep computes
+// what does not depend on earlier moves for all charges at once and decides
+// the charges in blocked rounds (see k_mc_prep / k_mc_round / k_mc_fold).
 //
 // Reference mode keeps the reference's proposal state machine and acceptance
 // exp(-b - (E_final - E_initial)/2) with E_initial frozen for the sweep.  With
@@ -14,7 +13,9 @@
 //   none:      mm>.5  -> right, else left
 //   blocked (chain end or occupied target) -> skip without a second draw
 //   x = exp(-b - (E(proposed) - E_initial)/2); accept if 1 <= x, else draw U <= x.
-// E(proposed) is kept incrementally (E_cur + dE_local) in FP64.
+// E(proposed) - E_initial is kept incrementally in FP64 (the accepted changes
+// so far plus the proposal's), so a decision can differ from the reference's
+// re-summed energies only where x lands within rounding of 1 or of U.
 #include "common.h"
 
 #define MC_THREADS 256
@@ -72,106 +73,232 @@ __device__ __forceinline__ float dev_uniform(uint64_t seed, uint64_t sweep, uint
   return ((float)(c[0] >> 8) + 0.5f) * (1.0f / 16777216.0f);
 }
 
-struct McShared {
-  double dsum[MC_THREADS / 64];
-  int imin[MC_THREADS / 64];
-  int target, kind, skip, accept;
-  double x;
+// ---------------------------------------------------------------------------
+// Reference-mode sweep in blocked rounds.  A charge's bead at its turn is its
+// bead at the start of the sweep (each charge moves at most once, at its own
+// turn), and the beads do not move during a sweep, so j* and the energy change
+// toward each of the three possible targets (c+1, c-1, j*) against the
+// start-of-sweep charge positions are computed for all charges at once
+// (k_mc_prep).  The charges are then decided in blocks of MC_BLOCK: one
+// wavefront walks a block in order (k_mc_round), adding the corrections for
+// moves made earlier in the same block lane-parallel, and a wide pass folds the
+// block's accepted moves into the energy changes of every later charge
+// (k_mc_fold).  For charge k moving c -> t after charge b moved cb -> tb:
+//   dE_k += q_k q_b [phi(t,tb) - phi(t,cb) - phi(c,tb) + phi(c,cb)],
+// with phi(a,a) = 0: a target that coincides with a charge's bead is blocked
+// at k's turn whenever that charge is still there, and otherwise the term
+// cancels against the one left out before.
+// The acceptance exp(-b - (E_final - E_initial)/2) is taken with
+// E_final - E_initial = D + dE_k, D the sum of the accepted changes so far.
+#define MC_BLOCK 64
+
+struct McRepState {
+  int cursor;       // uniforms consumed (host-fed stream)
+  unsigned draws;   // device draws used
+  int stop;         // stream ran out: later rounds do nothing
+  int pad;
+  double D;         // E_current - E_initial
 };
 
-__global__ __launch_bounds__(MC_THREADS) void k_mc_sweep(const RepDev* __restrict__ reps, int rep_begin,
-                                                         const double* __restrict__ rx, const double* __restrict__ ry,
-                                                         const double* __restrict__ rz, int* track, int* old,
-                                                         const int* sign, unsigned char* occ, long long occ_stride,
-                                                         const float* uniforms, int nu, const uint64_t* seeds,
-                                                         uint64_t sweep_id, int* consumed, int* log,
-                                                         const int* log_off, int* err, double* qdisp) {
-  int r = rep_begin + blockIdx.x;
+__device__ __forceinline__ double pair_phi(const RepDev& R, const double* rx, const double* ry, const double* rz, int a,
+                                           int b, int qa, int qb) {
+  if (a == b) return 0.0;
+  return pair_energy(R, rx, ry, rz, R.slot0 + a, R.slot0 + b, qa, qb);
+}
+
+// occupancy and start beads; state reset (one workgroup per replica)
+__global__ __launch_bounds__(MC_THREADS) void k_mc_begin(const RepDev* __restrict__ reps, int rep_begin,
+                                                         const int* track, int* start, unsigned char* occ,
+                                                         long long occ_stride, McRepState* st, int* consumed) {
+  const int r = rep_begin + blockIdx.x;
   const RepDev& R = reps[r];
   if (!R.active) return;
-  __shared__ McShared S;
-  const int N = R.n, Nb = R.nb, nq = R.nq;
-  unsigned char* oc = occ + (size_t)blockIdx.x * occ_stride;
-  for (int i = threadIdx.x; i < N; i += blockDim.x) oc[i] = 0;
+  unsigned char* oc = occ + (size_t)r * occ_stride;
+  for (int i = threadIdx.x; i < R.n; i += blockDim.x) oc[i] = 0;
   __syncthreads();
-  for (int k = threadIdx.x; k < nq; k += blockDim.x) oc[track[R.q0 + k]] = 1;
-  __syncthreads();
-  const double E_initial = charge_energy(R, rx, ry, rz, track, sign, S.dsum);
-  double E_cur = E_initial;
-  int cursor = 0;
-  uint32_t draws = 0;
-  const float* ur = uniforms ? uniforms + (size_t)r * nu : nullptr;
-  for (int k = 0; k < nq; ++k) {
+  for (int k = threadIdx.x; k < R.nq; k += blockDim.x) {
     const int c = track[R.q0 + k];
+    oc[c] = 1;
+    start[R.q0 + k] = c;
+  }
+  if (threadIdx.x == 0) {
+    st[r].cursor = 0;
+    st[r].draws = 0;
+    st[r].stop = 0;
+    st[r].D = 0.0;
+    if (consumed) consumed[r] = 0;
+  }
+}
+
+// j* and the three candidate energy changes of each charge: one wavefront per
+// charge; grid (ceil(max nq / 4), nrep)
+__global__ __launch_bounds__(MC_THREADS) void k_mc_prep(const RepDev* __restrict__ reps, int rep_begin,
+                                                        const double* __restrict__ rx, const double* __restrict__ ry,
+                                                        const double* __restrict__ rz, const int* __restrict__ start,
+                                                        const int* __restrict__ sign, int* jfirst, double* de0) {
+  const int r = rep_begin + blockIdx.y;
+  const RepDev& R = reps[r];
+  const int lane = threadIdx.x & 63;
+  const int k = blockIdx.x * (MC_THREADS / 64) + (threadIdx.x >> 6);
+  if (!R.active || k >= R.nq) return;
+  const int N = R.n, nq = R.nq;
+  const int c = start[R.q0 + k], qk = sign[R.q0 + k];
+  const int cs = R.slot0 + c;
+  const double xc = rx[cs], yc = ry[cs], zc = rz[cs];
+  // first j (ascending) not in {c-1,c,c+1} with |getNID(c,j)|^2 < 4.41
+  int jf = -1;
+  for (int base = 0; base < N; base += 64) {
+    const int j = base + lane;
+    bool hit = false;
+    if (j < N && j != c && j != c + 1 && j != c - 1) {
+      double d[3];
+      const int js = R.slot0 + j;
+      ref_nid(R, xc, yc, zc, rx[js], ry[js], rz[js], d[0], d[1], d[2]);
+      double rr;
+      {
+#pragma clang fp contract(off)
+        rr = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      }
+      hit = rr < 4.41;
+    }
+    const unsigned long long m = __ballot(hit);
+    if (m) {
+      jf = base + __ffsll((long long)m) - 1;
+      break;
+    }
+  }
+  const int t0 = (c + 1 < N) ? c + 1 : -1, t1 = c - 1, t2 = jf;
+  double e0 = 0.0, e1 = 0.0, e2 = 0.0;
+  for (int b = lane; b < nq; b += 64) {
+    if (b == k) continue;
+    const int pb = start[R.q0 + b], qb = sign[R.q0 + b];
+    const double ec = pair_phi(R, rx, ry, rz, c, pb, qk, qb);
+    if (t0 >= 0) e0 += pair_phi(R, rx, ry, rz, t0, pb, qk, qb) - ec;
+    if (t1 >= 0) e1 += pair_phi(R, rx, ry, rz, t1, pb, qk, qb) - ec;
+    if (t2 >= 0) e2 += pair_phi(R, rx, ry, rz, t2, pb, qk, qb) - ec;
+  }
+  e0 = warp_sum(e0);
+  e1 = warp_sum(e1);
+  e2 = warp_sum(e2);
+  if (lane == 0) {
+    jfirst[R.q0 + k] = jf;
+    double* d = de0 + 3 * (size_t)(R.q0 + k);
+    d[0] = e0;
+    d[1] = e1;
+    d[2] = e2;
+  }
+}
+
+// The charges k0 .. k0+MC_BLOCK-1 of each replica in order, one wavefront per
+// replica.  Lane j holds charge k0+j: its start bead, sign, j*, candidate
+// energy changes and the occupancy of its candidates at the start of the
+// round, then its outcome.  Lane 0 reads the uniforms and writes the results.
+__global__ __launch_bounds__(64) void k_mc_round(const RepDev* __restrict__ reps, int rep_begin,
+                                                 const double* __restrict__ rx, const double* __restrict__ ry,
+                                                 const double* __restrict__ rz, int* track, int* old,
+                                                 const int* __restrict__ sign, unsigned char* occ,
+                                                 long long occ_stride, const int* __restrict__ start,
+                                                 const int* __restrict__ jfirst, const double* __restrict__ de0,
+                                                 McRepState* st, const float* uniforms, int nu,
+                                                 const uint64_t* seeds, uint64_t sweep_id, int* consumed, int* log,
+                                                 const int* log_off, int* err, double* qdisp, int k0) {
+  const int r = rep_begin + blockIdx.x;
+  const RepDev& R = reps[r];
+  if (!R.active || k0 >= R.nq || st[r].stop) return;
+  const int lane = threadIdx.x;
+  const int N = R.n, Nb = R.nb;
+  const int kend = min(k0 + MC_BLOCK, R.nq);
+  unsigned char* oc = occ + (size_t)r * occ_stride;
+  // this lane's charge
+  const int kl = k0 + lane;
+  const bool mine = kl < kend;
+  int cl = 0, ql = 0, jl = -1, o0 = 1, o1 = 1, o2 = 1;
+  double dl0 = 0.0, dl1 = 0.0, dl2 = 0.0;
+  if (mine) {
+    cl = start[R.q0 + kl];
+    ql = sign[R.q0 + kl];
+    jl = jfirst[R.q0 + kl];
+    const double* d = de0 + 3 * (size_t)(R.q0 + kl);
+    dl0 = d[0];
+    dl1 = d[1];
+    dl2 = d[2];
+    o0 = (cl + 1 < N) ? oc[cl + 1] : 1;
+    o1 = (cl >= 1) ? oc[cl - 1] : 1;
+    o2 = (jl >= 0) ? oc[jl] : 1;
+  }
+  int cursor = st[r].cursor;
+  unsigned draws = st[r].draws;
+  double D = st[r].D;
+  const float* ur = uniforms ? uniforms + (size_t)r * nu : nullptr;
+  // the round's uniforms (at most two per charge)
+  float uw0 = 2.0f, uw1 = 2.0f;
+  const int cur0 = cursor;
+  if (ur) {
+    if (cur0 + lane < nu) uw0 = ur[cur0 + lane];
+    if (cur0 + 64 + lane < nu) uw1 = ur[cur0 + 64 + lane];
+  }
+  bool moved = false;
+  int tb = -1;
+  bool stop = false;
+  for (int k = k0; k < kend; ++k) {
+    const int i = k - k0;
+    const int c = __shfl(cl, i, 64), qk = __shfl(ql, i, 64), jf = __shfl(jl, i, 64);
     // draw mm
     float mmf;
     if (ur) {
       if (cursor >= nu) {
-        if (threadIdx.x == 0) atomicOr(err, 2);
+        stop = true;
         break;
       }
-      mmf = ur[cursor++];
+      const int w = cursor - cur0;
+      mmf = (w < 64) ? __shfl(uw0, w, 64) : __shfl(uw1, w - 64, 64);
+      ++cursor;
     } else {
       mmf = dev_uniform(seeds[r], sweep_id, draws++);
     }
     const double mm = (double)mmf;
-    // first j (ascending) not in {c-1,c,c+1} with |getNID(c,j)|^2 < 4.41
-    int jloc = INT_MAX;
-    {
-      const int cs = R.slot0 + c;
-      const double xc = rx[cs], yc = ry[cs], zc = rz[cs];
-      for (int j = threadIdx.x; j < N && jloc == INT_MAX; j += blockDim.x) {
-        if (j == c || j == c + 1 || j == c - 1) continue;
-        double d[3];
-        int js = R.slot0 + j;
-        ref_nid(R, xc, yc, zc, rx[js], ry[js], rz[js], d[0], d[1], d[2]);
-        double rr;
-        {
-#pragma clang fp contract(off)
-          rr = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-        }
-        if (rr < 4.41) jloc = j;
-      }
-    }
-    int jfirst = block_min_int(jloc, S.imin);
-    if (threadIdx.x == 0) {
-      int target, kind;
-      bool right_block = (c == N - 1) || (c % Nb == Nb - 1) || (c == Nb - 1) || (c + 1 < N && oc[c + 1]);
-      bool left_block = (c - 1 < 0) || (c == 0) || (c % Nb == 0) || (c - 1 >= 0 && oc[c - 1]);
-      bool blocked;
-      if (jfirst != INT_MAX) {
-        if (mm < .33) {
-          target = c + 1;
-          kind = 1;
-          blocked = right_block;
-        } else if (mm > 0.66) {
-          target = c - 1;
-          kind = 1;
-          blocked = left_block;
-        } else {
-          target = jfirst;
-          kind = 2;
-          blocked = oc[jfirst] != 0;
-        }
+    int target, kind, a;
+    if (jf >= 0) {
+      if (mm < .33) {
+        target = c + 1;
+        kind = 1;
+        a = 0;
+      } else if (mm > 0.66) {
+        target = c - 1;
+        kind = 1;
+        a = 1;
       } else {
-        kind = 0;
-        if (mm > .5) {
-          target = c + 1;
-          blocked = right_block;
-        } else {
-          target = c - 1;
-          blocked = left_block;
-        }
+        target = jf;
+        kind = 2;
+        a = 2;
       }
-      S.target = target;
-      S.kind = kind;
-      S.skip = blocked ? 1 : 0;
+    } else {
+      kind = 0;
+      if (mm > .5) {
+        target = c + 1;
+        a = 0;
+      } else {
+        target = c - 1;
+        a = 1;
+      }
     }
-    __syncthreads();
-    const int target = S.target, kind = S.kind;
+    // occupancy of the target now: at the start of the round, less a charge
+    // of this block that left it, plus one that moved onto it
+    const int oa = (a == 0) ? o0 : (a == 1) ? o1 : o2;
+    const bool occ0 = __shfl(oa, i, 64) != 0;
+    const bool freed = __ballot(moved && cl == target) != 0ull;
+    const bool filled = __ballot(moved && tb == target) != 0ull;
+    const bool occupied = (occ0 && !freed) || filled;
+    bool blocked;
+    if (a == 0)
+      blocked = (c == N - 1) || (c % Nb == Nb - 1) || (c == Nb - 1) || occupied;
+    else if (a == 1)
+      blocked = (c - 1 < 0) || (c == 0) || (c % Nb == 0) || occupied;
+    else
+      blocked = occupied;
     int* lg = log ? log + 6 * (size_t)(log_off[r] + k) : nullptr;
-    if (S.skip) {
-      if (threadIdx.x == 0 && lg) {
+    if (blocked) {
+      if (lane == 0 && lg) {
         lg[0] = k;
         lg[1] = c;
         lg[2] = -1;
@@ -179,47 +306,48 @@ __global__ __launch_bounds__(MC_THREADS) void k_mc_sweep(const RepDev* __restric
         lg[4] = 0;
         lg[5] = 1;
       }
-      __syncthreads();
       continue;
     }
-    // energy change of moving charge k from c to target
-    double de = 0.0;
-    {
-      const int qk = sign[R.q0 + k];
-      const int ts = R.slot0 + target, cs = R.slot0 + c;
-      for (int b = threadIdx.x; b < nq; b += blockDim.x) {
-        if (b == k) continue;
-        int bs = R.slot0 + track[R.q0 + b];
-        int qb = sign[R.q0 + b];
-        de += pair_energy(R, rx, ry, rz, ts, bs, qk, qb) - pair_energy(R, rx, ry, rz, cs, bs, qk, qb);
-      }
+    // energy change: the precomputed value plus this block's earlier moves
+    double corr = 0.0;
+    if (moved) {
+      const int qb = ql;
+      corr = pair_phi(R, rx, ry, rz, target, tb, qk, qb) - pair_phi(R, rx, ry, rz, target, cl, qk, qb) -
+             pair_phi(R, rx, ry, rz, c, tb, qk, qb) + pair_phi(R, rx, ry, rz, c, cl, qk, qb);
     }
-    de = block_sum(de, S.dsum);
-    if (threadIdx.x == 0) {
-      double E_final = (nq > 1) ? E_cur + de : 0.0;
-      double b = (kind == 2) ? R.barrier2 : R.barrier;
-      double x = exp((-b - 0.5 * (E_final - (nq > 1 ? E_initial : 0.0))));
-      int ndraw = 1, acc = 0;
-      if (1 <= x) {
-        acc = 1;
-      } else {
-        float dice;
-        bool ok = true;
-        if (ur) {
-          if (cursor >= nu) {
-            atomicOr(err, 2);
-            ok = false;
-            dice = 2.0f;
-          } else {
-            dice = ur[cursor++];
-          }
+    corr = warp_sum(corr);
+    const double dsel = (a == 0) ? dl0 : (a == 1) ? dl1 : dl2;
+    const double dE = __shfl(dsel, i, 64) + corr;
+    const double bar = (kind == 2) ? R.barrier2 : R.barrier;
+    const double x = exp((-bar - 0.5 * (D + dE)));
+    int ndraw = 1, acc = 0;
+    if (1 <= x) {
+      acc = 1;
+    } else {
+      float dice;
+      bool ok = true;
+      if (ur) {
+        if (cursor >= nu) {
+          ok = false;
+          dice = 2.0f;
         } else {
-          dice = dev_uniform(seeds[r], sweep_id, draws++);
+          const int w = cursor - cur0;
+          dice = (w < 64) ? __shfl(uw0, w, 64) : __shfl(uw1, w - 64, 64);
+          ++cursor;
         }
-        ndraw = 2;
-        if (ok && (double)dice <= x) acc = 1;
+      } else {
+        dice = dev_uniform(seeds[r], sweep_id, draws++);
       }
-      S.accept = acc;
+      ndraw = 2;
+      if (!ok && lane == 0) atomicOr(err, 2);
+      if (ok && (double)dice <= x) acc = 1;
+    }
+    if (acc) D += dE;
+    if (lane == i) {
+      moved = acc != 0;
+      tb = target;
+    }
+    if (lane == 0) {
       if (acc) {
         track[R.q0 + k] = target;
         old[R.q0 + k] = c;
@@ -242,31 +370,84 @@ __global__ __launch_bounds__(MC_THREADS) void k_mc_sweep(const RepDev* __restric
         lg[5] = ndraw;
       }
     }
-    __syncthreads();
-    if (S.accept) E_cur += de;
-    // (consumption of the dice draw is mirrored by thread 0's cursor)
-    if (ur) {
-      // broadcast cursor from thread 0
-      if (threadIdx.x == 0) S.kind = cursor;
-      __syncthreads();
-      cursor = S.kind;
-    } else {
-      if (threadIdx.x == 0) S.kind = (int)draws;
-      __syncthreads();
-      draws = (uint32_t)S.kind;
-    }
-    __syncthreads();
   }
-  if (threadIdx.x == 0 && consumed) consumed[r] = ur ? cursor : (int)draws;
+  if (lane == 0) {
+    if (stop) {
+      atomicOr(err, 2);
+      st[r].stop = 1;
+    }
+    st[r].cursor = cursor;
+    st[r].draws = draws;
+    st[r].D = D;
+    if (consumed) consumed[r] = ur ? cursor : (int)draws;
+  }
 }
 
-void launch_mc_sweep(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, const double* rx, const double* ry,
-                     const double* rz, int* track, int* old, const int* sign, unsigned char* occ,
-                     long long occ_stride, const float* uniforms, int nu, const uint64_t* seeds, uint64_t sweep_id,
-                     int* consumed, int* log, const int* log_off, int* err, double* qdisp) {
+// The moves of block k0 .. k0+MC_BLOCK-1 folded into the candidate energy
+// changes of every later charge: one wavefront per later charge, lane j for
+// charge k0+j of the block; grid (ceil(max nq / 4), nrep)
+__global__ __launch_bounds__(MC_THREADS) void k_mc_fold(const RepDev* __restrict__ reps, int rep_begin,
+                                                        const double* __restrict__ rx, const double* __restrict__ ry,
+                                                        const double* __restrict__ rz, const int* __restrict__ track,
+                                                        const int* __restrict__ start, const int* __restrict__ sign,
+                                                        const int* __restrict__ jfirst, const McRepState* st,
+                                                        double* de0, int k0) {
+  const int r = rep_begin + blockIdx.y;
+  const RepDev& R = reps[r];
+  const int lane = threadIdx.x & 63;
+  const int k = blockIdx.x * (MC_THREADS / 64) + (threadIdx.x >> 6);
+  const int kend = min(k0 + MC_BLOCK, R.nq);
+  if (!R.active || k < kend || k >= R.nq || st[r].stop) return;
+  const int b = k0 + lane;
+  int cb = -1, tb = -1, qb = 0;
+  bool moved = false;
+  if (b < kend) {
+    cb = start[R.q0 + b];
+    tb = track[R.q0 + b];
+    qb = sign[R.q0 + b];
+    moved = tb != cb;
+  }
+  if (__ballot(moved) == 0ull) return;
+  const int N = R.n;
+  const int c = start[R.q0 + k], qk = sign[R.q0 + k], jf = jfirst[R.q0 + k];
+  const int t0 = (c + 1 < N) ? c + 1 : -1, t1 = c - 1, t2 = jf;
+  double e0 = 0.0, e1 = 0.0, e2 = 0.0;
+  if (moved) {
+    const double ec = pair_phi(R, rx, ry, rz, c, tb, qk, qb) - pair_phi(R, rx, ry, rz, c, cb, qk, qb);
+    if (t0 >= 0) e0 = pair_phi(R, rx, ry, rz, t0, tb, qk, qb) - pair_phi(R, rx, ry, rz, t0, cb, qk, qb) - ec;
+    if (t1 >= 0) e1 = pair_phi(R, rx, ry, rz, t1, tb, qk, qb) - pair_phi(R, rx, ry, rz, t1, cb, qk, qb) - ec;
+    if (t2 >= 0) e2 = pair_phi(R, rx, ry, rz, t2, tb, qk, qb) - pair_phi(R, rx, ry, rz, t2, cb, qk, qb) - ec;
+  }
+  e0 = warp_sum(e0);
+  e1 = warp_sum(e1);
+  e2 = warp_sum(e2);
+  if (lane == 0) {
+    double* d = de0 + 3 * (size_t)(R.q0 + k);
+    d[0] += e0;
+    d[1] += e1;
+    d[2] += e2;
+  }
+}
+
+size_t mc_state_bytes() { return sizeof(McRepState); }
+
+void launch_mc_sweep(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, int max_nq, const double* rx,
+                     const double* ry, const double* rz, int* track, int* old, const int* sign, unsigned char* occ,
+                     long long occ_stride, int* start, int* jfirst, double* de0, void* state, const float* uniforms,
+                     int nu, const uint64_t* seeds, uint64_t sweep_id, int* consumed, int* log, const int* log_off,
+                     int* err, double* qdisp) {
   if (nrep <= 0) return;
-  k_mc_sweep<<<nrep, MC_THREADS, 0, st>>>(reps, rep_begin, rx, ry, rz, track, old, sign, occ, occ_stride, uniforms, nu,
-                                          seeds, sweep_id, consumed, log, log_off, err, qdisp);
+  McRepState* S = (McRepState*)state;
+  k_mc_begin<<<nrep, MC_THREADS, 0, st>>>(reps, rep_begin, track, start, occ, occ_stride, S, consumed);
+  if (max_nq <= 0) return;
+  const dim3 wide((max_nq + MC_THREADS / 64 - 1) / (MC_THREADS / 64), nrep);
+  k_mc_prep<<<wide, MC_THREADS, 0, st>>>(reps, rep_begin, rx, ry, rz, start, sign, jfirst, de0);
+  for (int k0 = 0; k0 < max_nq; k0 += MC_BLOCK) {
+    k_mc_round<<<nrep, 64, 0, st>>>(reps, rep_begin, rx, ry, rz, track, old, sign, occ, occ_stride, start, jfirst, de0,
+                                    S, uniforms, nu, seeds, sweep_id, consumed, log, log_off, err, qdisp, k0);
+    if (k0 + MC_BLOCK < max_nq)
+      k_mc_fold<<<wide, MC_THREADS, 0, st>>>(reps, rep_begin, rx, ry, rz, track, start, sign, jfirst, S, de0, k0);
+  }
 }
 
 __global__ __launch_bounds__(MC_THREADS) void k_mc_energy(const RepDev* __restrict__ reps, int rep_begin,
