@@ -293,6 +293,11 @@ int smdlt_mlp_loss_grad(smdlt_ctx* ctx, const float* weights, const float* featu
  * returns SMDLT_E_ARG if it exceeds cap.  Any output may be NULL. */
 int smdlt_mc_features(smdlt_ctx* ctx, int replica, float* features, int* from, int* to, int* accepted, int cap,
                       int* n);
+/* Whether SMDLT_MC_MH_* sweeps record the features of their proposed moves for
+ * smdlt_mc_features (on by default; no reference counterpart).  Off saves that
+ * work in uniform-proposal sweeps that no training reads; after such a sweep
+ * smdlt_mc_features returns SMDLT_E_STATE when asked for features. */
+int smdlt_mc_record_features(smdlt_ctx* ctx, int on);
 
 /* Per-stage device times of the last step, milliseconds (HIP events on the
  * context stream when profiling is enabled): see smdlt_stage_name(). */

@@ -75,7 +75,7 @@ def lib():
         L.orc_bf16_round.restype = C.c_float
         L.orc_bf16_round.argtypes = [C.c_float]
         L.orc_mh_features.argtypes = [C.POINTER(Lattice), C.POINTER(MHParams), _dp, _dp, _dp, C.c_void_p, C.c_int,
-                                      C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _dp]
+                                      C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _dp]
         L.orc_mlp_forward.restype = C.c_double
         L.orc_mlp_forward.argtypes = [C.c_void_p, _dp]
         L.orc_mh_sweep.argtypes = [C.POINTER(Lattice), C.POINTER(MHParams), _dp, _dp, _dp, C.c_int, C.c_void_p,
@@ -487,11 +487,12 @@ def bf16_round(w):
     return np.array([lib().orc_bf16_round(float(x)) for x in w], np.float32)
 
 
-def mh_features(lat, p, rx, ry, rz, qbead, src, dst, qk, clr=-1, set_=-1, ncand=0):
+def mh_features(lat, p, rx, ry, rz, qbead, src, dst, qk, clr=-1, set_=-1, ncand=0, anchor=None):
+    """anchor: the bead the pair distances are taken relative to (default src)."""
     f = np.zeros(16)
     qb = np.ascontiguousarray(qbead, np.int32)
     lib().orc_mh_features(C.byref(lat), C.byref(p), rx, ry, rz, qb.ctypes.data_as(C.c_void_p), src, dst, qk, clr,
-                          set_, ncand, f)
+                          set_, ncand, src if anchor is None else anchor, f)
     return f
 
 

@@ -1956,9 +1956,15 @@ static double mh_cos_at(const orc_lattice_t* lat, const double* rx, const double
   return (u[0] * v[0] + u[1] * v[1] + u[2] * v[2]) / (uu * vv);
 }
 
+/* distance between two beads from their vectors relative to a common anchor */
+static double mh_rel_dist(const double a[3], const double b[3]) {
+  double dx = b[0] - a[0], dy = b[1] - a[1], dz = b[2] - a[2];
+  return sqrt(dx * dx + dy * dy + dz * dz);
+}
+
 void orc_mh_features(const orc_lattice_t* lat, const orc_mh_params_t* p, const double* rx, const double* ry,
                      const double* rz, const int* qbead, int src, int dst, int qk, int clr, int set, int ncand,
-                     double f[16]) {
+                     int anchor, double f[16]) {
   const int nb = p->nb;
   const double R4 = p->feature_radius, kap = 1.0 / p->lambda_d;
   for (int k = 0; k < 16; ++k) f[k] = 0.0;
@@ -1967,9 +1973,14 @@ void orc_mh_features(const orc_lattice_t* lat, const orc_mh_params_t* p, const d
   int gap = abs(dst - src);
   f[1] = (same && gap == 1) ? 1.0 : 0.0;
   f[2] = same ? (double)(gap < 20 ? gap : 20) / 20.0 : 1.0;
+  double vs[3], vd[3];
+  orc_nid(lat, rx, ry, rz, src, anchor, vs);
+  orc_nid(lat, rx, ry, rz, dst, anchor, vd);
   for (int m = 0; m < p->n; ++m) {
     if (m == src || m == dst) continue;
-    double rdm = mh_dist(lat, rx, ry, rz, dst, m), rsm = mh_dist(lat, rx, ry, rz, src, m);
+    double vm[3];
+    orc_nid(lat, rx, ry, rz, m, anchor, vm);
+    double rdm = mh_rel_dist(vd, vm), rsm = mh_rel_dist(vs, vm);
     int qm = (m == clr) ? 0 : (m == set) ? qk : qbead[m];
     if (qm != 0 && (rdm < R4 || rsm < R4))
       f[3] += p->lambda_b * (double)(qk * qm) * (exp(-kap * rdm) / rdm - exp(-kap * rsm) / rsm);
@@ -2033,7 +2044,7 @@ static int mh_candidates(const orc_lattice_t* lat, const orc_mh_params_t* p, con
 /* proposal probabilities over the candidates (FP64) */
 static void mh_proposal(const orc_lattice_t* lat, const orc_mh_params_t* p, const double* rx, const double* ry,
                         const double* rz, const int* qbead, const float* w, int src, int qk, int clr, int set,
-                        const int* cand, int nc, double* q) {
+                        const int* cand, int nc, int anchor, double* q) {
   if (!w) {
     for (int c = 0; c < nc; ++c) q[c] = 1.0 / nc;
     return;
@@ -2041,7 +2052,7 @@ static void mh_proposal(const orc_lattice_t* lat, const orc_mh_params_t* p, cons
   double z = 0.0;
   for (int c = 0; c < nc; ++c) {
     double f[16];
-    orc_mh_features(lat, p, rx, ry, rz, qbead, src, cand[c], qk, clr, set, nc, f);
+    orc_mh_features(lat, p, rx, ry, rz, qbead, src, cand[c], qk, clr, set, nc, anchor, f);
     q[c] = 1.0 / (1.0 + exp(-orc_mlp_forward(w, f)));  /* predicted acceptance: Barker-type weight */
     z += q[c];
   }
@@ -2077,7 +2088,7 @@ int orc_mh_sweep(const orc_lattice_t* lat, const orc_mh_params_t* p, const doubl
       }
       continue;
     }
-    mh_proposal(lat, p, rx, ry, rz, qbead, wr, x, qk, -1, -1, cand, nc, q);
+    mh_proposal(lat, p, rx, ry, rz, qbead, wr, x, qk, -1, -1, cand, nc, x, q);
     double tot = 0.0;
     for (int c = 0; c < nc; ++c) tot += q[c];
     double t = (double)u1 * tot, cum = 0.0;
@@ -2093,7 +2104,7 @@ int orc_mh_sweep(const orc_lattice_t* lat, const orc_mh_params_t* p, const doubl
     double qf = q[pick];
     if (fa) {
       double f[16];
-      orc_mh_features(lat, p, rx, ry, rz, qbead, x, y, qk, -1, -1, nc, f);
+      orc_mh_features(lat, p, rx, ry, rz, qbead, x, y, qk, -1, -1, nc, x, f);
       for (int i = 0; i < 16; ++i) fa[i] = (float)f[i];
     }
     double de = 0.0;
@@ -2105,7 +2116,7 @@ int orc_mh_sweep(const orc_lattice_t* lat, const orc_mh_params_t* p, const doubl
     }
     /* reverse proposal in the state after the move */
     int nc2 = mh_candidates(lat, p, rx, ry, rz, qbead, y, x, y, cand);
-    mh_proposal(lat, p, rx, ry, rz, qbead, wr, y, qk, x, y, cand, nc2, q);
+    mh_proposal(lat, p, rx, ry, rz, qbead, wr, y, qk, x, y, cand, nc2, x, q);
     double qr = 0.0;
     for (int c = 0; c < nc2; ++c)
       if (cand[c] == x) qr = q[c];

@@ -171,10 +171,12 @@ float orc_bf16_round(float x);
 /* The 16 proposal features of moving the charge at bead src to bead dst.  qbead[n]:
  * charge sign per bead (0 = none) with the moving charge at src; clr / set (or -1):
  * a bead whose charge is taken as 0 / a bead taken to carry the moving charge
- * (the state after a move, for the reverse proposal).  ncand: candidates of src. */
+ * (the state after a move, for the reverse proposal).  ncand: candidates of src.
+ * The distances of src and dst to the other beads are the norms of differences
+ * of their getNID vectors relative to bead anchor (the attempt's start bead). */
 void orc_mh_features(const orc_lattice_t* lat, const orc_mh_params_t* p, const double* rx, const double* ry,
                      const double* rz, const int* qbead, int src, int dst, int qk, int clr, int set, int ncand,
-                     double f[16]);
+                     int anchor, double f[16]);
 /* MLP 16-64-64-1 with ReLU, FP64 arithmetic on the given (already rounded) weights
  * laid out W1[64][16], b1[64], W2[64][64], b2[64], W3[64], b3. */
 double orc_mlp_forward(const float* w, const double f[16]);

@@ -292,6 +292,14 @@ struct smdlt_ctx {
   int *mc_start = nullptr, *mc_jf = nullptr;  // reference sweep: start beads, j* per charge
   double* mc_de = nullptr;                     // reference sweep: candidate energy changes [nq_total][3]
   void* mc_state = nullptr;                    // reference sweep: per-replica cursor, draws, E - E_initial
+  int *mh_cnt = nullptr, *mh_off = nullptr;    // MH sweeps: neighbourhood list counts and offsets [nst + 1]
+  int* mh_list = nullptr;                      // MH sweeps: neighbourhood lists
+  long long mh_list_cap = 0;
+  int *mh_hcnt = nullptr, *mh_hoff = nullptr, *mh_hlist = nullptr;  // MH sweeps: hop lists
+  long long mh_hlist_cap = 0;
+  double* mh_V = nullptr;                      // MH sweeps: charge potential per bead [nst]
+  void* mh_moves = nullptr;                    // MH sweeps: a round's accepted moves [nrep][64] int4
+  int *mh_nmoves = nullptr, *mh_done = nullptr, *mh_stop = nullptr;  // [nrep]
   long long occ_stride = 0;
   int* log_off = nullptr;
   float4* p32 = nullptr;      // mixed precision: FP32 positions and scaled input vector
@@ -304,7 +312,9 @@ struct smdlt_ctx {
   float* mlp_w = nullptr;     // BF16-rounded proposal network weights (FP32 storage)
   float* mh_feats = nullptr;  // features of each MH attempt of the last sweep [nq_total][16]
   int* mh_log = nullptr;      // its log [nq_total][6]
-  bool mh_swept = false;      // an MH sweep has filled mh_feats / mh_log
+  bool mh_swept = false;      // an MH sweep has filled mh_log
+  bool mh_record = true;      // MH sweeps record the features of each proposed move
+  bool mh_feats_valid = false;  // mh_feats holds the last MH sweep's features
   float* mlp_ws = nullptr;    // training: weights, batch, per-block partial gradients
   float* mlp_x = nullptr;
   float* mlp_y = nullptr;
@@ -1146,7 +1156,9 @@ void smdlt_destroy(smdlt_ctx* c) {
                   c->partial2, c->grid, c->bsp, c->tw, c->tx, c->ty, c->tz, c->txy, c->bricks, c->bbase, c->brick_cnt, c->brick_start,
                   c->brick_fill, c->brick_beads, c->bw, c->V, c->W, c->partial, c->scal, c->bdiv, c->done,
                   c->iters, c->qtrack, c->qsign, c->qold, c->qidx, c->mc_err, c->mlp_w, c->occ, c->log_off,
-                  c->mc_start, c->mc_jf, c->mc_de, c->mc_state};
+                  c->mc_start, c->mc_jf, c->mc_de, c->mc_state, c->mh_cnt, c->mh_off, c->mh_list, c->mh_V,
+                  c->mh_hcnt, c->mh_hoff, c->mh_hlist,
+                  c->mh_moves, c->mh_nmoves, c->mh_done, c->mh_stop};
   for (void* p : ptrs)
     if (p) hipFree(p);
   for (auto& t : c->timers) {
@@ -2489,6 +2501,7 @@ int smdlt_upload_charges(smdlt_ctx* c, const int* nq, const int* track, const in
   c->occ = nullptr;
   c->mh_feats = nullptr;
   c->mh_swept = false;
+  c->mh_feats_valid = false;
   int st = 0;
   st |= alloc_dev(&c->qtrack, tot);
   st |= alloc_dev(&c->mh_feats, 16 * (size_t)std::max(1, tot));  // MH sweeps' attempt records
@@ -2537,23 +2550,81 @@ int smdlt_download_charges(smdlt_ctx* c, int* track, int* old) {
 // du (the replicas' host-fed uniforms, or null), dcons (nrep ints) and dlog
 // (reference mode: 6 ints per charge, or null).  The MH modes log into the
 // context's own buffers.
+// The neighbourhood lists of the MH sweeps for the active replicas at the
+// current positions: counts, offsets by an exclusive scan, then the lists.
+static int mh_build_lists(smdlt_ctx* c, int rb, int re, int max_n) {
+  const long long ns1 = (long long)c->nst + 1;
+  if (!c->mh_cnt) {
+    int st = 0;
+    st |= alloc_dev(&c->mh_cnt, ns1);
+    st |= alloc_dev(&c->mh_off, ns1);
+    st |= alloc_dev(&c->mh_hcnt, ns1);
+    st |= alloc_dev(&c->mh_hoff, ns1);
+    st |= alloc_dev(&c->mh_V, (size_t)std::max(1, c->nst));
+    st |= alloc_dev(&c->mh_nmoves, c->nrep);
+    st |= alloc_dev(&c->mh_done, c->nrep);
+    st |= alloc_dev(&c->mh_stop, c->nrep);
+    if (!st && hipMalloc(&c->mh_moves, 16 * 64 * (size_t)c->nrep) != hipSuccess) st = 1;
+    if (st) return SMDLT_E_NOMEM;
+  }
+  launch_fill_i32(c->stream, c->mh_cnt, 0, ns1);
+  launch_fill_i32(c->stream, c->mh_hcnt, 0, ns1);
+  launch_mh_lists(c->stream, c->d_reps, rb, re - rb, max_n, c->rx, c->ry, c->rz, c->mh_cnt, nullptr, nullptr,
+                  c->mh_hcnt, nullptr, nullptr, false);
+  launch_scan(c->stream, c->mh_cnt, c->mh_off, (int)ns1);
+  launch_scan(c->stream, c->mh_hcnt, c->mh_hoff, (int)ns1);
+  int total[2] = {0, 0};
+  HCK(hipMemcpyAsync(&total[0], c->mh_off + c->nst, 4, hipMemcpyDeviceToHost, c->stream));
+  HCK(hipMemcpyAsync(&total[1], c->mh_hoff + c->nst, 4, hipMemcpyDeviceToHost, c->stream));
+  HCK(hipStreamSynchronize(c->stream));
+  if (total[0] < 0 || total[1] < 0) return SMDLT_E_GEOMETRY;  // more than 2^31 entries
+  int** lists[2] = {&c->mh_list, &c->mh_hlist};
+  long long* caps[2] = {&c->mh_list_cap, &c->mh_hlist_cap};
+  for (int l = 0; l < 2; ++l) {
+    if (total[l] <= *caps[l]) continue;
+    if (*lists[l]) hipFree(*lists[l]);
+    *lists[l] = nullptr;
+    *caps[l] = 0;
+    const long long cap = (long long)total[l] + total[l] / 4 + 1024;
+    if (alloc_dev(lists[l], cap)) return SMDLT_E_NOMEM;
+    *caps[l] = cap;
+  }
+  launch_mh_lists(c->stream, c->d_reps, rb, re - rb, max_n, c->rx, c->ry, c->rz, nullptr, c->mh_off, c->mh_list,
+                  nullptr, c->mh_hoff, c->mh_hlist, true);
+  return 0;
+}
+
 static int mc_sweep_internal(smdlt_ctx* c, int mode, const float* du, int n_uniforms, int* dcons, int* dlog,
                              int* consumed, int* log) {
   const bool mh = mode != SMDLT_MC_REFERENCE;
   if (mh) dlog = c->mh_log;
   HCK(hipMemsetAsync(c->mc_err, 0, 4, c->stream));
   int rb = c->cur_rep_begin, re = c->cur_rep_end;
-  if (mh)
-    launch_mh_sweep(c->stream, c->d_reps, rb, re - rb, c->rx, c->ry, c->rz, c->qtrack, c->qold, c->qsign,
-                    (signed char*)c->occ, c->occ_stride, du, du ? n_uniforms : 0, c->d_seeds, c->mc_sweeps++,
-                    mode == SMDLT_MC_MH_LEARNED ? c->mlp_w : nullptr, dcons, dlog, c->mh_feats, c->log_off,
-                    c->mc_err, c->qdisp);
-  else
+  if (mh) {
+    int max_n = 0, max_nq = 0;
+    for (int r = rb; r < re; ++r) {
+      if (c->nq[r] > 0) max_n = std::max(max_n, c->rh[r].n);
+      max_nq = std::max(max_nq, c->nq[r]);
+    }
+    if (max_nq > 0) {
+      int st = mh_build_lists(c, rb, re, max_n);
+      if (st) return st;
+    }
+    launch_mh_sweep(c->stream, c->d_reps, rb, re - rb, max_n, max_nq, c->rx, c->ry, c->rz, c->qtrack, c->qold,
+                    c->qsign, (signed char*)c->occ, c->occ_stride, du, du ? n_uniforms : 0, c->d_seeds,
+                    c->mc_sweeps++, mode == SMDLT_MC_MH_LEARNED ? c->mlp_w : nullptr, dcons, dlog,
+                    c->mh_record ? c->mh_feats : nullptr, c->log_off, c->mc_err, c->qdisp, c->mh_off, c->mh_list, c->mh_hoff, c->mh_hlist,
+                    c->mh_V, c->mh_moves, c->mh_nmoves,
+                    c->mh_done, c->mh_stop);
+  } else
     launch_mc_sweep(c->stream, c->d_reps, rb, re - rb, c->max_nq, c->rx, c->ry, c->rz, c->qtrack, c->qold, c->qsign,
                     c->occ, c->occ_stride, c->mc_start, c->mc_jf, c->mc_de, c->mc_state, du, du ? n_uniforms : 0,
                     c->d_seeds, c->mc_sweeps++, dcons, dlog, c->log_off, c->mc_err, c->qdisp);
   HCK(hipGetLastError());
-  if (mh) c->mh_swept = true;
+  if (mh) {
+    c->mh_swept = true;
+    c->mh_feats_valid = c->mh_record;
+  }
   HCK(hipStreamSynchronize(c->stream));
   int e = 0;
   HCK(hipMemcpy(&e, c->mc_err, 4, hipMemcpyDeviceToHost));
@@ -2729,7 +2800,7 @@ int smdlt_mlp_loss_grad(smdlt_ctx* c, const float* w, const float* x, const floa
 int smdlt_mc_features(smdlt_ctx* c, int replica, float* features, int* from, int* to, int* accepted, int cap,
                       int* n) {
   if (!c || replica < 0 || replica >= c->nrep || !n) return SMDLT_E_ARG;
-  if (!c->mh_feats || !c->mh_swept) return SMDLT_E_STATE;
+  if (!c->mh_feats || !c->mh_swept || (features && !c->mh_feats_valid)) return SMDLT_E_STATE;
   HCK(hipSetDevice(c->device));
   const int nq = c->nq[replica], off = c->q0[replica];
   *n = nq;
@@ -2744,6 +2815,12 @@ int smdlt_mc_features(smdlt_ctx* c, int replica, float* features, int* from, int
       if (accepted) accepted[k] = lg[6 * k + 4];
     }
   }
+  return 0;
+}
+
+int smdlt_mc_record_features(smdlt_ctx* c, int on) {
+  if (!c) return SMDLT_E_ARG;
+  c->mh_record = on != 0;
   return 0;
 }
 

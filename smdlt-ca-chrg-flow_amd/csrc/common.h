@@ -427,10 +427,19 @@ void lz_reset_checks(hipStream_t st, double* scal, int rep_begin, int nrep);
 #define MLP_TILE 64
 void launch_mlp_loss_grad(hipStream_t st, const float* w, const float* X, const float* Y, int n, float* part,
                           float* out);
-void launch_mh_sweep(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, const double* rx, const double* ry,
-                     const double* rz, int* track, int* old, const int* sign, signed char* occ, long long occ_stride,
-                     const float* uniforms, int nu, const uint64_t* seeds, uint64_t sweep_id, const float* weights,
-                     int* consumed, int* log, float* feats, const int* log_off, int* err, double* qdisp);
+// MH sweeps: per-bead neighbourhood and hop lists (count, then fill at the
+// scanned offsets), then the sweep in rounds; V (nst doubles), moves (nrep x 64 int4),
+// nmoves/done/stop (nrep ints) are its working arrays
+void launch_mh_lists(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, int max_n, const double* rx,
+                     const double* ry, const double* rz, int* cnt, const int* off, int* list, int* hcnt,
+                     const int* hoff, int* hlist, bool fill);
+void launch_mh_sweep(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, int max_n, int max_nq,
+                     const double* rx, const double* ry, const double* rz, int* track, int* old, const int* sign,
+                     signed char* occ, long long occ_stride, const float* uniforms, int nu, const uint64_t* seeds,
+                     uint64_t sweep_id, const float* weights, int* consumed, int* log, float* feats,
+                     const int* log_off, int* err, double* qdisp, const int* nb_off, const int* nb_list,
+                     const int* hop_off, const int* hop_list, double* V, void* moves, int* nmoves, int* done,
+                     int* stop);
 // reference-mode sweep; start/jfirst (nq_total ints), de0 (3 nq_total doubles)
 // and state (nrep x mc_state_bytes()) are the sweep's working arrays
 size_t mc_state_bytes();

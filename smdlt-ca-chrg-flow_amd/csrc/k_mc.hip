@@ -16,6 +16,8 @@
 // E(proposed) - E_initial is kept incrementally in FP64 (the accepted changes
 // so far plus the proposal's), so a decision can differ from the reference's
 // re-summed energies only where x lands within rounding of 1 or of U.
+#include <type_traits>
+
 #include "common.h"
 
 #define MC_THREADS 256
@@ -478,15 +480,48 @@ void launch_mc_energy(hipStream_t st, const RepDev* reps, int rep_begin, int nre
 // proposals) scoring 16 features of each candidate move (BF16 weights, FP32
 // arithmetic).
 // q(y->x) is evaluated in the state after the move.  The beads within
-// 2 hop + feature radius of x (all any forward or reverse feature can see) are
-// gathered once per attempt in ascending order; features are summed by one
-// wavefront per candidate.  dE is exact FP64 over all charges.
+// 2 hop + feature radius of x (all any forward or reverse feature can see),
+// ascending, come from per-bead lists built once per sweep (the beads do not
+// move during a sweep), each entry flagged when it lies within the hop radius;
+// features are summed by one wavefront per candidate.  dE is exact FP64 over
+// all charges: with V(z) = sum_b q_b phi(z, b) over every charge b (phi(z,z) = 0)
+//   dE = lambda_B q_k [V(y) - V(x)] - lambda_B phi(x, y),
+// V computed for every bead at the start of the sweep and brought up to date
+// after each round of MH_ROUND attempts by a wide pass over the round's
+// accepted moves; within a round the earlier moves' changes of V at x and y
+// are added one term per thread.
 #define MH_MAXL 4096
 #define MH_MAXC 128
 #define MLP_NP (16 * 64 + 64 + 64 * 64 + 64 + 64 + 1)
 
+#define MH_ROUND 64
+#define MH_HOP_PRE 16  // hop-list entries of each attempt's bead fetched with the round
+// replicas up to these sizes keep the occupancy and charge beads in LDS
+#define MH_LDS_N 16384
+#define MH_LDS_Q 4096
+// neighbourhoods up to this size keep the source-side distances of the
+// features in LDS
+#define MH_SRC_L 1536
+struct MhState {
+  signed char qs[MH_LDS_N];
+  int trk[MH_LDS_Q];
+};
+struct MhNoState {
+  int unused;
+};
+
 struct MhShared {
   int loc[MH_MAXL];
+  double vx[MH_SRC_L], vy[MH_SRC_L], vz[MH_SRC_L];  // loc[] relative to the attempt's start bead
+  double rs[MH_SRC_L], es[MH_SRC_L];                // distance to the proposal's source, exp(-kappa r)/r
+  int4 mv[MH_ROUND];
+  float ur[3 * MH_ROUND];
+  int xp[MH_ROUND], op[MH_ROUND], oe[MH_ROUND];  // the round's attempts: bead, list ranges
+  int hp[MH_ROUND], he[MH_ROUND];                 // and V at the start of the round,
+  double vxp[MH_ROUND];                           // the charge's sign and x's hop list
+  int qkp[MH_ROUND];
+  int hl[MH_ROUND][MH_HOP_PRE];
+  int nmv;
   int cand[MH_MAXC];
   double q[MH_MAXC];
   float feat[MH_MAXC][16];
@@ -546,22 +581,67 @@ __device__ __forceinline__ void block_append(bool flag, int value, int* list, in
   __syncthreads();
 }
 
+// Distance between two beads from their vectors relative to a common anchor.
+__device__ __forceinline__ double mh_rel_dist(double ax, double ay, double az, double bx, double by, double bz) {
+#pragma clang fp contract(off)
+  const double dx = bx - ax, dy = by - ay, dz = bz - az;
+  return sqrt(dx * dx + dy * dy + dz * dz);
+}
+
 // 16 features of moving the charge at src (sign qk) to dst, summed by one
 // wavefront over the gathered neighbourhood; clr / set (or -1) give the state
-// after a move.  Lane 0 writes out[16].
-__device__ void mh_features_wave(const RepDev& R, const double* rx, const double* ry, const double* rz,
-                                 const signed char* qb, const int* loc, int nloc, int src, int dst, int qk, int clr,
-                                 int set, int ncand, float* out) {
+// after a move.  The distances of src and dst to the neighbourhood beads are
+// taken from the beads' getNID vectors relative to the anchor (the attempt's
+// start bead): vx/vy/vz hold them for loc[] (null: computed here); rs / es
+// (may be null) hold the distance to src and exp(-kappa r)/r at it.  Lane 0
+// writes out[16].
+__device__ void mh_features_rel(const RepDev& R, const double* rx, const double* ry, const double* rz,
+                                const signed char* qb, const int* loc, int nloc, int src, int dst, int qk, int clr,
+                                int set, int ncand, float* out, int anchor, const double* vx, const double* vy,
+                                const double* vz, const double* rs, const double* es) {
   const int lane = threadIdx.x & 63;
   double e = 0, n4d = 0, n4s = 0, s4d = 0, s4s = 0, b4d = 0, b4s = 0;
-  const double R4 = R.feat_r, kap = R.kappa_d;
+  const double R4 = R.feat_r, kap = R.kappa_d, R4hi = R4 * R4 * (1.0 + 1e-9);
+  const int sa = R.slot0 + anchor, ss = R.slot0 + src, sd = R.slot0 + dst;
+  double vs[3], vd[3];
+  ref_nid(R, rx[ss], ry[ss], rz[ss], rx[sa], ry[sa], rz[sa], vs[0], vs[1], vs[2]);
+  ref_nid(R, rx[sd], ry[sd], rz[sd], rx[sa], ry[sa], rz[sa], vd[0], vd[1], vd[2]);
+  // the pair's own geometry, ahead of the loop so that its loads overlap the above
+  const double f0 = mh_dist(R, rx, ry, rz, src, dst);
+  const double f10 = mh_cos_at(R, rx, ry, rz, dst), f11 = mh_cos_at(R, rx, ry, rz, src);
   for (int i = lane; i < nloc; i += 64) {
     const int m = loc[i];
     if (m == src || m == dst) continue;
-    const double rdm = mh_dist(R, rx, ry, rz, dst, m), rsm = mh_dist(R, rx, ry, rz, src, m);
+    double vm[3];
+    if (vx) {
+      vm[0] = vx[i];
+      vm[1] = vy[i];
+      vm[2] = vz[i];
+    } else {
+      const int sm = R.slot0 + m;
+      ref_nid(R, rx[sm], ry[sm], rz[sm], rx[sa], ry[sa], rz[sa], vm[0], vm[1], vm[2]);
+    }
+    // a bead beyond the feature radius of both src and dst adds nothing; the
+    // squared distances decide that without the square roots (R4hi lies above
+    // R4^2 by more than the rounding of the root)
+    double dd2, ds2 = 0.0;
+    {
+#pragma clang fp contract(off)
+      const double dx = vm[0] - vd[0], dy = vm[1] - vd[1], dz = vm[2] - vd[2];
+      dd2 = dx * dx + dy * dy + dz * dz;
+      if (!rs) {
+        const double sx = vm[0] - vs[0], sy = vm[1] - vs[1], sz = vm[2] - vs[2];
+        ds2 = sx * sx + sy * sy + sz * sz;
+      }
+    }
+    const bool near_d = dd2 < R4hi;
+    const bool near_s = rs ? rs[i] < R4 : ds2 < R4hi;
+    if (!near_d && !near_s) continue;
+    const double rdm = sqrt(dd2);
+    const double rsm = rs ? rs[i] : sqrt(ds2);
     const int qm = (m == clr) ? 0 : (m == set) ? qk : (int)qb[m];
     if (qm != 0 && (rdm < R4 || rsm < R4))
-      e += R.lambda_b * (double)(qk * qm) * (exp(-kap * rdm) / rdm - exp(-kap * rsm) / rsm);
+      e += R.lambda_b * (double)(qk * qm) * (exp(-kap * rdm) / rdm - (es ? es[i] : exp(-kap * rsm) / rsm));
     if (rdm < R4) {
       b4d += 0.1;
       if (qm != 0) {
@@ -587,7 +667,7 @@ __device__ void mh_features_wave(const RepDev& R, const double* rx, const double
   if (lane == 0) {
     const int nb = R.nb, gap = abs(dst - src);
     const bool same = (src / nb) == (dst / nb);
-    out[0] = (float)mh_dist(R, rx, ry, rz, src, dst);
+    out[0] = (float)f0;
     out[1] = (same && gap == 1) ? 1.f : 0.f;
     out[2] = same ? (float)((double)(gap < 20 ? gap : 20) / 20.0) : 1.f;
     out[3] = (float)e;
@@ -597,8 +677,8 @@ __device__ void mh_features_wave(const RepDev& R, const double* rx, const double
     out[7] = (float)s4s;
     out[8] = (float)b4d;
     out[9] = (float)b4s;
-    out[10] = (float)mh_cos_at(R, rx, ry, rz, dst);
-    out[11] = (float)mh_cos_at(R, rx, ry, rz, src);
+    out[10] = (float)f10;
+    out[11] = (float)f11;
     out[12] = (dst % nb == 0 || dst % nb == nb - 1) ? 1.f : 0.f;
     out[13] = (src % nb == 0 || src % nb == nb - 1) ? 1.f : 0.f;
     out[14] = (float)qk;
@@ -606,16 +686,29 @@ __device__ void mh_features_wave(const RepDev& R, const double* rx, const double
   }
 }
 
-// MLP score of one feature vector by one wavefront: lane j owns hidden unit j.
+// The network's weights as the MH kernels keep them in LDS: W1 and W2
+// transposed (W1T[i][j] = W1[j][i], W2T[i][j] = W2[j][i]) so that lane j's reads
+// of its hidden unit's row fall on consecutive banks; then b1, b2, W3, b3.
+__device__ __forceinline__ int mlp_lds_index(int g) {
+  if (g < 64 * 16) return (g % 16) * 64 + g / 16;  // W1[j][i] -> W1T[i][j]
+  g -= 64 * 16;
+  if (g < 64) return 64 * 16 + 64 * 64 + g;  // b1 after both matrices
+  g -= 64;
+  if (g < 64 * 64) return 64 * 16 + (g % 64) * 64 + g / 64;  // W2[j][i] -> W2T[i][j]
+  return 64 * 16 + 64 * 64 + 64 + (g - 64 * 64);  // b2, W3, b3
+}
+
+// MLP score of one feature vector by one wavefront: lane j owns hidden unit j;
+// the first layer's outputs reach every lane by readlane, in order.
 __device__ float mlp_score_wave(const float* w, const float* f) {
   const int j = threadIdx.x & 63;
-  const float *W1 = w, *b1 = W1 + 64 * 16, *W2 = b1 + 64, *b2 = W2 + 64 * 64, *W3 = b2 + 64, *b3 = W3 + 64;
+  const float *W1T = w, *W2T = W1T + 64 * 16, *b1 = W2T + 64 * 64, *b2 = b1 + 64, *W3 = b2 + 64, *b3 = W3 + 64;
   float a = b1[j];
 #pragma unroll
-  for (int i = 0; i < 16; ++i) a = fmaf(W1[j * 16 + i], f[i], a);
+  for (int i = 0; i < 16; ++i) a = fmaf(W1T[i * 64 + j], f[i], a);
   const float h1 = fmaxf(a, 0.f);
   float a2 = b2[j];
-  for (int i = 0; i < 64; ++i) a2 = fmaf(W2[j * 64 + i], __shfl(h1, i, 64), a2);
+  for (int i = 0; i < 64; ++i) a2 = fmaf(W2T[i * 64 + j], __shfl(h1, i, 64), a2);
   const float h2 = fmaxf(a2, 0.f);
   float s = W3[j] * h2;
   for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
@@ -634,83 +727,256 @@ __device__ void mh_proposal_probs(double* q, int nc, double eps) {
   for (int c = 0; c < nc; ++c) q[c] = (1.0 - eps) * q[c] / z + eps / nc;
 }
 
-__global__ __launch_bounds__(MC_THREADS) void k_mh_sweep(const RepDev* __restrict__ reps, int rep_begin,
+__device__ __forceinline__ double mh_phi(const RepDev& R, const double* rx, const double* ry, const double* rz, int a,
+                                         int b) {
+  if (a == b) return 0.0;
+  const double r = mh_dist(R, rx, ry, rz, a, b);
+  return exp(-R.kappa_d * r) / r;
+}
+
+// per-bead lists, ascending: the neighbourhood (within 2 hop + feature radius)
+// and the hop list (within the hop radius); one wavefront per bead, grid
+// (ceil(max n / 4), nrep)
+__global__ __launch_bounds__(MC_THREADS) void k_mh_count(const RepDev* __restrict__ reps, int rep_begin,
                                                          const double* __restrict__ rx, const double* __restrict__ ry,
-                                                         const double* __restrict__ rz, int* track, int* old,
-                                                         const int* sign, signed char* occ, long long occ_stride,
-                                                         const float* uniforms, int nu, const uint64_t* seeds,
-                                                         uint64_t sweep_id, const float* weights, int* consumed,
-                                                         int* mlog, float* feats, const int* log_off, int* err,
-                                                         double* qdisp) {
+                                                         const double* __restrict__ rz, int* cnt, int* hcnt) {
+  const int r = rep_begin + blockIdx.y;
+  const RepDev& R = reps[r];
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * (MC_THREADS / 64) + (threadIdx.x >> 6);
+  if (!R.active || R.nq == 0 || i >= R.n) return;
+  int c = 0, h = 0;
+  for (int base = 0; base < R.n; base += 64) {
+    const int j = base + lane;
+    double d2 = 0.0;
+    if (j < R.n) d2 = mh_dist2(R, rx, ry, rz, i, j);
+    c += __popcll(__ballot(j < R.n && d2 < R.nbhd2));
+    h += __popcll(__ballot(j < R.n && d2 < R.hop2));
+  }
+  if (lane == 0) {
+    cnt[R.slot0 + i] = c;
+    hcnt[R.slot0 + i] = h;
+  }
+}
+
+__global__ __launch_bounds__(MC_THREADS) void k_mh_fill(const RepDev* __restrict__ reps, int rep_begin,
+                                                        const double* __restrict__ rx, const double* __restrict__ ry,
+                                                        const double* __restrict__ rz, const int* __restrict__ off,
+                                                        int* list, const int* __restrict__ hoff, int* hlist) {
+  const int r = rep_begin + blockIdx.y;
+  const RepDev& R = reps[r];
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * (MC_THREADS / 64) + (threadIdx.x >> 6);
+  if (!R.active || R.nq == 0 || i >= R.n) return;
+  int* out = list + off[R.slot0 + i];
+  int* hout = hlist + hoff[R.slot0 + i];
+  int pos = 0, hpos = 0;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  for (int base = 0; base < R.n; base += 64) {
+    const int j = base + lane;
+    double d2 = 0.0;
+    if (j < R.n) d2 = mh_dist2(R, rx, ry, rz, i, j);
+    const bool f = j < R.n && d2 < R.nbhd2, h = j < R.n && d2 < R.hop2;
+    const unsigned long long m = __ballot(f), mh = __ballot(h);
+    if (f) out[pos + __popcll(m & below)] = j;
+    if (h) hout[hpos + __popcll(mh & below)] = j;
+    pos += __popcll(m);
+    hpos += __popcll(mh);
+  }
+}
+
+// V(z) for every bead of a replica with charges: one wavefront per bead
+__global__ __launch_bounds__(MC_THREADS) void k_mh_potential(const RepDev* __restrict__ reps, int rep_begin,
+                                                             const double* __restrict__ rx,
+                                                             const double* __restrict__ ry,
+                                                             const double* __restrict__ rz,
+                                                             const int* __restrict__ track,
+                                                             const int* __restrict__ sign, double* V) {
+  const int r = rep_begin + blockIdx.y;
+  const RepDev& R = reps[r];
+  const int lane = threadIdx.x & 63;
+  const int z = blockIdx.x * (MC_THREADS / 64) + (threadIdx.x >> 6);
+  if (!R.active || R.nq == 0 || z >= R.n) return;
+  double v = 0.0;
+  for (int b = lane; b < R.nq; b += 64) v += (double)sign[R.q0 + b] * mh_phi(R, rx, ry, rz, z, track[R.q0 + b]);
+  v = warp_sum(v);
+  if (lane == 0) V[R.slot0 + z] = v;
+}
+
+// occupancy (signed charge per bead) and the per-replica counters
+__global__ __launch_bounds__(MC_THREADS) void k_mh_begin(const RepDev* __restrict__ reps, int rep_begin,
+                                                         const int* track, const int* sign, signed char* occ,
+                                                         long long occ_stride, int* done, int* stop, int* consumed) {
   const int r = rep_begin + blockIdx.x;
   const RepDev& R = reps[r];
   if (!R.active) return;
+  signed char* qb = occ + (size_t)r * occ_stride;
+  for (int i = threadIdx.x; i < R.n; i += blockDim.x) qb[i] = 0;
+  __syncthreads();
+  for (int k = threadIdx.x; k < R.nq; k += blockDim.x) qb[track[R.q0 + k]] = (signed char)sign[R.q0 + k];
+  if (threadIdx.x == 0) {
+    done[r] = 0;
+    stop[r] = 0;
+    if (consumed) consumed[r] = 0;
+  }
+}
+
+// The changes of V by a round's accepted moves: one wavefront per bead, lane m
+// for move m
+__global__ __launch_bounds__(MC_THREADS) void k_mh_fold(const RepDev* __restrict__ reps, int rep_begin,
+                                                        const double* __restrict__ rx, const double* __restrict__ ry,
+                                                        const double* __restrict__ rz, const int4* __restrict__ moves,
+                                                        const int* __restrict__ nmoves, double* V) {
+  const int r = rep_begin + blockIdx.y;
+  const RepDev& R = reps[r];
+  const int lane = threadIdx.x & 63;
+  const int z = blockIdx.x * (MC_THREADS / 64) + (threadIdx.x >> 6);
+  if (!R.active || z >= R.n) return;
+  const int nm = nmoves[r];
+  if (nm == 0) return;
+  double v = 0.0;
+  if (lane < nm) {
+    const int4 mv = moves[(size_t)r * MH_ROUND + lane];
+    v = (double)mv.w * (mh_phi(R, rx, ry, rz, z, mv.z) - mh_phi(R, rx, ry, rz, z, mv.y));
+  }
+  v = warp_sum(v);
+  if (lane == 0) V[R.slot0 + z] += v;
+}
+
+// Attempts a0 .. a0 + MH_ROUND - 1 of each replica (one workgroup per replica).
+// LQ: the occupancy and the charges' beads are read from LDS copies (written
+// through to memory on every accepted move).
+template <bool LQ>
+__global__ __launch_bounds__(MC_THREADS) void k_mh_round(
+    const RepDev* __restrict__ reps, int rep_begin, const double* __restrict__ rx, const double* __restrict__ ry,
+    const double* __restrict__ rz, int* track, int* old, const int* sign, signed char* occ, long long occ_stride,
+    const float* uniforms, int nu, const uint64_t* seeds, uint64_t sweep_id, const float* weights, int* consumed,
+    int* mlog, float* feats, const int* log_off, int* err, double* qdisp, const int* __restrict__ nb_off,
+    const int* __restrict__ nb_list, const int* __restrict__ hop_off, const int* __restrict__ hop_list,
+    const double* __restrict__ V, int4* moves, int* nmoves, int* done, int* stop, int a0) {
+  const int r = rep_begin + blockIdx.x;
+  const RepDev& R = reps[r];
+  if (!R.active) return;
+  const int nq = R.nq, tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+  if (a0 >= nq || stop[r]) {
+    if (tid == 0) nmoves[r] = 0;
+    return;
+  }
   __shared__ MhShared S;
-  const int N = R.n, nq = R.nq, tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+  __shared__ typename std::conditional<LQ, MhState, MhNoState>::type L;
   const int nwv = blockDim.x >> 6;
   const bool learned = weights != nullptr;
-  signed char* qb = occ + (size_t)blockIdx.x * occ_stride;
-  for (int i = tid; i < N; i += blockDim.x) qb[i] = 0;
+  signed char* qg = occ + (size_t)r * occ_stride;
+  int* tg = track + R.q0;
+  signed char* qb = qg;
+  int* tv = tg;
+  if constexpr (LQ) {
+    qb = L.qs;
+    tv = L.trk;
+    for (int i = tid; i < R.n; i += blockDim.x) qb[i] = qg[i];
+    for (int i = tid; i < nq; i += blockDim.x) tv[i] = tg[i];
+  }
   if (learned)
-    for (int i = tid; i < MLP_NP; i += blockDim.x) S.w[i] = weights[i];
-  __syncthreads();
-  for (int k = tid; k < nq; k += blockDim.x) qb[track[R.q0 + k]] = (signed char)sign[R.q0 + k];
-  __syncthreads();
+    for (int i = tid; i < MLP_NP; i += blockDim.x) S.w[mlp_lds_index(i)] = weights[i];
+  const int aend = min(a0 + MH_ROUND, nq);
   const float* ur = uniforms ? uniforms + (size_t)r * nu : nullptr;
-  int done_att = 0;
-  for (int a = 0; a < nq; ++a) {
+  for (int i = tid; i < 3 * (aend - a0); i += blockDim.x) {
+    const int g = 3 * a0 + i;
+    S.ur[i] = ur ? (g < nu ? ur[g] : -1.0f) : dev_uniform(seeds[r], sweep_id, (uint32_t)g);
+  }
+  if (tid == 0) S.nmv = 0;
+  __syncthreads();
+  // each attempt's bead as of the start of the round, its list range and V there
+  for (int i = tid; i < aend - a0; i += blockDim.x) {
+    const float u0 = S.ur[3 * i];
+    int kk = (int)((u0 < 0.0f ? 0.0f : u0) * (float)nq);
+    kk = kk > nq - 1 ? nq - 1 : kk;
+    const int xp = tv[kk];
+    S.xp[i] = xp;
+    S.qkp[i] = sign[R.q0 + kk];
+    S.op[i] = nb_off[R.slot0 + xp];
+    S.oe[i] = nb_off[R.slot0 + xp + 1];
+    const int h0 = hop_off[R.slot0 + xp], h1 = hop_off[R.slot0 + xp + 1];
+    S.hp[i] = h0;
+    S.he[i] = h1;
+    S.vxp[i] = V[R.slot0 + xp];
+    for (int e = 0; e < MH_HOP_PRE && e < h1 - h0; ++e) S.hl[i][e] = hop_list[h0 + e];
+  }
+  __syncthreads();
+  int done_att = done[r];
+  bool halted = false;
+  for (int a = a0; a < aend; ++a) {
     if (tid == 0) {
       S.bad = 0;
       for (int i = 0; i < 3; ++i) {
-        if (ur) {
-          if (3 * a + i < nu) {
-            S.u[i] = ur[3 * a + i];
-          } else {
-            S.bad = 1;
-            S.u[i] = 0.f;
-          }
-        } else {
-          S.u[i] = dev_uniform(seeds[r], sweep_id, (uint32_t)(3 * a + i));
-        }
+        const float u = S.ur[3 * (a - a0) + i];
+        if (u < 0.0f) S.bad = 1;
+        S.u[i] = u < 0.0f ? 0.f : u;
       }
       int k = (int)(S.u[0] * (float)nq);
       S.k = k > nq - 1 ? nq - 1 : k;
-      S.x = track[R.q0 + S.k];
-      S.qk = sign[R.q0 + S.k];
-      S.nloc = 0;
+      S.x = tv[S.k];
+      S.qk = S.qkp[a - a0];
       S.ncand = 0;
     }
     __syncthreads();
     if (S.bad) {
       if (tid == 0) atomicOr(err, 2);
+      halted = true;
       break;
     }
     const int k = S.k, x = S.x, qk = S.qk;
     int* lg = mlog ? mlog + 6 * (size_t)(log_off[r] + a) : nullptr;
     float* fa = feats ? feats + 16 * (size_t)(log_off[r] + a) : nullptr;
-    // neighbourhood of x (x included), ascending
-    for (int base = 0; base < N; base += blockDim.x) {
-      const int b = base + tid;
-      const bool f = b < N && mh_dist2(R, rx, ry, rz, x, b) < R.nbhd2;
-      block_append(f, b, S.loc, &S.nloc, MH_MAXL, S.wcnt);
-    }
-    const int nloc = S.nloc;
+    // neighbourhood of x (x included), ascending, from its list: needed by the
+    // features only
+    const bool need_loc = learned || fa != nullptr;
+    const bool pre = S.xp[a - a0] == x;
+    const int o0 = pre ? S.op[a - a0] : nb_off[R.slot0 + x];
+    const int nloc = (pre ? S.oe[a - a0] : nb_off[R.slot0 + x + 1]) - o0;
+    const int h0 = pre ? S.hp[a - a0] : hop_off[R.slot0 + x];
+    const int nh = (pre ? S.he[a - a0] : hop_off[R.slot0 + x + 1]) - h0;
+    const double vx = pre ? S.vxp[a - a0] : V[R.slot0 + x];
     if (nloc > MH_MAXL) {
       if (tid == 0) atomicOr(err, 8);
+      halted = true;
       break;
     }
-    // candidates of x
-    for (int base = 0; base < nloc; base += blockDim.x) {
-      const int i = base + tid;
-      const int b = i < nloc ? S.loc[i] : -1;
-      const bool f = b >= 0 && b != x && qb[b] == 0 && mh_dist2(R, rx, ry, rz, x, b) < R.hop2;
-      block_append(f, b, S.cand, &S.ncand, MH_MAXC, S.wcnt);
+    if (need_loc)
+      for (int i = tid; i < nloc; i += blockDim.x) S.loc[i] = nb_list[o0 + i];
+    // candidates of x: its hop list less x and the charged beads (one wavefront)
+    if (wv == 0) {
+      int cnt = 0;
+      for (int base = 0; base < nh; base += 64) {
+        const int i = base + lane;
+        const int b = i < nh ? ((pre && nh <= MH_HOP_PRE) ? S.hl[a - a0][i] : hop_list[h0 + i]) : -1;
+        const bool f = b >= 0 && b != x && qb[b] == 0;
+        const unsigned long long m = __ballot(f);
+        const int p = cnt + __popcll(m & ((1ull << lane) - 1ull));
+        if (f && p < MH_MAXC) S.cand[p] = b;
+        cnt += __popcll(m);
+      }
+      if (lane == 0) S.ncand = cnt;
     }
+    __syncthreads();
     const int nc = S.ncand;
     if (nc > MH_MAXC) {
       if (tid == 0) atomicOr(err, 8);
+      halted = true;
       break;
     }
+    const bool src_lds = nloc <= MH_SRC_L;
+    if (need_loc && src_lds && nc > 0) {
+      const int sx = R.slot0 + x;
+      for (int i = tid; i < nloc; i += blockDim.x) {
+        const int sm = R.slot0 + S.loc[i];
+        ref_nid(R, rx[sm], ry[sm], rz[sm], rx[sx], ry[sx], rz[sx], S.vx[i], S.vy[i], S.vz[i]);
+      }
+      __syncthreads();
+    }
+    const double* VX = src_lds ? S.vx : nullptr;
+    const double* VY = src_lds ? S.vy : nullptr;
+    const double* VZ = src_lds ? S.vz : nullptr;
     if (nc == 0) {
       if (tid == 0) {
         if (lg) {
@@ -730,8 +996,18 @@ __global__ __launch_bounds__(MC_THREADS) void k_mh_sweep(const RepDev* __restric
     }
     // forward proposal
     if (learned) {
+      if (src_lds) {  // x's own vector is zero
+        for (int i = tid; i < nloc; i += blockDim.x) {
+          if (S.loc[i] == x) continue;
+          const double rsm = mh_rel_dist(0.0, 0.0, 0.0, S.vx[i], S.vy[i], S.vz[i]);
+          S.rs[i] = rsm;
+          S.es[i] = exp(-R.kappa_d * rsm) / rsm;
+        }
+        __syncthreads();
+      }
       for (int c = wv; c < nc; c += nwv) {
-        mh_features_wave(R, rx, ry, rz, qb, S.loc, nloc, x, S.cand[c], qk, -1, -1, nc, S.feat[c]);
+        mh_features_rel(R, rx, ry, rz, qb, S.loc, nloc, x, S.cand[c], qk, -1, -1, nc, S.feat[c], x, VX, VY, VZ,
+                        src_lds ? S.rs : nullptr, src_lds ? S.es : nullptr);
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
         const float s = mlp_score_wave(S.w, S.feat[c]);
@@ -767,39 +1043,67 @@ __global__ __launch_bounds__(MC_THREADS) void k_mh_sweep(const RepDev* __restric
       if (learned) {
         if (tid < 16) S.fsel[tid] = S.feat[S.pick][tid];
       } else if (wv == 0) {
-        mh_features_wave(R, rx, ry, rz, qb, S.loc, nloc, x, y, qk, -1, -1, nc, S.fsel);
+        mh_features_rel(R, rx, ry, rz, qb, S.loc, nloc, x, y, qk, -1, -1, nc, S.fsel, x, VX, VY, VZ, nullptr, nullptr);
       }
     }
-    // exact energy change over all charges
-    double de = 0.0;
-    for (int m = tid; m < nq; m += blockDim.x) {
-      if (m == k) continue;
-      const int bm = track[R.q0 + m], qm = sign[R.q0 + m];
-      const double rym = mh_dist(R, rx, ry, rz, y, bm), rxm = mh_dist(R, rx, ry, rz, x, bm);
-      de += R.lambda_b * (double)(qk * qm) * (exp(-R.kappa_d * rym) / rym - exp(-R.kappa_d * rxm) / rxm);
+    // exact energy change over all charges: V at the start of the round plus
+    // this round's earlier moves, one of the four terms of one move per thread
+    double dv = 0.0;
+    {
+      const int m = tid & 63, term = tid >> 6;
+      if (m < S.nmv && term < 4) {
+        const int4 mv = S.mv[m];
+        const int p = (term < 2) ? y : x, b = (term & 1) ? mv.y : mv.z;
+        const double ph = mh_phi(R, rx, ry, rz, p, b);
+        dv = (double)mv.w * ((term == 0 || term == 3) ? ph : -ph);
+      }
     }
-    de = block_sum(de, S.dsum);
-    // reverse proposal from y in the state after the move
-    if (tid == 0) S.ncand = 0;
+    const double vy = V[R.slot0 + y];
+    dv = block_sum(dv, S.dsum);
+    const double de = R.lambda_b * (double)qk * (vy - vx + dv) - R.lambda_b * mh_phi(R, rx, ry, rz, x, y);
+    // reverse proposal from y in the state after the move: y's hop list within
+    // x's neighbourhood, less y and the beads charged after the move (one wavefront)
+    if (wv == 0) {
+      const int g0 = hop_off[R.slot0 + y], ng = hop_off[R.slot0 + y + 1] - g0;
+      int cnt = 0;
+      for (int base = 0; base < ng; base += 64) {
+        const int i = base + lane;
+        const int b = i < ng ? hop_list[g0 + i] : -1;
+        bool f = false;
+        if (b >= 0 && b != y) {
+          const int qbb = (b == x) ? 0 : (int)qb[b];
+          f = qbb == 0 && mh_dist2(R, rx, ry, rz, x, b) < R.nbhd2;
+        }
+        const unsigned long long m = __ballot(f);
+        const int p = cnt + __popcll(m & ((1ull << lane) - 1ull));
+        if (f && p < MH_MAXC) S.cand[p] = b;
+        cnt += __popcll(m);
+      }
+      if (lane == 0) S.ncand = cnt;
+    }
     __syncthreads();
-    for (int base = 0; base < nloc; base += blockDim.x) {
-      const int i = base + tid;
-      const int b = i < nloc ? S.loc[i] : -1;
-      bool f = false;
-      if (b >= 0 && b != y) {
-        const int qbb = (b == x) ? 0 : (int)qb[b];
-        f = qbb == 0 && mh_dist2(R, rx, ry, rz, y, b) < R.hop2;
-      }
-      block_append(f, b, S.cand, &S.ncand, MH_MAXC, S.wcnt);
-    }
     const int nc2 = S.ncand;
     if (nc2 > MH_MAXC) {
       if (tid == 0) atomicOr(err, 8);
+      halted = true;
       break;
     }
     if (learned) {
+      if (src_lds) {
+        const int sy = R.slot0 + y, sx = R.slot0 + x;
+        double vy3[3];
+        ref_nid(R, rx[sy], ry[sy], rz[sy], rx[sx], ry[sx], rz[sx], vy3[0], vy3[1], vy3[2]);
+        for (int i = tid; i < nloc; i += blockDim.x) {
+          if (S.loc[i] == y) continue;
+          const double rsm = mh_rel_dist(vy3[0], vy3[1], vy3[2], S.vx[i], S.vy[i], S.vz[i]);
+          S.rs[i] = rsm;
+          S.es[i] = exp(-R.kappa_d * rsm) / rsm;
+        }
+        __syncthreads();
+      }
       for (int c = wv; c < nc2; c += nwv) {
-        mh_features_wave(R, rx, ry, rz, qb, S.loc, nloc, y, S.cand[c], qk, x, y, nc2, S.feat[c]);
+        mh_features_rel(R, rx, ry, rz, qb, S.loc, nloc, y, S.cand[c], qk, x, y, nc2, S.feat[c], x, VX, VY, VZ,
+                        src_lds ? S.rs : nullptr, src_lds ? S.es : nullptr);
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
         const float s = mlp_score_wave(S.w, S.feat[c]);
@@ -823,9 +1127,18 @@ __global__ __launch_bounds__(MC_THREADS) void k_mh_sweep(const RepDev* __restric
       S.accept = acc;
       old[R.q0 + k] = x;
       if (acc) {
-        track[R.q0 + k] = y;
-        qb[x] = 0;
-        qb[y] = (signed char)qk;
+        tg[k] = y;
+        qg[x] = 0;
+        qg[y] = (signed char)qk;
+        if constexpr (LQ) {
+          tv[k] = y;
+          qb[x] = 0;
+          qb[y] = (signed char)qk;
+        }
+        const int4 mv = make_int4(k, x, y, qk);
+        S.mv[S.nmv] = mv;
+        moves[(size_t)r * MH_ROUND + S.nmv] = mv;
+        S.nmv = S.nmv + 1;
         if (qdisp) {
           double d[3];
           hop_vector(R, rx, ry, rz, R.slot0 + y, R.slot0 + x, d);
@@ -845,14 +1158,50 @@ __global__ __launch_bounds__(MC_THREADS) void k_mh_sweep(const RepDev* __restric
     done_att = a + 1;
     __syncthreads();
   }
-  if (tid == 0 && consumed) consumed[r] = 3 * done_att;
+  if (tid == 0) {
+    nmoves[r] = S.nmv;
+    done[r] = done_att;
+    if (halted) stop[r] = 1;
+    if (consumed) consumed[r] = 3 * done_att;
+  }
 }
 
-void launch_mh_sweep(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, const double* rx, const double* ry,
-                     const double* rz, int* track, int* old, const int* sign, signed char* occ, long long occ_stride,
-                     const float* uniforms, int nu, const uint64_t* seeds, uint64_t sweep_id, const float* weights,
-                     int* consumed, int* log, float* feats, const int* log_off, int* err, double* qdisp) {
+void launch_mh_lists(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, int max_n, const double* rx,
+                     const double* ry, const double* rz, int* cnt, const int* off, int* list, int* hcnt,
+                     const int* hoff, int* hlist, bool fill) {
+  if (nrep <= 0 || max_n <= 0) return;
+  const dim3 g((max_n + MC_THREADS / 64 - 1) / (MC_THREADS / 64), nrep);
+  if (fill)
+    k_mh_fill<<<g, MC_THREADS, 0, st>>>(reps, rep_begin, rx, ry, rz, off, list, hoff, hlist);
+  else
+    k_mh_count<<<g, MC_THREADS, 0, st>>>(reps, rep_begin, rx, ry, rz, cnt, hcnt);
+}
+
+void launch_mh_sweep(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, int max_n, int max_nq,
+                     const double* rx, const double* ry, const double* rz, int* track, int* old, const int* sign,
+                     signed char* occ, long long occ_stride, const float* uniforms, int nu, const uint64_t* seeds,
+                     uint64_t sweep_id, const float* weights, int* consumed, int* log, float* feats,
+                     const int* log_off, int* err, double* qdisp, const int* nb_off, const int* nb_list,
+                     const int* hop_off, const int* hop_list, double* V, void* moves, int* nmoves, int* done,
+                     int* stop) {
   if (nrep <= 0) return;
-  k_mh_sweep<<<nrep, MC_THREADS, 0, st>>>(reps, rep_begin, rx, ry, rz, track, old, sign, occ, occ_stride, uniforms, nu,
-                                          seeds, sweep_id, weights, consumed, log, feats, log_off, err, qdisp);
+  k_mh_begin<<<nrep, MC_THREADS, 0, st>>>(reps, rep_begin, track, sign, occ, occ_stride, done, stop, consumed);
+  if (max_nq <= 0 || max_n <= 0) return;
+  const dim3 g((max_n + MC_THREADS / 64 - 1) / (MC_THREADS / 64), nrep);
+  k_mh_potential<<<g, MC_THREADS, 0, st>>>(reps, rep_begin, rx, ry, rz, track, sign, V);
+  const bool lq = max_n <= MH_LDS_N && max_nq <= MH_LDS_Q;
+  for (int a0 = 0; a0 < max_nq; a0 += MH_ROUND) {
+    if (lq)
+      k_mh_round<true><<<nrep, MC_THREADS, 0, st>>>(reps, rep_begin, rx, ry, rz, track, old, sign, occ, occ_stride,
+                                                    uniforms, nu, seeds, sweep_id, weights, consumed, log, feats,
+                                                    log_off, err, qdisp, nb_off, nb_list, hop_off, hop_list, V, (int4*)moves, nmoves,
+                                                    done, stop, a0);
+    else
+      k_mh_round<false><<<nrep, MC_THREADS, 0, st>>>(reps, rep_begin, rx, ry, rz, track, old, sign, occ, occ_stride,
+                                                     uniforms, nu, seeds, sweep_id, weights, consumed, log, feats,
+                                                     log_off, err, qdisp, nb_off, nb_list, hop_off, hop_list, V, (int4*)moves, nmoves,
+                                                     done, stop, a0);
+    if (a0 + MH_ROUND < max_nq)
+      k_mh_fold<<<g, MC_THREADS, 0, st>>>(reps, rep_begin, rx, ry, rz, (const int4*)moves, nmoves, V);
+  }
 }

@@ -112,6 +112,7 @@ def lib():
         L.smdlt_mlp_set_weights.argtypes = [C.c_void_p, _fp]
         L.smdlt_mlp_loss_grad.argtypes = [C.c_void_p, _fp, C.c_void_p, C.c_void_p, C.c_int, _fp, _fp]
         L.smdlt_mc_features.argtypes = [C.c_void_p, C.c_int, _fp, _ip, _ip, _ip, C.c_int, _ip]
+        L.smdlt_mc_record_features.argtypes = [C.c_void_p, C.c_int]
         L.smdlt_tea_avg_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.POINTER(C.c_void_p)]
         L.smdlt_tea_avg_destroy.argtypes = [C.c_void_p]
         L.smdlt_tea_avg_destroy.restype = None
@@ -438,6 +439,10 @@ class Context:
                                      _ptr(acc, C.c_int), cap, _ptr(n, C.c_int)), "mc_features")
         k = int(n[0])
         return f[:k], fr[:k], to[:k], acc[:k]
+
+    def mc_record_features(self, on=True):
+        """smdlt_mc_record_features: MH sweeps record their moves' features (default on)."""
+        _chk(lib().smdlt_mc_record_features(self.h, 1 if on else 0), "mc_record_features")
 
     def mc_energy(self):
         e = np.zeros(self.nrep)

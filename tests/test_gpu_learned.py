@@ -164,3 +164,73 @@ def tau_int_multi(E, c=5.0):
         if W >= c * tau:
             break
     return tau
+
+
+@pytest.mark.parametrize("learned", [False, True])
+def test_mh_many_rounds_match_oracle(learned):
+    # 150 charges: three rounds of 64 attempts with the potential folded between them
+    nq, nb, nc, L = 150, 60, 6, 26.0
+    ctx, lat, pos, track, sign, qbead, op = mh_case(1, nq, L=L, nb=nb, nc=nc, b=(0.1, 0.4), seed=21)
+    w = MLP.init(np.random.default_rng(9), scale=2.0) if learned else None
+    if learned:
+        ctx.mlp_set_weights(w)
+    mode = S.MC_MH_LEARNED if learned else S.MC_MH_UNIFORM
+    rng = np.random.default_rng(11)
+    tot = 0
+    for sweep in range(3):
+        u = rng.random(3 * nq).astype(np.float32)
+        cons, log = ctx.mc_sweep(mode, u[None, :])
+        acc, olog, _ = O.mh_sweep(lat, op, *pos, track, sign, qbead, w, u)
+        assert cons[0] == 3 * nq and np.array_equal(log, olog)
+        assert np.array_equal(ctx.download_charges()[0], track)
+        tot += acc
+    assert tot > 40
+    ctx.close()
+
+
+def test_mh_stream_exhaustion_and_feature_switch():
+    nq = 150
+    ctx, lat, pos, track, sign, qbead, op = mh_case(1, nq, L=26.0, nb=60, nc=6, b=(0.1, 0.4), seed=22)
+    u = np.random.default_rng(12).random(3 * 100 + 2).astype(np.float32)
+    from test_gpu_parity import _raw_sweep
+    code, cons, log = _raw_sweep(ctx, S.MC_MH_UNIFORM, u[None, :], nq)
+    acc, olog, _ = O.mh_sweep(lat, op, *pos, track, sign, qbead, None, u[:300])
+    assert code == -9 and cons[0] == 300
+    assert np.array_equal(log[:100], olog[:100])
+    assert np.array_equal(ctx.download_charges()[0], track)
+    # without feature records the sweep decides the same moves
+    ctx.mc_record_features(False)
+    u2 = np.random.default_rng(13).random(3 * nq).astype(np.float32)
+    cons, log = ctx.mc_sweep(S.MC_MH_UNIFORM, u2[None, :])
+    acc, olog, _ = O.mh_sweep(lat, op, *pos, track, sign, qbead, None, u2)
+    assert np.array_equal(log, olog)
+    with pytest.raises(S.SmdltError):
+        ctx.mc_features(0)
+    ctx.mc_record_features(True)
+    ctx.close()
+
+
+def test_mh_large_replica_matches_oracle():
+    # 40,000 beads: beyond the sizes whose occupancy stays in LDS
+    nb, nc, L, nq = 400, 100, 110.0, 300
+    ctx, p = ctx_for(1, L, nb, nc, edot=0.3, lambda_d=5.0, lambda_b=3.0, barrier=0.2, barrier2=0.5)
+    lat = O.lattice(1, L, 0.3, 1e-3, 1)
+    rx, ry, rz = xyz(random_walk(nb, nc, L, 23))
+    ctx.upload_positions(rx, ry, rz)
+    ctx.remap(5)
+    rx, ry, rz = oracle_remap(lat, 5, rx, ry, rz)
+    rng = np.random.default_rng(14)
+    track = rng.choice(nb * nc, nq, replace=False).astype(np.int32)
+    sign = np.where(np.arange(nq) % 2 == 0, 1, -1).astype(np.int32)
+    ctx.upload_charges([nq], track, sign)
+    op = O.mh_params(nb * nc, nb, hop_radius=3.0, feature_radius=4.0, floor_eps=0.05, lambda_b=3.0,
+                     lambda_d=5.0, barrier=0.2, barrier2=0.5)
+    qbead = np.zeros(nb * nc, np.int32)
+    qbead[track] = sign
+    track = track.copy()
+    u = rng.random(3 * nq).astype(np.float32)
+    cons, log = ctx.mc_sweep(S.MC_MH_UNIFORM, u[None, :])
+    acc, olog, _ = O.mh_sweep(lat, op, rx, ry, rz, track, sign, qbead, None, u)
+    print(f"40000-bead MH sweep: {acc} of {nq} accepted")
+    assert np.array_equal(log, olog) and np.array_equal(ctx.download_charges()[0], track)
+    ctx.close()

@@ -712,3 +712,69 @@ def test_mc_energy_matches_oracle():
     q[track] = sign
     e = ctx.mc_energy()[0]
     assert e == pytest.approx(O.coulomb_energy(lat, rx, ry, rz, q, 3.59, 3.0), rel=1e-12)
+
+
+def _raw_sweep(ctx, mode, u, ntot):
+    """smdlt_mc_sweep called directly, so that the consumed counts and records
+    of a sweep that ends with an error code can be read."""
+    import ctypes as C
+    u = np.ascontiguousarray(u, np.float32)
+    cons = np.zeros(ctx.nrep, np.int32)
+    log = np.zeros(6 * ntot, np.int32)
+    code = S.lib().smdlt_mc_sweep(ctx.h, mode, u.ctypes.data_as(C.POINTER(C.c_float)), u.shape[-1],
+                                  cons.ctypes.data_as(C.POINTER(C.c_int)), log.ctypes.data_as(C.POINTER(C.c_int)))
+    return code, cons, log.reshape(-1, 6)
+
+
+@pytest.mark.parametrize("flow,b1,b2", [(1, 0.4, 0.9), (2, 2.0, 0.0)])
+def test_mc_reference_many_rounds_bit_exact(flow, b1, b2):
+    # 200 charges: the sweep's blocks of 64 and the folds between them
+    L, nb, nc, nq = 30.0, 50, 8, 200
+    ctx, p = ctx_for(flow, L, nb, nc, edot=0.3, lambda_d=5.0, lambda_b=3.59, barrier=b1, barrier2=b2)
+    lat = O.lattice(flow, L, 0.3, 1e-3, 1)
+    rx, ry, rz = xyz(random_walk(nb, nc, L, 31))
+    ctx.upload_positions(rx, ry, rz)
+    ctx.remap(7)
+    rx, ry, rz = oracle_remap(lat, 7, rx, ry, rz)
+    track = np.random.default_rng(5).choice(nb * nc, nq, replace=False).astype(np.int32)
+    ctx.upload_charges([nq], track)
+    mc = O.ChargeHop(nb, nc, track)
+    rng = O.Ran1(4242)
+    acc = 0
+    for sweep in range(3):
+        u = rng.uniforms(2 * nq + 8)
+        cons, log = ctx.mc_sweep(S.MC_REFERENCE, u[None, :])
+        st, cur, olog, _ = mc.sweep(lat, rx, ry, rz, 3.59, 5.0, b1, b2, u)
+        assert st == 0 and cons[0] == cur
+        assert np.array_equal(ctx.download_charges()[0], mc.track)
+        assert np.array_equal(log[:, 1], olog[:, 1]) and np.array_equal(log[:, 4:], olog[:, 4:])
+        keep = olog[:, 3] >= 0
+        assert np.array_equal(log[keep, 2:4], olog[keep, 2:4])
+        acc += int(olog[:, 4].sum())
+    assert acc > 50
+    ctx.close()
+
+
+def test_mc_reference_stream_exhaustion_matches_oracle():
+    # the uniforms run out in the third block: the charges before it are decided
+    # as the oracle decides them, the count consumed is the whole stream and the
+    # sweep reports SMDLT_E_MC_STREAM
+    L, nb, nc, nq = 30.0, 50, 8, 200
+    ctx, p = ctx_for(1, L, nb, nc, edot=0.3, lambda_d=5.0, lambda_b=3.59, barrier=0.6, barrier2=0.9)
+    lat = O.lattice(1, L, 0.3, 1e-3, 1)
+    rx, ry, rz = xyz(random_walk(nb, nc, L, 32))
+    ctx.upload_positions(rx, ry, rz)
+    ctx.remap(3)
+    rx, ry, rz = oracle_remap(lat, 3, rx, ry, rz)
+    track = np.random.default_rng(6).choice(nb * nc, nq, replace=False).astype(np.int32)
+    ctx.upload_charges([nq], track)
+    mc = O.ChargeHop(nb, nc, track)
+    u = O.Ran1(99).uniforms(190)
+    code, cons, log = _raw_sweep(ctx, S.MC_REFERENCE, u[None, :], nq)
+    st, cur, olog, _ = mc.sweep(lat, rx, ry, rz, 3.59, 5.0, 0.6, 0.9, u)
+    assert code == -9 and st == -2 and cons[0] == cur == 190
+    assert np.array_equal(ctx.download_charges()[0], mc.track)
+    done = int(np.nonzero(olog[:, 5])[0].max()) + 1  # charges the oracle recorded
+    assert done > 128
+    assert np.array_equal(log[:done, 1], olog[:done, 1]) and np.array_equal(log[:done, 4:], olog[:done, 4:])
+    ctx.close()
