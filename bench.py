@@ -17,6 +17,13 @@ Also reported (`semidilute`): the 8000-bead semidilute polyelectrolyte under
 shear (40 x 200 beads, L = 80, Wi = 50, 4000 charges +-1), with its plain
 FP64 path and the M.F error of the mixed path against a tight FP64 evaluation.
 M.F and M^(1/2) z errors against the CPU oracle are measured by the tests.
+
+Also reported (`charge_hopping`): 20 chains x 300 beads with 2400 charges
+(hop radius 3): the sweep time of the reference chargeHop, of the exactly
+balanced hop MC with uniform and with learned proposals, Brownian dynamics
+with a learned-proposal sweep every 10 steps, and tau_int of the Coulomb
+energy and of the charge configuration per sweep, uniform against learned
+(16 replicas; the proposal trained on 3e5 uniform-proposal moves).
 """
 import argparse
 import json
@@ -33,6 +40,7 @@ from smdlt_workloads import (c2_charges, c2_params, c4_params, c4_specs, chains,
                              grow_chains_grouped, to_lab_frame)
 
 HBM_PEAK_GBS = 8000.0
+USE_GRAPH = True  # --eager: steps run eagerly (for tracing tools) instead of replaying hipGraphs
 
 
 def tuner_summary(cfgs):
@@ -58,10 +66,10 @@ def ensemble_leg(nrep, steps, warmup, plain_steps):
     print(f"[bench] ensemble: {nrep} replicas, {ntot} beads, chains grown in {time.perf_counter() - t0:.1f}s",
           file=sys.stderr, flush=True)
     ctx.upload_positions(allp[:, 0], allp[:, 1], allp[:, 2])
-    ctx.run(0, warmup, use_graph=True)
+    ctx.run(0, warmup, use_graph=USE_GRAPH)
     ctx.sync()
     t1 = time.perf_counter()
-    code = ctx.run(warmup, steps, use_graph=True)
+    code = ctx.run(warmup, steps, use_graph=USE_GRAPH)
     ctx.sync()
     el = time.perf_counter() - t1
     iters, status, _ = ctx.lanczos_status()
@@ -82,7 +90,7 @@ def ensemble_leg(nrep, steps, warmup, plain_steps):
     dom = max(kern, key=lambda k: kern[k][0])
     avg_ms = kern[dom][0] / kern[dom][1]
     ach = kb[dom] / (avg_ms * 1e-3) / 1e9
-    sym, traffic = pmc_traffic("c4", stage_kernels(dom, cfgs[0]["order"], bool(cfgs[0]["cluster_pairs"])))
+    sym, traffic = pmc_traffic("ensemble", stage_kernels(dom, cfgs[0]["order"], bool(cfgs[0]["cluster_pairs"])))
     roof = {"bound": "hbm", "kernel": sym or dom, "stage": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": ach / HBM_PEAK_GBS, "traffic": traffic, "bytes_per_launch": kb[dom], "avg_launch_ms": avg_ms,
             "grid_points": G, "list_entries": ent,
@@ -129,10 +137,10 @@ def semidilute_leg(steps, warmup, plain_steps):
     cfg = ctx.mobility_config()
     ctx.upload_positions(pos[:, 0], pos[:, 1], pos[:, 2])
     ctx.upload_charges([len(track)], track, sign)
-    ctx.run(0, warmup, use_graph=True)
+    ctx.run(0, warmup, use_graph=USE_GRAPH)
     ctx.sync()
     t0 = time.perf_counter()
-    code = ctx.run(warmup, steps, use_graph=True)
+    code = ctx.run(warmup, steps, use_graph=USE_GRAPH)
     ctx.sync()
     el = time.perf_counter() - t0
     value = n * steps / el
@@ -150,7 +158,7 @@ def semidilute_leg(steps, warmup, plain_steps):
     dom = max(kern, key=lambda k: kern[k][0])
     avg_ms = kern[dom][0] / kern[dom][1]
     achieved = kb[dom] / (avg_ms * 1e-3) / 1e9
-    sym, traffic = pmc_traffic("c2", stage_kernels(dom, cfg["order"], bool(mc["cluster_pairs"])))
+    sym, traffic = pmc_traffic("semidilute", stage_kernels(dom, cfg["order"], bool(mc["cluster_pairs"])))
     rng = np.random.default_rng(3)
     v = rng.normal(size=3 * n)
     p64 = c2_params(S.PREC_FP64)
@@ -271,17 +279,21 @@ def mh_study(ctx, R, nq, n, track, sign, train_moves, sweeps, burn, iters, hops_
 
     def chain(mode):
         ctx.upload_charges([nq] * R, np.tile(track, R), np.tile(sign, R))
+        ctx.mc_record_features(False)  # production sweeps: no training records
         E = np.zeros((sweeps, R))
         occ = np.zeros((sweeps, R, nq), np.int32)
         acc = 0
-        ts = time.perf_counter()
+        el = 0.0
         for s in range(sweeps):
+            ctx.sync()
+            ts = time.perf_counter()
             _, log = ctx.mc_sweep(mode)
+            el += time.perf_counter() - ts
             acc += int(log[:, 4].sum())
             E[s] = ctx.mc_energy()
             tr, _ = ctx.download_charges()
             occ[s] = tr.reshape(R, nq)
-        el = time.perf_counter() - ts
+        ctx.mc_record_features(True)
         sub = occ[burn:]
         ns = len(sub)
         O3 = np.zeros((ns, R, n), np.float32)
@@ -301,7 +313,7 @@ def mh_study(ctx, R, nq, n, track, sign, train_moves, sweeps, burn, iters, hops_
     return res, w
 
 
-def learned_leg(replicas=16, train_moves=1_000_000, sweeps=150, burn=30, iters=1000):
+def learned_leg(replicas=16, train_moves=1_000_000, sweeps=150, burn=30, iters=1000, strong=True):
     """Charge hopping with the learned proposal.
     (1) 20 chains x 300 beads (L = 60, 40 % charged, lambda_B = 3.59, lambda_D = 5,
     hop radius 3a): bead-steps/s of Brownian dynamics with one learned-proposal MC
@@ -330,27 +342,54 @@ def learned_leg(replicas=16, train_moves=1_000_000, sweeps=150, burn=30, iters=1
     bd = S.Context(params(0, 3.0))
     bd.upload_positions(pos[:, 0], pos[:, 1], pos[:, 2])
     bd.upload_charges([nq], track, sign)
-    bd.run(0, 2, use_graph=True)
+    bd.run(0, 2, use_graph=USE_GRAPH)
     x, y, z = bd.download_positions()
+    # sweep times of the three modes on one replica (reference chargeHop on
+    # device uniforms; MH without training records)
+    bd.mc_record_features(False)
+    sweep_ms = {}
+    for mode, name in ((S.MC_REFERENCE, "reference_chargehop"), (S.MC_MH_UNIFORM, "mh_uniform")):
+        bd.upload_charges([nq], track, sign)
+        bd.mc_sweep(mode)
+        bd.sync()
+        t0 = time.perf_counter()
+        for _ in range(5):
+            bd.mc_sweep(mode)
+        bd.sync()
+        sweep_ms[name] = 1e3 * (time.perf_counter() - t0) / 5
+    bd.upload_charges([nq], track, sign)
+    bd.mc_record_features(True)
     R = replicas
     ctx = S.Context([params(r, 3.0) for r in range(R)])
     ctx.upload_positions(np.tile(x, R), np.tile(y, R), np.tile(z, R))
     ctx.remap(2)
-    c5, w = mh_study(ctx, R, nq, n, track, sign, train_moves, sweeps, burn, iters, tag="20x300 hop 3")
+    hop3, w = mh_study(ctx, R, nq, n, track, sign, train_moves, sweeps, burn, iters, tag="20x300 hop 3")
     ctx.close()
     bd.mlp_set_weights(w)
+    bd.mc_record_features(False)
+    bd.mc_sweep(S.MC_MH_LEARNED)
     bd.sync()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        bd.mc_sweep(S.MC_MH_LEARNED)
+    bd.sync()
+    sweep_ms["mh_learned"] = 1e3 * (time.perf_counter() - t0) / 5
     t0 = time.perf_counter()
     nblk = 3
     for b in range(nblk):
-        bd.run(2 + 10 * b, 10, use_graph=True)
+        bd.run(2 + 10 * b, 10, use_graph=USE_GRAPH)
         bd.mc_sweep(S.MC_MH_LEARNED)
     bd.sync()
     el = time.perf_counter() - t0
     bd.close()
-    out.update({"bead_steps_per_s_bd_plus_mc": n * 10 * nblk / el, "mc_every": 10, "c5": c5,
-                "c5_tau_E_ratio": c5["uniform"]["tau_E"] / c5["learned"]["tau_E"],
-                "c5_tau_config_ratio": c5["uniform"]["tau_config"] / c5["learned"]["tau_config"]})
+    out.update({"bead_steps_per_s_bd_plus_mc": n * 10 * nblk / el, "mc_every": 10,
+                "sweep_ms_one_replica": sweep_ms, "chains": hop3,
+                "tau_E_ratio_uniform_over_learned": hop3["uniform"]["tau_E"] / hop3["learned"]["tau_E"],
+                "tau_config_ratio_uniform_over_learned": hop3["uniform"]["tau_config"] / hop3["learned"]["tau_config"]})
+    print(f"[bench] charge hopping: sweeps {sweep_ms} ms, tau_E ratio {out['tau_E_ratio_uniform_over_learned']:.2f}", file=sys.stderr,
+          flush=True)
+    if not strong:
+        return out
     # strongly coupled small system
     nb2, nc2, L2, nq2, R2 = 30, 4, 18.0, 40, 64
     pos2 = grow_chains([(nb2, nc2, L2)], 6)[0]
@@ -375,7 +414,7 @@ def learned_leg(replicas=16, train_moves=1_000_000, sweeps=150, burn=30, iters=1
     return out
 
 
-def c1_leg(steps=200, warmup=5):
+def single_chain_leg(steps=200, warmup=5):
     """The reference's own small case: one FENE chain of 100 beads (b_max = 10,
     H = 30), L = 60, shear Wi = 10 (edot = 10/175), dt = 1e-3, no charges, FP64,
     Lanczos tol 1e-6 (<= 30 iterations), mobility settings auto-tuned to 1e-5."""
@@ -385,10 +424,10 @@ def c1_leg(steps=200, warmup=5):
     pos = grow_chains([(100, 1, 60.0)], 8)[0]
     ctx = S.Context(p)
     ctx.upload_positions(pos[:, 0], pos[:, 1], pos[:, 2])
-    ctx.run(0, warmup, use_graph=True)
+    ctx.run(0, warmup, use_graph=USE_GRAPH)
     ctx.sync()
     t0 = time.perf_counter()
-    code = ctx.run(warmup, steps, use_graph=True)
+    code = ctx.run(warmup, steps, use_graph=USE_GRAPH)
     ctx.sync()
     el = time.perf_counter() - t0
     it, stt, ch = ctx.lanczos_status()
@@ -399,7 +438,7 @@ def c1_leg(steps=200, warmup=5):
             "grid": list(cfg["grid"]), "rc": cfg["rc"], "ewald_alpha": round(cfg["alpha"], 5), "return_code": code}
 
 
-def c3_leg(steps=5, warmup=2):
+def extension_leg(steps=5, warmup=2):
     """Planar extensional flow with Kraynik-Reinelt remaps: 200 chains x 250 beads
     (50000 beads), L = 160, Wi = 20 (edot = 20/175), 30 % charged (+-1, zero net),
     Debye length 3, dt = 2e-4, mixed precision, Lanczos tol 1e-6."""
@@ -419,12 +458,12 @@ def c3_leg(steps=5, warmup=2):
     ctx.upload_positions(pos[:, 0], pos[:, 1], pos[:, 2])
     ctx.upload_charges([nq], track, sign)
     cfg = ctx.mobility_config()
-    print(f"[bench] c3: {n} beads, {nq} charges, setup {time.perf_counter() - t0:.1f}s, {cfg}", file=sys.stderr,
+    print(f"[bench] planar extension: {n} beads, {nq} charges, setup {time.perf_counter() - t0:.1f}s, {cfg}", file=sys.stderr,
           flush=True)
-    ctx.run(0, warmup, use_graph=True)
+    ctx.run(0, warmup, use_graph=USE_GRAPH)
     ctx.sync()
     t1 = time.perf_counter()
-    code = ctx.run(warmup, steps, use_graph=True)
+    code = ctx.run(warmup, steps, use_graph=USE_GRAPH)
     ctx.sync()
     el = time.perf_counter() - t1
     it, stt, ch = ctx.lanczos_status()
@@ -440,7 +479,7 @@ def c3_leg(steps=5, warmup=2):
            "grid": list(cfg["grid"]), "rc": cfg["rc"], "ewald_alpha": round(cfg["alpha"], 5),
            "finite": bool(np.all(np.isfinite(x)) and np.all(np.isfinite(y)) and np.all(np.isfinite(z))),
            "stage_ms_one_eager_step": stages}
-    print(f"[bench] c3: {out['value']:.4g} bead-steps/s ({out['ms_per_step']:.1f} ms/step)", file=sys.stderr,
+    print(f"[bench] planar extension: {out['value']:.4g} bead-steps/s ({out['ms_per_step']:.1f} ms/step)", file=sys.stderr,
           flush=True)
     return out
 
@@ -493,14 +532,18 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--replicas", type=int, default=512)
     ap.add_argument("--plain-steps", type=int, default=3)
-    ap.add_argument("--c2-steps", type=int, default=30)
+    ap.add_argument("--semidilute-steps", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--workload", choices=["all", "ensemble", "c2", "learned", "c3", "c1"], default="all",
-                    help="ensemble: only the replica ensemble; c2: only the 8000-bead shear line")
+    ap.add_argument("--workload", choices=["all", "ensemble", "semidilute", "hop", "learned", "extension", "single-chain"], default="all",
+                    help="ensemble: only the replica ensemble; semidilute: only the 8000-bead shear line; hop: only the "
+                         "charge-hopping line; learned: charge hopping with the strongly coupled study")
+    ap.add_argument("--eager", action="store_true", help="run the timed steps eagerly instead of as hipGraphs")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 256 MiB device-to-device three times (HBM counter calibration)")
     args = ap.parse_args()
+    global USE_GRAPH
+    USE_GRAPH = not args.eager
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "replicas shard one per GPU by design but no multi-GPU path is built", file=sys.stderr)
@@ -515,24 +558,31 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes": a.numel() * 4, "copies": 3}))
         return
-    if args.workload == "c1":
-        print(json.dumps({"c1": c1_leg()}))
+    if args.workload == "single-chain":
+        print(json.dumps({"single_chain": single_chain_leg()}))
         return
-    if args.workload == "c3":
-        print(json.dumps({"c3": c3_leg()}))
+    if args.workload == "extension":
+        print(json.dumps({"planar_extension": extension_leg()}))
         return
     if args.workload == "learned":
         print(json.dumps({"learned": learned_leg()}, default=float))
         return
-    if args.workload == "c2":
-        print(json.dumps({"semidilute": semidilute_leg(args.c2_steps, 3, 0 if args.no_plain else 3)}))
+    if args.workload == "hop":
+        print(json.dumps({"charge_hopping": learned_leg(replicas=16, train_moves=300_000, sweeps=100, burn=20,
+                                                        iters=600, strong=False)}, default=float))
+        return
+    if args.workload == "semidilute":
+        print(json.dumps({"semidilute": semidilute_leg(args.semidilute_steps, 3, 0 if args.no_plain else 3)}))
         return
 
     ens = ensemble_leg(args.replicas, args.steps, args.warmup, 0 if args.no_plain else args.plain_steps)
     specs, pos = ens.pop("specs"), ens.pop("pos")
     semi = None
     if args.workload == "all":
-        semi = semidilute_leg(args.c2_steps, 3, 0 if args.no_plain else 3)
+        semi = semidilute_leg(args.semidilute_steps, 3, 0 if args.no_plain else 3)
+    hop = None
+    if args.workload == "all":
+        hop = learned_leg(replicas=16, train_moves=300_000, sweeps=100, burn=20, iters=600, strong=False)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(specs, pos)
@@ -555,8 +605,9 @@ def main():
         "roofline": ens["roofline"],
         "cpu_baseline": cpu,
         "semidilute": semi,
+        "charge_hopping": hop,
     }
-    print(json.dumps(out))
+    print(json.dumps(out, default=float))
 
 
 if __name__ == "__main__":

@@ -290,7 +290,7 @@ struct smdlt_ctx {
   int *qtrack = nullptr, *qsign = nullptr, *qold = nullptr, *qidx = nullptr, *mc_err = nullptr;
   unsigned char* occ = nullptr;
   int *mc_start = nullptr, *mc_jf = nullptr;  // reference sweep: start beads, j* per charge
-  double* mc_de = nullptr;                     // reference sweep: candidate energy changes [nq_total][3]
+  double* mc_de = nullptr;  // reference sweep: candidate energy changes [nq_total][3]; smdlt_mc_energy: row sums
   void* mc_state = nullptr;                    // reference sweep: per-replica cursor, draws, E - E_initial
   int *mh_cnt = nullptr, *mh_off = nullptr;    // MH sweeps: neighbourhood list counts and offsets [nst + 1]
   int* mh_list = nullptr;                      // MH sweeps: neighbourhood lists
@@ -2726,8 +2726,8 @@ int smdlt_mc_energy(smdlt_ctx* c, double* energy) {
   double* de;
   HCK(hipMalloc(&de, 8 * c->nrep));
   HCK(hipMemset(de, 0, 8 * c->nrep));
-  launch_mc_energy(c->stream, c->d_reps, c->cur_rep_begin, c->cur_rep_end - c->cur_rep_begin, c->rx, c->ry, c->rz,
-                   c->qtrack, c->qsign, de);
+  launch_mc_energy(c->stream, c->d_reps, c->cur_rep_begin, c->cur_rep_end - c->cur_rep_begin, c->max_nq, c->rx, c->ry,
+                   c->rz, c->qtrack, c->qsign, c->mc_de, de);
   HCK(hipStreamSynchronize(c->stream));
   HCK(hipMemcpy(energy, de, 8 * c->nrep, hipMemcpyDeviceToHost));
   hipFree(de);

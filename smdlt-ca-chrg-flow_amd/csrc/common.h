@@ -448,5 +448,7 @@ void launch_mc_sweep(hipStream_t st, const RepDev* reps, int rep_begin, int nrep
                      long long occ_stride, int* start, int* jfirst, double* de0, void* state, const float* uniforms,
                      int nu, const uint64_t* seeds, uint64_t sweep_id, int* consumed, int* log, const int* log_off,
                      int* err, double* qdisp);
-void launch_mc_energy(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, const double* rx, const double* ry,
-                      const double* rz, const int* track, const int* sign, double* e);
+// rows: nq_total doubles of working space
+void launch_mc_energy(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, int max_nq, const double* rx,
+                      const double* ry, const double* rz, const int* track, const int* sign, double* rows,
+                      double* e);

@@ -55,20 +55,6 @@ __device__ __forceinline__ double pair_energy(const RepDev& R, const double* rx,
   return (R.lambda_b * (double)(qa * qb) / r) * exp(-R.kappa_d * r);
 }
 
-// energy of all charge pairs of the replica (the double loop of chargeHop,
-// S/main26.c:1955-1970, over charged beads only)
-__device__ double charge_energy(const RepDev& R, const double* rx, const double* ry, const double* rz,
-                                const int* track, const int* sign, double* sh) {
-  double e = 0.0;
-  const int nq = R.nq;
-  for (int a = 0; a < nq - 1; ++a) {
-    int as = R.slot0 + track[R.q0 + a], qa = sign[R.q0 + a];
-    for (int b = a + 1 + threadIdx.x; b < nq; b += blockDim.x)
-      e += pair_energy(R, rx, ry, rz, as, R.slot0 + track[R.q0 + b], qa, sign[R.q0 + b]);
-  }
-  return block_sum(e, sh);
-}
-
 __device__ __forceinline__ float dev_uniform(uint64_t seed, uint64_t sweep, uint32_t draw) {
   uint32_t c[4] = {(uint32_t)sweep, (uint32_t)(sweep >> 32), draw, 0x4D43u};
   philox10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
@@ -452,20 +438,50 @@ void launch_mc_sweep(hipStream_t st, const RepDev* reps, int rep_begin, int nrep
   }
 }
 
-__global__ __launch_bounds__(MC_THREADS) void k_mc_energy(const RepDev* __restrict__ reps, int rep_begin,
-                                                          const double* rx, const double* ry, const double* rz,
-                                                          const int* track, const int* sign, double* e) {
-  int r = rep_begin + blockIdx.x;
+// Energy of all charge pairs of each replica (the double loop of chargeHop,
+// S/main26.c:1955-1970, over the charged beads): one wavefront per charge a
+// sums its pairs with b > a, then one workgroup per replica adds the rows in
+// order.
+__global__ __launch_bounds__(MC_THREADS) void k_mc_energy_rows(const RepDev* __restrict__ reps, int rep_begin,
+                                                               const double* __restrict__ rx,
+                                                               const double* __restrict__ ry,
+                                                               const double* __restrict__ rz,
+                                                               const int* __restrict__ track,
+                                                               const int* __restrict__ sign, double* rows) {
+  const int r = rep_begin + blockIdx.y;
+  const RepDev& R = reps[r];
+  const int lane = threadIdx.x & 63;
+  const int a = blockIdx.x * (MC_THREADS / 64) + (threadIdx.x >> 6);
+  if (!R.active || a >= R.nq) return;
+  const int as = R.slot0 + track[R.q0 + a], qa = sign[R.q0 + a];
+  double e = 0.0;
+  for (int b = a + 1 + lane; b < R.nq; b += 64)
+    e += pair_energy(R, rx, ry, rz, as, R.slot0 + track[R.q0 + b], qa, sign[R.q0 + b]);
+  e = warp_sum(e);
+  if (lane == 0) rows[R.q0 + a] = e;
+}
+
+__global__ __launch_bounds__(MC_THREADS) void k_mc_energy_sum(const RepDev* __restrict__ reps, int rep_begin,
+                                                              const double* __restrict__ rows, double* e) {
+  const int r = rep_begin + blockIdx.x;
   const RepDev& R = reps[r];
   if (!R.active) return;
   __shared__ double sh[MC_THREADS / 64];
-  double v = (R.nq > 1) ? charge_energy(R, rx, ry, rz, track, sign, sh) : 0.0;
+  double v = 0.0;
+  for (int a = threadIdx.x; a < R.nq; a += blockDim.x) v += rows[R.q0 + a];
+  v = block_sum(v, sh);
   if (threadIdx.x == 0) e[r] = v;
 }
 
-void launch_mc_energy(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, const double* rx, const double* ry,
-                      const double* rz, const int* track, const int* sign, double* e) {
-  if (nrep > 0) k_mc_energy<<<nrep, MC_THREADS, 0, st>>>(reps, rep_begin, rx, ry, rz, track, sign, e);
+void launch_mc_energy(hipStream_t st, const RepDev* reps, int rep_begin, int nrep, int max_nq, const double* rx,
+                      const double* ry, const double* rz, const int* track, const int* sign, double* rows,
+                      double* e) {
+  if (nrep <= 0) return;
+  if (max_nq > 0) {
+    const dim3 g((max_nq + MC_THREADS / 64 - 1) / (MC_THREADS / 64), nrep);
+    k_mc_energy_rows<<<g, MC_THREADS, 0, st>>>(reps, rep_begin, rx, ry, rz, track, sign, rows);
+  }
+  k_mc_energy_sum<<<nrep, MC_THREADS, 0, st>>>(reps, rep_begin, rows, e);
 }
 
 // ============================================================================
