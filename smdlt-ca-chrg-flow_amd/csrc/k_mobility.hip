@@ -361,8 +361,10 @@ __global__ void k_scale_v(MobArgs a) {
 // thread adds the ones whose stencil covers its point.  Each wavefront is one z
 // plane (8 x 8 points) and lists, per 4 x 4 quadrant of its lanes, the staged
 // beads that reach the quadrant; a lane walks its quadrant's list, which holds
-// about half the beads that reach the plane.  A point adds exactly the beads
-// covering it, in staging order.  No atomics: the grid needs no zeroing and
+// about half the beads that reach the plane.  While listing, the wavefront
+// stores each reaching bead's vector times its z weight at the plane and its
+// packed x and y bases, so a lane's step reads that record and two weights.
+// A point adds exactly the beads covering it, in staging order.  No atomics: the grid needs no zeroing and
 // the result does not depend on scheduling.
 #define GATHER_CHUNK 128
 template <typename T, int P>
@@ -371,6 +373,10 @@ __global__ __launch_bounds__(512) void k_spread_gather(MobArgs a) {
   __shared__ T sw[GATHER_CHUNK * 3 * P];
   __shared__ T sv[GATHER_CHUNK * 3];
   __shared__ unsigned char plist[BRICK][4][GATHER_CHUNK];  // per z plane and quadrant: staged beads reaching it
+  // per z plane and staged bead: the vector times the bead's z weight at the
+  // plane, and its x and y bases (packed), written by the plane's wavefront
+  __shared__ T zv[BRICK][GATHER_CHUNK][3];
+  __shared__ int zb[BRICK][GATHER_CHUNK];
   const BrickTask bt = a.bricks[blockIdx.x];
   const RepDev& R = a.reps[bt.rep];
   if (!R.active || (a.done && a.done[bt.rep])) return;
@@ -453,6 +459,13 @@ __global__ __launch_bounds__(512) void k_spread_gather(MobArgs a) {
         c1 = cz && x1 && y0;
         c2 = cz && x0 && y1;
         c3 = cz && x1 && y1;
+        if (cz) {
+          const T wzk = sw[(3 * k + 2) * P + (sb[3 * k + 2] - wz)];
+          zv[wz][k][0] = wzk * sv[3 * k];
+          zv[wz][k][1] = wzk * sv[3 * k + 1];
+          zv[wz][k][2] = wzk * sv[3 * k + 2];
+          zb[wz][k] = e0 | (e1 << 16);
+        }
       }
       const unsigned long long b0 = __ballot(c0), b1 = __ballot(c1), b2 = __ballot(c2), b3 = __ballot(c3);
       if (c0) plist[wz][0][n0 + __popcll(b0 & below)] = (unsigned char)k;
@@ -472,12 +485,13 @@ __global__ __launch_bounds__(512) void k_spread_gather(MobArgs a) {
     const unsigned char* pl = plist[wz][qd];
     for (int q = 0; q < mine; ++q) {
       const int k = pl[q];
-      const int d0 = sb[3 * k] - gr0, d1 = sb[3 * k + 1] - gr1, d2 = sb[3 * k + 2] - wz;
+      const int e = zb[wz][k];
+      const int d0 = (e & 0xffff) - gr0, d1 = (e >> 16) - gr1;
       if ((unsigned)d0 < (unsigned)P && (unsigned)d1 < (unsigned)P) {
-        T w = sw[(3 * k) * P + d0] * sw[(3 * k + 1) * P + d1] * sw[(3 * k + 2) * P + d2];
-        acc0 += w * sv[3 * k];
-        acc1 += w * sv[3 * k + 1];
-        acc2 += w * sv[3 * k + 2];
+        const T w = sw[(3 * k) * P + d0] * sw[(3 * k + 1) * P + d1];
+        acc0 += w * zv[wz][k][0];
+        acc1 += w * zv[wz][k][1];
+        acc2 += w * zv[wz][k][2];
       }
     }
   }
@@ -1040,9 +1054,11 @@ __global__ __launch_bounds__(64 * CL_WAVES) __attribute__((amdgpu_waves_per_eu(C
   }
   if constexpr (TAB_LDS) __syncthreads();
   // real-space part by cluster pairs.  The wave takes its list entries in groups
-  // of CL_UNROLL consecutive ones (read as scalars) and, every 64 entries, its
-  // lanes compute one entry's image shift each: the minimum image of the vector
-  // between the two clusters' first beads.  Positions are cluster-consistent, so
+  // of CL_UNROLL consecutive ones and, every 64 entries, its lanes load one
+  // entry each and compute its image shift: the minimum image of the vector
+  // between the two clusters' first beads.  The pair loop (wave-uniform: it
+  // starts from readfirstlane) broadcasts an entry and its shift from the lane
+  // holding them with readlane, so it issues no list loads.  Positions are cluster-consistent, so
   // when that vector's length plus both cluster radii is below half the shortest
   // lattice vector every bead pair of the two clusters has that shift as its
   // minimum image (any shorter image would need a lattice vector shorter than
@@ -1061,19 +1077,21 @@ __global__ __launch_bounds__(64 * CL_WAVES) __attribute__((amdgpu_waves_per_eu(C
   const int m = a.nclp[I];
   const int* row = a.lcl + (size_t)I * a.capc;
   const int ntab1 = R.ntab - 1;
-  __shared__ float4 sshift[CL_WAVES][64];
   const __amdgpu_buffer_rsrc_t rp32 = __builtin_amdgcn_make_buffer_rsrc((void*)a.p32, (short)0, a.nst * 16, 0x00020000);
   const __amdgpu_buffer_rsrc_t rv32 = __builtin_amdgcn_make_buffer_rsrc((void*)a.v32, (short)0, a.nst * 16, 0x00020000);
   const int wu = __builtin_amdgcn_readfirstlane(w);
   constexpr int G = CL_UNROLL;
   static_assert(64 % G == 0, "a wave's 64 precomputed shifts cover whole groups of CL_UNROLL entries");
   for (int c0 = wu * G; c0 < m; c0 += 64 * CL_WAVES) {
-    // lane l: entry c0 + CL_WAVES * G * (l / G) + l % G of this wave's next 64
+    // lane l: entry c0 + CL_WAVES * G * (l / G) + l % G of this wave's next 64,
+    // kept in the lane's registers and read by the pair loop with readlane
+    int e_lane = 0;  // past the row: no beads
+    float sh_x = 0.f, sh_y = 0.f, sh_z = 0.f, sh_w = 1.f;  // no pairs, no veto
     {
       const int e_idx = c0 + CL_WAVES * G * (lane / G) + lane % G;
-      float4 sh = make_float4(0.f, 0.f, 0.f, 1.f);  // past the row: no pairs, no veto
       if (e_idx < m) {
-        const float4 pr = a.p32[row[e_idx] >> 4];
+        e_lane = row[e_idx];
+        const float4 pr = a.p32[e_lane >> 4];
         float cx = pr.x - pc.x, cy = pr.y - pc.y, cz = pr.z - pc.z;
         const float n1 = rintf(rd0 * cx + rd1 * cy), n2 = rintf(rd2 * cx + rd3 * cy), n3 = rintf(cz * iLz);
         const float sx = -(n1 * rb0 + n2 * rb2), sy = -(n1 * rb1 + n2 * rb3), sz = -Lz * n3;
@@ -1081,13 +1099,13 @@ __global__ __launch_bounds__(64 * CL_WAVES) __attribute__((amdgpu_waves_per_eu(C
         cy += sy;
         cz += sz;
         const float reach = __builtin_amdgcn_sqrtf(cx * cx + cy * cy + cz * cz) + pr.w + pc.w;
-        sh = make_float4(sx, sy, sz, reach < halfw ? 1.f : 0.f);
+        sh_x = sx;
+        sh_y = sy;
+        sh_z = sz;
+        sh_w = reach < halfw ? 1.f : 0.f;
       }
-      sshift[w][lane] = sh;
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
+    const int flag_lane = sh_w != 0.f ? 1 : 0;
     const int c1 = min(m, c0 + 64 * CL_WAVES);
     // the chunk's (<= 64) terms of each lane are summed plainly in FP32, then
     // added to the lane's compensated sum
@@ -1100,15 +1118,17 @@ __global__ __launch_bounds__(64 * CL_WAVES) __attribute__((amdgpu_waves_per_eu(C
       int cflag = 1;
 #pragma unroll
       for (int u = 0; u < G; ++u) {
-        const int k = k0 + u;
-        const int e = row[k];  // the list is padded, so a group past the row's end still reads in bounds
-        const int fj = e >> 4, cj = k < m ? (e & 15) : 0;
+        const int e = __builtin_amdgcn_readlane(e_lane, q + u);  // 0 past the row's end
+        const int fj = e >> 4, cj = e & 15;
         ok[u] = ivalid && jj < cj && !(fj == first && jj == ii);
         sj[u] = fj + (jj < cj ? jj : 0);
-        S[u] = sshift[w][q + u];
-        cflag &= S[u].w != 0.f ? 1 : 0;
+        S[u] = make_float4(__builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, sh_x), q + u)),
+                           __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, sh_y), q + u)),
+                           __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, sh_z), q + u)),
+                           0.f);
+        cflag &= __builtin_amdgcn_readlane(flag_lane, q + u);
       }
-      const bool compact = __builtin_amdgcn_readfirstlane(cflag) != 0;
+      const bool compact = cflag != 0;
 #pragma unroll
       for (int u = 0; u < G; ++u) {  // 32-bit offsets into the position and vector copies
         const auto lp = __builtin_amdgcn_raw_buffer_load_b96(rp32, sj[u] * 16, 0, 0);
