@@ -40,6 +40,8 @@ from smdlt_workloads import (c2_charges, c2_params, c4_params, c4_specs, chains,
                              grow_chains_grouped, to_lab_frame)
 
 HBM_PEAK_GBS = 8000.0
+# VALU issue: 256 CUs x 4 SIMDs, each SIMD taking 4 cycles per wave64 FP32 instruction, at 2.4 GHz
+VALU_SIMDS, VALU_CLOCK_HZ, VALU_CYCLES_PER_INST = 1024, 2.4e9, 4
 USE_GRAPH = True  # --eager: steps run eagerly (for tracing tools) instead of replaying hipGraphs
 
 
@@ -93,9 +95,10 @@ def ensemble_leg(nrep, steps, warmup, plain_steps):
     sym, traffic = pmc_traffic("ensemble", stage_kernels(dom, cfgs[0]["order"], bool(cfgs[0]["cluster_pairs"])))
     roof = {"bound": "hbm", "kernel": sym or dom, "stage": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": ach / HBM_PEAK_GBS, "traffic": traffic, "bytes_per_launch": kb[dom], "avg_launch_ms": avg_ms,
-            "grid_points": G, "list_entries": ent,
+            "grid_points": G, "list_entries": ent, "valu_issue_frac": pmc_valu_frac("ensemble", sym, avg_ms),
             "note": "dominant kernel of one eager step (HIP events on the context stream); achieved = algorithmic "
-                    "bytes per launch / average launch time; traffic = PMC FETCH+WRITE bytes per dispatch"}
+                    "bytes per launch / average launch time; traffic = PMC FETCH+WRITE bytes per dispatch; "
+                    "valu_issue_frac = PMC SQ_INSTS_VALU per dispatch x 4 cycles / (launch time x 1024 SIMDs x 2.4 GHz)"}
     kernel_ms = {k: v[0] / v[1] for k, v in kern.items()}
     ctx.close()
     out = {"value": value, "ms_per_step": 1e3 * el / steps, "beads": ntot, "replicas": nrep,
@@ -184,7 +187,8 @@ def semidilute_leg(steps, warmup, plain_steps):
            "kernel_ms_avg": {k: v[0] / v[1] for k, v in kern.items()},
            "roofline": {"bound": "hbm", "kernel": sym or dom, "stage": dom, "achieved": achieved,
                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                        "bytes_per_launch": kb[dom], "avg_launch_ms": avg_ms, "list_entries": nbr}}
+                        "bytes_per_launch": kb[dom], "avg_launch_ms": avg_ms, "list_entries": nbr,
+                        "valu_issue_frac": pmc_valu_frac("semidilute", sym, avg_ms)}}
     if plain_steps > 0:
         c64 = S.Context(c2_params(S.PREC_FP64))
         c64.upload_positions(pos[:, 0], pos[:, 1], pos[:, 2])
@@ -508,6 +512,21 @@ def stage_kernels(stage, order, cluster):
     if stage == "k_mob_final" and cluster:
         return [f"k_mob_cl<{order}, "]
     return [stage, stage.split("(")[0]]
+
+
+def pmc_valu_frac(section, sym, avg_ms):
+    """Fraction of the VALU issue peak the kernel used: its SQ_INSTS_VALU per
+    dispatch from the committed counter summary, 4 cycles per instruction, over
+    the 1024 SIMDs for the measured average launch time; None when absent."""
+    tf = os.path.join(ROOT, "profiles", "hbm_traffic_pmc.json")
+    if not sym or not os.path.exists(tf):
+        return None
+    with open(tf) as fh:
+        d = json.load(fh).get(section + "_valu_insts", {})
+    n = d.get(sym)
+    if n is None or avg_ms <= 0:
+        return None
+    return n * VALU_CYCLES_PER_INST / (avg_ms * 1e-3 * VALU_SIMDS * VALU_CLOCK_HZ)
 
 
 def pmc_traffic(section, names):
