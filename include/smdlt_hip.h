@@ -265,7 +265,10 @@ int smdlt_mc_sweep(smdlt_ctx* ctx, int mode, const float* uniforms, int n_unifor
  *   + 4 n_uniforms nrep [host-fed uniforms only] <= SMDLT_MC_NOALLOC_MAX_BYTES
  * (align256 rounds up to a multiple of 256), else SMDLT_E_ARG.  All contexts
  * on one GPU share the array: calls that overlap in time on one GPU (from
- * several host threads) race; calls in sequence do not. */
+ * several host threads) race; calls in sequence do not.  The MH modes' per-bead
+ * neighbourhood lists live in the context with the capacity an earlier
+ * smdlt_mc_sweep in an MH mode reserved; lists that would outgrow it (or no
+ * such earlier sweep) give SMDLT_E_STATE, nothing is allocated. */
 #define SMDLT_MC_NOALLOC_MAX_BYTES (8 << 20)
 int smdlt_mc_sweep_noalloc(smdlt_ctx* ctx, int mode, const float* uniforms, int n_uniforms, int* consumed,
                            int* log);

@@ -2517,6 +2517,18 @@ int smdlt_upload_charges(smdlt_ctx* c, const int* nq, const int* track, const in
   st |= alloc_dev(&c->mc_jf, std::max(1, tot));
   st |= alloc_dev(&c->mc_de, 3 * (size_t)std::max(1, tot));
   if (!st && hipMalloc(&c->mc_state, mc_state_bytes() * c->nrep) != hipSuccess) st = 1;
+  if (!c->mh_cnt && !st) {  // the MH sweeps' per-bead and per-replica arrays (sized by the context)
+    const long long ns1 = (long long)c->nst + 1;
+    st |= alloc_dev(&c->mh_cnt, ns1);
+    st |= alloc_dev(&c->mh_off, ns1);
+    st |= alloc_dev(&c->mh_hcnt, ns1);
+    st |= alloc_dev(&c->mh_hoff, ns1);
+    st |= alloc_dev(&c->mh_V, (size_t)std::max(1, c->nst));
+    st |= alloc_dev(&c->mh_nmoves, c->nrep);
+    st |= alloc_dev(&c->mh_done, c->nrep);
+    st |= alloc_dev(&c->mh_stop, c->nrep);
+    if (!st && hipMalloc(&c->mh_moves, 16 * 64 * (size_t)c->nrep) != hipSuccess) st = 1;
+  }
   if (st) return SMDLT_E_NOMEM;
   std::vector<int> sg(tot, 1);
   if (sign) sg.assign(sign, sign + tot);
@@ -2552,21 +2564,11 @@ int smdlt_download_charges(smdlt_ctx* c, int* track, int* old) {
 // context's own buffers.
 // The neighbourhood lists of the MH sweeps for the active replicas at the
 // current positions: counts, offsets by an exclusive scan, then the lists.
-static int mh_build_lists(smdlt_ctx* c, int rb, int re, int max_n) {
+// may_grow = false (the allocation-free sweep): lists that outgrow the capacity
+// an earlier sweep reserved are an error, not a reallocation.
+static int mh_build_lists(smdlt_ctx* c, int rb, int re, int max_n, bool may_grow) {
   const long long ns1 = (long long)c->nst + 1;
-  if (!c->mh_cnt) {
-    int st = 0;
-    st |= alloc_dev(&c->mh_cnt, ns1);
-    st |= alloc_dev(&c->mh_off, ns1);
-    st |= alloc_dev(&c->mh_hcnt, ns1);
-    st |= alloc_dev(&c->mh_hoff, ns1);
-    st |= alloc_dev(&c->mh_V, (size_t)std::max(1, c->nst));
-    st |= alloc_dev(&c->mh_nmoves, c->nrep);
-    st |= alloc_dev(&c->mh_done, c->nrep);
-    st |= alloc_dev(&c->mh_stop, c->nrep);
-    if (!st && hipMalloc(&c->mh_moves, 16 * 64 * (size_t)c->nrep) != hipSuccess) st = 1;
-    if (st) return SMDLT_E_NOMEM;
-  }
+  if (!c->mh_cnt) return SMDLT_E_STATE;
   launch_fill_i32(c->stream, c->mh_cnt, 0, ns1);
   launch_fill_i32(c->stream, c->mh_hcnt, 0, ns1);
   launch_mh_lists(c->stream, c->d_reps, rb, re - rb, max_n, c->rx, c->ry, c->rz, c->mh_cnt, nullptr, nullptr,
@@ -2582,6 +2584,7 @@ static int mh_build_lists(smdlt_ctx* c, int rb, int re, int max_n) {
   long long* caps[2] = {&c->mh_list_cap, &c->mh_hlist_cap};
   for (int l = 0; l < 2; ++l) {
     if (total[l] <= *caps[l]) continue;
+    if (!may_grow) return SMDLT_E_STATE;
     if (*lists[l]) hipFree(*lists[l]);
     *lists[l] = nullptr;
     *caps[l] = 0;
@@ -2595,7 +2598,7 @@ static int mh_build_lists(smdlt_ctx* c, int rb, int re, int max_n) {
 }
 
 static int mc_sweep_internal(smdlt_ctx* c, int mode, const float* du, int n_uniforms, int* dcons, int* dlog,
-                             int* consumed, int* log) {
+                             int* consumed, int* log, bool may_alloc) {
   const bool mh = mode != SMDLT_MC_REFERENCE;
   if (mh) dlog = c->mh_log;
   HCK(hipMemsetAsync(c->mc_err, 0, 4, c->stream));
@@ -2607,7 +2610,7 @@ static int mc_sweep_internal(smdlt_ctx* c, int mode, const float* du, int n_unif
       max_nq = std::max(max_nq, c->nq[r]);
     }
     if (max_nq > 0) {
-      int st = mh_build_lists(c, rb, re, max_n);
+      int st = mh_build_lists(c, rb, re, max_n, may_alloc);
       if (st) return st;
     }
     launch_mh_sweep(c->stream, c->d_reps, rb, re - rb, max_n, max_nq, c->rx, c->ry, c->rz, c->qtrack, c->qold,
@@ -2660,7 +2663,7 @@ int smdlt_mc_sweep(smdlt_ctx* c, int mode, const float* uniforms, int n_uniforms
   }
   HCK(hipMalloc(&dcons, 4 * c->nrep));
   if (mode == SMDLT_MC_REFERENCE && log && c->nq_total) HCK(hipMalloc(&dlog, 24 * (size_t)c->nq_total));
-  st = mc_sweep_internal(c, mode, du, n_uniforms, dcons, dlog, consumed, log);
+  st = mc_sweep_internal(c, mode, du, n_uniforms, dcons, dlog, consumed, log, true);
   if (du) hipFree(du);
   hipFree(dcons);
   if (dlog) hipFree(dlog);
@@ -2714,7 +2717,7 @@ int smdlt_mc_sweep_noalloc(smdlt_ctx* c, int mode, const float* uniforms, int n_
   HCK(hipMemsetAsync(dcons, 0, b_cons, c->stream));
   if (dlog) HCK(hipMemsetAsync(dlog, 0, b_log, c->stream));
   if (du) HCK(hipMemcpyAsync(du, uniforms, b_u, hipMemcpyHostToDevice, c->stream));
-  return mc_sweep_internal(c, mode, du, n_uniforms, dcons, dlog, consumed, log);
+  return mc_sweep_internal(c, mode, du, n_uniforms, dcons, dlog, consumed, log, false);
 }
 
 int smdlt_mc_energy(smdlt_ctx* c, double* energy) {

@@ -44,6 +44,14 @@ def _same(a, b, mode, u):
 def test_noalloc_sweep_matches_allocating_sweep(mode):
     a, b = _pair(mode)
     rs = S.RefStream(830062920)
+    if mode != S.MC_REFERENCE:
+        # the MH lists' capacity comes from an allocating sweep: without one the
+        # allocation-free call refuses; then one identical sweep on both
+        with pytest.raises(S.SmdltError):
+            b.mc_sweep(mode, rs.uniforms(3 * 50)[None, :], noalloc=True)
+        u0 = rs.uniforms(3 * 50)[None, :]
+        a.mc_sweep(mode, u0)
+        b.mc_sweep(mode, u0)
     u1 = rs.uniforms(4 * 50)[None, :]
     cons = _same(a, b, mode, u1)
     assert cons[0] > 0
