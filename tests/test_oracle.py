@@ -721,3 +721,30 @@ def test_chargehop_branches_hand_worked(case):
     assert mc.track[0] == (expect[2] if moved else track[0])
     if expect[2] >= 0 and not moved:
         assert mc.old[0] == track[0]
+
+
+def test_bending_stress_closed_forms():
+    """The bending terms of the stress accumulators (S/main26.c:606-613) at two
+    geometries worked by hand.  Bead j has getNID vectors u = r_j - r_{j-1} and
+    w = r_j - r_{j+1}; cos = u.w/(|u||w|), theta = pi - acos(cos) and
+    coeff = -kappa_b theta / sqrt(1 - cos^2).
+    (1) u = (a,0,0), w = (0,b,0): cos = 0, theta = pi/2; of the four tauxx terms
+        only -u_x coeff u_x/(|u||w|) survives: tauxx = kappa_b pi a / (2 b); every
+        tauyy term carries u_y = 0 or w_z = 0 or cos = 0: tauyy = 0.
+    (2) u = (0,1,2), w = (0,0,1): |u| = sqrt5, |w| = 1, cos = 2/sqrt5; the tauyy
+        terms give coeff [2/(5 sqrt5) - 2 (1/sqrt5 - 2/(5 sqrt5))] = -4 coeff/(5 sqrt5),
+        so tauyy = 4 kappa_b theta / 5; its second term reads u_z, where a y-y
+        version would give coeff (2 - 3)/(5 sqrt5) instead.
+    The springs are Hookean with kappas = 0, so only bending contributes."""
+    L, kb = 40.0, 1.7
+    lat = O.lattice(0, L, 0.0, 1e-3)
+    c = np.array([1.3, -0.7, 0.4])
+    a, b = 1.5, 2.5
+    for (u, w), expect in (
+            (((a, 0.0, 0.0), (0.0, b, 0.0)), (kb * np.pi * a / (2 * b), 0.0, 0.0)),
+            (((0.0, 1.0, 2.0), (0.0, 0.0, 1.0)), (0.0, 4 * kb * (np.pi - np.arccos(2 / np.sqrt(5))) / 5, 0.0))):
+        p = np.array([c - np.array(u), c, c - np.array(w)])
+        rx, ry, rz = (np.ascontiguousarray(p[:, k]) for k in range(3))
+        st, f, tau = O.bond_force(lat, 3, 1, rx, ry, rz, 0, 0.0, 2.0, kb)
+        assert st == 0
+        assert np.allclose(tau, expect, rtol=1e-12, atol=1e-12), (tau, expect)
