@@ -688,16 +688,31 @@ static void rpy_real_host(double r, double alpha, double& t1, double& t2) {
   t2 = C3 * ec + C4 * g;
 }
 
-// Cubic coefficients per interval [R0 + i/RTAB_INV, R0 + (i+1)/RTAB_INV) of t1(r)
-// and t2(r)/r^2, interpolating at the 4 Chebyshev nodes of the interval (FP64
-// solve).  Below r = 2 the exact RPY overlap form replaces the far form, written
-// with erf so the 1/r^3 terms cancel analytically; r = 2 is an interval knot.
+// Cubic coefficients per table segment (rtab_locate: r^2 from the float bits
+// (RTAB_BASE + i) << RTAB_SHIFT to those of i + 1, position s in [1, 2) linear in
+// r^2) of t1(r) and t2(r)/r^2, interpolating at the 4 Chebyshev nodes of the
+// segment (FP64 solve).  Below r = 2 the exact RPY overlap form replaces the far
+// form, written with erf so the 1/r^3 terms cancel analytically; r^2 = 4 is an
+// octave start and so a segment knot.
+static double rtab_knot(int i) {
+  const uint32_t b = (uint32_t)(RTAB_BASE + (unsigned)i) << RTAB_SHIFT;
+  float f;
+  memcpy(&f, &b, 4);
+  return (double)f;
+}
+
+static int rtab_segments(double rc2) {
+  const float rcf = (float)rc2;  // the kernels' cut-off test
+  uint32_t b;
+  memcpy(&b, &rcf, 4);
+  return std::max(1, (int)((b >> RTAB_SHIFT) - RTAB_BASE) + 1) + 1;  // + the zero entry
+}
+
 static void build_rtab(double alpha, int ntab, float* out) {
-  const double h = 1.0 / RTAB_INV;
   double tn[4], V[4][4];
   for (int k = 0; k < 4; ++k) {
     tn[k] = 0.5 * (1.0 - cos((2 * k + 1) * M_PI / 8.0));
-    for (int p = 0; p < 4; ++p) V[k][p] = pow(tn[k], p);
+    for (int p = 0; p < 4; ++p) V[k][p] = pow(1.0 + tn[k], p);
   }
   // invert the 4x4 Vandermonde once (Gauss-Jordan)
   double A[4][8];
@@ -716,10 +731,12 @@ static void build_rtab(double alpha, int ntab, float* out) {
         for (int p = 0; p < 8; ++p) A[k][p] -= f * A[col][p];
       }
   }
-  for (int i = 0; i < ntab; ++i) {
+  for (int p = 0; p < 8; ++p) out[8 * (ntab - 1) + p] = 0.f;
+  for (int i = 0; i < ntab - 1; ++i) {
     double f1[4], f2[4];
     for (int k = 0; k < 4; ++k) {
-      double r = (double)RTAB_R0 + (i + tn[k]) * h, t1, t2;
+      const double lo = rtab_knot(i), hi = rtab_knot(i + 1);
+      double r = sqrt(lo + (hi - lo) * tn[k]), t1, t2;
       if (r >= 2.0) {
         rpy_real_host(r, alpha, t1, t2);
       } else {
@@ -1076,9 +1093,9 @@ int smdlt_create(const smdlt_params* params, int n_replicas, int device, smdlt_c
     st |= alloc_dev(&c->brick_beads, c->nst);
     st |= alloc_dev(&c->bbase, 3 * (size_t)c->nst);
     if (hipMalloc(&c->bw, (size_t)BW_STRIDE * c->nst * esz) != hipSuccess) st |= SMDLT_E_NOMEM;
-    if (esz == 4 && (hipMalloc(&c->p32, 16 * (size_t)c->nst) != hipSuccess ||
-                     hipMalloc(&c->v32, 16 * (size_t)c->nst) != hipSuccess))
-      st |= SMDLT_E_NOMEM;
+    // zeroed: the cluster pass loads (and discards) the slots that follow a
+    // cluster, which must hold finite values
+    if (esz == 4) st |= alloc_dev(&c->p32, c->nst) | alloc_dev(&c->v32, c->nst);
     if (st) {
       smdlt_destroy(c);
       return SMDLT_E_NOMEM;
@@ -1087,7 +1104,7 @@ int smdlt_create(const smdlt_params* params, int n_replicas, int device, smdlt_c
       int tot = 0;
       for (int r = 0; r < n_replicas; ++r) {
         RepDev& R = c->rh[r];
-        R.ntab = std::max(1, (int)ceil((sqrt(R.rc2) - (double)RTAB_R0) * RTAB_INV) + 1);
+        R.ntab = rtab_segments(R.rc2);
         R.tab0 = tot;
         tot += 2 * R.ntab;
       }
@@ -1114,7 +1131,7 @@ int smdlt_create(const smdlt_params* params, int n_replicas, int device, smdlt_c
         R.ncl = (int)cf.size() - R.cl0;
         maxncl = std::max(maxncl, R.ncl);
       }
-      if (c->nst >= (1 << 27)) {  // cluster-list entries pack the first slot in 27 bits
+      if (c->nst > (1 << 27) - 8) {  // cluster-list entries pack the first slot in 27 bits
         smdlt_destroy(c);
         return SMDLT_E_ARG;
       }

@@ -353,9 +353,10 @@ struct MobArgs {
   // the scaled input vector (per product), 16-byte aligned for one load per bead
   float4* p32;
   float4* v32;
-  // mixed precision: per replica, for r in [RTAB_R0, rc) on intervals of 1/RTAB_INV,
-  // cubic coefficients (in t = frac) of t1(r) and of t2(r)/r^2 with the RPY overlap
-  // correction below r = 2 (r = 2 is a knot): rtab[2i], rtab[2i+1]
+  // mixed precision: per replica, for r^2 in [RTAB_R0^2, rc^2) on the segments of
+  // rtab_locate, cubic coefficients (in its s) of t1(r) and of t2(r)/r^2 with the
+  // RPY overlap correction below r = 2 (r^2 = 4 is a knot): rtab[2i], rtab[2i+1];
+  // the last entry is zero
   const float4* rtab;
   int phase;                // bit 0: bead preparation and input scaling; bit 1: spread .. final
   int scale_in_spread;      // the spread stages v / beta itself and stores vout, v32 (set by the launcher)
@@ -369,8 +370,29 @@ struct MobArgs {
 #define RTAB_LDS_MAX 1280   // tables up to this many entries (40 KB) are staged in LDS
 #define CL_LIST_PAD 8       // entries past the last cluster-pair row, so a row's 4-entry groups read in bounds
 #define SCALE_FUSE_MAX_SLOTS 32768  // ranges up to this many slots normalise the input in the spread
-#define RTAB_INV 16
 #define RTAB_R0 1.0f  // pairs closer than this take the FP32 closed form
+// Real-space table segments: r^2 in [1, rc^2) cut into 2^RTAB_MBITS equal
+// segments per octave, so the segment is the float's exponent and top RTAB_MBITS
+// mantissa bits and the position in it is the remaining mantissa under exponent 0
+// (s in [1, 2), linear in r^2).  The spacing in r is r / 2^(RTAB_MBITS + 1): fine
+// near contact where the coefficients vary fastest, and the lookup needs no square
+// root or float conversion.  With RTAB_MBITS 5 the FP32 cubic is within ~1e-7 of
+// the coefficients' largest value for ewald_alpha 0.1 .. 0.9 and rc 5 .. 48.
+#define RTAB_MBITS 5
+#define RTAB_SHIFT (23 - RTAB_MBITS)
+#define RTAB_BASE (0x3f800000u >> RTAB_SHIFT)  // the segment starting at r^2 = RTAB_R0^2 = 1
+// A replica's table holds ntab - 1 segments from r^2 = 1 up to the one holding
+// rc^2, then one entry of zero coefficients.  rtab_top(ntab) is the first r^2 of
+// that zero entry; r^2 clamped to [1, rtab_top] selects an entry (the zero one for
+// pairs that are not summed) and its position s.
+__host__ __device__ __forceinline__ float rtab_top(int ntab) {
+  return __builtin_bit_cast(float, (unsigned)(RTAB_BASE + (unsigned)(ntab - 1)) << RTAB_SHIFT);
+}
+__device__ __forceinline__ void rtab_locate(float rrc, int& it, float& s) {
+  const unsigned b = __float_as_uint(rrc);
+  it = (int)((b >> RTAB_SHIFT) - RTAB_BASE);
+  s = __uint_as_float(((b << RTAB_MBITS) & 0x007fffffu) | 0x3f800000u);
+}
 
 // Overlap-corrected real-space coefficients in FP32 for r < RTAB_R0, written with
 // erf so that the 1/r^3 terms of the Ewald and RPY far forms cancel analytically:

@@ -925,9 +925,9 @@ __global__ __launch_bounds__(256) void k_mob_final(MobArgs a) {
           overlap = true;  // evaluated below in FP64
         } else if (rr < rcf) {
           // t1 and t2/r^2 from the replica's cubic table (FP64-fitted, ~1e-7 relative)
-          const float x = (sqrtf(rr) - RTAB_R0) * (float)RTAB_INV;
-          const int it = min((int)x, R.ntab - 1);
-          const float t = x - (float)it;
+          int it;
+          float t;
+          rtab_locate(rr, it, t);
           const float4 ca = a.rtab[R.tab0 + 2 * it], cb = a.rtab[R.tab0 + 2 * it + 1];
           const float t1 = ca.x + t * (ca.y + t * (ca.z + t * ca.w));
           const float b2 = cb.x + t * (cb.y + t * (cb.z + t * cb.w));
@@ -1021,6 +1021,7 @@ __global__ __launch_bounds__(256) void k_mob_final(MobArgs a) {
 #ifndef CL_WAVES_PER_EU
 #define CL_WAVES_PER_EU 6
 #endif
+typedef __attribute__((address_space(3))) const float4 lds_float4;
 template <int P, bool TAB_LDS>
 __global__ __launch_bounds__(64 * CL_WAVES) __attribute__((amdgpu_waves_per_eu(CL_WAVES_PER_EU))) void k_mob_cl(
     MobArgs a) {
@@ -1062,9 +1063,15 @@ __global__ __launch_bounds__(64 * CL_WAVES) __attribute__((amdgpu_waves_per_eu(C
   // when that vector's length plus both cluster radii is below half the shortest
   // lattice vector every bead pair of the two clusters has that shift as its
   // minimum image (any shorter image would need a lattice vector shorter than
-  // twice the bound), and the lanes skip their own minimum-image reduction.
+  // twice the bound), and the lanes skip their own minimum-image reduction; the
+  // entry's sign bit marks the pairs that may not (first slots fit 27 bits).
+  // Lanes past the partner cluster's end load the slots that follow it (zeros
+  // past the array, through the buffer's range check) and select the table's
+  // zero entry, as do pairs beyond rc and the bead with itself, so the sums take
+  // no selects.
   const int ii = lane >> 3, jj = lane & 7;
   const bool ivalid = ii < cnt;
+  const unsigned long long ivm = cnt >= 8 ? ~0ull : (1ull << (8 * cnt)) - 1ull;  // lanes with ii < cnt
   const int si = first + (ivalid ? ii : 0);
   const float4 pi = a.p32[si];
   const float4 pc = a.p32[first];  // first bead; .w = cluster radius
@@ -1076,7 +1083,8 @@ __global__ __launch_bounds__(64 * CL_WAVES) __attribute__((amdgpu_waves_per_eu(C
   CompSum<T> acc[3];
   const int m = a.nclp[I];
   const int* row = a.lcl + (size_t)I * a.capc;
-  const int ntab1 = R.ntab - 1;
+  const float rtop = rtab_top(R.ntab);
+  const unsigned stab_k = (unsigned)(size_t)(const lds_float4*)stab - 32u * RTAB_BASE;
   const __amdgpu_buffer_rsrc_t rp32 = __builtin_amdgcn_make_buffer_rsrc((void*)a.p32, (short)0, a.nst * 16, 0x00020000);
   const __amdgpu_buffer_rsrc_t rv32 = __builtin_amdgcn_make_buffer_rsrc((void*)a.v32, (short)0, a.nst * 16, 0x00020000);
   const int wu = __builtin_amdgcn_readfirstlane(w);
@@ -1086,7 +1094,7 @@ __global__ __launch_bounds__(64 * CL_WAVES) __attribute__((amdgpu_waves_per_eu(C
     // lane l: entry c0 + CL_WAVES * G * (l / G) + l % G of this wave's next 64,
     // kept in the lane's registers and read by the pair loop with readlane
     int e_lane = 0;  // past the row: no beads
-    float sh_x = 0.f, sh_y = 0.f, sh_z = 0.f, sh_w = 1.f;  // no pairs, no veto
+    float sh_x = 0.f, sh_y = 0.f, sh_z = 0.f;
     {
       const int e_idx = c0 + CL_WAVES * G * (lane / G) + lane % G;
       if (e_idx < m) {
@@ -1102,10 +1110,9 @@ __global__ __launch_bounds__(64 * CL_WAVES) __attribute__((amdgpu_waves_per_eu(C
         sh_x = sx;
         sh_y = sy;
         sh_z = sz;
-        sh_w = reach < halfw ? 1.f : 0.f;
+        if (!(reach < halfw)) e_lane |= (int)0x80000000u;
       }
     }
-    const int flag_lane = sh_w != 0.f ? 1 : 0;
     const int c1 = min(m, c0 + 64 * CL_WAVES);
     // the chunk's (<= 64) terms of each lane are summed plainly in FP32, then
     // added to the lane's compensated sum
@@ -1115,20 +1122,24 @@ __global__ __launch_bounds__(64 * CL_WAVES) __attribute__((amdgpu_waves_per_eu(C
       bool ok[G];
       float3 pj[G], vj[G];
       float4 S[G];
-      int cflag = 1;
+      int cflag = 0;
 #pragma unroll
       for (int u = 0; u < G; ++u) {
         const int e = __builtin_amdgcn_readlane(e_lane, q + u);  // 0 past the row's end
-        const int fj = e >> 4, cj = e & 15;
-        ok[u] = ivalid && jj < cj && !(fj == first && jj == ii);
-        sj[u] = fj + (jj < cj ? jj : 0);
+        const int fj = (e >> 4) & 0x07ffffff, cj = e & 15;
+        // lanes of a real bead pair as a wave mask on the scalar unit: bead ii
+        // of I present, bead jj < cj of J, not the bead with itself
+        unsigned long long om = ivm & ((0x0101010101010101ull << cj) - 0x0101010101010101ull);
+        if (fj == first) om &= ~0x8040201008040201ull;
+        ok[u] = __builtin_amdgcn_inverse_ballot_w64(om);
+        sj[u] = fj + jj;
         S[u] = make_float4(__builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, sh_x), q + u)),
                            __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, sh_y), q + u)),
                            __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, sh_z), q + u)),
                            0.f);
-        cflag &= __builtin_amdgcn_readlane(flag_lane, q + u);
+        cflag |= e;
       }
-      const bool compact = cflag != 0;
+      const bool compact = cflag >= 0;
 #pragma unroll
       for (int u = 0; u < G; ++u) {  // 32-bit offsets into the position and vector copies
         const auto lp = __builtin_amdgcn_raw_buffer_load_b96(rp32, sj[u] * 16, 0, 0);
@@ -1153,32 +1164,34 @@ __global__ __launch_bounds__(64 * CL_WAVES) __attribute__((amdgpu_waves_per_eu(C
           dz -= Lz * rintf(dz * iLz);
         }
         const float rr = dx * dx + dy * dy + dz * dz;
-        const bool use = ok[u] && rr < rcf;
-        // Table coefficients on every lane (index clamped into the table; the
-        // hardware square root, 1 ulp, is ample for the lookup), so the common
-        // path has no branch and the partner loads stay ahead of it; pairs
-        // closer than the table take the closed form (rare); unused lanes get
-        // zero coefficients.
-        const float x = fmaxf(__builtin_amdgcn_sqrtf(rr) - RTAB_R0, 0.f) * (float)RTAB_INV;
-        const int it = min((int)x, ntab1);
-        const float t = x - (float)it;
+        // Table coefficients on every lane (r^2 clamped into the table, the
+        // entry read from its bits; pairs not summed take the zero entry), so
+        // the common path has no branch and the partner loads stay ahead of
+        // it; pairs closer than the table take the closed form (rare).
+        const float rs = (ok[u] && rr < rcf) ? rr : rtop;
+        const unsigned b = __float_as_uint(__builtin_amdgcn_fmed3f(rs, 1.f, rtop));
+        const float t = __uint_as_float(((b << RTAB_MBITS) & 0x007fffffu) | 0x3f800000u);
         float4 ca, cb;
         if constexpr (TAB_LDS) {
-          ca = stab[2 * it];
-          cb = stab[2 * it + 1];
+          // the entry's LDS address from the unreduced segment bits (stab_k is
+          // the table's address less RTAB_BASE entries)
+          unsigned addr;  // (b >> RTAB_SHIFT) * 32 + stab_k in one instruction
+          asm("v_lshl_add_u32 %0, %1, 5, %2" : "=v"(addr) : "v"(b >> RTAB_SHIFT), "s"(stab_k));
+          const float4* e = (const float4*)(const lds_float4*)(size_t)addr;
+          ca = e[0];
+          cb = e[1];
         } else {
+          const int it = (int)((b >> RTAB_SHIFT) - RTAB_BASE);
           ca = a.rtab[R.tab0 + 2 * it];
           cb = a.rtab[R.tab0 + 2 * it + 1];
         }
         float t1 = ca.x + t * (ca.y + t * (ca.z + t * ca.w));
         float b2 = cb.x + t * (cb.y + t * (cb.z + t * cb.w));
-        if (use && rr < RTAB_R0 * RTAB_R0) {
+        if (rs < RTAB_R0 * RTAB_R0) {
           float t2;
           rpy_real_near_f32(sqrtf(rr), rr, alpha, t1, t2);
           b2 = t2 / rr;
         }
-        t1 = use ? t1 : 0.f;
-        b2 = use ? b2 : 0.f;
         const float dv = (dx * vj[u].x + dy * vj[u].y + dz * vj[u].z) * b2;
         px = fmaf(dv, dx, fmaf(t1, vj[u].x, px));
         py = fmaf(dv, dy, fmaf(t1, vj[u].y, py));
