@@ -304,6 +304,7 @@ struct smdlt_ctx {
   int* log_off = nullptr;
   float4* p32 = nullptr;      // mixed precision: FP32 positions and scaled input vector
   float4* v32 = nullptr;
+  int *bcand_cnt = nullptr, *bcand_start = nullptr, *bcand = nullptr;  // mixed precision: spread candidates per brick
   float4* rtab = nullptr;     // mixed precision: real-space coefficient tables
   // bead clusters and the cluster-pair list (mixed precision)
   int *cl_first = nullptr, *cl_cnt = nullptr, *slot_cl = nullptr, *lcl = nullptr, *nclp = nullptr;
@@ -1096,6 +1097,9 @@ int smdlt_create(const smdlt_params* params, int n_replicas, int device, smdlt_c
     // zeroed: the cluster pass loads (and discards) the slots that follow a
     // cluster, which must hold finite values
     if (esz == 4) st |= alloc_dev(&c->p32, c->nst) | alloc_dev(&c->v32, c->nst);
+    if (esz == 4)  // every bead is a candidate of 8 bricks
+      st |= alloc_dev(&c->bcand_cnt, vb.size() + 1) | alloc_dev(&c->bcand_start, vb.size() + 1) |
+            alloc_dev(&c->bcand, 8 * (size_t)c->nst);
     if (st) {
       smdlt_destroy(c);
       return SMDLT_E_NOMEM;
@@ -1167,7 +1171,7 @@ void smdlt_destroy(smdlt_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   destroy_graphs(c);
   if (c->h_flags) hipHostFree(c->h_flags);
-  void* ptrs[] = {c->adot_cl, c->tea_beta, c->tea_C, c->tea_diag, c->cl_first, c->cl_cnt, c->slot_cl, c->lcl, c->nclp, c->rtab, c->p32, c->v32, c->mh_feats, c->mh_log, c->mlp_ws, c->mlp_x, c->mlp_y, c->mlp_part, c->mlp_out, c->saved, c->vir, c->tau, c->obs, c->d_chain0, c->qdisp, c->qdisp_saved, c->qrel, c->d_reps, c->d_slot_rep, c->d_seeds, c->rx, c->ry, c->rz, c->dxv, c->dyv, c->dzv, c->dsp, c->f,
+  void* ptrs[] = {c->adot_cl, c->tea_beta, c->tea_C, c->tea_diag, c->cl_first, c->cl_cnt, c->slot_cl, c->lcl, c->nclp, c->rtab, c->p32, c->v32, c->bcand_cnt, c->bcand_start, c->bcand, c->mh_feats, c->mh_log, c->mlp_ws, c->mlp_x, c->mlp_y, c->mlp_part, c->mlp_out, c->saved, c->vir, c->tau, c->obs, c->d_chain0, c->qdisp, c->qdisp_saved, c->qrel, c->d_reps, c->d_slot_rep, c->d_seeds, c->rx, c->ry, c->rz, c->dxv, c->dyv, c->dzv, c->dsp, c->f,
                   c->z, c->mf, c->bz, c->tmp, c->tmp2, c->nrpy, c->lrpy, c->nlj, c->llj, c->overflow, c->d_err,
                   c->cell_of, c->cell_cnt, c->cell_start, c->cell_fill, c->cell_beads, c->cell_tmp, c->rep_flags, c->out2, c->aff,
                   c->partial2, c->grid, c->bsp, c->tw, c->tx, c->ty, c->tz, c->txy, c->bricks, c->bbase, c->brick_cnt, c->brick_start,
@@ -1672,6 +1676,9 @@ static MobArgs mob_args(smdlt_ctx* c, int rb, int re, const double* v, double* u
   m.brick_len = c->bk0[re] - c->bk0[rb];
   m.p32 = c->p32;
   m.v32 = c->v32;
+  m.bcand_cnt = c->bcand_cnt;
+  m.bcand_start = c->bcand_start;
+  m.bcand = c->bcand;
   m.rtab = c->rtab;
   m.cl_first = c->cl_first;
   m.cl_cnt = c->cl_cnt;

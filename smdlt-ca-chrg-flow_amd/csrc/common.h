@@ -184,7 +184,8 @@ template <typename T, int P> __device__ __forceinline__ void bspline_weights(T x
 #pragma unroll
     for (int j = P - 1; j >= 0; --j) {
       T lo = (j >= 1) ? A[j - 1] : (T)0;
-      A[j] = div * ((x + (T)j) * A[j] + ((T)k - x - (T)j) * lo);
+      // one explicit fma: every kernel that evaluates the weights rounds them alike
+      A[j] = div * fma(x + (T)j, A[j], ((T)k - x - (T)j) * lo);
     }
   }
 #pragma unroll
@@ -353,6 +354,9 @@ struct MobArgs {
   // the scaled input vector (per product), 16-byte aligned for one load per bead
   float4* p32;
   float4* v32;
+  // mixed precision: per brick, its candidate beads for the matrix-core spread
+  // (bcand[bcand_start[b] ..] , bcand_cnt[b] entries; rebuilt with the bricks)
+  int *bcand_cnt, *bcand_start, *bcand;
   // mixed precision: per replica, for r^2 in [RTAB_R0^2, rc^2) on the segments of
   // rtab_locate, cubic coefficients (in its s) of t1(r) and of t2(r)/r^2 with the
   // RPY overlap correction below r = 2 (r^2 = 4 is a knot): rtab[2i], rtab[2i+1];

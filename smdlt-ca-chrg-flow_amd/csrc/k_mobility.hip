@@ -503,6 +503,226 @@ __global__ __launch_bounds__(512) void k_spread_gather(MobArgs a) {
   g[idx + 2 * ncomp] = acc2;
 }
 
+// Candidate beads of each brick for the matrix-core spread: the beads of the
+// brick and of its +1 neighbours along each axis (the only ones whose stencil
+// reaches it), concatenated in source-brick order, each list in slot order.
+// Built once per position version; every bead is a candidate of exactly 8
+// bricks, so the lists hold 8 entries per bead.
+__global__ void k_brick_cand_count(MobArgs a) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= a.brick_len) return;
+  const BrickTask bt = a.bricks[j];
+  const RepDev& R = a.reps[bt.rep];
+  int n = 0;
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    const int c0 = (bt.b0 + (t & 1)) % R.nbrick[0], c1 = (bt.b1 + ((t >> 1) & 1)) % R.nbrick[1];
+    const int c2 = (bt.b2 + (t >> 2)) % R.nbrick[2];
+    n += a.brick_cnt[R.brick0 + (c2 * R.nbrick[1] + c1) * R.nbrick[0] + c0];
+  }
+  a.bcand_cnt[a.brick_begin + j] = n;
+}
+
+__global__ __launch_bounds__(64) void k_brick_cand_fill(MobArgs a) {
+  const int j = blockIdx.x, bi = a.brick_begin + j;
+  const BrickTask bt = a.bricks[j];
+  const RepDev& R = a.reps[bt.rep];
+  int* dst = a.bcand + a.bcand_start[bi];
+  for (int t = 0; t < 8; ++t) {
+    const int c0 = (bt.b0 + (t & 1)) % R.nbrick[0], c1 = (bt.b1 + ((t >> 1) & 1)) % R.nbrick[1];
+    const int c2 = (bt.b2 + (t >> 2)) % R.nbrick[2];
+    const int sb = R.brick0 + (c2 * R.nbrick[1] + c1) * R.nbrick[0] + c0;
+    const int n = a.brick_cnt[sb], src = a.brick_start[sb];
+    for (int i = threadIdx.x; i < n; i += 64) dst[i] = a.brick_beads[src + i];
+    dst += n;
+  }
+}
+
+// One bead's B-spline weights along one axis at the 8 points of a brick:
+// out[x] = W[e - x] where 0 <= e - x < P, else 0, for a relative base
+// e = 8h + f in [0, 16).  W[(f - x) mod 8] is a rotation of the reversed weights
+// by f: three conditional rotations as bitwise selects under a mask the compiler
+// treats as opaque (left to itself it turns the rotation into a compare chain
+// per output), kept where x <= f for h = 0 and where x > f for h = 1.
+template <int P>
+__device__ __forceinline__ void spm_expand(const float4 w0, const float4 w1, int e, float out[8]) {
+  const float W[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
+  unsigned u[8];
+#pragma unroll
+  for (int y = 0; y < 8; ++y) {
+    const int j = (8 - y) & 7;
+    u[y] = j < P ? __float_as_uint(W[j]) : 0u;  // u[y] = W[-y mod 8]; weights past the order are zero
+  }
+  const int f = e & 7, h = e >> 3;
+#pragma unroll
+  for (int b = 1; b < 8; b <<= 1) {  // rotate right by f: u[y] <- u[(y - f) mod 8] = W[(f - y) mod 8]
+    unsigned m = 0u - (unsigned)((f & b) != 0);
+    asm volatile("" : "+v"(m));
+    unsigned t[8];
+#pragma unroll
+    for (int y = 0; y < 8; ++y) t[y] = (u[(y + 8 - b) & 7] & m) | (u[y] & ~m);
+#pragma unroll
+    for (int y = 0; y < 8; ++y) u[y] = t[y];
+  }
+  // kept points: x <= f for h = 0, x > f for h = 1
+  const unsigned low = (2u << f) - 1u, keep = h == 0 ? low : ~low;
+#pragma unroll
+  for (int x = 0; x < 8; ++x) out[x] = __uint_as_float(u[x] & (0u - ((keep >> x) & 1u)));
+}
+
+// Spreading on the matrix cores (FP32 grids).  A brick's grid values are a sum of
+// separable terms, G[x,y,z,c] = sum_b wx_b(x) wy_b(y) wz_b(z) v_b(c), i.e. the
+// product of a 64 x K matrix A[(x,y), b] = wx_b(x) wy_b(y) and a K x 24 matrix
+// B[b, (z,c)] = wz_b(z) v_b(c) over the K candidate beads of the brick.  One
+// workgroup per brick, four wavefronts; wavefront w owns the 16 rows x = 0..7,
+// y = 2w, 2w+1 and both 16-column tiles of B (n = 3z + c < 24, the rest zero),
+// and accumulates them with v_mfma_f32_16x16x4f32 over the beads four at a
+// time.  Beads are staged in chunks by four threads each (x weights, y
+// weights, and the two column tiles of B), which load the bead's base, weights
+// and vector and write its expanded rows to LDS.  The sum runs over the beads
+// in staging order with the matrix cores' fixed order, so the result does not
+// depend on scheduling; the output tile goes through LDS so the stores are runs
+// of 8 consecutive points as in the gather.
+#define SPM_CHUNK 128
+#define SPM_OUT_PITCH 65
+typedef float spm_f32x4 __attribute__((ext_vector_type(4)));
+template <int P>
+__global__ __launch_bounds__(256) void k_spread_mfma(MobArgs a) {
+  __shared__ float4 sE[SPM_CHUNK][4];     // x weights at x = 0..7, then y weights at y = 0..7
+  __shared__ float4 sB[2][SPM_CHUNK][4];  // columns n = 16t + j of B; reused for the output tile
+  static_assert(sizeof(sB) >= 3 * BRICK * SPM_OUT_PITCH * sizeof(float), "output tile fits the B stage");
+  static_assert(SPM_CHUNK == 128, "the staging passes map 128 beads to half a workgroup");
+  const int bi = a.brick_begin + blockIdx.x;
+  const BrickTask bt = a.bricks[blockIdx.x];
+  const int cb = a.bcand_start[bi], total = a.bcand_cnt[bi], nhome = a.brick_cnt[bi];
+  const RepDev& R = a.reps[bt.rep];
+  if (!R.active || (a.done && a.done[bt.rep])) return;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int K0 = R.K[0], K1 = R.K[1], K2 = R.K[2];
+  const int lo[3] = {bt.b0 * BRICK, bt.b1 * BRICK, bt.b2 * BRICK}, Kd[3] = {K0, K1, K2};
+  const bool fs = a.scale_in_spread != 0;
+  const double* vin = (a.vout && !fs) ? a.vout : a.v;
+  const double vsc = (fs && a.beta) ? (a.beta[bt.rep] > 0.0 ? 1.0 / a.beta[bt.rep] : 0.0) : 1.0;
+  const float* bw = (const float*)a.bw;
+  // this lane's operands: row (ax, ay) of A, column bj of both tiles of B, bead kq of each group of four
+  const int ax = lane & 7, ay = 2 * w + ((lane >> 3) & 1), kq = lane >> 4, bj = lane & 15;
+  const float* sEf = (const float*)sE;
+  const float* sB0 = (const float*)sB[0];
+  const float* sB1 = (const float*)sB[1];
+  spm_f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < total; k0 += SPM_CHUNK) {
+    const int m = min(SPM_CHUNK, total - k0), mp = (m + 15) & ~15;
+    if (k0 > 0) __syncthreads();  // the previous chunk's products have read the stage
+    // part 0: x weights, 1: y weights, 2 and 3: the column tiles of B, in two passes:
+    // wavefronts 0-1 take parts 0 then 1, wavefronts 2-3 parts 2 then 3; rows
+    // m <= k < mp (padding to whole groups of 16) are zero
+    for (int pass = 0; pass < 2; ++pass) {
+      const int part = 2 * (tid >> 7) + pass, k = tid & 127;
+      if (k >= mp) continue;
+      float o[8];
+      float4* dst = part < 2 ? &sE[k][2 * part] : sB[part - 2][k];
+      if (k >= m) {
+        const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+        dst[0] = zero;
+        dst[1] = zero;
+        if (part >= 2) {
+          dst[2] = zero;
+          dst[3] = zero;
+        }
+        continue;
+      }
+      const int kk = k0 + k, s = a.bcand[cb + kk], d = part < 2 ? part : 2;
+      int e = a.bbase[3 * s + d] - lo[d];
+      e += e < 0 ? Kd[d] : 0;
+      const float4* wp = (const float4*)(bw + (size_t)BW_STRIDE * s + d * MAX_ORDER);
+      const float4 w0 = wp[0], w1 = wp[1];
+      if (part < 2) {
+        spm_expand<P>(w0, w1, e, o);
+        dst[0] = make_float4(o[0], o[1], o[2], o[3]);
+        dst[1] = make_float4(o[4], o[5], o[6], o[7]);
+        continue;
+      }
+      float v0, v1, v2;
+      if (fs) {
+        const double x0 = vin[3 * s] * vsc, x1 = vin[3 * s + 1] * vsc, x2 = vin[3 * s + 2] * vsc;
+        v0 = (float)x0;
+        v1 = (float)x1;
+        v2 = (float)x2;
+        if (part == 2 && kk < nhome) {  // every bead has one home brick: it stores the scaled input once
+          if (a.vout) {
+            a.vout[3 * s] = x0;
+            a.vout[3 * s + 1] = x1;
+            a.vout[3 * s + 2] = x2;
+          }
+          if (a.v32) a.v32[s] = make_float4(v0, v1, v2, 0.f);
+        }
+      } else if (a.v32) {
+        const float4 v4 = a.v32[s];
+        v0 = v4.x;
+        v1 = v4.y;
+        v2 = v4.z;
+      } else {
+        v0 = (float)vin[3 * s];
+        v1 = (float)vin[3 * s + 1];
+        v2 = (float)vin[3 * s + 2];
+      }
+      spm_expand<P>(w0, w1, e, o);
+      // row of B: n = 3z + c -> o[z] * v[c]; part 2 holds n = 0..15, part 3 n = 16..23 then zeros
+      if (part == 2) {
+        dst[0] = make_float4(o[0] * v0, o[0] * v1, o[0] * v2, o[1] * v0);
+        dst[1] = make_float4(o[1] * v1, o[1] * v2, o[2] * v0, o[2] * v1);
+        dst[2] = make_float4(o[2] * v2, o[3] * v0, o[3] * v1, o[3] * v2);
+        dst[3] = make_float4(o[4] * v0, o[4] * v1, o[4] * v2, o[5] * v0);
+      } else {
+        dst[0] = make_float4(o[5] * v1, o[5] * v2, o[6] * v0, o[6] * v1);
+        dst[1] = make_float4(o[6] * v2, o[7] * v0, o[7] * v1, o[7] * v2);
+        dst[2] = make_float4(0.f, 0.f, 0.f, 0.f);
+        dst[3] = make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+    }
+    __syncthreads();
+    for (int kk = 0; kk < mp; kk += 16) {  // four groups of four beads: operands first, then the products
+      float av[4], b0[4], b1[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int kb = kk + 4 * u + kq;
+        av[u] = sEf[16 * kb + ax] * sEf[16 * kb + 8 + ay];
+        b0[u] = sB0[16 * kb + bj];
+        b1[u] = sB1[16 * kb + bj];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u], b0[u], acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u], b1[u], acc1, 0, 0, 0);
+      }
+    }
+  }
+  __syncthreads();
+  // D[i][j] sits in register i % 4 of lane j + 16 (i / 4): row i = (x, y), column (z, c)
+  // out[(c * 8 + z) * SPM_OUT_PITCH + y * 8 + x]: the pitch of 65 puts the 16 (z, c) of one
+  // register on distinct banks
+  float* out = (float*)sB;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int i = 4 * kq + r, x = i & 7, y = 2 * w + (i >> 3);
+    {
+      const int n = bj, z = n / 3, c = n - 3 * z;
+      out[(c * BRICK + z) * SPM_OUT_PITCH + y * BRICK + x] = acc0[r];
+    }
+    if (bj < 3 * BRICK - 16) {
+      const int n = 16 + bj, z = n / 3, c = n - 3 * z;
+      out[(c * BRICK + z) * SPM_OUT_PITCH + y * BRICK + x] = acc1[r];
+    }
+  }
+  __syncthreads();
+  float* g = (float*)a.grid + R.grid0;
+  const long long ncomp = (long long)K0 * K1 * K2;
+  for (int idx = tid; idx < 3 * BRICK * BRICK * BRICK; idx += 256) {
+    const int cz = idx >> 6, y = (idx >> 3) & 7, x = idx & 7, c = cz >> 3, z = cz & 7;
+    g[c * ncomp + ((long long)(lo[2] + z) * K1 + lo[1] + y) * K0 + lo[0] + x] = out[cz * SPM_OUT_PITCH + (idx & 63)];
+  }
+}
+
 // ------------------------------------------------------------------- FFT passes
 // Copies a line length's twiddle table into LDS, so the Stockham stages read
 // their twiddles there; the barrier that follows the grid load orders it.
@@ -1250,6 +1470,11 @@ template <typename T, int P> static void launch_mob_p(hipStream_t st, const MobA
       launch_scan(st, a.brick_cnt + a.brick_begin, a.brick_start + a.brick_begin, a.brick_len);
       k_brick_fill<<<nt, 256, 0, st>>>(a);
       if (a.brick_len > 0) k_brick_sort<<<(a.brick_len + 255) / 256, 256, 0, st>>>(a);
+      if (a.bcand && a.brick_len > 0) {
+        k_brick_cand_count<<<(a.brick_len + 255) / 256, 256, 0, st>>>(a);
+        launch_scan(st, a.bcand_cnt + a.brick_begin, a.bcand_start + a.brick_begin, a.brick_len);
+        k_brick_cand_fill<<<a.brick_len, 64, 0, st>>>(a);
+      }
     }
     if ((a.vout || a.v32) && a.nslots > 0 && !fuse) k_scale_v<<<nt, 256, 0, st>>>(a);
   }
@@ -1258,7 +1483,10 @@ template <typename T, int P> static void launch_mob_p(hipStream_t st, const MobA
   if (a.nbricks > 0) {
     MobArgs as = a;
     as.scale_in_spread = fuse ? 1 : 0;
-    k_spread_gather<T, P><<<a.nbricks, 512, 0, st>>>(as);
+    if (sizeof(T) == 4 && a.bcand)
+      k_spread_mfma<P><<<a.nbricks, 256, 0, st>>>(as);
+    else
+      k_spread_gather<T, P><<<a.nbricks, 512, 0, st>>>(as);
   }
   mark(2);
   if (a.ntxy > 0) k_fft_xy<T><<<a.ntxy, XY_THREADS, 0, st>>>(a, -1);
