@@ -595,6 +595,8 @@ static MobChoice mob_candidate(double rc, int order, double tol, double skin, co
   return m;
 }
 
+static double rtab_knot_up(double x);
+
 static int choose_mobility(smdlt_ctx* c, int r) {
   const smdlt_params& p = c->P[r];
   RepDev& R = c->rh[r];
@@ -654,8 +656,8 @@ static int choose_mobility(smdlt_ctx* c, int r) {
   R.K[2] = best.K[2];
   R.order = best.order;
   R.alpha = best.alpha;
-  R.rc2 = rc * rc;
-  R.rlist2 = (rc + skin) * (rc + skin);
+  R.rc2 = c->prec == SMDLT_PREC_FP64 ? rc * rc : rtab_knot_up(rc * rc);
+  R.rlist2 = (sqrt(R.rc2) + skin) * (sqrt(R.rc2) + skin);
   R.skin = skin;
   R.selfconst = ewald_selfconst(best.alpha);
   c->nbr_est = std::max(c->nbr_est, best.nbr);
@@ -702,11 +704,28 @@ static double rtab_knot(int i) {
   return (double)f;
 }
 
+// Segments covering r^2 in [1, rc2), plus the zero entry.  Mixed-precision
+// cut-offs are table knots (rtab_knot_up), so the zero entry starts at rc2 and
+// the kernels' clamp of r^2 to [1, rc2] is their cut-off test.
 static int rtab_segments(double rc2) {
-  const float rcf = (float)rc2;  // the kernels' cut-off test
+  const float rcf = (float)rc2;
   uint32_t b;
   memcpy(&b, &rcf, 4);
-  return std::max(1, (int)((b >> RTAB_SHIFT) - RTAB_BASE) + 1) + 1;  // + the zero entry
+  const int idx = (int)((b >> RTAB_SHIFT) - RTAB_BASE);  // the segment holding rc2
+  const bool knot = (b & ((1u << RTAB_SHIFT) - 1u)) == 0u;
+  return std::max(1, knot ? idx : idx + 1) + 1;
+}
+
+// The smallest table knot >= x (x >= 1): r^2 cut-offs of the mixed-precision
+// kernels sit on knots, raising rc by less than 1/64 of itself.
+static double rtab_knot_up(double x) {
+  float f = (float)x;
+  if ((double)f < x) f = nextafterf(f, INFINITY);
+  uint32_t b;
+  memcpy(&b, &f, 4);
+  if (b & ((1u << RTAB_SHIFT) - 1u)) b = ((b >> RTAB_SHIFT) + 1u) << RTAB_SHIFT;
+  memcpy(&f, &b, 4);
+  return (double)f;
 }
 
 static void build_rtab(double alpha, int ntab, float* out) {

@@ -1304,6 +1304,10 @@ __global__ __launch_bounds__(64 * CL_WAVES) __attribute__((amdgpu_waves_per_eu(C
   const int m = a.nclp[I];
   const int* row = a.lcl + (size_t)I * a.capc;
   const float rtop = rtab_top(R.ntab);
+  // a VGPR copy, so the per-pair select reads one scalar operand (its mask) and the
+  // compiler does not rematerialise the copy in the loop
+  float vtop = rtop;
+  asm volatile("" : "+v"(vtop));
   const unsigned stab_k = (unsigned)(size_t)(const lds_float4*)stab - 32u * RTAB_BASE;
   const __amdgpu_buffer_rsrc_t rp32 = __builtin_amdgcn_make_buffer_rsrc((void*)a.p32, (short)0, a.nst * 16, 0x00020000);
   const __amdgpu_buffer_rsrc_t rv32 = __builtin_amdgcn_make_buffer_rsrc((void*)a.v32, (short)0, a.nst * 16, 0x00020000);
@@ -1339,7 +1343,7 @@ __global__ __launch_bounds__(64 * CL_WAVES) __attribute__((amdgpu_waves_per_eu(C
     float px = 0.f, py = 0.f, pz = 0.f;
     for (int k0 = c0, q = 0; k0 < c1; k0 += CL_WAVES * G, q += G) {
       int sj[G];
-      bool ok[G];
+      bool ok[G], zs[G];
       float3 pj[G], vj[G];
       float4 S[G];
       int cflag = 0;
@@ -1353,10 +1357,11 @@ __global__ __launch_bounds__(64 * CL_WAVES) __attribute__((amdgpu_waves_per_eu(C
         if (fj == first) om &= ~0x8040201008040201ull;
         ok[u] = __builtin_amdgcn_inverse_ballot_w64(om);
         sj[u] = fj + jj;
-        S[u] = make_float4(__builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, sh_x), q + u)),
-                           __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, sh_y), q + u)),
-                           __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, sh_z), q + u)),
-                           0.f);
+        const int ix = __builtin_amdgcn_readlane(__builtin_bit_cast(int, sh_x), q + u);
+        const int iy = __builtin_amdgcn_readlane(__builtin_bit_cast(int, sh_y), q + u);
+        const int iz = __builtin_amdgcn_readlane(__builtin_bit_cast(int, sh_z), q + u);
+        S[u] = make_float4(__builtin_bit_cast(float, ix), __builtin_bit_cast(float, iy), __builtin_bit_cast(float, iz), 0.f);
+        zs[u] = ((ix | iy | iz) & 0x7fffffff) == 0;  // on the scalar unit: +-0 in every component
         cflag |= e;
       }
       const bool compact = cflag >= 0;
@@ -1370,7 +1375,11 @@ __global__ __launch_bounds__(64 * CL_WAVES) __attribute__((amdgpu_waves_per_eu(C
 #pragma unroll
       for (int u = 0; u < G; ++u) {
         float dx, dy, dz;
-        if (compact) {
+        if (compact && zs[u]) {  // the clusters' common image is the unshifted one
+          dx = pj[u].x - pi.x;
+          dy = pj[u].y - pi.y;
+          dz = pj[u].z - pi.z;
+        } else if (compact) {
           dx = (pj[u].x + S[u].x) - pi.x;
           dy = (pj[u].y + S[u].y) - pi.y;
           dz = (pj[u].z + S[u].z) - pi.z;
@@ -1384,12 +1393,13 @@ __global__ __launch_bounds__(64 * CL_WAVES) __attribute__((amdgpu_waves_per_eu(C
           dz -= Lz * rintf(dz * iLz);
         }
         const float rr = dx * dx + dy * dy + dz * dz;
-        // Table coefficients on every lane (r^2 clamped into the table, the
-        // entry read from its bits; pairs not summed take the zero entry), so
-        // the common path has no branch and the partner loads stay ahead of
-        // it; pairs closer than the table take the closed form (rare).
-        const float rs = (ok[u] && rr < rcf) ? rr : rtop;
-        const unsigned b = __float_as_uint(__builtin_amdgcn_fmed3f(rs, 1.f, rtop));
+        // Table coefficients on every lane: r^2 clamped into [1, rc^2] (the cut-off
+        // is a table knot, where the zero entry starts; pairs not summed clamp to
+        // it from below), the entry read from the bits, so the common path has no
+        // branch and the partner loads stay ahead of it; pairs closer than the
+        // table take the closed form (rare).
+        const float rs = __builtin_amdgcn_fmed3f(rr, ok[u] ? 1.f : vtop, rtop);
+        const unsigned b = __float_as_uint(rs);
         const float t = __uint_as_float(((b << RTAB_MBITS) & 0x007fffffu) | 0x3f800000u);
         float4 ca, cb;
         if constexpr (TAB_LDS) {
@@ -1407,7 +1417,7 @@ __global__ __launch_bounds__(64 * CL_WAVES) __attribute__((amdgpu_waves_per_eu(C
         }
         float t1 = ca.x + t * (ca.y + t * (ca.z + t * ca.w));
         float b2 = cb.x + t * (cb.y + t * (cb.z + t * cb.w));
-        if (rs < RTAB_R0 * RTAB_R0) {
+        if (ok[u] && rr < RTAB_R0 * RTAB_R0) {
           float t2;
           rpy_real_near_f32(sqrtf(rr), rr, alpha, t1, t2);
           b2 = t2 / rr;
