@@ -578,8 +578,9 @@ __device__ __forceinline__ void spm_expand(const float4 w0, const float4 w1, int
 // y = 2w, 2w+1 and both 16-column tiles of B (n = 3z + c < 24, the rest zero),
 // and accumulates them with v_mfma_f32_16x16x4f32 over the beads four at a
 // time.  Beads are staged in chunks by four threads each (x weights, y
-// weights, and the two column tiles of B), which load the bead's base, weights
-// and vector and write its expanded rows to LDS.  The sum runs over the beads
+// weights, and columns 0-15 and 16-23 of B), which load the bead's base,
+// weights and vector and write its expanded rows to LDS (20 KB per workgroup, so
+// seven workgroups fit a CU and hide each other's staging latency).  The sum runs over the beads
 // in staging order with the matrix cores' fixed order, so the result does not
 // depend on scheduling; the output tile goes through LDS so the stores are runs
 // of 8 consecutive points as in the gather.
@@ -588,8 +589,11 @@ __device__ __forceinline__ void spm_expand(const float4 w0, const float4 w1, int
 typedef float spm_f32x4 __attribute__((ext_vector_type(4)));
 template <int P>
 __global__ __launch_bounds__(256) void k_spread_mfma(MobArgs a) {
-  __shared__ float4 sE[SPM_CHUNK][4];     // x weights at x = 0..7, then y weights at y = 0..7
-  __shared__ float4 sB[2][SPM_CHUNK][4];  // columns n = 16t + j of B; reused for the output tile
+  __shared__ float4 sE[SPM_CHUNK][4];  // x weights at x = 0..7, then y weights at y = 0..7
+  // rows of B (n = 0..23), then 8 floats that the last row's reads of the second
+  // column tile run into (columns n >= 24 of the product are never stored); reused
+  // for the output tile
+  __shared__ float4 sB[SPM_CHUNK * 6 + 2];
   static_assert(sizeof(sB) >= 3 * BRICK * SPM_OUT_PITCH * sizeof(float), "output tile fits the B stage");
   static_assert(SPM_CHUNK == 128, "the staging passes map 128 beads to half a workgroup");
   const int bi = a.brick_begin + blockIdx.x;
@@ -607,8 +611,7 @@ __global__ __launch_bounds__(256) void k_spread_mfma(MobArgs a) {
   // this lane's operands: row (ax, ay) of A, column bj of both tiles of B, bead kq of each group of four
   const int ax = lane & 7, ay = 2 * w + ((lane >> 3) & 1), kq = lane >> 4, bj = lane & 15;
   const float* sEf = (const float*)sE;
-  const float* sB0 = (const float*)sB[0];
-  const float* sB1 = (const float*)sB[1];
+  const float* sBf = (const float*)sB;
   spm_f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
   for (int k0 = 0; k0 < total; k0 += SPM_CHUNK) {
     const int m = min(SPM_CHUNK, total - k0), mp = (m + 15) & ~15;
@@ -620,12 +623,12 @@ __global__ __launch_bounds__(256) void k_spread_mfma(MobArgs a) {
       const int part = 2 * (tid >> 7) + pass, k = tid & 127;
       if (k >= mp) continue;
       float o[8];
-      float4* dst = part < 2 ? &sE[k][2 * part] : sB[part - 2][k];
+      float4* dst = part < 2 ? &sE[k][2 * part] : &sB[6 * k + 4 * (part - 2)];
       if (k >= m) {
         const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
         dst[0] = zero;
         dst[1] = zero;
-        if (part >= 2) {
+        if (part == 2) {
           dst[2] = zero;
           dst[3] = zero;
         }
@@ -676,8 +679,6 @@ __global__ __launch_bounds__(256) void k_spread_mfma(MobArgs a) {
       } else {
         dst[0] = make_float4(o[5] * v1, o[5] * v2, o[6] * v0, o[6] * v1);
         dst[1] = make_float4(o[6] * v2, o[7] * v0, o[7] * v1, o[7] * v2);
-        dst[2] = make_float4(0.f, 0.f, 0.f, 0.f);
-        dst[3] = make_float4(0.f, 0.f, 0.f, 0.f);
       }
     }
     __syncthreads();
@@ -687,8 +688,8 @@ __global__ __launch_bounds__(256) void k_spread_mfma(MobArgs a) {
       for (int u = 0; u < 4; ++u) {
         const int kb = kk + 4 * u + kq;
         av[u] = sEf[16 * kb + ax] * sEf[16 * kb + 8 + ay];
-        b0[u] = sB0[16 * kb + bj];
-        b1[u] = sB1[16 * kb + bj];
+        b0[u] = sBf[24 * kb + bj];
+        b1[u] = sBf[24 * kb + 16 + bj];  // bj >= 8: the next row's columns, in unused output columns
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
