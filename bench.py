@@ -499,7 +499,7 @@ def kernel_bytes(n, G, list_entries, order=8, esz=4):
     BW = 3 * order * esz
     return {
         "k_bead_prep": n * (24 + 16 + 12 + BW),
-        "k_spread_gather": n * (12 + 16 + BW) + g,
+        "k_spread": n * (12 + 16 + BW) + g,
         "k_fft_x(fwd)": 2 * g, "k_fft_y(fwd)": 2 * g, "k_fft_z_green": 2 * g,
         "k_fft_y(inv)": 2 * g, "k_fft_x(inv)": 2 * g,
         "k_mob_final": n * (16 + 16 + 24 + 24 + 12 + BW) + 4 * list_entries + g,
@@ -511,6 +511,8 @@ def stage_kernels(stage, order, cluster):
     mobility product runs the cluster pass (mixed precision) or the bead pass."""
     if stage == "k_mob_final" and cluster:
         return [f"k_mob_cl<{order}, "]
+    if stage == "k_spread":
+        return [f"k_spread_mfma<{order}>", "k_spread_gather<"]
     return [stage, stage.split("(")[0]]
 
 
@@ -539,7 +541,7 @@ def pmc_traffic(section, names):
             d = json.load(fh).get(section, {})
         for n in names:
             for k, v in d.items():
-                if k == n or (n.endswith(", ") and k.startswith(n)):
+                if k == n or (n.endswith((", ", "<")) and k.startswith(n)):
                     return k, v
     return None, None
 

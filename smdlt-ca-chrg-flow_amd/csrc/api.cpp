@@ -228,7 +228,7 @@ enum Stage {
 };
 static const char* kStageNames[ST_COUNT] = {
     "remap", "neighbors", "forces", "noise", "mobility_apply", "lanczos_other", "update", "mc",
-    "k_bead_prep", "k_spread_gather", "k_fft_x(fwd)", "k_fft_y(fwd)", "k_fft_z_green", "k_fft_y(inv)",
+    "k_bead_prep", "k_spread", "k_fft_x(fwd)", "k_fft_y(fwd)", "k_fft_z_green", "k_fft_y(inv)",
     "k_fft_x(inv)", "k_mob_final"};
 
 struct Timer {

@@ -596,8 +596,11 @@ __global__ __launch_bounds__(256) void k_spread_mfma(MobArgs a) {
   __shared__ float4 sB[SPM_CHUNK * 6 + 2];
   static_assert(sizeof(sB) >= 3 * BRICK * SPM_OUT_PITCH * sizeof(float), "output tile fits the B stage");
   static_assert(SPM_CHUNK == 128, "the staging passes map 128 beads to half a workgroup");
-  const int bi = a.brick_begin + blockIdx.x;
-  const BrickTask bt = a.bricks[blockIdx.x];
+  // XCD-aware order: each XCD takes a contiguous range of bricks, so the neighbours
+  // that share a bead's data, and that complete each other's cache lines of the
+  // grid, run under one L2
+  const int jb = xcd_block(blockIdx.x, gridDim.x), bi = a.brick_begin + jb;
+  const BrickTask bt = a.bricks[jb];
   const int cb = a.bcand_start[bi], total = a.bcand_cnt[bi], nhome = a.brick_cnt[bi];
   const RepDev& R = a.reps[bt.rep];
   if (!R.active || (a.done && a.done[bt.rep])) return;
