@@ -21,9 +21,11 @@ M.F and M^(1/2) z errors against the CPU oracle are measured by the tests.
 Also reported (`charge_hopping`): 20 chains x 300 beads with 2400 charges
 (hop radius 3): the sweep time of the reference chargeHop, of the exactly
 balanced hop MC with uniform and with learned proposals, Brownian dynamics
-with a learned-proposal sweep every 10 steps, and tau_int of the Coulomb
-energy and of the charge configuration per sweep, uniform against learned
-(16 replicas; the proposal trained on 3e5 uniform-proposal moves).
+with a learned-proposal sweep every 10 steps, and tau_int per sweep of the
+Coulomb energy, of the charge configuration and of the charge distribution over
+the chains, uniform against learned (64 replicas; the proposal trained on 3e5
+uniform-proposal moves); then the learned proposal over hops within 8a against
+the uniform proposal at 3a and at 8a (`far_hop`).
 """
 import argparse
 import json
@@ -257,9 +259,12 @@ def tau_int_multi(E, c=5.0, per_column_mean=False, groups=1):
     return tau
 
 
-def mh_study(ctx, R, nq, n, track, sign, train_moves, sweeps, burn, iters, hops_learned=True, tag=""):
+def mh_study(ctx, R, nq, n, track, sign, train_moves, sweeps, burn, iters, hops_learned=True, tag="", nb_chain=None):
     """Train the proposal on the uniform chains' moves of ctx (R replicas), then run
-    uniform and learned chains; tau_int of the energy and of the charge configuration."""
+    uniform and learned chains; tau_int of the energy, of the charge configuration
+    (per-bead occupancy) and of the charge distribution over the chains (net charge
+    per chain of nb_chain beads)."""
+    nb_chain = nb_chain or n
     t0 = time.perf_counter()
     ctx.upload_charges([nq] * R, np.tile(track, R), np.tile(sign, R))
     F, Y = [], []
@@ -303,8 +308,11 @@ def mh_study(ctx, R, nq, n, track, sign, train_moves, sweeps, burn, iters, hops_
         O3 = np.zeros((ns, R, n), np.float32)
         for r in range(R):
             O3[np.arange(ns)[:, None], r, sub[:, r, :]] = sign[None, :]
+        # net charge of each chain: the charge distribution over the chains
+        Qc = O3.reshape(ns, R, n // nb_chain, nb_chain).sum(3)
         out = {"acceptance": acc / (sweeps * nq * R), "tau_E": float(tau_int_multi(E[burn:])),
                "tau_config": float(tau_int_multi(O3.reshape(ns, -1), per_column_mean=True, groups=R)),
+               "tau_chain_charge": float(tau_int_multi(Qc.reshape(ns, -1), per_column_mean=True, groups=R)),
                "mean_E": float(E[burn:].mean()), "mean_E_se": float(E[burn:].mean(0).std(ddof=1) / np.sqrt(R)),
                "sweep_ms": 1e3 * el / sweeps}
         print(f"[bench] {tag} {'learned' if mode == S.MC_MH_LEARNED else 'uniform'}: {out}", file=sys.stderr,
@@ -317,13 +325,15 @@ def mh_study(ctx, R, nq, n, track, sign, train_moves, sweeps, burn, iters, hops_
     return res, w
 
 
-def learned_leg(replicas=16, train_moves=1_000_000, sweeps=150, burn=30, iters=1000, strong=True):
+def learned_leg(replicas=16, train_moves=1_000_000, sweeps=150, burn=30, iters=1000, strong=True, far_hop=8.0):
     """Charge hopping with the learned proposal.
     (1) 20 chains x 300 beads (L = 60, 40 % charged, lambda_B = 3.59, lambda_D = 5,
     hop radius 3a): bead-steps/s of Brownian dynamics with one learned-proposal MC
     sweep every 10 steps, and tau_int per sweep of the Coulomb energy and of the
     charge configuration, uniform against learned proposal (trained on 1e6
-    uniform-proposal moves).
+    uniform-proposal moves).  The same at hop radius far_hop: the learned proposal
+    (trained on uniform moves at that radius) against the uniform proposal at 3a
+    (the configuration's) and at far_hop.
     (2) A strongly coupled small system (4 x 30 beads, L = 18, 40 charges,
     lambda_B = 4) where the uniform proposal mixes slowly: uniform at 3a and 8a
     against the learned proposal at 8a."""
@@ -367,8 +377,16 @@ def learned_leg(replicas=16, train_moves=1_000_000, sweeps=150, burn=30, iters=1
     ctx = S.Context([params(r, 3.0) for r in range(R)])
     ctx.upload_positions(np.tile(x, R), np.tile(y, R), np.tile(z, R))
     ctx.remap(2)
-    hop3, w = mh_study(ctx, R, nq, n, track, sign, train_moves, sweeps, burn, iters, tag="20x300 hop 3")
+    hop3, w = mh_study(ctx, R, nq, n, track, sign, train_moves, sweeps, burn, iters, tag="20x300 hop 3", nb_chain=nb)
     ctx.close()
+    far = None
+    if far_hop:
+        ctx = S.Context([params(r, far_hop) for r in range(R)])
+        ctx.upload_positions(np.tile(x, R), np.tile(y, R), np.tile(z, R))
+        ctx.remap(2)
+        far, _ = mh_study(ctx, R, nq, n, track, sign, train_moves, sweeps, burn, iters, tag=f"20x300 hop {far_hop:g}",
+                          nb_chain=nb)
+        ctx.close()
     bd.mlp_set_weights(w)
     bd.mc_record_features(False)
     bd.mc_sweep(S.MC_MH_LEARNED)
@@ -389,9 +407,22 @@ def learned_leg(replicas=16, train_moves=1_000_000, sweeps=150, burn=30, iters=1
     out.update({"bead_steps_per_s_bd_plus_mc": n * 10 * nblk / el, "mc_every": 10,
                 "sweep_ms_one_replica": sweep_ms, "chains": hop3,
                 "tau_E_ratio_uniform_over_learned": hop3["uniform"]["tau_E"] / hop3["learned"]["tau_E"],
-                "tau_config_ratio_uniform_over_learned": hop3["uniform"]["tau_config"] / hop3["learned"]["tau_config"]})
-    print(f"[bench] charge hopping: sweeps {sweep_ms} ms, tau_E ratio {out['tau_E_ratio_uniform_over_learned']:.2f}", file=sys.stderr,
-          flush=True)
+                "tau_config_ratio_uniform_over_learned": hop3["uniform"]["tau_config"] / hop3["learned"]["tau_config"],
+                "tau_chain_charge_ratio_uniform_over_learned":
+                    hop3["uniform"]["tau_chain_charge"] / hop3["learned"]["tau_chain_charge"]})
+    if far is not None:
+        ratios = {}
+        for key in ("tau_E", "tau_config", "tau_chain_charge"):
+            ratios[key + "_ratio_uniform3_over_learned_far"] = hop3["uniform"][key] / far["learned"][key]
+            ratios[key + "_ratio_same_radius_far"] = far["uniform"][key] / far["learned"][key]
+        out["far_hop"] = {"hop_radius": far_hop, "chains": far, **ratios,
+                          "note": "learned proposal over the hops within the far radius (trained on uniform moves "
+                                  "there) against the configuration's uniform 3a proposal, and against the uniform "
+                                  "proposal at the same radius"}
+    print(f"[bench] charge hopping: sweeps {sweep_ms} ms, tau_E ratio {out['tau_E_ratio_uniform_over_learned']:.2f}"
+          + (f", learned at {far_hop:g}a over uniform at 3a: tau_E {out['far_hop']['tau_E_ratio_uniform3_over_learned_far']:.2f}, "
+             f"tau_chain_charge {out['far_hop']['tau_chain_charge_ratio_uniform3_over_learned_far']:.2f}" if far is not None else ""),
+          file=sys.stderr, flush=True)
     if not strong:
         return out
     # strongly coupled small system
@@ -589,7 +620,7 @@ def main():
         print(json.dumps({"learned": learned_leg()}, default=float))
         return
     if args.workload == "hop":
-        print(json.dumps({"charge_hopping": learned_leg(replicas=16, train_moves=300_000, sweeps=100, burn=20,
+        print(json.dumps({"charge_hopping": learned_leg(replicas=64, train_moves=300_000, sweeps=100, burn=20,
                                                         iters=600, strong=False)}, default=float))
         return
     if args.workload == "semidilute":
@@ -603,7 +634,7 @@ def main():
         semi = semidilute_leg(args.semidilute_steps, 3, 0 if args.no_plain else 3)
     hop = None
     if args.workload == "all":
-        hop = learned_leg(replicas=16, train_moves=300_000, sweeps=100, burn=20, iters=600, strong=False)
+        hop = learned_leg(replicas=64, train_moves=300_000, sweeps=100, burn=20, iters=600, strong=False)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(specs, pos)
