@@ -511,6 +511,11 @@ void launch_mc_energy(hipStream_t st, const RepDev* reps, int rep_begin, int nre
 #define MLP_NP (16 * 64 + 64 + 64 * 64 + 64 + 64 + 1)
 
 #define MH_ROUND 64
+// workgroup sizes of the MH round: the learned proposal scores its candidates one
+// wavefront each, so it takes more wavefronts than the uniform one
+#define MH_THREADS_UNIFORM MC_THREADS
+#define MH_THREADS_LEARNED 512
+#define MH_THREADS_MAX 1024
 #define MH_HOP_PRE 16  // hop-list entries of each attempt's bead fetched with the round
 // replicas up to these sizes keep the occupancy and charge beads in LDS
 #define MH_LDS_N 16384
@@ -543,8 +548,8 @@ struct MhShared {
   float feat[MH_MAXC][16];
   float fsel[16];
   float w[MLP_NP];
-  double dsum[MC_THREADS / 64];
-  int wcnt[MC_THREADS / 64];
+  double dsum[MH_THREADS_MAX / 64];
+  int wcnt[MH_THREADS_MAX / 64];
   int nloc, ncand, k, x, qk, y, pick, accept, bad;
   float u[3];
   double qf;
@@ -862,8 +867,8 @@ __global__ __launch_bounds__(MC_THREADS) void k_mh_fold(const RepDev* __restrict
 // Attempts a0 .. a0 + MH_ROUND - 1 of each replica (one workgroup per replica).
 // LQ: the occupancy and the charges' beads are read from LDS copies (written
 // through to memory on every accepted move).
-template <bool LQ>
-__global__ __launch_bounds__(MC_THREADS) void k_mh_round(
+template <bool LQ, int NT>
+__global__ __launch_bounds__(NT) void k_mh_round(
     const RepDev* __restrict__ reps, int rep_begin, const double* __restrict__ rx, const double* __restrict__ ry,
     const double* __restrict__ rz, int* track, int* old, const int* sign, signed char* occ, long long occ_stride,
     const float* uniforms, int nu, const uint64_t* seeds, uint64_t sweep_id, const float* weights, int* consumed,
@@ -1207,16 +1212,26 @@ void launch_mh_sweep(hipStream_t st, const RepDev* reps, int rep_begin, int nrep
   k_mh_potential<<<g, MC_THREADS, 0, st>>>(reps, rep_begin, rx, ry, rz, track, sign, V);
   const bool lq = max_n <= MH_LDS_N && max_nq <= MH_LDS_Q;
   for (int a0 = 0; a0 < max_nq; a0 += MH_ROUND) {
-    if (lq)
-      k_mh_round<true><<<nrep, MC_THREADS, 0, st>>>(reps, rep_begin, rx, ry, rz, track, old, sign, occ, occ_stride,
-                                                    uniforms, nu, seeds, sweep_id, weights, consumed, log, feats,
-                                                    log_off, err, qdisp, nb_off, nb_list, hop_off, hop_list, V, (int4*)moves, nmoves,
-                                                    done, stop, a0);
+    if (lq && weights)
+      k_mh_round<true, MH_THREADS_LEARNED><<<nrep, MH_THREADS_LEARNED, 0, st>>>(
+          reps, rep_begin, rx, ry, rz, track, old, sign, occ, occ_stride, uniforms, nu, seeds, sweep_id, weights,
+          consumed, log, feats, log_off, err, qdisp, nb_off, nb_list, hop_off, hop_list, V, (int4*)moves, nmoves, done,
+          stop, a0);
+    else if (lq)
+      k_mh_round<true, MH_THREADS_UNIFORM><<<nrep, MH_THREADS_UNIFORM, 0, st>>>(
+          reps, rep_begin, rx, ry, rz, track, old, sign, occ, occ_stride, uniforms, nu, seeds, sweep_id, weights,
+          consumed, log, feats, log_off, err, qdisp, nb_off, nb_list, hop_off, hop_list, V, (int4*)moves, nmoves, done,
+          stop, a0);
+    else if (weights)
+      k_mh_round<false, MH_THREADS_LEARNED><<<nrep, MH_THREADS_LEARNED, 0, st>>>(
+          reps, rep_begin, rx, ry, rz, track, old, sign, occ, occ_stride, uniforms, nu, seeds, sweep_id, weights,
+          consumed, log, feats, log_off, err, qdisp, nb_off, nb_list, hop_off, hop_list, V, (int4*)moves, nmoves, done,
+          stop, a0);
     else
-      k_mh_round<false><<<nrep, MC_THREADS, 0, st>>>(reps, rep_begin, rx, ry, rz, track, old, sign, occ, occ_stride,
-                                                     uniforms, nu, seeds, sweep_id, weights, consumed, log, feats,
-                                                     log_off, err, qdisp, nb_off, nb_list, hop_off, hop_list, V, (int4*)moves, nmoves,
-                                                     done, stop, a0);
+      k_mh_round<false, MH_THREADS_UNIFORM><<<nrep, MH_THREADS_UNIFORM, 0, st>>>(
+          reps, rep_begin, rx, ry, rz, track, old, sign, occ, occ_stride, uniforms, nu, seeds, sweep_id, weights,
+          consumed, log, feats, log_off, err, qdisp, nb_off, nb_list, hop_off, hop_list, V, (int4*)moves, nmoves, done,
+          stop, a0);
     if (a0 + MH_ROUND < max_nq)
       k_mh_fold<<<g, MC_THREADS, 0, st>>>(reps, rep_begin, rx, ry, rz, (const int4*)moves, nmoves, V);
   }
