@@ -184,7 +184,7 @@ template <typename T, int P> __device__ __forceinline__ void bspline_weights(T x
 #pragma unroll
     for (int j = P - 1; j >= 0; --j) {
       T lo = (j >= 1) ? A[j - 1] : (T)0;
-      // one explicit fma: every kernel that evaluates the weights rounds them alike
+      // one explicit fma, so the rounding does not depend on how the compiler contracts the expression
       A[j] = div * fma(x + (T)j, A[j], ((T)k - x - (T)j) * lo);
     }
   }

@@ -575,15 +575,16 @@ __device__ __forceinline__ void spm_expand(const float4 w0, const float4 w1, int
 // product of a 64 x K matrix A[(x,y), b] = wx_b(x) wy_b(y) and a K x 24 matrix
 // B[b, (z,c)] = wz_b(z) v_b(c) over the K candidate beads of the brick.  One
 // workgroup per brick, four wavefronts; wavefront w owns the 16 rows x = 0..7,
-// y = 2w, 2w+1 and both 16-column tiles of B (n = 3z + c < 24, the rest zero),
-// and accumulates them with v_mfma_f32_16x16x4f32 over the beads four at a
-// time.  Beads are staged in chunks by four threads each (x weights, y
+// y = 2w, 2w+1 and two 16-column tiles of the product (n = 3z + c < 24; the
+// second tile's columns 24-31 read whatever follows a row of B and are never
+// stored), and accumulates them with v_mfma_f32_16x16x4f32 over the beads four
+// at a time.  Beads are staged in chunks by four threads each (x weights, y
 // weights, and columns 0-15 and 16-23 of B), which load the bead's base,
 // weights and vector and write its expanded rows to LDS (20 KB per workgroup, so
-// seven workgroups fit a CU and hide each other's staging latency).  The sum runs over the beads
-// in staging order with the matrix cores' fixed order, so the result does not
-// depend on scheduling; the output tile goes through LDS so the stores are runs
-// of 8 consecutive points as in the gather.
+// seven workgroups fit a CU and hide each other's staging latency).  The sum
+// runs over the beads in staging order with the matrix cores' fixed order, so the
+// result does not depend on scheduling; the output tile goes through LDS so the
+// stores are runs of 8 consecutive points as in the gather.
 #define SPM_CHUNK 128
 #define SPM_OUT_PITCH 65
 typedef float spm_f32x4 __attribute__((ext_vector_type(4)));
