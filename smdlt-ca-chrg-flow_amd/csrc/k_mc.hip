@@ -552,7 +552,7 @@ struct MhShared {
   int wcnt[MH_THREADS_MAX / 64];
   int nloc, ncand, k, x, qk, y, pick, accept, bad;
   float u[3];
-  double qf;
+  double qf, zq;
 };
 
 __device__ __forceinline__ double mh_dist(const RepDev& R, const double* rx, const double* ry, const double* rz, int a,
@@ -736,16 +736,27 @@ __device__ float mlp_score_wave(const float* w, const float* f) {
   return s + b3[0];
 }
 
-// proposal over the scored candidates (thread 0): the network's predicted
-// acceptance sigmoid(score) normalised (a Barker-type locally balanced weight),
-// mixed with the uniform proposal at weight eps
-__device__ void mh_proposal_probs(double* q, int nc, double eps) {
-  double z = 0.0;
-  for (int c = 0; c < nc; ++c) {
-    q[c] = 1.0 / (1.0 + exp(-q[c]));
-    z += q[c];
+// proposal over the scored candidates (wavefront 0; the caller synchronises before
+// and after): the network's predicted acceptance sigmoid(score) normalised (a
+// Barker-type locally balanced weight), mixed with the uniform proposal at weight
+// eps.  The elementwise parts run lane-parallel and the normaliser is summed by
+// lane 0 in candidate order, so every value equals a sequential evaluation.
+__device__ void mh_proposal_probs(double* q, int nc, double eps, double* zsh) {
+  const int lane = threadIdx.x & 63;
+  for (int c = lane; c < nc; c += 64) q[c] = 1.0 / (1.0 + exp(-q[c]));
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  if (lane == 0) {
+    double z = 0.0;
+    for (int c = 0; c < nc; ++c) z += q[c];
+    *zsh = z;
   }
-  for (int c = 0; c < nc; ++c) q[c] = (1.0 - eps) * q[c] / z + eps / nc;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const double z = *zsh;
+  for (int c = lane; c < nc; c += 64) q[c] = (1.0 - eps) * q[c] / z + eps / nc;
 }
 
 __device__ __forceinline__ double mh_phi(const RepDev& R, const double* rx, const double* ry, const double* rz, int a,
@@ -1035,7 +1046,7 @@ __global__ __launch_bounds__(NT) void k_mh_round(
         if (lane == 0) S.q[c] = (double)s;
       }
       __syncthreads();
-      if (tid == 0) mh_proposal_probs(S.q, nc, R.pfloor);
+      if (wv == 0) mh_proposal_probs(S.q, nc, R.pfloor, &S.zq);
     } else if (tid == 0) {
       for (int c = 0; c < nc; ++c) S.q[c] = 1.0 / nc;
     }
@@ -1131,7 +1142,7 @@ __global__ __launch_bounds__(NT) void k_mh_round(
         if (lane == 0) S.q[c] = (double)s;
       }
       __syncthreads();
-      if (tid == 0) mh_proposal_probs(S.q, nc2, R.pfloor);
+      if (wv == 0) mh_proposal_probs(S.q, nc2, R.pfloor, &S.zq);
     }
     __syncthreads();
     if (tid == 0) {
