@@ -545,6 +545,8 @@ struct MhShared {
   int nmv;
   int cand[MH_MAXC];
   double q[MH_MAXC];
+  double gdst[MH_MAXC][5];  // learned proposal: each candidate's geometry (mh_geom_dst)
+  double gsrc[4];           // and the source's (mh_geom_src)
   float feat[MH_MAXC][16];
   float fsel[16];
   float w[MLP_NP];
@@ -619,17 +621,29 @@ __device__ __forceinline__ double mh_rel_dist(double ax, double ay, double az, d
 __device__ void mh_features_rel(const RepDev& R, const double* rx, const double* ry, const double* rz,
                                 const signed char* qb, const int* loc, int nloc, int src, int dst, int qk, int clr,
                                 int set, int ncand, float* out, int anchor, const double* vx, const double* vy,
-                                const double* vz, const double* rs, const double* es) {
+                                const double* vz, const double* rs, const double* es, const double* geo_src = nullptr,
+                                const double* geo_dst = nullptr) {
   const int lane = threadIdx.x & 63;
   double e = 0, n4d = 0, n4s = 0, s4d = 0, s4s = 0, b4d = 0, b4s = 0;
   const double R4 = R.feat_r, kap = R.kappa_d, R4hi = R4 * R4 * (1.0 + 1e-9);
   const int sa = R.slot0 + anchor, ss = R.slot0 + src, sd = R.slot0 + dst;
-  double vs[3], vd[3];
-  ref_nid(R, rx[ss], ry[ss], rz[ss], rx[sa], ry[sa], rz[sa], vs[0], vs[1], vs[2]);
-  ref_nid(R, rx[sd], ry[sd], rz[sd], rx[sa], ry[sa], rz[sa], vd[0], vd[1], vd[2]);
-  // the pair's own geometry, ahead of the loop so that its loads overlap the above
-  const double f0 = mh_dist(R, rx, ry, rz, src, dst);
-  const double f10 = mh_cos_at(R, rx, ry, rz, dst), f11 = mh_cos_at(R, rx, ry, rz, src);
+  double vs[3], vd[3], f0, f10, f11;
+  if (geo_src && geo_dst) {  // precomputed by mh_geom_src / mh_geom_dst (the same arithmetic)
+    for (int d = 0; d < 3; ++d) {
+      vs[d] = geo_src[d];
+      vd[d] = geo_dst[d];
+    }
+    f11 = geo_src[3];
+    f0 = geo_dst[3];
+    f10 = geo_dst[4];
+  } else {
+    ref_nid(R, rx[ss], ry[ss], rz[ss], rx[sa], ry[sa], rz[sa], vs[0], vs[1], vs[2]);
+    ref_nid(R, rx[sd], ry[sd], rz[sd], rx[sa], ry[sa], rz[sa], vd[0], vd[1], vd[2]);
+    // the pair's own geometry, ahead of the loop so that its loads overlap the above
+    f0 = mh_dist(R, rx, ry, rz, src, dst);
+    f10 = mh_cos_at(R, rx, ry, rz, dst);
+    f11 = mh_cos_at(R, rx, ry, rz, src);
+  }
   for (int i = lane; i < nloc; i += 64) {
     const int m = loc[i];
     if (m == src || m == dst) continue;
@@ -705,6 +719,25 @@ __device__ void mh_features_rel(const RepDev& R, const double* rx, const double*
     out[14] = (float)qk;
     out[15] = (float)((double)ncand / 10.0);
   }
+}
+
+// The source side of a proposal's features: src relative to the anchor and the
+// bend cosine at src, as mh_features_rel computes them.
+__device__ __forceinline__ void mh_geom_src(const RepDev& R, const double* rx, const double* ry, const double* rz,
+                                            int src, int anchor, double* g) {
+  const int sa = R.slot0 + anchor, ss = R.slot0 + src;
+  ref_nid(R, rx[ss], ry[ss], rz[ss], rx[sa], ry[sa], rz[sa], g[0], g[1], g[2]);
+  g[3] = mh_cos_at(R, rx, ry, rz, src);
+}
+
+// One candidate's side: dst relative to the anchor, |src - dst| and the bend
+// cosine at dst, as mh_features_rel computes them.
+__device__ __forceinline__ void mh_geom_dst(const RepDev& R, const double* rx, const double* ry, const double* rz,
+                                            int src, int dst, int anchor, double* g) {
+  const int sa = R.slot0 + anchor, sd = R.slot0 + dst;
+  ref_nid(R, rx[sd], ry[sd], rz[sd], rx[sa], ry[sa], rz[sa], g[0], g[1], g[2]);
+  g[3] = mh_dist(R, rx, ry, rz, src, dst);
+  g[4] = mh_cos_at(R, rx, ry, rz, dst);
 }
 
 // The network's weights as the MH kernels keep them in LDS: W1 and W2
@@ -1035,11 +1068,14 @@ __global__ __launch_bounds__(NT) void k_mh_round(
           S.rs[i] = rsm;
           S.es[i] = exp(-R.kappa_d * rsm) / rsm;
         }
-        __syncthreads();
       }
+      // the candidates' own geometry, thread-parallel, ahead of the wavefront-per-candidate sums
+      for (int c = tid; c < nc; c += blockDim.x) mh_geom_dst(R, rx, ry, rz, x, S.cand[c], x, S.gdst[c]);
+      if (tid == blockDim.x - 1) mh_geom_src(R, rx, ry, rz, x, x, S.gsrc);
+      __syncthreads();
       for (int c = wv; c < nc; c += nwv) {
         mh_features_rel(R, rx, ry, rz, qb, S.loc, nloc, x, S.cand[c], qk, -1, -1, nc, S.feat[c], x, VX, VY, VZ,
-                        src_lds ? S.rs : nullptr, src_lds ? S.es : nullptr);
+                        src_lds ? S.rs : nullptr, src_lds ? S.es : nullptr, S.gsrc, S.gdst[c]);
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
         const float s = mlp_score_wave(S.w, S.feat[c]);
@@ -1131,11 +1167,13 @@ __global__ __launch_bounds__(NT) void k_mh_round(
           S.rs[i] = rsm;
           S.es[i] = exp(-R.kappa_d * rsm) / rsm;
         }
-        __syncthreads();
       }
+      for (int c = tid; c < nc2; c += blockDim.x) mh_geom_dst(R, rx, ry, rz, y, S.cand[c], x, S.gdst[c]);
+      if (tid == blockDim.x - 1) mh_geom_src(R, rx, ry, rz, y, x, S.gsrc);
+      __syncthreads();
       for (int c = wv; c < nc2; c += nwv) {
         mh_features_rel(R, rx, ry, rz, qb, S.loc, nloc, y, S.cand[c], qk, x, y, nc2, S.feat[c], x, VX, VY, VZ,
-                        src_lds ? S.rs : nullptr, src_lds ? S.es : nullptr);
+                        src_lds ? S.rs : nullptr, src_lds ? S.es : nullptr, S.gsrc, S.gdst[c]);
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
         const float s = mlp_score_wave(S.w, S.feat[c]);
