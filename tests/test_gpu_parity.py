@@ -260,21 +260,23 @@ def test_mobility_long_cutoff_tables_match_fp64(rc):
     assert rel(mixed.mobility_apply(v), fp64.mobility_apply(v)) < 1e-5
 
 
-@pytest.mark.parametrize("order", [4, 6, 8])
+@pytest.mark.parametrize("order,grid", [(4, (48, 40, 32)), (6, (48, 40, 32)), (8, (48, 40, 32)),
+                                        (8, (16, 24, 40)), (6, (40, 16, 16))])
 @pytest.mark.parametrize("flow", [1, 2])
-def test_matrix_core_spread_matches_fp64_gather(flow, order):
+def test_matrix_core_spread_matches_fp64_gather(flow, order, grid):
     # the mixed path spreads on the matrix cores in FP32, the FP64 path gathers per
     # grid point (and equals the PME restatement to 1e-12); at the same split, grid
     # and spline order the two products agree to the mixed path's FP32 precision
-    kw = dict(grid=(48, 40, 32), spline_order=order, rc_real=8.0)
+    # (grids with an axis of two bricks make each brick's +1 neighbour wrap around)
+    kw = dict(grid=grid, spline_order=order, rc_real=8.0)
     mixed, lat, pos = mob_case(flow, nb=60, nc=4, **kw)
     fp64, _, _ = mob_case(flow, nb=60, nc=4, prec=S.PREC_FP64, **kw)
     cm, cf = mixed.mobility_config(), fp64.mobility_config()
-    assert cm["order"] == cf["order"] == order and list(cm["grid"]) == list(cf["grid"]) == [48, 40, 32]
+    assert cm["order"] == cf["order"] == order and list(cm["grid"]) == list(cf["grid"]) == list(grid)
     assert cm["alpha"] == cf["alpha"]
     v = np.random.default_rng(21).normal(size=pos.size)
     e = rel(mixed.mobility_apply(v), fp64.mobility_apply(v))
-    print(f"mixed (matrix-core spread) vs FP64 product, flow {flow}, p = {order}: {e:.2e}")
+    print(f"mixed (matrix-core spread) vs FP64 product, flow {flow}, p = {order}, grid {grid}: {e:.2e}")
     assert e < 1e-5
 
 
