@@ -578,10 +578,10 @@ __device__ __forceinline__ void spm_expand(const float4 w0, const float4 w1, int
 // y = 2w, 2w+1 and two 16-column tiles of the product (n = 3z + c < 24; the
 // second tile's columns 24-31 read whatever follows a row of B and are never
 // stored), and accumulates them with v_mfma_f32_16x16x4f32 over the beads four
-// at a time.  Beads are staged in chunks by four threads each (x weights, y
-// weights, and columns 0-15 and 16-23 of B), which load the bead's base,
-// weights and vector and write its expanded rows to LDS (20 KB per workgroup, so
-// seven workgroups fit a CU and hide each other's staging latency).  The sum
+// at a time.  Beads are staged in chunks by two threads each (x and y weights;
+// the row of B), which load the bead's base, weights and vector and write its
+// expanded rows to LDS (20 KB per workgroup, so seven workgroups fit a CU and
+// hide each other's staging latency).  The sum
 // runs over the beads in staging order with the matrix cores' fixed order, so the
 // result does not depend on scheduling; the output tile goes through LDS so the
 // stores are runs of 8 consecutive points as in the gather.
@@ -596,7 +596,7 @@ __global__ __launch_bounds__(256) void k_spread_mfma(MobArgs a) {
   // for the output tile
   __shared__ float4 sB[SPM_CHUNK * 6 + 2];
   static_assert(sizeof(sB) >= 3 * BRICK * SPM_OUT_PITCH * sizeof(float), "output tile fits the B stage");
-  static_assert(SPM_CHUNK == 128, "the staging passes map 128 beads to half a workgroup");
+  static_assert(SPM_CHUNK == 128, "staging maps the chunk's 128 beads to each half of the workgroup");
   // XCD-aware order: each XCD takes a contiguous range of bricks, so the neighbours
   // that share a bead's data, and that complete each other's cache lines of the
   // grid, run under one L2
@@ -607,7 +607,7 @@ __global__ __launch_bounds__(256) void k_spread_mfma(MobArgs a) {
   if (!R.active || (a.done && a.done[bt.rep])) return;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int K0 = R.K[0], K1 = R.K[1], K2 = R.K[2];
-  const int lo[3] = {bt.b0 * BRICK, bt.b1 * BRICK, bt.b2 * BRICK}, Kd[3] = {K0, K1, K2};
+  const int lo[3] = {bt.b0 * BRICK, bt.b1 * BRICK, bt.b2 * BRICK};
   const bool fs = a.scale_in_spread != 0;
   const double* vin = (a.vout && !fs) ? a.vout : a.v;
   const double vsc = (fs && a.beta) ? (a.beta[bt.rep] > 0.0 ? 1.0 / a.beta[bt.rep] : 0.0) : 1.0;
@@ -620,69 +620,68 @@ __global__ __launch_bounds__(256) void k_spread_mfma(MobArgs a) {
   for (int k0 = 0; k0 < total; k0 += SPM_CHUNK) {
     const int m = min(SPM_CHUNK, total - k0), mp = (m + 15) & ~15;
     if (k0 > 0) __syncthreads();  // the previous chunk's products have read the stage
-    // part 0: x weights, 1: y weights, 2 and 3: the column tiles of B, in two passes:
-    // wavefronts 0-1 take parts 0 then 1, wavefronts 2-3 parts 2 then 3; rows
-    // m <= k < mp (padding to whole groups of 16) are zero
-    for (int pass = 0; pass < 2; ++pass) {
-      const int part = 2 * (tid >> 7) + pass, k = tid & 127;
-      if (k >= mp) continue;
-      float o[8];
-      float4* dst = part < 2 ? &sE[k][2 * part] : &sB[6 * k + 4 * (part - 2)];
-      if (k >= m) {
+    // bead k = tid % 128 of the chunk: wavefronts 0-1 stage its x and y weights,
+    // wavefronts 2-3 its row of B (z weights times the vector); each thread issues
+    // all its loads at once; rows m <= k < mp (padding to whole groups of 16) are zero
+    {
+      const int k = tid & 127, half = tid >> 7;
+      if (k < mp && k >= m) {
         const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
-        dst[0] = zero;
-        dst[1] = zero;
-        if (part == 2) {
-          dst[2] = zero;
-          dst[3] = zero;
-        }
-        continue;
-      }
-      const int kk = k0 + k, s = a.bcand[cb + kk], d = part < 2 ? part : 2;
-      int e = a.bbase[3 * s + d] - lo[d];
-      e += e < 0 ? Kd[d] : 0;
-      const float4* wp = (const float4*)(bw + (size_t)BW_STRIDE * s + d * MAX_ORDER);
-      const float4 w0 = wp[0], w1 = wp[1];
-      if (part < 2) {
-        spm_expand<P>(w0, w1, e, o);
-        dst[0] = make_float4(o[0], o[1], o[2], o[3]);
-        dst[1] = make_float4(o[4], o[5], o[6], o[7]);
-        continue;
-      }
-      float v0, v1, v2;
-      if (fs) {
-        const double x0 = vin[3 * s] * vsc, x1 = vin[3 * s + 1] * vsc, x2 = vin[3 * s + 2] * vsc;
-        v0 = (float)x0;
-        v1 = (float)x1;
-        v2 = (float)x2;
-        if (part == 2 && kk < nhome) {  // every bead has one home brick: it stores the scaled input once
-          if (a.vout) {
-            a.vout[3 * s] = x0;
-            a.vout[3 * s + 1] = x1;
-            a.vout[3 * s + 2] = x2;
+        float4* dst = half == 0 ? sE[k] : &sB[6 * k];
+        for (int q = 0; q < (half == 0 ? 4 : 6); ++q) dst[q] = zero;
+      } else if (k < m) {
+        const int kk = k0 + k, s = a.bcand[cb + kk];
+        const float4* wp = (const float4*)(bw + (size_t)BW_STRIDE * s);
+        float o[8];
+        if (half == 0) {
+          int ex = a.bbase[3 * s] - lo[0], ey = a.bbase[3 * s + 1] - lo[1];
+          const float4 x0 = wp[0], x1 = wp[1], y0 = wp[2], y1 = wp[3];
+          ex += ex < 0 ? K0 : 0;
+          ey += ey < 0 ? K1 : 0;
+          spm_expand<P>(x0, x1, ex, o);
+          sE[k][0] = make_float4(o[0], o[1], o[2], o[3]);
+          sE[k][1] = make_float4(o[4], o[5], o[6], o[7]);
+          spm_expand<P>(y0, y1, ey, o);
+          sE[k][2] = make_float4(o[0], o[1], o[2], o[3]);
+          sE[k][3] = make_float4(o[4], o[5], o[6], o[7]);
+        } else {
+          int ez = a.bbase[3 * s + 2] - lo[2];
+          const float4 z0 = wp[2 * MAX_ORDER / 4], z1 = wp[2 * MAX_ORDER / 4 + 1];
+          ez += ez < 0 ? K2 : 0;
+          float v0, v1, v2;
+          if (fs) {
+            const double x0 = vin[3 * s] * vsc, x1 = vin[3 * s + 1] * vsc, x2 = vin[3 * s + 2] * vsc;
+            v0 = (float)x0;
+            v1 = (float)x1;
+            v2 = (float)x2;
+            if (kk < nhome) {  // every bead has one home brick: it stores the scaled input once
+              if (a.vout) {
+                a.vout[3 * s] = x0;
+                a.vout[3 * s + 1] = x1;
+                a.vout[3 * s + 2] = x2;
+              }
+              if (a.v32) a.v32[s] = make_float4(v0, v1, v2, 0.f);
+            }
+          } else if (a.v32) {
+            const float4 v4 = a.v32[s];
+            v0 = v4.x;
+            v1 = v4.y;
+            v2 = v4.z;
+          } else {
+            v0 = (float)vin[3 * s];
+            v1 = (float)vin[3 * s + 1];
+            v2 = (float)vin[3 * s + 2];
           }
-          if (a.v32) a.v32[s] = make_float4(v0, v1, v2, 0.f);
+          spm_expand<P>(z0, z1, ez, o);
+          // row of B: n = 3z + c -> o[z] * v[c], n = 0..23
+          float4* dst = &sB[6 * k];
+          dst[0] = make_float4(o[0] * v0, o[0] * v1, o[0] * v2, o[1] * v0);
+          dst[1] = make_float4(o[1] * v1, o[1] * v2, o[2] * v0, o[2] * v1);
+          dst[2] = make_float4(o[2] * v2, o[3] * v0, o[3] * v1, o[3] * v2);
+          dst[3] = make_float4(o[4] * v0, o[4] * v1, o[4] * v2, o[5] * v0);
+          dst[4] = make_float4(o[5] * v1, o[5] * v2, o[6] * v0, o[6] * v1);
+          dst[5] = make_float4(o[6] * v2, o[7] * v0, o[7] * v1, o[7] * v2);
         }
-      } else if (a.v32) {
-        const float4 v4 = a.v32[s];
-        v0 = v4.x;
-        v1 = v4.y;
-        v2 = v4.z;
-      } else {
-        v0 = (float)vin[3 * s];
-        v1 = (float)vin[3 * s + 1];
-        v2 = (float)vin[3 * s + 2];
-      }
-      spm_expand<P>(w0, w1, e, o);
-      // row of B: n = 3z + c -> o[z] * v[c]; part 2 holds n = 0..15, part 3 n = 16..23 then zeros
-      if (part == 2) {
-        dst[0] = make_float4(o[0] * v0, o[0] * v1, o[0] * v2, o[1] * v0);
-        dst[1] = make_float4(o[1] * v1, o[1] * v2, o[2] * v0, o[2] * v1);
-        dst[2] = make_float4(o[2] * v2, o[3] * v0, o[3] * v1, o[3] * v2);
-        dst[3] = make_float4(o[4] * v0, o[4] * v1, o[4] * v2, o[5] * v0);
-      } else {
-        dst[0] = make_float4(o[5] * v1, o[5] * v2, o[6] * v0, o[6] * v1);
-        dst[1] = make_float4(o[6] * v2, o[7] * v0, o[7] * v1, o[7] * v2);
       }
     }
     __syncthreads();
