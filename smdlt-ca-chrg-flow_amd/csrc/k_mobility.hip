@@ -1002,9 +1002,9 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_z_green(MobArgs a) {
     T kz = (T)m3 * (T)R.kb[3];
     T kk = kx * kx + ky * ky + kz * kz;
     T q = kk * ia2;
-    T gsc = ((T)1 - kk / (T)3) * ((T)1 + q + (T)2 * q * q) * ((T)18.849555921538759 / kk) * exp(-q) * invV;
+    T ikk = (T)1 / kk;  // the one division per wave vector
+    T gsc = ((T)1 - kk * (T)(1.0 / 3.0)) * ((T)1 + q + (T)2 * q * q) * ((T)18.849555921538759 * ikk) * exp(-q) * invV;
     gsc *= bsp[x] * bsp[K0 + y] * bsp[K0 + K1 + k];
-    T ikk = (T)1 / kk;
     C v0 = *p0, v1 = *p1, v2 = *p2;
     T dr = (kx * v0.x + ky * v1.x + kz * v2.x) * ikk;
     T di = (kx * v0.y + ky * v1.y + kz * v2.y) * ikk;
