@@ -1283,12 +1283,14 @@ __global__ __launch_bounds__(64 * CL_WAVES) __attribute__((amdgpu_waves_per_eu(C
   // entry each and compute its image shift: the minimum image of the vector
   // between the two clusters' first beads.  The pair loop (wave-uniform: it
   // starts from readfirstlane) broadcasts an entry and its shift from the lane
-  // holding them with readlane, so it issues no list loads.  Positions are cluster-consistent, so
-  // when that vector's length plus both cluster radii is below half the shortest
-  // lattice vector every bead pair of the two clusters has that shift as its
-  // minimum image (any shorter image would need a lattice vector shorter than
-  // twice the bound), and the lanes skip their own minimum-image reduction; the
-  // entry's sign bit marks the pairs that may not (first slots fit 27 bits).
+  // holding them with readlane, so it issues no list loads.  Positions are cluster-consistent:
+  // a bead pair's vector under the shift is at most s = (that vector's length plus
+  // both cluster radii) long, and every other image of it at least 2h - s, h being
+  // half the shortest lattice vector.  When s <= 2h - rc no other image lies within
+  // the cut-off, and a vector shorter than rc <= h is its own minimum image, so the
+  // shift is exact for every pair the sum takes and the lanes skip their own
+  // minimum-image reduction; the entry's sign bit marks the cluster pairs beyond
+  // that bound (first slots fit 27 bits).
   // Lanes past the partner cluster's end load the slots that follow it (zeros
   // past the array, through the buffer's range check) and select the table's
   // zero entry, as do pairs beyond rc and the bead with itself, so the sums take
@@ -1303,7 +1305,8 @@ __global__ __launch_bounds__(64 * CL_WAVES) __attribute__((amdgpu_waves_per_eu(C
   const float rb0 = (float)R.rb[0], rb1 = (float)R.rb[1], rb2 = (float)R.rb[2], rb3 = (float)R.rb[3];
   const float Lz = (float)R.L, iLz = (float)R.invL, rcf = (float)R.rc2;
   const float alpha = (float)R.alpha;
-  const float halfw = (float)R.halfw * 0.999f - 1e-3f;  // margin for the FP32 arithmetic of the test
+  // the bound 2h - rc on s, less a margin for the FP32 arithmetic of the test
+  const float smax = (float)(2.0 * R.halfw - sqrt(R.rc2)) * 0.999f - 1e-3f;
   CompSum<T> acc[3];
   const int m = a.nclp[I];
   const int* row = a.lcl + (size_t)I * a.capc;
@@ -1338,7 +1341,7 @@ __global__ __launch_bounds__(64 * CL_WAVES) __attribute__((amdgpu_waves_per_eu(C
         sh_x = sx;
         sh_y = sy;
         sh_z = sz;
-        if (!(reach < halfw)) e_lane |= (int)0x80000000u;
+        if (!(reach < smax)) e_lane |= (int)0x80000000u;
       }
     }
     const int c1 = min(m, c0 + 64 * CL_WAVES);

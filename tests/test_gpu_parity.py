@@ -260,6 +260,63 @@ def test_mobility_long_cutoff_tables_match_fp64(rc):
     assert rel(mixed.mobility_apply(v), fp64.mobility_apply(v)) < 1e-5
 
 
+def _images(cell, n=2):
+    r = np.arange(-n, n + 1)
+    i, j, k = [a.ravel() for a in np.meshgrid(r, r, r, indexing="ij")]
+    return np.stack([i * cell.a1x + j * cell.a2x, i * cell.a1y + j * cell.a2y, k * cell.Lz], 1)
+
+
+def _half_width(cell):
+    """Half the shortest lattice vector bound the cluster pass uses: half the smaller
+    of the reduced 2D cell's smallest width and Lz."""
+    b1, b2 = np.array([cell.a1x, cell.a1y]), np.array([cell.a2x, cell.a2y])
+    while True:  # Lagrange reduction
+        if b1 @ b1 > b2 @ b2:
+            b1, b2 = b2, b1
+        m = np.rint((b1 @ b2) / (b1 @ b1))
+        if m == 0:
+            break
+        b2 = b2 - m * b1
+    area = abs(b1[0] * b2[1] - b1[1] * b2[0])
+    return 0.5 * min(area / np.linalg.norm(b2), cell.Lz)
+
+
+@pytest.mark.parametrize("flow,rc", [(1, 6.0), (1, 8.0), (2, 6.0), (2, 8.0)])
+def test_cluster_image_shift_paths_match_dense_ewald(flow, rc):
+    # The cluster pass sums a cluster pair under one image shift when the first
+    # beads' minimum-image distance plus both cluster radii is at most 2h - rc
+    # (h: half the shortest lattice vector), and takes each pair's own minimum
+    # image otherwise. This box holds cluster pairs, with a bead pair within rc,
+    # on both sides of that bound; M.F against the dense FP64 Ewald sum.
+    nb, nc, L = 40, 3, 24.0
+    ctx, lat, pos = mob_case(flow, L=L, nb=nb, nc=nc, rc_real=rc)
+    cell = O.cell_of(lat)
+    rce = ctx.mobility_config()["rc"]
+    img = _images(cell)
+    first = [(ch * nb + j0, min(8, nb - j0)) for ch in range(nc) for j0 in range(0, nb, 8)]
+
+    def mi(d):  # minimum-image lengths of vectors d (..., 3)
+        return np.linalg.norm(d[..., None, :] + img, axis=-1).min(-1)
+
+    # cluster radius: the largest minimum-image distance from the cluster's first bead
+    rad = [mi(pos[f:f + k] - pos[f]).max() for f, k in first]
+    smax = 2 * _half_width(cell) - rce
+    n_one = n_own = 0
+    for I, (fi, ki) in enumerate(first):
+        for J, (fj, kj) in enumerate(first):
+            if mi(pos[fj:fj + kj, None, :] - pos[None, fi:fi + ki, :]).min() >= rce:
+                continue  # no bead pair within the cut-off
+            if mi(pos[fj] - pos[fi]) + rad[I] + rad[J] < smax:
+                n_one += 1
+            else:
+                n_own += 1
+    assert n_one > 0 and n_own > 0, (n_one, n_own)
+    Md = O.ewald_dense(cell, pos, 0.5)
+    for seed in (4, 5):
+        v = np.random.default_rng(seed).normal(size=pos.size)
+        assert rel(ctx.mobility_apply(v), Md @ v) < 1e-5
+
+
 @pytest.mark.parametrize("order,grid", [(4, (48, 40, 32)), (6, (48, 40, 32)), (8, (48, 40, 32)),
                                         (8, (16, 24, 40)), (6, (40, 16, 16))])
 @pytest.mark.parametrize("flow", [1, 2])
