@@ -233,8 +233,9 @@ static const char* kStageNames[ST_COUNT] = {
 
 struct Timer {
   hipEvent_t a, b;
-  int stage;
+  int stage;          // -1: events only, nothing accumulated
   bool own_a = true;  // return a to the pool after reading (shared events are returned once)
+  int calls = 1;      // calls counted (a pass split over FFT chunks counts once)
 };
 
 struct smdlt_ctx {
@@ -279,6 +280,8 @@ struct smdlt_ctx {
   uint64_t pos_version = 1, prep_version = 0;
   long long prep_range = -1;
   std::vector<int> tx0, ty0, tz0, txy0;  // per-replica task ranges (size nrep+1)
+  std::vector<int> fch_rep;  // first replicas of the FFT chunks after the first (FFT_CHUNK_BYTES)
+  std::vector<int> fch_buf;  // the chunk table of the last mob_args call (MobArgs::fch)
   // lanczos
   int maxit = 30;
   double *V = nullptr, *W = nullptr, *partial = nullptr, *partial2 = nullptr, *scal = nullptr, *bdiv = nullptr;
@@ -401,8 +404,10 @@ static void collect_timers(smdlt_ctx* c) {
   for (auto& t : c->timers) {
     float ms = 0;
     hipEventElapsedTime(&ms, t.a, t.b);
-    c->stage_ms[t.stage] += ms;
-    c->stage_cnt[t.stage] += 1;
+    if (t.stage >= 0) {
+      c->stage_ms[t.stage] += ms;
+      c->stage_cnt[t.stage] += t.calls;
+    }
     if (t.own_a) c->pool.push_back(t.a);
     c->pool.push_back(t.b);
   }
@@ -1070,6 +1075,18 @@ int smdlt_create(const smdlt_params* params, int n_replicas, int device, smdlt_c
       int zl = K1 * Kh;
       for (int l0 = 0; l0 < zl; l0 += nlz) vz.push_back({r, -1, l0, std::min(nlz, zl - l0)});
     }
+    // FFT chunks: consecutive replicas with at most FFT_CHUNK_BYTES of grid (a
+    // larger replica is a chunk of its own)
+    c->fch_rep.clear();
+    double acc = 0;
+    for (int r = 0; r < n_replicas; ++r) {
+      const double gb = (fp64 ? 24.0 : 12.0) * c->rh[r].K[0] * c->rh[r].K[1] * c->rh[r].K[2];
+      if (acc > 0 && acc + gb > FFT_CHUNK_BYTES) {
+        c->fch_rep.push_back(r);
+        acc = 0;
+      }
+      acc += gb;
+    }
     c->tx0[n_replicas] = (int)vx.size();
     c->ty0[n_replicas] = (int)vy.size();
     c->tz0[n_replicas] = (int)vz.size();
@@ -1708,28 +1725,56 @@ static MobArgs mob_args(smdlt_ctx* c, int rb, int re, const double* v, double* u
   m.ncl = b.cl0 + b.ncl - a.cl0;
   m.ntab_max = 0;
   for (int r = rb; r < re; ++r) m.ntab_max = std::max(m.ntab_max, c->rh[r].ntab);
+  c->fch_buf.clear();
+  auto bound = [&](int r) {
+    c->fch_buf.push_back(c->tx0[r] - c->tx0[rb]);
+    c->fch_buf.push_back(c->ty0[r] - c->ty0[rb]);
+    c->fch_buf.push_back(c->tz0[r] - c->tz0[rb]);
+    c->fch_buf.push_back(c->txy0[r] - c->txy0[rb]);
+  };
+  bound(rb);
+  for (int r : c->fch_rep)
+    if (r > rb && r < re) bound(r);
+  bound(re);
+  m.fch = c->fch_buf.data();
+  m.nfch = (int)c->fch_buf.size() / 4 - 1;
   return m;
 }
 
 static void mob_launch(smdlt_ctx* c, MobArgs& m) {
   hipEvent_t ev[9];
+  std::vector<hipEvent_t> evc;
+  const int nch = m.nfch > 1 ? m.nfch : 1;
   m.ev = nullptr;
+  m.evc = nullptr;
   if (c->prof) {
     for (int k = 0; k < 9; ++k) ev[k] = get_event(c);
     m.ev = ev;
+    if (nch > 1) {
+      evc.resize(6 * (size_t)nch);
+      for (auto& e : evc) e = get_event(c);
+      m.evc = evc.data();
+    }
   }
   if (c->prec == SMDLT_PREC_FP64)
     launch_mobility<double>(c->stream, m);
   else
     launch_mobility<float>(c->stream, m);
-  if (m.ev)  // kernel k runs between ev[k] and ev[k+1]; each event is returned once
-    for (int k = 0; k < 8; ++k) c->timers.push_back({ev[k], ev[k + 1], ST_K_ZERO + k, k == 0});
+  if (m.ev) {  // kernel k runs between ev[k] and ev[k+1]; each event is returned once
+    for (int k = 0; k < 8; ++k) {
+      const bool fft = k >= 2 && k <= 6;  // with FFT chunks, the chunks' own events time the passes
+      c->timers.push_back({ev[k], ev[k + 1], (nch > 1 && fft) ? -1 : ST_K_ZERO + k, k == 0});
+    }
+    for (int q = 0; q < (int)evc.size() / 6; ++q)
+      for (int k = 0; k < 5; ++k)
+        c->timers.push_back({evc[6 * q + k], evc[6 * q + k + 1], ST_K_ZERO + 2 + k, k == 0, q == 0 ? 1 : 0});
+  }
 }
 
-// One launch sequence for every replica of the range.  (Processing replicas in
-// groups whose grids fit the 256 MB last-level cache was measured slower on
-// 64 replicas, 380k beads: the smaller launches lose more to tails than the
-// grid passes gain from the cache.)
+// One launch sequence for every replica of the range, the grid passes in FFT
+// chunks.  (Running the whole sequence in replica groups whose grids fit the
+// 256 MB last-level cache was measured slower on 64 replicas, 380k beads: the
+// smaller launches lose more to tails than the grid passes gain from the cache.)
 static void mobility_internal(smdlt_ctx* c, const double* v, double* u, const double* beta, double* vout,
                               double* partial, const int* done) {
   StageScope sc(c, ST_MOB);

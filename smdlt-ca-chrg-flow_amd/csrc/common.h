@@ -31,6 +31,12 @@
 #define XY_ELEMS 8192
 #endif
 #define XY_MAX_POINTS 8.0e6  // fused only while the context's grids total at most this many points
+// The five FFT passes run replica chunk by chunk (x, y, z, y, x for one chunk,
+// then the next) with chunks of at most this many grid bytes, so a chunk's grid
+// stays in the 256 MB last-level cache between its passes
+#ifndef FFT_CHUNK_BYTES
+#define FFT_CHUNK_BYTES (256.0 * 1048576.0)
+#endif
 #define FFT_THREADS 256
 #define MAX_ORDER 8         // largest B-spline order
 #define LZ_MAXIT 128        // largest Lanczos iteration count supported
@@ -370,6 +376,11 @@ struct MobArgs {
   const int *cl_first, *cl_cnt, *lcl, *nclp;
   int capc, cl_begin, ncl;
   int ntab_max;             // largest coefficient table of the range (entries of 2 float4)
+  // host memory, read by the launcher only: FFT task offsets (x, y, z, x-y) of the
+  // nfch + 1 chunk boundaries, relative to tx, ty, tz, txy
+  const int* fch;
+  int nfch;
+  hipEvent_t* evc;          // profiling with several chunks: 6 events per chunk around its 5 passes
 };
 #define RTAB_LDS_MAX 1280   // tables up to this many entries (40 KB) are staged in LDS
 #define CL_LIST_PAD 8       // entries past the last cluster-pair row, so a row's 4-entry groups read in bounds

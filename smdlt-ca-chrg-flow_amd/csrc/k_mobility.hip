@@ -1506,22 +1506,56 @@ template <typename T, int P> static void launch_mob_p(hipStream_t st, const MobA
       k_spread_gather<T, P><<<a.nbricks, 512, 0, st>>>(as);
   }
   mark(2);
-  if (a.ntxy > 0) k_fft_xy<T><<<a.ntxy, XY_THREADS, 0, st>>>(a, -1);
-  if (a.ntx > 0) k_fft_x<T><<<a.ntx, FFT_THREADS, 0, st>>>(a, -1);
-  mark(3);
-  if (a.nty > 0) k_fft_y<T><<<a.nty, FFT_THREADS, 0, st>>>(a, -1);
-  mark(4);
-  if (a.ntz > 0 && a.ztile == FFT_TILE_Z_LARGE)
-    k_fft_z_green<T, FFT_TILE_Z_LARGE><<<a.ntz, FFT_THREADS, 0, st>>>(a);
-  else if (a.ntz > 0 && a.ztile == FFT_TILE_Z)
-    k_fft_z_green<T, FFT_TILE_Z><<<a.ntz, FFT_THREADS, 0, st>>>(a);
-  else if (a.ntz > 0)
-    k_fft_z_green<T, FFT_TILE_Z_SMALL><<<a.ntz, FFT_THREADS, 0, st>>>(a);
-  mark(5);
-  if (a.nty > 0) k_fft_y<T><<<a.nty, FFT_THREADS, 0, st>>>(a, 1);
-  mark(6);
-  if (a.ntxy > 0) k_fft_xy<T><<<a.ntxy, XY_THREADS, 0, st>>>(a, 1);
-  if (a.ntx > 0) k_fft_x<T><<<a.ntx, FFT_THREADS, 0, st>>>(a, 1);
+  // The grid passes, chunk by chunk over the replicas (one chunk: the whole range).
+  // One chunk is timed by the marks 3-6 between its passes; several chunks each
+  // record six events of their own around their five passes (evc), and the marks
+  // 3-6 follow the last chunk.
+  const int nch = a.nfch > 1 ? a.nfch : 1;
+  for (int q = 0; q < nch; ++q) {
+    auto pmark = [&](int k) {
+      if (nch == 1) {
+        if (k >= 1 && k <= 4) mark(2 + k);
+      } else if (a.evc) {
+        (void)hipEventRecord(a.evc[6 * q + k], st);
+      }
+    };
+    MobArgs f = a;
+    if (nch > 1) {
+      const int *b = a.fch + 4 * q, *e = b + 4;
+      f.tx = a.tx + b[0];
+      f.ty = a.ty + b[1];
+      f.tz = a.tz + b[2];
+      f.txy = a.txy + b[3];
+      f.ntx = e[0] - b[0];
+      f.nty = e[1] - b[1];
+      f.ntz = e[2] - b[2];
+      f.ntxy = e[3] - b[3];
+    }
+    pmark(0);
+    if (f.ntxy > 0) k_fft_xy<T><<<f.ntxy, XY_THREADS, 0, st>>>(f, -1);
+    if (f.ntx > 0) k_fft_x<T><<<f.ntx, FFT_THREADS, 0, st>>>(f, -1);
+    pmark(1);
+    if (f.nty > 0) k_fft_y<T><<<f.nty, FFT_THREADS, 0, st>>>(f, -1);
+    pmark(2);
+    if (f.ntz > 0 && f.ztile == FFT_TILE_Z_LARGE)
+      k_fft_z_green<T, FFT_TILE_Z_LARGE><<<f.ntz, FFT_THREADS, 0, st>>>(f);
+    else if (f.ntz > 0 && f.ztile == FFT_TILE_Z)
+      k_fft_z_green<T, FFT_TILE_Z><<<f.ntz, FFT_THREADS, 0, st>>>(f);
+    else if (f.ntz > 0)
+      k_fft_z_green<T, FFT_TILE_Z_SMALL><<<f.ntz, FFT_THREADS, 0, st>>>(f);
+    pmark(3);
+    if (f.nty > 0) k_fft_y<T><<<f.nty, FFT_THREADS, 0, st>>>(f, 1);
+    pmark(4);
+    if (f.ntxy > 0) k_fft_xy<T><<<f.ntxy, XY_THREADS, 0, st>>>(f, 1);
+    if (f.ntx > 0) k_fft_x<T><<<f.ntx, FFT_THREADS, 0, st>>>(f, 1);
+    pmark(5);
+  }
+  if (nch > 1) {
+    mark(3);
+    mark(4);
+    mark(5);
+    mark(6);
+  }
   mark(7);
   if (sizeof(T) == 4 && a.lcl) {
     if (a.ncl > 0 && a.ntab_max <= RTAB_LDS_MAX)

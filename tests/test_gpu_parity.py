@@ -698,20 +698,28 @@ def test_batched_replicas_equal_single_contexts():
 
 
 def test_large_grid_replicas_equal_single_contexts():
-    # four mixed-precision replicas on 128^3 grids (100 MB of grids in one launch
-    # sequence) give exactly the products of one context per replica
+    # four mixed-precision replicas on 192^3 grids (340 MB of grids in one launch
+    # sequence, so the grid passes run in two chunks of at most 256 MB: the first
+    # three replicas, then the fourth) give exactly the products of one context
+    # per replica
     specs = [(1, 40.0, 50, 4), (2, 44.0, 40, 5), (1, 36.0, 60, 3), (0, 40.0, 30, 6)]
     params, poss = [], []
     for k, (flow, L, nb, nc) in enumerate(specs):
         params.append(S.default_params(nb=nb, nc=nc, L=L, flow_type=flow, edot=0.4, dt=1e-3,
-                                       precision=S.PREC_MIXED, rc_real=8.0, grid=(128, 128, 128), spline_order=6))
+                                       precision=S.PREC_MIXED, rc_real=8.0, grid=(192, 192, 192), spline_order=6))
         poss.append(random_walk(nb, nc, L, 40 + k))
     big = S.Context(params)
     allp = np.concatenate(poss)
     big.upload_positions(*xyz(allp))
     big.remap(60)
     v = np.random.default_rng(1).normal(size=3 * len(allp))
+    big.set_profiling(True)
     ub = big.mobility_apply(v)
+    st = big.stage_times()
+    big.set_profiling(False)
+    # each grid pass is timed once per product, over both chunks
+    assert all(st[k][1] == 1 for k in ("k_fft_x(fwd)", "k_fft_y(fwd)", "k_fft_z_green", "k_fft_y(inv)",
+                                        "k_fft_x(inv)", "k_mob_final"))
     off = 0
     for k, prm in enumerate(params):
         one = S.Context(prm)
