@@ -1048,9 +1048,10 @@ int smdlt_create(const smdlt_params* params, int n_replicas, int device, smdlt_c
     c->ztile = (!fp64 && z_tasks(FFT_TILE_Z_LARGE) >= FFT_Z_LARGE_MIN_TASKS) ? FFT_TILE_Z_LARGE
                : (z_tasks(FFT_TILE_Z) >= FFT_Z_MIN_TASKS || kzmax > FFT_TILE_Z_SMALL) ? FFT_TILE_Z
                                                                                       : FFT_TILE_Z_SMALL;
-    static_assert(FFT_TILE_Z >= KMAX && FFT_TILE_X >= KMAX && FFT_TILE_Y64 >= KMAX && FFT_TILE_Y >= KMAX,
+    static_assert(FFT_TILE_Z >= KMAX && FFT_TILE_X >= KMAX && FFT_TILE_X64 >= KMAX && FFT_TILE_Y64 >= KMAX &&
+                      FFT_TILE_Y >= KMAX,
                   "every FFT tile except FFT_TILE_Z_SMALL must hold a whole line of the longest length");
-    const int ztile = c->ztile, ytile = fp64 ? FFT_TILE_Y64 : FFT_TILE_Y;
+    const int ztile = c->ztile, ytile = fp64 ? FFT_TILE_Y64 : FFT_TILE_Y, xtile = fp64 ? FFT_TILE_X64 : FFT_TILE_X;
     for (int r = 0; r < n_replicas; ++r) {
       const RepDev& R = c->rh[r];
       int K0 = R.K[0], K1 = R.K[1], K2 = R.K[2], Kh = K0 / 2;
@@ -1058,7 +1059,7 @@ int smdlt_create(const smdlt_params* params, int n_replicas, int device, smdlt_c
       c->ty0[r] = (int)vy.size();
       c->tz0[r] = (int)vz.size();
       c->txy0[r] = (int)vxy.size();
-      int nlx = std::max(1, std::min(FFT_MAXL, FFT_TILE_X / K0));
+      int nlx = std::max(1, std::min(FFT_MAXL, xtile / K0));
       int nly = std::max(1, std::min(FFT_MAXL, ytile / K1));
       int nlz = std::max(1, std::min(FFT_MAXL, ztile / K2));
       const bool fused = K0 * K1 / 2 <= XY_ELEMS && gtot <= XY_MAX_POINTS;  // the plane fits one workgroup's LDS

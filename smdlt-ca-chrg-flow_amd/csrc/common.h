@@ -8,8 +8,9 @@
 #define KMAX 512            // largest FFT length (lengths are 2^a 3^b 5^c, multiples of 8)
 // complex elements per component per FFT workgroup, by pass; at most FFT_MAXL lines
 #ifndef FFT_TILE_X
-#define FFT_TILE_X 1024
+#define FFT_TILE_X 2048
 #endif
+#define FFT_TILE_X64 1024  // FP64 grids keep the smaller x tile, as they do in y
 #ifndef FFT_TILE_Y
 #define FFT_TILE_Y 2048
 #endif

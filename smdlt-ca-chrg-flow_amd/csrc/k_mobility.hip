@@ -740,7 +740,8 @@ template <typename C> __device__ __forceinline__ void stage_twiddles(C* dst, con
 template <typename T>
 __global__ __launch_bounds__(FFT_THREADS) void k_fft_x(MobArgs a, int dir) {
   typedef typename Prec<T>::C C;
-  __shared__ C sh[FFT_TILE_X + FFT_MAXL];
+  constexpr int TILE = sizeof(T) == 4 ? FFT_TILE_X : FFT_TILE_X64;
+  __shared__ C sh[TILE + FFT_MAXL];
   __shared__ C tw[KMAX];
   FftTask tk = a.tx[blockIdx.x];
   const RepDev& R = a.reps[tk.rep];
@@ -761,7 +762,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_x(MobArgs a, int dir) {
       sh[l * pitch + x] = z;
     }
     __syncthreads();
-    fft_lines<C, FFT_TILE_X>(sh, K, pitch, nl, -1, tw);
+    fft_lines<C, TILE>(sh, K, pitch, nl, -1, tw);
     for (int e = threadIdx.x; e < nl * Kh; e += blockDim.x) {
       int l = dKh.div(e), k = e - l * Kh;
       C zk = sh[l * pitch + k], zm = sh[l * pitch + (k == 0 ? 0 : K - k)];
@@ -796,7 +797,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_x(MobArgs a, int dir) {
       }
     }
     __syncthreads();
-    fft_lines<C, FFT_TILE_X>(sh, K, pitch, nl, 1, tw);
+    fft_lines<C, TILE>(sh, K, pitch, nl, 1, tw);
     for (int e = threadIdx.x; e < nl * K; e += blockDim.x) {
       int l = dK.div(e), x = e - l * K;
       long long row = 2LL * (tk.l0 + l);
