@@ -67,11 +67,14 @@ __device__ __forceinline__ void fft_stage(C* sh, int K, int pitch, int nl, int N
   const int nb = nl * KR;
   const FDiv dKR(KR), dNs(Ns);
   C v[MAXB][R];
+  int lq[MAXB], jq[MAXB];  // each butterfly's line and position, kept for the write
 #pragma unroll
   for (int q = 0; q < MAXB; ++q) {
     int b = t + q * TH;
     if (b < nb) {
       int l = dKR.div(b), j = b - l * KR;
+      lq[q] = l;
+      jq[q] = j;
       const C* src = sh + l * pitch + j;
 #pragma unroll
       for (int r = 0; r < R; ++r) v[q][r] = src[r * KR];
@@ -83,13 +86,15 @@ __device__ __forceinline__ void fft_stage(C* sh, int K, int pitch, int nl, int N
   for (int q = 0; q < MAXB; ++q) {
     int b = t + q * TH;
     if (b < nb) {
-      int l = dKR.div(b), j = b - l * KR;
+      const int l = lq[q], j = jq[q];
       int jm = j - dNs.div(j) * Ns;
+      if (Ns > 1) {  // the first stage's twiddles are all 1
 #pragma unroll
-      for (int r = 1; r < R; ++r) {
-        C w = tw[jm * r * tstep];
-        if (dir > 0) w = cconj(w);
-        v[q][r] = cmul(v[q][r], w);
+        for (int r = 1; r < R; ++r) {
+          C w = tw[jm * r * tstep];
+          if (dir > 0) w = cconj(w);
+          v[q][r] = cmul(v[q][r], w);
+        }
       }
       C* dst = sh + l * pitch + (j - jm) * R + jm;
       if (R == 2) {
