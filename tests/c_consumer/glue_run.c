@@ -1,5 +1,5 @@
 /* A reference-side driver over the C ABI, written in C99 as the reference's
- * own code would call it: the glue of INTEGRATION.md (gpu_init, the step, the
+ * own code would call it: the reference-side glue (gpu_init, the step, the
  * chargeHop sweep fed by the reference's random stream) around the time loop
  * of S/main26.c:65-159 for a small shear run.  One ran1 stream serves
  * initChains, getNoise and chargeHop in the reference's order; a sweep is

@@ -1,5 +1,7 @@
 """Pins of the CPU oracle (the checker) against closed forms, invariants,
-brute force and the values the survey measured with an independent restatement.
+brute force and values of the reference's calcHI computed by an independent restatement
+(the self term 0.95272918 at L = 60, 0.94307920 with its self-image loop, and the
+overlap-branch errors below).
 These run without a GPU."""
 import numpy as np
 import pytest
@@ -290,7 +292,7 @@ def test_ewald_large_box_approaches_free_rpy():
         assert dev < 3.2 / L and dev > 2.0 / L  # the leading periodic correction is -2.837/L
 
 
-def test_overlap_branch_matches_survey_measurements():
+def test_overlap_branch_matches_independent_measurements():
     # relative error of the reference's r<=2 I-coefficient at alpha=4/60
     for r, e in ((1.0, 2.67e-2), (1.5, 4.13e-3), (1.9, 2.14e-3), (1.99, 3.21e-4)):
         te, _ = O.rpy_real_coeffs(r, 4 / 60, 0)
@@ -302,7 +304,7 @@ def test_overlap_branch_matches_survey_measurements():
     assert abs(a - c) < 1e-8 and abs(b - d) < 1e-8
 
 
-def test_calchi_self_term_survey_values():
+def test_calchi_self_term_independent_values():
     L = 60.0
     lat = O.lattice(1, L, 0.01, 1e-3, 1, 100)  # t%tp <= tp/2: rce = L
     b = O.bins(lat, 2.0)
@@ -310,7 +312,8 @@ def test_calchi_self_term_survey_values():
     M, far, kc = O.calchi_ref(lat, *xyz(pos), 100, 8, b, None, L ** 3)
     assert M[0, 0] == pytest.approx(0.95272918, abs=2e-8)
     # second half of a period with an undeformed lattice: rce = 2L, alpha = 2/L,
-    # the self-image loop with && and dy*dy gives 0.94307920 (survey, App. A.1)
+    # the self-image loop with && and dy*dy gives 0.94307920 (the reference's calcHI
+    # evaluated by an independent restatement)
     lat.treal = lat.tp // 2 + 1
     lat.L2[0] = 0.0
     lat.L2xp = 0.0
@@ -359,7 +362,7 @@ def test_ewald_bin_self_block_is_the_ewald_diagonal():
     # at ewaldBin's split (alpha = 3.5/L, k-cut 2*3.5^2/L): the dense Ewald sum with
     # the same split and cuts (its k range edge differs from the reference's by
     # ~1e-6), and within the M_HI = 3.5 truncation (~3e-5) of the converged value,
-    # the survey's 0.95272918 at L = 60 (Hasimoto 1 - 2.837297/L + 4.18879/L^3)
+    # 0.95272918 at L = 60 (Hasimoto 1 - 2.837297/L + 4.18879/L^3)
     L = 60.0
     pos = np.array([[0.0, 0.0, 0.0], [3.0, 1.0, 0.5]])
     for flow, t in ((0, 0), (1, 100)):
