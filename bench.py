@@ -585,6 +585,8 @@ def main():
     ap.add_argument("--replicas", type=int, default=512)
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--semidilute-steps", type=int, default=30)
+    ap.add_argument("--hop-sweeps", type=int, default=100,
+                    help="sweeps per chain of the charge-hopping workload (burn-in: a tenth more)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--workload", choices=["all", "ensemble", "semidilute", "hop", "learned", "extension", "single-chain"], default="all",
@@ -620,8 +622,9 @@ def main():
         print(json.dumps({"learned": learned_leg()}, default=float))
         return
     if args.workload == "hop":
-        print(json.dumps({"charge_hopping": learned_leg(replicas=64, train_moves=300_000, sweeps=100, burn=20,
-                                                        iters=600, strong=False)}, default=float))
+        burn = max(20, args.hop_sweeps // 10)
+        print(json.dumps({"charge_hopping": learned_leg(replicas=64, train_moves=300_000, sweeps=args.hop_sweeps,
+                                                        burn=burn, iters=600, strong=False)}, default=float))
         return
     if args.workload == "semidilute":
         print(json.dumps({"semidilute": semidilute_leg(args.semidilute_steps, 3, 0 if args.no_plain else 3)}))
